@@ -1,0 +1,231 @@
+"""Benchmark of the GIR construction + query hot path on one MI355X.
+
+One step = buildWCC + buildSCC_DAG (condensed DAG, multiplicities, ranks)
++ BFS from 64 sources + delta-stepping SSSP from 8 sources, on a directed
+RMAT scale-22 graph (4,194,304 vertices, 2^26 samples, a/b/c/d =
+.57/.19/.19/.05, seed 2, self loops and duplicates removed, int32 CSR,
+uint32 weights uniform [1, 64]).  Inputs are resident in HBM before the
+timed region; outputs (levels, distances) are written to device tensors.
+
+Prints one JSON line.  value = WCC + SCC/DAG build time in ms (lower is
+better); the line also carries BFS / SSSP GTEPS (Graph500 convention:
+sum over sources of the out-arcs of reached vertices / time), the roofline
+of the dominant phase and the CPU baseline (the compiled reference, timed on
+this host in the same run).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--profile]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "graph-intermedia-representation_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "SCC/WCC build ms + BFS/SSSP GTEPS at RMAT-22; learned vs default tuning"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "rmat22": dict(kind="rmat", scale=22, ef=16, a=0.57, b=0.19, c=0.19, seed=2, wlo=1, whi=64),
+    "rmat16": dict(kind="rmat", scale=16, ef=16, a=0.57, b=0.19, c=0.19, seed=1, wlo=1, whi=64),
+    "grid4800": dict(kind="grid", rows=4800, cols=4800, p_keep=0.9, p_bidir=0.8, seed=3, wlo=1, whi=1000),
+}
+N_BFS, N_SSSP = 64, 8
+SEED_W, SEED_BFS, SEED_SSSP = 12, 7, 8
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_graph(cfg):
+    import girhip
+    if cfg["kind"] == "rmat":
+        return girhip.rmat_graph(cfg["scale"], cfg["ef"], cfg["a"], cfg["b"], cfg["c"], cfg["seed"])
+    return girhip.grid_graph(cfg["rows"], cfg["cols"], cfg["p_keep"], cfg["p_bidir"], cfg["seed"])
+
+
+def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2):
+    """Time the compiled reference (oracle/_ref/gir_ref: the reference's own
+    gir.cpp / gir_query.cpp) on this host: one buildWCC + buildSCC_DAG, BFS
+    from n_bfs and SSSP (its hash weights, delta 16) from n_sssp sources."""
+    import girhip
+    ref = os.path.join(ROOT, "oracle", "_ref", "gir_ref")
+    if not os.path.exists(ref):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        girhip.write_csr(os.path.join(td, "g"), fb, fa, bb, ba)
+        bfs_src[:n_bfs].astype(np.int32).tofile(os.path.join(td, "b.bin"))
+        sssp_src[:n_sssp].astype(np.int32).tofile(os.path.join(td, "s.bin"))
+        env = dict(os.environ)
+        env.setdefault("OMP_PROC_BIND", "close")
+        t0 = time.time()
+        out = subprocess.run([ref, os.path.join(td, "g") + "/", td + "/", "--bfs", os.path.join(td, "b.bin"),
+                              "--sssp", os.path.join(td, "s.bin"), "--no-output"], capture_output=True, text=True,
+                             env=env, check=True).stdout
+        wall = time.time() - t0
+    rec = {}
+    for line in out.splitlines():
+        if line.startswith("@@"):
+            d = json.loads(line[2:])
+            if "phase" in d:
+                rec[d["phase"]] = d
+            else:
+                rec.update(d)
+    return rec, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--profile", action="store_true", help="one extra step with per-kernel HIP event timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import girhip
+
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    dev = torch.device("cuda:0")
+    cfg = WORKLOADS[args.workload]
+    t0 = time.time()
+    V, fb, fa, bb, ba = make_graph(cfg)
+    E = len(fa)
+    w = girhip.weights(E, cfg["wlo"], cfg["whi"], SEED_W)
+    bfs_src = girhip.sources(fb, N_BFS, SEED_BFS)
+    sssp_src = girhip.sources(fb, N_SSSP, SEED_SSSP)
+    log(f"graph {args.workload}: V={V} E={E} generated in {time.time() - t0:.1f}s")
+
+    d_csr = [torch.from_numpy(x).to(dev) for x in (fb, fa, bb, ba)]
+    d_w = torch.from_numpy(w.view(np.int32)).to(dev)
+    G = girhip.Graph(*d_csr)
+    G.set_weights(d_w)
+    lab = torch.empty(V, dtype=torch.int32, device=dev)
+    scc = torch.empty(V, dtype=torch.int32, device=dev)
+    lv = torch.empty((N_BFS, V), dtype=torch.int32, device=dev)
+    dist = torch.empty((N_SSSP, V), dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        G.wcc(out=lab)
+        G.scc_dag(out=scc)
+        G.bfs(bfs_src, out=lv)
+        G.sssp(sssp_src, 16, out=dist)
+        return {k: G.last_ms(k) for k in ("wcc", "scc_dag", "bfs", "sssp")}
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i} done")
+    phases = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        phases.append(step())
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    ms_per_step = 1000.0 * t_total / max(args.steps, 1)
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step")
+
+    med = {k: float(np.median([p[k] for p in phases])) for k in phases[0]}
+    # traversed arcs and reached vertices per source (outside the timed region)
+    _, m_bfs = G.bfs(bfs_src, out=lv, edges=True)
+    n_bfs = (lv >= 0).sum(dim=1).cpu().numpy()
+    _, m_sssp = G.sssp(sssp_src, 16, out=dist, edges=True)
+    n_sssp = (dist != -1).sum(dim=1).cpu().numpy()
+    bfs_gteps = float(m_bfs.sum()) / (med["bfs"] * 1e-3) / 1e9
+    sssp_gteps = float(m_sssp.sum()) / (med["sssp"] * 1e-3) / 1e9
+    build_ms = med["wcc"] + med["scc_dag"]
+
+    # algorithmic bytes (per unit: one build, one source) -> achieved GB/s
+    alg = {
+        "wcc": 4 * (V + 1) + 4 * E + 8 * V,
+        "bfs": float(sum(4 * V + 8 * n + 8 * m for n, m in zip(n_bfs, m_bfs))),
+        "sssp": float(sum(8 * V + 8 * n + 16 * m for n, m in zip(n_sssp, m_sssp))),
+    }
+    _, n_scc, n_dag = G.get_scc()
+    alg["scc_dag"] = 2 * (4 * (V + 1) + 4 * E) + 8 * V + 4 * (V + 1) + 8 * E + 12 * n_dag
+    gbs = {k: alg[k] / (med[k] * 1e-3) / 1e9 for k in alg}
+    dom = max(med, key=lambda k: med[k])
+    units = {"wcc": 1, "scc_dag": 1, "bfs": N_BFS, "sssp": N_SSSP}[dom]
+    roofline = {"bound": "hbm", "achieved": round(gbs[dom], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs[dom] / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": f"{dom} phase ({units} unit(s) per call)",
+                "bytes_per_unit": round(alg[dom] / units), "ms_per_unit": round(med[dom] / units, 4)}
+
+    kernels = None
+    if args.profile:
+        G.set_param("profile", 1)
+        G.reset_kernel_stats()
+        step()
+        names = ["wcc_init", "wcc_hook_round", "wcc_compress", "wcc_sample", "wcc_hook_rest", "count_roots",
+                 "scc_compact", "scc_trim1", "scc_pivot", "reach_seed", "reach_topdown", "reach_q2bits",
+                 "reach_bottomup", "scc_fwbw_min", "scc_fwbw_assign", "scc_color_init", "scc_color_prop",
+                 "scc_color_roots", "scc_color_assign", "dag_keys", "radix_hist", "radix_scatter", "scan_reduce",
+                 "scan_blocks", "scan_down", "dag_flags", "dag_starts", "dag_write", "dag_beg", "rank_seeds",
+                 "rank_kahn", "sssp_setcnt", "sssp_relax", "sssp_farmin", "sssp_split", "fill"]
+        ks = {n: G.kernel_stats(n) for n in names}
+        kernels = sorted(({"name": n, "launches": c, "ms": round(m, 3)} for n, (c, m) in ks.items() if c),
+                         key=lambda x: -x["ms"])[:12]
+        G.set_param("profile", 0)
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (compiled reference) ...")
+        r = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src)
+        if r is not None:
+            rec, wall = r
+            b = rec.get("build", {})
+            cpu_build = b.get("wcc_ms", 0) + b.get("scc_dag_ms", 0)
+            cpu = {"value": round(cpu_build, 3), "unit": "ms", "cores": rec.get("threads"), "kind": "reference",
+                   "sample": "compiled reference (oracle/_ref/gir_ref) on this host: one buildWCC + buildSCC_DAG "
+                             f"on the same graph, BFS from {rec['bfs']['sources']} and SSSP (reference hash "
+                             f"weights, delta 16) from {rec['sssp']['sources']} of the same sources; "
+                             f"{wall:.1f} s wall",
+                   "wcc_ms": b.get("wcc_ms"), "scc_dag_ms": b.get("scc_dag_ms"),
+                   "bfs_ms_per_source": rec["bfs"]["ms"] / rec["bfs"]["sources"],
+                   "sssp_ms_per_source": rec["sssp"]["ms"] / rec["sssp"]["sources"]}
+            nb = rec["bfs"]["sources"]
+            cpu["bfs_gteps"] = float(m_bfs[:nb].sum()) / (rec["bfs"]["ms"] * 1e-3) / 1e9
+            log(f"cpu build {cpu_build:.1f} ms")
+
+    out = {
+        "metric": METRIC, "value": round(build_ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32/uint64", "data": "synthetic",
+        "config": {"workload": args.workload, **{k: v for k, v in cfg.items()}, "V": V, "E": E,
+                   "bfs_sources": N_BFS, "sssp_sources": N_SSSP, "delta": 16},
+        "wcc_ms": round(med["wcc"], 3), "scc_dag_ms": round(med["scc_dag"], 3),
+        "bfs_ms": round(med["bfs"], 3), "sssp_ms": round(med["sssp"], 3),
+        "bfs_gteps": round(bfs_gteps, 3), "sssp_gteps": round(sssp_gteps, 3),
+        "n_wcc": int(G.get_wcc()[1]), "n_scc": int(n_scc), "n_dag": int(n_dag),
+        "phase_gbs": {k: round(v, 2) for k, v in gbs.items()},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_build_vs_cpu"] = round(cpu["value"] / build_ms, 2)
+        out["speedup_bfs_vs_cpu"] = round(bfs_gteps / cpu["bfs_gteps"], 2) if cpu["bfs_gteps"] else None
+    if kernels:
+        out["kernels"] = kernels
+    G.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

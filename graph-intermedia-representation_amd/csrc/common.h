@@ -1,0 +1,327 @@
+// Internal header of the gfx950 GIR library: the handle, device buffers,
+// error handling, kernel launch wrapper and wave64 helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "girhip.h"
+
+namespace gir {
+
+constexpr int WAVE = 64;
+
+// Internal errors are C++ exceptions; every extern "C" entry point catches
+// them and turns them into a status code plus girhip_last_error().
+struct Error {
+    int code;
+    std::string msg;
+};
+[[noreturn]] void raise(int code, const std::string& msg);
+void setLastError(const std::string& m);
+
+#define HIPCHK(x)                                                                                        \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess)                                                                            \
+            ::gir::raise(e_ == hipErrorOutOfMemory ? GIRHIP_ENOMEM : GIRHIP_EHIP,                        \
+                         std::string(#x) + ": " + hipGetErrorString(e_));                                \
+    } while (0)
+
+// Grow-only device buffer.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void ensure(size_t want) {
+        if (want <= n && p) return;
+        release();
+        size_t bytes = sizeof(T) * (want > 0 ? want : 1);
+        hipError_t e = hipMalloc((void**)&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            raise(GIRHIP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+        n = want;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
+};
+
+// Tunables (only change work order; never results).
+struct Params {
+    double bfs_alpha = 15, bfs_beta = 18, bfs_min_reach = 4096;
+    double sssp_delta = 16;
+    double block = 256;
+    double wcc_rounds = 2;
+    double scc_trim_rounds = 3;
+    double scc_pivot = 1;
+    double scc_color_min = 0;
+    double profile = 0;  // 1: time every kernel launch with HIP events
+    double msbfs = 1;    // 1: k>1 BFS runs 64 sources per bit-parallel sweep
+};
+
+struct KStat {
+    long long count = 0;
+    double ms = 0;
+};
+
+// Scratch shared by the kernels of one handle (grown on demand).
+struct Scratch {
+    DevBuf<int32_t> q[3];
+    DevBuf<uint64_t> bits[4];
+    DevBuf<int32_t> iv[4];
+    DevBuf<uint8_t> bytes[2];
+    DevBuf<uint64_t> keys[2];
+    DevBuf<uint32_t> vals[2];
+    DevBuf<int64_t> off, off2;
+    DevBuf<uint8_t> stage;
+};
+
+}  // namespace gir
+
+struct girhip_graph {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int32_t V = 0;
+    int64_t E = 0;
+    gir::DevBuf<int32_t> fw_beg, fw_adj, bw_beg, bw_adj;
+    gir::DevBuf<uint32_t> w_fw;  // weights per fw slot (valid if has_w)
+    bool has_w = false;
+
+    // layers
+    bool wcc_req = false, scc_req = false;  // built at least once (repair on update)
+    bool wcc_ok = false, scc_ok = false;    // current with the edge store
+    bool pending = false;                   // ingested changes not yet synced
+    gir::DevBuf<int32_t> wcc;
+    int32_t n_wcc = 0;
+    gir::DevBuf<int32_t> scc;
+    int32_t n_scc = 0;
+    gir::DevBuf<int32_t> dag_src, dag_dst, dag_mult;
+    gir::DevBuf<int32_t> dag_beg;  // DAG CSR offsets by source SCC id (V+1)
+    int64_t n_dag = 0;
+    gir::DevBuf<int64_t> rank;
+
+    // host-pinned counters for small readbacks
+    int64_t* hcnt = nullptr;
+    gir::DevBuf<int64_t> dcnt;  // device counters (64 slots)
+
+    gir::Params prm;
+    std::map<std::string, double> last_ms;
+    std::map<std::string, gir::KStat> kstats;
+    struct PendingEv {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<PendingEv> pend_ev;
+    std::vector<hipEvent_t> ev_pool;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    gir::Scratch ws;
+};
+
+namespace gir {
+
+// Every entry point: make the handle's device current.
+void useDevice(girhip_graph* h);
+// Wrap a body: catches Error / std::bad_alloc, sets last error, returns status.
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return GIRHIP_OK;
+    } catch (const Error& e) {
+        setLastError(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        setLastError("host allocation failed");
+        return GIRHIP_ENOMEM;
+    } catch (const std::exception& e) {
+        setLastError(e.what());
+        return GIRHIP_EHIP;
+    }
+}
+
+hipEvent_t takeEvent(girhip_graph* h);
+void flushKernelStats(girhip_graph* h);  // after a stream sync
+
+inline unsigned gridFor(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 2147483647LL) g = 2147483647LL;
+    return (unsigned)g;
+}
+
+template <class K, class... Args>
+void launch(girhip_graph* h, const char* name, K kernel, unsigned grid, unsigned block, size_t shmem,
+            Args... args) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (h->prm.profile != 0) {
+        a = takeEvent(h);
+        HIPCHK(hipEventRecord(a, h->st));
+    }
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, h->st, args...);
+    HIPCHK(hipGetLastError());
+    if (a) {
+        b = takeEvent(h);
+        HIPCHK(hipEventRecord(b, h->st));
+        h->pend_ev.push_back({name, a, b});
+    }
+}
+
+void syncStream(girhip_graph* h);
+// Begin/end an entry point's timed region (HIP events on the handle's stream).
+void tBegin(girhip_graph* h);
+void tEnd(girhip_graph* h, const char* kind);  // syncs the stream
+
+// Pointer classification: true if p is device memory usable by the kernels.
+bool isDevicePtr(const void* p);
+// Copy helpers (hipMemcpyDefault: host or device on either side).
+void copyIn(girhip_graph* h, void* dst_dev, const void* src, size_t bytes);
+void copyOut(girhip_graph* h, void* dst, const void* src_dev, size_t bytes);
+// Read n int64 device counters (synchronises the stream).
+void readCounters(girhip_graph* h, const int64_t* dev, int n, int64_t* out);
+
+// ---- primitives (prims.hip) ----
+void fillI32(girhip_graph* h, int32_t* p, int32_t v, int64_t n);
+void fillU32(girhip_graph* h, uint32_t* p, uint32_t v, int64_t n);
+void fillI64(girhip_graph* h, int64_t* p, int64_t v, int64_t n);
+void fillU64(girhip_graph* h, uint64_t* p, uint64_t v, int64_t n);
+void fillU8(girhip_graph* h, uint8_t* p, uint8_t v, int64_t n);
+// out[0..n] = exclusive prefix sums of in[0..n) (out[n] = total).
+void scanI32toI64(girhip_graph* h, const int32_t* in, int64_t* out, int64_t n);
+void scanI32toI32(girhip_graph* h, const int32_t* in, int32_t* out, int64_t n);
+// Stable LSD radix sort of (key, value) pairs by the low key_bits of key.
+// vals may be null.  Result is in keys/vals.
+void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits);
+// Number of bits needed to hold values up to x.
+inline int bitsFor(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (x >> b) != 0) ++b;
+    return b < 1 ? 1 : b;
+}
+
+// ---- graph store (core.hip / batch.hip) ----
+// (Re)build canonical CSR on the device from unsorted rows.
+void normalizeCsr(girhip_graph* h, DevBuf<int32_t>& beg, DevBuf<int32_t>& adj, int64_t& E);
+void buildTranspose(girhip_graph* h);
+void invalidateLayers(girhip_graph* h);
+
+// ---- layers ----
+void runWcc(girhip_graph* h);
+void runScc(girhip_graph* h);
+void runDag(girhip_graph* h);
+
+inline Scratch& scratch(girhip_graph* h) { return h->ws; }
+
+}  // namespace gir
+
+// ---- device helpers ----
+namespace gir {
+__device__ __forceinline__ int laneId() { return (int)__lane_id(); }
+__device__ __forceinline__ uint64_t lanesBelow() { return (1ULL << laneId()) - 1ULL; }
+
+// Wave-aggregated slot reservation: every active lane with pred gets a
+// distinct index in [old, old + popcount) of *counter.  Call with the whole
+// active set of the wave present.
+__device__ __forceinline__ int64_t wavePush(int64_t* counter, bool pred) {
+    uint64_t m = __ballot(pred);
+    if (m == 0) return -1;
+    int leader = __ffsll((long long)m) - 1;
+    int64_t base = 0;
+    if (laneId() == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    return pred ? base + __popcll(m & lanesBelow()) : -1;
+}
+
+// Relaxed, device-coherent loads/stores (bypass the non-coherent CU cache).
+template <class T>
+__device__ __forceinline__ T ldAcq(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void stRel(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Visit every arc slot e in [b, end) of lane-owned rows across a wave: each
+// lane first walks up to SMALL slots of its own row, then rows with slots
+// left are finished cooperatively by the whole wave, 64 slots per step.
+// Call with every lane of the wave present (act selects the lanes that own
+// a row); f(v, e) may itself use wave-level ops.
+template <int SMALL = 16, class F>
+__device__ __forceinline__ void waveRows(bool act, int32_t v, int64_t b, int64_t end, F&& f) {
+    int64_t i = b;
+    if (act) {
+        const int64_t stop = end < b + SMALL ? end : b + SMALL;
+        for (; i < stop; ++i) f(v, i);
+    }
+    uint64_t heavy = __ballot(act && i < end);
+    while (heavy) {
+        const int l = __ffsll((long long)heavy) - 1;
+        heavy &= heavy - 1;
+        const int32_t hv = __shfl(v, l);
+        const int64_t hb = __shfl(i, l), he = __shfl(end, l);
+        for (int64_t j = hb + laneId(); j < he; j += WAVE) f(hv, j);
+    }
+}
+
+// waveRows with a per-row payload handed to f(v, payload, e) (the owner
+// lane's value is broadcast when the wave finishes a row cooperatively).
+template <int SMALL = 16, class P, class F>
+__device__ __forceinline__ void waveRowsP(bool act, int32_t v, P pay, int64_t b, int64_t end, F&& f) {
+    int64_t i = b;
+    if (act) {
+        const int64_t stop = end < b + SMALL ? end : b + SMALL;
+        for (; i < stop; ++i) f(v, pay, i);
+    }
+    uint64_t heavy = __ballot(act && i < end);
+    while (heavy) {
+        const int l = __ffsll((long long)heavy) - 1;
+        heavy &= heavy - 1;
+        const int32_t hv = __shfl(v, l);
+        const P hp = __shfl(pay, l);
+        const int64_t hb = __shfl(i, l), he = __shfl(end, l);
+        for (int64_t j = hb + laneId(); j < he; j += WAVE) f(hv, hp, j);
+    }
+}
+
+// Small utility kernels shared by several translation units (one private
+// copy per unit).
+// cnt += number of v with lab[v] == v (component representatives).
+static __global__ void kCountSelf(const int32_t* __restrict__ lab, int32_t V, int64_t* cnt) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool root = v < V && lab[v] == (int32_t)v;
+    const uint64_t m = __ballot(root);
+    if (m && laneId() == __ffsll((long long)m) - 1) atomicAdd((unsigned long long*)cnt, (unsigned long long)__popcll(m));
+}
+static __global__ void kI64toI32(const int64_t* __restrict__ in, int32_t* __restrict__ out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (int32_t)in[i];
+}
+
+// The reference's edge weight (gir_query.cpp:50-54): 1 + fmix64(key) mod 100.
+__device__ __forceinline__ uint32_t hashWeight(int32_t u, int32_t v) {
+    uint64_t x = ((uint64_t)(uint32_t)u << 32) | (uint32_t)v;
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return 1u + (uint32_t)(x % 100ULL);
+}
+}  // namespace gir

@@ -1,0 +1,206 @@
+// Device primitives: fills, exclusive scans and a stable LSD radix sort of
+// 64-bit keys with 32-bit values.  All wave64 / LDS based, written for gfx950.
+#include "common.h"
+
+namespace gir {
+
+template <class T>
+__global__ void kFill(T* __restrict__ p, T v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+template <class T>
+static void fillT(girhip_graph* h, T* p, T v, int64_t n) {
+    if (n <= 0) return;
+    unsigned g = gridFor(n, 256);
+    if (g > 8192) g = 8192;
+    launch(h, "fill", kFill<T>, g, 256, 0, p, v, n);
+}
+void fillI32(girhip_graph* h, int32_t* p, int32_t v, int64_t n) { fillT(h, p, v, n); }
+void fillU32(girhip_graph* h, uint32_t* p, uint32_t v, int64_t n) { fillT(h, p, v, n); }
+void fillI64(girhip_graph* h, int64_t* p, int64_t v, int64_t n) { fillT(h, p, v, n); }
+void fillU64(girhip_graph* h, uint64_t* p, uint64_t v, int64_t n) { fillT(h, p, v, n); }
+void fillU8(girhip_graph* h, uint8_t* p, uint8_t v, int64_t n) { fillT(h, p, v, n); }
+
+// ---------------------------------------------------------------------------
+// Exclusive scan: tile = 256 threads x 8 items.
+constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
+
+// Inclusive scan of v across the block; returns the inclusive value and the
+// block total in *total.
+__device__ __forceinline__ int64_t blockInclusiveScan(int64_t v, int64_t* total) {
+    __shared__ int64_t wsum[SCAN_T / WAVE];
+    const int lane = laneId(), w = threadIdx.x / WAVE;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        int64_t o = __shfl_up(v, d);
+        if (lane >= d) v += o;
+    }
+    if (lane == WAVE - 1) wsum[w] = v;
+    __syncthreads();
+    int64_t pre = 0, tot = 0;
+    for (int i = 0; i < SCAN_T / WAVE; ++i) {
+        if (i < w) pre += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return v + pre;
+}
+
+template <class In>
+__global__ void kScanReduce(const In* __restrict__ in, int64_t n, int64_t* __restrict__ bsum) {
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int64_t s = 0;
+    for (int i = 0; i < SCAN_I; ++i) {
+        int64_t idx = base + (int64_t)i * SCAN_T + threadIdx.x;
+        if (idx < n) s += (int64_t)in[idx];
+    }
+    int64_t tot;
+    blockInclusiveScan(s, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// Exclusive scan of bsum[0..nb) in one block (in place); bsum[nb] = total.
+__global__ void kScanBlockSums(int64_t* bsum, int64_t nb) {
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nb; base += SCAN_T) {
+        int64_t idx = base + threadIdx.x;
+        int64_t v = idx < nb ? bsum[idx] : 0;
+        int64_t tot;
+        int64_t inc = blockInclusiveScan(v, &tot);
+        if (idx < nb) bsum[idx] = carry + inc - v;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+template <class In, class Out>
+__global__ void kScanDown(const In* __restrict__ in, Out* __restrict__ out, int64_t n, const int64_t* __restrict__ bsum) {
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_I;
+    int64_t v[SCAN_I];
+    int64_t s = 0;
+    for (int i = 0; i < SCAN_I; ++i) {
+        int64_t idx = base + i;
+        v[i] = idx < n ? (int64_t)in[idx] : 0;
+        s += v[i];
+    }
+    int64_t tot;
+    int64_t inc = blockInclusiveScan(s, &tot);
+    int64_t run = bsum[blockIdx.x] + inc - s;
+    for (int i = 0; i < SCAN_I; ++i) {
+        int64_t idx = base + i;
+        if (idx < n) out[idx] = (Out)run;
+        run += v[i];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = (Out)bsum[gridDim.x];
+}
+
+template <class In, class Out>
+static void scanT(girhip_graph* h, const In* in, Out* out, int64_t n) {
+    if (n <= 0) {
+        fillT<Out>(h, out, (Out)0, 1);
+        return;
+    }
+    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    Scratch& S = scratch(h);
+    S.off.ensure(nb + 1);
+    launch(h, "scan_reduce", kScanReduce<In>, (unsigned)nb, SCAN_T, 0, in, n, S.off.p);
+    launch(h, "scan_blocks", kScanBlockSums, 1, SCAN_T, 0, S.off.p, nb);
+    launch(h, "scan_down", kScanDown<In, Out>, (unsigned)nb, SCAN_T, 0, in, out, n, (const int64_t*)S.off.p);
+}
+
+void scanI32toI64(girhip_graph* h, const int32_t* in, int64_t* out, int64_t n) { scanT(h, in, out, n); }
+void scanI32toI32(girhip_graph* h, const int32_t* in, int32_t* out, int64_t n) { scanT(h, in, out, n); }
+
+// ---------------------------------------------------------------------------
+// LSD radix sort, 8-bit digits.  Tile = 256 threads x 16 rounds.
+constexpr int RS_T = 256, RS_R = 16, RS_TILE = RS_T * RS_R, RS_D = 256;
+
+__global__ void kRadixHist(const uint64_t* __restrict__ keys, int64_t n, int shift, int32_t* __restrict__ hist,
+                           int nb) {
+    __shared__ int32_t c[RS_D];
+    c[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_R; ++r) {
+        int64_t idx = base + (int64_t)r * RS_T + threadIdx.x;
+        if (idx < n) atomicAdd(&c[(keys[idx] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nb + blockIdx.x] = c[threadIdx.x];
+}
+
+// Stable scatter.  For each round of 256 consecutive keys, lanes of a wave
+// holding the same digit are matched with 8 ballots; ranks across the 4
+// waves come from per-wave digit counts in LDS; base[d] carries the block's
+// running offset per digit (from the scanned digit-major histogram).
+__global__ void kRadixScatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                              uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
+                              const int64_t* __restrict__ hscan, int nb) {
+    __shared__ int64_t base[RS_D];
+    __shared__ int32_t wc[RS_T / WAVE][RS_D];
+    const int t = threadIdx.x, lane = laneId(), w = t / WAVE;
+    base[t] = hscan[(int64_t)t * nb + blockIdx.x];
+    const int64_t tile = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_R; ++r) {
+        const int64_t idx = tile + (int64_t)r * RS_T + t;
+        const bool valid = idx < n;
+        uint64_t key = valid ? kin[idx] : 0;
+        uint32_t val = (valid && vin) ? vin[idx] : 0;
+        const int d = (int)((key >> shift) & 255);
+        for (int k = 0; k < RS_T / WAVE; ++k) wc[k][t] = 0;
+        __syncthreads();
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & lanesBelow());
+        if (valid && rank == 0) wc[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int64_t off = base[d] + rank;
+            for (int k = 0; k < w; ++k) off += wc[k][d];
+            kout[off] = key;
+            if (vout) vout[off] = val;
+        }
+        __syncthreads();
+        int64_t add = 0;
+        for (int k = 0; k < RS_T / WAVE; ++k) add += wc[k][t];
+        base[t] += add;
+        __syncthreads();
+    }
+}
+
+void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits) {
+    if (n <= 1) return;
+    Scratch& S = scratch(h);
+    const int nb = (int)((n + RS_TILE - 1) / RS_TILE);
+    S.keys[1].ensure(n);
+    if (vals) S.vals[1].ensure(n);
+    DevBuf<int32_t>& hist = S.iv[3];
+    hist.ensure((size_t)RS_D * nb);
+    DevBuf<int64_t>& hscan = S.off2;  // (the scan helper itself uses S.off)
+    hscan.ensure((size_t)RS_D * nb + 1);
+    uint64_t *ka = keys, *kb = S.keys[1].p;
+    uint32_t *va = vals, *vb = vals ? S.vals[1].p : nullptr;
+    const int passes = (key_bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        launch(h, "radix_hist", kRadixHist, (unsigned)nb, RS_T, 0, (const uint64_t*)ka, n, shift, hist.p, nb);
+        scanI32toI64(h, hist.p, hscan.p, (int64_t)RS_D * nb);
+        launch(h, "radix_scatter", kRadixScatter, (unsigned)nb, RS_T, 0, (const uint64_t*)ka, (const uint32_t*)va, kb,
+               vb, n, shift, (const int64_t*)hscan.p, nb);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    if (ka != keys) {
+        HIPCHK(hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, h->st));
+        if (vals) HIPCHK(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, h->st));
+    }
+}
+
+}  // namespace gir

@@ -1,0 +1,196 @@
+// Direction-optimising reachability engine (device side + host driver).
+//
+// The structure follows GIR::bfs (gir_query.cpp:108-294): a top-down step
+// expands a frontier queue over push-direction arcs and claims vertices in a
+// visited bitmap; a bottom-up step lets every unvisited vertex look for a
+// parent in a frontier bitmap over pull-direction arcs and stops at the
+// first hit; the switch rule is the reference's (alpha, beta, minimum
+// reachable size).  On the GPU:
+//   - top-down: one lane per frontier vertex, rows longer than 16 arcs are
+//     finished by the whole wave (waveRows); claims are atomicOr on the
+//     64-bit visited word; new vertices are appended with one atomic per
+//     wave (ballot + prefix);
+//   - bottom-up: one wave per 64-vertex bitmap word, so the next-frontier
+//     word is a single ballot and the visited word is written without
+//     atomics;
+//   - the filter Flt decides which arcs may be followed (child, parent), so
+//     the same kernels run plain BFS, SCC forward/backward reachability over
+//     the still-unassigned vertices, and the colour-restricted backward
+//     sweeps of the colouring phase.
+#pragma once
+#include "common.h"
+
+namespace gir {
+
+struct ReachBufs {
+    int32_t* q[2];
+    uint64_t *vis, *fbits, *nbits;
+    int64_t W;  // bitmap words
+};
+
+// Seed: mark the initial frontier visited and call on(v, 0).
+template <class Vis>
+__global__ void kReachSeed(const int32_t* __restrict__ q, int64_t n, uint64_t* vis, Vis on) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t v = q[i];
+    atomicOr((unsigned long long*)&vis[v >> 6], 1ULL << (v & 63));
+    on(v, 0);
+}
+
+// cnt[0] = next queue length, cnt[1] = sum of push-direction degrees of the
+// claimed vertices; cz points at the counter pair of the following level,
+// which this kernel zeroes.
+template <class Flt, class Vis>
+__global__ void kTopDown(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ beg,
+                         const int32_t* __restrict__ adj, uint64_t* vis, int32_t L, int32_t* __restrict__ qout,
+                         int64_t* cnt, int64_t* cz, Flt flt, Vis on) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) cz[threadIdx.x] = 0;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < nin;
+    const int32_t u = act ? qin[i] : 0;
+    const int32_t b = act ? beg[u] : 0, e = act ? beg[u + 1] : 0;
+    int64_t m = 0;
+    waveRows(act, u, b, e, [&](int32_t p, int64_t j) {
+        const int32_t w = adj[j];
+        bool claim = false;
+        if (flt(w, p)) {
+            const uint64_t bit = 1ULL << (w & 63);
+            if (!(ldAcq(&vis[w >> 6]) & bit))
+                claim = !(atomicOr((unsigned long long*)&vis[w >> 6], (unsigned long long)bit) & bit);
+        }
+        const int64_t pos = wavePush(&cnt[0], claim);
+        if (claim) {
+            qout[pos] = w;
+            on(w, L + 1);
+            m += beg[w + 1] - beg[w];
+        }
+    });
+    // wave-reduce m
+    for (int d = WAVE / 2; d > 0; d >>= 1) m += __shfl_xor(m, d);
+    if (laneId() == 0 && m) atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)m);
+}
+
+template <class Flt, class Vis>
+__global__ void kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __restrict__ radj,
+                          const int32_t* __restrict__ pbeg, uint64_t* vis, const uint64_t* __restrict__ fbits,
+                          uint64_t* __restrict__ nbits, int32_t V, int32_t L, int32_t* __restrict__ qout, int64_t* cnt,
+                          int64_t* cz, Flt flt, Vis on) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) cz[threadIdx.x] = 0;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t word = v >> 6;
+    const bool word_ok = (word << 6) < V;
+    const uint64_t w0 = word_ok ? vis[word] : ~0ULL;
+    const bool act = v < V && !((w0 >> laneId()) & 1) && flt.self((int32_t)v);
+    bool found = false;
+    if (act) {
+        const int32_t b = rbeg[v], e = rbeg[v + 1];
+        for (int32_t j = b; j < e; ++j) {
+            const int32_t p = radj[j];
+            if (((fbits[p >> 6] >> (p & 63)) & 1) && flt((int32_t)v, p)) {
+                found = true;
+                break;
+            }
+        }
+    }
+    const uint64_t nb = __ballot(found);
+    if (laneId() == 0 && word_ok) {
+        nbits[word] = nb;
+        if (nb) vis[word] = w0 | nb;
+    }
+    int64_t m = 0;
+    const int64_t pos = wavePush(&cnt[0], found);
+    if (found) {
+        qout[pos] = (int32_t)v;
+        on((int32_t)v, L + 1);
+        m = pbeg[v + 1] - pbeg[v];
+    }
+    for (int d = WAVE / 2; d > 0; d >>= 1) m += __shfl_xor(m, d);
+    if (laneId() == 0 && m) atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)m);
+}
+
+static __global__ void kQueueToBits(const int32_t* __restrict__ q, int64_t n, uint64_t* bits) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicOr((unsigned long long*)&bits[q[i] >> 6], 1ULL << (q[i] & 63));
+}
+
+// Arc filters and visit hooks.
+struct AllArcs {
+    __device__ bool operator()(int32_t, int32_t) const { return true; }
+    __device__ bool self(int32_t) const { return true; }
+};
+struct NoVisit {
+    __device__ void operator()(int32_t, int32_t) const {}
+};
+struct WriteLevel {
+    int32_t* level;
+    __device__ void operator()(int32_t v, int32_t L) const { level[v] = L; }
+};
+
+struct ReachStats {
+    int32_t levels = 0;
+    int64_t reached = 0;  // including the seeds
+};
+
+// Runs reachability from the n0 seeds in B.q[0] (caller fills them).  The
+// visited bitmap B.vis must be clear on entry except for bits the caller
+// wants treated as already visited.  pbeg/padj: push-direction CSR; rbeg/radj:
+// pull-direction CSR.  m0 = push degree sum of the seeds; n_reach / m_total:
+// the size estimates of the switch rule.
+template <class Flt, class Vis>
+ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, const int32_t* rbeg,
+                    const int32_t* radj, ReachBufs& B, int64_t n0, int64_t m0, int64_t n_reach, int64_t m_total,
+                    Flt flt, Vis on, bool allow_bottom_up = true) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    ReachStats st;
+    if (n0 <= 0) return st;
+    launch(h, "reach_seed", kReachSeed<Vis>, gridFor(n0, blk), blk, 0, (const int32_t*)B.q[0], n0, B.vis, on);
+    int64_t* c = h->dcnt.p + 16;  // two counter pairs: c[0..1], c[2..3]
+    fillI64(h, c, 0, 4);
+    const int64_t alpha = (int64_t)h->prm.bfs_alpha, beta = (int64_t)h->prm.bfs_beta;
+    const int64_t min_reach = (int64_t)h->prm.bfs_min_reach;
+    int64_t nf = n0, mf = m0, m_unexplored = m_total - m0;
+    bool bu = false, bits_valid = false;
+    int cur = 0;
+    int32_t L = 0;
+    st.reached = n0;
+    while (nf > 0) {
+        if (allow_bottom_up && n_reach >= min_reach) {
+            if (!bu && mf * alpha > m_unexplored) bu = true;
+            else if (bu && nf * beta < n_reach) bu = false;
+        }
+        int64_t* cn = c + 2 * (L & 1);
+        int64_t* cz = c + 2 * ((L + 1) & 1);
+        if (!bu) {
+            launch(h, "reach_topdown", kTopDown<Flt, Vis>, gridFor(nf, blk), blk, 0, (const int32_t*)B.q[cur], nf, pbeg,
+                   padj, B.vis, L, B.q[cur ^ 1], cn, cz, flt, on);
+            bits_valid = false;
+        } else {
+            if (!bits_valid) {
+                fillU64(h, B.fbits, 0, B.W);
+                launch(h, "reach_q2bits", kQueueToBits, gridFor(nf, blk), blk, 0, (const int32_t*)B.q[cur], nf,
+                       B.fbits);
+            }
+            launch(h, "reach_bottomup", kBottomUp<Flt, Vis>, gridFor(V, blk), blk, 0, rbeg, radj, pbeg, B.vis,
+                   (const uint64_t*)B.fbits, B.nbits, V, L, B.q[cur ^ 1], cn, cz, flt, on);
+            std::swap(B.fbits, B.nbits);
+            bits_valid = true;
+        }
+        int64_t r[2];
+        readCounters(h, cn, 2, r);
+        nf = r[0];
+        mf = r[1];
+        m_unexplored -= mf;
+        st.reached += nf;
+        cur ^= 1;
+        ++L;
+    }
+    st.levels = L;
+    return st;
+}
+
+// Reachability buffers from the handle's scratch (q[0], q[1], bits[0..2]).
+ReachBufs reachBufs(girhip_graph* h);
+
+}  // namespace gir

@@ -1,0 +1,491 @@
+// Strongly connected components + condensation DAG + topological ranks.
+//
+// Replaces GIR::buildSCC_DAG (gir.cpp:798-839), which runs a sequential
+// iterative Tarjan (gir.cpp:494-537), counts condensation arcs in per-SCC
+// hash maps (gir.cpp:818-831) and ranks SCCs by reversed emission order
+// (gir.cpp:833-836).  Here, over the vertices not yet assigned ("active"):
+//
+//   trim-1   a vertex with no active out-neighbour or no active in-neighbour
+//            (self loops ignored) is a singleton SCC; repeated;
+//   FW-BW    from the pivot maximising outdeg x indeg (the rule of the
+//            reference's iSpan-style side tree, includes/scc_cpu/
+//            scc_common.h:98-158), forward and backward reachability over the
+//            active vertices; their intersection is the pivot's SCC;
+//   colour   every active vertex starts with colour = its id and the minimum
+//            is propagated along arcs to a fixpoint, so colour[v] = the
+//            smallest active vertex that reaches v.  A vertex whose colour is
+//            its own id is the minimum of its SCC; a backward sweep from all
+//            such roots at once, restricted to arcs inside one colour, marks
+//            exactly their SCCs.  Repeat on what is left.
+//
+// Every SCC is labelled by its minimum vertex (trim: itself; FW-BW: a min
+// reduction; colouring: the root), the canonical id.
+//
+// DAG: one pass over the forward arcs emits the packed key (scc[u], scc[v])
+// of every crossing arc; a radix sort and a run-length pass give the distinct
+// condensation arcs with their multiplicities, sorted by (src, dst).
+// Ranks: Kahn levels over the DAG, rank = (level << 32) | scc_id, so ranks are
+// distinct and every arc a -> b has rank[a] < rank[b].
+#include "common.h"
+#include "reach.h"
+
+namespace gir {
+
+struct ActiveOnly {
+    const int32_t* scc;
+    __device__ bool operator()(int32_t c, int32_t) const { return ldAcq(&scc[c]) == -1; }
+    __device__ bool self(int32_t v) const { return ldAcq(&scc[v]) == -1; }
+};
+struct SameColour {
+    const int32_t* scc;
+    const int32_t* color;
+    __device__ bool operator()(int32_t c, int32_t p) const { return ldAcq(&scc[c]) == -1 && color[c] == color[p]; }
+    __device__ bool self(int32_t v) const { return ldAcq(&scc[v]) == -1; }
+};
+
+__global__ void kCompactActive(const int32_t* __restrict__ in, int64_t n, bool all, const int32_t* __restrict__ scc,
+                               int32_t* __restrict__ out, int64_t* cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t v = 0;
+    bool keep = false;
+    if (i < n) {
+        v = all ? (int32_t)i : in[i];
+        keep = scc[v] == -1;
+    }
+    const int64_t pos = wavePush(cnt, keep);
+    if (keep) out[pos] = v;
+}
+
+__device__ __forceinline__ bool anyActive(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj,
+                                          const int32_t* scc, int32_t v) {
+    const int32_t b = beg[v], e = beg[v + 1];
+    for (int32_t j = b; j < e; ++j) {
+        const int32_t w = adj[j];
+        if (w != v && ldAcq(&scc[w]) == -1) return true;
+    }
+    return false;
+}
+
+__global__ void kTrim1(const int32_t* __restrict__ list, int64_t n, const int32_t* __restrict__ fb,
+                       const int32_t* __restrict__ fa, const int32_t* __restrict__ bb, const int32_t* __restrict__ ba,
+                       int32_t* scc, int64_t* cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool trim = false;
+    if (i < n) {
+        const int32_t v = list[i];
+        if (ldAcq(&scc[v]) == -1) {
+            trim = !anyActive(fb, fa, scc, v) || !anyActive(bb, ba, scc, v);
+            if (trim) stRel(&scc[v], v);
+        }
+    }
+    const uint64_t m = __ballot(trim);
+    if (m && laneId() == __ffsll((long long)m) - 1) atomicAdd((unsigned long long*)cnt, (unsigned long long)__popcll(m));
+}
+
+// argmax over the active list of min(outdeg * indeg, 2^33 - 1), ties to the
+// smaller vertex id: key = (score << 31) | (2^31 - 1 - v).
+__global__ void kPivot(const int32_t* __restrict__ list, int64_t n, const int32_t* __restrict__ fb,
+                       const int32_t* __restrict__ bb, unsigned long long* best) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long key = 0;
+    if (i < n) {
+        const int32_t v = list[i];
+        unsigned long long s = (unsigned long long)(fb[v + 1] - fb[v]) * (unsigned long long)(bb[v + 1] - bb[v]);
+        if (s > (1ULL << 33) - 1) s = (1ULL << 33) - 1;
+        key = (s << 31) | (unsigned long long)(0x7fffffff - v);
+    }
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        unsigned long long o = __shfl_xor(key, d);
+        key = o > key ? o : key;
+    }
+    if (laneId() == 0) atomicMax(best, key);
+}
+
+__device__ __forceinline__ bool bitOf(const uint64_t* b, int32_t v) { return (b[v >> 6] >> (v & 63)) & 1; }
+
+__global__ void kFwBwMin(const int32_t* __restrict__ list, int64_t n, const uint64_t* __restrict__ fwb,
+                         const uint64_t* __restrict__ bwb, int64_t* minv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t x = INT64_MAX;
+    if (i < n) {
+        const int32_t v = list[i];
+        if (bitOf(fwb, v) && bitOf(bwb, v)) x = v;
+    }
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        int64_t o = __shfl_xor(x, d);
+        x = o < x ? o : x;
+    }
+    if (laneId() == 0 && x != INT64_MAX) atomicMin((long long*)minv, (long long)x);
+}
+
+__global__ void kFwBwAssign(const int32_t* __restrict__ list, int64_t n, const uint64_t* __restrict__ fwb,
+                            const uint64_t* __restrict__ bwb, const int64_t* __restrict__ minv, int32_t* scc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t v = list[i];
+    if (bitOf(fwb, v) && bitOf(bwb, v)) scc[v] = (int32_t)minv[0];
+}
+
+__global__ void kColorInit(const int32_t* __restrict__ list, int64_t n, int32_t* color) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) color[list[i]] = list[i];
+}
+
+// One propagation step: every queued vertex pushes its colour to its active
+// out-neighbours (atomicMin); a neighbour whose colour dropped is queued once
+// per step (stamp).
+__global__ void kColorProp(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ fb,
+                           const int32_t* __restrict__ fa, const int32_t* __restrict__ scc, int32_t* color,
+                           int32_t* stamp, int32_t it, int32_t* __restrict__ qout, int64_t* cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < nin;
+    const int32_t x = act ? qin[i] : 0;
+    const int32_t b = act ? fb[x] : 0, e = act ? fb[x + 1] : 0;
+    waveRows(act, x, b, e, [&](int32_t p, int64_t j) {
+        const int32_t y = fa[j];
+        bool push = false;
+        if (ldAcq(&scc[y]) == -1) {
+            const int32_t cx = ldAcq(&color[p]);
+            if (cx < ldAcq(&color[y])) {
+                const int32_t old = atomicMin(&color[y], cx);
+                push = cx < old && atomicExch(&stamp[y], it) != it;
+            }
+        }
+        const int64_t pos = wavePush(cnt, push);
+        if (push) qout[pos] = y;
+    });
+}
+
+__global__ void kColorRoots(const int32_t* __restrict__ list, int64_t n, const int32_t* __restrict__ color,
+                            int32_t* __restrict__ out, int64_t* cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t v = 0;
+    bool root = false;
+    if (i < n) {
+        v = list[i];
+        root = color[v] == v;
+    }
+    const int64_t pos = wavePush(cnt, root);
+    if (root) out[pos] = v;
+}
+
+__global__ void kColorAssign(const int32_t* __restrict__ list, int64_t n, const uint64_t* __restrict__ vis,
+                             const int32_t* __restrict__ color, int32_t* scc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t v = list[i];
+    if (bitOf(vis, v)) scc[v] = color[v];
+}
+
+// ---- DAG ----
+__global__ void kDagKeys(const int32_t* __restrict__ fb, const int32_t* __restrict__ fa,
+                         const int32_t* __restrict__ scc, int32_t V, int B, uint64_t* __restrict__ keys, int64_t* cnt) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = v < V;
+    const int32_t b = act ? fb[v] : 0, e = act ? fb[v + 1] : 0;
+    const int32_t su = act ? scc[v] : 0;
+    waveRows(act, (int32_t)v, b, e, [&](int32_t u, int64_t j) {
+        const int32_t a = scc[u];
+        const int32_t c = scc[fa[j]];
+        const bool cross = a != c;
+        const int64_t pos = wavePush(cnt, cross);
+        if (cross) keys[pos] = ((uint64_t)(uint32_t)a << B) | (uint32_t)c;
+    });
+    (void)su;
+}
+
+__global__ void kRunFlags(const uint64_t* __restrict__ keys, int64_t n, int32_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+__global__ void kRunStarts(const int32_t* __restrict__ flag, const int64_t* __restrict__ pos, int64_t n,
+                           int32_t* __restrict__ start) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) start[pos[i]] = (int32_t)i;
+}
+
+__global__ void kDagWrite(const uint64_t* __restrict__ keys, const int32_t* __restrict__ start, int64_t nd,
+                          int64_t ncross, int B, int32_t* __restrict__ src, int32_t* __restrict__ dst,
+                          int32_t* __restrict__ mult, int32_t* __restrict__ outdeg, int32_t* __restrict__ indeg) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nd) return;
+    const int64_t i = start[r], j = r + 1 < nd ? start[r + 1] : ncross;
+    const uint64_t k = keys[i];
+    const int32_t a = (int32_t)(k >> B), b = (int32_t)(k & ((1ULL << B) - 1));
+    src[r] = a;
+    dst[r] = b;
+    mult[r] = (int32_t)(j - i);
+    atomicAdd(&outdeg[a], 1);
+    atomicAdd(&indeg[b], 1);
+}
+
+// ---- ranks (Kahn levels) ----
+__global__ void kRankSeeds(const int32_t* __restrict__ scc, const int32_t* __restrict__ indeg, int32_t V,
+                           int64_t* rank, int32_t* __restrict__ out, int64_t* cnt) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool seed = false;
+    if (v < V) {
+        rank[v] = -1;
+        seed = scc[v] == (int32_t)v && indeg[v] == 0;
+    }
+    const int64_t pos = wavePush(cnt, seed);
+    if (seed) out[pos] = (int32_t)v;
+}
+
+__global__ void kKahnStep(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ dbeg,
+                          const int32_t* __restrict__ ddst, int32_t* indeg, int64_t* rank, int32_t L,
+                          int32_t* __restrict__ qout, int64_t* cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < nin;
+    const int32_t s = act ? qin[i] : 0;
+    if (act) rank[s] = ((int64_t)L << 32) | (int64_t)(uint32_t)s;
+    const int32_t b = act ? dbeg[s] : 0, e = act ? dbeg[s + 1] : 0;
+    waveRows(act, s, b, e, [&](int32_t, int64_t j) {
+        const int32_t t = ddst[j];
+        const bool ready = atomicSub(&indeg[t], 1) == 1;
+        const int64_t pos = wavePush(cnt, ready);
+        if (ready) qout[pos] = t;
+    });
+}
+
+namespace {
+
+int64_t compactActive(girhip_graph* h, const int32_t* in, int64_t n, bool all, int32_t* out) {
+    const int blk = (int)h->prm.block;
+    fillI64(h, h->dcnt.p + 2, 0, 1);
+    launch(h, "scc_compact", kCompactActive, gridFor(n, blk), blk, 0, in, n, all, (const int32_t*)h->scc.p, out,
+           h->dcnt.p + 2);
+    int64_t c;
+    readCounters(h, h->dcnt.p + 2, 1, &c);
+    return c;
+}
+
+// Trim-1 rounds over list (in place: the list is re-compacted between rounds).
+int64_t trimRounds(girhip_graph* h, int32_t*& list, int32_t*& alt, int64_t n, int rounds) {
+    const int blk = (int)h->prm.block;
+    for (int r = 0; r < rounds && n > 0; ++r) {
+        fillI64(h, h->dcnt.p + 3, 0, 1);
+        launch(h, "scc_trim1", kTrim1, gridFor(n, blk), blk, 0, (const int32_t*)list, n, (const int32_t*)h->fw_beg.p,
+               (const int32_t*)h->fw_adj.p, (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p, h->scc.p,
+               h->dcnt.p + 3);
+        int64_t trimmed;
+        readCounters(h, h->dcnt.p + 3, 1, &trimmed);
+        if (trimmed == 0) break;
+        n = compactActive(h, list, n, false, alt);
+        std::swap(list, alt);
+    }
+    return n;
+}
+
+}  // namespace
+
+void runScc(girhip_graph* h) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    h->scc.ensure((size_t)(V > 0 ? V : 1));
+    if (V == 0) {
+        h->n_scc = 0;
+        return;
+    }
+    fillI32(h, h->scc.p, -1, V);
+    S.q[2].ensure(V);
+    S.iv[0].ensure(V);
+    S.iv[1].ensure(V);
+    S.iv[2].ensure(V);
+    S.bits[3].ensure(((size_t)V + 63) / 64);
+    int32_t* list = S.q[2].p;
+    int32_t* alt = S.iv[0].p;
+    int32_t* color = S.iv[1].p;
+    int32_t* stamp = S.iv[2].p;
+    ReachBufs B = reachBufs(h);
+    int64_t n = compactActive(h, nullptr, V, true, list);
+    n = trimRounds(h, list, alt, n, (int)h->prm.scc_trim_rounds);
+
+    // FW-BW from the pivot
+    if (n > 0 && h->prm.scc_pivot != 0 && n > (int64_t)h->prm.scc_color_min) {
+        fillI64(h, h->dcnt.p + 4, 0, 1);
+        launch(h, "scc_pivot", kPivot, gridFor(n, blk), blk, 0, (const int32_t*)list, n, (const int32_t*)h->fw_beg.p,
+               (const int32_t*)h->bw_beg.p, (unsigned long long*)(h->dcnt.p + 4));
+        int64_t key;
+        readCounters(h, h->dcnt.p + 4, 1, &key);
+        const int32_t pivot = (int32_t)(0x7fffffff - (key & 0x7fffffffLL));
+        const ActiveOnly act{h->scc.p};
+        fillU64(h, B.vis, 0, B.W);
+        fillI32(h, B.q[0], pivot, 1);
+        runReach(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
+        HIPCHK(hipMemcpyAsync(S.bits[3].p, B.vis, sizeof(uint64_t) * B.W, hipMemcpyDeviceToDevice, h->st));
+        fillU64(h, B.vis, 0, B.W);
+        fillI32(h, B.q[0], pivot, 1);
+        runReach(h, h->bw_beg.p, h->bw_adj.p, h->fw_beg.p, h->fw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
+        fillI64(h, h->dcnt.p + 5, INT64_MAX, 1);
+        launch(h, "scc_fwbw_min", kFwBwMin, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
+               (const uint64_t*)S.bits[3].p, (const uint64_t*)B.vis, h->dcnt.p + 5);
+        launch(h, "scc_fwbw_assign", kFwBwAssign, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
+               (const uint64_t*)S.bits[3].p, (const uint64_t*)B.vis, (const int64_t*)h->dcnt.p + 5, h->scc.p);
+        n = compactActive(h, list, n, false, alt);
+        std::swap(list, alt);
+        n = trimRounds(h, list, alt, n, (int)h->prm.scc_trim_rounds);
+    }
+
+    // colouring on the remainder
+    fillI32(h, stamp, 0, V);
+    int32_t it = 0;
+    while (n > 0) {
+        launch(h, "scc_color_init", kColorInit, gridFor(n, blk), blk, 0, (const int32_t*)list, n, color);
+        // propagate to a fixpoint, starting from every active vertex
+        HIPCHK(hipMemcpyAsync(B.q[0], list, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, h->st));
+        int64_t nq = n;
+        int cur = 0;
+        while (nq > 0) {
+            ++it;
+            fillI64(h, h->dcnt.p + 6, 0, 1);
+            launch(h, "scc_color_prop", kColorProp, gridFor(nq, blk), blk, 0, (const int32_t*)B.q[cur], nq,
+                   (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p, (const int32_t*)h->scc.p, color, stamp,
+                   it, B.q[cur ^ 1], h->dcnt.p + 6);
+            readCounters(h, h->dcnt.p + 6, 1, &nq);
+            cur ^= 1;
+        }
+        // roots seed one backward sweep restricted to their colour classes
+        fillI64(h, h->dcnt.p + 7, 0, 1);
+        launch(h, "scc_color_roots", kColorRoots, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
+               (const int32_t*)color, B.q[0], h->dcnt.p + 7);
+        int64_t nr;
+        readCounters(h, h->dcnt.p + 7, 1, &nr);
+        fillU64(h, B.vis, 0, B.W);
+        runReach(h, h->bw_beg.p, h->bw_adj.p, h->fw_beg.p, h->fw_adj.p, B, nr, 0, n, h->E,
+                 SameColour{h->scc.p, color}, NoVisit{});
+        launch(h, "scc_color_assign", kColorAssign, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
+               (const uint64_t*)B.vis, (const int32_t*)color, h->scc.p);
+        n = compactActive(h, list, n, false, alt);
+        std::swap(list, alt);
+        n = trimRounds(h, list, alt, n, 1);
+    }
+    fillI64(h, h->dcnt.p, 0, 1);
+    launch(h, "count_roots", kCountSelf, gridFor(V, blk), blk, 0, (const int32_t*)h->scc.p, V, h->dcnt.p);
+    int64_t ns;
+    readCounters(h, h->dcnt.p, 1, &ns);
+    h->n_scc = (int32_t)ns;
+}
+
+void runDag(girhip_graph* h) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    const int B = bitsFor((uint64_t)(V > 1 ? V - 1 : 1));
+    h->rank.ensure((size_t)(V > 0 ? V : 1));
+    h->dag_beg.ensure((size_t)V + 1);
+    S.iv[1].ensure((size_t)V + 1);
+    S.iv[2].ensure((size_t)V + 1);
+    int32_t* odeg = S.iv[1].p;
+    int32_t* ideg = S.iv[2].p;
+    fillI32(h, odeg, 0, (int64_t)V + 1);
+    fillI32(h, ideg, 0, (int64_t)V + 1);
+    int64_t ncross = 0;
+    if (h->E > 0) {
+        S.keys[0].ensure((size_t)h->E);
+        fillI64(h, h->dcnt.p + 9, 0, 1);
+        launch(h, "dag_keys", kDagKeys, gridFor(V, blk), blk, 0, (const int32_t*)h->fw_beg.p,
+               (const int32_t*)h->fw_adj.p, (const int32_t*)h->scc.p, V, B, S.keys[0].p, h->dcnt.p + 9);
+        readCounters(h, h->dcnt.p + 9, 1, &ncross);
+    }
+    int64_t nd = 0;
+    if (ncross > 0) {
+        radixSort(h, S.keys[0].p, nullptr, ncross, 2 * B);
+        S.iv[0].ensure((size_t)ncross);
+        S.off2.ensure((size_t)ncross + 1);
+        launch(h, "dag_flags", kRunFlags, gridFor(ncross, blk), blk, 0, (const uint64_t*)S.keys[0].p, ncross,
+               S.iv[0].p);
+        scanI32toI64(h, S.iv[0].p, S.off2.p, ncross);
+        readCounters(h, S.off2.p + ncross, 1, &nd);
+        S.q[2].ensure((size_t)nd);
+        launch(h, "dag_starts", kRunStarts, gridFor(ncross, blk), blk, 0, (const int32_t*)S.iv[0].p,
+               (const int64_t*)S.off2.p, ncross, S.q[2].p);
+        h->dag_src.ensure((size_t)nd);
+        h->dag_dst.ensure((size_t)nd);
+        h->dag_mult.ensure((size_t)nd);
+        launch(h, "dag_write", kDagWrite, gridFor(nd, blk), blk, 0, (const uint64_t*)S.keys[0].p,
+               (const int32_t*)S.q[2].p, nd, ncross, B, h->dag_src.p, h->dag_dst.p, h->dag_mult.p, odeg, ideg);
+    }
+    h->n_dag = nd;
+    // DAG CSR offsets by source SCC (arcs are sorted by src)
+    {
+        DevBuf<int64_t>& pos = S.off2;
+        pos.ensure((size_t)V + 1);
+        scanI32toI64(h, odeg, pos.p, V);
+        launch(h, "dag_beg", kI64toI32, gridFor((int64_t)V + 1, blk), blk, 0, (const int64_t*)pos.p, h->dag_beg.p,
+               (int64_t)V + 1);
+    }
+    // Kahn levels
+    ReachBufs R = reachBufs(h);
+    fillI64(h, h->dcnt.p + 10, 0, 1);
+    launch(h, "rank_seeds", kRankSeeds, gridFor(V, blk), blk, 0, (const int32_t*)h->scc.p, (const int32_t*)ideg, V,
+           h->rank.p, R.q[0], h->dcnt.p + 10);
+    int64_t nq;
+    readCounters(h, h->dcnt.p + 10, 1, &nq);
+    int cur = 0;
+    int32_t L = 0;
+    while (nq > 0) {
+        fillI64(h, h->dcnt.p + 11, 0, 1);
+        launch(h, "rank_kahn", kKahnStep, gridFor(nq, blk), blk, 0, (const int32_t*)R.q[cur], nq,
+               (const int32_t*)h->dag_beg.p, (const int32_t*)h->dag_dst.p, ideg, h->rank.p, L, R.q[cur ^ 1],
+               h->dcnt.p + 11);
+        readCounters(h, h->dcnt.p + 11, 1, &nq);
+        cur ^= 1;
+        ++L;
+    }
+}
+
+}  // namespace gir
+
+using namespace gir;
+
+extern "C" int girhip_scc_dag(girhip_graph* h, int32_t* scc, int32_t* n_scc, int64_t* n_dag) {
+    return guarded([&] {
+        useDevice(h);
+        tBegin(h);
+        runScc(h);
+        runDag(h);
+        tEnd(h, "scc_dag");
+        h->scc_req = true;
+        h->scc_ok = true;
+        if (scc) copyOut(h, scc, h->scc.p, sizeof(int32_t) * (size_t)h->V);
+        syncStream(h);
+        if (n_scc) *n_scc = h->n_scc;
+        if (n_dag) *n_dag = h->n_dag;
+    });
+}
+
+extern "C" int girhip_get_scc(girhip_graph* h, int32_t* scc, int32_t* n_scc, int64_t* n_dag) {
+    return guarded([&] {
+        useDevice(h);
+        if (!h->scc_req) raise(GIRHIP_ESTATE, "get_scc: call girhip_scc_dag first");
+        if (scc) copyOut(h, scc, h->scc.p, sizeof(int32_t) * (size_t)h->V);
+        syncStream(h);
+        if (n_scc) *n_scc = h->n_scc;
+        if (n_dag) *n_dag = h->n_dag;
+    });
+}
+
+extern "C" int girhip_get_dag(girhip_graph* h, int32_t* src, int32_t* dst, int32_t* mult) {
+    return guarded([&] {
+        useDevice(h);
+        if (!h->scc_req) raise(GIRHIP_ESTATE, "get_dag: call girhip_scc_dag first");
+        const size_t b = sizeof(int32_t) * (size_t)h->n_dag;
+        if (src) copyOut(h, src, h->dag_src.p, b);
+        if (dst) copyOut(h, dst, h->dag_dst.p, b);
+        if (mult) copyOut(h, mult, h->dag_mult.p, b);
+        syncStream(h);
+    });
+}
+
+extern "C" int girhip_get_ranks(girhip_graph* h, int64_t* rank) {
+    return guarded([&] {
+        useDevice(h);
+        if (!h->scc_req) raise(GIRHIP_ESTATE, "get_ranks: call girhip_scc_dag first");
+        if (!rank) raise(GIRHIP_EINVAL, "get_ranks: null output");
+        copyOut(h, rank, h->rank.p, sizeof(int64_t) * (size_t)h->V);
+        syncStream(h);
+    });
+}

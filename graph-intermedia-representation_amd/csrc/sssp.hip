@@ -1,0 +1,231 @@
+// Single-source shortest paths: replaces GIR::sssp / GIR::deltaStepping
+// (gir_query.cpp:629-703) for k sources.
+//
+// Delta-stepping with device-resident buckets in near/far form: the current
+// bucket [lo, lo + delta) is a queue that is relaxed until it empties (the
+// reference re-processes bucket k until nothing new lands in it); an
+// improvement that lands beyond the bucket goes, once (bitmap flag), to the
+// far pile.  When the bucket empties, the smallest far distance picks the
+// next non-empty bucket and one pass splits the far pile into that bucket's
+// queue, the remaining pile, and stale entries (settled earlier).
+// Relaxation is an atomicMin on 64-bit distances; duplicates in a queue are
+// skipped with a per-round stamp, like the reference's round_mark.  Weights:
+// one uint32 per forward slot, or the reference's hash GIR::edgeWeight.
+//
+// HBM traffic per source (algorithmic): 8V (init + final distances) + 8 n_s
+// (row offsets) + 16 m_s (adjacency 4 + weight 4 + distance 8 per arc).
+#include <vector>
+
+#include "common.h"
+
+namespace gir {
+
+std::vector<int32_t> hostSources(girhip_graph* h, const int32_t* src, int32_t k);
+
+constexpr uint64_t INF = ~0ULL;
+
+// cnt[0]: next near-queue length; cnt[1]: far-pile length (appended to).
+__global__ void kSsspRelax(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ fb,
+                           const int32_t* __restrict__ fa, const uint32_t* __restrict__ w, uint64_t* dist,
+                           int32_t* stamp, int32_t round, uint64_t hi, int32_t* __restrict__ qnear,
+                           int32_t* __restrict__ qfar, uint64_t* farbits, int64_t* cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = i < nin;
+    const int32_t u = act ? qin[i] : 0;
+    if (act && atomicExch(&stamp[u], round) == round) act = false;  // duplicate in this round
+    const uint64_t du = act ? ldAcq(&dist[u]) : 0;
+    const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+    waveRowsP(act, u, du, b, e, [&](int32_t p, uint64_t dp, int64_t j) {
+        const int32_t x = fa[j];
+        const uint64_t nd = dp + (w ? (uint64_t)w[j] : (uint64_t)hashWeight(p, x));
+        bool near = false, far = false;
+        if (nd < ldAcq(&dist[x])) {
+            const uint64_t old = atomicMin((unsigned long long*)&dist[x], (unsigned long long)nd);
+            if (nd < old) {
+                if (nd < hi) {
+                    near = true;
+                } else {
+                    const uint64_t bit = 1ULL << (x & 63);
+                    far = !(atomicOr((unsigned long long*)&farbits[x >> 6], (unsigned long long)bit) & bit);
+                }
+            }
+        }
+        const int64_t pn = wavePush(&cnt[0], near);
+        if (near) qnear[pn] = x;
+        const int64_t pf = wavePush(&cnt[1], far);
+        if (far) qfar[pf] = x;
+    });
+}
+
+// Smallest distance >= hi among far entries (entries below hi are settled).
+__global__ void kFarMin(const int32_t* __restrict__ qfar, int64_t n, const uint64_t* __restrict__ dist, uint64_t hi,
+                        unsigned long long* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long d = INF;
+    if (i < n) {
+        const uint64_t x = dist[qfar[i]];
+        if (x >= hi) d = x;
+    }
+    for (int s = WAVE / 2; s > 0; s >>= 1) {
+        unsigned long long o = __shfl_xor(d, s);
+        d = o < d ? o : d;
+    }
+    if (laneId() == 0 && d != INF) atomicMin(out, d);
+}
+
+// cnt[0]: near-queue length, cnt[1]: kept far length.
+__global__ void kFarSplit(const int32_t* __restrict__ qfar, int64_t n, const uint64_t* __restrict__ dist, uint64_t lo,
+                          uint64_t hi, uint64_t* farbits, int32_t* __restrict__ qnear, int32_t* __restrict__ qkeep,
+                          int64_t* cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool near = false, keep = false;
+    int32_t x = 0;
+    if (i < n) {
+        x = qfar[i];
+        const uint64_t d = dist[x];
+        keep = d >= hi;
+        near = d >= lo && d < hi;
+        if (!keep) atomicAnd((unsigned long long*)&farbits[x >> 6], ~(1ULL << (x & 63)));
+    }
+    const int64_t pn = wavePush(&cnt[0], near);
+    if (near) qnear[pn] = x;
+    const int64_t pk = wavePush(&cnt[1], keep);
+    if (keep) qkeep[pk] = x;
+}
+
+__global__ void kSetCounters(int64_t* c, int64_t a, int64_t b) {
+    if (threadIdx.x == 0) {
+        c[0] = a;
+        c[1] = b;
+    }
+}
+
+__global__ void kReachedArcsD(const uint64_t* __restrict__ dist, const int32_t* __restrict__ beg, int32_t V,
+                              int64_t* out) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t m = (v < V && dist[v] != INF) ? (int64_t)(beg[v + 1] - beg[v]) : 0;
+    for (int d = WAVE / 2; d > 0; d >>= 1) m += __shfl_xor(m, d);
+    if (laneId() == 0 && m) atomicAdd((unsigned long long*)out, (unsigned long long)m);
+}
+
+struct SsspBufs {
+    int32_t *qn[2], *qf[2];
+    uint64_t* farbits;
+    int32_t* stamp;
+    int64_t W;
+};
+
+static void ssspOne(girhip_graph* h, SsspBufs& Bf, int32_t s, uint64_t delta, uint64_t* dist, int32_t& round) {
+    const int blk = (int)h->prm.block;
+    const uint32_t* w = h->has_w ? h->w_fw.p : nullptr;
+    fillU64(h, dist, INF, h->V);
+    fillU64(h, dist + s, 0, 1);
+    fillI32(h, Bf.qn[0], s, 1);
+    int64_t* c = h->dcnt.p + 24;
+    int64_t nnear = 1, nfar = 0;
+    uint64_t lo = 0, hi = delta;
+    int cn = 0, cf = 0;
+    while (true) {
+        while (nnear > 0) {
+            ++round;
+            launch(h, "sssp_setcnt", kSetCounters, 1, 64, 0, c, (int64_t)0, nfar);
+            launch(h, "sssp_relax", kSsspRelax, gridFor(nnear, blk), blk, 0, (const int32_t*)Bf.qn[cn], nnear,
+                   (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p, w, dist, Bf.stamp, round, hi,
+                   Bf.qn[cn ^ 1], Bf.qf[cf], Bf.farbits, c);
+            int64_t r[2];
+            readCounters(h, c, 2, r);
+            nnear = r[0];
+            nfar = r[1];
+            cn ^= 1;
+        }
+        if (nfar == 0) break;
+        fillU64(h, (uint64_t*)(c + 2), INF, 1);
+        launch(h, "sssp_farmin", kFarMin, gridFor(nfar, blk), blk, 0, (const int32_t*)Bf.qf[cf], nfar,
+               (const uint64_t*)dist, hi, (unsigned long long*)(c + 2));
+        int64_t mn;
+        readCounters(h, c + 2, 1, &mn);
+        const uint64_t m = (uint64_t)mn;
+        lo = (m == INF) ? INF : (m / delta) * delta;
+        hi = (m == INF || lo > INF - delta) ? INF : lo + delta;
+        launch(h, "sssp_setcnt", kSetCounters, 1, 64, 0, c, (int64_t)0, (int64_t)0);
+        launch(h, "sssp_split", kFarSplit, gridFor(nfar, blk), blk, 0, (const int32_t*)Bf.qf[cf], nfar,
+               (const uint64_t*)dist, lo, hi, Bf.farbits, Bf.qn[cn], Bf.qf[cf ^ 1], c);
+        int64_t r[2];
+        readCounters(h, c, 2, r);
+        nnear = r[0];
+        nfar = r[1];
+        cf ^= 1;
+        if (m == INF) break;  // every far entry was stale
+    }
+}
+
+}  // namespace gir
+
+using namespace gir;
+
+extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint64_t delta, uint64_t* dist,
+                           int64_t* edges_out) {
+    return guarded([&] {
+        useDevice(h);
+        if (k < 0 || (k > 0 && !src)) raise(GIRHIP_EINVAL, "sssp: bad source list");
+        if (delta == 0) delta = (uint64_t)h->prm.sssp_delta;
+        if (delta == 0) raise(GIRHIP_EINVAL, "sssp: delta must be > 0");
+        const int32_t V = h->V;
+        std::vector<int32_t> s = hostSources(h, src, k);
+        const size_t n = (size_t)k * (size_t)V;
+        uint64_t* out = dist;
+        const bool direct = dist && isDevicePtr(dist);
+        Scratch& S = scratch(h);
+        if (!direct) {
+            S.stage.ensure(sizeof(uint64_t) * (n > 0 ? n : 1));
+            out = (uint64_t*)S.stage.p;
+        }
+        const size_t Vs = (size_t)(V > 0 ? V : 1);
+        S.q[0].ensure(Vs);
+        S.q[1].ensure(Vs);
+        S.iv[0].ensure(Vs);
+        S.iv[1].ensure(Vs);
+        S.iv[2].ensure(Vs);
+        S.bits[0].ensure((Vs + 63) / 64);
+        SsspBufs Bf;
+        // near queues may hold duplicates: size them by arcs + vertices
+        const size_t qcap = (size_t)h->E + Vs;
+        S.keys[0].ensure((qcap + 1) / 2);  // reinterpret 2 x int32 per element
+        S.keys[1].ensure((qcap + 1) / 2);
+        Bf.qn[0] = (int32_t*)S.keys[0].p;
+        Bf.qn[1] = (int32_t*)S.keys[1].p;
+        Bf.qf[0] = S.q[0].p;
+        Bf.qf[1] = S.q[1].p;
+        Bf.stamp = S.iv[2].p;
+        Bf.farbits = S.bits[0].p;
+        Bf.W = (int64_t)(Vs + 63) / 64;
+        tBegin(h);
+        fillI32(h, Bf.stamp, 0, V);
+        fillU64(h, Bf.farbits, 0, Bf.W);
+        int32_t round = 0;
+        for (int32_t i = 0; i < k; ++i) {
+            uint64_t* row = out + (size_t)i * V;
+            if (s[i] < 0 || s[i] >= V) {
+                fillU64(h, row, INF, V);
+                continue;
+            }
+            ssspOne(h, Bf, s[i], delta, row, round);
+        }
+        tEnd(h, "sssp");
+        if (edges_out) {
+            std::vector<int64_t> e(k);
+            for (int32_t i0 = 0; i0 < k; i0 += 32) {
+                const int c = std::min<int32_t>(32, k - i0);
+                fillI64(h, h->dcnt.p + 32, 0, 32);
+                for (int32_t i = i0; i < i0 + c; ++i)
+                    launch(h, "reached_arcs", kReachedArcsD, gridFor(V, 256), 256, 0,
+                           (const uint64_t*)(out + (size_t)i * V), (const int32_t*)h->fw_beg.p, V,
+                           h->dcnt.p + 32 + (i - i0));
+                readCounters(h, h->dcnt.p + 32, c, e.data() + i0);
+            }
+            HIPCHK(hipMemcpy(edges_out, e.data(), sizeof(int64_t) * (size_t)k, hipMemcpyDefault));
+        }
+        if (dist && !direct) copyOut(h, dist, out, sizeof(uint64_t) * n);
+        syncStream(h);
+    });
+}

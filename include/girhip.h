@@ -1,0 +1,178 @@
+/* girhip.h -- C ABI of the MI355X (gfx950) GIR construction/query path.
+ *
+ * One opaque handle (girhip_graph) owns a directed graph resident in HBM as
+ * forward (out) and backward (in) int32 CSR, optional per-arc weights, and
+ * every layer built from it (WCC labels, SCC labels, condensation DAG with
+ * arc multiplicities and topological ranks).  All compute runs as HIP
+ * kernels on the handle's device and stream; there is no CPU fallback: if
+ * no usable GPU is present every entry point fails with GIRHIP_ENODEV.
+ *
+ * Conventions
+ *   - Return value: 0 (GIRHIP_OK) or a status code below; the message of the
+ *     last failure on the calling thread is girhip_last_error().
+ *   - Pointers may be host memory or device memory of the handle's device
+ *     (resolved with hipPointerGetAttributes).  Device outputs are written by
+ *     the kernels in place; host outputs are staged through device memory.
+ *     Calls return after the result is complete (the stream is synchronised).
+ *   - Component ids are canonical: a component is named by its minimum
+ *     vertex id.  (The reference names WCCs by union-find root and SCCs by
+ *     Tarjan emission index, gir.cpp:649-652 and 809-816; its own tests
+ *     compare partitions, gir_test.cpp:365-374.)
+ *   - A handle is not thread-safe (like the reference GIR class, whose
+ *     query scratch is mutable); use one thread per handle.
+ *   - Invalid updates (u == v or out of range) are dropped silently and
+ *     out-of-range query sources give all-unreachable rows, as in the
+ *     reference (gir.cpp:3534-3540, gir_query.cpp:112, 699).
+ */
+#ifndef GIRHIP_H
+#define GIRHIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GIRHIP_OK = 0,
+    GIRHIP_EINVAL = 1, /* bad argument or malformed CSR */
+    GIRHIP_EHIP = 2,   /* HIP runtime error */
+    GIRHIP_ENOMEM = 3, /* device allocation failed */
+    GIRHIP_ESTATE = 4, /* layer not built yet (e.g. get_dag before scc_dag) */
+    GIRHIP_ENODEV = 5  /* no usable GPU */
+};
+
+typedef struct girhip_graph girhip_graph;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* girhip_last_error(void);
+
+/* Replaces GIR::GIR(graph*) / GIR(path, path) + initAfterLoad
+ * (gir.h:93-96, gir.cpp:38-63) and graph::graph's validation
+ * (graph.cpp:95-114).  Copies the CSR to the current HIP device.  Rows need
+ * not be sorted and may hold duplicates: they are sorted and deduplicated on
+ * the device as normalizeBaseCsr does (gir.cpp:95-130); if the deduplicated
+ * forward and backward CSR disagree in arc count the call fails (the
+ * reference exits there).  bw_beg/bw_adj may both be NULL: the backward CSR
+ * is then built on the device as the transpose of the forward one.
+ * Malformed offsets or ids fail with GIRHIP_EINVAL. */
+int girhip_create(const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg, const int32_t* bw_adj,
+                  int32_t V, int64_t E, girhip_graph** out);
+
+/* Replaces GIR::~GIR (gir.cpp:65-68).  NULL is accepted. */
+int girhip_destroy(girhip_graph* h);
+
+/* Replaces GIR::vertexCount (gir.h:136) and graph::edge_count. */
+int girhip_size(const girhip_graph* h, int32_t* V, int64_t* E);
+
+/* Copy the current canonical CSR out (any of the four may be NULL).
+ * Replaces getDirectedGraph()/getDirectedOutNeighbors/InNeighbors
+ * (gir.h:153-158).  Arrays hold V+1 / E entries. */
+int girhip_get_csr(girhip_graph* h, int32_t* fw_beg, int32_t* fw_adj, int32_t* bw_beg, int32_t* bw_adj);
+
+/* Per-arc weights aligned to the current forward CSR (E entries, each > 0).
+ * NULL restores the reference's weight function GIR::edgeWeight
+ * (gir_query.cpp:50-54): 1 + fmix64((u << 32) | v) mod 100.  Explicit
+ * weights are carried through later update batches: an inserted arc gets
+ * GIR::edgeWeight unless a weight is given with the batch. */
+int girhip_set_weights(girhip_graph* h, const uint32_t* w_fw);
+
+/* Replaces GIR::buildWCC + getNodeToWcc/wccOf (gir.cpp:593-655, 4122-4126).
+ * label (V entries, may be NULL to keep it on the device only) receives the
+ * canonical WCC id of every vertex; *n_wcc (may be NULL) the count. */
+int girhip_wcc(girhip_graph* h, int32_t* label, int32_t* n_wcc);
+
+/* Replaces GIR::buildSCC_DAG + getNodeToScc/sccOf (gir.cpp:798-839,
+ * 4117-4121).  scc (V entries, may be NULL) receives canonical SCC ids;
+ * *n_scc and *n_dag (may be NULL) the SCC count and the number of distinct
+ * condensation arcs. */
+int girhip_scc_dag(girhip_graph* h, int32_t* scc, int32_t* n_scc, int64_t* n_dag);
+
+/* Maintained layers without rebuilding: replace GIR::getNodeToWcc / wccOf
+ * and getNodeToScc / sccOf (gir.cpp:4117-4126, gir.h:130-131).  They describe
+ * the graph as of the last build, applyBatch or sync (like the reference's
+ * getters while ingested changes are pending).  GIRHIP_ESTATE if the layer
+ * was never built.  Any output may be NULL. */
+int girhip_get_wcc(girhip_graph* h, int32_t* label, int32_t* n_wcc);
+int girhip_get_scc(girhip_graph* h, int32_t* scc, int32_t* n_scc, int64_t* n_dag);
+
+/* Replaces GIR::getDAG (gir.cpp:4127-4132) and the private dag_out
+ * multiplicities (gir.cpp:818-831).  Arrays hold n_dag entries (any may be
+ * NULL): arc src -> dst between canonical SCC ids, sorted by (src, dst),
+ * mult = number of graph arcs between the two SCCs. */
+int girhip_get_dag(girhip_graph* h, int32_t* src, int32_t* dst, int32_t* mult);
+
+/* Replaces GIR::sccRank (gir.h:131, ranks set at gir.cpp:833-836).  rank
+ * holds V entries indexed by canonical SCC id: distinct values with
+ * rank[a] < rank[b] for every DAG arc a -> b; -1 at non-representatives. */
+int girhip_get_ranks(girhip_graph* h, int64_t* rank);
+
+/* Replaces GIR::bfs (gir_query.cpp:108-294) for k sources.  levels holds
+ * k*V int32 (row i = hops from src[i], -1 unreachable).  May be NULL to
+ * leave the result only in girhip_last_levels' device buffer.  When
+ * edges_out is not NULL, edges_out[i] receives m_s: the out-arc count summed
+ * over the vertices reached from src[i] (the traversed-arc count of the
+ * GTEPS figure). */
+int girhip_bfs(girhip_graph* h, const int32_t* src, int32_t k, int32_t* levels, int64_t* edges_out);
+
+/* Replaces GIR::sssp (gir_query.cpp:695-703) for k sources: delta-stepping
+ * with bucket width delta (0 = the handle's tuned default, 16 unless
+ * changed by girhip_set_param).  dist holds k*V uint64, UINT64_MAX =
+ * unreachable; edges_out as for girhip_bfs. */
+int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint64_t delta, uint64_t* dist,
+                int64_t* edges_out);
+
+typedef struct girhip_batch_stats {
+    int64_t inserted; /* arcs added to the edge set */
+    int64_t deleted;  /* arcs removed */
+    int64_t noop;     /* inserts of present arcs (the reference's C4_NOOP count) */
+    int64_t invalid;  /* dropped ops (u == v or out of range) */
+} girhip_batch_stats;
+
+/* Replaces GIR::applyBatch (gir.cpp:3634-4111) and, for n == 1,
+ * GIR::updateEdgeDirected / insertEdge / deleteEdge (gir.cpp:3043-3133).
+ * ops i: arc u[i] -> v[i], insert if ins[i] != 0 else delete; last op per
+ * arc wins and ops are classified against the current edge set
+ * (normalizeBatch, gir.cpp:3523-3586).  w (may be NULL) gives the weight of
+ * inserted arcs when explicit weights are set.  Every layer already built
+ * (WCC, SCC + DAG + ranks) is repaired so that it equals a from-scratch build
+ * on the new edge set.  stats may be NULL. */
+int girhip_apply_batch(girhip_graph* h, const int32_t* u, const int32_t* v, const uint8_t* ins, const uint32_t* w,
+                       int64_t n, girhip_batch_stats* stats);
+
+/* Replaces GIR::ingestBatch (gir.cpp:3588-3597): the edge store only; the
+ * WCC/SCC/DAG layers describe the graph as of the last sync until
+ * girhip_sync.  BFS/SSSP see every ingested change. */
+int girhip_ingest_batch(girhip_graph* h, const int32_t* u, const int32_t* v, const uint8_t* ins, const uint32_t* w,
+                        int64_t n, girhip_batch_stats* stats);
+
+/* Replaces GIR::sync (gir.cpp:3602-3632) / GIR::synced (gir.h:117). */
+int girhip_sync(girhip_graph* h);
+int girhip_synced(const girhip_graph* h, int32_t* is_synced);
+
+/* Replaces GIR::hasDirectedEdge (gir.h:160): *present = 1 if u -> v. */
+int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
+
+/* Tuning knobs (the reference's setQueryGranularity and constants,
+ * gir.h:170-240, gir_query.cpp:161).  Names:
+ *   "bfs_alpha" (15), "bfs_beta" (18), "bfs_min_reach" (4096),
+ *   "sssp_delta" (16), "block" (256), "wcc_rounds" (2),
+ *   "scc_trim_rounds" (3), "scc_pivot" (1 = max outdeg*indeg, 0 = none),
+ *   "scc_color_min" (0: vertices left before colouring replaces FW-BW)
+ * Values only change work order; results never depend on them. */
+int girhip_set_param(girhip_graph* h, const char* name, double value);
+int girhip_get_param(const girhip_graph* h, const char* name, double* value);
+
+/* Device time (ms, HIP events) of the kernels the last call of each kind
+ * launched: kind "wcc", "scc_dag", "bfs", "sssp", "batch". */
+int girhip_last_ms(const girhip_graph* h, const char* kind, double* ms);
+
+/* Per-kernel device time when the "profile" parameter is 1: every launch is
+ * bracketed by HIP events on the handle's stream; name -> launch count and
+ * summed ms since the last reset.  No counterpart in the reference (its
+ * timing is std::chrono around whole calls, gir.cpp:800-811). */
+int girhip_kernel_stats(girhip_graph* h, const char* name, int64_t* count, double* ms);
+int girhip_reset_kernel_stats(girhip_graph* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,171 @@
+"""ctypes binding of oracle/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY: the CPU restatement of the reference path used as
+the checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline.
+The product library never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_BIN = os.path.join(HERE, "_ref", "gir_ref")
+
+_vp = ctypes.c_void_p
+_i32, _i64, _u64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
+_SIGS = {
+    "orc_edge_weight": (ctypes.c_uint32, [_i32, _i32]),
+    "orc_normalize_csr": (ctypes.c_int, [_i32, _vp, _vp, ctypes.POINTER(_i64)]),
+    "orc_wcc": (ctypes.c_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
+    "orc_scc": (ctypes.c_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
+    "orc_dag": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64)]),
+    "orc_scc_ranks": (ctypes.c_int, [_i32, _vp, _vp, _vp]),
+    "orc_bfs": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _i32, _vp]),
+    "orc_sssp": (ctypes.c_int, [_i32, _vp, _vp, _vp, _i32, _u64, _vp]),
+    "orc_normalize_batch": (ctypes.c_int, [_i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64),
+                                           _vp, _vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "orc_apply_batch": (ctypes.c_int, [_i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                       ctypes.POINTER(_i64)]),
+    "orc_bfs_textbook": (ctypes.c_int, [_i32, _vp, _vp, _i32, _vp]),
+    "orc_dijkstra": (ctypes.c_int, [_i32, _vp, _vp, _vp, _i32, _vp]),
+    "orc_wcc_bfs": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp]),
+    "orc_scc_kosaraju": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp]),
+    "orc_num_threads": (ctypes.c_int, []),
+}
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built (make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        for n, (r, a) in _SIGS.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = r, a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise ValueError(f"oracle {what} failed ({rc})")
+
+
+def edge_weight(u, v):
+    return lib().orc_edge_weight(int(u), int(v))
+
+
+def normalize_csr(beg, adj):
+    beg = beg.astype(np.int32).copy()
+    adj = adj.astype(np.int32).copy()
+    E = _i64()
+    _chk(lib().orc_normalize_csr(len(beg) - 1, _p(beg), _p(adj), ctypes.byref(E)), "normalize_csr")
+    return beg, adj[:E.value].copy()
+
+
+def wcc(fb, fa):
+    V = len(fb) - 1
+    lab, n = np.empty(V, np.int32), _i32()
+    _chk(lib().orc_wcc(V, _p(fb), _p(fa), _p(lab), ctypes.byref(n)), "wcc")
+    return lab, n.value
+
+
+def scc(fb, fa):
+    V = len(fb) - 1
+    lab, n = np.empty(V, np.int32), _i32()
+    _chk(lib().orc_scc(V, _p(fb), _p(fa), _p(lab), ctypes.byref(n)), "scc")
+    return lab, n.value
+
+
+def dag(fb, fa, lab):
+    V = len(fb) - 1
+    n = _i64()
+    _chk(lib().orc_dag(V, _p(fb), _p(fa), _p(lab), None, None, None, ctypes.byref(n)), "dag")
+    s, d, m = (np.empty(max(n.value, 1), np.int32) for _ in range(3))
+    _chk(lib().orc_dag(V, _p(fb), _p(fa), _p(lab), _p(s), _p(d), _p(m), ctypes.byref(n)), "dag")
+    return s[:n.value], d[:n.value], m[:n.value]
+
+
+def scc_ranks(fb, fa):
+    V = len(fb) - 1
+    r = np.empty(V, np.int64)
+    _chk(lib().orc_scc_ranks(V, _p(fb), _p(fa), _p(r)), "scc_ranks")
+    return r
+
+
+def bfs(fb, fa, bb, ba, src):
+    V = len(fb) - 1
+    lv = np.empty(V, np.int32)
+    _chk(lib().orc_bfs(V, _p(fb), _p(fa), _p(bb), _p(ba), int(src), _p(lv)), "bfs")
+    return lv
+
+
+def sssp(fb, fa, w, src, delta=16):
+    V = len(fb) - 1
+    d = np.empty(V, np.uint64)
+    _chk(lib().orc_sssp(V, _p(fb), _p(fa), _p(w), int(src), int(delta), _p(d)), "sssp")
+    return d
+
+
+def normalize_batch(fb, fa, u, v, ins):
+    V = len(fb) - 1
+    u, v = np.ascontiguousarray(u, np.int32), np.ascontiguousarray(v, np.int32)
+    ins = np.ascontiguousarray(ins, np.uint8)
+    n = len(u)
+    iu, iv, du, dv = (np.empty(max(n, 1), np.int32) for _ in range(4))
+    ni, nd, nn = _i64(), _i64(), _i64()
+    _chk(lib().orc_normalize_batch(V, _p(fb), _p(fa), n, _p(u), _p(v), _p(ins), _p(iu), _p(iv), ctypes.byref(ni),
+                                   _p(du), _p(dv), ctypes.byref(nd), ctypes.byref(nn)), "normalize_batch")
+    return (iu[:ni.value], iv[:ni.value]), (du[:nd.value], dv[:nd.value]), nn.value
+
+
+def apply_batch(fb, fa, u, v, ins):
+    V = len(fb) - 1
+    u, v = np.ascontiguousarray(u, np.int32), np.ascontiguousarray(v, np.int32)
+    ins = np.ascontiguousarray(ins, np.uint8)
+    n = len(u)
+    cap = len(fa) + n + 1
+    nfb, nbb = np.empty(V + 1, np.int32), np.empty(V + 1, np.int32)
+    nfa, nba = np.empty(cap, np.int32), np.empty(cap, np.int32)
+    E = _i64()
+    _chk(lib().orc_apply_batch(V, _p(fb), _p(fa), n, _p(u), _p(v), _p(ins), _p(nfb), _p(nfa), _p(nbb), _p(nba),
+                               ctypes.byref(E)), "apply_batch")
+    return nfb, nfa[:E.value].copy(), nbb, nba[:E.value].copy()
+
+
+def bfs_textbook(fb, fa, src):
+    lv = np.empty(len(fb) - 1, np.int32)
+    _chk(lib().orc_bfs_textbook(len(fb) - 1, _p(fb), _p(fa), int(src), _p(lv)), "bfs_textbook")
+    return lv
+
+
+def dijkstra(fb, fa, w, src):
+    d = np.empty(len(fb) - 1, np.uint64)
+    _chk(lib().orc_dijkstra(len(fb) - 1, _p(fb), _p(fa), _p(w), int(src), _p(d)), "dijkstra")
+    return d
+
+
+def wcc_bfs(fb, fa, bb, ba):
+    lab = np.empty(len(fb) - 1, np.int32)
+    _chk(lib().orc_wcc_bfs(len(fb) - 1, _p(fb), _p(fa), _p(bb), _p(ba), _p(lab)), "wcc_bfs")
+    return lab
+
+
+def scc_kosaraju(fb, fa, bb, ba):
+    lab = np.empty(len(fb) - 1, np.int32)
+    _chk(lib().orc_scc_kosaraju(len(fb) - 1, _p(fb), _p(fa), _p(bb), _p(ba), _p(lab)), "scc_kosaraju")
+    return lab
+
+
+def num_threads():
+    return lib().orc_num_threads()

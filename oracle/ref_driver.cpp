@@ -1,0 +1,199 @@
+// Driver for the compiled reference (TEST INFRASTRUCTURE ONLY).
+//
+// Links the reference's own gir.cpp / gir_query.cpp / graph.cpp /
+// graph_undirected.cpp, built from where they lie by oracle/Makefile into
+// oracle/_ref/.  It loads a four-file .bin CSR through the reference's
+// GIR(path, path) constructor (gir.cpp:38-44, graph.cpp:2-120), runs the
+// hot path through the reference's public API (gir.h:98-240) and writes the
+// results with canonical (minimum-vertex) component ids:
+//
+//   gir_ref CSR_DIR OUT_DIR [--bfs SRC.bin] [--sssp SRC.bin] [--delta D]
+//           [--batch OPS.bin] [--pairs PAIRS.bin] [--no-output] [--reps R]
+//
+//   wcc.bin  int32[V]   buildWCC labels (gir.cpp:593-655), canonical
+//   scc.bin  int32[V]   buildSCC_DAG node_scc (gir.cpp:798-839), canonical
+//   dag.bin  int32[2*k] getDAG() arcs (src, dst) in canonical ids, sorted
+//   bfs.bin  int32[k*V] GIR::bfs levels per source (gir_query.cpp:108-294)
+//   sssp.bin uint64[k*V] GIR::sssp distances, delta D (gir_query.cpp:695-703)
+//   weight.bin uint32[k] GIR::edgeWeight of each (u, v) in PAIRS.bin
+//   post_{wcc,scc,dag}.bin  the same after GIR::applyBatch of OPS.bin
+//                        (int32 triples u, v, insert) with blocks on demand
+// One JSON line per timed phase goes to stdout, prefixed "@@".
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <omp.h>
+
+#include "gir.h"
+#include "graph.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double msSince(Clock::time_point t0) { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+
+template <class T>
+std::vector<T> readVec(const std::string& p) {
+    std::vector<T> v;
+    FILE* f = std::fopen(p.c_str(), "rb");
+    if (!f) { std::fprintf(stderr, "cannot open %s\n", p.c_str()); std::exit(2); }
+    std::fseek(f, 0, SEEK_END);
+    long n = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    v.resize(n / sizeof(T));
+    if (!v.empty() && std::fread(v.data(), sizeof(T), v.size(), f) != v.size()) { std::fprintf(stderr, "short read %s\n", p.c_str()); std::exit(2); }
+    std::fclose(f);
+    return v;
+}
+template <class T>
+void writeVec(const std::string& p, const T* d, size_t n) {
+    FILE* f = std::fopen(p.c_str(), "wb");
+    if (!f || (n && std::fwrite(d, sizeof(T), n, f) != n)) { std::fprintf(stderr, "cannot write %s\n", p.c_str()); std::exit(2); }
+    std::fclose(f);
+}
+
+// Canonical labels from an id -> group map: each group named by its minimum vertex.
+std::vector<int> canon(int V, const std::vector<int>& raw) {
+    int maxid = 0;
+    for (int x : raw) maxid = std::max(maxid, x);
+    std::vector<int> mn(maxid + 1, INT32_MAX), out(V);
+    for (int v = 0; v < V; ++v) mn[raw[v]] = std::min(mn[raw[v]], v);
+    for (int v = 0; v < V; ++v) out[v] = mn[raw[v]];
+    return out;
+}
+
+struct Layers {
+    std::vector<int> wcc, scc, dag;
+    bool rank_ok = true;
+};
+
+Layers snapshot(GIR& g) {
+    const int V = g.vertexCount();
+    Layers L;
+    std::vector<int> w(V), s(V);
+    for (int v = 0; v < V; ++v) { w[v] = g.wccOf(v); s[v] = g.sccOf(v); }
+    L.wcc = canon(V, w);
+    L.scc = canon(V, s);
+    // SCC id -> canonical id
+    const auto& sccs = g.getSCCs();
+    std::vector<int> idmap(sccs.size(), -1);
+    for (size_t i = 0; i < sccs.size(); ++i)
+        if (!sccs[i].empty()) idmap[i] = L.scc[sccs[i][0]];
+    auto dag = g.getDAG();
+    std::vector<std::pair<int, int>> arcs;
+    for (size_t a = 0; a < dag.size(); ++a)
+        for (int b : dag[a]) {
+            if (idmap[a] < 0 || idmap[b] < 0) continue;
+            arcs.push_back({idmap[a], idmap[b]});
+            if (!(g.sccRank((int)a) < g.sccRank(b))) L.rank_ok = false;
+        }
+    std::sort(arcs.begin(), arcs.end());
+    for (auto& p : arcs) { L.dag.push_back(p.first); L.dag.push_back(p.second); }
+    return L;
+}
+
+void writeLayers(const std::string& out, const std::string& pre, const Layers& L) {
+    writeVec(out + pre + "wcc.bin", L.wcc.data(), L.wcc.size());
+    writeVec(out + pre + "scc.bin", L.scc.data(), L.scc.size());
+    writeVec(out + pre + "dag.bin", L.dag.data(), L.dag.size());
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--pairs F] [--no-output] [--reps R]\n");
+        return 2;
+    }
+    std::string dir = argv[1], out = argv[2];
+    if (dir.back() != '/') dir += '/';
+    if (out.back() != '/') out += '/';
+    std::string bfs_f, sssp_f, batch_f, pairs_f;
+    uint64_t delta = 16;
+    bool write = true;
+    int reps = 1;
+    for (int i = 3; i < argc; ++i) {
+        std::string a = argv[i];
+        auto next = [&]() { if (i + 1 >= argc) { std::fprintf(stderr, "missing value for %s\n", a.c_str()); std::exit(2); } return std::string(argv[++i]); };
+        if (a == "--bfs") bfs_f = next();
+        else if (a == "--sssp") sssp_f = next();
+        else if (a == "--delta") delta = std::strtoull(next().c_str(), nullptr, 10);
+        else if (a == "--batch") batch_f = next();
+        else if (a == "--pairs") pairs_f = next();
+        else if (a == "--no-output") write = false;
+        else if (a == "--reps") reps = std::atoi(next().c_str());
+        else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
+    }
+    std::printf("@@{\"threads\": %d}\n", omp_get_max_threads());
+    if (!pairs_f.empty()) {
+        auto pr = readVec<int>(pairs_f);
+        std::vector<uint32_t> w(pr.size() / 2);
+        for (size_t i = 0; i < w.size(); ++i) w[i] = GIR::edgeWeight(pr[2 * i], pr[2 * i + 1]);
+        writeVec(out + "weight.bin", w.data(), w.size());
+    }
+    GIR g(dir, dir);
+    const int V = g.vertexCount();
+    for (int r = 0; r < reps; ++r) {
+        auto t0 = Clock::now();
+        g.buildWCC();
+        double t_wcc = msSince(t0);
+        auto t1 = Clock::now();
+        g.buildSCC_DAG();
+        double t_scc = msSince(t1);
+        std::printf("@@{\"phase\": \"build\", \"wcc_ms\": %.3f, \"scc_dag_ms\": %.3f}\n", t_wcc, t_scc);
+    }
+    Layers L = snapshot(g);
+    std::printf("@@{\"rank_ok\": %d, \"n_dag\": %zu}\n", (int)L.rank_ok, L.dag.size() / 2);
+    if (write) writeLayers(out, "", L);
+    if (!bfs_f.empty()) {
+        auto src = readVec<int>(bfs_f);
+        std::vector<int> lv;
+        if (write) lv.reserve((size_t)src.size() * V);
+        auto t0 = Clock::now();
+        long long edges = 0;
+        for (int s : src) {
+            auto l = g.bfs(s);
+            if (write) lv.insert(lv.end(), l.begin(), l.end());
+            else edges += l[0];  // keep the result live
+        }
+        std::printf("@@{\"phase\": \"bfs\", \"sources\": %zu, \"ms\": %.3f, \"keep\": %lld}\n", src.size(), msSince(t0), edges);
+        if (write) writeVec(out + "bfs.bin", lv.data(), lv.size());
+    }
+    if (!sssp_f.empty()) {
+        auto src = readVec<int>(sssp_f);
+        std::vector<uint64_t> dv;
+        auto t0 = Clock::now();
+        unsigned long long keep = 0;
+        for (int s : src) {
+            auto d = g.sssp(s, delta);
+            if (write) dv.insert(dv.end(), d.begin(), d.end());
+            else keep += d[0];
+        }
+        std::printf("@@{\"phase\": \"sssp\", \"sources\": %zu, \"ms\": %.3f, \"keep\": %llu}\n", src.size(), msSince(t0), keep);
+        if (write) writeVec(out + "sssp.bin", dv.data(), dv.size());
+    }
+    if (!batch_f.empty()) {
+        // The reference's drivers build the block layer before any update
+        // (gir_selftest.cpp:115-187, bench_main.cpp:164), then may switch
+        // blocks to on-demand (gir_selftest.cpp:161, bench_main.cpp:362).
+        g.buildBCT();
+        g.setBlocksOnDemand(true);
+        auto ops = readVec<int>(batch_f);
+        std::vector<GIR::Update> b(ops.size() / 3);
+        for (size_t i = 0; i < b.size(); ++i) b[i] = {ops[3 * i], ops[3 * i + 1], ops[3 * i + 2] != 0};
+        auto t0 = Clock::now();
+        g.applyBatch(b);
+        std::printf("@@{\"phase\": \"batch\", \"ops\": %zu, \"ms\": %.3f}\n", b.size(), msSince(t0));
+        Layers P = snapshot(g);
+        std::printf("@@{\"post_rank_ok\": %d}\n", (int)P.rank_ok);
+        if (write) writeLayers(out, "post_", P);
+    }
+    return 0;
+}

@@ -1,0 +1,116 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference.
+
+Runs oracle/_ref/gir_ref (the reference's own gir.cpp / gir_query.cpp /
+graph.cpp built from their sources by `make -C oracle ref`) on small seeded
+graphs and stores inputs and outputs as compressed .npz files:
+
+  csr            fw_beg, fw_adj, bw_beg, bw_adj (canonical)
+  wcc, scc       canonical labels (GIR::buildWCC / buildSCC_DAG)
+  dag            condensation arcs (src, dst) from GIR::getDAG, sorted
+  bfs_src, bfs   GIR::bfs levels per source
+  sssp_src, sssp GIR::sssp distances per source (GIR::edgeWeight weights, delta 16)
+  pairs, weight  GIR::edgeWeight of sampled (u, v)
+  ops            int32 (u, v, insert) batch; post_wcc / post_scc / post_dag
+                 after GIR::applyBatch (blocks on demand)
+
+Usage (in the build container, where the reference sources exist):
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "graph-intermedia-representation_amd"))
+import girhip  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "gir_ref")
+OUT = os.path.dirname(os.path.abspath(__file__))
+EXAMPLE = "/root/reference/data/example_graph"
+
+
+def batch_ops(fb, fa, n, seed):
+    """Mixed batch: deletes of present arcs, inserts of random pairs, repeats
+    of the same arc (last op wins), insert/delete cancelling pairs and
+    invalid ops (self loops, out of range)."""
+    rng = np.random.default_rng(seed)
+    V = len(fb) - 1
+    E = len(fa)
+    src_of = np.repeat(np.arange(V, dtype=np.int32), np.diff(fb))
+    ops = []
+    for _ in range(n):
+        r = rng.random()
+        if r < 0.4 and E:
+            e = rng.integers(E)
+            ops.append((src_of[e], fa[e], 0))
+        elif r < 0.8:
+            ops.append((rng.integers(V), rng.integers(V), 1))
+        elif r < 0.9 and ops:
+            u, v, _ = ops[rng.integers(len(ops))]
+            ops.append((u, v, int(rng.integers(2))))
+        elif r < 0.95:
+            x = rng.integers(V)
+            ops.append((x, x, 1))
+        else:
+            ops.append((V + 3, rng.integers(V), 1))
+    return np.array(ops, dtype=np.int32).reshape(-1, 3)
+
+
+def run_case(name, V, fb, fa, bb, ba, n_bfs, n_sssp, n_ops, seed):
+    with tempfile.TemporaryDirectory() as td:
+        g = os.path.join(td, "g")
+        girhip.write_csr(g, fb, fa, bb, ba)
+        rng = np.random.default_rng(seed)
+        has_out = np.nonzero(np.diff(fb) > 0)[0].astype(np.int32)
+        bsrc = rng.choice(has_out, size=min(n_bfs, len(has_out)), replace=False).astype(np.int32)
+        ssrc = bsrc[:n_sssp].copy()
+        pairs = rng.integers(0, V, size=(64, 2)).astype(np.int32)
+        ops = batch_ops(fb, fa, n_ops, seed + 1)
+        bsrc.tofile(os.path.join(td, "b.bin"))
+        ssrc.tofile(os.path.join(td, "s.bin"))
+        pairs.tofile(os.path.join(td, "p.bin"))
+        ops.tofile(os.path.join(td, "o.bin"))
+        out = os.path.join(td, "out") + "/"
+        os.makedirs(out)
+        subprocess.run([REF, g + "/", out, "--bfs", os.path.join(td, "b.bin"), "--sssp", os.path.join(td, "s.bin"),
+                        "--pairs", os.path.join(td, "p.bin"), "--batch", os.path.join(td, "o.bin")],
+                       check=True, stdout=subprocess.DEVNULL)
+
+        def rd(f, dt):
+            return np.fromfile(os.path.join(out, f), dtype=dt)
+
+        np.savez_compressed(
+            os.path.join(OUT, f"{name}.npz"), fw_beg=fb, fw_adj=fa, bw_beg=bb, bw_adj=ba,
+            wcc=rd("wcc.bin", np.int32), scc=rd("scc.bin", np.int32), dag=rd("dag.bin", np.int32).reshape(-1, 2),
+            bfs_src=bsrc, bfs=rd("bfs.bin", np.int32).reshape(len(bsrc), V),
+            sssp_src=ssrc, sssp=rd("sssp.bin", np.uint64).reshape(len(ssrc), V),
+            pairs=pairs, weight=rd("weight.bin", np.uint32), ops=ops,
+            post_wcc=rd("post_wcc.bin", np.int32), post_scc=rd("post_scc.bin", np.int32),
+            post_dag=rd("post_dag.bin", np.int32).reshape(-1, 2))
+        print(name, "V", V, "E", len(fa))
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit(f"{REF} missing: run `make -C oracle ref` first")
+    # the reference's own 18-vertex fixture (data/example_graph)
+    fb, fa, bb, ba = girhip.read_csr(EXAMPLE)
+    run_case("example_graph", len(fb) - 1, fb, fa, bb, ba, 18, 6, 40, 101)
+    V, fb, fa, bb, ba = girhip.rmat_graph(10, seed=5)
+    run_case("rmat10", V, fb, fa, bb, ba, 8, 4, 300, 102)
+    V, fb, fa, bb, ba = girhip.rmat_graph(12, edge_factor=8, a=0.45, b=0.22, c=0.22, seed=6)
+    run_case("rmat12", V, fb, fa, bb, ba, 8, 4, 2000, 103)
+    V, fb, fa, bb, ba = girhip.grid_graph(40, 40, seed=7)
+    run_case("grid40", V, fb, fa, bb, ba, 6, 3, 300, 104)
+    s, d = girhip.uniform_edges(3000, 6000, seed=8)
+    fb, fa, bb, ba = girhip.csr_from_edges(3000, s, d)
+    run_case("er3000", 3000, fb, fa, bb, ba, 6, 3, 500, 105)
+
+
+if __name__ == "__main__":
+    main()

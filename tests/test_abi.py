@@ -1,0 +1,117 @@
+"""CPU tests of the C ABI boundary and the host input formats (no GPU
+compute: the HIP library is loaded and inspected, and must refuse to run
+without a device)."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import girhip
+from conftest import ROOT
+
+HEADERS = {"girhip.h": girhip.LIB_PATH, "girgen.h": girhip.GEN_PATH}
+
+
+def declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:girhip|girgen)_[a-z0-9_]+)\s*\(", txt)))
+
+
+@pytest.mark.parametrize("header", sorted(HEADERS))
+def test_library_exports_every_declared_symbol(header):
+    names = declared(header)
+    assert names, header
+    out = subprocess.run(["nm", "-D", "--defined-only", HEADERS[header]], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (\w+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    lib = ctypes.CDLL(HEADERS[header])
+    for n in names:
+        assert getattr(lib, n)
+
+
+def test_binding_covers_header():
+    assert set(declared("girhip.h")) == set(girhip.exported_symbols())
+
+
+def test_product_library_does_not_link_the_oracle():
+    out = subprocess.run(["ldd", girhip.LIB_PATH], capture_output=True, text=True).stdout
+    assert "oracle" not in out
+    out = subprocess.run(["nm", "-D", girhip.LIB_PATH], capture_output=True, text=True).stdout
+    assert "orc_" not in out
+
+
+def test_no_device_fails_loudly():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(girhip.GirError) as e:
+        girhip.Graph(np.array([0, 1, 1], np.int32), np.array([1], np.int32))
+    assert e.value.code == 5  # GIRHIP_ENODEV
+
+
+def test_csr_from_edges_semantics():
+    # self loops and duplicates dropped, rows sorted, bw = transpose (el_to_csr.cpp:96-116)
+    src = np.array([2, 0, 0, 1, 0, 2, 2, 1], np.int32)
+    dst = np.array([0, 2, 1, 1, 2, 1, 0, 0], np.int32)
+    fb, fa, bb, ba = girhip.csr_from_edges(3, src, dst)
+    assert fb.tolist() == [0, 2, 3, 5] and fa.tolist() == [1, 2, 0, 0, 1]
+    assert bb.tolist() == [0, 2, 4, 5] and ba.tolist() == [1, 2, 0, 2, 0]
+
+
+def test_csr_matches_numpy_on_random_edges():
+    V = 500
+    s, d = girhip.uniform_edges(V, 4000, seed=9)
+    fb, fa, bb, ba = girhip.csr_from_edges(V, s, d)
+    keep = s != d
+    pairs = np.unique(np.stack([s[keep], d[keep]], 1), axis=0)
+    assert len(fa) == len(pairs)
+    assert (np.repeat(np.arange(V), np.diff(fb)) == pairs[:, 0]).all() and (fa == pairs[:, 1]).all()
+    t = pairs[np.lexsort((pairs[:, 0], pairs[:, 1]))]
+    assert (np.repeat(np.arange(V), np.diff(bb)) == t[:, 1]).all() and (ba == t[:, 0]).all()
+
+
+def test_generators_are_deterministic():
+    a = girhip.rmat_edges(12, 10000, seed=4)
+    b = girhip.rmat_edges(12, 10000, seed=4)
+    c = girhip.rmat_edges(12, 10000, seed=5)
+    assert all((x == y).all() for x, y in zip(a, b))
+    assert not (a[0] == c[0]).all()
+    assert (a[0] >= 0).all() and (a[0] < 4096).all()
+    # quadrant a = .57 dominates: the top bit of u and v is mostly 0
+    assert ((a[0] >> 11) & 1).mean() < 0.35
+
+
+def test_grid_generator_counts():
+    s, d = girhip.grid_edges(50, 60, p_keep=1.0, p_bidir=1.0, seed=1)
+    assert len(s) == 2 * (50 * 59 + 49 * 60)
+    s, d = girhip.grid_edges(50, 60, p_keep=1.0, p_bidir=0.0, seed=1)
+    assert len(s) == 50 * 59 + 49 * 60
+    assert (np.abs(s - d) % 60 == 0).sum() + (np.abs(s - d) == 1).sum() == len(s)
+
+
+def test_weights_and_sources():
+    w = girhip.weights(100000, 1, 64, 3)
+    assert w.min() == 1 and w.max() == 64
+    V, fb, fa, bb, ba = girhip.rmat_graph(10, seed=3)
+    src = girhip.sources(fb, 64, 7)
+    assert (np.diff(fb)[src] > 0).all()
+
+
+def test_bin_csr_roundtrip_and_validation():
+    V, fb, fa, bb, ba = girhip.rmat_graph(8, seed=2)
+    with tempfile.TemporaryDirectory() as td:
+        girhip.write_csr(td, fb, fa, bb, ba)
+        r = girhip.read_csr(td)
+        assert all((x == y).all() for x, y in zip(r, (fb, fa, bb, ba)))
+        bad = fa.copy()
+        bad[3] = V + 5  # id out of range: refused (graph.cpp:95-114)
+        girhip.write_csr(td, fb, bad, bb, ba)
+        with pytest.raises(ValueError):
+            girhip.read_csr(td)

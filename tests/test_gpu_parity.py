@@ -1,0 +1,266 @@
+"""GPU parity tests: the gfx950 library through its C ABI against the oracle
+and the reference's golden vectors.  Integer results must be bit-exact
+(canonical component ids, BFS levels, SSSP distances, DAG arcs with
+multiplicities)."""
+import numpy as np
+import pytest
+
+import girhip
+import oracle
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+INF = np.iinfo(np.uint64).max
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def check_layers(G, fb, fa, bb, ba, tag="", rebuild=True):
+    """WCC, SCC, DAG (+mult), ranks of G against the oracle on (fb, fa).
+    rebuild=False reads the maintained layers (after updates) instead of
+    building them again."""
+    lab, n = G.wcc() if rebuild else G.get_wcc()
+    olab, on = oracle.wcc(fb, fa)
+    assert n == on and (lab == olab).all(), f"wcc {tag}"
+    s, ns, nd = G.scc_dag() if rebuild else G.get_scc()
+    os_, ons = oracle.scc(fb, fa)
+    assert ns == ons and (s == os_).all(), f"scc {tag}"
+    src, dst, mult = G.dag()
+    osrc, odst, omult = oracle.dag(fb, fa, os_)
+    assert nd == len(osrc)
+    assert (src == osrc).all() and (dst == odst).all() and (mult == omult).all(), f"dag {tag}"
+    r = G.ranks()
+    reps = s == np.arange(len(s))
+    assert ((r >= 0) == reps).all(), f"rank reps {tag}"
+    assert (r[src] < r[dst]).all(), f"rank order {tag}"
+    assert len(np.unique(r[reps])) == reps.sum()
+    return s
+
+
+def test_golden_cases(golden):
+    name, g = golden
+    fb, fa, bb, ba = g["fw_beg"], g["fw_adj"], g["bw_beg"], g["bw_adj"]
+    say("golden", name, "V", len(fb) - 1, "E", len(fa))
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        lab, _ = G.wcc()
+        assert (lab == g["wcc"]).all()
+        s, _, _ = G.scc_dag()
+        assert (s == g["scc"]).all()
+        src, dst, _ = G.dag()
+        assert (np.stack([src, dst], 1) == g["dag"]).all()
+        say("golden bfs", name)
+        assert (G.bfs(g["bfs_src"]) == g["bfs"]).all()
+        say("golden sssp", name)
+        assert (G.sssp(g["sssp_src"], 16) == g["sssp"]).all()
+        say("golden batch", name)
+        ops = g["ops"]
+        G.apply_batch(ops[:, 0], ops[:, 1], ops[:, 2])
+        assert (G.get_wcc()[0] == g["post_wcc"]).all()
+        s, _, _ = G.get_scc()
+        assert (s == g["post_scc"]).all()
+        src, dst, _ = G.dag()
+        assert (np.stack([src, dst], 1) == g["post_dag"]).all()
+
+
+def test_example_graph_targeted_split():
+    # gir_test.cpp:704-753: insert 3->12 merges the WCCs, deleting it splits
+    # them again, deleting 9->2 splits SCC {0,1,2,9,10,11}.
+    g = load_golden("example_graph")
+    say("example targeted")
+    with girhip.Graph(g["fw_beg"], g["fw_adj"]) as G:
+        G.wcc()
+        G.scc_dag()
+        G.update_edge_directed(3, 12, True)
+        assert G.get_wcc()[1] == 1
+        G.update_edge_directed(3, 12, False)
+        lab, n = G.get_wcc()
+        assert n == 2 and lab.tolist() == [0] * 12 + [12] * 6
+        G.update_edge_directed(9, 2, False)
+        s, ns, _ = G.get_scc()
+        assert s[[0, 1, 2]].tolist() == [0, 0, 0] and s[[9, 10, 11]].tolist() == [9, 9, 9]
+
+
+@pytest.mark.parametrize("scale,ef,seed", [(10, 16, 31), (12, 8, 32), (16, 16, 1)])
+def test_rmat_parity(scale, ef, seed):
+    V, fb, fa, bb, ba = girhip.rmat_graph(scale, edge_factor=ef, seed=seed)
+    say("rmat", scale, ef, seed, "E", len(fa))
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        check_layers(G, fb, fa, bb, ba, f"rmat{scale}")
+        src = girhip.sources(fb, 64 if scale == 16 else 16, seed + 100)
+        say("rmat bfs", scale)
+        lv, m = G.bfs(src, edges=True)
+        for i, x in enumerate(src):
+            want = oracle.bfs(fb, fa, bb, ba, x)
+            assert (lv[i] == want).all(), f"bfs src {x}"
+            assert m[i] == int(np.diff(fb)[want >= 0].sum())
+        w = girhip.weights(len(fa), 1, 64, seed + 200)
+        ss = src[:8]
+        say("rmat sssp hash", scale)
+        d = G.sssp(ss, 16)
+        for i, x in enumerate(ss):
+            assert (d[i] == oracle.sssp(fb, fa, None, x, 16)).all(), f"sssp hash {x}"
+        say("rmat sssp weights", scale)
+        G.set_weights(w)
+        d, m = G.sssp(ss, 16, edges=True)
+        for i, x in enumerate(ss):
+            want = oracle.dijkstra(fb, fa, w, x)
+            assert (d[i] == want).all(), f"sssp w {x}"
+            assert m[i] == int(np.diff(fb)[want != INF].sum())
+
+
+def test_grid_parity():
+    V, fb, fa, bb, ba = girhip.grid_graph(120, 150, seed=33)
+    say("grid 120x150 E", len(fa))
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        check_layers(G, fb, fa, bb, ba, "grid")
+        src = girhip.sources(fb, 4, 5)
+        lv = G.bfs(src)
+        for i, x in enumerate(src):
+            assert (lv[i] == oracle.bfs_textbook(fb, fa, x)).all()
+        w = girhip.weights(len(fa), 1, 1000, 6)
+        G.set_weights(w)
+        d = G.sssp(src, 0)
+        for i, x in enumerate(src):
+            assert (d[i] == oracle.dijkstra(fb, fa, w, x)).all()
+
+
+def test_tuning_knobs_do_not_change_results():
+    V, fb, fa, bb, ba = girhip.rmat_graph(12, seed=34)
+    say("knobs rmat12")
+    src = girhip.sources(fb, 4, 1)
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        base = (G.wcc()[0], G.scc_dag()[0], G.bfs(src), G.sssp(src, 16))
+        for k, v in [("block", 64), ("block", 1024), ("bfs_alpha", 1), ("bfs_beta", 1000), ("bfs_min_reach", 0),
+                     ("wcc_rounds", 0), ("scc_trim_rounds", 0), ("scc_pivot", 0), ("sssp_delta", 1),
+                     ("sssp_delta", 100000)]:
+            say("knob", k, v)
+            old = G.get_param(k)
+            G.set_param(k, v)
+            got = (G.wcc()[0], G.scc_dag()[0], G.bfs(src), G.sssp(src, 0 if k == "sssp_delta" else 16))
+            for a, b in zip(base, got):
+                assert (a == b).all(), (k, v)
+            G.set_param(k, old)
+
+
+def test_unsorted_duplicate_input_is_normalised():
+    V, fb, fa, bb, ba = girhip.rmat_graph(9, seed=35)
+    rng = np.random.default_rng(0)
+    # shuffle every row and duplicate some arcs (GIR normalises: gir.cpp:95-130)
+    rows = np.repeat(np.arange(V), np.diff(fb))
+    dup = rng.random(len(fa)) < 0.2
+    r2 = np.concatenate([rows, rows[dup]])
+    a2 = np.concatenate([fa, fa[dup]])
+    perm = rng.permutation(len(r2))
+    r2, a2 = r2[perm], a2[perm]
+    order = np.argsort(r2, kind="stable")
+    nfb = np.zeros(V + 1, np.int32)
+    np.add.at(nfb, r2 + 1, 1)
+    nfb = np.cumsum(nfb).astype(np.int32)
+    nfa = a2[order].astype(np.int32)
+    say("normalise rmat9")
+    with girhip.Graph(nfb, nfa) as G:
+        assert G.E == len(fa)
+        gfb, gfa, gbb, gba = G.csr()
+        assert (gfb == fb).all() and (gfa == fa).all() and (gbb == bb).all() and (gba == ba).all()
+
+
+def test_bad_inputs_are_refused():
+    say("bad inputs")
+    with pytest.raises(girhip.GirError):
+        girhip.Graph(np.array([0, 2, 1], np.int32), np.array([1, 0], np.int32))  # offsets not monotone
+    with pytest.raises(girhip.GirError):
+        girhip.Graph(np.array([0, 1, 2], np.int32), np.array([1, 7], np.int32))  # id out of range
+    with pytest.raises(girhip.GirError):  # bw not the transpose
+        girhip.Graph(np.array([0, 1, 1], np.int32), np.array([1], np.int32), np.array([0, 1, 1], np.int32),
+                     np.array([1], np.int32))
+
+
+def test_edge_cases():
+    say("edge cases")
+    # no arcs
+    with girhip.Graph(np.zeros(6, np.int32), np.zeros(0, np.int32)) as G:
+        assert G.wcc()[1] == 5 and G.scc_dag()[1] == 5 and len(G.dag()[0]) == 0
+        assert (G.bfs([2]) == np.array([[-1, -1, 0, -1, -1]])).all()
+        assert G.sssp([4])[0].tolist() == [INF] * 4 + [0]
+    # self loops only (kept by normalisation, no effect on components)
+    with girhip.Graph(np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32)) as G:
+        assert G.wcc()[1] == 2 and G.scc_dag()[1] == 2 and len(G.dag()[0]) == 0
+    # out-of-range sources -> unreachable rows (gir_query.cpp:112, 699)
+    V, fb, fa, bb, ba = girhip.rmat_graph(6, seed=1)
+    with girhip.Graph(fb, fa) as G:
+        lv = G.bfs([-1, V, 0])
+        assert (lv[0] == -1).all() and (lv[1] == -1).all() and lv[2][0] == 0
+        d = G.sssp([V + 10])
+        assert (d == INF).all()
+        # invalid ops are dropped and counted
+        st = G.apply_batch([1, 2, -1, V], [1, 3, 0, 0], [1, 1, 1, 1])
+        assert st["invalid"] == 3
+
+
+def _random_batch(rng, fb, fa, n, V):
+    rows = np.repeat(np.arange(V, dtype=np.int32), np.diff(fb))
+    k = n // 2
+    e = rng.integers(0, len(fa), k)
+    u = np.concatenate([rows[e], rng.integers(0, V, n - k)]).astype(np.int32)
+    v = np.concatenate([fa[e], rng.integers(0, V, n - k)]).astype(np.int32)
+    ins = np.concatenate([np.zeros(k, np.uint8), np.ones(n - k, np.uint8)])
+    p = rng.permutation(n)
+    return u[p], v[p], ins[p]
+
+
+def test_batch_sequence_matches_recompute():
+    V, fb, fa, bb, ba = girhip.rmat_graph(12, seed=36)
+    rng = np.random.default_rng(7)
+    w = girhip.weights(len(fa), 1, 64, 8)
+    src = girhip.sources(fb, 3, 9)
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.set_weights(w)
+        G.wcc()
+        G.scc_dag()
+        wmap = {(int(a), int(b)): int(c) for a, b, c in zip(np.repeat(np.arange(V), np.diff(fb)), fa, w)}
+        for step, n in enumerate([1, 10, 1000, 5000, 300]):
+            u, v, ins = _random_batch(rng, fb, fa, n, V)
+            if step == 2:
+                ins[:] = 1  # insert-only batch (incremental WCC hooking)
+            say("batch step", step, "n", n)
+            (iu, iv), (du, dv), nn = oracle.normalize_batch(fb, fa, u, v, ins)
+            st = G.apply_batch(u, v, ins)
+            assert st["inserted"] == len(iu) and st["deleted"] == len(du) and st["noop"] == nn
+            fb, fa, bb, ba = oracle.apply_batch(fb, fa, u, v, ins)
+            gfb, gfa, gbb, gba = G.csr()
+            assert (gfb == fb).all() and (gfa == fa).all() and (gbb == bb).all() and (gba == ba).all()
+            check_layers(G, fb, fa, bb, ba, f"step{step}", rebuild=False)
+            # weights follow their arcs; inserted arcs get GIR::edgeWeight
+            for a, b in zip(iu.tolist(), iv.tolist()):
+                wmap[(a, b)] = oracle.edge_weight(a, b)
+            wv = np.array([wmap[(int(a), int(b))] for a, b in zip(np.repeat(np.arange(V), np.diff(fb)), fa)],
+                          np.uint32)
+            d = G.sssp(src, 16)
+            for i, x in enumerate(src):
+                assert (d[i] == oracle.dijkstra(fb, fa, wv, x)).all()
+            lv = G.bfs(src)
+            for i, x in enumerate(src):
+                assert (lv[i] == oracle.bfs_textbook(fb, fa, x)).all()
+
+
+def test_ingest_then_sync():
+    V, fb, fa, bb, ba = girhip.rmat_graph(11, seed=37)
+    rng = np.random.default_rng(11)
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        lab0, _ = G.wcc()
+        s0, _, _ = G.scc_dag()
+        u, v, ins = _random_batch(rng, fb, fa, 2000, V)
+        say("ingest")
+        G.ingest_batch(u, v, ins)
+        assert not G.synced()
+        nfb, nfa, nbb, nba = oracle.apply_batch(fb, fa, u, v, ins)
+        # the store sees the change at once (BFS reads the store)
+        assert (G.bfs([int(girhip.sources(nfb, 1, 2)[0])])[0] ==
+                oracle.bfs_textbook(nfb, nfa, int(girhip.sources(nfb, 1, 2)[0]))).all()
+        assert (G.get_wcc()[0] == lab0).all()  # layers lag until sync (gir.h:107-116)
+        assert (G.get_scc()[0] == s0).all()
+        G.sync()
+        assert G.synced()
+        check_layers(G, nfb, nfa, nbb, nba, "sync", rebuild=False)

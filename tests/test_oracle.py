@@ -1,0 +1,152 @@
+"""CPU tests that pin the oracle (the CPU restatement of the reference path).
+
+Each oracle function is checked against (a) the golden vectors written by the
+compiled reference (tests/golden/make_golden.py), (b) hand-derived answers for
+the reference's 18-vertex fixture, and (c) an independent textbook algorithm
+on seeded graphs (Kosaraju vs Tarjan, BFS-labelling vs union-find, queue BFS
+vs direction-optimising BFS, Dijkstra vs delta-stepping).
+"""
+import numpy as np
+import pytest
+
+import girhip
+import oracle
+from conftest import load_golden
+
+# ---- hand-derived answers for data/example_graph (brute force on 18 vertices) ----
+EX_WCC = [0] * 12 + [12] * 6
+EX_SCC = [0, 0, 0, 3, 3, 5, 6, 6, 6, 0, 0, 0, 12, 12, 12, 12, 16, 17]
+EX_DAG = [(0, 3, 3), (0, 5, 1), (3, 5, 1), (3, 6, 1), (5, 6, 1), (12, 16, 1), (16, 17, 1)]
+EX_BFS = {
+    9: [2, 3, 1, 2, 3, 1, 2, 3, 4, 0, 1, 2, -1, -1, -1, -1, -1, -1],
+    0: [0, 1, 2, 1, 2, 2, 3, 2, 3, 3, 4, 5, -1, -1, -1, -1, -1, -1],
+    12: [-1] * 12 + [0, 1, 2, 3, 2, 3],
+}
+
+
+def test_example_graph_hand_answers():
+    g = load_golden("example_graph")
+    fb, fa, bb, ba = g["fw_beg"], g["fw_adj"], g["bw_beg"], g["bw_adj"]
+    lab, n = oracle.wcc(fb, fa)
+    assert lab.tolist() == EX_WCC and n == 2
+    s, ns = oracle.scc(fb, fa)
+    assert s.tolist() == EX_SCC and ns == 7
+    src, dst, mult = oracle.dag(fb, fa, s)
+    assert list(zip(src.tolist(), dst.tolist(), mult.tolist())) == EX_DAG
+    # 26 arcs - 17 intra-SCC arcs = 9 condensation arcs counted with multiplicity
+    assert mult.sum() == 9
+    for x, want in EX_BFS.items():
+        assert oracle.bfs(fb, fa, bb, ba, x).tolist() == want
+
+
+def test_oracle_matches_reference_golden(golden):
+    name, g = golden
+    fb, fa, bb, ba = g["fw_beg"], g["fw_adj"], g["bw_beg"], g["bw_adj"]
+    assert (oracle.wcc(fb, fa)[0] == g["wcc"]).all()
+    s, _ = oracle.scc(fb, fa)
+    assert (s == g["scc"]).all()
+    src, dst, mult = oracle.dag(fb, fa, s)
+    assert (np.stack([src, dst], 1) == g["dag"]).all()
+    for i, x in enumerate(g["bfs_src"]):
+        assert (oracle.bfs(fb, fa, bb, ba, x) == g["bfs"][i]).all(), f"{name} bfs {x}"
+    for i, x in enumerate(g["sssp_src"]):
+        assert (oracle.sssp(fb, fa, None, x, 16) == g["sssp"][i]).all(), f"{name} sssp {x}"
+    w = [oracle.edge_weight(u, v) for u, v in g["pairs"]]
+    assert w == g["weight"].tolist()
+
+
+def test_oracle_batch_matches_reference_golden(golden):
+    name, g = golden
+    fb, fa = g["fw_beg"], g["fw_adj"]
+    ops = g["ops"]
+    nfb, nfa, nbb, nba = oracle.apply_batch(fb, fa, ops[:, 0], ops[:, 1], ops[:, 2])
+    assert (oracle.wcc(nfb, nfa)[0] == g["post_wcc"]).all(), name
+    s, _ = oracle.scc(nfb, nfa)
+    assert (s == g["post_scc"]).all(), name
+    src, dst, _ = oracle.dag(nfb, nfa, s)
+    assert (np.stack([src, dst], 1) == g["post_dag"]).all(), name
+
+
+def _graphs():
+    yield "rmat", girhip.rmat_graph(11, edge_factor=8, seed=21)
+    yield "grid", girhip.grid_graph(30, 30, seed=22)
+    V = 2000
+    s, d = girhip.uniform_edges(V, 3000, seed=23)
+    yield "er", (V,) + girhip.csr_from_edges(V, s, d)
+
+
+@pytest.mark.parametrize("name,G", list(_graphs()))
+def test_oracle_vs_textbook(name, G):
+    V, fb, fa, bb, ba = G
+    assert (oracle.wcc(fb, fa)[0] == oracle.wcc_bfs(fb, fa, bb, ba)).all()
+    assert (oracle.scc(fb, fa)[0] == oracle.scc_kosaraju(fb, fa, bb, ba)).all()
+    src = girhip.sources(fb, 6, 5)
+    w = girhip.weights(len(fa), 1, 64, 9)
+    for x in src:
+        assert (oracle.bfs(fb, fa, bb, ba, x) == oracle.bfs_textbook(fb, fa, x)).all()
+    for x in src[:3]:
+        assert (oracle.sssp(fb, fa, w, x, 16) == oracle.dijkstra(fb, fa, w, x)).all()
+        assert (oracle.sssp(fb, fa, None, x, 7) == oracle.dijkstra(fb, fa, None, x)).all()
+
+
+@pytest.mark.parametrize("name,G", list(_graphs()))
+def test_oracle_ranks_and_dag_invariants(name, G):
+    V, fb, fa, bb, ba = G
+    s, ns = oracle.scc(fb, fa)
+    src, dst, mult = oracle.dag(fb, fa, s)
+    r = oracle.scc_ranks(fb, fa)
+    assert (r[src] < r[dst]).all()
+    assert ((r >= 0) == (s == np.arange(V))).all()
+    # multiplicities add up to the crossing arcs
+    rows = np.repeat(np.arange(V), np.diff(fb))
+    assert mult.sum() == int((s[rows] != s[fa]).sum())
+    assert (src != dst).all()
+
+
+def test_normalize_batch_last_op_wins():
+    rng = np.random.default_rng(3)
+    V = 12
+    s, d = girhip.uniform_edges(V, 30, seed=2)
+    fb, fa, bb, ba = girhip.csr_from_edges(V, s, d)
+    present = {(u, int(v)) for u in range(V) for v in fa[fb[u]:fb[u + 1]]}
+    for trial in range(50):
+        n = int(rng.integers(0, 40))
+        u = rng.integers(-1, V + 1, n).astype(np.int32)
+        v = rng.integers(-1, V + 1, n).astype(np.int32)
+        ins = rng.integers(0, 2, n).astype(np.uint8)
+        last = {}
+        for a, b, c in zip(u.tolist(), v.tolist(), ins.tolist()):
+            if a == b or min(a, b) < 0 or max(a, b) >= V:
+                continue
+            last[(a, b)] = c
+        want_ins = sorted(k for k, c in last.items() if c and k not in present)
+        want_del = sorted(k for k, c in last.items() if not c and k in present)
+        want_noop = sum(1 for k, c in last.items() if c and k in present)
+        (iu, iv), (du, dv), nn = oracle.normalize_batch(fb, fa, u, v, ins)
+        assert list(zip(iu.tolist(), iv.tolist())) == want_ins
+        assert list(zip(du.tolist(), dv.tolist())) == want_del
+        assert nn == want_noop
+        nfb, nfa, nbb, nba = oracle.apply_batch(fb, fa, u, v, ins)
+        after = (present | set(want_ins)) - set(want_del)
+        got = {(x, int(y)) for x in range(V) for y in nfa[nfb[x]:nfb[x + 1]]}
+        assert got == after
+        gotb = {(int(y), x) for x in range(V) for y in nba[nbb[x]:nbb[x + 1]]}
+        assert gotb == after
+
+
+def test_normalize_csr_sorts_and_dedups():
+    beg = np.array([0, 4, 4, 7], np.int32)
+    adj = np.array([2, 1, 2, 0, 1, 1, 0], np.int32)
+    b, a = oracle.normalize_csr(beg, adj)
+    assert b.tolist() == [0, 3, 3, 5] and a.tolist() == [0, 1, 2, 0, 1]
+
+
+def test_planted_mistake_is_caught():
+    # A wrong component label planted in the oracle's output must fail the
+    # comparison the GPU tests use (canonical labels compared element-wise).
+    g = load_golden("rmat10")
+    lab, _ = oracle.scc(g["fw_beg"], g["fw_adj"])
+    bad = lab.copy()
+    i = int(np.nonzero(bad != np.arange(len(bad)))[0][0])
+    bad[i] = i
+    assert not (bad == g["scc"]).all()
