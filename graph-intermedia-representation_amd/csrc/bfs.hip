@@ -57,6 +57,141 @@ int64_t outDegree(girhip_graph* h, int32_t v) {
     return (int64_t)p[1] - p[0];
 }
 
+// ---------------------------------------------------------------------------
+// Bit-parallel multi-source BFS (MS-BFS, Then et al., VLDB 2015): up to 64
+// sources share one traversal.  Bit s of a vertex's seen / frontier / next
+// word belongs to source s, so an arc is read once per level for all sources
+// whose frontier holds its tail.  Push levels OR the tail's frontier word into
+// the head's next word (atomicOr, skipped when nothing new); pull levels let
+// every vertex that some source has not reached yet OR the frontier words of
+// its in-neighbours, stopping once every source has reached it.  The same
+// alpha/beta rule as the single-source engine picks the direction on the
+// union frontier.  Levels are written once per (source, vertex): at the
+// level a bit first appears, or -1 in a final pass over the unseen bits.
+
+__global__ void kMsSeed(const int32_t* __restrict__ src, int k, int32_t V, uint64_t* seen, uint64_t* front,
+                        int32_t* __restrict__ levels, int64_t* cnt) {
+    const int s = threadIdx.x;
+    if (s >= k) return;
+    const int32_t v = src[s];
+    if (v < 0 || v >= V) return;
+    atomicOr((unsigned long long*)&seen[v], 1ULL << s);
+    atomicOr((unsigned long long*)&front[v], 1ULL << s);
+    levels[(size_t)s * V + v] = 0;
+}
+
+__global__ void kMsPush(const int32_t* __restrict__ fb, const int32_t* __restrict__ fa,
+                        const uint64_t* __restrict__ front, const uint64_t* seen, uint64_t* next, int32_t V,
+                        int64_t* cz) {
+    if (blockIdx.x == 0 && threadIdx.x < 3) cz[threadIdx.x] = 0;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t f = v < V ? front[v] : 0;
+    const bool act = f != 0;
+    const int32_t b = act ? fb[v] : 0, e = act ? fb[v + 1] : 0;
+    waveRowsP(act, (int32_t)v, f, b, e, [&](int32_t, uint64_t fp, int64_t j) {
+        const int32_t w = fa[j];
+        const uint64_t m = fp & ~(seen[w] | ldAcq(&next[w]));
+        if (m) atomicOr((unsigned long long*)&next[w], (unsigned long long)m);
+    });
+}
+
+__global__ void kMsPull(const int32_t* __restrict__ bb, const int32_t* __restrict__ ba,
+                        const uint64_t* __restrict__ front, const uint64_t* __restrict__ seen,
+                        uint64_t* __restrict__ next, int32_t V, uint64_t all, int64_t* cz) {
+    if (blockIdx.x == 0 && threadIdx.x < 3) cz[threadIdx.x] = 0;
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= V) return;
+    const uint64_t sw = seen[w];
+    uint64_t acc = 0;
+    if (sw != all) {
+        const int32_t b = bb[w], e = bb[w + 1];
+        for (int32_t j = b; j < e; ++j) {
+            acc |= front[ba[j]];
+            if ((acc | sw) == all) break;
+        }
+    }
+    next[w] = acc & ~sw;
+}
+
+// New bits -> seen, frontier, level writes; cnt[0] = union frontier size,
+// cnt[1] = its out-arc count.
+__global__ void kMsFinish(uint64_t* seen, uint64_t* front, uint64_t* next, int32_t* __restrict__ levels, int32_t V,
+                          int32_t L, const int32_t* __restrict__ fb, int64_t* cnt) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t nb = 0;
+    if (w < V) {
+        const uint64_t sw = seen[w];
+        nb = next[w] & ~sw;
+        seen[w] = sw | nb;
+        front[w] = nb;
+        next[w] = 0;
+    }
+    if (__ballot(nb != 0)) {
+        for (int s = 0; s < 64; ++s)
+            if ((nb >> s) & 1) levels[(size_t)s * V + w] = L + 1;
+    }
+    int64_t c = nb != 0, m = nb != 0 ? (int64_t)(fb[w + 1] - fb[w]) : 0;
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        c += __shfl_xor(c, d);
+        m += __shfl_xor(m, d);
+    }
+    if (laneId() == 0 && c) {
+        atomicAdd((unsigned long long*)&cnt[0], (unsigned long long)c);
+        atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)m);
+    }
+}
+
+__global__ void kMsUnreached(const uint64_t* __restrict__ seen, int32_t* __restrict__ levels, int32_t V,
+                             uint64_t all) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t un = w < V ? (~seen[w] & all) : 0;
+    if (__ballot(un != 0)) {
+        for (int s = 0; s < 64; ++s)
+            if ((un >> s) & 1) levels[(size_t)s * V + w] = -1;
+    }
+}
+
+// Up to 64 sources (device array src, k entries) into levels rows 0..k-1.
+void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels, int64_t m0) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    for (int i = 0; i < 3; ++i) S.ms[i].ensure((size_t)(V > 0 ? V : 1));
+    uint64_t *seen = S.ms[0].p, *front = S.ms[1].p, *next = S.ms[2].p;
+    const uint64_t all = k >= 64 ? ~0ULL : ((1ULL << k) - 1);
+    for (int i = 0; i < 3; ++i) fillU64(h, S.ms[i].p, 0, V);
+    int64_t* c = h->dcnt.p + 16;  // counter pairs for alternate levels (3 slots each)
+    fillI64(h, c, 0, 6);
+    launch(h, "msbfs_seed", kMsSeed, 1, 64, 0, dsrc, k, V, seen, front, levels, c);
+    const int64_t alpha = (int64_t)h->prm.bfs_alpha, beta = (int64_t)h->prm.bfs_beta;
+    const int64_t min_reach = (int64_t)h->prm.bfs_min_reach;
+    int64_t nf = k, mf = m0, m_unexplored = h->E - m0;
+    bool pull = false;
+    const unsigned g = gridFor(V, blk);
+    for (int32_t L = 0; nf > 0; ++L) {
+        if ((int64_t)V >= min_reach) {
+            if (!pull && mf * alpha > m_unexplored) pull = true;
+            else if (pull && nf * beta < (int64_t)V) pull = false;
+        }
+        int64_t* cn = c + 3 * (L & 1);
+        int64_t* cz = c + 3 * ((L + 1) & 1);
+        if (pull)
+            launch(h, "msbfs_pull", kMsPull, g, blk, 0, (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p,
+                   (const uint64_t*)front, (const uint64_t*)seen, next, V, all, cz);
+        else
+            launch(h, "msbfs_push", kMsPush, g, blk, 0, (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p,
+                   (const uint64_t*)front, (const uint64_t*)seen, next, V, cz);
+        launch(h, "msbfs_finish", kMsFinish, g, blk, 0, seen, front, next, levels, V, L, (const int32_t*)h->fw_beg.p,
+               cn);
+        int64_t r[2];
+        readCounters(h, cn, 2, r);
+        nf = r[0];
+        mf = r[1];
+        m_unexplored -= mf;
+    }
+    launch(h, "msbfs_unreached", kMsUnreached, g, blk, 0, (const uint64_t*)seen, levels, V, all);
+}
+
 }  // namespace gir
 
 using namespace gir;
@@ -76,16 +211,30 @@ extern "C" int girhip_bfs(girhip_graph* h, const int32_t* src, int32_t k, int32_
             out = (int32_t*)S.stage.p;
         }
         ReachBufs B = reachBufs(h);
+        const bool bitpar = k > 1 && h->prm.msbfs != 0;
+        DevBuf<int32_t> dsrc;
+        if (bitpar) {
+            dsrc.ensure((size_t)k);
+            copyIn(h, dsrc.p, s.data(), sizeof(int32_t) * (size_t)k);
+            syncStream(h);
+        }
         tBegin(h);
-        fillI32(h, out, -1, (int64_t)n);
-        for (int32_t i = 0; i < k; ++i) {
-            const int32_t x = s[i];
-            if (x < 0 || x >= V) continue;
-            fillU64(h, B.vis, 0, B.W);
-            fillI32(h, B.q[0], x, 1);
-            const int64_t m0 = outDegree(h, x);
-            runReach(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, B, 1, m0, (int64_t)V, h->E, AllArcs{},
-                     WriteLevel{out + (size_t)i * V});
+        if (bitpar) {
+            for (int32_t i0 = 0; i0 < k; i0 += 64) {
+                const int c = std::min<int32_t>(64, k - i0);
+                msbfs(h, dsrc.p + i0, c, out + (size_t)i0 * V, 0);
+            }
+        } else {
+            fillI32(h, out, -1, (int64_t)n);
+            for (int32_t i = 0; i < k; ++i) {
+                const int32_t x = s[i];
+                if (x < 0 || x >= V) continue;
+                fillU64(h, B.vis, 0, B.W);
+                fillI32(h, B.q[0], x, 1);
+                const int64_t m0 = outDegree(h, x);
+                runReach(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, B, 1, m0, (int64_t)V, h->E,
+                         AllArcs{}, WriteLevel{out + (size_t)i * V});
+            }
         }
         tEnd(h, "bfs");
         if (edges_out) {

@@ -91,6 +91,7 @@ struct Scratch {
     DevBuf<uint32_t> vals[2];
     DevBuf<int64_t> off, off2;
     DevBuf<uint8_t> stage;
+    DevBuf<uint64_t> ms[3];  // bit-parallel BFS: seen / frontier / next words
 };
 
 }  // namespace gir

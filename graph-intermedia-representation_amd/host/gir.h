@@ -1,0 +1,147 @@
+// Drop-in GIR front end over the gfx950 C ABI (include/girhip.h).
+//
+// Mirrors the hot-path part of the reference's public class GIR
+// (gir_constructor/gir.h:72-240): same method names, argument meaning and
+// error behaviour, so callers keep buildWCC() / buildSCC_DAG() /
+// getNodeToScc() / updateEdgeDirected() / applyBatch() / bfs() / sssp().
+// The graph, every layer and all compute live on the GPU; the host keeps
+// lazily refreshed copies of the labels for the per-vertex getters.
+//
+// Differences a caller can observe:
+//   - component ids are canonical (minimum vertex id of the component); the
+//     reference uses union-find roots / Tarjan emission indices.  sccOf(v)
+//     indexes getSCCs() and getDAG() exactly as in the reference (entries of
+//     ids that name no SCC are empty, like the reference's retired ids);
+//   - ranks are Kahn levels (level << 32 | id) instead of (S - s) << 32; the
+//     contract rank(a) < rank(b) for every condensation arc is the same;
+//   - the block-cut (BiCC) layer is not built (setBlocksOnDemand is accepted
+//     and ignored).
+#ifndef GIRHIP_GIR_H
+#define GIRHIP_GIR_H
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "graph.h"
+
+struct girhip_graph;
+
+// Query results, as in the reference (gir.h:17-37): vectors whose elements
+// are left default-initialised (they are filled right away).
+template <class T>
+struct DefaultInitAllocator : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = DefaultInitAllocator<U>;
+    };
+    DefaultInitAllocator() = default;
+    template <class U>
+    DefaultInitAllocator(const DefaultInitAllocator<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+        ::new ((void*)p) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new ((void*)p) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using QueryVec = std::vector<T, DefaultInitAllocator<T>>;
+
+class GIR {
+public:
+    struct Update {
+        int u, v;
+        bool insert;
+    };
+
+    // gir.h:93-96.  path_directed holds fw_begin.bin ... bw_adjacent.bin;
+    // path_undirected is accepted for signature compatibility (the
+    // reference's GIR never reads it either, gir.cpp:70-77).
+    GIR(const std::string& path_directed, const std::string& path_undirected);
+    explicit GIR(graph* g);  // takes ownership (gir.cpp:46, 65-68)
+    ~GIR();
+    GIR(const GIR&) = delete;
+    GIR& operator=(const GIR&) = delete;
+
+    // Construction (gir.h:98-101).
+    void buildWCC();
+    void buildSCC_DAG();
+    void buildBCT() {}  // the block layer is not part of this path
+
+    // Updates (gir.h:104-119).
+    void insertEdge(int u, int v);
+    void deleteEdge(int u, int v);
+    void applyBatch(const std::vector<Update>& batch);
+    void ingestBatch(const std::vector<Update>& batch);
+    void sync();
+    bool synced() const;
+    void updateEdgeDirected(int u, int v, bool insert);
+    void updateEdgeUndirected(int u, int v, bool insert) { updateEdgeDirected(u, v, insert); }
+
+    // Per-vertex queries (gir.h:121-136).
+    int sccOf(int v) const;
+    int wccOf(int v) const;
+    long long sccRank(int s) const;
+    bool sameScc(int u, int v);
+    bool sameWcc(int u, int v);
+    int vertexCount() const { return V; }
+
+    // Materialised views (gir.h:140-145).
+    std::map<int, int> getNodeToScc() const;
+    std::map<int, int> getNodeToWcc() const;
+    const std::vector<std::vector<int>>& getSCCs() const;
+    std::vector<std::vector<int>> getDAG() const;
+    std::vector<std::vector<int>> getWCCs() const;
+    // Condensation arcs with multiplicities (the reference keeps them in the
+    // private dag_out maps, gir.cpp:818-831).
+    std::vector<std::vector<std::pair<int, int>>> getDAGWithCounts() const;
+
+    // Edge store (gir.h:153-160).  getDirectedGraph() returns a host copy of
+    // the current CSR, refreshed on call.
+    graph* getDirectedGraph() const;
+    std::vector<int> getDirectedOutNeighbors(int v) const;
+    std::vector<int> getDirectedInNeighbors(int v) const;
+    bool hasDirectedEdge(int u, int v) const;
+
+    // Queries (gir.h:204-240).
+    QueryVec<int> bfs(int source);
+    static const uint32_t kMaxWeight = 100;
+    static uint32_t edgeWeight(int u, int v);
+    QueryVec<uint64_t> sssp(int source, uint64_t delta = 16);
+    // Many sources in one call (k x V, row i for sources[i]).
+    QueryVec<int> bfsMany(const std::vector<int>& sources);
+    QueryVec<uint64_t> ssspMany(const std::vector<int>& sources, uint64_t delta = 16);
+    // Explicit per-arc weights aligned to the forward CSR (nullptr restores
+    // edgeWeight); not in the reference, whose weights are always the hash.
+    void setWeights(const uint32_t* w_fw);
+
+    // Knobs (gir.h:170-240) that exist on this path.
+    void setQueryGranularity(size_t par_frontier, size_t large_unit);
+    void setBlocksOnDemand(bool) {}
+    bool blocksOnDemand() const { return true; }
+    void setParam(const char* name, double value);
+
+    girhip_graph* handle() const { return h; }
+
+private:
+    void init(const int32_t* fb, const int32_t* fa, const int32_t* bb, const int32_t* ba, int32_t V_, int64_t E_);
+    void refresh() const;  // pull labels after a change
+    void applyOps(const std::vector<Update>& batch, bool repair);
+
+    girhip_graph* h = nullptr;
+    int V = 0;
+    bool built_wcc = false, built_scc = false;
+    mutable bool stale = true;
+    mutable std::vector<int> wcc_label, node_scc;
+    mutable std::vector<long long> scc_rank;
+    mutable std::vector<std::vector<int>> sccs;
+    mutable bool sccs_ok = false;
+    mutable std::unique_ptr<graph> g_copy;
+};
+
+#endif

@@ -95,6 +95,10 @@ def test_rmat_parity(scale, ef, seed):
             want = oracle.bfs(fb, fa, bb, ba, x)
             assert (lv[i] == want).all(), f"bfs src {x}"
             assert m[i] == int(np.diff(fb)[want >= 0].sum())
+        say("rmat bfs single-source engine", scale)
+        G.set_param("msbfs", 0)
+        assert (G.bfs(src[:4]) == lv[:4]).all()
+        G.set_param("msbfs", 1)
         w = girhip.weights(len(fa), 1, 64, seed + 200)
         ss = src[:8]
         say("rmat sssp hash", scale)
@@ -132,7 +136,8 @@ def test_tuning_knobs_do_not_change_results():
     src = girhip.sources(fb, 4, 1)
     with girhip.Graph(fb, fa, bb, ba) as G:
         base = (G.wcc()[0], G.scc_dag()[0], G.bfs(src), G.sssp(src, 16))
-        for k, v in [("block", 64), ("block", 1024), ("bfs_alpha", 1), ("bfs_beta", 1000), ("bfs_min_reach", 0),
+        for k, v in [("msbfs", 0), ("block", 64), ("block", 1024), ("bfs_alpha", 1), ("bfs_beta", 1000),
+                     ("bfs_min_reach", 0),
                      ("wcc_rounds", 0), ("scc_trim_rounds", 0), ("scc_pivot", 0), ("sssp_delta", 1),
                      ("sssp_delta", 100000)]:
             say("knob", k, v)
