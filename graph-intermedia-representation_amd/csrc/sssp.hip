@@ -159,6 +159,241 @@ static void ssspOne(girhip_graph* h, SsspBufs& Bf, int32_t s, uint64_t delta, ui
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched form: up to SB sources advance together, one grid row (blockIdx.y)
+// per source, each with its own near queues (deduplicated at push time by a
+// per-source stamp), far pile (deduplicated by a per-source bitmap) and
+// bucket bounds, all in device memory.  One round relaxes every non-empty
+// near queue; then every source whose near queue ran dry picks its next
+// bucket from its far pile (a min pass and a split pass over the pile, grid
+// stride).  The host reads the per-source queue lengths once per round, to
+// size the next relax grid and to stop when every queue and pile is empty.
+
+constexpr int SB = 32;  // sources per batch
+
+// Per-source control block (int64 slots, SB each):
+enum : int {
+    C_NEAR0 = 0,          // near-queue length, parity 0
+    C_NEAR1 = SB,         // near-queue length, parity 1
+    C_FAR = 2 * SB,       // far-pile length
+    C_KEEP = 3 * SB,      // kept far entries of a split
+    C_FPAR = 4 * SB,      // which of the two far buffers holds the pile
+    C_ADV = 5 * SB,       // 1: this source advances its bucket this round
+    C_LO = 6 * SB,        // bucket [lo, hi) (uint64)
+    C_HI = 7 * SB,
+    C_FMIN = 8 * SB,      // far minimum (uint64)
+    C_SLOTS = 9 * SB
+};
+
+struct SsspBatch {
+    int32_t* qn[2];     // [SB][cap] near queues by parity
+    int32_t* qf[2];     // [SB][V]   far piles (per-source parity in C_FPAR)
+    uint64_t* farbits;  // [SB][W]
+    int32_t* stamp;     // [SB][V]
+    int64_t* ctl;       // C_SLOTS
+    uint64_t* dist;     // [SB][V] output rows
+    int64_t cap, V, W;
+};
+
+__global__ void kBInit(SsspBatch B, const int32_t* __restrict__ src, int k, uint64_t delta) {
+    const int s = threadIdx.x;
+    if (s >= SB) return;
+    int64_t* c = B.ctl;
+    for (int j = 0; j < 9; ++j) c[j * SB + s] = 0;
+    c[C_HI + s] = (int64_t)delta;
+    if (s < k) {
+        const int32_t v = src[s];
+        if (v >= 0 && v < B.V) {
+            B.dist[(size_t)s * B.V + v] = 0;
+            B.qn[0][(size_t)s * B.cap] = v;
+            c[C_NEAR0 + s] = 1;
+        }
+    }
+}
+
+__global__ void kBRelax(SsspBatch B, int par, const int32_t* __restrict__ fb, const int32_t* __restrict__ fa,
+                        const uint32_t* __restrict__ w, int32_t round) {
+    const int s = blockIdx.y;
+    int64_t* c = B.ctl;
+    const int64_t nin = c[(par ? C_NEAR1 : C_NEAR0) + s];
+    if ((int64_t)blockIdx.x * blockDim.x >= nin) return;  // whole block idle for this source
+    const int64_t V = B.V;
+    const int32_t* qin = B.qn[par] + (size_t)s * B.cap;
+    int32_t* qout = B.qn[par ^ 1] + (size_t)s * B.cap;
+    int32_t* qfar = B.qf[c[C_FPAR + s]] + (size_t)s * V;
+    uint64_t* fbits = B.farbits + (size_t)s * B.W;
+    int32_t* stamp = B.stamp + (size_t)s * V;
+    uint64_t* d = B.dist + (size_t)s * V;
+    int64_t* nnext = c + (par ? C_NEAR0 : C_NEAR1) + s;
+    int64_t* nfar = c + C_FAR + s;
+    const uint64_t hi = (uint64_t)c[C_HI + s];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < nin;
+    const int32_t u = act ? qin[i] : 0;
+    const uint64_t du = act ? ldAcq(&d[u]) : 0;
+    const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+    waveRowsP(act, u, du, b, e, [&](int32_t p, uint64_t dp, int64_t j) {
+        const int32_t x = fa[j];
+        const uint64_t nd = dp + (w ? (uint64_t)w[j] : (uint64_t)hashWeight(p, x));
+        bool near = false, far = false;
+        if (nd < ldAcq(&d[x])) {
+            const uint64_t old = atomicMin((unsigned long long*)&d[x], (unsigned long long)nd);
+            if (nd < old) {
+                if (nd < hi) {
+                    near = atomicExch(&stamp[x], round) != round;
+                } else {
+                    const uint64_t bit = 1ULL << (x & 63);
+                    far = !(atomicOr((unsigned long long*)&fbits[x >> 6], (unsigned long long)bit) & bit);
+                }
+            }
+        }
+        const int64_t pn = wavePush(nnext, near);
+        if (near) qout[pn] = x;
+        const int64_t pf = wavePush(nfar, far);
+        if (far) qfar[pf] = x;
+    });
+}
+
+// The consumed queue is emptied; a source advances when its next queue is
+// empty and its far pile is not.
+__global__ void kBPrep(SsspBatch B, int par) {
+    const int s = threadIdx.x;
+    if (s >= SB) return;
+    int64_t* c = B.ctl;
+    c[(par ? C_NEAR1 : C_NEAR0) + s] = 0;
+    const bool adv = c[(par ? C_NEAR0 : C_NEAR1) + s] == 0 && c[C_FAR + s] > 0;
+    c[C_ADV + s] = adv;
+    c[C_FMIN + s] = (int64_t)INF;
+    c[C_KEEP + s] = 0;
+}
+
+__global__ void kBFarMin(SsspBatch B) {
+    const int s = blockIdx.y;
+    const int64_t* c = B.ctl;
+    if (!c[C_ADV + s]) return;
+    const int64_t n = c[C_FAR + s];
+    const int32_t* qfar = B.qf[c[C_FPAR + s]] + (size_t)s * B.V;
+    const uint64_t* d = B.dist + (size_t)s * B.V;
+    const uint64_t hi = (uint64_t)c[C_HI + s];
+    unsigned long long m = INF;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = d[qfar[i]];
+        if (x >= hi && x < m) m = x;
+    }
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_xor(m, o);
+        m = t < m ? t : m;
+    }
+    if (laneId() == 0 && m != INF) atomicMin((unsigned long long*)&B.ctl[C_FMIN + s], m);
+}
+
+__global__ void kBAdvance(SsspBatch B, uint64_t delta) {
+    const int s = threadIdx.x;
+    if (s >= SB) return;
+    int64_t* c = B.ctl;
+    if (!c[C_ADV + s]) return;
+    const uint64_t m = (uint64_t)c[C_FMIN + s];
+    const uint64_t lo = m == INF ? INF : (m / delta) * delta;
+    c[C_LO + s] = (int64_t)lo;
+    c[C_HI + s] = (int64_t)((m == INF || lo > INF - delta) ? INF : lo + delta);
+}
+
+// Far pile -> next near queue (d in [lo, hi)), kept pile (d >= hi) or
+// dropped (d < lo: settled).  Every entry leaving the pile clears its bit.
+__global__ void kBSplit(SsspBatch B, int par) {
+    const int s = blockIdx.y;
+    int64_t* c = B.ctl;
+    if (!c[C_ADV + s]) return;
+    const int64_t n = c[C_FAR + s];
+    const int fp = (int)c[C_FPAR + s];
+    const int32_t* qfar = B.qf[fp] + (size_t)s * B.V;
+    int32_t* qkeep = B.qf[fp ^ 1] + (size_t)s * B.V;
+    int32_t* qnear = B.qn[par ^ 1] + (size_t)s * B.cap;
+    uint64_t* fbits = B.farbits + (size_t)s * B.W;
+    const uint64_t* d = B.dist + (size_t)s * B.V;
+    const uint64_t lo = (uint64_t)c[C_LO + s], hi = (uint64_t)c[C_HI + s];
+    int64_t* nnear = c + (par ? C_NEAR0 : C_NEAR1) + s;
+    int64_t* nkeep = c + C_KEEP + s;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // uniform trip count across the wave so wavePush sees the whole wave
+    const int64_t base0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1));
+    for (int64_t base = base0; base < n; base += stride) {
+        const int64_t i = base + laneId();
+        bool near = false, keep = false;
+        int32_t x = 0;
+        if (i < n) {
+            x = qfar[i];
+            const uint64_t dx = d[x];
+            keep = dx >= hi;
+            near = dx >= lo && dx < hi;
+            if (!keep) atomicAnd((unsigned long long*)&fbits[x >> 6], ~(1ULL << (x & 63)));
+        }
+        const int64_t pn = wavePush(nnear, near);
+        if (near) qnear[pn] = x;
+        const int64_t pk = wavePush(nkeep, keep);
+        if (keep) qkeep[pk] = x;
+    }
+}
+
+__global__ void kBFinish(SsspBatch B) {
+    const int s = threadIdx.x;
+    if (s >= SB) return;
+    int64_t* c = B.ctl;
+    if (!c[C_ADV + s]) return;
+    c[C_FAR + s] = c[C_KEEP + s];
+    c[C_FPAR + s] ^= 1;
+}
+
+// k <= SB sources (device array dsrc) into rows of dist.
+static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, uint64_t* dist, int32_t& round,
+                      DevBuf<int32_t>* qn, DevBuf<int32_t>* qf, DevBuf<uint64_t>& fbits, DevBuf<int32_t>& stamp,
+                      DevBuf<int64_t>& ctl) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    const uint32_t* w = h->has_w ? h->w_fw.p : nullptr;
+    SsspBatch B;
+    B.V = V;
+    B.W = ((int64_t)V + 63) / 64;
+    B.cap = V;  // a vertex enters a near queue at most once per round
+    for (int p = 0; p < 2; ++p) {
+        B.qn[p] = qn[p].p;
+        B.qf[p] = qf[p].p;
+    }
+    B.farbits = fbits.p;
+    B.stamp = stamp.p;
+    B.ctl = ctl.p;
+    B.dist = dist;
+    fillU64(h, dist, INF, (int64_t)k * V);
+    fillU64(h, fbits.p, 0, (int64_t)SB * B.W);
+    launch(h, "sssp_b_init", kBInit, 1, 64, 0, B, dsrc, k, delta);
+    const unsigned fgrid = 256;  // grid-stride width of the far passes (per source)
+    int par = 0;
+    std::vector<int64_t> r(2 * SB);
+    int64_t maxn = 1;
+    while (true) {
+        ++round;
+        launch(h, "sssp_b_relax", kBRelax, dim3(gridFor(maxn, blk), k), blk, 0, B, par, (const int32_t*)h->fw_beg.p,
+               (const int32_t*)h->fw_adj.p, w, round);
+        launch(h, "sssp_b_prep", kBPrep, 1, 64, 0, B, par);
+        launch(h, "sssp_b_farmin", kBFarMin, dim3(fgrid, k), blk, 0, B);
+        launch(h, "sssp_b_advance", kBAdvance, 1, 64, 0, B, delta);
+        launch(h, "sssp_b_split", kBSplit, dim3(fgrid, k), blk, 0, B, par);
+        launch(h, "sssp_b_finish", kBFinish, 1, 64, 0, B);
+        par ^= 1;
+        // next near lengths + far lengths
+        readCounters(h, ctl.p + (par ? C_NEAR1 : C_NEAR0), SB, r.data());
+        readCounters(h, ctl.p + C_FAR, SB, r.data() + SB);
+        maxn = 0;
+        int64_t far = 0;
+        for (int s = 0; s < k; ++s) {
+            maxn = std::max(maxn, r[s]);
+            far += r[SB + s];
+        }
+        if (maxn == 0 && far == 0) break;
+        if (maxn == 0) maxn = 1;  // (only far piles left: an empty relax round, then they advance)
+    }
+}
+
 }  // namespace gir
 
 using namespace gir;
