@@ -80,19 +80,26 @@ __global__ void kMsSeed(const int32_t* __restrict__ src, int k, int32_t V, uint6
     levels[(size_t)s * V + v] = 0;
 }
 
-__global__ void kMsPush(const int32_t* __restrict__ fb, const int32_t* __restrict__ fa,
-                        const uint64_t* __restrict__ front, const uint64_t* seen, uint64_t* next, int32_t V,
-                        int64_t* cz) {
-    if (blockIdx.x == 0 && threadIdx.x < 3) cz[threadIdx.x] = 0;
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t f = v < V ? front[v] : 0;
-    const bool act = f != 0;
-    const int32_t b = act ? fb[v] : 0, e = act ? fb[v + 1] : 0;
-    waveRowsP(act, (int32_t)v, f, b, e, [&](int32_t, uint64_t fp, int64_t j) {
+struct MsPushF {
+    const int32_t* fa;
+    const uint64_t* seen;
+    uint64_t* next;
+    __device__ int64_t operator()(int32_t, uint64_t fp, int32_t, int64_t j) const {
         const int32_t w = fa[j];
         const uint64_t m = fp & ~(seen[w] | ldAcq(&next[w]));
         if (m) atomicOr((unsigned long long*)&next[w], (unsigned long long)m);
-    });
+        return 0;
+    }
+};
+
+__global__ void kMsPush(const int32_t* __restrict__ fb, const uint64_t* __restrict__ front, int32_t V, MsPushF f,
+                        int64_t* cz, HeavyList H) {
+    if (blockIdx.x == 0 && threadIdx.x < 3) cz[threadIdx.x] = 0;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t fr = v < V ? front[v] : 0;
+    const bool act = fr != 0;
+    const int32_t b = act ? fb[v] : 0, e = act ? fb[v + 1] : 0;
+    waveRowsH(act, (int32_t)v, fr, 0, b, e, H, f);
 }
 
 __global__ void kMsPull(const int32_t* __restrict__ bb, const int32_t* __restrict__ ba,
@@ -178,9 +185,12 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels, int64_t
         if (pull)
             launch(h, "msbfs_pull", kMsPull, g, blk, 0, (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p,
                    (const uint64_t*)front, (const uint64_t*)seen, next, V, all, cz);
-        else
-            launch(h, "msbfs_push", kMsPush, g, blk, 0, (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p,
-                   (const uint64_t*)front, (const uint64_t*)seen, next, V, cz);
+        else {
+            const MsPushF f{h->fw_adj.p, seen, next};
+            launch(h, "msbfs_push", kMsPush, g, blk, 0, (const int32_t*)h->fw_beg.p, (const uint64_t*)front, V, f, cz,
+                   heavyList(h));
+            launchHeavy(h, "msbfs_push_heavy", f);
+        }
         launch(h, "msbfs_finish", kMsFinish, g, blk, 0, seen, front, next, levels, V, L, (const int32_t*)h->fw_beg.p,
                cn);
         int64_t r[2];

@@ -74,6 +74,7 @@ struct Params {
     double scc_color_min = 0;
     double profile = 0;  // 1: time every kernel launch with HIP events
     double msbfs = 1;    // 1: k>1 BFS runs 64 sources per bit-parallel sweep
+    double sssp_batch = 1;  // 1: SSSP sources of a call advance together
 };
 
 struct KStat {
@@ -92,6 +93,11 @@ struct Scratch {
     DevBuf<int64_t> off, off2;
     DevBuf<uint8_t> stage;
     DevBuf<uint64_t> ms[3];  // bit-parallel BFS: seen / frontier / next words
+    // heavy-row list
+    DevBuf<int64_t> hv_ctl, hv_b, hv_e;
+    DevBuf<int32_t> hv_row, hv_aux;
+    DevBuf<uint64_t> hv_pay;
+    size_t hv_cap = 0;
 };
 
 }  // namespace gir
@@ -168,14 +174,13 @@ inline unsigned gridFor(int64_t n, int block) {
 }
 
 template <class K, class... Args>
-void launch(girhip_graph* h, const char* name, K kernel, unsigned grid, unsigned block, size_t shmem,
-            Args... args) {
+void launch(girhip_graph* h, const char* name, K kernel, dim3 grid, dim3 block, size_t shmem, Args... args) {
     hipEvent_t a = nullptr, b = nullptr;
     if (h->prm.profile != 0) {
         a = takeEvent(h);
         HIPCHK(hipEventRecord(a, h->st));
     }
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, h->st, args...);
+    hipLaunchKernelGGL(kernel, grid, block, shmem, h->st, args...);
     HIPCHK(hipGetLastError());
     if (a) {
         b = takeEvent(h);
@@ -301,6 +306,88 @@ __device__ __forceinline__ void waveRowsP(bool act, int32_t v, P pay, int64_t b,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Heavy rows.  RMAT hubs hold 10^5 arcs; a row walked by one wave costs
+// thousands of dependent steps while the rest of the GPU idles.  A row kernel
+// therefore hands every row longer than HEAVY_T arcs to a heavy list instead
+// of walking it, and a follow-up launch of kHeavyRows walks each listed row
+// with the whole grid (grid-stride over the row's arcs).  The row work is a
+// functor shared by both kernels: f(v, pay, aux, e) -> int64 contribution
+// (summed per lane; the kernels reduce it per wave).  The list is reset by
+// the last block of kHeavyRows, so consecutive launches need no host work.
+constexpr int64_t HEAVY_T = 2048;
+
+struct HeavyList {
+    int64_t* ctl;  // [0] count, [1] blocks done
+    int32_t* row;
+    int32_t* aux;
+    int64_t* b;
+    int64_t* e;
+    uint64_t* pay;
+};
+
+template <int SMALL = 16, class F>
+__device__ __forceinline__ int64_t waveRowsH(bool act, int32_t v, uint64_t pay, int32_t aux, int64_t b, int64_t end,
+                                             const HeavyList& H, const F& f) {
+    const bool heavy = act && (end - b) > HEAVY_T;
+    const int64_t pos = wavePush(&H.ctl[0], heavy);
+    if (heavy) {
+        H.row[pos] = v;
+        H.aux[pos] = aux;
+        H.b[pos] = b;
+        H.e[pos] = end;
+        H.pay[pos] = pay;
+    }
+    int64_t acc = 0;
+    const bool mine = act && !heavy;
+    int64_t i = b;
+    if (mine) {
+        const int64_t stop = end < b + SMALL ? end : b + SMALL;
+        for (; i < stop; ++i) acc += f(v, pay, aux, i);
+    }
+    uint64_t rest = __ballot(mine && i < end);
+    while (rest) {
+        const int l = __ffsll((long long)rest) - 1;
+        rest &= rest - 1;
+        const int32_t hv = __shfl(v, l), ha = __shfl(aux, l);
+        const uint64_t hp = __shfl(pay, l);
+        const int64_t hb = __shfl(i, l), he = __shfl(end, l);
+        for (int64_t j = hb + laneId(); j < he; j += WAVE) acc += f(hv, hp, ha, j);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ int64_t waveSum(int64_t x) {
+    for (int d = WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d);
+    return x;
+}
+
+template <class F>
+__global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
+    const int64_t n = ldAcq(&H.ctl[0]);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t acc = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        const int32_t v = H.row[k], aux = H.aux[k];
+        const int64_t b = H.b[k], e = H.e[k];
+        const uint64_t pay = H.pay[k];
+        for (int64_t j = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < e; j += stride) acc += f(v, pay, aux, j);
+    }
+    if (sum) {
+        acc = waveSum(acc);
+        if (laneId() == 0 && acc) atomicAdd((unsigned long long*)sum, (unsigned long long)acc);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd((unsigned long long*)&H.ctl[1], 1ULL) == gridDim.x - 1) {
+            H.ctl[0] = 0;
+            H.ctl[1] = 0;
+            __threadfence();
+        }
+    }
+}
+
 // Small utility kernels shared by several translation units (one private
 // copy per unit).
 // cnt += number of v with lab[v] == v (component representatives).
@@ -324,5 +411,16 @@ __device__ __forceinline__ uint32_t hashWeight(int32_t u, int32_t v) {
     x *= 0xc4ceb9fe1a85ec53ULL;
     x ^= x >> 33;
     return 1u + (uint32_t)(x % 100ULL);
+}
+}  // namespace gir
+
+namespace gir {
+// The handle's heavy list (capacity follows E; zeroed on allocation).
+HeavyList heavyList(girhip_graph* h);
+// Launch kHeavyRows for functor f after a row kernel (sum: optional int64
+// accumulator for the functor's contributions).
+template <class F>
+void launchHeavy(girhip_graph* h, const char* name, const F& f, int64_t* sum = nullptr) {
+    launch(h, name, kHeavyRows<F>, 512, 256, 0, heavyList(h), f, sum);
 }
 }  // namespace gir

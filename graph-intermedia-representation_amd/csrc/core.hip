@@ -80,6 +80,24 @@ void copyOut(girhip_graph* h, void* dst, const void* src_dev, size_t bytes) {
     HIPCHK(hipMemcpyAsync(dst, src_dev, bytes, hipMemcpyDefault, h->st));
 }
 
+HeavyList heavyList(girhip_graph* h) {
+    Scratch& S = h->ws;
+    // a row is listed at most once per source row of a launch (batched SSSP
+    // runs up to 32 sources per launch)
+    const size_t cap = (size_t)(34 * (h->E / HEAVY_T) + 64);
+    if (S.hv_cap < cap || !S.hv_ctl.p) {
+        S.hv_ctl.ensure(2);
+        S.hv_row.ensure(cap);
+        S.hv_aux.ensure(cap);
+        S.hv_b.ensure(cap);
+        S.hv_e.ensure(cap);
+        S.hv_pay.ensure(cap);
+        fillI64(h, S.hv_ctl.p, 0, 2);
+        S.hv_cap = cap;
+    }
+    return HeavyList{S.hv_ctl.p, S.hv_row.p, S.hv_aux.p, S.hv_b.p, S.hv_e.p, S.hv_pay.p};
+}
+
 void readCounters(girhip_graph* h, const int64_t* dev, int n, int64_t* out) {
     HIPCHK(hipMemcpyAsync(h->hcnt, dev, sizeof(int64_t) * n, hipMemcpyDeviceToHost, h->st));
     syncStream(h);
@@ -410,7 +428,7 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"block", &Params::block},                 {"wcc_rounds", &Params::wcc_rounds},
                {"scc_trim_rounds", &Params::scc_trim_rounds}, {"scc_pivot", &Params::scc_pivot},
                {"scc_color_min", &Params::scc_color_min}, {"profile", &Params::profile},
-               {"msbfs", &Params::msbfs}};
+               {"msbfs", &Params::msbfs},                 {"sssp_batch", &Params::sssp_batch}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;

@@ -41,17 +41,19 @@ __global__ void kReachSeed(const int32_t* __restrict__ q, int64_t n, uint64_t* v
 // cnt[0] = next queue length, cnt[1] = sum of push-direction degrees of the
 // claimed vertices; cz points at the counter pair of the following level,
 // which this kernel zeroes.
+// Top-down work on arc slot j of row p: claim the head, queue it, return its
+// push-direction degree (summed into cnt[1]).
 template <class Flt, class Vis>
-__global__ void kTopDown(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ beg,
-                         const int32_t* __restrict__ adj, uint64_t* vis, int32_t L, int32_t* __restrict__ qout,
-                         int64_t* cnt, int64_t* cz, Flt flt, Vis on) {
-    if (blockIdx.x == 0 && threadIdx.x < 2) cz[threadIdx.x] = 0;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = i < nin;
-    const int32_t u = act ? qin[i] : 0;
-    const int32_t b = act ? beg[u] : 0, e = act ? beg[u + 1] : 0;
-    int64_t m = 0;
-    waveRows(act, u, b, e, [&](int32_t p, int64_t j) {
+struct TopDownF {
+    const int32_t* beg;
+    const int32_t* adj;
+    uint64_t* vis;
+    int32_t L;
+    int32_t* qout;
+    int64_t* cnt;
+    Flt flt;
+    Vis on;
+    __device__ int64_t operator()(int32_t p, uint64_t, int32_t, int64_t j) const {
         const int32_t w = adj[j];
         bool claim = false;
         if (flt(w, p)) {
@@ -63,12 +65,23 @@ __global__ void kTopDown(const int32_t* __restrict__ qin, int64_t nin, const int
         if (claim) {
             qout[pos] = w;
             on(w, L + 1);
-            m += beg[w + 1] - beg[w];
+            return beg[w + 1] - beg[w];
         }
-    });
-    // wave-reduce m
-    for (int d = WAVE / 2; d > 0; d >>= 1) m += __shfl_xor(m, d);
-    if (laneId() == 0 && m) atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)m);
+        return 0;
+    }
+};
+
+template <class Flt, class Vis>
+__global__ void kTopDown(const int32_t* __restrict__ qin, int64_t nin, TopDownF<Flt, Vis> f, int64_t* cz,
+                         HeavyList H) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) cz[threadIdx.x] = 0;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < nin;
+    const int32_t u = act ? qin[i] : 0;
+    const int32_t b = act ? f.beg[u] : 0, e = act ? f.beg[u + 1] : 0;
+    int64_t m = waveRowsH(act, u, 0, 0, b, e, H, f);
+    m = waveSum(m);
+    if (laneId() == 0 && m) atomicAdd((unsigned long long*)&f.cnt[1], (unsigned long long)m);
 }
 
 template <class Flt, class Vis>
@@ -163,8 +176,10 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
         int64_t* cn = c + 2 * (L & 1);
         int64_t* cz = c + 2 * ((L + 1) & 1);
         if (!bu) {
-            launch(h, "reach_topdown", kTopDown<Flt, Vis>, gridFor(nf, blk), blk, 0, (const int32_t*)B.q[cur], nf, pbeg,
-                   padj, B.vis, L, B.q[cur ^ 1], cn, cz, flt, on);
+            const TopDownF<Flt, Vis> f{pbeg, padj, B.vis, L, B.q[cur ^ 1], cn, flt, on};
+            launch(h, "reach_topdown", kTopDown<Flt, Vis>, gridFor(nf, blk), blk, 0, (const int32_t*)B.q[cur], nf, f,
+                   cz, heavyList(h));
+            launchHeavy(h, "reach_topdown_heavy", f, cn + 1);
             bits_valid = false;
         } else {
             if (!bits_valid) {

@@ -134,14 +134,15 @@ __global__ void kColorInit(const int32_t* __restrict__ list, int64_t n, int32_t*
 // One propagation step: every queued vertex pushes its colour to its active
 // out-neighbours (atomicMin); a neighbour whose colour dropped is queued once
 // per step (stamp).
-__global__ void kColorProp(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ fb,
-                           const int32_t* __restrict__ fa, const int32_t* __restrict__ scc, int32_t* color,
-                           int32_t* stamp, int32_t it, int32_t* __restrict__ qout, int64_t* cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = i < nin;
-    const int32_t x = act ? qin[i] : 0;
-    const int32_t b = act ? fb[x] : 0, e = act ? fb[x + 1] : 0;
-    waveRows(act, x, b, e, [&](int32_t p, int64_t j) {
+struct ColorPropF {
+    const int32_t* fa;
+    const int32_t* scc;
+    int32_t* color;
+    int32_t* stamp;
+    int32_t it;
+    int32_t* qout;
+    int64_t* cnt;
+    __device__ int64_t operator()(int32_t p, uint64_t, int32_t, int64_t j) const {
         const int32_t y = fa[j];
         bool push = false;
         if (ldAcq(&scc[y]) == -1) {
@@ -153,7 +154,17 @@ __global__ void kColorProp(const int32_t* __restrict__ qin, int64_t nin, const i
         }
         const int64_t pos = wavePush(cnt, push);
         if (push) qout[pos] = y;
-    });
+        return 0;
+    }
+};
+
+__global__ void kColorProp(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ fb, ColorPropF f,
+                           HeavyList H) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < nin;
+    const int32_t x = act ? qin[i] : 0;
+    const int32_t b = act ? fb[x] : 0, e = act ? fb[x + 1] : 0;
+    waveRowsH(act, x, 0, 0, b, e, H, f);
 }
 
 __global__ void kColorRoots(const int32_t* __restrict__ list, int64_t n, const int32_t* __restrict__ color,
@@ -178,20 +189,30 @@ __global__ void kColorAssign(const int32_t* __restrict__ list, int64_t n, const 
 }
 
 // ---- DAG ----
-__global__ void kDagKeys(const int32_t* __restrict__ fb, const int32_t* __restrict__ fa,
-                         const int32_t* __restrict__ scc, int32_t V, int B, uint64_t* __restrict__ keys, int64_t* cnt) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = v < V;
-    const int32_t b = act ? fb[v] : 0, e = act ? fb[v + 1] : 0;
-    const int32_t su = act ? scc[v] : 0;
-    waveRows(act, (int32_t)v, b, e, [&](int32_t u, int64_t j) {
-        const int32_t a = scc[u];
+// Crossing arc slot j of row u -> packed key (scc[u], scc[head]); the row's
+// label travels as the payload.
+struct DagKeysF {
+    const int32_t* fa;
+    const int32_t* scc;
+    int B;
+    uint64_t* keys;
+    int64_t* cnt;
+    __device__ int64_t operator()(int32_t, uint64_t su, int32_t, int64_t j) const {
+        const int32_t a = (int32_t)su;
         const int32_t c = scc[fa[j]];
         const bool cross = a != c;
         const int64_t pos = wavePush(cnt, cross);
         if (cross) keys[pos] = ((uint64_t)(uint32_t)a << B) | (uint32_t)c;
-    });
-    (void)su;
+        return 0;
+    }
+};
+
+__global__ void kDagKeys(const int32_t* __restrict__ fb, int32_t V, DagKeysF f, HeavyList H) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = v < V;
+    const int32_t b = act ? fb[v] : 0, e = act ? fb[v + 1] : 0;
+    const uint64_t su = act ? (uint64_t)(uint32_t)f.scc[v] : 0;
+    waveRowsH(act, (int32_t)v, su, 0, b, e, H, f);
 }
 
 __global__ void kRunFlags(const uint64_t* __restrict__ keys, int64_t n, int32_t* __restrict__ flag) {
@@ -207,7 +228,7 @@ __global__ void kRunStarts(const int32_t* __restrict__ flag, const int64_t* __re
 
 __global__ void kDagWrite(const uint64_t* __restrict__ keys, const int32_t* __restrict__ start, int64_t nd,
                           int64_t ncross, int B, int32_t* __restrict__ src, int32_t* __restrict__ dst,
-                          int32_t* __restrict__ mult, int32_t* __restrict__ outdeg, int32_t* __restrict__ indeg) {
+                          int32_t* __restrict__ mult, int32_t* __restrict__ indeg) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nd) return;
     const int64_t i = start[r], j = r + 1 < nd ? start[r + 1] : ncross;
@@ -216,8 +237,20 @@ __global__ void kDagWrite(const uint64_t* __restrict__ keys, const int32_t* __re
     src[r] = a;
     dst[r] = b;
     mult[r] = (int32_t)(j - i);
-    atomicAdd(&outdeg[a], 1);
     atomicAdd(&indeg[b], 1);
+}
+
+// DAG CSR offsets: arcs are sorted by source, so beg[s] = lower_bound(src, s).
+__global__ void kDagBeg(const int32_t* __restrict__ src, int64_t nd, int32_t V, int32_t* __restrict__ beg) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > V) return;
+    int64_t lo = 0, hi = nd;
+    while (lo < hi) {
+        const int64_t m = lo + ((hi - lo) >> 1);
+        if (src[m] < s) lo = m + 1;
+        else hi = m;
+    }
+    beg[s] = (int32_t)lo;
 }
 
 // ---- ranks (Kahn levels) ----
@@ -233,20 +266,28 @@ __global__ void kRankSeeds(const int32_t* __restrict__ scc, const int32_t* __res
     if (seed) out[pos] = (int32_t)v;
 }
 
+struct KahnF {
+    const int32_t* ddst;
+    int32_t* indeg;
+    int32_t* qout;
+    int64_t* cnt;
+    __device__ int64_t operator()(int32_t, uint64_t, int32_t, int64_t j) const {
+        const int32_t t = ddst[j];
+        const bool ready = atomicSub(&indeg[t], 1) == 1;
+        const int64_t pos = wavePush(cnt, ready);
+        if (ready) qout[pos] = t;
+        return 0;
+    }
+};
+
 __global__ void kKahnStep(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ dbeg,
-                          const int32_t* __restrict__ ddst, int32_t* indeg, int64_t* rank, int32_t L,
-                          int32_t* __restrict__ qout, int64_t* cnt) {
+                          int64_t* rank, int32_t L, KahnF f, HeavyList H) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < nin;
     const int32_t s = act ? qin[i] : 0;
     if (act) rank[s] = ((int64_t)L << 32) | (int64_t)(uint32_t)s;
     const int32_t b = act ? dbeg[s] : 0, e = act ? dbeg[s + 1] : 0;
-    waveRows(act, s, b, e, [&](int32_t, int64_t j) {
-        const int32_t t = ddst[j];
-        const bool ready = atomicSub(&indeg[t], 1) == 1;
-        const int64_t pos = wavePush(cnt, ready);
-        if (ready) qout[pos] = t;
-    });
+    waveRowsH(act, s, 0, 0, b, e, H, f);
 }
 
 namespace {
@@ -341,9 +382,10 @@ void runScc(girhip_graph* h) {
         while (nq > 0) {
             ++it;
             fillI64(h, h->dcnt.p + 6, 0, 1);
+            const ColorPropF f{h->fw_adj.p, h->scc.p, color, stamp, it, B.q[cur ^ 1], h->dcnt.p + 6};
             launch(h, "scc_color_prop", kColorProp, gridFor(nq, blk), blk, 0, (const int32_t*)B.q[cur], nq,
-                   (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p, (const int32_t*)h->scc.p, color, stamp,
-                   it, B.q[cur ^ 1], h->dcnt.p + 6);
+                   (const int32_t*)h->fw_beg.p, f, heavyList(h));
+            launchHeavy(h, "scc_color_prop_heavy", f);
             readCounters(h, h->dcnt.p + 6, 1, &nq);
             cur ^= 1;
         }
@@ -376,18 +418,16 @@ void runDag(girhip_graph* h) {
     const int B = bitsFor((uint64_t)(V > 1 ? V - 1 : 1));
     h->rank.ensure((size_t)(V > 0 ? V : 1));
     h->dag_beg.ensure((size_t)V + 1);
-    S.iv[1].ensure((size_t)V + 1);
     S.iv[2].ensure((size_t)V + 1);
-    int32_t* odeg = S.iv[1].p;
     int32_t* ideg = S.iv[2].p;
-    fillI32(h, odeg, 0, (int64_t)V + 1);
     fillI32(h, ideg, 0, (int64_t)V + 1);
     int64_t ncross = 0;
     if (h->E > 0) {
         S.keys[0].ensure((size_t)h->E);
         fillI64(h, h->dcnt.p + 9, 0, 1);
-        launch(h, "dag_keys", kDagKeys, gridFor(V, blk), blk, 0, (const int32_t*)h->fw_beg.p,
-               (const int32_t*)h->fw_adj.p, (const int32_t*)h->scc.p, V, B, S.keys[0].p, h->dcnt.p + 9);
+        const DagKeysF f{h->fw_adj.p, h->scc.p, B, S.keys[0].p, h->dcnt.p + 9};
+        launch(h, "dag_keys", kDagKeys, gridFor(V, blk), blk, 0, (const int32_t*)h->fw_beg.p, V, f, heavyList(h));
+        launchHeavy(h, "dag_keys_heavy", f);
         readCounters(h, h->dcnt.p + 9, 1, &ncross);
     }
     int64_t nd = 0;
@@ -406,17 +446,12 @@ void runDag(girhip_graph* h) {
         h->dag_dst.ensure((size_t)nd);
         h->dag_mult.ensure((size_t)nd);
         launch(h, "dag_write", kDagWrite, gridFor(nd, blk), blk, 0, (const uint64_t*)S.keys[0].p,
-               (const int32_t*)S.q[2].p, nd, ncross, B, h->dag_src.p, h->dag_dst.p, h->dag_mult.p, odeg, ideg);
+               (const int32_t*)S.q[2].p, nd, ncross, B, h->dag_src.p, h->dag_dst.p, h->dag_mult.p, ideg);
     }
     h->n_dag = nd;
     // DAG CSR offsets by source SCC (arcs are sorted by src)
-    {
-        DevBuf<int64_t>& pos = S.off2;
-        pos.ensure((size_t)V + 1);
-        scanI32toI64(h, odeg, pos.p, V);
-        launch(h, "dag_beg", kI64toI32, gridFor((int64_t)V + 1, blk), blk, 0, (const int64_t*)pos.p, h->dag_beg.p,
-               (int64_t)V + 1);
-    }
+    launch(h, "dag_beg", kDagBeg, gridFor((int64_t)V + 1, blk), blk, 0, (const int32_t*)h->dag_src.p, nd, V,
+           h->dag_beg.p);
     // Kahn levels
     ReachBufs R = reachBufs(h);
     fillI64(h, h->dcnt.p + 10, 0, 1);
@@ -428,9 +463,10 @@ void runDag(girhip_graph* h) {
     int32_t L = 0;
     while (nq > 0) {
         fillI64(h, h->dcnt.p + 11, 0, 1);
+        const KahnF f{h->dag_dst.p, ideg, R.q[cur ^ 1], h->dcnt.p + 11};
         launch(h, "rank_kahn", kKahnStep, gridFor(nq, blk), blk, 0, (const int32_t*)R.q[cur], nq,
-               (const int32_t*)h->dag_beg.p, (const int32_t*)h->dag_dst.p, ideg, h->rank.p, L, R.q[cur ^ 1],
-               h->dcnt.p + 11);
+               (const int32_t*)h->dag_beg.p, h->rank.p, L, f, heavyList(h));
+        launchHeavy(h, "rank_kahn_heavy", f);
         readCounters(h, h->dcnt.p + 11, 1, &nq);
         cur ^= 1;
         ++L;

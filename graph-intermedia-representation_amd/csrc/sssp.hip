@@ -24,18 +24,18 @@ std::vector<int32_t> hostSources(girhip_graph* h, const int32_t* src, int32_t k)
 
 constexpr uint64_t INF = ~0ULL;
 
+// Relaxation of one arc slot j of row p whose owner had distance dp.
 // cnt[0]: next near-queue length; cnt[1]: far-pile length (appended to).
-__global__ void kSsspRelax(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ fb,
-                           const int32_t* __restrict__ fa, const uint32_t* __restrict__ w, uint64_t* dist,
-                           int32_t* stamp, int32_t round, uint64_t hi, int32_t* __restrict__ qnear,
-                           int32_t* __restrict__ qfar, uint64_t* farbits, int64_t* cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool act = i < nin;
-    const int32_t u = act ? qin[i] : 0;
-    if (act && atomicExch(&stamp[u], round) == round) act = false;  // duplicate in this round
-    const uint64_t du = act ? ldAcq(&dist[u]) : 0;
-    const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
-    waveRowsP(act, u, du, b, e, [&](int32_t p, uint64_t dp, int64_t j) {
+struct RelaxF {
+    const int32_t* fa;
+    const uint32_t* w;
+    uint64_t* dist;
+    uint64_t hi;
+    int32_t* qnear;
+    int32_t* qfar;
+    uint64_t* farbits;
+    int64_t* cnt;
+    __device__ int64_t operator()(int32_t p, uint64_t dp, int32_t, int64_t j) const {
         const int32_t x = fa[j];
         const uint64_t nd = dp + (w ? (uint64_t)w[j] : (uint64_t)hashWeight(p, x));
         bool near = false, far = false;
@@ -54,7 +54,19 @@ __global__ void kSsspRelax(const int32_t* __restrict__ qin, int64_t nin, const i
         if (near) qnear[pn] = x;
         const int64_t pf = wavePush(&cnt[1], far);
         if (far) qfar[pf] = x;
-    });
+        return 0;
+    }
+};
+
+__global__ void kSsspRelax(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ fb,
+                           int32_t* stamp, int32_t round, RelaxF f, HeavyList H) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = i < nin;
+    const int32_t u = act ? qin[i] : 0;
+    if (act && atomicExch(&stamp[u], round) == round) act = false;  // duplicate in this round
+    const uint64_t du = act ? ldAcq(&f.dist[u]) : 0;
+    const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+    waveRowsH(act, u, du, 0, b, e, H, f);
 }
 
 // Smallest distance >= hi among far entries (entries below hi are settled).
@@ -129,9 +141,10 @@ static void ssspOne(girhip_graph* h, SsspBufs& Bf, int32_t s, uint64_t delta, ui
         while (nnear > 0) {
             ++round;
             launch(h, "sssp_setcnt", kSetCounters, 1, 64, 0, c, (int64_t)0, nfar);
+            const RelaxF f{h->fw_adj.p, w, dist, hi, Bf.qn[cn ^ 1], Bf.qf[cf], Bf.farbits, c};
             launch(h, "sssp_relax", kSsspRelax, gridFor(nnear, blk), blk, 0, (const int32_t*)Bf.qn[cn], nnear,
-                   (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p, w, dist, Bf.stamp, round, hi,
-                   Bf.qn[cn ^ 1], Bf.qf[cf], Bf.farbits, c);
+                   (const int32_t*)h->fw_beg.p, Bf.stamp, round, f, heavyList(h));
+            launchHeavy(h, "sssp_relax_heavy", f);
             int64_t r[2];
             readCounters(h, c, 2, r);
             nnear = r[0];
@@ -211,47 +224,51 @@ __global__ void kBInit(SsspBatch B, const int32_t* __restrict__ src, int k, uint
     }
 }
 
-__global__ void kBRelax(SsspBatch B, int par, const int32_t* __restrict__ fb, const int32_t* __restrict__ fa,
-                        const uint32_t* __restrict__ w, int32_t round) {
-    const int s = blockIdx.y;
-    int64_t* c = B.ctl;
-    const int64_t nin = c[(par ? C_NEAR1 : C_NEAR0) + s];
-    if ((int64_t)blockIdx.x * blockDim.x >= nin) return;  // whole block idle for this source
-    const int64_t V = B.V;
-    const int32_t* qin = B.qn[par] + (size_t)s * B.cap;
-    int32_t* qout = B.qn[par ^ 1] + (size_t)s * B.cap;
-    int32_t* qfar = B.qf[c[C_FPAR + s]] + (size_t)s * V;
-    uint64_t* fbits = B.farbits + (size_t)s * B.W;
-    int32_t* stamp = B.stamp + (size_t)s * V;
-    uint64_t* d = B.dist + (size_t)s * V;
-    int64_t* nnext = c + (par ? C_NEAR0 : C_NEAR1) + s;
-    int64_t* nfar = c + C_FAR + s;
-    const uint64_t hi = (uint64_t)c[C_HI + s];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = i < nin;
-    const int32_t u = act ? qin[i] : 0;
-    const uint64_t du = act ? ldAcq(&d[u]) : 0;
-    const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
-    waveRowsP(act, u, du, b, e, [&](int32_t p, uint64_t dp, int64_t j) {
+// Relaxation of arc slot j of row p (owner distance dp) for source s.
+struct BRelaxF {
+    SsspBatch B;
+    int par;
+    const int32_t* fa;
+    const uint32_t* w;
+    int32_t round;
+    __device__ int64_t operator()(int32_t p, uint64_t dp, int32_t s, int64_t j) const {
+        int64_t* c = B.ctl;
+        uint64_t* d = B.dist + (size_t)s * B.V;
         const int32_t x = fa[j];
         const uint64_t nd = dp + (w ? (uint64_t)w[j] : (uint64_t)hashWeight(p, x));
         bool near = false, far = false;
         if (nd < ldAcq(&d[x])) {
             const uint64_t old = atomicMin((unsigned long long*)&d[x], (unsigned long long)nd);
             if (nd < old) {
-                if (nd < hi) {
-                    near = atomicExch(&stamp[x], round) != round;
+                if (nd < (uint64_t)c[C_HI + s]) {
+                    near = atomicExch(&B.stamp[(size_t)s * B.V + x], round) != round;
                 } else {
                     const uint64_t bit = 1ULL << (x & 63);
-                    far = !(atomicOr((unsigned long long*)&fbits[x >> 6], (unsigned long long)bit) & bit);
+                    far = !(atomicOr((unsigned long long*)&B.farbits[(size_t)s * B.W + (x >> 6)],
+                                     (unsigned long long)bit) &
+                            bit);
                 }
             }
         }
-        const int64_t pn = wavePush(nnext, near);
-        if (near) qout[pn] = x;
-        const int64_t pf = wavePush(nfar, far);
-        if (far) qfar[pf] = x;
-    });
+        const int64_t pn = wavePush(c + (par ? C_NEAR0 : C_NEAR1) + s, near);
+        if (near) B.qn[par ^ 1][(size_t)s * B.cap + pn] = x;
+        const int64_t pf = wavePush(c + C_FAR + s, far);
+        if (far) B.qf[c[C_FPAR + s]][(size_t)s * B.V + pf] = x;
+        return 0;
+    }
+};
+
+__global__ void kBRelax(BRelaxF f, const int32_t* __restrict__ fb, HeavyList H) {
+    const int s = blockIdx.y;
+    const SsspBatch& B = f.B;
+    const int64_t nin = B.ctl[(f.par ? C_NEAR1 : C_NEAR0) + s];
+    if ((int64_t)blockIdx.x * blockDim.x >= nin) return;  // whole block idle for this source
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < nin;
+    const int32_t u = act ? B.qn[f.par][(size_t)s * B.cap + i] : 0;
+    const uint64_t du = act ? ldAcq(&B.dist[(size_t)s * B.V + u]) : 0;
+    const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+    waveRowsH(act, u, du, s, b, e, H, f);
 }
 
 // The consumed queue is emptied; a source advances when its next queue is
@@ -372,8 +389,10 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
     int64_t maxn = 1;
     while (true) {
         ++round;
-        launch(h, "sssp_b_relax", kBRelax, dim3(gridFor(maxn, blk), k), blk, 0, B, par, (const int32_t*)h->fw_beg.p,
-               (const int32_t*)h->fw_adj.p, w, round);
+        const BRelaxF f{B, par, h->fw_adj.p, w, round};
+        launch(h, "sssp_b_relax", kBRelax, dim3(gridFor(maxn, blk), k), blk, 0, f, (const int32_t*)h->fw_beg.p,
+               heavyList(h));
+        launchHeavy(h, "sssp_b_relax_heavy", f);
         launch(h, "sssp_b_prep", kBPrep, 1, 64, 0, B, par);
         launch(h, "sssp_b_farmin", kBFarMin, dim3(fgrid, k), blk, 0, B);
         launch(h, "sssp_b_advance", kBAdvance, 1, 64, 0, B, delta);
@@ -416,6 +435,30 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
             out = (uint64_t*)S.stage.p;
         }
         const size_t Vs = (size_t)(V > 0 ? V : 1);
+        if (h->prm.sssp_batch != 0 && k > 0) {
+            // all sources of a chunk advance together
+            const int kc = std::min<int32_t>(k, SB);
+            DevBuf<int32_t> qn[2], qf[2], stamp, dsrc;
+            DevBuf<uint64_t> fbits;
+            DevBuf<int64_t> ctl;
+            for (int p = 0; p < 2; ++p) {
+                qn[p].ensure((size_t)kc * Vs);
+                qf[p].ensure((size_t)kc * Vs);
+            }
+            stamp.ensure((size_t)kc * Vs);
+            fbits.ensure((size_t)SB * ((Vs + 63) / 64));
+            ctl.ensure(C_SLOTS);
+            dsrc.ensure((size_t)k);
+            copyIn(h, dsrc.p, s.data(), sizeof(int32_t) * (size_t)k);
+            tBegin(h);
+            int32_t round = 0;
+            fillI32(h, stamp.p, 0, (int64_t)kc * V);
+            for (int32_t i0 = 0; i0 < k; i0 += SB) {
+                const int c = std::min<int32_t>(SB, k - i0);
+                ssspBatch(h, dsrc.p + i0, c, delta, out + (size_t)i0 * V, round, qn, qf, fbits, stamp, ctl);
+            }
+            tEnd(h, "sssp");
+        } else {
         S.q[0].ensure(Vs);
         S.q[1].ensure(Vs);
         S.iv[0].ensure(Vs);
@@ -447,6 +490,7 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
             ssspOne(h, Bf, s[i], delta, row, round);
         }
         tEnd(h, "sssp");
+        }
         if (edges_out) {
             std::vector<int64_t> e(k);
             for (int32_t i0 = 0; i0 < k; i0 += 32) {

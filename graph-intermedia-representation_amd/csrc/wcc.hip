@@ -94,12 +94,22 @@ __global__ void kWccSample(const int32_t* __restrict__ comp, int32_t V, uint64_t
     if (t == 0) out[0] = (int64_t)(uint32_t)(best & 0xffffffffULL);
 }
 
-__global__ void kWccHookRest(const int32_t* __restrict__ fb, const int32_t* __restrict__ fa,
-                             const int32_t* __restrict__ bb, const int32_t* __restrict__ ba, int32_t* comp, int32_t V,
-                             int rounds, const int64_t* __restrict__ giant) {
+// aux 0: slot j of the forward row, aux 1: of the backward row.
+struct HookF {
+    const int32_t* fa;
+    const int32_t* ba;
+    int32_t* comp;
+    __device__ int64_t operator()(int32_t u, uint64_t, int32_t aux, int64_t j) const {
+        hook(u, aux ? ba[j] : fa[j], comp);
+        return 0;
+    }
+};
+
+__global__ void kWccHookRest(const int32_t* __restrict__ fb, const int32_t* __restrict__ bb, int32_t V, int rounds,
+                             const int64_t* __restrict__ giant, HookF f, HeavyList H) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t c = (int32_t)giant[0];
-    const bool act = v < V && ldAcq(&comp[v]) != c;
+    const bool act = v < V && ldAcq(&f.comp[v]) != c;
     int32_t b = 0, e = 0, ib = 0, ie = 0;
     if (act) {
         b = fb[v] + rounds;
@@ -108,8 +118,8 @@ __global__ void kWccHookRest(const int32_t* __restrict__ fb, const int32_t* __re
         ib = bb[v];
         ie = bb[v + 1];
     }
-    waveRows(act, (int32_t)v, b, e, [&](int32_t u, int64_t j) { hook(u, fa[j], comp); });
-    waveRows(act, (int32_t)v, ib, ie, [&](int32_t u, int64_t j) { hook(u, ba[j], comp); });
+    waveRowsH(act, (int32_t)v, 0, 0, b, e, H, f);
+    waveRowsH(act, (int32_t)v, 0, 1, ib, ie, H, f);
 }
 
 void runWcc(girhip_graph* h) {
@@ -130,8 +140,10 @@ void runWcc(girhip_graph* h) {
         launch(h, "wcc_compress", kWccCompress, g, blk, 0, comp, V);
     }
     launch(h, "wcc_sample", kWccSample, 1, 1024, 0, (const int32_t*)comp, V, (uint64_t)0x5EED, h->dcnt.p + 8);
-    launch(h, "wcc_hook_rest", kWccHookRest, g, blk, 0, (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p,
-           (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p, comp, V, rounds, (const int64_t*)h->dcnt.p + 8);
+    const HookF f{h->fw_adj.p, h->bw_adj.p, comp};
+    launch(h, "wcc_hook_rest", kWccHookRest, g, blk, 0, (const int32_t*)h->fw_beg.p, (const int32_t*)h->bw_beg.p, V,
+           rounds, (const int64_t*)h->dcnt.p + 8, f, heavyList(h));
+    launchHeavy(h, "wcc_hook_rest_heavy", f);
     launch(h, "wcc_compress", kWccCompress, g, blk, 0, comp, V);
     fillI64(h, h->dcnt.p, 0, 1);
     launch(h, "count_roots", kCountSelf, g, blk, 0, (const int32_t*)comp, V, h->dcnt.p);
