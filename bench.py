@@ -92,6 +92,10 @@ def main():
     ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--profile", action="store_true", help="one extra step with per-kernel HIP event timing")
+    ap.add_argument("--phase", default="all", choices=["all", "wcc", "scc_dag", "bfs", "sssp"],
+                    help="time one phase only (for hardware-counter runs); prints a short JSON line")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -104,6 +108,15 @@ def main():
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
     dev = torch.device("cuda:0")
+    if args.calib_copy_mib:
+        # a plain copy of known size: read + write calib bytes (counter scale)
+        n = args.calib_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.int32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written")
+        del a, b
     cfg = WORKLOADS[args.workload]
     t0 = time.time()
     V, fb, fa, bb, ba = make_graph(cfg)
@@ -123,12 +136,23 @@ def main():
     dist = torch.empty((N_SSSP, V), dtype=torch.int64, device=dev)
     torch.cuda.synchronize()
 
+    calls = {"wcc": lambda: G.wcc(out=lab), "scc_dag": lambda: G.scc_dag(out=scc),
+             "bfs": lambda: G.bfs(bfs_src, out=lv), "sssp": lambda: G.sssp(sssp_src, 16, out=dist)}
+    run = list(calls) if args.phase == "all" else [args.phase]
+
     def step():
-        G.wcc(out=lab)
-        G.scc_dag(out=scc)
-        G.bfs(bfs_src, out=lv)
-        G.sssp(sssp_src, 16, out=dist)
-        return {k: G.last_ms(k) for k in ("wcc", "scc_dag", "bfs", "sssp")}
+        for k in run:
+            calls[k]()
+        return {k: G.last_ms(k) for k in run}
+
+    if args.phase != "all":
+        for i in range(args.warmup):
+            step()
+        ph = [step() for _ in range(args.steps)]
+        print(json.dumps({"phase": args.phase, "workload": args.workload, "V": V, "E": E,
+                          "ms": [p[args.phase] for p in ph]}), flush=True)
+        G.close()
+        return
 
     for i in range(args.warmup):
         step()
@@ -174,15 +198,9 @@ def main():
         G.set_param("profile", 1)
         G.reset_kernel_stats()
         step()
-        names = ["wcc_init", "wcc_hook_round", "wcc_compress", "wcc_sample", "wcc_hook_rest", "count_roots",
-                 "scc_compact", "scc_trim1", "scc_pivot", "reach_seed", "reach_topdown", "reach_q2bits",
-                 "reach_bottomup", "scc_fwbw_min", "scc_fwbw_assign", "scc_color_init", "scc_color_prop",
-                 "scc_color_roots", "scc_color_assign", "dag_keys", "radix_hist", "radix_scatter", "scan_reduce",
-                 "scan_blocks", "scan_down", "dag_flags", "dag_starts", "dag_write", "dag_beg", "rank_seeds",
-                 "rank_kahn", "sssp_setcnt", "sssp_relax", "sssp_farmin", "sssp_split", "fill"]
-        ks = {n: G.kernel_stats(n) for n in names}
+        ks = G.all_kernel_stats()
         kernels = sorted(({"name": n, "launches": c, "ms": round(m, 3)} for n, (c, m) in ks.items() if c),
-                         key=lambda x: -x["ms"])[:12]
+                         key=lambda x: -x["ms"])[:16]
         G.set_param("profile", 0)
 
     cpu = None

@@ -35,10 +35,10 @@ ReachBufs reachBufs(girhip_graph* h) {
 // m_s per row: out-arc count summed over reached vertices (level >= 0).
 __global__ void kReachedArcs(const int32_t* __restrict__ level, const int32_t* __restrict__ beg, int32_t V,
                              int64_t* out) {
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t m = (v < V && level[v] >= 0) ? (int64_t)(beg[v + 1] - beg[v]) : 0;
-    for (int d = WAVE / 2; d > 0; d >>= 1) m += __shfl_xor(m, d);
-    if (laneId() == 0 && m) atomicAdd((unsigned long long*)out, (unsigned long long)m);
+    int64_t m = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
+        if (level[v] >= 0) m += beg[v + 1] - beg[v];
+    blockAtomicAdd(m, out);
 }
 
 std::vector<int32_t> hostSources(girhip_graph* h, const int32_t* src, int32_t k) {
@@ -137,15 +137,8 @@ __global__ void kMsFinish(uint64_t* seen, uint64_t* front, uint64_t* next, int32
         for (int s = 0; s < 64; ++s)
             if ((nb >> s) & 1) levels[(size_t)s * V + w] = L + 1;
     }
-    int64_t c = nb != 0, m = nb != 0 ? (int64_t)(fb[w + 1] - fb[w]) : 0;
-    for (int d = WAVE / 2; d > 0; d >>= 1) {
-        c += __shfl_xor(c, d);
-        m += __shfl_xor(m, d);
-    }
-    if (laneId() == 0 && c) {
-        atomicAdd((unsigned long long*)&cnt[0], (unsigned long long)c);
-        atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)m);
-    }
+    blockAtomicAdd(nb != 0, &cnt[0]);
+    blockAtomicAdd(nb != 0 ? (int64_t)(fb[w + 1] - fb[w]) : 0, &cnt[1]);
 }
 
 __global__ void kMsUnreached(const uint64_t* __restrict__ seen, int32_t* __restrict__ levels, int32_t V,
@@ -254,7 +247,7 @@ extern "C" int girhip_bfs(girhip_graph* h, const int32_t* src, int32_t k, int32_
                 const int c = std::min<int32_t>(32, k - i0);
                 fillI64(h, h->dcnt.p + 32, 0, 32);
                 for (int32_t i = i0; i < i0 + c; ++i)
-                    launch(h, "reached_arcs", kReachedArcs, gridFor(V, 256), 256, 0,
+                    launch(h, "reached_arcs", kReachedArcs, boundedGrid(V, 256), 256, 0,
                            (const int32_t*)(out + (size_t)i * V), (const int32_t*)h->fw_beg.p, V,
                            h->dcnt.p + 32 + (i - i0));
                 readCounters(h, h->dcnt.p + 32, c, e.data() + i0);

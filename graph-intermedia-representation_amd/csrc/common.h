@@ -74,7 +74,6 @@ struct Params {
     double scc_color_min = 0;
     double profile = 0;  // 1: time every kernel launch with HIP events
     double msbfs = 1;    // 1: k>1 BFS runs 64 sources per bit-parallel sweep
-    double sssp_batch = 1;  // 1: SSSP sources of a call advance together
 };
 
 struct KStat {
@@ -362,6 +361,22 @@ __device__ __forceinline__ int64_t waveSum(int64_t x) {
     return x;
 }
 
+// Block-wide sum added to *target with one atomic per block.  Every thread
+// of the block must call it (uniform control flow).
+__device__ __forceinline__ void blockAtomicAdd(int64_t x, int64_t* target) {
+    __shared__ int64_t part[1024 / WAVE];
+    x = waveSum(x);
+    const int w = threadIdx.x / WAVE, nw = (blockDim.x + WAVE - 1) / WAVE;
+    if (laneId() == 0) part[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int i = 0; i < nw; ++i) t += part[i];
+        if (t) atomicAdd((unsigned long long*)target, (unsigned long long)t);
+    }
+    __syncthreads();
+}
+
 template <class F>
 __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
     const int64_t n = ldAcq(&H.ctl[0]);
@@ -391,11 +406,16 @@ __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
 // Small utility kernels shared by several translation units (one private
 // copy per unit).
 // cnt += number of v with lab[v] == v (component representatives).
+// (grid-stride; launch with a bounded grid)
 static __global__ void kCountSelf(const int32_t* __restrict__ lab, int32_t V, int64_t* cnt) {
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool root = v < V && lab[v] == (int32_t)v;
-    const uint64_t m = __ballot(root);
-    if (m && laneId() == __ffsll((long long)m) - 1) atomicAdd((unsigned long long*)cnt, (unsigned long long)__popcll(m));
+    int64_t c = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
+        c += lab[v] == (int32_t)v;
+    blockAtomicAdd(c, cnt);
+}
+inline unsigned boundedGrid(int64_t n, int block) {
+    const unsigned g = gridFor(n, block);
+    return g < 2048 ? g : 2048;
 }
 static __global__ void kI64toI32(const int64_t* __restrict__ in, int32_t* __restrict__ out, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -153,15 +153,27 @@ __global__ void kRowKeys(const int32_t* __restrict__ beg, const int32_t* __restr
     });
 }
 
-// Unique flags over sorted keys: flag = first of its run.  Row counts of the
-// unique keys go to deg[row].
-__global__ void kUniqueCount(const uint64_t* __restrict__ keys, int64_t n, int B, int32_t* __restrict__ flag,
-                             int32_t* __restrict__ deg) {
+// Unique flags over sorted keys: flag = first of its run.
+__global__ void kUniqueFlags(const uint64_t* __restrict__ keys, int64_t n, int32_t* __restrict__ flag) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const bool first = i == 0 || keys[i] != keys[i - 1];
-    flag[i] = first;
-    if (first) atomicAdd(&deg[(int32_t)(keys[i] >> B)], 1);
+    if (i < n) flag[i] = i == 0 || keys[i] != keys[i - 1];
+}
+
+// Row offsets of the deduplicated CSR: beg[r] = number of unique keys with a
+// row below r = pos[lower_bound(keys, r << B)] (pos = exclusive scan of the
+// unique flags, pos[n] = total).  No atomics: hub rows cost nothing extra.
+__global__ void kBegFromKeys(const uint64_t* __restrict__ keys, int64_t n, const int64_t* __restrict__ pos, int B,
+                             int32_t V, int32_t* __restrict__ beg) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > V) return;
+    const uint64_t key = (uint64_t)r << B;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t m = lo + ((hi - lo) >> 1);
+        if (keys[m] < key) lo = m + 1;
+        else hi = m;
+    }
+    beg[r] = (int32_t)pos[lo];
 }
 
 __global__ void kCompactAdj(const uint64_t* __restrict__ keys, const int32_t* __restrict__ flag,
@@ -195,6 +207,30 @@ CheckResult checkCsr(girhip_graph* h, const int32_t* beg, const int32_t* adj, in
 
 }  // namespace
 
+// Sorted packed keys (row << B | neighbour) in S.keys[0] -> deduplicated CSR
+// (beg: V+1 entries, adj grown to the unique count).  Returns that count.
+static int64_t keysToCsr(girhip_graph* h, int64_t E, int B, DevBuf<int32_t>& beg, DevBuf<int32_t>& adj) {
+    const int32_t V = h->V;
+    Scratch& S = scratch(h);
+    S.iv[0].ensure(E);
+    launch(h, "csr_unique", kUniqueFlags, gridFor(E, 256), 256, 0, (const uint64_t*)S.keys[0].p, E, S.iv[0].p);
+    S.off2.ensure((size_t)E + 1);
+    DevBuf<int64_t>& pos = S.off2;
+    scanI32toI64(h, S.iv[0].p, pos.p, E);
+    int64_t nE;
+    readCounters(h, pos.p + E, 1, &nE);
+    adj.ensure((size_t)(nE > 0 ? nE : 1));
+    const uint64_t mask = (1ULL << B) - 1;
+    launch(h, "csr_compact", kCompactAdj, gridFor(E, 256), 256, 0, (const uint64_t*)S.keys[0].p,
+           (const int32_t*)S.iv[0].p, (const int64_t*)pos.p, E, mask, adj.p, (const uint32_t*)nullptr,
+           (uint32_t*)nullptr);
+    beg.ensure((size_t)V + 1);
+    launch(h, "csr_beg", kBegFromKeys, gridFor((int64_t)V + 1, 256), 256, 0, (const uint64_t*)S.keys[0].p, E,
+           (const int64_t*)pos.p, B, V, beg.p);
+    syncStream(h);
+    return nE;
+}
+
 // Sort + dedup every row on the device (gir.cpp:95-116) through one global
 // radix sort of packed (row, neighbour) keys.  Self loops are kept, as the
 // reference keeps them.
@@ -207,29 +243,9 @@ void normalizeCsr(girhip_graph* h, DevBuf<int32_t>& beg, DevBuf<int32_t>& adj, i
     launch(h, "norm_keys", kRowKeys, gridFor(V, 256), 256, 0, (const int32_t*)beg.p, (const int32_t*)adj.p, V, B,
            false, S.keys[0].p, (uint32_t*)nullptr);
     radixSort(h, S.keys[0].p, nullptr, E, 2 * B);
-    S.iv[0].ensure(E);
-    S.iv[1].ensure((size_t)V + 1);
-    fillI32(h, S.iv[1].p, 0, (int64_t)V + 1);
-    launch(h, "norm_unique", kUniqueCount, gridFor(E, 256), 256, 0, (const uint64_t*)S.keys[0].p, E, B, S.iv[0].p,
-           S.iv[1].p);
-    S.off2.ensure((size_t)E + 1);
-    DevBuf<int64_t>& pos = S.off2;
-    scanI32toI64(h, S.iv[0].p, pos.p, E);
-    int64_t nE;
-    readCounters(h, pos.p + E, 1, &nE);
     DevBuf<int32_t> nadj;
-    nadj.ensure((size_t)(nE > 0 ? nE : 1));
-    const uint64_t mask = (1ULL << B) - 1;
-    launch(h, "norm_compact", kCompactAdj, gridFor(E, 256), 256, 0, (const uint64_t*)S.keys[0].p,
-           (const int32_t*)S.iv[0].p, (const int64_t*)pos.p, E, mask, nadj.p, (const uint32_t*)nullptr,
-           (uint32_t*)nullptr);
-    // new offsets = exclusive scan of row counts
-    scanI32toI64(h, S.iv[1].p, pos.p, V);
-    launch(h, "norm_beg", kI64toI32, gridFor((int64_t)V + 1, 256), 256, 0, (const int64_t*)pos.p, beg.p,
-           (int64_t)V + 1);
-    syncStream(h);
+    E = keysToCsr(h, E, B, beg, nadj);
     adj.swap(nadj);
-    E = nE;
 }
 
 // bw CSR = transpose of fw: radix sort of packed (neighbour, row) keys.
@@ -248,22 +264,7 @@ void buildTranspose(girhip_graph* h) {
     launch(h, "tr_keys", kRowKeys, gridFor(V, 256), 256, 0, (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p,
            V, B, true, S.keys[0].p, (uint32_t*)nullptr);
     radixSort(h, S.keys[0].p, nullptr, E, 2 * B);
-    S.iv[0].ensure(E);
-    S.iv[1].ensure((size_t)V + 1);
-    fillI32(h, S.iv[1].p, 0, (int64_t)V + 1);
-    launch(h, "tr_unique", kUniqueCount, gridFor(E, 256), 256, 0, (const uint64_t*)S.keys[0].p, E, B, S.iv[0].p,
-           S.iv[1].p);
-    S.off2.ensure((size_t)E + 1);
-    DevBuf<int64_t>& pos = S.off2;
-    scanI32toI64(h, S.iv[0].p, pos.p, E);
-    const uint64_t mask = (1ULL << B) - 1;
-    launch(h, "tr_compact", kCompactAdj, gridFor(E, 256), 256, 0, (const uint64_t*)S.keys[0].p,
-           (const int32_t*)S.iv[0].p, (const int64_t*)pos.p, E, mask, h->bw_adj.p, (const uint32_t*)nullptr,
-           (uint32_t*)nullptr);
-    scanI32toI64(h, S.iv[1].p, pos.p, V);
-    launch(h, "tr_beg", kI64toI32, gridFor((int64_t)V + 1, 256), 256, 0, (const int64_t*)pos.p, h->bw_beg.p,
-           (int64_t)V + 1);
-    syncStream(h);
+    keysToCsr(h, E, B, h->bw_beg, h->bw_adj);
 }
 
 void invalidateLayers(girhip_graph* h) {
@@ -428,7 +429,7 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"block", &Params::block},                 {"wcc_rounds", &Params::wcc_rounds},
                {"scc_trim_rounds", &Params::scc_trim_rounds}, {"scc_pivot", &Params::scc_pivot},
                {"scc_color_min", &Params::scc_color_min}, {"profile", &Params::profile},
-               {"msbfs", &Params::msbfs},                 {"sssp_batch", &Params::sssp_batch}};
+               {"msbfs", &Params::msbfs}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;
@@ -503,6 +504,20 @@ extern "C" int girhip_kernel_stats(girhip_graph* h, const char* name, int64_t* c
         if (ms) *ms = it == h->kstats.end() ? 0 : it->second.ms;
     });
 }
+extern "C" int girhip_kernel_stat_names(girhip_graph* h, char* buf, int64_t cap, int64_t* needed) {
+    return guarded([&] {
+        if (!h) raise(GIRHIP_EINVAL, "kernel_stat_names: null handle");
+        std::string all;
+        for (auto& kv : h->kstats) all += kv.first + "\n";
+        if (needed) *needed = (int64_t)all.size() + 1;
+        if (buf && cap > 0) {
+            const size_t n = std::min<size_t>(all.size(), (size_t)cap - 1);
+            std::memcpy(buf, all.data(), n);
+            buf[n] = 0;
+        }
+    });
+}
+
 extern "C" int girhip_reset_kernel_stats(girhip_graph* h) {
     return guarded([&] {
         if (!h) raise(GIRHIP_EINVAL, "reset_kernel_stats: null handle");

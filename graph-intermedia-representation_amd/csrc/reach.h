@@ -79,9 +79,8 @@ __global__ void kTopDown(const int32_t* __restrict__ qin, int64_t nin, TopDownF<
     const bool act = i < nin;
     const int32_t u = act ? qin[i] : 0;
     const int32_t b = act ? f.beg[u] : 0, e = act ? f.beg[u + 1] : 0;
-    int64_t m = waveRowsH(act, u, 0, 0, b, e, H, f);
-    m = waveSum(m);
-    if (laneId() == 0 && m) atomicAdd((unsigned long long*)&f.cnt[1], (unsigned long long)m);
+    const int64_t m = waveRowsH(act, u, 0, 0, b, e, H, f);
+    blockAtomicAdd(m, &f.cnt[1]);
 }
 
 template <class Flt, class Vis>
@@ -118,8 +117,7 @@ __global__ void kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __res
         on((int32_t)v, L + 1);
         m = pbeg[v + 1] - pbeg[v];
     }
-    for (int d = WAVE / 2; d > 0; d >>= 1) m += __shfl_xor(m, d);
-    if (laneId() == 0 && m) atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)m);
+    blockAtomicAdd(m, &cnt[1]);
 }
 
 static __global__ void kQueueToBits(const int32_t* __restrict__ q, int64_t n, uint64_t* bits) {

@@ -405,7 +405,7 @@ void runScc(girhip_graph* h) {
         n = trimRounds(h, list, alt, n, 1);
     }
     fillI64(h, h->dcnt.p, 0, 1);
-    launch(h, "count_roots", kCountSelf, gridFor(V, blk), blk, 0, (const int32_t*)h->scc.p, V, h->dcnt.p);
+    launch(h, "count_roots", kCountSelf, boundedGrid(V, blk), blk, 0, (const int32_t*)h->scc.p, V, h->dcnt.p);
     int64_t ns;
     readCounters(h, h->dcnt.p, 1, &ns);
     h->n_scc = (int32_t)ns;

@@ -146,7 +146,7 @@ void runWcc(girhip_graph* h) {
     launchHeavy(h, "wcc_hook_rest_heavy", f);
     launch(h, "wcc_compress", kWccCompress, g, blk, 0, comp, V);
     fillI64(h, h->dcnt.p, 0, 1);
-    launch(h, "count_roots", kCountSelf, g, blk, 0, (const int32_t*)comp, V, h->dcnt.p);
+    launch(h, "count_roots", kCountSelf, boundedGrid(V, blk), blk, 0, (const int32_t*)comp, V, h->dcnt.p);
     int64_t n;
     readCounters(h, h->dcnt.p, 1, &n);
     h->n_wcc = (int32_t)n;
@@ -169,7 +169,7 @@ void runWccHookArcs(girhip_graph* h, const uint64_t* keys, int64_t n, int B) {
     launch(h, "wcc_hook_keys", kWccHookKeys, gridFor(n, blk), blk, 0, keys, n, B, comp);
     launch(h, "wcc_compress", kWccCompress, gridFor(V, blk), blk, 0, comp, V);
     fillI64(h, h->dcnt.p, 0, 1);
-    launch(h, "count_roots", kCountSelf, gridFor(V, blk), blk, 0, (const int32_t*)comp, V, h->dcnt.p);
+    launch(h, "count_roots", kCountSelf, boundedGrid(V, blk), blk, 0, (const int32_t*)comp, V, h->dcnt.p);
     int64_t c;
     readCounters(h, h->dcnt.p, 1, &c);
     h->n_wcc = (int32_t)c;

@@ -72,6 +72,7 @@ _SIGS = {
     "girhip_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_double)]),
     "girhip_reset_kernel_stats": (ctypes.c_int, [_vp]),
+    "girhip_kernel_stat_names": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
 }
 
 _GEN_SIGS = {
@@ -362,6 +363,15 @@ class Graph:
 
     def reset_kernel_stats(self):
         _check(lib().girhip_reset_kernel_stats(self._h))
+
+    def all_kernel_stats(self):
+        """{kernel name: (launches, ms)} recorded in profile mode."""
+        need = ctypes.c_int64()
+        _check(lib().girhip_kernel_stat_names(self._h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _check(lib().girhip_kernel_stat_names(self._h, buf, need.value, None))
+        names = [n for n in buf.value.decode().split("\n") if n]
+        return {n: self.kernel_stats(n) for n in names}
 
 
 # ---------------------------------------------------------------------------

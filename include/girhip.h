@@ -171,6 +171,9 @@ int girhip_last_ms(const girhip_graph* h, const char* kind, double* ms);
  * timing is std::chrono around whole calls, gir.cpp:800-811). */
 int girhip_kernel_stats(girhip_graph* h, const char* name, int64_t* count, double* ms);
 int girhip_reset_kernel_stats(girhip_graph* h);
+/* Names with statistics, newline separated, into buf (cap bytes, always
+ * NUL-terminated); *needed (may be NULL) receives the full length + 1. */
+int girhip_kernel_stat_names(girhip_graph* h, char* buf, int64_t cap, int64_t* needed);
 
 #ifdef __cplusplus
 }

@@ -136,7 +136,7 @@ def test_tuning_knobs_do_not_change_results():
     src = girhip.sources(fb, 4, 1)
     with girhip.Graph(fb, fa, bb, ba) as G:
         base = (G.wcc()[0], G.scc_dag()[0], G.bfs(src), G.sssp(src, 16))
-        for k, v in [("msbfs", 0), ("sssp_batch", 0), ("block", 64), ("block", 1024), ("bfs_alpha", 1), ("bfs_beta", 1000),
+        for k, v in [("msbfs", 0), ("block", 64), ("block", 1024), ("bfs_alpha", 1), ("bfs_beta", 1000),
                      ("bfs_min_reach", 0),
                      ("wcc_rounds", 0), ("scc_trim_rounds", 0), ("scc_pivot", 0), ("sssp_delta", 1),
                      ("sssp_delta", 100000)]:
