@@ -322,6 +322,7 @@ void applyStore(girhip_graph* h, const int32_t* u, const int32_t* v, const uint8
     mergeCsr(h, h->fw_beg, h->fw_adj, h->has_w ? &h->w_fw : nullptr, C, B, newE);
     mergeCsr(h, h->bw_beg, h->bw_adj, nullptr, T, B, newE);
     h->E = newE;
+    h->src_ok = false;
 }
 
 // Bring every requested layer up to date.  C = the changes just applied to a

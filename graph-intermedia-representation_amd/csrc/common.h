@@ -93,7 +93,7 @@ struct Scratch {
     DevBuf<uint8_t> stage;
     DevBuf<uint64_t> ms[3];  // bit-parallel BFS: seen / frontier / next words
     // heavy-row list
-    DevBuf<int64_t> hv_ctl, hv_b, hv_e;
+    DevBuf<int64_t> hv_ctl, hv_b, hv_e, hv_off;
     DevBuf<int32_t> hv_row, hv_aux;
     DevBuf<uint64_t> hv_pay;
     size_t hv_cap = 0;
@@ -109,6 +109,8 @@ struct girhip_graph {
     gir::DevBuf<int32_t> fw_beg, fw_adj, bw_beg, bw_adj;
     gir::DevBuf<uint32_t> w_fw;  // weights per fw slot (valid if has_w)
     bool has_w = false;
+    gir::DevBuf<int32_t> fw_src;  // row of every fw slot (COO), built on demand
+    bool src_ok = false;
 
     // layers
     bool wcc_req = false, scc_req = false;  // built at least once (repair on update)
@@ -225,6 +227,8 @@ inline int bitsFor(uint64_t x) {
 void normalizeCsr(girhip_graph* h, DevBuf<int32_t>& beg, DevBuf<int32_t>& adj, int64_t& E);
 void buildTranspose(girhip_graph* h);
 void invalidateLayers(girhip_graph* h);
+// The row of every forward slot (built once per edge-store version).
+const int32_t* fwSrc(girhip_graph* h);
 
 // ---- layers ----
 void runWcc(girhip_graph* h);
@@ -323,6 +327,7 @@ struct HeavyList {
     int64_t* b;
     int64_t* e;
     uint64_t* pay;
+    int64_t* off;  // exclusive prefix of the listed row lengths (count + 1)
 };
 
 template <int SMALL = 16, class F>
@@ -377,16 +382,84 @@ __device__ __forceinline__ void blockAtomicAdd(int64_t x, int64_t* target) {
     __syncthreads();
 }
 
+// Block-wide exclusive scan of n per thread; returns this thread's offset in
+// the output and reserves the block's range with one atomicAdd on *cnt (the
+// LDS-staged alternative to one atomic per wave).  Every thread of the block
+// must call it.
+__device__ __forceinline__ int64_t blockReserve(int n, int64_t* cnt) {
+    __shared__ int64_t wtot[1024 / WAVE];
+    __shared__ int64_t base;
+    const int lane = laneId(), w = threadIdx.x / WAVE, nw = (blockDim.x + WAVE - 1) / WAVE;
+    int64_t v = n;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int64_t o = __shfl_up(v, d);
+        if (lane >= d) v += o;
+    }
+    if (lane == WAVE - 1) wtot[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int i = 0; i < nw; ++i) {
+            const int64_t x = wtot[i];
+            wtot[i] = t;
+            t += x;
+        }
+        base = t ? (int64_t)atomicAdd((unsigned long long*)cnt, (unsigned long long)t) : 0;
+    }
+    __syncthreads();
+    const int64_t off = base + wtot[w] + v - n;
+    __syncthreads();
+    return off;
+}
+
+// One block: off[k] = sum of the lengths of listed rows before k.
+static __global__ void kHeavyPrefix(HeavyList H) {
+    __shared__ int64_t wtot[1024 / WAVE];
+    const int64_t n = ldAcq(&H.ctl[0]);
+    const int lane = laneId(), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    int64_t carry = 0;
+    for (int64_t base = 0; base < n; base += blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        const int64_t len = i < n ? H.e[i] - H.b[i] : 0;
+        int64_t v = len;
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int64_t o = __shfl_up(v, d);
+            if (lane >= d) v += o;
+        }
+        if (lane == WAVE - 1) wtot[w] = v;
+        __syncthreads();
+        int64_t pre = 0, tot = 0;
+        for (int k = 0; k < nw; ++k) {
+            if (k < w) pre += wtot[k];
+            tot += wtot[k];
+        }
+        if (i < n) H.off[i] = carry + pre + v - len;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) H.off[n] = carry;
+}
+
+// The listed rows' arcs as one range [0, off[n]): every thread strides over
+// it and finds the owning row by binary search in off.
 template <class F>
 __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
     const int64_t n = ldAcq(&H.ctl[0]);
+    const int64_t T = n ? H.off[n] : 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     int64_t acc = 0;
-    for (int64_t k = 0; k < n; ++k) {
-        const int32_t v = H.row[k], aux = H.aux[k];
-        const int64_t b = H.b[k], e = H.e[k];
-        const uint64_t pay = H.pay[k];
-        for (int64_t j = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < e; j += stride) acc += f(v, pay, aux, j);
+    int64_t k = 0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < T; p += stride) {
+        if (!(p >= H.off[k] && p < H.off[k + 1])) {
+            int64_t lo = 0, hi = n - 1;  // last k with off[k] <= p
+            while (lo < hi) {
+                const int64_t m = (lo + hi + 1) >> 1;
+                if (H.off[m] <= p) lo = m;
+                else hi = m - 1;
+            }
+            k = lo;
+        }
+        acc += f(H.row[k], H.pay[k], H.aux[k], H.b[k] + (p - H.off[k]));
     }
     if (sum) {
         acc = waveSum(acc);
@@ -441,6 +514,8 @@ HeavyList heavyList(girhip_graph* h);
 // accumulator for the functor's contributions).
 template <class F>
 void launchHeavy(girhip_graph* h, const char* name, const F& f, int64_t* sum = nullptr) {
-    launch(h, name, kHeavyRows<F>, 512, 256, 0, heavyList(h), f, sum);
+    const HeavyList H = heavyList(h);
+    launch(h, "heavy_prefix", kHeavyPrefix, 1, 1024, 0, H);
+    launch(h, name, kHeavyRows<F>, 1024, 256, 0, H, f, sum);
 }
 }  // namespace gir

@@ -92,10 +92,11 @@ HeavyList heavyList(girhip_graph* h) {
         S.hv_b.ensure(cap);
         S.hv_e.ensure(cap);
         S.hv_pay.ensure(cap);
+        S.hv_off.ensure(cap + 1);
         fillI64(h, S.hv_ctl.p, 0, 2);
         S.hv_cap = cap;
     }
-    return HeavyList{S.hv_ctl.p, S.hv_row.p, S.hv_aux.p, S.hv_b.p, S.hv_e.p, S.hv_pay.p};
+    return HeavyList{S.hv_ctl.p, S.hv_row.p, S.hv_aux.p, S.hv_b.p, S.hv_e.p, S.hv_pay.p, S.hv_off.p};
 }
 
 void readCounters(girhip_graph* h, const int64_t* dev, int n, int64_t* out) {
@@ -270,6 +271,23 @@ void buildTranspose(girhip_graph* h) {
 void invalidateLayers(girhip_graph* h) {
     h->wcc_ok = false;
     h->scc_ok = false;
+}
+
+__global__ void kRowOfSlot(const int32_t* __restrict__ beg, int32_t V, int32_t* __restrict__ src) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = v < V;
+    const int32_t b = act ? beg[v] : 0, e = act ? beg[v + 1] : 0;
+    waveRows(act, (int32_t)v, b, e, [&](int32_t u, int64_t j) { src[j] = u; });
+}
+
+const int32_t* fwSrc(girhip_graph* h) {
+    if (!h->src_ok) {
+        h->fw_src.ensure((size_t)(h->E > 0 ? h->E : 1));
+        launch(h, "row_of_slot", kRowOfSlot, gridFor(h->V, 256), 256, 0, (const int32_t*)h->fw_beg.p, h->V,
+               h->fw_src.p);
+        h->src_ok = true;
+    }
+    return h->fw_src.p;
 }
 
 }  // namespace gir

@@ -52,7 +52,7 @@ __global__ void kCompactActive(const int32_t* __restrict__ in, int64_t n, bool a
         v = all ? (int32_t)i : in[i];
         keep = scc[v] == -1;
     }
-    const int64_t pos = wavePush(cnt, keep);
+    const int64_t pos = blockReserve(keep, cnt);
     if (keep) out[pos] = v;
 }
 
@@ -176,7 +176,7 @@ __global__ void kColorRoots(const int32_t* __restrict__ list, int64_t n, const i
         v = list[i];
         root = color[v] == v;
     }
-    const int64_t pos = wavePush(cnt, root);
+    const int64_t pos = blockReserve(root, cnt);
     if (root) out[pos] = v;
 }
 
@@ -189,30 +189,56 @@ __global__ void kColorAssign(const int32_t* __restrict__ list, int64_t n, const 
 }
 
 // ---- DAG ----
-// Crossing arc slot j of row u -> packed key (scc[u], scc[head]); the row's
-// label travels as the payload.
-struct DagKeysF {
-    const int32_t* fa;
-    const int32_t* scc;
-    int B;
-    uint64_t* keys;
-    int64_t* cnt;
-    __device__ int64_t operator()(int32_t, uint64_t su, int32_t, int64_t j) const {
-        const int32_t a = (int32_t)su;
-        const int32_t c = scc[fa[j]];
-        const bool cross = a != c;
-        const int64_t pos = wavePush(cnt, cross);
-        if (cross) keys[pos] = ((uint64_t)(uint32_t)a << B) | (uint32_t)c;
-        return 0;
+// Edge-parallel: DK_ITEMS coalesced slots per thread and block step, one
+// reservation per block step for the crossing arcs' packed keys.
+constexpr int DK_ITEMS = 16;
+__global__ void kDagKeysEdge(const int32_t* __restrict__ src, const int32_t* __restrict__ adj, int64_t E,
+                             const int32_t* __restrict__ scc, int B, uint64_t* __restrict__ keys, int64_t* cnt) {
+    const int64_t span = (int64_t)blockDim.x * DK_ITEMS;
+    for (int64_t base = (int64_t)blockIdx.x * span; base < E; base += (int64_t)gridDim.x * span) {
+        uint64_t kv[DK_ITEMS];
+        bool cr[DK_ITEMS];
+        int n = 0;
+#pragma unroll
+        for (int t = 0; t < DK_ITEMS; ++t) {
+            const int64_t e = base + (int64_t)t * blockDim.x + threadIdx.x;
+            cr[t] = false;
+            kv[t] = 0;
+            if (e < E) {
+                const int32_t a = scc[src[e]], c = scc[adj[e]];
+                cr[t] = a != c;
+                kv[t] = ((uint64_t)(uint32_t)a << B) | (uint32_t)c;
+            }
+            n += cr[t];
+        }
+        int64_t pos = blockReserve(n, cnt);
+#pragma unroll
+        for (int t = 0; t < DK_ITEMS; ++t)
+            if (cr[t]) keys[pos++] = kv[t];
     }
-};
+}
 
-__global__ void kDagKeys(const int32_t* __restrict__ fb, int32_t V, DagKeysF f, HeavyList H) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = v < V;
-    const int32_t b = act ? fb[v] : 0, e = act ? fb[v + 1] : 0;
-    const uint64_t su = act ? (uint64_t)(uint32_t)f.scc[v] : 0;
-    waveRowsH(act, (int32_t)v, su, 0, b, e, H, f);
+__global__ void kDagTKeys(const int32_t* __restrict__ src, const int32_t* __restrict__ dst, int64_t nd, int B,
+                          uint64_t* __restrict__ tk) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nd) tk[r] = ((uint64_t)(uint32_t)dst[r] << B) | (uint32_t)src[r];
+}
+
+// In-degree of every SCC id from the arcs sorted by destination.
+__global__ void kIndegFromKeys(const uint64_t* __restrict__ tk, int64_t nd, int B, int32_t V,
+                               int32_t* __restrict__ indeg) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= V) return;
+    auto lb = [&](uint64_t key) {
+        int64_t lo = 0, hi = nd;
+        while (lo < hi) {
+            const int64_t m = lo + ((hi - lo) >> 1);
+            if (tk[m] < key) lo = m + 1;
+            else hi = m;
+        }
+        return lo;
+    };
+    indeg[t] = (int32_t)(lb((uint64_t)(t + 1) << B) - lb((uint64_t)t << B));
 }
 
 __global__ void kRunFlags(const uint64_t* __restrict__ keys, int64_t n, int32_t* __restrict__ flag) {
@@ -228,7 +254,7 @@ __global__ void kRunStarts(const int32_t* __restrict__ flag, const int64_t* __re
 
 __global__ void kDagWrite(const uint64_t* __restrict__ keys, const int32_t* __restrict__ start, int64_t nd,
                           int64_t ncross, int B, int32_t* __restrict__ src, int32_t* __restrict__ dst,
-                          int32_t* __restrict__ mult, int32_t* __restrict__ indeg) {
+                          int32_t* __restrict__ mult) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nd) return;
     const int64_t i = start[r], j = r + 1 < nd ? start[r + 1] : ncross;
@@ -237,7 +263,6 @@ __global__ void kDagWrite(const uint64_t* __restrict__ keys, const int32_t* __re
     src[r] = a;
     dst[r] = b;
     mult[r] = (int32_t)(j - i);
-    atomicAdd(&indeg[b], 1);
 }
 
 // DAG CSR offsets: arcs are sorted by source, so beg[s] = lower_bound(src, s).
@@ -262,7 +287,7 @@ __global__ void kRankSeeds(const int32_t* __restrict__ scc, const int32_t* __res
         rank[v] = -1;
         seed = scc[v] == (int32_t)v && indeg[v] == 0;
     }
-    const int64_t pos = wavePush(cnt, seed);
+    const int64_t pos = blockReserve(seed, cnt);
     if (seed) out[pos] = (int32_t)v;
 }
 
@@ -425,9 +450,10 @@ void runDag(girhip_graph* h) {
     if (h->E > 0) {
         S.keys[0].ensure((size_t)h->E);
         fillI64(h, h->dcnt.p + 9, 0, 1);
-        const DagKeysF f{h->fw_adj.p, h->scc.p, B, S.keys[0].p, h->dcnt.p + 9};
-        launch(h, "dag_keys", kDagKeys, gridFor(V, blk), blk, 0, (const int32_t*)h->fw_beg.p, V, f, heavyList(h));
-        launchHeavy(h, "dag_keys_heavy", f);
+        const int32_t* src = fwSrc(h);
+        const unsigned g = std::min<unsigned>(gridFor(h->E, blk * DK_ITEMS), 8192);
+        launch(h, "dag_keys", kDagKeysEdge, g, blk, 0, src, (const int32_t*)h->fw_adj.p, h->E,
+               (const int32_t*)h->scc.p, B, S.keys[0].p, h->dcnt.p + 9);
         readCounters(h, h->dcnt.p + 9, 1, &ncross);
     }
     int64_t nd = 0;
@@ -446,7 +472,12 @@ void runDag(girhip_graph* h) {
         h->dag_dst.ensure((size_t)nd);
         h->dag_mult.ensure((size_t)nd);
         launch(h, "dag_write", kDagWrite, gridFor(nd, blk), blk, 0, (const uint64_t*)S.keys[0].p,
-               (const int32_t*)S.q[2].p, nd, ncross, B, h->dag_src.p, h->dag_dst.p, h->dag_mult.p, ideg);
+               (const int32_t*)S.q[2].p, nd, ncross, B, h->dag_src.p, h->dag_dst.p, h->dag_mult.p);
+        // in-degrees from the arcs sorted by destination (no contended atomics)
+        launch(h, "dag_tkeys", kDagTKeys, gridFor(nd, blk), blk, 0, (const int32_t*)h->dag_src.p,
+               (const int32_t*)h->dag_dst.p, nd, B, S.keys[0].p);
+        radixSort(h, S.keys[0].p, nullptr, nd, 2 * B);
+        launch(h, "dag_indeg", kIndegFromKeys, gridFor(V, blk), blk, 0, (const uint64_t*)S.keys[0].p, nd, B, V, ideg);
     }
     h->n_dag = nd;
     // DAG CSR offsets by source SCC (arcs are sorted by src)

@@ -40,13 +40,16 @@ enum : int {
     C_LO = 3 * SB,    // bucket [lo, hi) (uint64)
     C_HI = 4 * SB,
     C_FMIN = 5 * SB,  // smallest far distance >= hi (uint64)
-    C_SLOTS = 6 * SB
+    C_HQ = 6 * SB,    // heavy-phase queue length
+    C_SLOTS = 7 * SB
 };
 
 struct SsspBatch {
     int32_t* qn[2];      // [SB][V] queues by parity
+    int32_t* qh;         // [SB][V] vertices whose heavy arcs are due
     uint64_t* nearbits;  // [SB][W]
     uint64_t* farbits;   // [SB][W]
+    uint64_t* donebits;  // [SB][W] processed in the current bucket
     int64_t* ctl;        // C_SLOTS
     uint64_t* dist;      // [SB][V] output rows
     int64_t V, W;
@@ -69,14 +72,28 @@ __global__ void kBInit(SsspBatch B, const int32_t* __restrict__ src, int k, uint
 }
 
 // Relaxation of arc slot j of row p (owner distance dp) for source s.
+// heavy = 0: only light arcs (w <= delta, the bucket's inner rounds);
+// heavy = 1: only heavy arcs (w > delta, once per vertex and bucket).
 struct RelaxF {
     SsspBatch B;
     const int32_t* fa;
     const uint32_t* w;
+    uint64_t delta;
+    int heavy;
     __device__ int64_t operator()(int32_t p, uint64_t dp, int32_t s, int64_t j) const {
         uint64_t* d = B.dist + (size_t)s * B.V;
-        const int32_t x = fa[j];
-        const uint64_t nd = dp + (w ? (uint64_t)w[j] : (uint64_t)hashWeight(p, x));
+        uint64_t wt;
+        int32_t x;
+        if (w) {
+            wt = w[j];
+            if ((wt > delta) != (heavy != 0)) return 0;
+            x = fa[j];
+        } else {
+            x = fa[j];
+            wt = hashWeight(p, x);
+            if ((wt > delta) != (heavy != 0)) return 0;
+        }
+        const uint64_t nd = dp + wt;
         if (nd < ldAcq(&d[x])) {
             const uint64_t old = atomicMin((unsigned long long*)&d[x], (unsigned long long)nd);
             if (nd < old) {
@@ -100,6 +117,7 @@ __global__ void kBRelax(RelaxF f, int par, const int32_t* __restrict__ fb, Heavy
         c[(par ? C_NEAR0 : C_NEAR1) + s] = 0;
         c[C_FMIN + s] = (int64_t)INF;
         c[C_ADV + s] = 0;
+        c[C_HQ + s] = 0;
     }
     const int64_t nin = c[(par ? C_NEAR1 : C_NEAR0) + s];
     if ((int64_t)blockIdx.x * blockDim.x >= nin) return;  // whole block idle for this source
@@ -107,37 +125,50 @@ __global__ void kBRelax(RelaxF f, int par, const int32_t* __restrict__ fb, Heavy
     const bool act = i < nin;
     const int32_t u = act ? B.qn[par][(size_t)s * B.V + i] : 0;
     const uint64_t du = act ? ldAcq(&B.dist[(size_t)s * B.V + u]) : 0;
+    if (act) {  // its heavy arcs are due when the bucket drains
+        uint64_t* wp = &B.donebits[(size_t)s * B.W + (u >> 6)];
+        const uint64_t bit = 1ULL << (u & 63);
+        if (!(ldAcq(wp) & bit)) atomicOr((unsigned long long*)wp, (unsigned long long)bit);
+    }
     const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
     waveRowsH(act, u, du, s, b, e, H, f);
 }
 
-// Block-wide exclusive scan of n (<= 64 per thread); returns this thread's
-// offset into the block's output and reserves the block's range with one
-// atomicAdd on *cnt.  Every thread of the block must call it.
-__device__ __forceinline__ int64_t blockReserve(int n, int64_t* cnt) {
-    __shared__ int64_t wtot[1024 / WAVE];
-    __shared__ int64_t base;
-    const int lane = laneId(), w = threadIdx.x / WAVE, nw = (blockDim.x + WAVE - 1) / WAVE;
-    int64_t v = n;
-    for (int d = 1; d < WAVE; d <<= 1) {
-        const int64_t o = __shfl_up(v, d);
-        if (lane >= d) v += o;
+// Sources whose bucket drained (next queue empty): the vertices processed
+// in it -> heavy-phase queue (bits cleared).
+__global__ void kBCompactDone(SsspBatch B, int par) {
+    const int s = blockIdx.y;
+    if (B.ctl[(par ? C_NEAR0 : C_NEAR1) + s] != 0) return;
+    const int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t m = 0;
+    if (wi < B.W) {
+        uint64_t* wp = B.donebits + (size_t)s * B.W + wi;
+        m = *wp;
+        if (m) *wp = 0;
     }
-    if (lane == WAVE - 1) wtot[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t t = 0;
-        for (int i = 0; i < nw; ++i) {
-            const int64_t x = wtot[i];
-            wtot[i] = t;
-            t += x;
-        }
-        base = t ? (int64_t)atomicAdd((unsigned long long*)cnt, (unsigned long long)t) : 0;
+    int64_t pos = blockReserve(__popcll(m), B.ctl + C_HQ + s);
+    int32_t* q = B.qh + (size_t)s * B.V;
+    while (m) {
+        const int b = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        q[pos++] = (int32_t)(wi * 64 + b);
     }
-    __syncthreads();
-    const int64_t off = base + wtot[w] + v - n;
-    __syncthreads();
-    return off;
+}
+
+// Heavy arcs of the heavy-phase queue (grid stride, uniform per wave).
+__global__ void kBRelaxHeavy(RelaxF f, const int32_t* __restrict__ fb, HeavyList H) {
+    const int s = blockIdx.y;
+    const SsspBatch& B = f.B;
+    const int64_t n = B.ctl[C_HQ + s];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+        const int64_t i = base + laneId();
+        const bool act = i < n;
+        const int32_t u = act ? B.qh[(size_t)s * B.V + i] : 0;
+        const uint64_t du = act ? ldAcq(&B.dist[(size_t)s * B.V + u]) : 0;
+        const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+        waveRowsH(act, u, du, s, b, e, H, f);
+    }
 }
 
 // Near bitmap of every source -> its next queue (bits cleared).
@@ -237,8 +268,8 @@ __global__ void kReachedArcsD(const uint64_t* __restrict__ dist, const int32_t* 
 }
 
 struct SsspBufs {
-    DevBuf<int32_t> qn[2];
-    DevBuf<uint64_t> nearbits, farbits;
+    DevBuf<int32_t> qn[2], qh;
+    DevBuf<uint64_t> nearbits, farbits, donebits;
     DevBuf<int64_t> ctl;
 };
 
@@ -251,25 +282,33 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
     B.V = V;
     B.W = ((int64_t)V + 63) / 64;
     for (int p = 0; p < 2; ++p) B.qn[p] = Bf.qn[p].p;
+    B.qh = Bf.qh.p;
     B.nearbits = Bf.nearbits.p;
     B.farbits = Bf.farbits.p;
+    B.donebits = Bf.donebits.p;
     B.ctl = Bf.ctl.p;
     B.dist = dist;
     fillU64(h, dist, INF, (int64_t)k * V);
     fillU64(h, B.nearbits, 0, (int64_t)k * B.W);
     fillU64(h, B.farbits, 0, (int64_t)k * B.W);
+    fillU64(h, B.donebits, 0, (int64_t)k * B.W);
     launch(h, "sssp_init", kBInit, 1, 64, 0, B, dsrc, k, delta);
-    const RelaxF f{B, h->fw_adj.p, w};
+    const RelaxF fl{B, h->fw_adj.p, w, delta, 0}, fh{B, h->fw_adj.p, w, delta, 1};
     const unsigned wgrid = gridFor(B.W, blk);
     const unsigned fgrid = std::min<unsigned>(wgrid, 256);
+    const unsigned hgrid = std::min<unsigned>(gridFor(V, blk), 1024);
     int par = 0;
     std::vector<int64_t> r(SB);
     int64_t maxn = 1;
     while (true) {
-        launch(h, "sssp_relax", kBRelax, dim3(gridFor(maxn, blk), k), blk, 0, f, par, (const int32_t*)h->fw_beg.p,
+        launch(h, "sssp_relax", kBRelax, dim3(gridFor(maxn, blk), k), blk, 0, fl, par, (const int32_t*)h->fw_beg.p,
                heavyList(h));
-        launchHeavy(h, "sssp_relax_heavy", f);
+        launchHeavy(h, "sssp_relax_rows", fl);
         launch(h, "sssp_compact", kBCompact, dim3(wgrid, k), blk, 0, B, par);
+        launch(h, "sssp_compact_done", kBCompactDone, dim3(wgrid, k), blk, 0, B, par);
+        launch(h, "sssp_relax_heavy", kBRelaxHeavy, dim3(hgrid, k), blk, 0, fh, (const int32_t*)h->fw_beg.p,
+               heavyList(h));
+        launchHeavy(h, "sssp_relax_heavy_rows", fh);
         launch(h, "sssp_farmin", kBFarMin, dim3(fgrid, k), blk, 0, B, par);
         launch(h, "sssp_advance", kBAdvance, 1, 64, 0, B, par, k, delta);
         launch(h, "sssp_split", kBSplit, dim3(wgrid, k), blk, 0, B, par);
@@ -306,8 +345,10 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         const int kc = std::max(1, std::min<int32_t>(k, SB));
         SsspBufs Bf;
         for (int p = 0; p < 2; ++p) Bf.qn[p].ensure((size_t)kc * Vs);
+        Bf.qh.ensure((size_t)kc * Vs);
         Bf.nearbits.ensure((size_t)kc * Ws);
         Bf.farbits.ensure((size_t)kc * Ws);
+        Bf.donebits.ensure((size_t)kc * Ws);
         Bf.ctl.ensure(C_SLOTS);
         DevBuf<int32_t> dsrc;
         dsrc.ensure((size_t)(k > 0 ? k : 1));
