@@ -73,6 +73,19 @@ _SIGS = {
                                            ctypes.POINTER(ctypes.c_double)]),
     "girhip_reset_kernel_stats": (ctypes.c_int, [_vp]),
     "girhip_kernel_stat_names": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
+    "girhip_features": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "girhip_autotune": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_double), _vp]),
+    "girhip_mlp_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "girhip_mlp_destroy": (ctypes.c_int, [_vp]),
+    "girhip_mlp_num_params": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
+    "girhip_mlp_get_params": (ctypes.c_int, [_vp, _vp]),
+    "girhip_mlp_set_params": (ctypes.c_int, [_vp, _vp]),
+    "girhip_mlp_forward": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp]),
+    "girhip_mlp_grad": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, _vp, ctypes.POINTER(ctypes.c_float)]),
+    "girhip_mlp_train": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                        ctypes.POINTER(ctypes.c_float)]),
+    "girhip_learned_tune": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
 }
 
 _GEN_SIGS = {
@@ -364,6 +377,34 @@ class Graph:
     def reset_kernel_stats(self):
         _check(lib().girhip_reset_kernel_stats(self._h))
 
+    # ---- tuning ----
+    def features(self, k):
+        """16 graph/query features (see csrc/tune.hip) for k query sources."""
+        f = np.empty(16, np.float32)
+        _check(lib().girhip_features(self._h, int(k), f.ctypes.data))
+        return f
+
+    def autotune(self, what, sources, candidates, reps=1):
+        """Time every candidate value of knob `what`, keep the fastest ->
+        (best value, per-candidate ms)."""
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
+        cand = np.ascontiguousarray(np.asarray(candidates, dtype=np.float64))
+        times = np.empty(len(cand), np.float64)
+        best = ctypes.c_double()
+        _check(lib().girhip_autotune(self._h, what.encode(), src.ctypes.data if len(src) else None, len(src),
+                                     cand.ctypes.data, len(cand), int(reps), ctypes.byref(best), times.ctypes.data))
+        return best.value, times
+
+    def learned_tune(self, model, norm, sources, judge=0):
+        """Set delta and SCC trim rounds from the learned model (defaults as
+        fallback / judge) -> dict(delta, trim, raw_delta, used)."""
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
+        norm = np.ascontiguousarray(np.asarray(norm, dtype=np.float32))
+        out = np.zeros(4, np.float64)
+        _check(lib().girhip_learned_tune(self._h, model._m, norm.ctypes.data, src.ctypes.data if len(src) else None,
+                                         len(src), int(judge), out.ctypes.data))
+        return {"delta": out[0], "trim": out[1], "raw_delta": out[2], "used": bool(out[3])}
+
     def all_kernel_stats(self):
         """{kernel name: (launches, ms)} recorded in profile mode."""
         need = ctypes.c_int64()
@@ -372,6 +413,65 @@ class Graph:
         _check(lib().girhip_kernel_stat_names(self._h, buf, need.value, None))
         names = [n for n in buf.value.decode().split("\n") if n]
         return {n: self.kernel_stats(n) for n in names}
+
+
+class Mlp:
+    """The learned-tuning MLP (16 -> 64 -> 64 -> 2) on the GPU (girhip_mlp)."""
+
+    NI, NH, NO = 16, 64, 2
+
+    def __init__(self, seed=1):
+        m = _vp()
+        _check(lib().girhip_mlp_create(int(seed), ctypes.byref(m)))
+        self._m = m
+        n = ctypes.c_int32()
+        _check(lib().girhip_mlp_num_params(ctypes.byref(n)))
+        self.num_params = n.value
+
+    def close(self):
+        if getattr(self, "_m", None):
+            lib().girhip_mlp_destroy(self._m)
+            self._m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = np.empty(self.num_params, np.float32)
+        _check(lib().girhip_mlp_get_params(self._m, p.ctypes.data))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        _check(lib().girhip_mlp_set_params(self._m, p.ctypes.data))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.NI)
+        y = np.empty((len(x), self.NO), np.float32)
+        _check(lib().girhip_mlp_forward(self._m, x.ctypes.data, len(x), y.ctypes.data))
+        return y
+
+    def grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.NI)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, self.NO)
+        g = np.empty(self.num_params, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().girhip_mlp_grad(self._m, x.ctypes.data, t.ctypes.data, len(x), g.ctypes.data, ctypes.byref(loss)))
+        return loss.value, g
+
+    def train(self, x, t, epochs, lr=1e-3):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.NI)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, self.NO)
+        loss = ctypes.c_float()
+        _check(lib().girhip_mlp_train(self._m, x.ctypes.data, t.ctypes.data, len(x), int(epochs), float(lr),
+                                      ctypes.byref(loss)))
+        return loss.value
 
 
 # ---------------------------------------------------------------------------

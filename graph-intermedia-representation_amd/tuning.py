@@ -1,0 +1,202 @@
+"""Learned tuning of the GIR path: record collection, training, model files.
+
+The method generates its own training data: for each graph of a family set
+(RMAT with varied skew, percolated 2D lattices, Erdos-Renyi graphs) and a few
+query-batch sizes k, the GPU times SSSP for a ladder of bucket widths delta
+and buildSCC_DAG for a ladder of trim-round counts; each timing is one
+record (features, knob, value, ms).  The MLP (16 -> 64 -> 64 -> 2, trained
+on the GPU by the library's own kernels) learns
+    y0 = log2(fastest delta)          for (graph, k)
+    y1 = log2(fastest trim rounds + 1) for the graph.
+At run time girhip_learned_tune applies it with the reference's defaults as
+fallback and judge.  Knobs only change work order; results never depend on
+them (tests/test_gpu_tuning.py checks that).
+
+    python tuning.py collect OUT.npz [--graphs N] [--seed S]   (GPU)
+    python tuning.py train RECORDS.npz [...] --out MODEL.npz  (GPU)
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import girhip  # noqa: E402
+
+DELTAS = [2.0 ** i for i in range(0, 17)]  # 1 .. 65536
+TRIMS = [0, 1, 2, 3, 4, 6, 8]
+KS = [1, 8, 32]
+MODEL_PATH = os.path.join(HERE, "tuned_mlp.npz")
+
+
+def log(*a):
+    print("[tuning]", *a, file=sys.stderr, flush=True)
+
+
+def family_graph(rng):
+    """One training graph drawn from the families (name, V, fb, fa, bb, ba,
+    weight spec)."""
+    kind = rng.choice(["rmat", "rmat", "grid", "er"])
+    if kind == "rmat":
+        scale = int(rng.integers(14, 21))
+        a = float(rng.uniform(0.45, 0.65))
+        d = 0.05
+        b = c = (1 - a - d) / 2
+        ef = int(rng.choice([8, 16, 16, 32]))
+        g = girhip.rmat_graph(scale, ef, a, b, c, seed=int(rng.integers(1 << 30)))
+        name = f"rmat{scale}_a{a:.2f}_ef{ef}"
+    elif kind == "grid":
+        side = int(rng.integers(100, 1001))
+        pk = float(rng.uniform(0.7, 0.95))
+        pb = float(rng.uniform(0.3, 0.9))
+        g = girhip.grid_graph(side, side, pk, pb, seed=int(rng.integers(1 << 30)))
+        name = f"grid{side}_k{pk:.2f}_b{pb:.2f}"
+    else:
+        V = int(2 ** rng.integers(14, 20))
+        deg = float(rng.uniform(4, 32))
+        s, dd = girhip.uniform_edges(V, int(V * deg), seed=int(rng.integers(1 << 30)))
+        g = (V,) + girhip.csr_from_edges(V, s, dd)
+        name = f"er{V}_d{deg:.1f}"
+    wmax = int(rng.choice([0, 16, 64, 256, 1000, 10000]))  # 0: the reference's hash weights
+    return name, g, wmax
+
+
+def time_sssp(G, src, delta, reps=1):
+    best = 1e300
+    for _ in range(reps):
+        G.sssp(src, delta)
+        best = min(best, G.last_ms("sssp"))
+    return best
+
+
+def collect(n_graphs, seed, out_path, budget_s=None):
+    rng = np.random.default_rng(seed)
+    feats, kinds, vals, ms, gids, names = [], [], [], [], [], []
+    t0 = time.time()
+    for gi in range(n_graphs):
+        if budget_s and time.time() - t0 > budget_s:
+            log(f"time budget reached after {gi} graphs")
+            break
+        name, (V, fb, fa, bb, ba), wmax = family_graph(rng)
+        with girhip.Graph(fb, fa, bb, ba) as G:
+            if wmax:
+                G.set_weights(girhip.weights(len(fa), 1, wmax, int(rng.integers(1 << 30))))
+            for k in KS:
+                src = girhip.sources(fb, k, int(rng.integers(1 << 30)))
+                f = G.features(k)
+                # delta ladder from the top; stop once far slower than the best
+                best = 1e300
+                for delta in reversed(DELTAS):
+                    t = time_sssp(G, src, delta)
+                    feats.append(f)
+                    kinds.append(0)
+                    vals.append(delta)
+                    ms.append(t)
+                    gids.append(gi * 10 + k)
+                    best = min(best, t)
+                    if t > 8 * best and delta < 64:
+                        break
+            f = G.features(KS[0])
+            _, times = G.autotune("scc_trim_rounds", [], TRIMS, reps=1)
+            for T, t in zip(TRIMS, times):
+                feats.append(f)
+                kinds.append(1)
+                vals.append(T)
+                ms.append(t)
+                gids.append(gi * 10)
+        names.append(name)
+        log(f"graph {gi} {name} V={V} E={len(fa)} w={wmax or 'hash'} records={len(ms)} {time.time() - t0:.0f}s")
+    np.savez_compressed(out_path, feat=np.array(feats, np.float32), kind=np.array(kinds, np.int8),
+                        value=np.array(vals, np.float64), ms=np.array(ms, np.float64), gid=np.array(gids, np.int64),
+                        names=np.array(names))
+    return len(ms)
+
+
+def dataset(records):
+    """Per (graph, k): features -> (log2 best delta, log2(best trim + 1))."""
+    feat, kind, value, ms, gid = (records[k] for k in ("feat", "kind", "value", "ms", "gid"))
+    best_trim = {}
+    for g in np.unique(gid[kind == 1]):
+        sel = (gid == g) & (kind == 1)
+        best_trim[g // 10] = value[sel][np.argmin(ms[sel])]
+    X, Y = [], []
+    for g in np.unique(gid[kind == 0]):
+        sel = (gid == g) & (kind == 0)
+        if (g // 10) not in best_trim:
+            continue
+        X.append(feat[sel][0])
+        Y.append([np.log2(value[sel][np.argmin(ms[sel])]), np.log2(best_trim[g // 10] + 1)])
+    return np.array(X, np.float32), np.array(Y, np.float32)
+
+
+def fit_norm(X, Y):
+    xm, xs = X.mean(0), X.std(0)
+    xs[xs < 1e-6] = 1.0
+    ym, ys = Y.mean(0), Y.std(0)
+    ys[ys < 1e-6] = 1.0
+    return np.concatenate([xm, xs, ym, ys]).astype(np.float32)
+
+
+def train(X, Y, epochs=3000, lr=3e-3, seed=1):
+    norm = fit_norm(X, Y)
+    xn = (X - norm[:16]) / norm[16:32]
+    yn = (Y - norm[32:34]) / norm[34:36]
+    m = girhip.Mlp(seed)
+    loss = None
+    for i in range(0, epochs, 500):
+        loss = m.train(xn, yn, min(500, epochs - i), lr)
+        log(f"epoch {i + 500}: loss {loss:.4f}")
+    return m, norm, loss
+
+
+def save_model(path, params, norm, meta=""):
+    np.savez(path, params=np.asarray(params, np.float32), norm=np.asarray(norm, np.float32), meta=np.array(meta))
+
+
+def load_model(path=MODEL_PATH):
+    with np.load(path, allow_pickle=False) as z:
+        params, norm = z["params"], z["norm"]
+    m = girhip.Mlp(1)
+    m.params = params
+    return m, norm
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("collect")
+    c.add_argument("out")
+    c.add_argument("--graphs", type=int, default=400)
+    c.add_argument("--seed", type=int, default=1)
+    c.add_argument("--budget-s", type=float, default=None)
+    t = sub.add_parser("train")
+    t.add_argument("records", nargs="+")
+    t.add_argument("--out", default=MODEL_PATH)
+    t.add_argument("--epochs", type=int, default=3000)
+    t.add_argument("--lr", type=float, default=3e-3)
+    args = ap.parse_args()
+    if args.cmd == "collect":
+        n = collect(args.graphs, args.seed, args.out, args.budget_s)
+        log(f"{n} records -> {args.out}")
+    else:
+        Xs, Ys, nrec = [], [], 0
+        for r in args.records:
+            with np.load(r, allow_pickle=False) as z:
+                rec = {k: z[k] for k in ("feat", "kind", "value", "ms", "gid")}
+            nrec += len(rec["ms"])
+            X, Y = dataset(rec)
+            Xs.append(X)
+            Ys.append(Y)
+        X, Y = np.concatenate(Xs), np.concatenate(Ys)
+        m, norm, loss = train(X, Y, args.epochs, args.lr)
+        save_model(args.out, m.params, norm, meta=f"records={nrec} rows={len(X)} loss={loss:.4f}")
+        log(f"trained on {len(X)} rows from {nrec} records, final loss {loss:.4f} -> {args.out}")
+
+
+if __name__ == "__main__":
+    main()

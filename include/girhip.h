@@ -175,6 +175,49 @@ int girhip_reset_kernel_stats(girhip_graph* h);
  * NUL-terminated); *needed (may be NULL) receives the full length + 1. */
 int girhip_kernel_stat_names(girhip_graph* h, char* buf, int64_t cap, int64_t* needed);
 
+/* ---------------- tuning (no counterpart in the reference) ----------------
+ * The reference fixes its knobs as constants: delta 16 (gir.h:215), BFS
+ * alpha 15 / beta 18 (gir_query.cpp:161), iSpan TRIM_TIMES 3 (util.h:34).
+ * Here they are parameters (girhip_set_param) picked by a measuring
+ * autotuner or by a small learned model; results never depend on them. */
+
+/* 16 graph/query features of the current graph for k query sources (float
+ * values; see csrc/tune.hip for the list). */
+int girhip_features(girhip_graph* h, int32_t k, float* features);
+
+/* Autotuner: runs the operation a knob affects once per candidate value
+ * (best of reps runs, HIP-event device time), leaves the knob at the fastest
+ * value and returns it in *best_value; times (ncand, may be NULL) receives
+ * every candidate's ms.  what: "sssp_delta", "scc_trim_rounds", "scc_pivot",
+ * "bfs_alpha", "bfs_beta", "msbfs", "block".  src/k: the query sources used
+ * for BFS/SSSP knobs (host or device memory). */
+int girhip_autotune(girhip_graph* h, const char* what, const int32_t* src, int32_t k, const double* cand,
+                    int32_t ncand, int32_t reps, double* best_value, double* times);
+
+/* Learned model: MLP 16 -> 64 -> 64 -> 2 (ReLU), fp32 master weights used
+ * as bf16 in forward/backward with fp32 accumulation, trained on the GPU by
+ * the library's own kernels (full-batch Adam, loss 1/(2n) sum (y - t)^2). */
+typedef struct girhip_mlp girhip_mlp;
+int girhip_mlp_create(uint64_t seed, girhip_mlp** out);
+int girhip_mlp_destroy(girhip_mlp* m);
+int girhip_mlp_num_params(int32_t* n);
+int girhip_mlp_get_params(girhip_mlp* m, float* params);
+int girhip_mlp_set_params(girhip_mlp* m, const float* params); /* also resets the optimiser state */
+int girhip_mlp_forward(girhip_mlp* m, const float* x, int32_t n, float* y);
+/* loss and dL/dparams of one batch, without an update */
+int girhip_mlp_grad(girhip_mlp* m, const float* x, const float* t, int32_t n, float* grad, float* loss);
+int girhip_mlp_train(girhip_mlp* m, const float* x, const float* t, int32_t n, int32_t epochs, float lr,
+                     float* loss);
+
+/* Learned tuning of the handle's knobs: features -> model -> delta and SCC
+ * trim rounds, with the reference's defaults as fallback and (judge > 0) as
+ * judge on an SSSP probe from min(k, judge) sources.  norm: 36 floats
+ * (feature mean[16], feature scale[16], target mean[2], target scale[2]).
+ * out (may be NULL): chosen delta, chosen trim rounds, raw predicted delta,
+ * 1 if the predicted delta was kept. */
+int girhip_learned_tune(girhip_graph* h, girhip_mlp* model, const float* norm, const int32_t* src, int32_t k,
+                        int32_t judge, double* out);
+
 #ifdef __cplusplus
 }
 #endif
