@@ -96,6 +96,8 @@ def main():
                     help="time one phase only (for hardware-counter runs); prints a short JSON line")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the learned-tuning, query-batch and update-batch sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -184,6 +186,7 @@ def main():
         "sssp": float(sum(8 * V + 8 * n + 16 * m for n, m in zip(n_sssp, m_sssp))),
     }
     _, n_scc, n_dag = G.get_scc()
+    n_wcc = G.get_wcc()[1]
     alg["scc_dag"] = 2 * (4 * (V + 1) + 4 * E) + 8 * V + 4 * (V + 1) + 8 * E + 12 * n_dag
     gbs = {k: alg[k] / (med[k] * 1e-3) / 1e9 for k in alg}
     dom = max(med, key=lambda k: med[k])
@@ -202,6 +205,82 @@ def main():
         kernels = sorted(({"name": n, "launches": c, "ms": round(m, 3)} for n, (c, m) in ks.items() if c),
                          key=lambda x: -x["ms"])[:16]
         G.set_param("profile", 0)
+
+    # ---- learned vs default tuning (same graph, same sources, same GPU) ----
+    learned = None
+    model_path = os.path.join(PKG, "tuned_mlp.npz")
+    if os.path.exists(model_path) and not args.no_extras:
+        import tuning
+        model, norm = tuning.load_model(model_path)
+        ref_dist = dist.clone()
+        ref_scc = scc.clone()
+        res = G.learned_tune(model, norm, sssp_src, judge=0)
+        t_l = []
+        for _ in range(max(args.steps, 2)):
+            G.sssp(sssp_src, 0, out=dist)
+            s_ms = G.last_ms("sssp")
+            G.scc_dag(out=scc)
+            t_l.append((s_ms, G.last_ms("scc_dag")))
+        s_ms = float(np.median([t[0] for t in t_l]))
+        c_ms = float(np.median([t[1] for t in t_l]))
+        same = bool(torch.equal(dist, ref_dist) and torch.equal(scc, ref_scc))
+        learned = {"delta": res["delta"], "trim_rounds": res["trim"], "raw_delta": round(res["raw_delta"], 2),
+                   "sssp_ms_default": round(med["sssp"], 3), "sssp_ms_learned": round(s_ms, 3),
+                   "sssp_gain": round(med["sssp"] / s_ms, 3),
+                   "scc_dag_ms_default": round(med["scc_dag"], 3), "scc_dag_ms_learned": round(c_ms, 3),
+                   "scc_dag_gain": round(med["scc_dag"] / c_ms, 3), "results_identical": same}
+        G.set_param("sssp_delta", 16)
+        G.set_param("scc_trim_rounds", 3)
+        del ref_dist, ref_scc
+        log(f"learned: {learned}")
+
+    # ---- batched queries (1,024 mixed BFS/SSSP from uniform random sources) ----
+    queries = None
+    if not args.no_extras:
+        rng = np.random.default_rng(17)
+        nq = 1024
+        qsrc = rng.integers(0, V, nq).astype(np.int32)
+        qkind = (np.arange(nq) % 2).astype(np.uint8)
+        del lv, dist
+        torch.cuda.empty_cache()
+        qlv = torch.empty((nq // 2, V), dtype=torch.int32, device=dev)
+        qd = torch.empty((nq // 2, V), dtype=torch.int64, device=dev)
+        G.query_batch(qsrc[:64], qkind[:64], 16, levels=qlv, dist=qd)  # warm
+        t0 = time.perf_counter()
+        G.query_batch(qsrc, qkind, 16, levels=qlv, dist=qd)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        dev_ms = G.last_ms("query_batch")
+        queries = {"n": nq, "bfs": nq // 2, "sssp": nq // 2, "device_ms": round(dev_ms, 3),
+                   "queries_per_s": round(nq / (dev_ms * 1e-3), 1), "wall_ms": round(wall * 1e3, 3)}
+        del qlv, qd
+        torch.cuda.empty_cache()
+        log(f"queries: {queries}")
+
+    # ---- update batches (50% inserts of uniform pairs, 50% deletes of arcs) ----
+    updates = None
+    if not args.no_extras:
+        rng = np.random.default_rng(23)
+        updates = []
+        rows = None
+        for nops in (1000, 100000, 1000000):
+            cfb, cfa, _, _ = G.csr()
+            rows = np.repeat(np.arange(V, dtype=np.int32), np.diff(cfb))
+            nd = nops // 2
+            e = rng.integers(0, len(cfa), nd)
+            u = np.concatenate([rows[e], rng.integers(0, V, nops - nd).astype(np.int32)])
+            v = np.concatenate([cfa[e], rng.integers(0, V, nops - nd).astype(np.int32)])
+            ins = np.concatenate([np.zeros(nd, np.uint8), np.ones(nops - nd, np.uint8)])
+            perm = rng.permutation(nops)
+            st = G.apply_batch(u[perm], v[perm], ins[perm])
+            b_ms = G.last_ms("batch")
+            G.wcc(out=lab)
+            G.scc_dag(out=scc)
+            rebuild = G.last_ms("wcc") + G.last_ms("scc_dag")
+            updates.append({"ops": nops, "ms": round(b_ms, 3), "meps": round(nops / (b_ms * 1e-3) / 1e6, 3),
+                            "inserted": st["inserted"], "deleted": st["deleted"],
+                            "rebuild_wcc_scc_ms": round(rebuild, 3)})
+        log(f"updates: {updates}")
 
     cpu = None
     if not args.no_cpu:
@@ -232,13 +311,16 @@ def main():
         "wcc_ms": round(med["wcc"], 3), "scc_dag_ms": round(med["scc_dag"], 3),
         "bfs_ms": round(med["bfs"], 3), "sssp_ms": round(med["sssp"], 3),
         "bfs_gteps": round(bfs_gteps, 3), "sssp_gteps": round(sssp_gteps, 3),
-        "n_wcc": int(G.get_wcc()[1]), "n_scc": int(n_scc), "n_dag": int(n_dag),
+        "n_wcc": int(n_wcc), "n_scc": int(n_scc), "n_dag": int(n_dag),
         "phase_gbs": {k: round(v, 2) for k, v in gbs.items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if cpu:
         out["speedup_build_vs_cpu"] = round(cpu["value"] / build_ms, 2)
         out["speedup_bfs_vs_cpu"] = round(bfs_gteps / cpu["bfs_gteps"], 2) if cpu["bfs_gteps"] else None
+    out["learned_vs_default"] = learned
+    out["query_batch"] = queries
+    out["update_batches"] = updates
     if kernels:
         out["kernels"] = kernels
     G.close()

@@ -218,6 +218,34 @@ extern "C" int girhip_autotune(girhip_graph* h, const char* what, const int32_t*
     });
 }
 
+extern "C" int girhip_query_batch(girhip_graph* h, const int32_t* src, const uint8_t* kind, int32_t n,
+                                  uint64_t delta, int32_t* levels, uint64_t* dist) {
+    return guarded([&] {
+        useDevice(h);
+        if (n < 0 || (n > 0 && (!src || !kind))) raise(GIRHIP_EINVAL, "query_batch: bad argument");
+        std::vector<int32_t> s(n);
+        std::vector<uint8_t> kd(n);
+        if (n) {
+            HIPCHK(hipMemcpy(s.data(), src, sizeof(int32_t) * n, hipMemcpyDefault));
+            HIPCHK(hipMemcpy(kd.data(), kind, (size_t)n, hipMemcpyDefault));
+        }
+        std::vector<int32_t> b, p;
+        for (int32_t i = 0; i < n; ++i) (kd[i] ? p : b).push_back(s[i]);
+        double ms = 0;
+        if (!b.empty()) {
+            if (girhip_bfs(h, b.data(), (int32_t)b.size(), levels, nullptr) != GIRHIP_OK)
+                raise(GIRHIP_EHIP, girhip_last_error());
+            ms += h->last_ms["bfs"];
+        }
+        if (!p.empty()) {
+            if (girhip_sssp(h, p.data(), (int32_t)p.size(), delta, dist, nullptr) != GIRHIP_OK)
+                raise(GIRHIP_EHIP, girhip_last_error());
+            ms += h->last_ms["sssp"];
+        }
+        h->last_ms["query_batch"] = ms;
+    });
+}
+
 // Learned tuning: features -> MLP (on the device) -> delta and trim rounds.
 // norm: 36 floats = feature mean[16], feature scale[16], target mean[2],
 // target scale[2]; outputs are log2(delta) and log2(trim rounds + 1).

@@ -86,6 +86,7 @@ _SIGS = {
     "girhip_mlp_train": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                                         ctypes.POINTER(ctypes.c_float)]),
     "girhip_learned_tune": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    "girhip_query_batch": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_uint64, _vp, _vp]),
 }
 
 _GEN_SIGS = {
@@ -376,6 +377,21 @@ class Graph:
 
     def reset_kernel_stats(self):
         _check(lib().girhip_reset_kernel_stats(self._h))
+
+    def query_batch(self, sources, kinds, delta=0, levels=None, dist=None):
+        """Mixed BFS (kind 0) / SSSP (kind 1) queries -> (levels rows of the
+        BFS queries, dist rows of the SSSP queries), each in query order."""
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
+        kd = np.ascontiguousarray(np.asarray(kinds, dtype=np.uint8))
+        nb, ns, V = int((kd == 0).sum()), int((kd != 0).sum()), self.V
+        if levels is None and nb:
+            levels = np.empty((nb, V), np.int32)
+        if dist is None and ns:
+            dist = np.empty((ns, V), np.uint64)
+        _check(lib().girhip_query_batch(self._h, src.ctypes.data, kd.ctypes.data, len(src), int(delta),
+                                        _ptr(levels, np.int32, nb * V, "levels") if nb else None,
+                                        _ptr(dist, np.uint64, ns * V, "dist") if ns else None))
+        return levels, dist
 
     # ---- tuning ----
     def features(self, k):

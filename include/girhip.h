@@ -175,6 +175,17 @@ int girhip_reset_kernel_stats(girhip_graph* h);
  * NUL-terminated); *needed (may be NULL) receives the full length + 1. */
 int girhip_kernel_stat_names(girhip_graph* h, char* buf, int64_t cap, int64_t* needed);
 
+/* Mixed query batch: query i is a BFS (kind[i] == 0) or an SSSP (kind[i] ==
+ * 1, bucket width delta, 0 = tuned) from src[i].  BFS queries run in
+ * bit-parallel sweeps of 64 sources, SSSP queries in batches of 32 sources
+ * advancing together.  levels receives the BFS rows in query order (one V row
+ * per BFS query), dist the SSSP rows in query order (one V row per SSSP
+ * query); either may be NULL.  The reference answers such a batch with one
+ * GIR::bfs / GIR::sssp call per query (gir_query.cpp:108, 695;
+ * bench_main.cpp:538-542). */
+int girhip_query_batch(girhip_graph* h, const int32_t* src, const uint8_t* kind, int32_t n, uint64_t delta,
+                       int32_t* levels, uint64_t* dist);
+
 /* ---------------- tuning (no counterpart in the reference) ----------------
  * The reference fixes its knobs as constants: delta 16 (gir.h:215), BFS
  * alpha 15 / beta 18 (gir_query.cpp:161), iSpan TRIM_TIMES 3 (util.h:34).
