@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--delta", type=int, default=16,
+                    help="delta-stepping bucket width of the timed SSSP (16 = the reference's default)")
     ap.add_argument("--profile", action="store_true", help="one extra step with per-kernel HIP event timing")
     ap.add_argument("--phase", default="all", choices=["all", "wcc", "scc_dag", "bfs", "sssp"],
                     help="time one phase only (for hardware-counter runs); prints a short JSON line")
@@ -139,7 +141,7 @@ def main():
     torch.cuda.synchronize()
 
     calls = {"wcc": lambda: G.wcc(out=lab), "scc_dag": lambda: G.scc_dag(out=scc),
-             "bfs": lambda: G.bfs(bfs_src, out=lv), "sssp": lambda: G.sssp(sssp_src, 16, out=dist)}
+             "bfs": lambda: G.bfs(bfs_src, out=lv), "sssp": lambda: G.sssp(sssp_src, args.delta, out=dist)}
     run = list(calls) if args.phase == "all" else [args.phase]
 
     def step():
@@ -173,7 +175,7 @@ def main():
     # traversed arcs and reached vertices per source (outside the timed region)
     _, m_bfs = G.bfs(bfs_src, out=lv, edges=True)
     n_bfs = (lv >= 0).sum(dim=1).cpu().numpy()
-    _, m_sssp = G.sssp(sssp_src, 16, out=dist, edges=True)
+    _, m_sssp = G.sssp(sssp_src, args.delta, out=dist, edges=True)
     n_sssp = (dist != -1).sum(dim=1).cpu().numpy()
     bfs_gteps = float(m_bfs.sum()) / (med["bfs"] * 1e-3) / 1e9
     sssp_gteps = float(m_sssp.sum()) / (med["sssp"] * 1e-3) / 1e9
@@ -229,7 +231,7 @@ def main():
                    "sssp_gain": round(med["sssp"] / s_ms, 3),
                    "scc_dag_ms_default": round(med["scc_dag"], 3), "scc_dag_ms_learned": round(c_ms, 3),
                    "scc_dag_gain": round(med["scc_dag"] / c_ms, 3), "results_identical": same}
-        G.set_param("sssp_delta", 16)
+        G.set_param("sssp_delta", args.delta)
         G.set_param("scc_trim_rounds", 3)
         del ref_dist, ref_scc
         log(f"learned: {learned}")
@@ -307,7 +309,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32/uint64", "data": "synthetic",
         "config": {"workload": args.workload, **{k: v for k, v in cfg.items()}, "V": V, "E": E,
-                   "bfs_sources": N_BFS, "sssp_sources": N_SSSP, "delta": 16},
+                   "bfs_sources": N_BFS, "sssp_sources": N_SSSP, "delta": args.delta},
         "wcc_ms": round(med["wcc"], 3), "scc_dag_ms": round(med["scc_dag"], 3),
         "bfs_ms": round(med["bfs"], 3), "sssp_ms": round(med["sssp"], 3),
         "bfs_gteps": round(bfs_gteps, 3), "sssp_gteps": round(sssp_gteps, 3),

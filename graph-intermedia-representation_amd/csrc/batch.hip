@@ -323,6 +323,8 @@ void applyStore(girhip_graph* h, const int32_t* u, const int32_t* v, const uint8
     mergeCsr(h, h->bw_beg, h->bw_adj, nullptr, T, B, newE);
     h->E = newE;
     h->src_ok = false;
+    h->max_deg = -1;
+    h->max_w = -1;
 }
 
 // Bring every requested layer up to date.  C = the changes just applied to a

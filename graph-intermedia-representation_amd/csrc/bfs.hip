@@ -64,20 +64,140 @@ int64_t outDegree(girhip_graph* h, int32_t v) {
 // whose frontier holds its tail.  Push levels OR the tail's frontier word into
 // the head's next word (atomicOr, skipped when nothing new); pull levels let
 // every vertex that some source has not reached yet OR the frontier words of
-// its in-neighbours, stopping once every source has reached it.  The same
-// alpha/beta rule as the single-source engine picks the direction on the
-// union frontier.  Levels are written once per (source, vertex): at the
-// level a bit first appears, or -1 in a final pass over the unseen bits.
+// its in-neighbours, stopping once every source has reached it.  Levels are
+// written once per (source, vertex): at the level a bit first appears, or -1
+// in a final pass over the unseen bits.
+//
+// Besides the per-vertex words, the union frontier is a vertex bitmap F and
+// the heads touched by a push a bitmap T, each with a summary bitmap (one bit
+// per 64-bit word).  A level runs in one of three modes:
+//   pull   - every unfinished vertex scans its in-arcs (large frontiers);
+//   dense  - push from every vertex, finish over all V;
+//   sparse - push from F, finish over T: O(frontier) work per level, no
+//            single-address counters (bitmap words are owned by one wave).
+
+// Per-level counters in a ring (nf = union frontier size, mf = its out-arc
+// count).  A level's push zeroes the next level's pair; its finish fills it.
+struct MsCnt {
+    int64_t nf, mf;
+};
+
+struct MsBits {
+    uint64_t *F, *FS, *T, *TS;
+    int64_t W, W2;
+};
 
 __global__ void kMsSeed(const int32_t* __restrict__ src, int k, int32_t V, uint64_t* seen, uint64_t* front,
-                        int32_t* __restrict__ levels, int64_t* cnt) {
+                        int32_t* __restrict__ levels, MsBits M, const int32_t* __restrict__ fb, MsCnt* cnt) {
     const int s = threadIdx.x;
     if (s >= k) return;
     const int32_t v = src[s];
     if (v < 0 || v >= V) return;
     atomicOr((unsigned long long*)&seen[v], 1ULL << s);
-    atomicOr((unsigned long long*)&front[v], 1ULL << s);
+    const uint64_t old = atomicOr((unsigned long long*)&front[v], 1ULL << s);
     levels[(size_t)s * V + v] = 0;
+    if (old == 0) {  // first source at v
+        setBit(M.F, M.FS, v);
+        atomicAdd((unsigned long long*)&cnt->nf, 1ULL);
+        atomicAdd((unsigned long long*)&cnt->mf, (unsigned long long)(fb[v + 1] - fb[v]));
+    }
+}
+
+// Sparse push: the head's next word gets the tail's frontier bits; the first
+// bits to reach a head mark it in T.
+struct MsPushSF {
+    const int32_t* fa;
+    const uint64_t* seen;
+    uint64_t* next;
+    MsBits M;
+    __device__ int64_t operator()(int32_t, uint64_t fp, int32_t, int64_t j) const {
+        const int32_t w = fa[j];
+        const uint64_t m = fp & ~(seen[w] | ldAcq(&next[w]));
+        if (m && atomicOr((unsigned long long*)&next[w], (unsigned long long)m) == 0) setBit(M.T, M.TS, w);
+        return 0;
+    }
+};
+
+// Walks F's summary: the 64 bitmap words under a non-zero summary word are
+// consumed (cleared) and every frontier vertex in them pushes its row with
+// its frontier word, which is cleared too.
+__global__ void kMsPushS(const int32_t* __restrict__ fb, uint64_t* front, MsPushSF f, MsCnt* nxt, HeavyList H,
+                         int G) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = MsCnt{0, 0};
+    const MsBits& M = f.M;
+    forSummary(M.FS, M.W2, G, [&](int64_t g, uint64_t sm) {
+        const int l = laneId();
+        uint64_t m = 0;
+        if ((sm >> l) & 1) {
+            uint64_t* wp = M.F + g * 64 + l;
+            m = *wp;
+            if (m) *wp = 0;
+        }
+        if (l == 0) M.FS[g] = 0;
+        const int64_t base = (g * 64 + l) * 64;
+        while (__ballot(m != 0)) {
+            const bool act0 = m != 0;
+            int32_t u = 0;
+            if (act0) {
+                u = (int32_t)(base + __ffsll((long long)m) - 1);
+                m &= m - 1;
+            }
+            uint64_t fr = 0;
+            if (act0) {
+                fr = front[u];
+                front[u] = 0;
+            }
+            const bool act = fr != 0;
+            const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+            waveRowsH(act, u, fr, 0, b, e, H, f);
+        }
+    });
+}
+
+// Sparse finish over T's summary: the wave that holds a summary word owns
+// its 64 T words and the F words of the same vertices (one per lane), so F
+// and its summary are written without atomics.  New bits -> seen, frontier word, level writes, F.
+__global__ void kMsFinishS(MsBits M, uint64_t* seen, uint64_t* front, uint64_t* next, int32_t* __restrict__ levels,
+                           int32_t V, int32_t L, const int32_t* __restrict__ fb, MsCnt* nxt, int G) {
+    int64_t nf = 0, mf = 0;
+    forSummary(M.TS, M.W2, G, [&](int64_t g, uint64_t sm) {
+        const int l = laneId();
+        uint64_t t = 0;
+        if ((sm >> l) & 1) {
+            uint64_t* wp = M.T + g * 64 + l;
+            t = *wp;
+            if (t) *wp = 0;
+        }
+        if (l == 0) M.TS[g] = 0;
+        const int64_t base = (g * 64 + l) * 64;
+        uint64_t fw = 0;
+        while (t) {
+            const int b = __ffsll((long long)t) - 1;
+            t &= t - 1;
+            const int32_t w = (int32_t)(base + b);
+            const uint64_t sw = seen[w];
+            const uint64_t nb = next[w] & ~sw;
+            next[w] = 0;
+            if (nb) {
+                seen[w] = sw | nb;
+                front[w] = nb;
+                fw |= 1ULL << b;
+                ++nf;
+                mf += fb[w + 1] - fb[w];
+                uint64_t x = nb;
+                while (x) {
+                    const int s = __ffsll((long long)x) - 1;
+                    x &= x - 1;
+                    levels[(size_t)s * V + w] = L + 1;
+                }
+            }
+        }
+        if (fw) M.F[g * 64 + l] = fw;
+        const uint64_t any = __ballot(fw != 0);
+        if (l == 0 && any) M.FS[g] = any;
+    });
+    blockAtomicAdd(nf, &nxt->nf);
+    blockAtomicAdd(mf, &nxt->mf);
 }
 
 struct MsPushF {
@@ -93,8 +213,8 @@ struct MsPushF {
 };
 
 __global__ void kMsPush(const int32_t* __restrict__ fb, const uint64_t* __restrict__ front, int32_t V, MsPushF f,
-                        int64_t* cz, HeavyList H) {
-    if (blockIdx.x == 0 && threadIdx.x < 3) cz[threadIdx.x] = 0;
+                        MsCnt* nxt, HeavyList H) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = MsCnt{0, 0};
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t fr = v < V ? front[v] : 0;
     const bool act = fr != 0;
@@ -104,8 +224,8 @@ __global__ void kMsPush(const int32_t* __restrict__ fb, const uint64_t* __restri
 
 __global__ void kMsPull(const int32_t* __restrict__ bb, const int32_t* __restrict__ ba,
                         const uint64_t* __restrict__ front, const uint64_t* __restrict__ seen,
-                        uint64_t* __restrict__ next, int32_t V, uint64_t all, int64_t* cz) {
-    if (blockIdx.x == 0 && threadIdx.x < 3) cz[threadIdx.x] = 0;
+                        uint64_t* __restrict__ next, int32_t V, uint64_t all, MsCnt* nxt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = MsCnt{0, 0};
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= V) return;
     const uint64_t sw = seen[w];
@@ -120,10 +240,11 @@ __global__ void kMsPull(const int32_t* __restrict__ bb, const int32_t* __restric
     next[w] = acc & ~sw;
 }
 
-// New bits -> seen, frontier, level writes; cnt[0] = union frontier size,
-// cnt[1] = its out-arc count.
+// Dense finish over all V: new bits -> seen, frontier words, level writes;
+// every F word is rewritten (one wave = one word), F's summary was cleared
+// before the launch.
 __global__ void kMsFinish(uint64_t* seen, uint64_t* front, uint64_t* next, int32_t* __restrict__ levels, int32_t V,
-                          int32_t L, const int32_t* __restrict__ fb, int64_t* cnt) {
+                          int32_t L, const int32_t* __restrict__ fb, MsBits M, MsCnt* nxt) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t nb = 0;
     if (w < V) {
@@ -133,12 +254,18 @@ __global__ void kMsFinish(uint64_t* seen, uint64_t* front, uint64_t* next, int32
         front[w] = nb;
         next[w] = 0;
     }
-    if (__ballot(nb != 0)) {
+    const uint64_t fw = __ballot(nb != 0);
+    if (fw) {
         for (int s = 0; s < 64; ++s)
             if ((nb >> s) & 1) levels[(size_t)s * V + w] = L + 1;
     }
-    blockAtomicAdd(nb != 0, &cnt[0]);
-    blockAtomicAdd(nb != 0 ? (int64_t)(fb[w + 1] - fb[w]) : 0, &cnt[1]);
+    const int64_t word = w >> 6;
+    if (laneId() == 0 && word < M.W) {
+        M.F[word] = fw;
+        if (fw) atomicOr((unsigned long long*)&M.FS[word >> 6], 1ULL << (word & 63));
+    }
+    blockAtomicAdd(nb != 0, &nxt->nf);
+    blockAtomicAdd(nb != 0 ? (int64_t)(fb[w + 1] - fb[w]) : 0, &nxt->mf);
 }
 
 __global__ void kMsUnreached(const uint64_t* __restrict__ seen, int32_t* __restrict__ levels, int32_t V,
@@ -151,46 +278,109 @@ __global__ void kMsUnreached(const uint64_t* __restrict__ seen, int32_t* __restr
     }
 }
 
+// Levels per host readback while the frontier is tiny (high-diameter graphs).
+constexpr int MS_CHUNK = 16;
+// Counter ring: one MsCnt per level, indexed by level mod MS_RING.
+constexpr int MS_RING = 64;
+static_assert(MS_RING >= MS_CHUNK + 2, "ring must hold a chunk and its neighbours");
+
 // Up to 64 sources (device array src, k entries) into levels rows 0..k-1.
-void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels, int64_t m0) {
+//
+// Direction: the reference's alpha rule compares the frontier's arcs with the
+// arcs not yet explored, which a union frontier cannot track (a vertex joins
+// it once per distinct level at which sources reach it), so pull starts when
+// the frontier's arcs exceed E / alpha and ends when its size falls below
+// V / beta.  Push levels are sparse while the frontier holds under a quarter
+// of V; sparse levels whose frontier is tiny and not growing fast run
+// MS_CHUNK at a time with one readback per chunk.
+void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
     Scratch& S = scratch(h);
-    for (int i = 0; i < 3; ++i) S.ms[i].ensure((size_t)(V > 0 ? V : 1));
+    const size_t nV = (size_t)(V > 0 ? V : 1);
+    MsBits M;
+    M.W = ((int64_t)nV + 63) / 64;
+    M.W2 = (M.W + 63) / 64;
+    for (int i = 0; i < 3; ++i) S.ms[i].ensure(nV);
+    S.msbm[0].ensure((size_t)M.W);
+    S.msbm[1].ensure((size_t)M.W2);
+    S.msbm[2].ensure((size_t)M.W);
+    S.msbm[3].ensure((size_t)M.W2);
+    M.F = S.msbm[0].p;
+    M.FS = S.msbm[1].p;
+    M.T = S.msbm[2].p;
+    M.TS = S.msbm[3].p;
+    S.mscnt.ensure(2 * MS_RING);
     uint64_t *seen = S.ms[0].p, *front = S.ms[1].p, *next = S.ms[2].p;
+    MsCnt* ring = (MsCnt*)S.mscnt.p;
     const uint64_t all = k >= 64 ? ~0ULL : ((1ULL << k) - 1);
     for (int i = 0; i < 3; ++i) fillU64(h, S.ms[i].p, 0, V);
-    int64_t* c = h->dcnt.p + 16;  // counter pairs for alternate levels (3 slots each)
-    fillI64(h, c, 0, 6);
-    launch(h, "msbfs_seed", kMsSeed, 1, 64, 0, dsrc, k, V, seen, front, levels, c);
+    fillU64(h, M.F, 0, M.W);
+    fillU64(h, M.FS, 0, M.W2);
+    fillU64(h, M.T, 0, M.W);
+    fillU64(h, M.TS, 0, M.W2);
+    fillI64(h, S.mscnt.p, 0, 2 * MS_RING);
+    const int32_t* fb = h->fw_beg.p;
+    launch(h, "msbfs_seed", kMsSeed, 1, 64, 0, dsrc, k, V, seen, front, levels, M, fb, ring);
+    int64_t r0[2];
+    readCounters(h, S.mscnt.p, 2, r0);
     const int64_t alpha = (int64_t)h->prm.bfs_alpha, beta = (int64_t)h->prm.bfs_beta;
     const int64_t min_reach = (int64_t)h->prm.bfs_min_reach;
-    int64_t nf = k, mf = m0, m_unexplored = h->E - m0;
+    const int64_t E = h->E;
+    int64_t nf = r0[0], mf = r0[1];
     bool pull = false;
+    int64_t mf_prev = 0;  // frontier arcs one readback earlier (0: none / dense)
+    int c_prev = 1;
     const unsigned g = gridFor(V, blk);
-    for (int32_t L = 0; nf > 0; ++L) {
+    int32_t L = 0;
+    while (nf > 0) {
         if ((int64_t)V >= min_reach) {
-            if (!pull && mf * alpha > m_unexplored) pull = true;
+            if (!pull && mf * alpha > E) pull = true;
             else if (pull && nf * beta < (int64_t)V) pull = false;
         }
-        int64_t* cn = c + 3 * (L & 1);
-        int64_t* cz = c + 3 * ((L + 1) & 1);
-        if (pull)
-            launch(h, "msbfs_pull", kMsPull, g, blk, 0, (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p,
-                   (const uint64_t*)front, (const uint64_t*)seen, next, V, all, cz);
-        else {
-            const MsPushF f{h->fw_adj.p, seen, next};
-            launch(h, "msbfs_push", kMsPush, g, blk, 0, (const int32_t*)h->fw_beg.p, (const uint64_t*)front, V, f, cz,
-                   heavyList(h));
-            launchHeavy(h, "msbfs_push_heavy", f);
+        const bool sparse = !pull && nf * 4 < (int64_t)V;
+        if (!sparse) {
+            MsCnt* nxt = &ring[(L + 1) % MS_RING];
+            if (pull)
+                launch(h, "msbfs_pull", kMsPull, g, blk, 0, (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p,
+                       (const uint64_t*)front, (const uint64_t*)seen, next, V, all, nxt);
+            else {
+                const MsPushF f{h->fw_adj.p, seen, next};
+                launch(h, "msbfs_push", kMsPush, g, blk, 0, fb, (const uint64_t*)front, V, f, nxt, heavyList(h));
+                launchHeavyStore(h, "msbfs_push_heavy", f);
+            }
+            fillU64(h, M.FS, 0, M.W2);
+            launch(h, "msbfs_finish", kMsFinish, g, blk, 0, seen, front, next, levels, V, L, fb, M, nxt);
+            int64_t r[2];
+            readCounters(h, (const int64_t*)nxt, 2, r);
+            nf = r[0];
+            mf = r[1];
+            mf_prev = 0;
+            ++L;
+            continue;
         }
-        launch(h, "msbfs_finish", kMsFinish, g, blk, 0, seen, front, next, levels, V, L, (const int32_t*)h->fw_beg.p,
-               cn);
-        int64_t r[2];
-        readCounters(h, cn, 2, r);
-        nf = r[0];
-        mf = r[1];
-        m_unexplored -= mf;
+        const int C = (mf_prev > 0 && mf * 64 < E && mf <= 2 * mf_prev) ? std::min(MS_CHUNK, 2 * c_prev) : 1;
+        c_prev = C;
+        mf_prev = mf;
+        const MsPushSF f{h->fw_adj.p, seen, next, M};
+        // one wave per summary word: the passes are latency-bound
+        const int G = 1;
+        const unsigned gs = summaryGrid(M.W2, G, 2, blk);
+        for (int t = 0; t < C; ++t) {
+            MsCnt* nxt = &ring[(L + t + 1) % MS_RING];
+            launch(h, "msbfs_push_s", kMsPushS, gs, blk, 0, fb, front, f, nxt, heavyList(h), G);
+            launchHeavyStore(h, "msbfs_push_heavy", f);
+            launch(h, "msbfs_finish_s", kMsFinishS, gs, blk, 0, M, seen, front, next, levels, V, L + t, fb, nxt, G);
+        }
+        int64_t r[2 * MS_RING];
+        readCounters(h, S.mscnt.p, 2 * MS_RING, r);
+        for (int t = 1; t <= C; ++t) {
+            const int64_t* c = r + 2 * ((L + t) % MS_RING);
+            nf = c[0];
+            mf = c[1];
+            if (nf == 0) break;
+        }
+        L += C;
     }
     launch(h, "msbfs_unreached", kMsUnreached, g, blk, 0, (const uint64_t*)seen, levels, V, all);
 }
@@ -225,7 +415,7 @@ extern "C" int girhip_bfs(girhip_graph* h, const int32_t* src, int32_t k, int32_
         if (bitpar) {
             for (int32_t i0 = 0; i0 < k; i0 += 64) {
                 const int c = std::min<int32_t>(64, k - i0);
-                msbfs(h, dsrc.p + i0, c, out + (size_t)i0 * V, 0);
+                msbfs(h, dsrc.p + i0, c, out + (size_t)i0 * V);
             }
         } else {
             fillI32(h, out, -1, (int64_t)n);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -92,6 +93,9 @@ struct Scratch {
     DevBuf<int64_t> off, off2;
     DevBuf<uint8_t> stage;
     DevBuf<uint64_t> ms[3];  // bit-parallel BFS: seen / frontier / next words
+    DevBuf<int64_t> mscnt;   // bit-parallel BFS: per-level counter ring
+    DevBuf<uint64_t> msbm[4];  // bit-parallel BFS: frontier / touched bitmaps and summaries
+    DevBuf<int64_t> rcnt;    // reachability engine: per-level counter ring
     // heavy-row list
     DevBuf<int64_t> hv_ctl, hv_b, hv_e, hv_off;
     DevBuf<int32_t> hv_row, hv_aux;
@@ -100,6 +104,8 @@ struct Scratch {
 };
 
 }  // namespace gir
+
+constexpr int HCNT_SLOTS = 256;
 
 struct girhip_graph {
     int device = 0;
@@ -111,6 +117,8 @@ struct girhip_graph {
     bool has_w = false;
     gir::DevBuf<int32_t> fw_src;  // row of every fw slot (COO), built on demand
     bool src_ok = false;
+    int64_t max_deg = -1;  // largest fw/bw row length; -1 = not yet computed
+    int64_t max_w = -1;    // largest stored weight; -1 = not yet computed
 
     // layers
     bool wcc_req = false, scc_req = false;  // built at least once (repair on update)
@@ -125,7 +133,7 @@ struct girhip_graph {
     int64_t n_dag = 0;
     gir::DevBuf<int64_t> rank;
 
-    // host-pinned counters for small readbacks
+    // host-pinned counters for small readbacks (HCNT_SLOTS entries)
     int64_t* hcnt = nullptr;
     gir::DevBuf<int64_t> dcnt;  // device counters (64 slots)
 
@@ -412,6 +420,68 @@ __device__ __forceinline__ int64_t blockReserve(int n, int64_t* cnt) {
     return off;
 }
 
+// Wave-wide exclusive scan of n per lane; reserves the wave's range with one
+// atomicAdd on *cnt.  Every lane of the wave must call it.
+__device__ __forceinline__ int64_t waveReserve(int n, int64_t* cnt) {
+    const int lane = laneId();
+    int64_t v = n;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int64_t o = __shfl_up(v, d);
+        if (lane >= d) v += o;
+    }
+    int64_t base = 0;
+    const int64_t tot = __shfl(v, WAVE - 1);
+    if (lane == 0 && tot) base = (int64_t)atomicAdd((unsigned long long*)cnt, (unsigned long long)tot);
+    base = __shfl(base, 0);
+    return base + v - n;
+}
+
+// Sets bit x of a bitmap; the first bit set in a word also sets the word's
+// bit in the summary (bit g of summary word i <=> bitmap word 64i+g may be
+// non-zero), so a non-zero word always has its summary bit.  Returns true
+// for the call that made the word non-zero.
+__device__ __forceinline__ bool setBit(uint64_t* bits, uint64_t* sum, int32_t x) {
+    const uint64_t bit = 1ULL << (x & 63);
+    uint64_t* wp = &bits[x >> 6];
+    if (ldAcq(wp) & bit) return false;
+    if (atomicOr((unsigned long long*)wp, (unsigned long long)bit) != 0) return false;
+    const uint64_t sb = 1ULL << ((x >> 6) & 63);
+    uint64_t* sp = &sum[x >> 12];
+    if (!(ldAcq(sp) & sb)) atomicOr((unsigned long long*)sp, (unsigned long long)sb);
+    return true;
+}
+
+// Wave-level walk over a summary bitmap: the waves of the grid take tasks of
+// G consecutive summary words (1 <= G <= 64, one coalesced load per task) and
+// call body(g, sm) for every non-zero word g, sm = its value (wave-uniform
+// arguments: the whole wave runs the body).  Small G spreads dense bitmaps
+// over many waves; large G lets few waves sweep a sparse one.
+template <class F>
+__device__ __forceinline__ void forSummary(const uint64_t* __restrict__ sum, int64_t W2, int G, F&& body) {
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / WAVE;
+    const int l = laneId();
+    for (int64_t t0 = wave * G; t0 < W2; t0 += nw * G) {
+        const int64_t gi = t0 + l;
+        const uint64_t mine = (l < G && gi < W2) ? sum[gi] : 0;
+        uint64_t act = __ballot(mine != 0);
+        while (act) {
+            const int j = __ffsll((long long)act) - 1;
+            act &= act - 1;
+            body(t0 + j, (uint64_t)__shfl((unsigned long long)mine, j));
+        }
+    }
+}
+
+// Grid for a forSummary kernel over tasks summary-word tasks (about two
+// tasks per wave, at most 1024 blocks).
+inline unsigned summaryGrid(int64_t W2, int G, int64_t reps, int block) {
+    const int64_t tasks = ((W2 + G - 1) / G) * reps;
+    const int64_t waves = (tasks + 1) / 2;
+    const int64_t b = (waves * WAVE + block - 1) / block;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, b));
+}
+
 // One block: off[k] = sum of the lengths of listed rows before k.
 static __global__ void kHeavyPrefix(HeavyList H) {
     __shared__ int64_t wtot[1024 / WAVE];
@@ -517,5 +587,17 @@ void launchHeavy(girhip_graph* h, const char* name, const F& f, int64_t* sum = n
     const HeavyList H = heavyList(h);
     launch(h, "heavy_prefix", kHeavyPrefix, 1, 1024, 0, H);
     launch(h, name, kHeavyRows<F>, 1024, 256, 0, H, f, sum);
+}
+// Largest row of the edge store's fw and bw CSR (cached until the store
+// changes).
+int64_t maxDegree(girhip_graph* h);
+// Largest arc weight SSSP uses: the stored weights' maximum (cached until
+// they change), or the hash weights' bound 100.
+int64_t maxWeight(girhip_graph* h);
+// launchHeavy for row kernels over the edge store's CSR: skipped when no
+// row of the store can reach the heavy list.
+template <class F>
+void launchHeavyStore(girhip_graph* h, const char* name, const F& f, int64_t* sum = nullptr) {
+    if (maxDegree(h) > HEAVY_T) launchHeavy(h, name, f, sum);
 }
 }  // namespace gir

@@ -100,6 +100,7 @@ HeavyList heavyList(girhip_graph* h) {
 }
 
 void readCounters(girhip_graph* h, const int64_t* dev, int n, int64_t* out) {
+    if (n > HCNT_SLOTS) raise(GIRHIP_EINVAL, "readCounters: too many slots");
     HIPCHK(hipMemcpyAsync(h->hcnt, dev, sizeof(int64_t) * n, hipMemcpyDeviceToHost, h->st));
     syncStream(h);
     std::memcpy(out, h->hcnt, sizeof(int64_t) * n);
@@ -280,6 +281,44 @@ __global__ void kRowOfSlot(const int32_t* __restrict__ beg, int32_t V, int32_t* 
     waveRows(act, (int32_t)v, b, e, [&](int32_t u, int64_t j) { src[j] = u; });
 }
 
+__global__ void kMaxRow(const int32_t* __restrict__ beg, int32_t V, int64_t* out) {
+    int64_t m = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
+        m = std::max<int64_t>(m, beg[v + 1] - beg[v]);
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = std::max<int64_t>(m, __shfl_xor(m, o));
+    if (laneId() == 0 && m) atomicMax((unsigned long long*)out, (unsigned long long)m);
+}
+
+int64_t maxDegree(girhip_graph* h) {
+    if (h->max_deg < 0) {
+        fillI64(h, h->dcnt.p, 0, 1);
+        const unsigned g = boundedGrid(h->V, 256);
+        launch(h, "max_row", kMaxRow, g, 256, 0, (const int32_t*)h->fw_beg.p, h->V, h->dcnt.p);
+        launch(h, "max_row", kMaxRow, g, 256, 0, (const int32_t*)h->bw_beg.p, h->V, h->dcnt.p);
+        readCounters(h, h->dcnt.p, 1, &h->max_deg);
+    }
+    return h->max_deg;
+}
+
+__global__ void kMaxWeight(const uint32_t* __restrict__ w, int64_t E, int64_t* out) {
+    uint32_t m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E; i += (int64_t)gridDim.x * blockDim.x)
+        m = std::max(m, w[i]);
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = std::max(m, (uint32_t)__shfl_xor(m, o));
+    if (laneId() == 0 && m) atomicMax((unsigned long long*)out, (unsigned long long)m);
+}
+
+int64_t maxWeight(girhip_graph* h) {
+    if (!h->has_w) return 100;
+    if (h->max_w < 0) {
+        fillI64(h, h->dcnt.p, 0, 1);
+        launch(h, "max_weight", kMaxWeight, boundedGrid(h->E, 256), 256, 0, (const uint32_t*)h->w_fw.p, h->E,
+               h->dcnt.p);
+        readCounters(h, h->dcnt.p, 1, &h->max_w);
+    }
+    return h->max_w;
+}
+
 const int32_t* fwSrc(girhip_graph* h) {
     if (!h->src_ok) {
         h->fw_src.ensure((size_t)(h->E > 0 ? h->E : 1));
@@ -316,7 +355,7 @@ extern "C" int girhip_create(const int32_t* fw_beg, const int32_t* fw_adj, const
             HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
             HIPCHK(hipEventCreate(&h->t0));
             HIPCHK(hipEventCreate(&h->t1));
-            HIPCHK(hipHostMalloc((void**)&h->hcnt, sizeof(int64_t) * 64, hipHostMallocDefault));
+            HIPCHK(hipHostMalloc((void**)&h->hcnt, sizeof(int64_t) * HCNT_SLOTS, hipHostMallocDefault));
             h->dcnt.ensure(64);
             h->V = V;
             h->E = E;
@@ -421,6 +460,7 @@ extern "C" int girhip_set_weights(girhip_graph* h, const uint32_t* w_fw) {
             h->w_fw.release();
             return;
         }
+        h->max_w = -1;
         h->w_fw.ensure((size_t)(h->E > 0 ? h->E : 1));
         copyIn(h, h->w_fw.p, w_fw, sizeof(uint32_t) * (size_t)h->E);
         fillI64(h, h->dcnt.p, 0, 1);

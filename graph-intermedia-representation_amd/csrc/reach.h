@@ -40,7 +40,7 @@ __global__ void kReachSeed(const int32_t* __restrict__ q, int64_t n, uint64_t* v
 
 // cnt[0] = next queue length, cnt[1] = sum of push-direction degrees of the
 // claimed vertices; cz points at the counter pair of the following level,
-// which this kernel zeroes.
+// which the level kernels zero.
 // Top-down work on arc slot j of row p: claim the head, queue it, return its
 // push-direction degree (summed into cnt[1]).
 template <class Flt, class Vis>
@@ -71,15 +71,23 @@ struct TopDownF {
     }
 };
 
+// The input queue length is read on the device (the previous level's
+// counter), so several levels can be queued without a host readback; the
+// grid strides over the queue, uniformly per wave.
 template <class Flt, class Vis>
-__global__ void kTopDown(const int32_t* __restrict__ qin, int64_t nin, TopDownF<Flt, Vis> f, int64_t* cz,
+__global__ void kTopDown(const int32_t* __restrict__ qin, const int64_t* nin, TopDownF<Flt, Vis> f, int64_t* cz,
                          HeavyList H) {
     if (blockIdx.x == 0 && threadIdx.x < 2) cz[threadIdx.x] = 0;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = i < nin;
-    const int32_t u = act ? qin[i] : 0;
-    const int32_t b = act ? f.beg[u] : 0, e = act ? f.beg[u + 1] : 0;
-    const int64_t m = waveRowsH(act, u, 0, 0, b, e, H, f);
+    const int64_t n = *nin;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t m = 0;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+        const int64_t i = base + laneId();
+        const bool act = i < n;
+        const int32_t u = act ? qin[i] : 0;
+        const int32_t b = act ? f.beg[u] : 0, e = act ? f.beg[u + 1] : 0;
+        m += waveRowsH(act, u, 0, 0, b, e, H, f);
+    }
     blockAtomicAdd(m, &f.cnt[1]);
 }
 
@@ -143,6 +151,14 @@ struct ReachStats {
     int64_t reached = 0;  // including the seeds
 };
 
+// Levels per host readback while a top-down frontier is tiny.
+constexpr int REACH_CHUNK = 16;
+// Counter ring: level L writes pair L mod REACH_RING (next queue length,
+// its push degree sum), reads pair L-1 as its input length and zeroes pair
+// L+1 for the level after it.
+constexpr int REACH_RING = 64;
+static_assert(REACH_RING >= REACH_CHUNK + 3, "ring must hold a chunk and its neighbours");
+
 // Runs reachability from the n0 seeds in B.q[0] (caller fills them).  The
 // visited bitmap B.vis must be clear on entry except for bits the caller
 // wants treated as already visited.  pbeg/padj: push-direction CSR; rbeg/radj:
@@ -157,12 +173,18 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
     ReachStats st;
     if (n0 <= 0) return st;
     launch(h, "reach_seed", kReachSeed<Vis>, gridFor(n0, blk), blk, 0, (const int32_t*)B.q[0], n0, B.vis, on);
-    int64_t* c = h->dcnt.p + 16;  // two counter pairs: c[0..1], c[2..3]
-    fillI64(h, c, 0, 4);
+    Scratch& S = scratch(h);
+    S.rcnt.ensure(2 * REACH_RING);
+    int64_t* c = S.rcnt.p;
+    auto pair = [&](int64_t L) { return c + 2 * ((L % REACH_RING + REACH_RING) % REACH_RING); };
+    fillI64(h, c, 0, 2 * REACH_RING);
+    fillI64(h, pair(-1), n0, 1);  // input length of level 0
     const int64_t alpha = (int64_t)h->prm.bfs_alpha, beta = (int64_t)h->prm.bfs_beta;
     const int64_t min_reach = (int64_t)h->prm.bfs_min_reach;
     int64_t nf = n0, mf = m0, m_unexplored = m_total - m0;
     bool bu = false, bits_valid = false;
+    int64_t mf_prev = 0;  // frontier arcs one readback earlier
+    int c_prev = 1;
     int cur = 0;
     int32_t L = 0;
     st.reached = n0;
@@ -171,33 +193,58 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
             if (!bu && mf * alpha > m_unexplored) bu = true;
             else if (bu && nf * beta < n_reach) bu = false;
         }
-        int64_t* cn = c + 2 * (L & 1);
-        int64_t* cz = c + 2 * ((L + 1) & 1);
-        if (!bu) {
-            const TopDownF<Flt, Vis> f{pbeg, padj, B.vis, L, B.q[cur ^ 1], cn, flt, on};
-            launch(h, "reach_topdown", kTopDown<Flt, Vis>, gridFor(nf, blk), blk, 0, (const int32_t*)B.q[cur], nf, f,
-                   cz, heavyList(h));
-            launchHeavy(h, "reach_topdown_heavy", f, cn + 1);
-            bits_valid = false;
-        } else {
+        if (bu) {
+            int64_t* cn = pair(L);
             if (!bits_valid) {
                 fillU64(h, B.fbits, 0, B.W);
                 launch(h, "reach_q2bits", kQueueToBits, gridFor(nf, blk), blk, 0, (const int32_t*)B.q[cur], nf,
                        B.fbits);
             }
             launch(h, "reach_bottomup", kBottomUp<Flt, Vis>, gridFor(V, blk), blk, 0, rbeg, radj, pbeg, B.vis,
-                   (const uint64_t*)B.fbits, B.nbits, V, L, B.q[cur ^ 1], cn, cz, flt, on);
+                   (const uint64_t*)B.fbits, B.nbits, V, L, B.q[cur ^ 1], cn, pair(L + 1), flt, on);
             std::swap(B.fbits, B.nbits);
             bits_valid = true;
+            int64_t r[2];
+            readCounters(h, cn, 2, r);
+            nf = r[0];
+            mf = r[1];
+            m_unexplored -= mf;
+            st.reached += nf;
+            cur ^= 1;
+            ++L;
+            mf_prev = 0;
+            continue;
         }
-        int64_t r[2];
-        readCounters(h, cn, 2, r);
-        nf = r[0];
-        mf = r[1];
-        m_unexplored -= mf;
-        st.reached += nf;
-        cur ^= 1;
-        ++L;
+        bits_valid = false;
+        // tiny frontiers, far below the switch point and not growing fast: a
+        // chunk of levels
+        const int C = (mf_prev > 0 && mf * 64 < (int64_t)V && mf * alpha * 8 < m_unexplored && mf <= 2 * mf_prev)
+                          ? std::min(REACH_CHUNK, 2 * c_prev)
+                          : 1;
+        c_prev = C;
+        mf_prev = mf;
+        const int64_t work = std::max<int64_t>(nf, mf) * (C > 1 ? 4 : 1);
+        const unsigned g =
+            C > 1 ? (unsigned)std::min<int64_t>(1024, std::max<int64_t>(64, (work + blk - 1) / blk)) : gridFor(nf, blk);
+        for (int t = 0; t < C; ++t) {
+            const TopDownF<Flt, Vis> f{pbeg, padj, B.vis, L + t, B.q[cur ^ 1], pair(L + t), flt, on};
+            launch(h, "reach_topdown", kTopDown<Flt, Vis>, g, blk, 0, (const int32_t*)B.q[cur],
+                   (const int64_t*)pair(L + t - 1), f, pair(L + t + 1), heavyList(h));
+            launchHeavyStore(h, "reach_topdown_heavy", f, pair(L + t) + 1);
+            cur ^= 1;
+        }
+        int64_t r[2 * REACH_RING];
+        readCounters(h, c, 2 * REACH_RING, r);
+        const int32_t L0 = L;
+        for (int t = 0; t < C; ++t) {
+            const int64_t* p = r + (pair(L0 + t) - c);
+            nf = p[0];
+            mf = p[1];
+            m_unexplored -= mf;
+            st.reached += nf;
+            ++L;
+            if (nf == 0) break;
+        }
     }
     st.levels = L;
     return st;

@@ -410,7 +410,7 @@ void runScc(girhip_graph* h) {
             const ColorPropF f{h->fw_adj.p, h->scc.p, color, stamp, it, B.q[cur ^ 1], h->dcnt.p + 6};
             launch(h, "scc_color_prop", kColorProp, gridFor(nq, blk), blk, 0, (const int32_t*)B.q[cur], nq,
                    (const int32_t*)h->fw_beg.p, f, heavyList(h));
-            launchHeavy(h, "scc_color_prop_heavy", f);
+            launchHeavyStore(h, "scc_color_prop_heavy", f);
             readCounters(h, h->dcnt.p + 6, 1, &nq);
             cur ^= 1;
         }

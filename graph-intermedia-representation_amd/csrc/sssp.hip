@@ -2,22 +2,26 @@
 // (gir_query.cpp:629-703) for k sources.
 //
 // Delta-stepping with device-resident buckets.  All sources of a call (up to
-// SB per batch) advance together, one grid row (blockIdx.y) per source.  Per
-// source:
+// SB per batch) advance together in rounds,
+// one grid row (blockIdx.y) per source.  Per source:
 //   - the current bucket is [lo, lo + delta); its work list is a queue;
 //   - a relaxation that improves a distance sets the head's bit in the near
 //     bitmap (lands inside the bucket) or in the far bitmap (beyond it), so
-//     duplicates collapse and no queue counter is hit per arc;
+//     duplicates collapse and no queue counter is hit per arc; the first
+//     near bit of a round also raises the round's near flag;
 //   - after each round a compaction pass turns the near bitmap into the next
-//     queue (block-wide scan, one atomic per block) - the reference
-//     re-processes bucket k until nothing new lands in it;
-//   - when the queue is empty, a pass over the far bitmap finds the smallest
-//     distance >= hi, which picks the next non-empty bucket, and a split pass
-//     moves the far entries inside it into the queue (entries below the new
-//     bucket are settled and dropped).
-// Relaxation is an atomicMin on 64-bit distances.  Weights: one uint32 per
-// forward slot, or the reference's hash GIR::edgeWeight.  Rows longer than
-// HEAVY_T arcs are walked by the whole grid (kHeavyRows).
+//     queue - the reference re-processes bucket k until nothing new lands
+//     in it;
+//   - a round that raised no near flag has drained the bucket: the heavy
+//     arcs (w > delta) of the vertices processed in it are relaxed once, a
+//     pass over the far bitmap finds the smallest distance >= hi, which
+//     picks the next non-empty bucket, and a split pass moves the far
+//     entries inside it into the queue (entries below it are settled).
+// Every bitmap has a summary (one bit per 64-bit word) so the passes touch
+// only non-zero words.  Relaxation is an atomicMin on 64-bit distances.
+// Weights: one uint32 per forward slot, or the reference's hash
+// GIR::edgeWeight.  Rows longer than HEAVY_T arcs are walked by the whole
+// grid (kHeavyRows).
 //
 // HBM traffic per source (algorithmic): 8V (init + final distances) + 8 n_s
 // (row offsets) + 16 m_s (adjacency 4 + weight 4 + distance 8 per arc).
@@ -34,14 +38,15 @@ constexpr int SB = 32;  // sources per batch
 
 // Per-source control block (int64 slots, SB each).
 enum : int {
-    C_NEAR0 = 0,      // queue length, parity 0
-    C_NEAR1 = SB,     // queue length, parity 1
-    C_ADV = 2 * SB,   // 1: advances its bucket this round
-    C_LO = 3 * SB,    // bucket [lo, hi) (uint64)
-    C_HI = 4 * SB,
-    C_FMIN = 5 * SB,  // smallest far distance >= hi (uint64)
-    C_HQ = 6 * SB,    // heavy-phase queue length
-    C_SLOTS = 7 * SB
+    C_NEAR0 = 0,       // queue length, parity 0
+    C_NEAR1 = SB,      // queue length, parity 1
+    C_NSET0 = 2 * SB,  // 1: the round of parity 0 put a vertex into the bucket
+    C_NSET1 = 3 * SB,  //    (parity 1)
+    C_LO = 4 * SB,     // bucket [lo, hi) (uint64)
+    C_HI = 5 * SB,
+    C_FMIN = 6 * SB,   // smallest far distance >= hi (uint64)
+    C_HQ = 7 * SB,     // heavy-phase queue length
+    C_SLOTS = 8 * SB
 };
 
 struct SsspBatch {
@@ -50,18 +55,29 @@ struct SsspBatch {
     uint64_t* nearbits;  // [SB][W]
     uint64_t* farbits;   // [SB][W]
     uint64_t* donebits;  // [SB][W] processed in the current bucket
+    uint64_t* nsum;      // [SB][W2] summaries: bit g of word i <=> word 64i+g
+    uint64_t* fsum;      //          of the bitmap may be non-zero
+    uint64_t* dsum;
     int64_t* ctl;        // C_SLOTS
     uint64_t* dist;      // [SB][V] output rows
-    int64_t V, W;
+    int64_t V, W, W2;
+    int k;               // sources in the batch
 };
 
-__global__ void kBInit(SsspBatch B, const int32_t* __restrict__ src, int k, uint64_t delta) {
+__device__ __forceinline__ int64_t* nearCount(const SsspBatch& B, int par, int s) {
+    return B.ctl + (par ? C_NEAR1 : C_NEAR0) + s;
+}
+__device__ __forceinline__ int64_t* nearSet(const SsspBatch& B, int par, int s) {
+    return B.ctl + (par ? C_NSET1 : C_NSET0) + s;
+}
+
+__global__ void kBInit(SsspBatch B, const int32_t* __restrict__ src, uint64_t delta) {
     const int s = threadIdx.x;
     if (s >= SB) return;
     int64_t* c = B.ctl;
     for (int j = 0; j < C_SLOTS / SB; ++j) c[j * SB + s] = 0;
     c[C_HI + s] = (int64_t)delta;
-    if (s < k) {
+    if (s < B.k) {
         const int32_t v = src[s];
         if (v >= 0 && v < B.V) {
             B.dist[(size_t)s * B.V + v] = 0;
@@ -80,6 +96,7 @@ struct RelaxF {
     const uint32_t* w;
     uint64_t delta;
     int heavy;
+    int par;
     __device__ int64_t operator()(int32_t p, uint64_t dp, int32_t s, int64_t j) const {
         uint64_t* d = B.dist + (size_t)s * B.V;
         uint64_t wt;
@@ -97,116 +114,92 @@ struct RelaxF {
         if (nd < ldAcq(&d[x])) {
             const uint64_t old = atomicMin((unsigned long long*)&d[x], (unsigned long long)nd);
             if (nd < old) {
-                uint64_t* bits = (nd < (uint64_t)B.ctl[C_HI + s]) ? B.nearbits : B.farbits;
-                const uint64_t bit = 1ULL << (x & 63);
-                uint64_t* wp = &bits[(size_t)s * B.W + (x >> 6)];
-                if (!(ldAcq(wp) & bit)) atomicOr((unsigned long long*)wp, (unsigned long long)bit);
+                if (nd < (uint64_t)B.ctl[C_HI + s]) {
+                    // the first bit of a word raises the flag: the near bitmap
+                    // is empty at the start of every round
+                    if (setBit(B.nearbits + (size_t)s * B.W, B.nsum + (size_t)s * B.W2, x)) *nearSet(B, par, s) = 1;
+                } else {
+                    setBit(B.farbits + (size_t)s * B.W, B.fsum + (size_t)s * B.W2, x);
+                }
             }
         }
         return 0;
     }
 };
 
-// Relax the queue of parity par; block (0, s) also resets the per-round
-// slots of source s that this kernel does not read.
-__global__ void kBRelax(RelaxF f, int par, const int32_t* __restrict__ fb, HeavyList H) {
-    const int s = blockIdx.y;
+// Relax source s's queue of parity par (grid stride, uniform per wave).
+// Block (0, s) resets the slots this round fills later and the next round's
+// near flag.  track_done: mark processed vertices (their heavy arcs are due
+// when the bucket drains).
+__global__ void kBRelax(RelaxF f, const int32_t* __restrict__ fb, HeavyList H, int track_done) {
+    const int par = f.par;
     const SsspBatch& B = f.B;
     int64_t* c = B.ctl;
+    const int s = blockIdx.y;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        c[(par ? C_NEAR0 : C_NEAR1) + s] = 0;
+        *nearCount(B, par ^ 1, s) = 0;
+        *nearSet(B, par ^ 1, s) = 0;
         c[C_FMIN + s] = (int64_t)INF;
-        c[C_ADV + s] = 0;
         c[C_HQ + s] = 0;
     }
-    const int64_t nin = c[(par ? C_NEAR1 : C_NEAR0) + s];
-    if ((int64_t)blockIdx.x * blockDim.x >= nin) return;  // whole block idle for this source
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = i < nin;
-    const int32_t u = act ? B.qn[par][(size_t)s * B.V + i] : 0;
-    const uint64_t du = act ? ldAcq(&B.dist[(size_t)s * B.V + u]) : 0;
-    if (act) {  // its heavy arcs are due when the bucket drains
-        uint64_t* wp = &B.donebits[(size_t)s * B.W + (u >> 6)];
-        const uint64_t bit = 1ULL << (u & 63);
-        if (!(ldAcq(wp) & bit)) atomicOr((unsigned long long*)wp, (unsigned long long)bit);
-    }
-    const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
-    waveRowsH(act, u, du, s, b, e, H, f);
-}
-
-// Sources whose bucket drained (next queue empty): the vertices processed
-// in it -> heavy-phase queue (bits cleared).
-__global__ void kBCompactDone(SsspBatch B, int par) {
-    const int s = blockIdx.y;
-    if (B.ctl[(par ? C_NEAR0 : C_NEAR1) + s] != 0) return;
-    const int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t m = 0;
-    if (wi < B.W) {
-        uint64_t* wp = B.donebits + (size_t)s * B.W + wi;
-        m = *wp;
-        if (m) *wp = 0;
-    }
-    int64_t pos = blockReserve(__popcll(m), B.ctl + C_HQ + s);
-    int32_t* q = B.qh + (size_t)s * B.V;
-    while (m) {
-        const int b = __ffsll((long long)m) - 1;
-        m &= m - 1;
-        q[pos++] = (int32_t)(wi * 64 + b);
-    }
-}
-
-// Heavy arcs of the heavy-phase queue (grid stride, uniform per wave).
-__global__ void kBRelaxHeavy(RelaxF f, const int32_t* __restrict__ fb, HeavyList H) {
-    const int s = blockIdx.y;
-    const SsspBatch& B = f.B;
-    const int64_t n = B.ctl[C_HQ + s];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
-        const int64_t i = base + laneId();
-        const bool act = i < n;
-        const int32_t u = act ? B.qh[(size_t)s * B.V + i] : 0;
-        const uint64_t du = act ? ldAcq(&B.dist[(size_t)s * B.V + u]) : 0;
-        const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
-        waveRowsH(act, u, du, s, b, e, H, f);
-    }
-}
-
-// Near bitmap of every source -> its next queue (bits cleared).
-__global__ void kBCompact(SsspBatch B, int par) {
-    const int s = blockIdx.y;
-    const int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t* wp = B.nearbits + (size_t)s * B.W + wi;
-    uint64_t m = 0;
-    if (wi < B.W) {
-        m = *wp;
-        if (m) *wp = 0;
-    }
-    int64_t pos = blockReserve(__popcll(m), B.ctl + (par ? C_NEAR0 : C_NEAR1) + s);
-    int32_t* q = B.qn[par ^ 1] + (size_t)s * B.V;
-    while (m) {
-        const int b = __ffsll((long long)m) - 1;
-        m &= m - 1;
-        q[pos++] = (int32_t)(wi * 64 + b);
-    }
-}
-
-// Sources whose next queue is empty: smallest far distance >= hi.
-__global__ void kBFarMin(SsspBatch B, int par) {
-    const int s = blockIdx.y;
-    const int64_t* c = B.ctl;
-    if (c[(par ? C_NEAR0 : C_NEAR1) + s] != 0) return;
-    const uint64_t hi = (uint64_t)c[C_HI + s];
-    const uint64_t* d = B.dist + (size_t)s * B.V;
-    unsigned long long mn = INF;
-    for (int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; wi < B.W; wi += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t m = B.farbits[(size_t)s * B.W + wi];
-        while (m) {
-            const int b = __ffsll((long long)m) - 1;
-            m &= m - 1;
-            const uint64_t x = d[wi * 64 + b];
-            if (x >= hi && x < mn) mn = x;
+    {
+        const int64_t nin = *nearCount(B, par, s);
+        const int32_t* q = B.qn[par] + (size_t)s * B.V;
+        const uint64_t* d = B.dist + (size_t)s * B.V;
+        for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < nin;
+             base += stride) {
+            const int64_t i = base + laneId();
+            const bool act = i < nin;
+            const int32_t u = act ? q[i] : 0;
+            const uint64_t du = act ? ldAcq(&d[u]) : 0;
+            if (act && track_done) setBit(B.donebits + (size_t)s * B.W, B.dsum + (size_t)s * B.W2, u);
+            const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+            waveRowsH(act, u, du, s, b, e, H, f);
         }
     }
+}
+
+// Drains one summary word (64 bitmap words) of a bitmap: clears the set words
+// and the summary word, and appends their vertices to q at counter *cnt (one
+// atomic per wave).
+__device__ __forceinline__ void drainGroup(uint64_t* bits, uint64_t* sum, int64_t g, uint64_t sm, int32_t* q,
+                                           int64_t* cnt) {
+    const int l = laneId();
+    uint64_t m = 0;
+    if ((sm >> l) & 1) {
+        uint64_t* wp = bits + g * 64 + l;
+        m = *wp;
+        if (m) *wp = 0;
+    }
+    if (l == 0) sum[g] = 0;
+    int64_t pos = waveReserve(__popcll(m), cnt);
+    const int64_t wi = g * 64 + l;
+    while (m) {
+        const int b = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        q[pos++] = (int32_t)(wi * 64 + b);
+    }
+}
+
+// Smallest distance >= hi over one summary word of source s's far bitmap
+// (per lane; reduced by farMinCommit).
+__device__ __forceinline__ uint64_t farMinGroup(const SsspBatch& B, int s, int64_t g, uint64_t sm, uint64_t hi,
+                                                uint64_t mn) {
+    const int l = laneId();
+    uint64_t m = ((sm >> l) & 1) ? B.farbits[(size_t)s * B.W + g * 64 + l] : 0;
+    const uint64_t* d = B.dist + (size_t)s * B.V;
+    const int64_t wi = g * 64 + l;
+    while (m) {
+        const int b = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const uint64_t x = d[wi * 64 + b];
+        if (x >= hi && x < mn) mn = x;
+    }
+    return mn;
+}
+
+__device__ __forceinline__ void farMinCommit(const SsspBatch& B, int s, unsigned long long mn) {
     for (int o = WAVE / 2; o > 0; o >>= 1) {
         const unsigned long long t = __shfl_xor(mn, o);
         mn = t < mn ? t : mn;
@@ -214,48 +207,111 @@ __global__ void kBFarMin(SsspBatch B, int par) {
     if (laneId() == 0 && mn != INF) atomicMin((unsigned long long*)&B.ctl[C_FMIN + s], mn);
 }
 
-__global__ void kBAdvance(SsspBatch B, int par, int k, uint64_t delta) {
-    const int s = threadIdx.x;
-    if (s >= k) return;
-    int64_t* c = B.ctl;
-    if (c[(par ? C_NEAR0 : C_NEAR1) + s] != 0) return;
-    const uint64_t m = (uint64_t)c[C_FMIN + s];
-    if (m == INF) return;  // no far work left: this source is done
-    const uint64_t lo = (m / delta) * delta;
-    c[C_LO + s] = (int64_t)lo;
-    c[C_HI + s] = (int64_t)(lo > INF - delta ? INF : lo + delta);
-    c[C_ADV + s] = 1;
+// After the relaxation of parity par, per source: if the round put vertices
+// into the bucket, its near bitmap -> next queue; otherwise the bucket has
+// drained: with heavy arcs pending (heavy = 1) the processed vertices ->
+// heavy-phase queue, else the smallest far distance is found right here.
+__global__ void kBCompact(SsspBatch B, int par, int heavy, int G) {
+    const int s = blockIdx.y;
+    {
+        const size_t bo = (size_t)s * B.W, so = (size_t)s * B.W2, qo = (size_t)s * B.V;
+        if (*nearSet(B, par, s)) {
+            forSummary(B.nsum + so, B.W2, G, [&](int64_t g, uint64_t sm) {
+                drainGroup(B.nearbits + bo, B.nsum + so, g, sm, B.qn[par ^ 1] + qo, nearCount(B, par ^ 1, s));
+            });
+        } else if (heavy) {
+            forSummary(B.dsum + so, B.W2, G, [&](int64_t g, uint64_t sm) {
+                drainGroup(B.donebits + bo, B.dsum + so, g, sm, B.qh + qo, B.ctl + C_HQ + s);
+            });
+        } else {
+            const uint64_t hi = (uint64_t)B.ctl[C_HI + s];
+            uint64_t mn = INF;
+            forSummary(B.fsum + so, B.W2, G, [&](int64_t g, uint64_t sm) { mn = farMinGroup(B, s, g, sm, hi, mn); });
+            farMinCommit(B, s, mn);
+        }
+    }
 }
 
-// Far entries of advancing sources: inside [lo, hi) -> next queue, below lo
-// -> settled (dropped), both leave the far bitmap; the rest stay.
-__global__ void kBSplit(SsspBatch B, int par) {
+// Heavy arcs of source s's heavy-phase queue (grid stride, uniform per
+// wave).
+__global__ void kBRelaxHeavy(RelaxF f, const int32_t* __restrict__ fb, HeavyList H) {
+    const SsspBatch& B = f.B;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int s = blockIdx.y;
-    const int64_t* c = B.ctl;
-    if (!c[C_ADV + s]) return;
-    const uint64_t lo = (uint64_t)c[C_LO + s], hi = (uint64_t)c[C_HI + s];
-    const uint64_t* d = B.dist + (size_t)s * B.V;
-    const int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t take = 0;
-    if (wi < B.W) {
-        uint64_t* wp = B.farbits + (size_t)s * B.W + wi;
-        const uint64_t m = *wp;
-        uint64_t keep = 0, t = m;
-        while (t) {
-            const int b = __ffsll((long long)t) - 1;
-            t &= t - 1;
-            const uint64_t x = d[wi * 64 + b];
-            if (x >= hi) keep |= 1ULL << b;
-            else if (x >= lo) take |= 1ULL << b;
+    {
+        const int64_t n = B.ctl[C_HQ + s];
+        const int32_t* q = B.qh + (size_t)s * B.V;
+        const uint64_t* d = B.dist + (size_t)s * B.V;
+        for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n;
+             base += stride) {
+            const int64_t i = base + laneId();
+            const bool act = i < n;
+            const int32_t u = act ? q[i] : 0;
+            const uint64_t du = act ? ldAcq(&d[u]) : 0;
+            const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+            waveRowsH(act, u, du, s, b, e, H, f);
         }
-        if (keep != m) *wp = keep;
     }
-    int64_t pos = blockReserve(__popcll(take), B.ctl + (par ? C_NEAR0 : C_NEAR1) + s);
-    int32_t* q = B.qn[par ^ 1] + (size_t)s * B.V;
-    while (take) {
-        const int b = __ffsll((long long)take) - 1;
-        take &= take - 1;
-        q[pos++] = (int32_t)(wi * 64 + b);
+}
+
+// Drained sources after their heavy phase: smallest far distance >= hi.
+__global__ void kBFarMin(SsspBatch B, int par, int G) {
+    const int s = blockIdx.y;
+    {
+        if (*nearSet(B, par, s)) return;
+        const uint64_t hi = (uint64_t)B.ctl[C_HI + s];
+        uint64_t mn = INF;
+        forSummary(B.fsum + (size_t)s * B.W2, B.W2, G,
+                   [&](int64_t g, uint64_t sm) { mn = farMinGroup(B, s, g, sm, hi, mn); });
+        farMinCommit(B, s, mn);
+    }
+}
+
+// Drained sources with far work left advance to the bucket holding the
+// smallest far distance (every block derives the same [lo, hi); block
+// (0, s) stores it for the next round).  Far entries inside it -> next queue, below
+// it -> settled (dropped), both leave the far bitmap; the rest stay.  A
+// summary word is cleared once its 64 bitmap words are all empty.
+__global__ void kBSplit(SsspBatch B, int par, uint64_t delta, int G) {
+    const int s = blockIdx.y;
+    {
+        if (*nearSet(B, par, s)) return;
+        const uint64_t mfar = (uint64_t)B.ctl[C_FMIN + s];
+        if (mfar == INF) return;  // no far work left: this source is done
+        const uint64_t lo = (mfar / delta) * delta;
+        const uint64_t hi = lo > INF - delta ? INF : lo + delta;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            B.ctl[C_LO + s] = (int64_t)lo;
+            B.ctl[C_HI + s] = (int64_t)hi;
+        }
+        const uint64_t* d = B.dist + (size_t)s * B.V;
+        int32_t* q = B.qn[par ^ 1] + (size_t)s * B.V;
+        uint64_t* fs = B.fsum + (size_t)s * B.W2;
+        uint64_t* fbits = B.farbits + (size_t)s * B.W;
+        forSummary(fs, B.W2, G, [&](int64_t g, uint64_t sm) {
+            const int l = laneId();
+            const int64_t wi = g * 64 + l;
+            uint64_t take = 0, keep = 0;
+            if ((sm >> l) & 1) {
+                const uint64_t m = fbits[wi];
+                uint64_t t = m;
+                while (t) {
+                    const int b = __ffsll((long long)t) - 1;
+                    t &= t - 1;
+                    const uint64_t x = d[wi * 64 + b];
+                    if (x >= hi) keep |= 1ULL << b;
+                    else if (x >= lo) take |= 1ULL << b;
+                }
+                if (keep != m) fbits[wi] = keep;
+            }
+            if (__ballot(keep != 0) == 0 && l == 0) fs[g] = 0;
+            int64_t pos = waveReserve(__popcll(take), nearCount(B, par ^ 1, s));
+            while (take) {
+                const int b = __ffsll((long long)take) - 1;
+                take &= take - 1;
+                q[pos++] = (int32_t)(wi * 64 + b);
+            }
+        });
     }
 }
 
@@ -269,9 +325,14 @@ __global__ void kReachedArcsD(const uint64_t* __restrict__ dist, const int32_t* 
 
 struct SsspBufs {
     DevBuf<int32_t> qn[2], qh;
-    DevBuf<uint64_t> nearbits, farbits, donebits;
+    DevBuf<uint64_t> nearbits, farbits, donebits, nsum, fsum, dsum;
     DevBuf<int64_t> ctl;
 };
+
+// Rounds per host readback while every queue is small (high-diameter
+// graphs): all round kernels read their work sizes on the device, and a
+// round with nothing left to do is a no-op.
+constexpr int SSSP_CHUNK = 16;
 
 // k <= SB sources (device array dsrc) into rows of dist.
 static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, uint64_t* dist, SsspBufs& Bf) {
@@ -281,42 +342,62 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
     SsspBatch B;
     B.V = V;
     B.W = ((int64_t)V + 63) / 64;
+    B.W2 = (B.W + 63) / 64;
     for (int p = 0; p < 2; ++p) B.qn[p] = Bf.qn[p].p;
     B.qh = Bf.qh.p;
     B.nearbits = Bf.nearbits.p;
     B.farbits = Bf.farbits.p;
     B.donebits = Bf.donebits.p;
+    B.nsum = Bf.nsum.p;
+    B.fsum = Bf.fsum.p;
+    B.dsum = Bf.dsum.p;
     B.ctl = Bf.ctl.p;
     B.dist = dist;
     fillU64(h, dist, INF, (int64_t)k * V);
     fillU64(h, B.nearbits, 0, (int64_t)k * B.W);
     fillU64(h, B.farbits, 0, (int64_t)k * B.W);
     fillU64(h, B.donebits, 0, (int64_t)k * B.W);
-    launch(h, "sssp_init", kBInit, 1, 64, 0, B, dsrc, k, delta);
-    const RelaxF fl{B, h->fw_adj.p, w, delta, 0}, fh{B, h->fw_adj.p, w, delta, 1};
-    const unsigned wgrid = gridFor(B.W, blk);
-    const unsigned fgrid = std::min<unsigned>(wgrid, 256);
+    fillU64(h, B.nsum, 0, (int64_t)k * B.W2);
+    fillU64(h, B.fsum, 0, (int64_t)k * B.W2);
+    fillU64(h, B.dsum, 0, (int64_t)k * B.W2);
+    B.k = k;
+    launch(h, "sssp_init", kBInit, 1, 64, 0, B, dsrc, delta);
+    // heavy arcs exist only if some weight exceeds delta
+    const int heavy = (uint64_t)maxWeight(h) > delta ? 1 : 0;
+    const RelaxF fl[2] = {{B, h->fw_adj.p, w, delta, 0, 0}, {B, h->fw_adj.p, w, delta, 0, 1}};
+    const RelaxF fh[2] = {{B, h->fw_adj.p, w, delta, 1, 0}, {B, h->fw_adj.p, w, delta, 1, 1}};
     const unsigned hgrid = std::min<unsigned>(gridFor(V, blk), 1024);
+    const int32_t* fb = h->fw_beg.p;
     int par = 0;
     std::vector<int64_t> r(SB);
     int64_t maxn = 1;
+    int R = 1;  // rounds in the next chunk
+    // one wave per summary word (dense and sparse bitmaps alike: the passes
+    // are latency-bound, so they want many waves)
+    const int G = 1;
+    const unsigned sgrid = summaryGrid(B.W2, G, 2, blk);
     while (true) {
-        launch(h, "sssp_relax", kBRelax, dim3(gridFor(maxn, blk), k), blk, 0, fl, par, (const int32_t*)h->fw_beg.p,
-               heavyList(h));
-        launchHeavy(h, "sssp_relax_rows", fl);
-        launch(h, "sssp_compact", kBCompact, dim3(wgrid, k), blk, 0, B, par);
-        launch(h, "sssp_compact_done", kBCompactDone, dim3(wgrid, k), blk, 0, B, par);
-        launch(h, "sssp_relax_heavy", kBRelaxHeavy, dim3(hgrid, k), blk, 0, fh, (const int32_t*)h->fw_beg.p,
-               heavyList(h));
-        launchHeavy(h, "sssp_relax_heavy_rows", fh);
-        launch(h, "sssp_farmin", kBFarMin, dim3(fgrid, k), blk, 0, B, par);
-        launch(h, "sssp_advance", kBAdvance, 1, 64, 0, B, par, k, delta);
-        launch(h, "sssp_split", kBSplit, dim3(wgrid, k), blk, 0, B, par);
-        par ^= 1;
+        const unsigned rgrid = R > 1 ? std::min<unsigned>(1024, std::max<unsigned>(64, gridFor(4 * maxn, blk)))
+                                     : gridFor(maxn, blk);
+        for (int t = 0; t < R; ++t) {
+            launch(h, "sssp_relax", kBRelax, dim3(rgrid, k), blk, 0, fl[par], fb, heavyList(h), heavy);
+            launchHeavyStore(h, "sssp_relax_rows", fl[par]);
+            launch(h, "sssp_compact", kBCompact, dim3(sgrid, k), blk, 0, B, par, heavy, G);
+            if (heavy) {
+                launch(h, "sssp_relax_heavy", kBRelaxHeavy, dim3(hgrid, k), blk, 0, fh[par], fb, heavyList(h));
+                launchHeavyStore(h, "sssp_relax_heavy_rows", fh[par]);
+                launch(h, "sssp_farmin", kBFarMin, dim3(sgrid, k), blk, 0, B, par, G);
+            }
+            launch(h, "sssp_split", kBSplit, dim3(sgrid, k), blk, 0, B, par, delta, G);
+            par ^= 1;
+        }
         readCounters(h, Bf.ctl.p + (par ? C_NEAR1 : C_NEAR0), SB, r.data());
-        maxn = 0;
-        for (int s = 0; s < k; ++s) maxn = std::max(maxn, r[s]);
-        if (maxn == 0) break;  // every queue empty after the advance: all sources settled
+        int64_t mx = 0;
+        for (int s = 0; s < k; ++s) mx = std::max(mx, r[s]);
+        if (mx == 0) break;  // every queue empty after the advance: all sources settled
+        // queues small and not growing fast: longer chunks, else one round
+        R = (mx * 64 < (int64_t)V && mx <= 2 * maxn && maxn > 1) ? std::min(SSSP_CHUNK, 2 * R) : 1;
+        maxn = mx;
     }
 }
 
@@ -349,6 +430,10 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         Bf.nearbits.ensure((size_t)kc * Ws);
         Bf.farbits.ensure((size_t)kc * Ws);
         Bf.donebits.ensure((size_t)kc * Ws);
+        const size_t W2s = (Ws + 63) / 64;
+        Bf.nsum.ensure((size_t)kc * W2s);
+        Bf.fsum.ensure((size_t)kc * W2s);
+        Bf.dsum.ensure((size_t)kc * W2s);
         Bf.ctl.ensure(C_SLOTS);
         DevBuf<int32_t> dsrc;
         dsrc.ensure((size_t)(k > 0 ? k : 1));

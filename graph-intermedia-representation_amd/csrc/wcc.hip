@@ -143,7 +143,7 @@ void runWcc(girhip_graph* h) {
     const HookF f{h->fw_adj.p, h->bw_adj.p, comp};
     launch(h, "wcc_hook_rest", kWccHookRest, g, blk, 0, (const int32_t*)h->fw_beg.p, (const int32_t*)h->bw_beg.p, V,
            rounds, (const int64_t*)h->dcnt.p + 8, f, heavyList(h));
-    launchHeavy(h, "wcc_hook_rest_heavy", f);
+    launchHeavyStore(h, "wcc_hook_rest_heavy", f);
     launch(h, "wcc_compress", kWccCompress, g, blk, 0, comp, V);
     fillI64(h, h->dcnt.p, 0, 1);
     launch(h, "count_roots", kCountSelf, boundedGrid(V, blk), blk, 0, (const int32_t*)comp, V, h->dcnt.p);

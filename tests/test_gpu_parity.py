@@ -130,6 +130,28 @@ def test_grid_parity():
             assert (d[i] == oracle.dijkstra(fb, fa, w, x)).all()
 
 
+def test_bfs_many_levels_bit_parallel():
+    # long thin grid: hundreds of tiny levels -> sparse queue mode, chunked
+    # levels and the counter ring wrapping; duplicate and invalid sources
+    V, fb, fa, bb, ba = girhip.grid_graph(40, 700, seed=35)
+    say("grid 40x700 msbfs E", len(fa))
+    src = np.concatenate([girhip.sources(fb, 60, 7), [0, 0, V - 1, -3]]).astype(np.int32)
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        lv, m = G.bfs(src, edges=True)
+        for i, x in enumerate(src):
+            want = oracle.bfs_textbook(fb, fa, x) if 0 <= x < V else np.full(V, -1, np.int32)
+            assert (lv[i] == want).all(), f"bfs src {x}"
+            assert m[i] == int(np.diff(fb)[want >= 0].sum())
+        assert lv.max() > 300
+        # same answers with the per-source engine and with pull forced early
+        G.set_param("msbfs", 0)
+        assert (G.bfs(src[:3]) == lv[:3]).all()
+        G.set_param("msbfs", 1)
+        G.set_param("bfs_alpha", 1)
+        G.set_param("bfs_min_reach", 0)
+        assert (G.bfs(src) == lv).all()
+
+
 def test_tuning_knobs_do_not_change_results():
     V, fb, fa, bb, ba = girhip.rmat_graph(12, seed=34)
     say("knobs rmat12")
