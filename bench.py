@@ -54,10 +54,11 @@ def make_graph(cfg):
     return girhip.grid_graph(cfg["rows"], cfg["cols"], cfg["p_keep"], cfg["p_bidir"], cfg["seed"])
 
 
-def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2):
+def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=None):
     """Time the compiled reference (oracle/_ref/gir_ref: the reference's own
     gir.cpp / gir_query.cpp) on this host: one buildWCC + buildSCC_DAG, BFS
-    from n_bfs and SSSP (its hash weights, delta 16) from n_sssp sources."""
+    from n_bfs and SSSP (its hash weights, delta 16) from n_sssp sources, and
+    GIR::applyBatch of batch (u, v, insert arrays) when given."""
     import girhip
     ref = os.path.join(ROOT, "oracle", "_ref", "gir_ref")
     if not os.path.exists(ref):
@@ -66,12 +67,15 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2):
         girhip.write_csr(os.path.join(td, "g"), fb, fa, bb, ba)
         bfs_src[:n_bfs].astype(np.int32).tofile(os.path.join(td, "b.bin"))
         sssp_src[:n_sssp].astype(np.int32).tofile(os.path.join(td, "s.bin"))
+        cmd = [ref, os.path.join(td, "g") + "/", td + "/", "--bfs", os.path.join(td, "b.bin"),
+               "--sssp", os.path.join(td, "s.bin"), "--no-output"]
+        if batch is not None:
+            np.stack([np.asarray(x, np.int32) for x in batch], axis=1).tofile(os.path.join(td, "ops.bin"))
+            cmd += ["--batch", os.path.join(td, "ops.bin")]
         env = dict(os.environ)
         env.setdefault("OMP_PROC_BIND", "close")
         t0 = time.time()
-        out = subprocess.run([ref, os.path.join(td, "g") + "/", td + "/", "--bfs", os.path.join(td, "b.bin"),
-                              "--sssp", os.path.join(td, "s.bin"), "--no-output"], capture_output=True, text=True,
-                             env=env, check=True).stdout
+        out = subprocess.run(cmd, capture_output=True, text=True, env=env, check=True).stdout
         wall = time.time() - t0
     rec = {}
     for line in out.splitlines():
@@ -180,6 +184,14 @@ def main():
     bfs_gteps = float(m_bfs.sum()) / (med["bfs"] * 1e-3) / 1e9
     sssp_gteps = float(m_sssp.sum()) / (med["sssp"] * 1e-3) / 1e9
     build_ms = med["wcc"] + med["scc_dag"]
+    # like-for-like with the CPU reference: its hash weights (GIR::edgeWeight), delta 16
+    G.set_weights(None)
+    hash_ms = []
+    for _ in range(2):
+        G.sssp(sssp_src, 16, out=dist)
+        hash_ms.append(G.last_ms("sssp"))
+    sssp_hash_ms_per_source = min(hash_ms) / N_SSSP
+    G.set_weights(d_w)
 
     # algorithmic bytes (per unit: one build, one source) -> achieved GB/s
     alg = {
@@ -265,6 +277,7 @@ def main():
         rng = np.random.default_rng(23)
         updates = []
         rows = None
+        first_batch = None
         for nops in (1000, 100000, 1000000):
             cfb, cfa, _, _ = G.csr()
             rows = np.repeat(np.arange(V, dtype=np.int32), np.diff(cfb))
@@ -274,6 +287,8 @@ def main():
             v = np.concatenate([cfa[e], rng.integers(0, V, nops - nd).astype(np.int32)])
             ins = np.concatenate([np.zeros(nd, np.uint8), np.ones(nops - nd, np.uint8)])
             perm = rng.permutation(nops)
+            if first_batch is None:
+                first_batch = (u[perm], v[perm], ins[perm])
             st = G.apply_batch(u[perm], v[perm], ins[perm])
             b_ms = G.last_ms("batch")
             G.wcc(out=lab)
@@ -287,7 +302,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (compiled reference) ...")
-        r = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src)
+        r = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, batch=first_batch if updates else None)
         if r is not None:
             rec, wall = r
             b = rec.get("build", {})
@@ -302,6 +317,9 @@ def main():
                    "sssp_ms_per_source": rec["sssp"]["ms"] / rec["sssp"]["sources"]}
             nb = rec["bfs"]["sources"]
             cpu["bfs_gteps"] = float(m_bfs[:nb].sum()) / (rec["bfs"]["ms"] * 1e-3) / 1e9
+            if "batch" in rec:
+                cpu["batch_ops"] = rec["batch"]["ops"]
+                cpu["batch_ms"] = rec["batch"]["ms"]
             log(f"cpu build {cpu_build:.1f} ms")
 
     out = {
@@ -317,9 +335,13 @@ def main():
         "phase_gbs": {k: round(v, 2) for k, v in gbs.items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    out["sssp_hash_ms_per_source"] = round(sssp_hash_ms_per_source, 3)
     if cpu:
         out["speedup_build_vs_cpu"] = round(cpu["value"] / build_ms, 2)
         out["speedup_bfs_vs_cpu"] = round(bfs_gteps / cpu["bfs_gteps"], 2) if cpu["bfs_gteps"] else None
+        out["speedup_sssp_vs_cpu"] = round(cpu["sssp_ms_per_source"] / sssp_hash_ms_per_source, 2)
+        if "batch_ms" in cpu and updates:
+            out["speedup_batch1e3_vs_cpu"] = round(cpu["batch_ms"] / updates[0]["ms"], 2)
     out["learned_vs_default"] = learned
     out["query_batch"] = queries
     out["update_batches"] = updates

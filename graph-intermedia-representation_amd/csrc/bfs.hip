@@ -359,7 +359,8 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
             ++L;
             continue;
         }
-        const int C = (mf_prev > 0 && mf * 64 < E && mf <= 2 * mf_prev) ? std::min(MS_CHUNK, 2 * c_prev) : 1;
+        const int cmax = std::max(1, std::min(MS_CHUNK, (int)h->prm.chunk));
+        const int C = (mf_prev > 0 && mf * 64 < E && mf <= 2 * mf_prev) ? std::min(cmax, 2 * c_prev) : 1;
         c_prev = C;
         mf_prev = mf;
         const MsPushSF f{h->fw_adj.p, seen, next, M};

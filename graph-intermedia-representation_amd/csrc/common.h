@@ -75,6 +75,7 @@ struct Params {
     double scc_color_min = 0;
     double profile = 0;  // 1: time every kernel launch with HIP events
     double msbfs = 1;    // 1: k>1 BFS runs 64 sources per bit-parallel sweep
+    double chunk = 16;   // most BFS levels / SSSP rounds queued per host readback
 };
 
 struct KStat {

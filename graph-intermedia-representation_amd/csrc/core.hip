@@ -487,7 +487,7 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"block", &Params::block},                 {"wcc_rounds", &Params::wcc_rounds},
                {"scc_trim_rounds", &Params::scc_trim_rounds}, {"scc_pivot", &Params::scc_pivot},
                {"scc_color_min", &Params::scc_color_min}, {"profile", &Params::profile},
-               {"msbfs", &Params::msbfs}};
+               {"msbfs", &Params::msbfs},         {"chunk", &Params::chunk}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;
@@ -501,6 +501,8 @@ extern "C" int girhip_set_param(girhip_graph* h, const char* name, double value)
         if (std::strcmp(name, "sssp_delta") == 0 && !(value >= 1)) raise(GIRHIP_EINVAL, "set_param: sssp_delta >= 1");
         if (std::strcmp(name, "block") == 0 && !(value == 64 || value == 128 || value == 256 || value == 512 || value == 1024))
             raise(GIRHIP_EINVAL, "set_param: block must be 64, 128, 256, 512 or 1024");
+        if (std::strcmp(name, "chunk") == 0 && !(value >= 1 && value <= 16))
+            raise(GIRHIP_EINVAL, "set_param: chunk must be in [1, 16]");
         if (!(value == value)) raise(GIRHIP_EINVAL, "set_param: NaN");
         *s = value;
     });

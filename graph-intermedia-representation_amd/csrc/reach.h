@@ -219,7 +219,7 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
         // tiny frontiers, far below the switch point and not growing fast: a
         // chunk of levels
         const int C = (mf_prev > 0 && mf * 64 < (int64_t)V && mf * alpha * 8 < m_unexplored && mf <= 2 * mf_prev)
-                          ? std::min(REACH_CHUNK, 2 * c_prev)
+                          ? std::min(std::max(1, std::min(REACH_CHUNK, (int)h->prm.chunk)), 2 * c_prev)
                           : 1;
         c_prev = C;
         mf_prev = mf;

@@ -372,6 +372,7 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
     std::vector<int64_t> r(SB);
     int64_t maxn = 1;
     int R = 1;  // rounds in the next chunk
+    const int cmax = std::max(1, std::min(SSSP_CHUNK, (int)h->prm.chunk));
     // one wave per summary word (dense and sparse bitmaps alike: the passes
     // are latency-bound, so they want many waves)
     const int G = 1;
@@ -396,7 +397,7 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
         for (int s = 0; s < k; ++s) mx = std::max(mx, r[s]);
         if (mx == 0) break;  // every queue empty after the advance: all sources settled
         // queues small and not growing fast: longer chunks, else one round
-        R = (mx * 64 < (int64_t)V && mx <= 2 * maxn && maxn > 1) ? std::min(SSSP_CHUNK, 2 * R) : 1;
+        R = (mx * 64 < (int64_t)V && mx <= 2 * maxn && maxn > 1) ? std::min(cmax, 2 * R) : 1;
         maxn = mx;
     }
 }

@@ -172,6 +172,10 @@ def _ptr(x, dtype, n=None, name="array"):
         raise TypeError(f"{name}: not contiguous")
     if n is not None and x.numel() < n:
         raise ValueError(f"{name}: needs {n} entries, has {x.numel()}")
+    if x.is_cuda:
+        # the library works on its own stream: work torch has queued on this
+        # tensor (a pending clone, fill or read) must finish first
+        torch.cuda.current_stream(x.device).synchronize()
     return x.data_ptr()
 
 

@@ -14,6 +14,9 @@
  *     (resolved with hipPointerGetAttributes).  Device outputs are written by
  *     the kernels in place; host outputs are staged through device memory.
  *     Calls return after the result is complete (the stream is synchronised).
+ *     The handle's stream is its own: work the caller queued on other
+ *     streams that reads or writes a buffer passed in must be complete
+ *     before the call.
  *   - Component ids are canonical: a component is named by its minimum
  *     vertex id.  (The reference names WCCs by union-find root and SCCs by
  *     Tarjan emission index, gir.cpp:649-652 and 809-816; its own tests
@@ -156,7 +159,10 @@ int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
  *   "bfs_alpha" (15), "bfs_beta" (18), "bfs_min_reach" (4096),
  *   "sssp_delta" (16), "block" (256), "wcc_rounds" (2),
  *   "scc_trim_rounds" (3), "scc_pivot" (1 = max outdeg*indeg, 0 = none),
- *   "scc_color_min" (0: vertices left before colouring replaces FW-BW)
+ *   "scc_color_min" (0: vertices left before colouring replaces FW-BW),
+ *   "msbfs" (1: k > 1 BFS sources share bit-parallel sweeps of 64),
+ *   "chunk" (16: most BFS levels / SSSP rounds queued per host readback
+ *   while frontiers are tiny, 1..16), "profile" (0: 1 times every kernel)
  * Values only change work order; results never depend on them. */
 int girhip_set_param(girhip_graph* h, const char* name, double value);
 int girhip_get_param(const girhip_graph* h, const char* name, double* value);
