@@ -114,120 +114,167 @@ __global__ void kTransposeKeys(const uint64_t* __restrict__ in, int64_t n, int B
     if (idx) idx[i] = (uint32_t)i;
 }
 
-__global__ void kRowCount(const uint64_t* __restrict__ keys, int64_t n, int B, int32_t* cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[(int32_t)(keys[i] >> B)], 1);
+// ---- CSR merge ------------------------------------------------------------
+// The CSR of one direction is the globally sorted key sequence (row, col).
+// With the sorted change lists, old slot j moves to j - D(j) + I(j), where
+// D(j) = deleted old slots before j and I(j) = inserted keys whose insertion
+// point (first old slot with a larger key) is <= j; insert k lands at
+// p_k - D(p_k) + k.  Row offsets follow from key counts: new beg[u] =
+// beg[u] - #deletes in rows < u + #inserts in rows < u.
+
+// Old-array position of every change key: deletes at their slot (which
+// must hold the key), inserts at their insertion point (which must not).
+__global__ void kChangePos(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj, int B,
+                           const uint64_t* __restrict__ keys, int64_t n, bool del, int64_t* __restrict__ pos,
+                           int64_t* err) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint64_t key = keys[k];
+    const int32_t u = (int32_t)(key >> B), x = (int32_t)(key & ((1ULL << B) - 1));
+    const int32_t p = lowerBoundI(adj, beg[u], beg[u + 1], x);
+    const bool hit = p < beg[u + 1] && adj[p] == x;
+    if (hit != del) atomicAdd((unsigned long long*)err, 1ULL);
+    pos[k] = p;
 }
 
-__global__ void kNewDeg(const int32_t* __restrict__ beg, const int32_t* __restrict__ ic, const int32_t* __restrict__ dc,
-                        int32_t V, int32_t* __restrict__ nd) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V) nd[v] = beg[v + 1] - beg[v] + ic[v] - dc[v];
+// New row offsets (int64, V + 1 entries).
+__global__ void kNewBeg(const int32_t* __restrict__ beg, int32_t V, int B, const uint64_t* __restrict__ ik,
+                        int64_t ni, const uint64_t* __restrict__ dk, int64_t nd, int64_t* __restrict__ nb) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u > V) return;
+    const uint64_t key = (uint64_t)u << B;
+    nb[u] = (int64_t)beg[u] - lowerBoundKey(dk, 0, nd, key) + lowerBoundKey(ik, 0, ni, key);
 }
 
-// Old slots -> new positions (deleted slots dropped).
-__global__ void kMergeOld(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj,
-                          const uint32_t* __restrict__ w, int32_t V, int B, const int64_t* __restrict__ ib,
-                          const uint64_t* __restrict__ ik, const int64_t* __restrict__ db,
-                          const uint64_t* __restrict__ dk, const int64_t* __restrict__ nb, int32_t* __restrict__ nadj,
-                          uint32_t* __restrict__ nw, int64_t* err) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = v < V;
-    const int32_t b = act ? beg[v] : 0, e = act ? beg[v + 1] : 0;
-    waveRows(act, (int32_t)v, b, e, [&](int32_t u, int64_t j) {
-        const int64_t i0 = ib[u], i1 = ib[u + 1], d0 = db[u], d1 = db[u + 1];
-        const int32_t x = adj[j];
-        int64_t pos = nb[u] + (j - beg[u]);
-        if (i0 != i1 || d0 != d1) {
-            const uint64_t key = ((uint64_t)(uint32_t)u << B) | (uint32_t)x;
-            const int64_t dpos = lowerBoundKey(dk, d0, d1, key);
-            if (dpos < d1 && dk[dpos] == key) return;  // deleted
-            pos += (lowerBoundKey(ik, i0, i1, key) - i0) - (dpos - d0);
-        }
-        if (pos < nb[u] || pos >= nb[u + 1]) {  // (an inconsistent store; never written out of the row)
-            atomicAdd((unsigned long long*)err, 1ULL);
+__device__ __forceinline__ int64_t countBelow(const int64_t* __restrict__ a, int64_t lo, int64_t hi, int64_t x) {
+    while (lo < hi) {  // first index in [lo, hi) with a[index] >= x
+        const int64_t m = lo + ((hi - lo) >> 1);
+        if (a[m] < x) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+constexpr int MERGE_TILE = 4096;
+
+// Old slots, one tile of MERGE_TILE per block: a tile without change
+// positions is a shifted copy; otherwise each slot counts the changes in
+// the tile's range that precede it.
+__global__ void kShiftCopy(const int32_t* __restrict__ adj, const uint32_t* __restrict__ w, int64_t E,
+                           const int64_t* __restrict__ ipos, int64_t ni, const int64_t* __restrict__ dpos, int64_t nd,
+                           int32_t* __restrict__ nadj, uint32_t* __restrict__ nw, int64_t newE, int64_t* err) {
+    __shared__ int64_t rng[4];
+    const int64_t t0 = (int64_t)blockIdx.x * MERGE_TILE;
+    if (t0 >= E) return;
+    const int64_t t1 = t0 + MERGE_TILE < E ? t0 + MERGE_TILE : E;
+    if (threadIdx.x == 0) {
+        rng[0] = countBelow(ipos, 0, ni, t0 + 1);  // inserts with point <= t0
+        rng[1] = countBelow(ipos, 0, ni, t1);      // ... with point <= t1 - 1
+        rng[2] = countBelow(dpos, 0, nd, t0);      // deleted slots < t0
+        rng[3] = countBelow(dpos, 0, nd, t1);      // ... < t1
+    }
+    __syncthreads();
+    const int64_t i0 = rng[0], i1 = rng[1], d0 = rng[2], d1 = rng[3];
+    if (i0 == i1 && d0 == d1) {
+        const int64_t sh = i0 - d0;
+        if (t0 + sh < 0 || t1 + sh > newE) {  // (inconsistent change lists; never written out of range)
+            if (threadIdx.x == 0) atomicAdd((unsigned long long*)err, 1ULL);
             return;
         }
-        nadj[pos] = x;
-        if (nw) nw[pos] = w[j];
-    });
+        for (int64_t j = t0 + threadIdx.x; j < t1; j += blockDim.x) {
+            nadj[j + sh] = adj[j];
+            if (nw) nw[j + sh] = w[j];
+        }
+        return;
+    }
+    for (int64_t j = t0 + threadIdx.x; j < t1; j += blockDim.x) {
+        const int64_t dj = countBelow(dpos, d0, d1, j);
+        if (dj < d1 && dpos[dj] == j) continue;  // deleted
+        const int64_t ij = countBelow(ipos, i0, i1, j + 1);
+        const int64_t q = j - dj + ij;
+        if (q < 0 || q >= newE) {
+            atomicAdd((unsigned long long*)err, 1ULL);
+            continue;
+        }
+        nadj[q] = adj[j];
+        if (nw) nw[q] = w[j];
+    }
 }
 
-__global__ void kMergeIns(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj, int B,
-                          const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iw, int64_t nins,
-                          const int64_t* __restrict__ ib, const int64_t* __restrict__ db,
-                          const uint64_t* __restrict__ dk, const int64_t* __restrict__ nb, int32_t* __restrict__ nadj,
-                          uint32_t* __restrict__ nw, int64_t* err) {
+__global__ void kInsertPlace(const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iw, int64_t ni, int B,
+                             const int64_t* __restrict__ ipos, const int64_t* __restrict__ dpos, int64_t nd,
+                             int32_t* __restrict__ nadj, uint32_t* __restrict__ nw, int64_t newE, int64_t* err) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nins) return;
-    const uint64_t key = ik[k];
-    const int32_t u = (int32_t)(key >> B), x = (int32_t)(key & ((1ULL << B) - 1));
-    const int32_t before_old = lowerBoundI(adj, beg[u], beg[u + 1], x) - beg[u];
-    const int64_t before_del = lowerBoundKey(dk, db[u], db[u + 1], key) - db[u];
-    const int64_t pos = nb[u] + (k - ib[u]) + before_old - before_del;
-    if (pos < nb[u] || pos >= nb[u + 1]) {
+    if (k >= ni) return;
+    const int64_t p = ipos[k];
+    const int64_t q = p - countBelow(dpos, 0, nd, p) + k;
+    if (q < 0 || q >= newE) {
         atomicAdd((unsigned long long*)err, 1ULL);
         return;
     }
-    nadj[pos] = x;
-    if (nw) nw[pos] = iw[k];
+    nadj[q] = (int32_t)(ik[k] & ((1ULL << B) - 1));
+    if (nw) nw[q] = iw[k];
 }
 
+}  // namespace gir
+
+namespace gir {
 namespace {
 
+// Sorted change lists of one CSR direction (buffers live in the handle's
+// scratch).
 struct Changes {
     int64_t ni = 0, nd = 0;
-    DevBuf<uint64_t> ik, dk;  // sorted packed (row, col) keys
-    DevBuf<uint32_t> iw;      // weights of inserted arcs (fw only)
+    uint64_t* ik = nullptr;  // sorted packed (row, col) keys of inserted arcs
+    uint64_t* dk = nullptr;  // ... of deleted arcs
+    uint32_t* iw = nullptr;  // weights of inserted arcs (fw only)
 };
 
-// Apply sorted change lists to one CSR (beg/adj/w), producing new arrays.
-void mergeCsr(girhip_graph* h, DevBuf<int32_t>& beg, DevBuf<int32_t>& adj, DevBuf<uint32_t>* w, const Changes& C,
-              int B, int64_t newE) {
+// Apply sorted change lists to one CSR (beg/adj/w), merging into the spare
+// arrays (adj2/w2), which are then swapped with the live ones.
+void mergeCsr(girhip_graph* h, DevBuf<int32_t>& beg, DevBuf<int32_t>& adj, DevBuf<int32_t>& adj2,
+              DevBuf<uint32_t>* w, DevBuf<uint32_t>* w2, const Changes& C, int B, int64_t newE) {
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
-    DevBuf<int32_t> ic, dc, nd;
-    ic.ensure((size_t)V + 1);
-    dc.ensure((size_t)V + 1);
-    nd.ensure((size_t)V + 1);
-    fillI32(h, ic.p, 0, (int64_t)V + 1);
-    fillI32(h, dc.p, 0, (int64_t)V + 1);
-    if (C.ni) launch(h, "batch_rowcount", kRowCount, gridFor(C.ni, blk), blk, 0, (const uint64_t*)C.ik.p, C.ni, B, ic.p);
-    if (C.nd) launch(h, "batch_rowcount", kRowCount, gridFor(C.nd, blk), blk, 0, (const uint64_t*)C.dk.p, C.nd, B, dc.p);
-    DevBuf<int64_t> ib, db, nb;
-    ib.ensure((size_t)V + 1);
-    db.ensure((size_t)V + 1);
-    nb.ensure((size_t)V + 1);
-    scanI32toI64(h, ic.p, ib.p, V);
-    scanI32toI64(h, dc.p, db.p, V);
-    launch(h, "batch_newdeg", kNewDeg, gridFor(V, blk), blk, 0, (const int32_t*)beg.p, (const int32_t*)ic.p,
-           (const int32_t*)dc.p, V, nd.p);
-    scanI32toI64(h, nd.p, nb.p, V);
-    DevBuf<int32_t> nadj;
-    nadj.ensure((size_t)(newE > 0 ? newE : 1));
-    DevBuf<uint32_t> nw;
-    if (w) nw.ensure((size_t)(newE > 0 ? newE : 1));
+    Scratch& S = scratch(h);
+    S.b_pi.ensureSlack((size_t)(C.ni > 0 ? C.ni : 1));
+    S.b_pd.ensureSlack((size_t)(C.nd > 0 ? C.nd : 1));
+    S.b_nb.ensure((size_t)V + 1);
+    int64_t *ipos = S.b_pi.p, *dpos = S.b_pd.p, *nb = S.b_nb.p;
     int64_t* err = h->dcnt.p + 48;
-    fillI64(h, err, 0, 2);
-    launch(h, "batch_merge_old", kMergeOld, gridFor(V, blk), blk, 0, (const int32_t*)beg.p, (const int32_t*)adj.p,
-           (const uint32_t*)(w ? w->p : nullptr), V, B, (const int64_t*)ib.p, (const uint64_t*)C.ik.p,
-           (const int64_t*)db.p, (const uint64_t*)C.dk.p, (const int64_t*)nb.p, nadj.p, w ? nw.p : (uint32_t*)nullptr,
-           err);
+    fillI64(h, err, 0, 1);
     if (C.ni)
-        launch(h, "batch_merge_ins", kMergeIns, gridFor(C.ni, blk), blk, 0, (const int32_t*)beg.p,
-               (const int32_t*)adj.p, B, (const uint64_t*)C.ik.p, (const uint32_t*)(w ? C.iw.p : nullptr), C.ni,
-               (const int64_t*)ib.p, (const int64_t*)db.p, (const uint64_t*)C.dk.p, (const int64_t*)nb.p, nadj.p,
-               w ? nw.p : (uint32_t*)nullptr, err);
-    // the new row sizes must add up to the new arc count
-    HIPCHK(hipMemcpyAsync(err + 1, nb.p + V, sizeof(int64_t), hipMemcpyDeviceToDevice, h->st));
+        launch(h, "batch_pos", kChangePos, gridFor(C.ni, blk), blk, 0, (const int32_t*)beg.p, (const int32_t*)adj.p, B,
+               (const uint64_t*)C.ik, C.ni, false, ipos, err);
+    if (C.nd)
+        launch(h, "batch_pos", kChangePos, gridFor(C.nd, blk), blk, 0, (const int32_t*)beg.p, (const int32_t*)adj.p, B,
+               (const uint64_t*)C.dk, C.nd, true, dpos, err);
+    launch(h, "batch_newbeg", kNewBeg, gridFor((int64_t)V + 1, blk), blk, 0, (const int32_t*)beg.p, V, B,
+           (const uint64_t*)C.ik, C.ni, (const uint64_t*)C.dk, C.nd, nb);
+    // the change lists must match the store before anything is moved
+    int64_t e1;
+    readCounters(h, err, 1, &e1);
+    if (e1 != 0) raise(GIRHIP_EHIP, "batch: change lists do not match the edge store");
+    adj2.ensureSlack((size_t)(newE > 0 ? newE : 1));
+    if (w) w2->ensureSlack((size_t)(newE > 0 ? newE : 1));
+    const int64_t E = h->E;
+    if (E > 0)
+        launch(h, "batch_shift_copy", kShiftCopy, gridFor(E, MERGE_TILE), blk, 0, (const int32_t*)adj.p,
+               (const uint32_t*)(w ? w->p : nullptr), E, (const int64_t*)ipos, C.ni, (const int64_t*)dpos, C.nd,
+               adj2.p, w ? w2->p : (uint32_t*)nullptr, newE, err);
+    if (C.ni)
+        launch(h, "batch_insert", kInsertPlace, gridFor(C.ni, blk), blk, 0, (const uint64_t*)C.ik,
+               (const uint32_t*)(w ? C.iw : nullptr), C.ni, B, (const int64_t*)ipos, (const int64_t*)dpos, C.nd,
+               adj2.p, w ? w2->p : (uint32_t*)nullptr, newE, err);
+    // nothing out of range, and the rows must add up
+    HIPCHK(hipMemcpyAsync(err + 1, nb + V, sizeof(int64_t), hipMemcpyDeviceToDevice, h->st));
     int64_t e2[2];
     readCounters(h, err, 2, e2);
     if (e2[0] != 0 || e2[1] != newE) raise(GIRHIP_EHIP, "batch: internal merge inconsistency");
-    launch(h, "batch_beg", kI64toI32, gridFor((int64_t)V + 1, blk), blk, 0, (const int64_t*)nb.p, beg.p,
+    launch(h, "batch_beg", kI64toI32, gridFor((int64_t)V + 1, blk), blk, 0, (const int64_t*)nb, beg.p,
            (int64_t)V + 1);
-    syncStream(h);
-    adj.swap(nadj);
-    if (w) w->swap(nw);
+    adj.swap(adj2);
+    if (w) w->swap(*w2);
 }
 
 // Normalise + classify the batch; fills C (fw order) and the stats.
@@ -236,38 +283,34 @@ void normalizeBatchDev(girhip_graph* h, const int32_t* u, const int32_t* v, cons
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
     Scratch& S = scratch(h);
-    DevBuf<int32_t> du, dv;
-    DevBuf<uint8_t> dins;
-    DevBuf<uint32_t> dw;
-    du.ensure((size_t)n);
-    dv.ensure((size_t)n);
-    dins.ensure((size_t)n);
-    copyIn(h, du.p, u, sizeof(int32_t) * n);
-    copyIn(h, dv.p, v, sizeof(int32_t) * n);
-    copyIn(h, dins.p, ins, (size_t)n);
-    if (w && h->has_w) {
-        dw.ensure((size_t)n);
-        copyIn(h, dw.p, w, sizeof(uint32_t) * n);
+    const bool ww = w && h->has_w;
+    S.b_u.ensureSlack((size_t)n);
+    S.b_v.ensureSlack((size_t)n);
+    S.b_ins.ensureSlack((size_t)n);
+    copyIn(h, S.b_u.p, u, sizeof(int32_t) * n);
+    copyIn(h, S.b_v.p, v, sizeof(int32_t) * n);
+    copyIn(h, S.b_ins.p, ins, (size_t)n);
+    if (ww) {
+        S.b_w.ensureSlack((size_t)n);
+        copyIn(h, S.b_w.p, w, sizeof(uint32_t) * n);
     }
     S.keys[0].ensure((size_t)n);
     S.vals[0].ensure((size_t)n);
     fillI64(h, h->dcnt.p + 40, 0, 4);
-    launch(h, "batch_keys", kBatchKeys, gridFor(n, blk), blk, 0, (const int32_t*)du.p, (const int32_t*)dv.p, n, V, B,
-           S.keys[0].p, S.vals[0].p, h->dcnt.p + 40);
+    launch(h, "batch_keys", kBatchKeys, gridFor(n, blk), blk, 0, (const int32_t*)S.b_u.p, (const int32_t*)S.b_v.p, n,
+           V, B, S.keys[0].p, S.vals[0].p, h->dcnt.p + 40);
     radixSort(h, S.keys[0].p, S.vals[0].p, n, 2 * B + 1);
-    DevBuf<int32_t> fi, fd;
-    fi.ensure((size_t)n);
-    fd.ensure((size_t)n);
+    S.b_fi.ensureSlack((size_t)n);
+    S.b_fd.ensureSlack((size_t)n);
     launch(h, "batch_classify", kBatchClassify, gridFor(n, blk), blk, 0, (const uint64_t*)S.keys[0].p,
-           (const uint32_t*)S.vals[0].p, n, B, (const uint8_t*)dins.p, (const int32_t*)h->fw_beg.p,
-           (const int32_t*)h->fw_adj.p, fi.p, fd.p, h->dcnt.p + 41);
-    DevBuf<int64_t> pi, pd;
-    pi.ensure((size_t)n + 1);
-    pd.ensure((size_t)n + 1);
-    scanI32toI64(h, fi.p, pi.p, n);
-    scanI32toI64(h, fd.p, pd.p, n);
-    HIPCHK(hipMemcpyAsync(h->dcnt.p + 42, pi.p + n, sizeof(int64_t), hipMemcpyDeviceToDevice, h->st));
-    HIPCHK(hipMemcpyAsync(h->dcnt.p + 43, pd.p + n, sizeof(int64_t), hipMemcpyDeviceToDevice, h->st));
+           (const uint32_t*)S.vals[0].p, n, B, (const uint8_t*)S.b_ins.p, (const int32_t*)h->fw_beg.p,
+           (const int32_t*)h->fw_adj.p, S.b_fi.p, S.b_fd.p, h->dcnt.p + 41);
+    S.b_pi.ensureSlack((size_t)n + 1);
+    S.b_pd.ensureSlack((size_t)n + 1);
+    scanI32toI64(h, S.b_fi.p, S.b_pi.p, n);
+    scanI32toI64(h, S.b_fd.p, S.b_pd.p, n);
+    HIPCHK(hipMemcpyAsync(h->dcnt.p + 42, S.b_pi.p + n, sizeof(int64_t), hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->dcnt.p + 43, S.b_pd.p + n, sizeof(int64_t), hipMemcpyDeviceToDevice, h->st));
     int64_t c[4];
     readCounters(h, h->dcnt.p + 40, 4, c);
     st.invalid = c[0];
@@ -276,34 +319,43 @@ void normalizeBatchDev(girhip_graph* h, const int32_t* u, const int32_t* v, cons
     C.nd = c[3];
     st.inserted = C.ni;
     st.deleted = C.nd;
-    C.ik.ensure((size_t)(C.ni > 0 ? C.ni : 1));
-    C.dk.ensure((size_t)(C.nd > 0 ? C.nd : 1));
-    if (h->has_w) C.iw.ensure((size_t)(C.ni > 0 ? C.ni : 1));
+    S.b_ik[0].ensureSlack((size_t)(C.ni > 0 ? C.ni : 1));
+    S.b_dk[0].ensureSlack((size_t)(C.nd > 0 ? C.nd : 1));
+    C.ik = S.b_ik[0].p;
+    C.dk = S.b_dk[0].p;
+    C.iw = nullptr;
+    if (h->has_w) {
+        S.b_iw.ensureSlack((size_t)(C.ni > 0 ? C.ni : 1));
+        C.iw = S.b_iw.p;
+    }
     launch(h, "batch_compact", kBatchCompact, gridFor(n, blk), blk, 0, (const uint64_t*)S.keys[0].p,
-           (const uint32_t*)S.vals[0].p, n, B, (const int32_t*)fi.p, (const int64_t*)pi.p, (const uint32_t*)dw.p,
-           !(w && h->has_w), C.ik.p, h->has_w ? C.iw.p : (uint32_t*)nullptr);
+           (const uint32_t*)S.vals[0].p, n, B, (const int32_t*)S.b_fi.p, (const int64_t*)S.b_pi.p,
+           (const uint32_t*)(ww ? S.b_w.p : nullptr), !ww, C.ik, C.iw);
     launch(h, "batch_compact", kBatchCompact, gridFor(n, blk), blk, 0, (const uint64_t*)S.keys[0].p,
-           (const uint32_t*)S.vals[0].p, n, B, (const int32_t*)fd.p, (const int64_t*)pd.p, (const uint32_t*)nullptr,
-           true, C.dk.p, (uint32_t*)nullptr);
-    syncStream(h);
+           (const uint32_t*)S.vals[0].p, n, B, (const int32_t*)S.b_fd.p, (const int64_t*)S.b_pd.p,
+           (const uint32_t*)nullptr, true, C.dk, (uint32_t*)nullptr);
 }
 
 // Transposed change lists (sorted by (col, row)) for the bw CSR.
 void transposeChanges(girhip_graph* h, const Changes& C, Changes& T, int B) {
     const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
     T.ni = C.ni;
     T.nd = C.nd;
-    T.ik.ensure((size_t)(C.ni > 0 ? C.ni : 1));
-    T.dk.ensure((size_t)(C.nd > 0 ? C.nd : 1));
+    S.b_ik[1].ensureSlack((size_t)(C.ni > 0 ? C.ni : 1));
+    S.b_dk[1].ensureSlack((size_t)(C.nd > 0 ? C.nd : 1));
+    T.ik = S.b_ik[1].p;
+    T.dk = S.b_dk[1].p;
+    T.iw = nullptr;
     if (C.ni) {
-        launch(h, "batch_tkeys", kTransposeKeys, gridFor(C.ni, blk), blk, 0, (const uint64_t*)C.ik.p, C.ni, B, T.ik.p,
+        launch(h, "batch_tkeys", kTransposeKeys, gridFor(C.ni, blk), blk, 0, (const uint64_t*)C.ik, C.ni, B, T.ik,
                (uint32_t*)nullptr);
-        radixSort(h, T.ik.p, nullptr, C.ni, 2 * B);
+        radixSort(h, T.ik, nullptr, C.ni, 2 * B);
     }
     if (C.nd) {
-        launch(h, "batch_tkeys", kTransposeKeys, gridFor(C.nd, blk), blk, 0, (const uint64_t*)C.dk.p, C.nd, B, T.dk.p,
+        launch(h, "batch_tkeys", kTransposeKeys, gridFor(C.nd, blk), blk, 0, (const uint64_t*)C.dk, C.nd, B, T.dk,
                (uint32_t*)nullptr);
-        radixSort(h, T.dk.p, nullptr, C.nd, 2 * B);
+        radixSort(h, T.dk, nullptr, C.nd, 2 * B);
     }
 }
 
@@ -319,8 +371,9 @@ void applyStore(girhip_graph* h, const int32_t* u, const int32_t* v, const uint8
     if (newE >= 0x7fffffffLL) raise(GIRHIP_EINVAL, "batch: arc count would exceed int32 offsets");
     Changes T;
     transposeChanges(h, C, T, B);
-    mergeCsr(h, h->fw_beg, h->fw_adj, h->has_w ? &h->w_fw : nullptr, C, B, newE);
-    mergeCsr(h, h->bw_beg, h->bw_adj, nullptr, T, B, newE);
+    mergeCsr(h, h->fw_beg, h->fw_adj, h->fw_adj2, h->has_w ? &h->w_fw : nullptr, h->has_w ? &h->w_fw2 : nullptr, C,
+             B, newE);
+    mergeCsr(h, h->bw_beg, h->bw_adj, h->bw_adj2, nullptr, nullptr, T, B, newE);
     h->E = newE;
     h->src_ok = false;
     h->max_deg = -1;
@@ -333,7 +386,7 @@ void repairLayers(girhip_graph* h, const Changes* C, int B) {
     const bool changed = C && (C->ni + C->nd) > 0;
     if (h->wcc_req) {
         if (h->wcc_ok && C && C->nd == 0) {
-            if (C->ni) runWccHookArcs(h, C->ik.p, C->ni, B);  // insert-only: hook the new arcs
+            if (C->ni) runWccHookArcs(h, C->ik, C->ni, B);  // insert-only: hook the new arcs
         } else if (!h->wcc_ok || changed) {
             runWcc(h);
         }

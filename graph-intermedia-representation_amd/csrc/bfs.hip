@@ -406,17 +406,18 @@ extern "C" int girhip_bfs(girhip_graph* h, const int32_t* src, int32_t k, int32_
         }
         ReachBufs B = reachBufs(h);
         const bool bitpar = k > 1 && h->prm.msbfs != 0;
-        DevBuf<int32_t> dsrc;
+        int32_t* dsrc = nullptr;
         if (bitpar) {
-            dsrc.ensure((size_t)k);
-            copyIn(h, dsrc.p, s.data(), sizeof(int32_t) * (size_t)k);
+            S.src.ensure((size_t)k);
+            dsrc = S.src.p;
+            copyIn(h, dsrc, s.data(), sizeof(int32_t) * (size_t)k);
             syncStream(h);
         }
         tBegin(h);
         if (bitpar) {
             for (int32_t i0 = 0; i0 < k; i0 += 64) {
                 const int c = std::min<int32_t>(64, k - i0);
-                msbfs(h, dsrc.p + i0, c, out + (size_t)i0 * V);
+                msbfs(h, dsrc + i0, c, out + (size_t)i0 * V);
             }
         } else {
             fillI32(h, out, -1, (int64_t)n);

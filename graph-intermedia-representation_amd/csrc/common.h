@@ -49,6 +49,12 @@ struct DevBuf {
         }
         n = want;
     }
+    // like ensure, but a grown buffer gets 1/16 headroom (arrays that grow a
+    // little per update batch)
+    void ensureSlack(size_t want) {
+        if (want <= n && p) return;
+        ensure(want + want / 16 + 64);
+    }
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -97,6 +103,20 @@ struct Scratch {
     DevBuf<int64_t> mscnt;   // bit-parallel BFS: per-level counter ring
     DevBuf<uint64_t> msbm[4];  // bit-parallel BFS: frontier / touched bitmaps and summaries
     DevBuf<int64_t> rcnt;    // reachability engine: per-level counter ring
+    // update batches: per-op arrays (grown to the largest batch; b_pi / b_pd
+    // also hold the change positions of the CSR merge) and the new offsets
+    DevBuf<int32_t> b_u, b_v, b_fi, b_fd;
+    DevBuf<uint8_t> b_ins;
+    DevBuf<uint32_t> b_w;
+    DevBuf<int64_t> b_pi, b_pd;
+    DevBuf<int64_t> b_nb;
+    DevBuf<uint64_t> b_ik[2], b_dk[2];  // sorted change keys: fw order, bw order
+    DevBuf<uint32_t> b_iw;
+    // delta-stepping batch state (sized for the largest source batch)
+    DevBuf<int32_t> s_q[2], s_qh;
+    DevBuf<uint64_t> s_near, s_far, s_done, s_nsum, s_fsum, s_dsum;
+    DevBuf<int64_t> s_ctl;
+    DevBuf<int32_t> src;  // query sources
     // heavy-row list
     DevBuf<int64_t> hv_ctl, hv_b, hv_e, hv_off;
     DevBuf<int32_t> hv_row, hv_aux;
@@ -115,6 +135,10 @@ struct girhip_graph {
     int64_t E = 0;
     gir::DevBuf<int32_t> fw_beg, fw_adj, bw_beg, bw_adj;
     gir::DevBuf<uint32_t> w_fw;  // weights per fw slot (valid if has_w)
+    // spare adjacency / weight arrays: an update batch merges into these and
+    // swaps them with the live ones
+    gir::DevBuf<int32_t> fw_adj2, bw_adj2;
+    gir::DevBuf<uint32_t> w_fw2;
     bool has_w = false;
     gir::DevBuf<int32_t> fw_src;  // row of every fw slot (COO), built on demand
     bool src_ok = false;

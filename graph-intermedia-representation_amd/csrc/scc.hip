@@ -291,19 +291,44 @@ __global__ void kRankSeeds(const int32_t* __restrict__ scc, const int32_t* __res
     if (seed) out[pos] = (int32_t)v;
 }
 
+// Kahn step work on DAG arc j: one in-degree decrement of its head; the
+// decrement that reaches zero queues the head.  Arcs into the hub (the SCC
+// with the most DAG in-arcs, the giant component of a power-law graph) are
+// summed per wave first, so the hub's counter takes one atomic per wave
+// instead of one per arc.
 struct KahnF {
     const int32_t* ddst;
     int32_t* indeg;
     int32_t* qout;
     int64_t* cnt;
+    int32_t hub;
     __device__ int64_t operator()(int32_t, uint64_t, int32_t, int64_t j) const {
         const int32_t t = ddst[j];
-        const bool ready = atomicSub(&indeg[t], 1) == 1;
+        const bool hot = t == hub;
+        const uint64_t hm = __ballot(hot);
+        bool ready = false;
+        if (hot) {
+            if (laneId() == __ffsll((long long)hm) - 1) {
+                const int c = __popcll(hm);
+                ready = atomicSub(&indeg[t], c) == c;
+            }
+        } else {
+            ready = atomicSub(&indeg[t], 1) == 1;
+        }
         const int64_t pos = wavePush(cnt, ready);
         if (ready) qout[pos] = t;
         return 0;
     }
 };
+
+// hub = the id with the largest in-degree (packed (indeg, id) maximum)
+__global__ void kHub(const int32_t* __restrict__ indeg, int32_t V, int64_t* out) {
+    unsigned long long m = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
+        m = std::max<unsigned long long>(m, ((unsigned long long)(uint32_t)indeg[v] << 32) | (uint32_t)v);
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = std::max<unsigned long long>(m, __shfl_xor(m, o));
+    if (laneId() == 0 && m) atomicMax((unsigned long long*)out, m);
+}
 
 __global__ void kKahnStep(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ dbeg,
                           int64_t* rank, int32_t L, KahnF f, HeavyList H) {
@@ -485,6 +510,11 @@ void runDag(girhip_graph* h) {
            h->dag_beg.p);
     // Kahn levels
     ReachBufs R = reachBufs(h);
+    fillI64(h, h->dcnt.p + 12, 0, 1);
+    launch(h, "dag_hub", kHub, boundedGrid(V, blk), blk, 0, (const int32_t*)ideg, V, h->dcnt.p + 12);
+    int64_t hubk;
+    readCounters(h, h->dcnt.p + 12, 1, &hubk);
+    const int32_t hub = hubk ? (int32_t)(hubk & 0xffffffffLL) : -1;
     fillI64(h, h->dcnt.p + 10, 0, 1);
     launch(h, "rank_seeds", kRankSeeds, gridFor(V, blk), blk, 0, (const int32_t*)h->scc.p, (const int32_t*)ideg, V,
            h->rank.p, R.q[0], h->dcnt.p + 10);
@@ -494,7 +524,7 @@ void runDag(girhip_graph* h) {
     int32_t L = 0;
     while (nq > 0) {
         fillI64(h, h->dcnt.p + 11, 0, 1);
-        const KahnF f{h->dag_dst.p, ideg, R.q[cur ^ 1], h->dcnt.p + 11};
+        const KahnF f{h->dag_dst.p, ideg, R.q[cur ^ 1], h->dcnt.p + 11, hub};
         launch(h, "rank_kahn", kKahnStep, gridFor(nq, blk), blk, 0, (const int32_t*)R.q[cur], nq,
                (const int32_t*)h->dag_beg.p, h->rank.p, L, f, heavyList(h));
         launchHeavy(h, "rank_kahn_heavy", f);

@@ -323,19 +323,16 @@ __global__ void kReachedArcsD(const uint64_t* __restrict__ dist, const int32_t* 
     blockAtomicAdd(m, out);
 }
 
-struct SsspBufs {
-    DevBuf<int32_t> qn[2], qh;
-    DevBuf<uint64_t> nearbits, farbits, donebits, nsum, fsum, dsum;
-    DevBuf<int64_t> ctl;
-};
-
 // Rounds per host readback while every queue is small (high-diameter
 // graphs): all round kernels read their work sizes on the device, and a
 // round with nothing left to do is a no-op.
 constexpr int SSSP_CHUNK = 16;
 
 // k <= SB sources (device array dsrc) into rows of dist.
-static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, uint64_t* dist, SsspBufs& Bf) {
+// Batch state lives in the handle's scratch (s_* buffers, sized for SB
+// sources by the caller).
+static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, uint64_t* dist) {
+    Scratch& S = scratch(h);
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
     const uint32_t* w = h->has_w ? h->w_fw.p : nullptr;
@@ -343,15 +340,15 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
     B.V = V;
     B.W = ((int64_t)V + 63) / 64;
     B.W2 = (B.W + 63) / 64;
-    for (int p = 0; p < 2; ++p) B.qn[p] = Bf.qn[p].p;
-    B.qh = Bf.qh.p;
-    B.nearbits = Bf.nearbits.p;
-    B.farbits = Bf.farbits.p;
-    B.donebits = Bf.donebits.p;
-    B.nsum = Bf.nsum.p;
-    B.fsum = Bf.fsum.p;
-    B.dsum = Bf.dsum.p;
-    B.ctl = Bf.ctl.p;
+    for (int p = 0; p < 2; ++p) B.qn[p] = S.s_q[p].p;
+    B.qh = S.s_qh.p;
+    B.nearbits = S.s_near.p;
+    B.farbits = S.s_far.p;
+    B.donebits = S.s_done.p;
+    B.nsum = S.s_nsum.p;
+    B.fsum = S.s_fsum.p;
+    B.dsum = S.s_dsum.p;
+    B.ctl = S.s_ctl.p;
     B.dist = dist;
     fillU64(h, dist, INF, (int64_t)k * V);
     fillU64(h, B.nearbits, 0, (int64_t)k * B.W);
@@ -392,7 +389,7 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
             launch(h, "sssp_split", kBSplit, dim3(sgrid, k), blk, 0, B, par, delta, G);
             par ^= 1;
         }
-        readCounters(h, Bf.ctl.p + (par ? C_NEAR1 : C_NEAR0), SB, r.data());
+        readCounters(h, S.s_ctl.p + (par ? C_NEAR1 : C_NEAR0), SB, r.data());
         int64_t mx = 0;
         for (int s = 0; s < k; ++s) mx = std::max(mx, r[s]);
         if (mx == 0) break;  // every queue empty after the advance: all sources settled
@@ -425,24 +422,23 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         }
         const size_t Vs = (size_t)(V > 0 ? V : 1), Ws = (Vs + 63) / 64;
         const int kc = std::max(1, std::min<int32_t>(k, SB));
-        SsspBufs Bf;
-        for (int p = 0; p < 2; ++p) Bf.qn[p].ensure((size_t)kc * Vs);
-        Bf.qh.ensure((size_t)kc * Vs);
-        Bf.nearbits.ensure((size_t)kc * Ws);
-        Bf.farbits.ensure((size_t)kc * Ws);
-        Bf.donebits.ensure((size_t)kc * Ws);
+        for (int p = 0; p < 2; ++p) S.s_q[p].ensure((size_t)kc * Vs);
+        S.s_qh.ensure((size_t)kc * Vs);
+        S.s_near.ensure((size_t)kc * Ws);
+        S.s_far.ensure((size_t)kc * Ws);
+        S.s_done.ensure((size_t)kc * Ws);
         const size_t W2s = (Ws + 63) / 64;
-        Bf.nsum.ensure((size_t)kc * W2s);
-        Bf.fsum.ensure((size_t)kc * W2s);
-        Bf.dsum.ensure((size_t)kc * W2s);
-        Bf.ctl.ensure(C_SLOTS);
-        DevBuf<int32_t> dsrc;
-        dsrc.ensure((size_t)(k > 0 ? k : 1));
-        copyIn(h, dsrc.p, s.data(), sizeof(int32_t) * (size_t)k);
+        S.s_nsum.ensure((size_t)kc * W2s);
+        S.s_fsum.ensure((size_t)kc * W2s);
+        S.s_dsum.ensure((size_t)kc * W2s);
+        S.s_ctl.ensure(C_SLOTS);
+        S.src.ensure((size_t)(k > 0 ? k : 1));
+        const int32_t* dsrc = S.src.p;
+        copyIn(h, S.src.p, s.data(), sizeof(int32_t) * (size_t)k);
         tBegin(h);
         for (int32_t i0 = 0; i0 < k; i0 += SB) {
             const int c = std::min<int32_t>(SB, k - i0);
-            ssspBatch(h, dsrc.p + i0, c, delta, out + (size_t)i0 * V, Bf);
+            ssspBatch(h, dsrc + i0, c, delta, out + (size_t)i0 * V);
         }
         tEnd(h, "sssp");
         if (edges_out) {
