@@ -378,6 +378,7 @@ void applyStore(girhip_graph* h, const int32_t* u, const int32_t* v, const uint8
     h->src_ok = false;
     h->max_deg = -1;
     h->max_w = -1;
+    h->sym_ok = false;
 }
 
 // Bring every requested layer up to date.  C = the changes just applied to a

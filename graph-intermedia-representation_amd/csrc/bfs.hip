@@ -82,6 +82,12 @@ struct MsCnt {
     int64_t nf, mf;
 };
 
+// Levels per host readback while the frontier is tiny (high-diameter graphs).
+constexpr int MS_CHUNK = 16;
+// Counter ring: one MsCnt per level, indexed by level mod MS_RING.
+constexpr int MS_RING = 64;
+static_assert(MS_RING >= MS_CHUNK + 2, "ring must hold a chunk and its neighbours");
+
 struct MsBits {
     uint64_t *F, *FS, *T, *TS;
     int64_t W, W2;
@@ -118,86 +124,134 @@ struct MsPushSF {
     }
 };
 
-// Walks F's summary: the 64 bitmap words under a non-zero summary word are
-// consumed (cleared) and every frontier vertex in them pushes its row with
-// its frontier word, which is cleared too.
-__global__ void kMsPushS(const int32_t* __restrict__ fb, uint64_t* front, MsPushSF f, MsCnt* nxt, HeavyList H,
-                         int G) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = MsCnt{0, 0};
+// Sparse push body for one summary word g of F: the 64 bitmap words under
+// it are consumed (cleared) and every frontier vertex in them pushes its row
+// with its frontier word, which is cleared too.
+__device__ __forceinline__ void msPushGroup(const int32_t* __restrict__ fb, uint64_t* front, const MsPushSF& f,
+                                            HeavyList H, int64_t g, uint64_t sm, int32_t* lds) {
     const MsBits& M = f.M;
-    forSummary(M.FS, M.W2, G, [&](int64_t g, uint64_t sm) {
-        const int l = laneId();
-        uint64_t m = 0;
-        if ((sm >> l) & 1) {
-            uint64_t* wp = M.F + g * 64 + l;
-            m = *wp;
-            if (m) *wp = 0;
+    const int l = laneId();
+    uint64_t m = 0;
+    if ((sm >> l) & 1) {
+        uint64_t* wp = M.F + g * 64 + l;
+        m = *wp;
+        if (m) *wp = 0;
+    }
+    if (l == 0) M.FS[g] = 0;
+    waveBits(m, g * 64 + l, lds, [&](bool act0, int32_t u) {
+        uint64_t fr = 0;
+        if (act0) {
+            fr = front[u];
+            front[u] = 0;
         }
-        if (l == 0) M.FS[g] = 0;
-        const int64_t base = (g * 64 + l) * 64;
-        while (__ballot(m != 0)) {
-            const bool act0 = m != 0;
-            int32_t u = 0;
-            if (act0) {
-                u = (int32_t)(base + __ffsll((long long)m) - 1);
-                m &= m - 1;
-            }
-            uint64_t fr = 0;
-            if (act0) {
-                fr = front[u];
-                front[u] = 0;
-            }
-            const bool act = fr != 0;
-            const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
-            waveRowsH(act, u, fr, 0, b, e, H, f);
-        }
+        const bool act = fr != 0;
+        const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+        waveRowsH(act, u, fr, 0, b, e, H, f);
     });
 }
 
-// Sparse finish over T's summary: the wave that holds a summary word owns
-// its 64 T words and the F words of the same vertices (one per lane), so F
-// and its summary are written without atomics.  New bits -> seen, frontier word, level writes, F.
-__global__ void kMsFinishS(MsBits M, uint64_t* seen, uint64_t* front, uint64_t* next, int32_t* __restrict__ levels,
-                           int32_t V, int32_t L, const int32_t* __restrict__ fb, MsCnt* nxt, int G) {
-    int64_t nf = 0, mf = 0;
-    forSummary(M.TS, M.W2, G, [&](int64_t g, uint64_t sm) {
-        const int l = laneId();
-        uint64_t t = 0;
-        if ((sm >> l) & 1) {
-            uint64_t* wp = M.T + g * 64 + l;
-            t = *wp;
-            if (t) *wp = 0;
-        }
-        if (l == 0) M.TS[g] = 0;
-        const int64_t base = (g * 64 + l) * 64;
-        uint64_t fw = 0;
-        while (t) {
-            const int b = __ffsll((long long)t) - 1;
-            t &= t - 1;
-            const int32_t w = (int32_t)(base + b);
-            const uint64_t sw = seen[w];
-            const uint64_t nb = next[w] & ~sw;
-            next[w] = 0;
-            if (nb) {
-                seen[w] = sw | nb;
-                front[w] = nb;
-                fw |= 1ULL << b;
-                ++nf;
-                mf += fb[w + 1] - fb[w];
-                uint64_t x = nb;
-                while (x) {
-                    const int s = __ffsll((long long)x) - 1;
-                    x &= x - 1;
-                    levels[(size_t)s * V + w] = L + 1;
-                }
+// Sparse finish body for one summary word g of T: new bits -> seen,
+// frontier word, level writes, F.  The wave owns the 64 T words and the F
+// words of the same vertices; F bits are gathered in the wave's LDS words
+// (fwl, 64 entries) and written once per word, with F's summary word.
+__device__ __forceinline__ void msFinishGroup(const MsBits& M, uint64_t* seen, uint64_t* front, uint64_t* next,
+                                              int32_t* __restrict__ levels, int32_t V, int32_t L,
+                                              const int32_t* __restrict__ fb, int64_t g, uint64_t sm, int32_t* lds,
+                                              uint64_t* fwl, int64_t& nf, int64_t& mf) {
+    const int l = laneId();
+    uint64_t t = 0;
+    if ((sm >> l) & 1) {
+        uint64_t* wp = M.T + g * 64 + l;
+        t = *wp;
+        if (t) *wp = 0;
+    }
+    if (l == 0) M.TS[g] = 0;
+    fwl[l] = 0;
+    waveSync();
+    waveBits(t, g * 64 + l, lds, [&](bool act, int32_t w) {
+        if (!act) return;
+        const uint64_t sw = seen[w];
+        const uint64_t nb = next[w] & ~sw;
+        next[w] = 0;
+        if (nb) {
+            seen[w] = sw | nb;
+            front[w] = nb;
+            atomicOr((unsigned long long*)&fwl[(w >> 6) & 63], 1ULL << (w & 63));
+            ++nf;
+            mf += fb[w + 1] - fb[w];
+            uint64_t x = nb;
+            while (x) {
+                const int s = __ffsll((long long)x) - 1;
+                x &= x - 1;
+                levels[(size_t)s * V + w] = L + 1;
             }
         }
-        if (fw) M.F[g * 64 + l] = fw;
-        const uint64_t any = __ballot(fw != 0);
-        if (l == 0 && any) M.FS[g] = any;
+    });
+    waveSync();
+    const uint64_t fw = fwl[l];
+    if (fw) M.F[g * 64 + l] = fw;
+    const uint64_t any = __ballot(fw != 0);
+    if (l == 0 && any) M.FS[g] = any;
+}
+
+// LDS of the sparse passes: per wave, WB_CAP ids and 64 F words.
+struct MsLds {
+    int32_t ids[1024 / WAVE][WB_CAP];
+    uint64_t fw[1024 / WAVE][64];
+};
+
+__global__ void kMsPushS(const int32_t* __restrict__ fb, uint64_t* front, MsPushSF f, MsCnt* nxt, HeavyList H,
+                         int G) {
+    __shared__ MsLds sh;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = MsCnt{0, 0};
+    int32_t* lds = sh.ids[threadIdx.x / WAVE];
+    forSummary(f.M.FS, f.M.W2, G, [&](int64_t g, uint64_t sm) { msPushGroup(fb, front, f, H, g, sm, lds); });
+}
+
+__global__ void kMsFinishS(MsBits M, uint64_t* seen, uint64_t* front, uint64_t* next, int32_t* __restrict__ levels,
+                           int32_t V, int32_t L, const int32_t* __restrict__ fb, MsCnt* nxt, int G) {
+    __shared__ MsLds sh;
+    const int wv = threadIdx.x / WAVE;
+    int64_t nf = 0, mf = 0;
+    forSummary(M.TS, M.W2, G, [&](int64_t g, uint64_t sm) {
+        msFinishGroup(M, seen, front, next, levels, V, L, fb, g, sm, sh.ids[wv], sh.fw[wv], nf, mf);
     });
     blockAtomicAdd(nf, &nxt->nf);
     blockAtomicAdd(mf, &nxt->mf);
+}
+
+// Persistent sparse levels: one cooperative launch runs push / barrier /
+// finish / barrier per level for as long as the levels stay sparse and push
+// (the host's rule: nf * 4 < V, and mf * alpha <= E unless alpha is 0), up
+// to level Lmax.  *Lio: first level in, first level not run out.  Every
+// block reads the same counters after a barrier, so all leave the loop
+// together.  Used only when no row of the store can reach the heavy list.
+__global__ void __launch_bounds__(256) kMsPersist(MsPushSF f, const int32_t* __restrict__ fb, uint64_t* front,
+                                                  int32_t* __restrict__ levels, int32_t V, MsCnt* ring, int64_t* Lio,
+                                                  int64_t E, int64_t alpha, int32_t Lmax, HeavyList H) {
+    __shared__ MsLds sh;
+    const int wv = threadIdx.x / WAVE;
+    const MsBits& M = f.M;
+    int32_t L = (int32_t)ldAcq(Lio);
+    while (L < Lmax) {
+        MsCnt* cur = &ring[L % MS_RING];
+        const int64_t nf = ldAcq(&cur->nf), mf = ldAcq(&cur->mf);
+        if (nf == 0 || nf * 4 >= (int64_t)V || (alpha > 0 && mf * alpha > E)) break;
+        MsCnt* nxt = &ring[(L + 1) % MS_RING];
+        if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = MsCnt{0, 0};
+        forSummary(M.FS, M.W2, 1, [&](int64_t g, uint64_t sm) { msPushGroup(fb, front, f, H, g, sm, sh.ids[wv]); });
+        gridSync();
+        int64_t a = 0, b = 0;
+        forSummary(M.TS, M.W2, 1, [&](int64_t g, uint64_t sm) {
+            msFinishGroup(M, const_cast<uint64_t*>(f.seen), front, f.next, levels, V, L, fb, g, sm, sh.ids[wv],
+                          sh.fw[wv], a, b);
+        });
+        blockAtomicAdd(a, &nxt->nf);
+        blockAtomicAdd(b, &nxt->mf);
+        gridSync();
+        ++L;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *Lio = L;
 }
 
 struct MsPushF {
@@ -278,11 +332,6 @@ __global__ void kMsUnreached(const uint64_t* __restrict__ seen, int32_t* __restr
     }
 }
 
-// Levels per host readback while the frontier is tiny (high-diameter graphs).
-constexpr int MS_CHUNK = 16;
-// Counter ring: one MsCnt per level, indexed by level mod MS_RING.
-constexpr int MS_RING = 64;
-static_assert(MS_RING >= MS_CHUNK + 2, "ring must hold a chunk and its neighbours");
 
 // Up to 64 sources (device array src, k entries) into levels rows 0..k-1.
 //
@@ -310,7 +359,7 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
     M.FS = S.msbm[1].p;
     M.T = S.msbm[2].p;
     M.TS = S.msbm[3].p;
-    S.mscnt.ensure(2 * MS_RING);
+    S.mscnt.ensure(2 * MS_RING + 1);  // + the persistent kernel's level slot
     uint64_t *seen = S.ms[0].p, *front = S.ms[1].p, *next = S.ms[2].p;
     MsCnt* ring = (MsCnt*)S.mscnt.p;
     const uint64_t all = k >= 64 ? ~0ULL : ((1ULL << k) - 1);
@@ -331,6 +380,8 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
     bool pull = false;
     int64_t mf_prev = 0;  // frontier arcs one readback earlier (0: none / dense)
     int c_prev = 1;
+    // persistent sparse levels need every row below the heavy-list threshold
+    bool persist_ok = h->prm.persist != 0 && maxDegree(h) <= HEAVY_T;
     const unsigned g = gridFor(V, blk);
     int32_t L = 0;
     while (nf > 0) {
@@ -358,6 +409,24 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
             mf_prev = 0;
             ++L;
             continue;
+        }
+        if (persist_ok) {
+            int64_t* Lio = S.mscnt.p + 2 * MS_RING;
+            fillI64(h, Lio, L, 1);
+            const MsPushSF f{h->fw_adj.p, seen, next, M};
+            const int64_t a = (int64_t)V >= min_reach ? alpha : 0;
+            const int32_t Lmax = V + 1;  // a BFS has fewer levels than vertices
+            if (launchPersistent(h, "msbfs_persist", kMsPersist, 256, 2, f, fb, front, levels, V, ring, Lio, E, a, Lmax,
+                                 heavyList(h))) {
+                int64_t r[2 * MS_RING + 1];
+                readCounters(h, S.mscnt.p, 2 * MS_RING + 1, r);
+                L = (int32_t)r[2 * MS_RING];
+                nf = r[2 * (L % MS_RING)];
+                mf = r[2 * (L % MS_RING) + 1];
+                mf_prev = 0;
+                continue;
+            }
+            persist_ok = false;  // no cooperative launch here: host-driven levels
         }
         const int cmax = std::max(1, std::min(MS_CHUNK, (int)h->prm.chunk));
         const int C = (mf_prev > 0 && mf * 64 < E && mf <= 2 * mf_prev) ? std::min(cmax, 2 * c_prev) : 1;
@@ -446,6 +515,68 @@ extern "C" int girhip_bfs(girhip_graph* h, const int32_t* src, int32_t k, int32_
             }
             HIPCHK(hipMemcpy(edges_out, e.data(), sizeof(int64_t) * (size_t)k, hipMemcpyDefault));
         }
+        if (levels && !direct) copyOut(h, levels, out, sizeof(int32_t) * n);
+        syncStream(h);
+    });
+}
+
+namespace gir {
+// Vertices of one component: label[w] == c.
+struct SameLabel {
+    const int32_t* lab;
+    int32_t c;
+    __device__ bool operator()(int32_t w, int32_t) const { return lab[w] == c; }
+    __device__ bool self(int32_t v) const { return lab[v] == c; }
+};
+}  // namespace gir
+
+extern "C" int girhip_bfs_component(girhip_graph* h, const int32_t* src, int32_t k, int32_t undirected,
+                                    int32_t* levels, int64_t* visited) {
+    return guarded([&] {
+        useDevice(h);
+        if (k < 0 || (k > 0 && !src)) raise(GIRHIP_EINVAL, "bfs_component: bad source list");
+        if (undirected ? !h->wcc_req : !h->scc_req)
+            raise(GIRHIP_ESTATE, undirected ? "bfs_component: buildWCC first" : "bfs_component: buildSCC_DAG first");
+        const int32_t V = h->V;
+        std::vector<int32_t> s = hostSources(h, src, k);
+        const size_t n = (size_t)k * (size_t)V;
+        int32_t* out = levels;
+        const bool direct = levels && isDevicePtr(levels);
+        Scratch& S = scratch(h);
+        if (!direct) {
+            S.stage.ensure(sizeof(int32_t) * (n > 0 ? n : 1));
+            out = (int32_t*)S.stage.p;
+        }
+        // push / pull CSR: out- and in-arcs (directed), the symmetric closure
+        // for both (undirected)
+        const int32_t *pb = h->fw_beg.p, *pa = h->fw_adj.p, *rb = h->bw_beg.p, *ra = h->bw_adj.p;
+        int64_t m_total = h->E;
+        if (undirected) {
+            m_total = symCsr(h);
+            pb = rb = h->sym_beg.p;
+            pa = ra = h->sym_adj.p;
+        }
+        const int32_t* lab = undirected ? h->wcc.p : h->scc.p;
+        ReachBufs B = reachBufs(h);
+        std::vector<int64_t> vis(k, 0);
+        tBegin(h);
+        fillI32(h, out, -1, (int64_t)n);
+        for (int32_t i = 0; i < k; ++i) {
+            const int32_t x = s[i];
+            if (x < 0 || x >= V) continue;
+            int32_t c = 0;
+            int32_t deg[2];
+            HIPCHK(hipMemcpyAsync(&c, lab + x, sizeof(int32_t), hipMemcpyDeviceToHost, h->st));
+            HIPCHK(hipMemcpyAsync(deg, pb + x, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, h->st));
+            syncStream(h);
+            fillU64(h, B.vis, 0, B.W);
+            fillI32(h, B.q[0], x, 1);
+            const ReachStats st = runReach(h, pb, pa, rb, ra, B, 1, (int64_t)deg[1] - deg[0], (int64_t)V, m_total,
+                                           SameLabel{lab, c}, WriteLevel{out + (size_t)i * V});
+            vis[i] = st.reached;
+        }
+        tEnd(h, "bfs_component");
+        if (visited) HIPCHK(hipMemcpy(visited, vis.data(), sizeof(int64_t) * (size_t)k, hipMemcpyDefault));
         if (levels && !direct) copyOut(h, levels, out, sizeof(int32_t) * n);
         syncStream(h);
     });

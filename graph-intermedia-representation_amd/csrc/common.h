@@ -1,6 +1,7 @@
 // Internal header of the gfx950 GIR library: the handle, device buffers,
 // error handling, kernel launch wrapper and wave64 helpers.
 #pragma once
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -82,6 +83,7 @@ struct Params {
     double profile = 0;  // 1: time every kernel launch with HIP events
     double msbfs = 1;    // 1: k>1 BFS runs 64 sources per bit-parallel sweep
     double chunk = 16;   // most BFS levels / SSSP rounds queued per host readback
+    double persist = 0;  // 1: tiny-frontier phases run in one persistent cooperative kernel
 };
 
 struct KStat {
@@ -130,6 +132,8 @@ constexpr int HCNT_SLOTS = 256;
 
 struct girhip_graph {
     int device = 0;
+    int n_cu = 0;        // compute units of the device
+    int coop = -1;       // cooperative launch supported (-1: not yet queried)
     hipStream_t st = nullptr;
     int32_t V = 0;
     int64_t E = 0;
@@ -142,6 +146,9 @@ struct girhip_graph {
     bool has_w = false;
     gir::DevBuf<int32_t> fw_src;  // row of every fw slot (COO), built on demand
     bool src_ok = false;
+    gir::DevBuf<int32_t> sym_beg, sym_adj;  // symmetric closure (no self loops), built on demand
+    bool sym_ok = false;
+    int64_t sym_E = 0;
     int64_t max_deg = -1;  // largest fw/bw row length; -1 = not yet computed
     int64_t max_w = -1;    // largest stored weight; -1 = not yet computed
 
@@ -221,6 +228,37 @@ void launch(girhip_graph* h, const char* name, K kernel, dim3 grid, dim3 block, 
         HIPCHK(hipEventRecord(b, h->st));
         h->pend_ev.push_back({name, a, b});
     }
+}
+
+// Persistent kernels: a cooperative launch of at most blocks_per_cu blocks
+// per compute unit (all blocks co-resident, so grid-wide barriers are
+// safe).  Returns false, launching nothing, where cooperative launch is
+// unavailable or the kernel cannot keep that many blocks resident.
+bool persistentGrid(girhip_graph* h, const void* kernel, int block, int blocks_per_cu, unsigned* grid);
+template <class K, class... Args>
+bool launchPersistent(girhip_graph* h, const char* name, K kernel, int block, int blocks_per_cu, Args... args) {
+    unsigned grid = 0;
+    if (!persistentGrid(h, (const void*)kernel, block, blocks_per_cu, &grid)) return false;
+    void* ptrs[] = {(void*)&args...};
+    hipEvent_t a = nullptr;
+    if (h->prm.profile != 0) {
+        a = takeEvent(h);
+        HIPCHK(hipEventRecord(a, h->st));
+    }
+    HIPCHK(hipLaunchCooperativeKernel((const void*)kernel, dim3(grid), dim3(block), ptrs, 0, h->st));
+    if (a) {
+        hipEvent_t b = takeEvent(h);
+        HIPCHK(hipEventRecord(b, h->st));
+        h->pend_ev.push_back({name, a, b});
+    }
+    return true;
+}
+
+// Grid-wide barrier of a persistent kernel, then an agent-scope acquire so
+// no stale cached line survives it.
+__device__ __forceinline__ void gridSync() {
+    cooperative_groups::this_grid().sync();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 void syncStream(girhip_graph* h);
@@ -498,6 +536,61 @@ __device__ __forceinline__ void forSummary(const uint64_t* __restrict__ sum, int
     }
 }
 
+// Wave-level synchronisation of LDS traffic between the lanes of one wave.
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS entries per wave for waveBits.
+constexpr int WB_CAP = 256;
+
+// Lane-parallel expansion of 64 bitmap words held one per lane (m = this
+// lane's word, word index wi): the set bits' ids (wi * 64 + bit) are
+// compacted into the wave's LDS slice (lds, WB_CAP entries) by a wave
+// prefix sum, then body(act, x) runs for 64 ids at a time with the whole
+// wave (act false on idle lanes), so the memory accesses of up to 64 ids are
+// in flight together instead of one chain per lane.  More than WB_CAP set
+// bits: each lane walks its own bits (same calls, less parallel).
+template <class F>
+__device__ __forceinline__ void waveBits(uint64_t m, int64_t wi, int32_t* lds, F&& body) {
+    const int lane = laneId();
+    const int c = __popcll(m);
+    int incl = c;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int o = __shfl_up(incl, d);
+        if (lane >= d) incl += o;
+    }
+    const int total = __shfl(incl, WAVE - 1);
+    if (total == 0) return;
+    if (total <= WB_CAP) {
+        int pos = incl - c;
+        while (m) {
+            const int b = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            lds[pos++] = (int32_t)(wi * 64 + b);
+        }
+        waveSync();
+        for (int i0 = 0; i0 < total; i0 += WAVE) {
+            const int i = i0 + lane;
+            const bool act = i < total;
+            body(act, act ? lds[i] : 0);
+        }
+        waveSync();
+        return;
+    }
+    while (__ballot(m != 0)) {
+        const bool act = m != 0;
+        int32_t x = 0;
+        if (act) {
+            x = (int32_t)(wi * 64 + __ffsll((long long)m) - 1);
+            m &= m - 1;
+        }
+        body(act, x);
+    }
+}
+
 // Grid for a forSummary kernel over tasks summary-word tasks (about two
 // tasks per wave, at most 1024 blocks).
 inline unsigned summaryGrid(int64_t W2, int G, int64_t reps, int block) {
@@ -613,6 +706,11 @@ void launchHeavy(girhip_graph* h, const char* name, const F& f, int64_t* sum = n
     launch(h, "heavy_prefix", kHeavyPrefix, 1, 1024, 0, H);
     launch(h, name, kHeavyRows<F>, 1024, 256, 0, H, f, sum);
 }
+// Symmetric closure of the edge store (both directions, deduplicated, no
+// self loops) as CSR in h->sym_beg / sym_adj (cached until the store
+// changes); the undirected graph of the reference's buildWCC
+// (gir.cpp:593-620).  Returns its arc count.
+int64_t symCsr(girhip_graph* h);
 // Largest row of the edge store's fw and bw CSR (cached until the store
 // changes).
 int64_t maxDegree(girhip_graph* h);

@@ -308,6 +308,28 @@ __global__ void kMaxWeight(const uint32_t* __restrict__ w, int64_t E, int64_t* o
     if (laneId() == 0 && m) atomicMax((unsigned long long*)out, (unsigned long long)m);
 }
 
+bool persistentGrid(girhip_graph* h, const void* kernel, int block, int blocks_per_cu, unsigned* grid) {
+    if (h->prm.persist == 0) return false;
+    if (h->coop < 0) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeCooperativeLaunch, h->device) != hipSuccess) c = 0;
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) n = 0;
+        (void)hipGetLastError();
+        h->n_cu = n;
+        h->coop = (c && n > 0) ? 1 : 0;
+    }
+    if (!h->coop) return false;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (per_cu < 1) return false;
+    *grid = (unsigned)(std::min(per_cu, blocks_per_cu) * h->n_cu);
+    return true;
+}
+
 int64_t maxWeight(girhip_graph* h) {
     if (!h->has_w) return 100;
     if (h->max_w < 0) {
@@ -317,6 +339,60 @@ int64_t maxWeight(girhip_graph* h) {
         readCounters(h, h->dcnt.p, 1, &h->max_w);
     }
     return h->max_w;
+}
+
+// Both orientations of every arc as packed keys; self loops get the
+// sentinel 1 << 2B (sorts after every real key).
+__global__ void kSymKeys(const int32_t* __restrict__ src, const int32_t* __restrict__ adj, int64_t E, int B,
+                         uint64_t* __restrict__ keys) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= E) return;
+    const uint32_t u = (uint32_t)src[j], v = (uint32_t)adj[j];
+    const uint64_t sent = 1ULL << (2 * B);
+    keys[2 * j] = u == v ? sent : (((uint64_t)u << B) | v);
+    keys[2 * j + 1] = u == v ? sent : (((uint64_t)v << B) | u);
+}
+
+__global__ void kCountBelow(const uint64_t* __restrict__ keys, int64_t n, uint64_t x, int64_t* out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t m = lo + ((hi - lo) >> 1);
+        if (keys[m] < x) lo = m + 1;
+        else hi = m;
+    }
+    *out = lo;
+}
+
+int64_t symCsr(girhip_graph* h) {
+    if (h->sym_ok) return h->sym_E;
+    const int32_t V = h->V;
+    const int64_t E = h->E;
+    const int B = bitsFor((uint64_t)(V > 1 ? V - 1 : 1));
+    if (E == 0) {
+        h->sym_beg.ensure((size_t)V + 1);
+        h->sym_adj.ensure(1);
+        fillI32(h, h->sym_beg.p, 0, (int64_t)V + 1);
+        h->sym_E = 0;
+        h->sym_ok = true;
+        return 0;
+    }
+    Scratch& S = scratch(h);
+    S.keys[0].ensure((size_t)(2 * E));
+    const int32_t* src = fwSrc(h);
+    launch(h, "sym_keys", kSymKeys, gridFor(E, 256), 256, 0, src, (const int32_t*)h->fw_adj.p, E, B, S.keys[0].p);
+    radixSort(h, S.keys[0].p, nullptr, 2 * E, 2 * B + 1);
+    launch(h, "sym_count", kCountBelow, 1, 64, 0, (const uint64_t*)S.keys[0].p, 2 * E, 1ULL << (2 * B), h->dcnt.p);
+    int64_t n2;
+    readCounters(h, h->dcnt.p, 1, &n2);
+    h->sym_E = n2 > 0 ? keysToCsr(h, n2, B, h->sym_beg, h->sym_adj) : 0;
+    if (n2 == 0) {
+        h->sym_beg.ensure((size_t)V + 1);
+        h->sym_adj.ensure(1);
+        fillI32(h, h->sym_beg.p, 0, (int64_t)V + 1);
+    }
+    h->sym_ok = true;
+    return h->sym_E;
 }
 
 const int32_t* fwSrc(girhip_graph* h) {
@@ -487,7 +563,8 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"block", &Params::block},                 {"wcc_rounds", &Params::wcc_rounds},
                {"scc_trim_rounds", &Params::scc_trim_rounds}, {"scc_pivot", &Params::scc_pivot},
                {"scc_color_min", &Params::scc_color_min}, {"profile", &Params::profile},
-               {"msbfs", &Params::msbfs},         {"chunk", &Params::chunk}};
+               {"msbfs", &Params::msbfs},         {"chunk", &Params::chunk},
+               {"persist", &Params::persist}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;

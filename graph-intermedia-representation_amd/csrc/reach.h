@@ -159,6 +159,55 @@ constexpr int REACH_CHUNK = 16;
 constexpr int REACH_RING = 64;
 static_assert(REACH_RING >= REACH_CHUNK + 3, "ring must hold a chunk and its neighbours");
 
+// Persistent top-down levels: one cooperative launch runs level after level
+// (top-down step, grid barrier) until the frontier is empty or the switch
+// rule asks for bottom-up (mf * alpha > arcs unexplored; alpha 0 = never).
+// io[0..3] in and out: level, queue parity, vertices reached, arcs
+// unexplored.  Every block reads the same counters after the barrier, so
+// all leave the loop together.  Used only when no row of the store can
+// reach the heavy list.
+template <class Flt, class Vis>
+__global__ void __launch_bounds__(256) kReachPersist(const int32_t* __restrict__ pbeg, const int32_t* __restrict__ padj,
+                                                     uint64_t* vis, int32_t* q0, int32_t* q1, int64_t* ring,
+                                                     int64_t* io, int64_t alpha, Flt flt, Vis on, HeavyList H) {
+    int32_t L = (int32_t)ldAcq(&io[0]);
+    int cur = (int)ldAcq(&io[1]);
+    int64_t reached = ldAcq(&io[2]), mun = ldAcq(&io[3]);
+    auto pair = [&](int64_t l) { return ring + 2 * ((l % REACH_RING + REACH_RING) % REACH_RING); };
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    while (true) {
+        const int64_t* pin = pair(L - 1);
+        const int64_t nf = ldAcq(&pin[0]), mf = ldAcq(&pin[1]);
+        if (nf == 0 || (alpha > 0 && mf * alpha > mun)) break;
+        const int32_t* qin = cur ? q1 : q0;
+        int32_t* qout = cur ? q0 : q1;
+        int64_t* cn = pair(L);
+        if (blockIdx.x == 0 && threadIdx.x < 2) pair(L + 1)[threadIdx.x] = 0;
+        const TopDownF<Flt, Vis> f{pbeg, padj, vis, L, qout, cn, flt, on};
+        int64_t m = 0;
+        for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < nf;
+             base += stride) {
+            const int64_t i = base + laneId();
+            const bool act = i < nf;
+            const int32_t u = act ? qin[i] : 0;
+            const int32_t b = act ? pbeg[u] : 0, e = act ? pbeg[u + 1] : 0;
+            m += waveRowsH(act, u, 0, 0, b, e, H, f);
+        }
+        blockAtomicAdd(m, &cn[1]);
+        gridSync();
+        reached += ldAcq(&cn[0]);
+        mun -= ldAcq(&cn[1]);
+        cur ^= 1;
+        ++L;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        io[0] = L;
+        io[1] = cur;
+        io[2] = reached;
+        io[3] = mun;
+    }
+}
+
 // Runs reachability from the n0 seeds in B.q[0] (caller fills them).  The
 // visited bitmap B.vis must be clear on entry except for bits the caller
 // wants treated as already visited.  pbeg/padj: push-direction CSR; rbeg/radj:
@@ -174,7 +223,7 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
     if (n0 <= 0) return st;
     launch(h, "reach_seed", kReachSeed<Vis>, gridFor(n0, blk), blk, 0, (const int32_t*)B.q[0], n0, B.vis, on);
     Scratch& S = scratch(h);
-    S.rcnt.ensure(2 * REACH_RING);
+    S.rcnt.ensure(2 * REACH_RING + 4);  // + the persistent kernel's state
     int64_t* c = S.rcnt.p;
     auto pair = [&](int64_t L) { return c + 2 * ((L % REACH_RING + REACH_RING) % REACH_RING); };
     fillI64(h, c, 0, 2 * REACH_RING);
@@ -186,6 +235,8 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
     int64_t mf_prev = 0;  // frontier arcs one readback earlier
     int c_prev = 1;
     int cur = 0;
+    // persistent levels need every row below the heavy-list threshold
+    bool persist_ok = h->prm.persist != 0 && maxDegree(h) <= HEAVY_T;
     int32_t L = 0;
     st.reached = n0;
     while (nf > 0) {
@@ -216,6 +267,30 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
             continue;
         }
         bits_valid = false;
+        if (persist_ok) {
+            int64_t* io = S.rcnt.p + 2 * REACH_RING;
+            fillI64(h, io, L, 1);
+            fillI64(h, io + 1, cur, 1);
+            fillI64(h, io + 2, st.reached, 1);
+            fillI64(h, io + 3, m_unexplored, 1);
+            const int64_t a = (allow_bottom_up && n_reach >= min_reach) ? alpha : 0;
+            if (launchPersistent(h, "reach_persist", kReachPersist<Flt, Vis>, 256, 2, pbeg, padj, B.vis, B.q[0], B.q[1],
+                                 c, io, a, flt, on, heavyList(h))) {
+                int64_t r[2 * REACH_RING + 4];
+                readCounters(h, c, 2 * REACH_RING + 4, r);
+                const int64_t* o = r + 2 * REACH_RING;
+                L = (int32_t)o[0];
+                cur = (int)o[1];
+                st.reached = o[2];
+                m_unexplored = o[3];
+                const int64_t* p = r + (pair(L - 1) - c);
+                nf = p[0];
+                mf = p[1];
+                mf_prev = 0;
+                continue;
+            }
+            persist_ok = false;  // no cooperative launch here: host-driven levels
+        }
         // tiny frontiers, far below the switch point and not growing fast: a
         // chunk of levels
         const int C = (mf_prev > 0 && mf * 64 < (int64_t)V && mf * alpha * 8 < m_unexplored && mf <= 2 * mf_prev)

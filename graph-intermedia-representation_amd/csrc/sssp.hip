@@ -127,36 +127,33 @@ struct RelaxF {
     }
 };
 
-// Relax source s's queue of parity par (grid stride, uniform per wave).
-// Block (0, s) resets the slots this round fills later and the next round's
-// near flag.  track_done: mark processed vertices (their heavy arcs are due
-// when the bucket drains).
-__global__ void kBRelax(RelaxF f, const int32_t* __restrict__ fb, HeavyList H, int track_done) {
-    const int par = f.par;
+// ---- round phases (per source s; the grid strides over x) -----------------
+
+// Per-round slots of source s that the round fills later, and the next
+// round's near flag (one thread).
+__device__ __forceinline__ void resetRound(const SsspBatch& B, int par, int s) {
+    *nearCount(B, par ^ 1, s) = 0;
+    *nearSet(B, par ^ 1, s) = 0;
+    B.ctl[C_FMIN + s] = (int64_t)INF;
+    B.ctl[C_HQ + s] = 0;
+}
+
+// Relax the n queued vertices q of source s (grid stride, uniform per wave).
+// track_done: mark them processed (their heavy arcs are due when the bucket
+// drains).
+__device__ __forceinline__ void relaxQueue(const RelaxF& f, const int32_t* __restrict__ fb, HeavyList H,
+                                           const int32_t* q, int64_t n, int s, bool track_done) {
     const SsspBatch& B = f.B;
-    int64_t* c = B.ctl;
-    const int s = blockIdx.y;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *nearCount(B, par ^ 1, s) = 0;
-        *nearSet(B, par ^ 1, s) = 0;
-        c[C_FMIN + s] = (int64_t)INF;
-        c[C_HQ + s] = 0;
-    }
+    const uint64_t* d = B.dist + (size_t)s * B.V;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    {
-        const int64_t nin = *nearCount(B, par, s);
-        const int32_t* q = B.qn[par] + (size_t)s * B.V;
-        const uint64_t* d = B.dist + (size_t)s * B.V;
-        for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < nin;
-             base += stride) {
-            const int64_t i = base + laneId();
-            const bool act = i < nin;
-            const int32_t u = act ? q[i] : 0;
-            const uint64_t du = act ? ldAcq(&d[u]) : 0;
-            if (act && track_done) setBit(B.donebits + (size_t)s * B.W, B.dsum + (size_t)s * B.W2, u);
-            const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
-            waveRowsH(act, u, du, s, b, e, H, f);
-        }
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+        const int64_t i = base + laneId();
+        const bool act = i < n;
+        const int32_t u = act ? q[i] : 0;
+        const uint64_t du = act ? ldAcq(&d[u]) : 0;
+        if (act && track_done) setBit(B.donebits + (size_t)s * B.W, B.dsum + (size_t)s * B.W2, u);
+        const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+        waveRowsH(act, u, du, s, b, e, H, f);
     }
 }
 
@@ -182,20 +179,24 @@ __device__ __forceinline__ void drainGroup(uint64_t* bits, uint64_t* sum, int64_
     }
 }
 
+// LDS of the far-bitmap passes: per wave, WB_CAP ids and 64 bitmap words.
+struct FarLds {
+    int32_t ids[1024 / WAVE][WB_CAP];
+    uint64_t w[1024 / WAVE][64];
+};
+
 // Smallest distance >= hi over one summary word of source s's far bitmap
-// (per lane; reduced by farMinCommit).
+// (per lane, lane-parallel distance reads; reduced by farMinCommit).
 __device__ __forceinline__ uint64_t farMinGroup(const SsspBatch& B, int s, int64_t g, uint64_t sm, uint64_t hi,
-                                                uint64_t mn) {
+                                                uint64_t mn, int32_t* lds) {
     const int l = laneId();
-    uint64_t m = ((sm >> l) & 1) ? B.farbits[(size_t)s * B.W + g * 64 + l] : 0;
+    const uint64_t m = ((sm >> l) & 1) ? B.farbits[(size_t)s * B.W + g * 64 + l] : 0;
     const uint64_t* d = B.dist + (size_t)s * B.V;
-    const int64_t wi = g * 64 + l;
-    while (m) {
-        const int b = __ffsll((long long)m) - 1;
-        m &= m - 1;
-        const uint64_t x = d[wi * 64 + b];
-        if (x >= hi && x < mn) mn = x;
-    }
+    waveBits(m, g * 64 + l, lds, [&](bool act, int32_t x) {
+        if (!act) return;
+        const uint64_t v = d[x];
+        if (v >= hi && v < mn) mn = v;
+    });
     return mn;
 }
 
@@ -207,112 +208,156 @@ __device__ __forceinline__ void farMinCommit(const SsspBatch& B, int s, unsigned
     if (laneId() == 0 && mn != INF) atomicMin((unsigned long long*)&B.ctl[C_FMIN + s], mn);
 }
 
-// After the relaxation of parity par, per source: if the round put vertices
-// into the bucket, its near bitmap -> next queue; otherwise the bucket has
-// drained: with heavy arcs pending (heavy = 1) the processed vertices ->
-// heavy-phase queue, else the smallest far distance is found right here.
-__global__ void kBCompact(SsspBatch B, int par, int heavy, int G) {
-    const int s = blockIdx.y;
-    {
-        const size_t bo = (size_t)s * B.W, so = (size_t)s * B.W2, qo = (size_t)s * B.V;
-        if (*nearSet(B, par, s)) {
-            forSummary(B.nsum + so, B.W2, G, [&](int64_t g, uint64_t sm) {
-                drainGroup(B.nearbits + bo, B.nsum + so, g, sm, B.qn[par ^ 1] + qo, nearCount(B, par ^ 1, s));
-            });
-        } else if (heavy) {
-            forSummary(B.dsum + so, B.W2, G, [&](int64_t g, uint64_t sm) {
-                drainGroup(B.donebits + bo, B.dsum + so, g, sm, B.qh + qo, B.ctl + C_HQ + s);
-            });
-        } else {
-            const uint64_t hi = (uint64_t)B.ctl[C_HI + s];
-            uint64_t mn = INF;
-            forSummary(B.fsum + so, B.W2, G, [&](int64_t g, uint64_t sm) { mn = farMinGroup(B, s, g, sm, hi, mn); });
-            farMinCommit(B, s, mn);
-        }
-    }
-}
-
-// Heavy arcs of source s's heavy-phase queue (grid stride, uniform per
-// wave).
-__global__ void kBRelaxHeavy(RelaxF f, const int32_t* __restrict__ fb, HeavyList H) {
-    const SsspBatch& B = f.B;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int s = blockIdx.y;
-    {
-        const int64_t n = B.ctl[C_HQ + s];
-        const int32_t* q = B.qh + (size_t)s * B.V;
-        const uint64_t* d = B.dist + (size_t)s * B.V;
-        for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n;
-             base += stride) {
-            const int64_t i = base + laneId();
-            const bool act = i < n;
-            const int32_t u = act ? q[i] : 0;
-            const uint64_t du = act ? ldAcq(&d[u]) : 0;
-            const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
-            waveRowsH(act, u, du, s, b, e, H, f);
-        }
-    }
-}
-
-// Drained sources after their heavy phase: smallest far distance >= hi.
-__global__ void kBFarMin(SsspBatch B, int par, int G) {
-    const int s = blockIdx.y;
-    {
-        if (*nearSet(B, par, s)) return;
-        const uint64_t hi = (uint64_t)B.ctl[C_HI + s];
+// After the relaxation of parity par, for source s: if the round put
+// vertices into the bucket, its near bitmap -> next queue; otherwise the
+// bucket has drained: with heavy arcs pending (heavy = 1) the processed
+// vertices -> heavy-phase queue, else the smallest far distance is found
+// right here.
+__device__ __forceinline__ void compactPhase(const SsspBatch& B, int par, int heavy, int G, int s, FarLds& sh) {
+    const size_t bo = (size_t)s * B.W, so = (size_t)s * B.W2, qo = (size_t)s * B.V;
+    if (ldAcq(nearSet(B, par, s))) {
+        forSummary(B.nsum + so, B.W2, G, [&](int64_t g, uint64_t sm) {
+            drainGroup(B.nearbits + bo, B.nsum + so, g, sm, B.qn[par ^ 1] + qo, nearCount(B, par ^ 1, s));
+        });
+    } else if (heavy) {
+        forSummary(B.dsum + so, B.W2, G, [&](int64_t g, uint64_t sm) {
+            drainGroup(B.donebits + bo, B.dsum + so, g, sm, B.qh + qo, B.ctl + C_HQ + s);
+        });
+    } else {
+        const uint64_t hi = (uint64_t)ldAcq(&B.ctl[C_HI + s]);
         uint64_t mn = INF;
-        forSummary(B.fsum + (size_t)s * B.W2, B.W2, G,
-                   [&](int64_t g, uint64_t sm) { mn = farMinGroup(B, s, g, sm, hi, mn); });
+        int32_t* lds = sh.ids[threadIdx.x / WAVE];
+        forSummary(B.fsum + so, B.W2, G,
+                   [&](int64_t g, uint64_t sm) { mn = farMinGroup(B, s, g, sm, hi, mn, lds); });
         farMinCommit(B, s, mn);
     }
 }
 
-// Drained sources with far work left advance to the bucket holding the
-// smallest far distance (every block derives the same [lo, hi); block
-// (0, s) stores it for the next round).  Far entries inside it -> next queue, below
-// it -> settled (dropped), both leave the far bitmap; the rest stay.  A
-// summary word is cleared once its 64 bitmap words are all empty.
-__global__ void kBSplit(SsspBatch B, int par, uint64_t delta, int G) {
-    const int s = blockIdx.y;
-    {
-        if (*nearSet(B, par, s)) return;
-        const uint64_t mfar = (uint64_t)B.ctl[C_FMIN + s];
-        if (mfar == INF) return;  // no far work left: this source is done
-        const uint64_t lo = (mfar / delta) * delta;
-        const uint64_t hi = lo > INF - delta ? INF : lo + delta;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            B.ctl[C_LO + s] = (int64_t)lo;
-            B.ctl[C_HI + s] = (int64_t)hi;
-        }
-        const uint64_t* d = B.dist + (size_t)s * B.V;
-        int32_t* q = B.qn[par ^ 1] + (size_t)s * B.V;
-        uint64_t* fs = B.fsum + (size_t)s * B.W2;
-        uint64_t* fbits = B.farbits + (size_t)s * B.W;
-        forSummary(fs, B.W2, G, [&](int64_t g, uint64_t sm) {
-            const int l = laneId();
-            const int64_t wi = g * 64 + l;
-            uint64_t take = 0, keep = 0;
-            if ((sm >> l) & 1) {
-                const uint64_t m = fbits[wi];
-                uint64_t t = m;
-                while (t) {
-                    const int b = __ffsll((long long)t) - 1;
-                    t &= t - 1;
-                    const uint64_t x = d[wi * 64 + b];
-                    if (x >= hi) keep |= 1ULL << b;
-                    else if (x >= lo) take |= 1ULL << b;
-                }
-                if (keep != m) fbits[wi] = keep;
-            }
-            if (__ballot(keep != 0) == 0 && l == 0) fs[g] = 0;
-            int64_t pos = waveReserve(__popcll(take), nearCount(B, par ^ 1, s));
-            while (take) {
-                const int b = __ffsll((long long)take) - 1;
-                take &= take - 1;
-                q[pos++] = (int32_t)(wi * 64 + b);
-            }
-        });
+// Drained source s after its heavy phase: smallest far distance >= hi.
+__device__ __forceinline__ void farMinPhase(const SsspBatch& B, int par, int G, int s, FarLds& sh) {
+    if (ldAcq(nearSet(B, par, s))) return;
+    const uint64_t hi = (uint64_t)ldAcq(&B.ctl[C_HI + s]);
+    uint64_t mn = INF;
+    int32_t* lds = sh.ids[threadIdx.x / WAVE];
+    forSummary(B.fsum + (size_t)s * B.W2, B.W2, G,
+               [&](int64_t g, uint64_t sm) { mn = farMinGroup(B, s, g, sm, hi, mn, lds); });
+    farMinCommit(B, s, mn);
+}
+
+// Drained source s with far work left advances to the bucket holding the
+// smallest far distance (every block derives the same [lo, hi); block 0
+// stores it for the next round).  Far entries inside it -> next queue,
+// below it -> settled (dropped), both leave the far bitmap; the rest stay.
+// A summary word is cleared once its 64 bitmap words are all empty.
+__device__ __forceinline__ void splitPhase(const SsspBatch& B, int par, uint64_t delta, int G, int s, FarLds& sh) {
+    if (ldAcq(nearSet(B, par, s))) return;
+    const uint64_t mfar = (uint64_t)ldAcq(&B.ctl[C_FMIN + s]);
+    if (mfar == INF) return;  // no far work left: this source is done
+    const uint64_t lo = (mfar / delta) * delta;
+    const uint64_t hi = lo > INF - delta ? INF : lo + delta;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        B.ctl[C_LO + s] = (int64_t)lo;
+        B.ctl[C_HI + s] = (int64_t)hi;
     }
+    const uint64_t* d = B.dist + (size_t)s * B.V;
+    int32_t* q = B.qn[par ^ 1] + (size_t)s * B.V;
+    uint64_t* fs = B.fsum + (size_t)s * B.W2;
+    uint64_t* fbits = B.farbits + (size_t)s * B.W;
+    int64_t* cnt = nearCount(B, par ^ 1, s);
+    const int wv = threadIdx.x / WAVE;
+    int32_t* lds = sh.ids[wv];
+    uint64_t* keepw = sh.w[wv];
+    forSummary(fs, B.W2, G, [&](int64_t g, uint64_t sm) {
+        const int l = laneId();
+        const int64_t wi = g * 64 + l;
+        const uint64_t m = ((sm >> l) & 1) ? fbits[wi] : 0;
+        keepw[l] = 0;
+        waveSync();
+        waveBits(m, wi, lds, [&](bool act, int32_t x) {
+            bool take = false;
+            if (act) {
+                const uint64_t v = d[x];
+                if (v >= hi) atomicOr((unsigned long long*)&keepw[(x >> 6) & 63], 1ULL << (x & 63));
+                else take = v >= lo;
+            }
+            const int64_t pos = waveReserve(take ? 1 : 0, cnt);
+            if (take) q[pos] = x;
+        });
+        waveSync();
+        const uint64_t keep = keepw[l];
+        if (keep != m) fbits[wi] = keep;
+        if (__ballot(keep != 0) == 0 && l == 0) fs[g] = 0;
+    });
+}
+
+// ---- one kernel per phase (grid row y = source) ----------------------------
+
+__global__ void kBRelax(RelaxF f, const int32_t* __restrict__ fb, HeavyList H, int track_done) {
+    const int s = blockIdx.y;
+    const SsspBatch& B = f.B;
+    if (blockIdx.x == 0 && threadIdx.x == 0) resetRound(B, f.par, s);
+    relaxQueue(f, fb, H, B.qn[f.par] + (size_t)s * B.V, *nearCount(B, f.par, s), s, track_done != 0);
+}
+
+__global__ void kBCompact(SsspBatch B, int par, int heavy, int G) {
+    __shared__ FarLds sh;
+    compactPhase(B, par, heavy, G, blockIdx.y, sh);
+}
+
+__global__ void kBRelaxHeavy(RelaxF f, const int32_t* __restrict__ fb, HeavyList H) {
+    const int s = blockIdx.y;
+    const SsspBatch& B = f.B;
+    relaxQueue(f, fb, H, B.qh + (size_t)s * B.V, B.ctl[C_HQ + s], s, false);
+}
+
+__global__ void kBFarMin(SsspBatch B, int par, int G) {
+    __shared__ FarLds sh;
+    farMinPhase(B, par, G, blockIdx.y, sh);
+}
+
+__global__ void kBSplit(SsspBatch B, int par, uint64_t delta, int G) {
+    __shared__ FarLds sh;
+    splitPhase(B, par, delta, G, blockIdx.y, sh);
+}
+
+// ---- persistent rounds -----------------------------------------------------
+// One cooperative launch runs whole rounds (relax / compact / [heavy relax /
+// far minimum] / split, a grid barrier after each phase, sources in turn
+// within a phase) while every queue holds at most cap vertices, for at most
+// max_rounds rounds.  io[0]: round parity in and out.  Every block reads the
+// same counters after a barrier, so all leave the loop together.  Used only
+// when no row of the store can reach the heavy list.
+__global__ void __launch_bounds__(256) kSsspPersist(RelaxF fl, RelaxF fh, const int32_t* __restrict__ fb, HeavyList H,
+                                                    int heavy, uint64_t delta, int G, int64_t cap, int32_t max_rounds,
+                                                    int64_t* io) {
+    __shared__ FarLds sh;
+    const SsspBatch& B = fl.B;
+    const int k = B.k;
+    int par = (int)ldAcq(io);
+    for (int32_t r = 0; r < max_rounds; ++r) {
+        int64_t mx = 0;
+        for (int s = 0; s < k; ++s) mx = std::max<int64_t>(mx, ldAcq(nearCount(B, par, s)));
+        if (mx == 0 || mx > cap) break;
+        if (blockIdx.x == 0 && threadIdx.x < k) resetRound(B, par, threadIdx.x);
+        fl.par = par;
+        fh.par = par;
+        for (int s = 0; s < k; ++s) relaxQueue(fl, fb, H, B.qn[par] + (size_t)s * B.V, ldAcq(nearCount(B, par, s)),
+                                               s, heavy != 0);
+        gridSync();
+        for (int s = 0; s < k; ++s) compactPhase(B, par, heavy, G, s, sh);
+        gridSync();
+        if (heavy) {
+            for (int s = 0; s < k; ++s) relaxQueue(fh, fb, H, B.qh + (size_t)s * B.V, ldAcq(&B.ctl[C_HQ + s]), s,
+                                                   false);
+            gridSync();
+            for (int s = 0; s < k; ++s) farMinPhase(B, par, G, s, sh);
+            gridSync();
+        }
+        for (int s = 0; s < k; ++s) splitPhase(B, par, delta, G, s, sh);
+        gridSync();
+        par ^= 1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *io = par;
 }
 
 __global__ void kReachedArcsD(const uint64_t* __restrict__ dist, const int32_t* __restrict__ beg, int32_t V,
@@ -374,7 +419,28 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
     // are latency-bound, so they want many waves)
     const int G = 1;
     const unsigned sgrid = summaryGrid(B.W2, G, 2, blk);
+    // persistent rounds need every row below the heavy-list threshold
+    bool persist_ok = h->prm.persist != 0 && maxDegree(h) <= HEAVY_T;
+    const int64_t cap = std::max<int64_t>(1, (int64_t)V / 64);  // queues up to this size stay persistent
     while (true) {
+        if (persist_ok && maxn <= cap) {
+            int64_t* io = S.s_ctl.p + C_SLOTS;
+            fillI64(h, io, par, 1);
+            if (launchPersistent(h, "sssp_persist", kSsspPersist, 256, 2, fl[0], fh[0], fb, heavyList(h), heavy, delta,
+                                 4, cap, (int32_t)0x7fffffff, io)) {
+                int64_t pr;
+                readCounters(h, io, 1, &pr);
+                par = (int)pr;
+                readCounters(h, S.s_ctl.p + (par ? C_NEAR1 : C_NEAR0), SB, r.data());
+                int64_t mx = 0;
+                for (int s = 0; s < k; ++s) mx = std::max(mx, r[s]);
+                if (mx == 0) break;
+                maxn = mx;
+                R = 1;
+                continue;
+            }
+            persist_ok = false;  // no cooperative launch here: host-driven rounds
+        }
         const unsigned rgrid = R > 1 ? std::min<unsigned>(1024, std::max<unsigned>(64, gridFor(4 * maxn, blk)))
                                      : gridFor(maxn, blk);
         for (int t = 0; t < R; ++t) {
@@ -431,7 +497,7 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         S.s_nsum.ensure((size_t)kc * W2s);
         S.s_fsum.ensure((size_t)kc * W2s);
         S.s_dsum.ensure((size_t)kc * W2s);
-        S.s_ctl.ensure(C_SLOTS);
+        S.s_ctl.ensure(C_SLOTS + 1);  // + the persistent kernel's parity slot
         S.src.ensure((size_t)(k > 0 ? k : 1));
         const int32_t* dsrc = S.src.p;
         copyIn(h, S.src.p, s.data(), sizeof(int32_t) * (size_t)k);

@@ -60,6 +60,7 @@ _SIGS = {
     "girhip_get_dag": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "girhip_get_ranks": (ctypes.c_int, [_vp, _vp]),
     "girhip_bfs": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, _vp]),
+    "girhip_bfs_component": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
     "girhip_sssp": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_uint64, _vp, _vp]),
     "girhip_apply_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]),
     "girhip_ingest_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]),
@@ -398,6 +399,18 @@ class Graph:
         return levels, dist
 
     # ---- tuning ----
+    def bfs_component(self, sources, undirected=False, out=None):
+        """ParallelBFS harness (gir_test.cpp:143-343): BFS inside each
+        source's SCC (directed) or WCC (undirected) -> (levels k x V, visited
+        per source)."""
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
+        k = len(src)
+        lv = out if out is not None else np.empty((k, self.V), np.int32)
+        vis = np.zeros(k, np.int64)
+        _check(lib().girhip_bfs_component(self._h, src.ctypes.data if k else None, k, int(bool(undirected)),
+                                          _ptr(lv, np.int32, k * self.V, "levels"), vis.ctypes.data))
+        return lv, vis
+
     def features(self, k):
         """16 graph/query features (see csrc/tune.hip) for k query sources."""
         f = np.empty(16, np.float32)

@@ -1,4 +1,6 @@
 // Drop-in GIR front end over the gfx950 C ABI.  See gir.h.
+#include <chrono>
+#include <iostream>
 #include "gir.h"
 
 #include <sys/stat.h>
@@ -323,6 +325,29 @@ QueryVec<uint64_t> GIR::ssspMany(const std::vector<int>& sources, uint64_t delta
     return d;
 }
 QueryVec<int> GIR::bfs(int source) { return bfsMany({source}); }
+
+QueryVec<int> GIR::bfsInComponent(int source, bool directed, long long* visited) {
+    if (directed ? !built_scc : !built_wcc) {
+        if (directed) buildSCC_DAG();
+        else buildWCC();
+    }
+    QueryVec<int> lv((size_t)V);
+    int64_t vis = 0;
+    const int32_t s = source;
+    check(girhip_bfs_component(h, &s, 1, directed ? 0 : 1, lv.data(), &vis), "bfsInComponent");
+    if (visited) *visited = (long long)vis;
+    return lv;
+}
+
+QueryVec<int> ParallelBFS::testParallelBFS(GIR& gir, int source, bool is_directed) {
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    long long visited = 0;
+    QueryVec<int> lv = gir.bfsInComponent(source, is_directed, &visited);
+    const std::chrono::duration<double> dt = std::chrono::high_resolution_clock::now() - t0;
+    std::cout << "BFS completed in: " << dt.count() * 1000.0 << " ms" << std::endl;
+    std::cout << "Nodes visited in component: " << visited << std::endl;
+    return lv;
+}
 QueryVec<uint64_t> GIR::sssp(int source, uint64_t delta) { return ssspMany({source}, delta); }
 
 void GIR::setWeights(const uint32_t* w_fw) { check(girhip_set_weights(h, w_fw), "setWeights"); }

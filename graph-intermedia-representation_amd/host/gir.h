@@ -128,6 +128,9 @@ public:
 
     girhip_graph* handle() const { return h; }
 
+    // BFS inside the source's component (the ParallelBFS harness below).
+    QueryVec<int> bfsInComponent(int source, bool directed, long long* visited = nullptr);
+
 private:
     void init(const int32_t* fb, const int32_t* fa, const int32_t* bb, const int32_t* ba, int32_t V_, int64_t E_);
     void refresh() const;  // pull labels after a change
@@ -142,6 +145,15 @@ private:
     mutable std::vector<std::vector<int>> sccs;
     mutable bool sccs_ok = false;
     mutable std::unique_ptr<graph> g_copy;
+};
+
+// The reference's ParallelBFS harness (gir_test.cpp:143-343): a
+// direction-optimising BFS from source restricted to the source's SCC
+// (is_directed, over out-arcs) or WCC (over the undirected closure).  Like
+// the reference it prints the elapsed time and "Nodes visited in component";
+// it also returns the levels (-1 outside the component).
+struct ParallelBFS {
+    static QueryVec<int> testParallelBFS(GIR& gir, int source, bool is_directed);
 };
 
 #endif

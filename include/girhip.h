@@ -109,12 +109,25 @@ int girhip_get_dag(girhip_graph* h, int32_t* src, int32_t* dst, int32_t* mult);
 int girhip_get_ranks(girhip_graph* h, int64_t* rank);
 
 /* Replaces GIR::bfs (gir_query.cpp:108-294) for k sources.  levels holds
- * k*V int32 (row i = hops from src[i], -1 unreachable).  May be NULL to
- * leave the result only in girhip_last_levels' device buffer.  When
+ * k*V int32 (row i = hops from src[i], -1 unreachable).  May be NULL: the
+ * levels are then computed into scratch and not returned (timing, or
+ * edges_out only).  When
  * edges_out is not NULL, edges_out[i] receives m_s: the out-arc count summed
  * over the vertices reached from src[i] (the traversed-arc count of the
  * GTEPS figure). */
 int girhip_bfs(girhip_graph* h, const int32_t* src, int32_t k, int32_t* levels, int64_t* edges_out);
+
+/* Replaces the reference's ParallelBFS harness (ParallelBFS::testParallelBFS
+ * with TDStep / BUStep, gir_test.cpp:143-343): direction-optimising BFS from
+ * each source restricted to the source's component.  undirected == 0: out-
+ * arcs, vertices of the source's SCC (needs girhip_scc_dag); undirected !=
+ * 0: the symmetric closure, vertices of the source's WCC (needs girhip_wcc).
+ * levels holds k*V int32 (-1 outside the component or unreachable, may be
+ * NULL); visited (k entries, may be NULL) the vertices reached per source
+ * (the harness's "Nodes visited in component").  GIRHIP_ESTATE if the
+ * layer was never built. */
+int girhip_bfs_component(girhip_graph* h, const int32_t* src, int32_t k, int32_t undirected, int32_t* levels,
+                         int64_t* visited);
 
 /* Replaces GIR::sssp (gir_query.cpp:695-703) for k sources: delta-stepping
  * with bucket width delta (0 = the handle's tuned default, 16 unless
@@ -162,7 +175,10 @@ int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
  *   "scc_color_min" (0: vertices left before colouring replaces FW-BW),
  *   "msbfs" (1: k > 1 BFS sources share bit-parallel sweeps of 64),
  *   "chunk" (16: most BFS levels / SSSP rounds queued per host readback
- *   while frontiers are tiny, 1..16), "profile" (0: 1 times every kernel)
+ *   while frontiers are tiny, 1..16), "persist" (0; 1: tiny-frontier BFS
+ *   levels, reach levels and SSSP rounds run inside one persistent
+ *   cooperative kernel with grid barriers instead of host-driven launches),
+ *   "profile" (0: 1 times every kernel)
  * Values only change work order; results never depend on them. */
 int girhip_set_param(girhip_graph* h, const char* name, double value);
 int girhip_get_param(const girhip_graph* h, const char* name, double* value);

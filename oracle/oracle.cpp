@@ -485,6 +485,39 @@ extern "C" int orc_bfs_textbook(int32_t V, const int32_t* fb, const int32_t* fc,
     return 0;
 }
 
+// ParallelBFS::testParallelBFS (gir_test.cpp:143-343): hop levels from src
+// restricted to the vertices whose component label equals src's label
+// (TDStep/BUStep test nodeToComponent.at(v) == source_comp_id,
+// gir_test.cpp:158, 200).  directed: out-arcs (fw) with SCC labels;
+// undirected: the symmetric closure (fw and bw rows) with WCC labels.  The
+// order of a level does not change its set, so a plain queue BFS restates
+// it.  *visited = vertices with a level.
+extern "C" int orc_bfs_component(int32_t V, const int32_t* fb, const int32_t* fc, const int32_t* bb,
+                                 const int32_t* bc, const int32_t* label, int32_t undirected, int32_t src,
+                                 int32_t* level, int64_t* visited) {
+    if (V < 0 || (V > 0 && (!fb || !label || !level)) || (undirected && V > 0 && !bb)) return 1;
+    for (int32_t i = 0; i < V; ++i) level[i] = -1;
+    if (visited) *visited = 0;
+    if (src < 0 || src >= V) return 0;
+    const int32_t c = label[src];
+    std::vector<int32_t> q{src};
+    level[src] = 0;
+    auto visit = [&](int32_t u, int32_t w) {
+        if (label[w] == c && level[w] == -1) {
+            level[w] = level[u] + 1;
+            q.push_back(w);
+        }
+    };
+    for (size_t h = 0; h < q.size(); ++h) {
+        const int32_t u = q[h];
+        for (int32_t e = fb[u]; e < fb[u + 1]; ++e) visit(u, fc[e]);
+        if (undirected)
+            for (int32_t e = bb[u]; e < bb[u + 1]; ++e) visit(u, bc[e]);
+    }
+    if (visited) *visited = (int64_t)q.size();
+    return 0;
+}
+
 extern "C" int orc_dijkstra(int32_t V, const int32_t* fb, const int32_t* fc, const uint32_t* w_fw, int32_t src,
                             uint64_t* dist) {
     if (V < 0 || (V > 0 && (!fb || !dist))) return 1;

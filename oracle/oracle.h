@@ -20,6 +20,9 @@
  *   orc_scc        golden vectors; orc_scc_kosaraju (a different algorithm)
  *   orc_dag        golden DAG arc sets; hand-derived example_graph answers
  *   orc_bfs        golden levels; orc_bfs_textbook (plain queue BFS)
+ *   orc_bfs_component  ParallelBFS harness: BFS inside the source's SCC /
+ *                  WCC; pinned by |SCC| = visited and by orc_bfs_textbook
+ *                  on a graph whose labels are all equal
  *   orc_sssp       golden distances (hash weights); orc_dijkstra
  *   orc_edge_weight  golden weights from the compiled reference
  *   orc_normalize_batch / orc_apply_batch  golden post-batch partitions;
@@ -91,6 +94,10 @@ int orc_apply_batch(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, int
 int orc_bfs_textbook(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, int32_t src, int32_t* level);
 int orc_dijkstra(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const uint32_t* w_fw, int32_t src,
                  uint64_t* dist);
+/* Component-restricted BFS of the reference's ParallelBFS harness. */
+int orc_bfs_component(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,
+                      const int32_t* bw_adj, const int32_t* label, int32_t undirected, int32_t src, int32_t* level,
+                      int64_t* visited);
 int orc_wcc_bfs(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,
                 const int32_t* bw_adj, int32_t* label);
 int orc_scc_kosaraju(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,

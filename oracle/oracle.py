@@ -32,6 +32,7 @@ _SIGS = {
                                        ctypes.POINTER(_i64)]),
     "orc_bfs_textbook": (ctypes.c_int, [_i32, _vp, _vp, _i32, _vp]),
     "orc_dijkstra": (ctypes.c_int, [_i32, _vp, _vp, _vp, _i32, _vp]),
+    "orc_bfs_component": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp]),
     "orc_wcc_bfs": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp]),
     "orc_scc_kosaraju": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp]),
     "orc_num_threads": (ctypes.c_int, []),
@@ -147,6 +148,19 @@ def bfs_textbook(fb, fa, src):
     lv = np.empty(len(fb) - 1, np.int32)
     _chk(lib().orc_bfs_textbook(len(fb) - 1, _p(fb), _p(fa), int(src), _p(lv)), "bfs_textbook")
     return lv
+
+
+def bfs_component(fb, fa, bb, ba, label, src, undirected):
+    """ParallelBFS::testParallelBFS (gir_test.cpp:143-343): levels inside
+    src's component (label = SCC labels, directed; WCC labels, undirected)
+    -> (levels, visited count)."""
+    V = len(fb) - 1
+    lv = np.empty(V, np.int32)
+    vis = np.zeros(1, np.int64)
+    label = np.ascontiguousarray(label, np.int32)
+    _chk(lib().orc_bfs_component(V, _p(fb), _p(fa), _p(bb), _p(ba), _p(label), int(bool(undirected)), int(src),
+                                 _p(lv), _p(vis)), "bfs_component")
+    return lv, int(vis[0])
 
 
 def dijkstra(fb, fa, w, src):

@@ -128,6 +128,15 @@ def test_grid_parity():
         d = G.sssp(src, 0)
         for i, x in enumerate(src):
             assert (d[i] == oracle.dijkstra(fb, fa, w, x)).all()
+        # the persistent cooperative kernels (reach levels of the SCC sweep,
+        # SSSP rounds with and without a heavy phase) give the same results
+        say("grid persistent kernels")
+        G.set_param("persist", 1)
+        s0 = G.get_scc()[0].copy()
+        assert (G.scc_dag()[0] == s0).all()
+        for delta in (0, 100, 2000):
+            assert (G.sssp(src, delta) == d).all(), delta
+        assert (G.bfs(src) == lv).all()
 
 
 def test_bfs_many_levels_bit_parallel():
@@ -147,6 +156,13 @@ def test_bfs_many_levels_bit_parallel():
         G.set_param("msbfs", 0)
         assert (G.bfs(src[:3]) == lv[:3]).all()
         G.set_param("msbfs", 1)
+        # persistent cooperative kernel, and one level per readback
+        G.set_param("persist", 1)
+        assert (G.bfs(src) == lv).all()
+        G.set_param("persist", 0)
+        G.set_param("chunk", 1)
+        assert (G.bfs(src) == lv).all()
+        G.set_param("chunk", 16)
         G.set_param("bfs_alpha", 1)
         G.set_param("bfs_min_reach", 0)
         assert (G.bfs(src) == lv).all()
@@ -161,6 +177,7 @@ def test_tuning_knobs_do_not_change_results():
         for k, v in [("msbfs", 0), ("block", 64), ("block", 1024), ("bfs_alpha", 1), ("bfs_beta", 1000),
                      ("bfs_min_reach", 0),
                      ("wcc_rounds", 0), ("scc_trim_rounds", 0), ("scc_pivot", 0), ("sssp_delta", 1),
+                     ("persist", 1), ("chunk", 1),
                      ("sssp_delta", 100000)]:
             say("knob", k, v)
             old = G.get_param(k)
@@ -291,3 +308,22 @@ def test_ingest_then_sync():
         G.sync()
         assert G.synced()
         check_layers(G, nfb, nfa, nbb, nba, "sync", rebuild=False)
+
+
+def test_bfs_in_component_harness():
+    # ParallelBFS::testParallelBFS (gir_test.cpp:143-343): BFS restricted to
+    # the source's SCC (directed) / WCC (undirected) against the oracle
+    for V, fb, fa, bb, ba in [girhip.rmat_graph(12, edge_factor=4, seed=36),
+                              girhip.grid_graph(60, 90, seed=37)]:
+        say("component bfs V", V)
+        src = np.concatenate([girhip.sources(fb, 6, 2), [-1]]).astype(np.int32)
+        with girhip.Graph(fb, fa, bb, ba) as G:
+            with pytest.raises(girhip.GirError):
+                G.bfs_component(src[:1])  # SCC layer not built yet
+            scc, _, _ = G.scc_dag()
+            wcc, _ = G.wcc()
+            for und, lab in [(False, scc), (True, wcc)]:
+                lv, vis = G.bfs_component(src, undirected=und)
+                for i, x in enumerate(src):
+                    want, wv = oracle.bfs_component(fb, fa, bb, ba, lab, x, und)
+                    assert (lv[i] == want).all() and vis[i] == wv, (und, x)

@@ -150,3 +150,45 @@ def test_planted_mistake_is_caught():
     i = int(np.nonzero(bad != np.arange(len(bad)))[0][0])
     bad[i] = i
     assert not (bad == g["scc"]).all()
+
+
+def _sym_csr(fb, fa):
+    """Symmetric closure (both directions, no self loops) as CSR."""
+    V = len(fb) - 1
+    rows = np.repeat(np.arange(V, dtype=np.int64), np.diff(fb))
+    cols = fa.astype(np.int64)
+    keep = rows != cols
+    s = np.concatenate([rows[keep], cols[keep]])
+    d = np.concatenate([cols[keep], rows[keep]])
+    k = np.unique(s * V + d)
+    beg = np.zeros(V + 1, np.int32)
+    np.add.at(beg, (k // V).astype(np.int64) + 1, 1)
+    return np.cumsum(beg).astype(np.int32), (k % V).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_oracle_bfs_component_pinned(seed):
+    # ParallelBFS harness (gir_test.cpp:143-343) restated: pinned by
+    # (i) a single label reduces it to the textbook BFS, (ii) inside an SCC
+    # every member is reached (visited = |SCC|, levels only on members),
+    # (iii) undirected + WCC labels equals textbook BFS on the symmetric
+    # closure and reaches exactly the WCC.
+    V, fb, fa, bb, ba = girhip.rmat_graph(11, edge_factor=4, seed=seed)
+    src = girhip.sources(fb, 6, seed)
+    one = np.zeros(V, np.int32)
+    scc = oracle.scc_kosaraju(fb, fa, bb, ba)
+    wcc = oracle.wcc_bfs(fb, fa, bb, ba)
+    sb, sa = _sym_csr(fb, fa)
+    for x in src:
+        lv, vis = oracle.bfs_component(fb, fa, bb, ba, one, x, False)
+        assert (lv == oracle.bfs_textbook(fb, fa, x)).all() and vis == (lv >= 0).sum()
+        lv, vis = oracle.bfs_component(fb, fa, bb, ba, scc, x, False)
+        members = scc == scc[x]
+        assert vis == members.sum() and ((lv >= 0) == members).all()
+        free = oracle.bfs_textbook(fb, fa, x)
+        assert (lv[members] >= free[members]).all()  # a restriction can only lengthen paths
+        lv, vis = oracle.bfs_component(fb, fa, bb, ba, wcc, x, True)
+        assert (lv == oracle.bfs_textbook(sb, sa, x)).all()
+        assert vis == (wcc == wcc[x]).sum()
+    lv, vis = oracle.bfs_component(fb, fa, bb, ba, scc, -1, False)
+    assert vis == 0 and (lv == -1).all()
