@@ -182,7 +182,8 @@ __device__ __forceinline__ void drainGroup(uint64_t* bits, uint64_t* sum, int64_
 // LDS of the far-bitmap passes: per wave, WB_CAP ids and 64 bitmap words.
 struct FarLds {
     int32_t ids[1024 / WAVE][WB_CAP];
-    uint64_t w[1024 / WAVE][64];
+    uint64_t w[1024 / WAVE][64];  // keep bits of the wave's 64 words
+    uint64_t t[1024 / WAVE][64];  // take bits
 };
 
 // Smallest distance >= hi over one summary word of source s's far bitmap
@@ -267,26 +268,31 @@ __device__ __forceinline__ void splitPhase(const SsspBatch& B, int par, uint64_t
     const int wv = threadIdx.x / WAVE;
     int32_t* lds = sh.ids[wv];
     uint64_t* keepw = sh.w[wv];
+    uint64_t* takew = sh.t[wv];
     forSummary(fs, B.W2, G, [&](int64_t g, uint64_t sm) {
         const int l = laneId();
         const int64_t wi = g * 64 + l;
         const uint64_t m = ((sm >> l) & 1) ? fbits[wi] : 0;
         keepw[l] = 0;
+        takew[l] = 0;
         waveSync();
         waveBits(m, wi, lds, [&](bool act, int32_t x) {
-            bool take = false;
-            if (act) {
-                const uint64_t v = d[x];
-                if (v >= hi) atomicOr((unsigned long long*)&keepw[(x >> 6) & 63], 1ULL << (x & 63));
-                else take = v >= lo;
-            }
-            const int64_t pos = waveReserve(take ? 1 : 0, cnt);
-            if (take) q[pos] = x;
+            if (!act) return;
+            const uint64_t v = d[x];
+            uint64_t* mw = v >= hi ? keepw : (v >= lo ? takew : nullptr);
+            if (mw) atomicOr((unsigned long long*)&mw[(x >> 6) & 63], 1ULL << (x & 63));
         });
         waveSync();
         const uint64_t keep = keepw[l];
+        uint64_t take = takew[l];
         if (keep != m) fbits[wi] = keep;
         if (__ballot(keep != 0) == 0 && l == 0) fs[g] = 0;
+        int64_t pos = waveReserve(__popcll(take), cnt);
+        while (take) {
+            const int b = __ffsll((long long)take) - 1;
+            take &= take - 1;
+            q[pos++] = (int32_t)(wi * 64 + b);
+        }
     });
 }
 

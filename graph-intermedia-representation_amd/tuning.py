@@ -91,6 +91,7 @@ def collect(n_graphs, seed, out_path, budget_s=None):
                 src = girhip.sources(fb, k, int(rng.integers(1 << 30)))
                 f = G.features(k)
                 # delta ladder from the top; stop once far slower than the best
+                # or slow in absolute terms (the slope below is then known)
                 best = 1e300
                 for delta in reversed(DELTAS):
                     t = time_sssp(G, src, delta)
@@ -100,7 +101,7 @@ def collect(n_graphs, seed, out_path, budget_s=None):
                     ms.append(t)
                     gids.append(gi * 10 + k)
                     best = min(best, t)
-                    if t > 8 * best and delta < 64:
+                    if (t > 8 * best and delta < 64) or (t > 3 * best and t > 1500):
                         break
             f = G.features(KS[0])
             _, times = G.autotune("scc_trim_rounds", [], TRIMS, reps=1)
@@ -112,9 +113,10 @@ def collect(n_graphs, seed, out_path, budget_s=None):
                 gids.append(gi * 10)
         names.append(name)
         log(f"graph {gi} {name} V={V} E={len(fa)} w={wmax or 'hash'} records={len(ms)} {time.time() - t0:.0f}s")
-    np.savez_compressed(out_path, feat=np.array(feats, np.float32), kind=np.array(kinds, np.int8),
-                        value=np.array(vals, np.float64), ms=np.array(ms, np.float64), gid=np.array(gids, np.int64),
-                        names=np.array(names))
+        # saved after every graph, so an interrupted run keeps what it measured
+        np.savez_compressed(out_path, feat=np.array(feats, np.float32), kind=np.array(kinds, np.int8),
+                            value=np.array(vals, np.float64), ms=np.array(ms, np.float64),
+                            gid=np.array(gids, np.int64), names=np.array(names))
     return len(ms)
 
 
