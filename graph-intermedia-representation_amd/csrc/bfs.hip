@@ -118,7 +118,7 @@ struct MsPushSF {
     MsBits M;
     __device__ int64_t operator()(int32_t, uint64_t fp, int32_t, int64_t j) const {
         const int32_t w = fa[j];
-        const uint64_t m = fp & ~(seen[w] | ldAcq(&next[w]));
+        const uint64_t m = fp & ~(seen[w] | next[w]);  // (next only gains bits here: safe pre-check)
         if (m && atomicOr((unsigned long long*)&next[w], (unsigned long long)m) == 0) setBit(M.T, M.TS, w);
         return 0;
     }
@@ -260,7 +260,7 @@ struct MsPushF {
     uint64_t* next;
     __device__ int64_t operator()(int32_t, uint64_t fp, int32_t, int64_t j) const {
         const int32_t w = fa[j];
-        const uint64_t m = fp & ~(seen[w] | ldAcq(&next[w]));
+        const uint64_t m = fp & ~(seen[w] | next[w]);
         if (m) atomicOr((unsigned long long*)&next[w], (unsigned long long)m);
         return 0;
     }

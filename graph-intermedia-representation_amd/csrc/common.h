@@ -506,11 +506,11 @@ __device__ __forceinline__ int64_t waveReserve(int n, int64_t* cnt) {
 __device__ __forceinline__ bool setBit(uint64_t* bits, uint64_t* sum, int32_t x) {
     const uint64_t bit = 1ULL << (x & 63);
     uint64_t* wp = &bits[x >> 6];
-    if (ldAcq(wp) & bit) return false;
+    if (*wp & bit) return false;  // (bits are only set while a kernel runs: a stale read is a safe pre-check)
     if (atomicOr((unsigned long long*)wp, (unsigned long long)bit) != 0) return false;
     const uint64_t sb = 1ULL << ((x >> 6) & 63);
     uint64_t* sp = &sum[x >> 12];
-    if (!(ldAcq(sp) & sb)) atomicOr((unsigned long long*)sp, (unsigned long long)sb);
+    if (!(*sp & sb)) atomicOr((unsigned long long*)sp, (unsigned long long)sb);
     return true;
 }
 

@@ -58,7 +58,7 @@ struct TopDownF {
         bool claim = false;
         if (flt(w, p)) {
             const uint64_t bit = 1ULL << (w & 63);
-            if (!(ldAcq(&vis[w >> 6]) & bit))
+            if (!(vis[w >> 6] & bit))  // (visited bits are only set: safe pre-check)
                 claim = !(atomicOr((unsigned long long*)&vis[w >> 6], (unsigned long long)bit) & bit);
         }
         const int64_t pos = wavePush(&cnt[0], claim);

@@ -111,7 +111,7 @@ struct RelaxF {
             if ((wt > delta) != (heavy != 0)) return 0;
         }
         const uint64_t nd = dp + wt;
-        if (nd < ldAcq(&d[x])) {
+        if (nd < d[x]) {  // plain read: distances only fall, so a stale value is a safe pre-check
             const uint64_t old = atomicMin((unsigned long long*)&d[x], (unsigned long long)nd);
             if (nd < old) {
                 if (nd < (uint64_t)B.ctl[C_HI + s]) {
@@ -150,7 +150,7 @@ __device__ __forceinline__ void relaxQueue(const RelaxF& f, const int32_t* __res
         const int64_t i = base + laneId();
         const bool act = i < n;
         const int32_t u = act ? q[i] : 0;
-        const uint64_t du = act ? ldAcq(&d[u]) : 0;
+        const uint64_t du = act ? d[u] : 0;
         if (act && track_done) setBit(B.donebits + (size_t)s * B.W, B.dsum + (size_t)s * B.W2, u);
         const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
         waveRowsH(act, u, du, s, b, e, H, f);
