@@ -205,10 +205,19 @@ def main():
     gbs = {k: alg[k] / (med[k] * 1e-3) / 1e9 for k in alg}
     dom = max(med, key=lambda k: med[k])
     units = {"wcc": 1, "scc_dag": 1, "bfs": N_BFS, "sssp": N_SSSP}[dom]
+    # measured HBM bytes per phase call (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated), when
+    # profiles/ holds them for this workload
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{args.workload}_phase_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("phase_bytes", {}).get(dom)
     roofline = {"bound": "hbm", "achieved": round(gbs[dom], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs[dom] / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(gbs[dom] / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": f"{dom} phase ({units} unit(s) per call)",
-                "bytes_per_unit": round(alg[dom] / units), "ms_per_unit": round(med[dom] / units, 4)}
+                "bytes_per_unit": round(alg[dom] / units), "ms_per_unit": round(med[dom] / units, 4),
+                "algorithmic_bytes_per_call": round(alg[dom]), "traffic_unit": "bytes per phase call",
+                "traffic_gbs": round(traffic / (med[dom] * 1e-3) / 1e9, 1) if traffic else None}
 
     kernels = None
     if args.profile:
