@@ -31,16 +31,19 @@
 
 namespace gir {
 
+// Labels are fixed during a reachability sweep, and elsewhere they and the
+// colours only move one way (unassigned -> assigned, colour down), so plain
+// loads are safe pre-checks: a stale value only costs extra work.
 struct ActiveOnly {
     const int32_t* scc;
-    __device__ bool operator()(int32_t c, int32_t) const { return ldAcq(&scc[c]) == -1; }
-    __device__ bool self(int32_t v) const { return ldAcq(&scc[v]) == -1; }
+    __device__ bool operator()(int32_t c, int32_t) const { return scc[c] == -1; }
+    __device__ bool self(int32_t v) const { return scc[v] == -1; }
 };
 struct SameColour {
     const int32_t* scc;
     const int32_t* color;
-    __device__ bool operator()(int32_t c, int32_t p) const { return ldAcq(&scc[c]) == -1 && color[c] == color[p]; }
-    __device__ bool self(int32_t v) const { return ldAcq(&scc[v]) == -1; }
+    __device__ bool operator()(int32_t c, int32_t p) const { return scc[c] == -1 && color[c] == color[p]; }
+    __device__ bool self(int32_t v) const { return scc[v] == -1; }
 };
 
 __global__ void kCompactActive(const int32_t* __restrict__ in, int64_t n, bool all, const int32_t* __restrict__ scc,
@@ -61,7 +64,7 @@ __device__ __forceinline__ bool anyActive(const int32_t* __restrict__ beg, const
     const int32_t b = beg[v], e = beg[v + 1];
     for (int32_t j = b; j < e; ++j) {
         const int32_t w = adj[j];
-        if (w != v && ldAcq(&scc[w]) == -1) return true;
+        if (w != v && scc[w] == -1) return true;
     }
     return false;
 }
@@ -73,7 +76,7 @@ __global__ void kTrim1(const int32_t* __restrict__ list, int64_t n, const int32_
     bool trim = false;
     if (i < n) {
         const int32_t v = list[i];
-        if (ldAcq(&scc[v]) == -1) {
+        if (scc[v] == -1) {
             trim = !anyActive(fb, fa, scc, v) || !anyActive(bb, ba, scc, v);
             if (trim) stRel(&scc[v], v);
         }
@@ -145,9 +148,9 @@ struct ColorPropF {
     __device__ int64_t operator()(int32_t p, uint64_t, int32_t, int64_t j) const {
         const int32_t y = fa[j];
         bool push = false;
-        if (ldAcq(&scc[y]) == -1) {
-            const int32_t cx = ldAcq(&color[p]);
-            if (cx < ldAcq(&color[y])) {
+        if (scc[y] == -1) {
+            const int32_t cx = color[p];
+            if (cx < color[y]) {
                 const int32_t old = atomicMin(&color[y], cx);
                 push = cx < old && atomicExch(&stamp[y], it) != it;
             }
