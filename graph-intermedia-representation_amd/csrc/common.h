@@ -84,6 +84,7 @@ struct Params {
     double msbfs = 1;    // 1: k>1 BFS runs 64 sources per bit-parallel sweep
     double chunk = 16;   // most BFS levels / SSSP rounds queued per host readback
     double persist = 0;  // 1: tiny-frontier phases run in one persistent cooperative kernel
+    double sssp_dist32 = 1;  // 1: SSSP works on 32-bit distances (64-bit redo on overflow)
 };
 
 struct KStat {
@@ -116,6 +117,7 @@ struct Scratch {
     DevBuf<uint32_t> b_iw;
     // delta-stepping batch state (sized for the largest source batch)
     DevBuf<int32_t> s_q[2], s_qh;
+    DevBuf<uint32_t> s_d32;  // 32-bit distances of a batch
     DevBuf<uint64_t> s_near, s_far, s_done, s_nsum, s_fsum, s_dsum;
     DevBuf<int64_t> s_ctl;
     DevBuf<int32_t> src;  // query sources
@@ -401,6 +403,39 @@ struct HeavyList {
     int64_t* off;  // exclusive prefix of the listed row lengths (count + 1)
 };
 
+// The first (up to SMALL) arcs of every lane's row, spread over the wave:
+// the lanes' segments are concatenated (wave prefix sum of their lengths)
+// and the wave takes 64 arcs at a time, each lane finding its arc's owner by
+// a binary search over the prefix (lane shuffles).  A wave of short, uneven
+// rows then costs ceil(total / 64) steps of independent loads instead of
+// max(length) dependent steps per lane.
+template <class F>
+__device__ __forceinline__ int64_t waveFlat(int32_t v, uint64_t pay, int32_t aux, int64_t b, int cnt, const F& f) {
+    const int lane = laneId();
+    int incl = cnt;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int o = __shfl_up(incl, d);
+        if (lane >= d) incl += o;
+    }
+    const int total = __shfl(incl, WAVE - 1);
+    int64_t acc = 0;
+    for (int t0 = 0; t0 < total; t0 += WAVE) {
+        const int t = t0 + lane;
+        // owner: the first lane whose inclusive end exceeds t (a fixed six
+        // steps, so every lane takes part in every shuffle)
+        int below = 0;  // lanes whose segment ends at or before t
+        for (int step = WAVE / 2; step >= 1; step >>= 1)
+            if (__shfl(incl, below + step - 1) <= t) below += step;
+        const int own = below < WAVE ? below : WAVE - 1;
+        const int excl = __shfl(incl, own) - __shfl(cnt, own);
+        const int32_t ov = __shfl(v, own), oa = __shfl(aux, own);
+        const uint64_t op = __shfl(pay, own);
+        const int64_t ob = __shfl(b, own);
+        if (t < total) acc += f(ov, op, oa, ob + (t - excl));
+    }
+    return acc;
+}
+
 template <int SMALL = 16, class F>
 __device__ __forceinline__ int64_t waveRowsH(bool act, int32_t v, uint64_t pay, int32_t aux, int64_t b, int64_t end,
                                              const HeavyList& H, const F& f) {
@@ -413,13 +448,10 @@ __device__ __forceinline__ int64_t waveRowsH(bool act, int32_t v, uint64_t pay, 
         H.e[pos] = end;
         H.pay[pos] = pay;
     }
-    int64_t acc = 0;
     const bool mine = act && !heavy;
-    int64_t i = b;
-    if (mine) {
-        const int64_t stop = end < b + SMALL ? end : b + SMALL;
-        for (; i < stop; ++i) acc += f(v, pay, aux, i);
-    }
+    const int first = mine ? (int)(end - b < SMALL ? end - b : SMALL) : 0;
+    int64_t acc = waveFlat(v, pay, aux, b, first, f);
+    const int64_t i = b + first;
     uint64_t rest = __ballot(mine && i < end);
     while (rest) {
         const int l = __ffsll((long long)rest) - 1;

@@ -58,11 +58,36 @@ struct SsspBatch {
     uint64_t* nsum;      // [SB][W2] summaries: bit g of word i <=> word 64i+g
     uint64_t* fsum;      //          of the bitmap may be non-zero
     uint64_t* dsum;
-    int64_t* ctl;        // C_SLOTS
-    uint64_t* dist;      // [SB][V] output rows
+    int64_t* ctl;        // C_SLOTS (+ persistent parity, overflow flag)
+    void* dist;          // [SB][V] distances: uint32 (d32) or uint64
     int64_t V, W, W2;
     int k;               // sources in the batch
+    int d32;             // 1: 32-bit distances, UINT32_MAX = unreachable
 };
+
+// 32-bit distances halve the array the random relaxations hit (a batch of
+// RMAT-22 distances then fits the 256 MB Infinity Cache).  A relaxation that
+// would reach UINT32_MAX raises this flag instead, and the batch is redone
+// with 64-bit distances.
+constexpr int C_OVF = C_SLOTS + 1;
+
+__device__ __forceinline__ uint64_t dGet(const SsspBatch& B, int s, int32_t x) {
+    const size_t i = (size_t)s * B.V + x;
+    if (B.d32) {
+        const uint32_t v = ((const uint32_t*)B.dist)[i];
+        return v == 0xffffffffu ? INF : (uint64_t)v;
+    }
+    return ((const uint64_t*)B.dist)[i];
+}
+// atomicMin; returns the previous distance
+__device__ __forceinline__ uint64_t dMin(const SsspBatch& B, int s, int32_t x, uint64_t nd) {
+    const size_t i = (size_t)s * B.V + x;
+    if (B.d32) {
+        const uint32_t old = atomicMin((unsigned int*)B.dist + i, (unsigned int)nd);
+        return old == 0xffffffffu ? INF : (uint64_t)old;
+    }
+    return atomicMin((unsigned long long*)B.dist + i, (unsigned long long)nd);
+}
 
 __device__ __forceinline__ int64_t* nearCount(const SsspBatch& B, int par, int s) {
     return B.ctl + (par ? C_NEAR1 : C_NEAR0) + s;
@@ -77,10 +102,12 @@ __global__ void kBInit(SsspBatch B, const int32_t* __restrict__ src, uint64_t de
     int64_t* c = B.ctl;
     for (int j = 0; j < C_SLOTS / SB; ++j) c[j * SB + s] = 0;
     c[C_HI + s] = (int64_t)delta;
+    if (s == 0) c[C_OVF] = 0;
     if (s < B.k) {
         const int32_t v = src[s];
         if (v >= 0 && v < B.V) {
-            B.dist[(size_t)s * B.V + v] = 0;
+            if (B.d32) ((uint32_t*)B.dist)[(size_t)s * B.V + v] = 0;
+            else ((uint64_t*)B.dist)[(size_t)s * B.V + v] = 0;
             B.qn[0][(size_t)s * B.V] = v;
             c[C_NEAR0 + s] = 1;
         }
@@ -98,7 +125,6 @@ struct RelaxF {
     int heavy;
     int par;
     __device__ int64_t operator()(int32_t p, uint64_t dp, int32_t s, int64_t j) const {
-        uint64_t* d = B.dist + (size_t)s * B.V;
         uint64_t wt;
         int32_t x;
         if (w) {
@@ -111,8 +137,12 @@ struct RelaxF {
             if ((wt > delta) != (heavy != 0)) return 0;
         }
         const uint64_t nd = dp + wt;
-        if (nd < d[x]) {  // plain read: distances only fall, so a stale value is a safe pre-check
-            const uint64_t old = atomicMin((unsigned long long*)&d[x], (unsigned long long)nd);
+        if (B.d32 && nd >= 0xffffffffull) {  // beyond 32 bits: the batch is redone with 64
+            B.ctl[C_OVF] = 1;
+            return 0;
+        }
+        if (nd < dGet(B, s, x)) {  // plain read: distances only fall, so a stale value is a safe pre-check
+            const uint64_t old = dMin(B, s, x, nd);
             if (nd < old) {
                 if (nd < (uint64_t)B.ctl[C_HI + s]) {
                     // the first bit of a word raises the flag: the near bitmap
@@ -144,13 +174,12 @@ __device__ __forceinline__ void resetRound(const SsspBatch& B, int par, int s) {
 __device__ __forceinline__ void relaxQueue(const RelaxF& f, const int32_t* __restrict__ fb, HeavyList H,
                                            const int32_t* q, int64_t n, int s, bool track_done) {
     const SsspBatch& B = f.B;
-    const uint64_t* d = B.dist + (size_t)s * B.V;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
         const int64_t i = base + laneId();
         const bool act = i < n;
         const int32_t u = act ? q[i] : 0;
-        const uint64_t du = act ? d[u] : 0;
+        const uint64_t du = act ? dGet(B, s, u) : 0;
         if (act && track_done) setBit(B.donebits + (size_t)s * B.W, B.dsum + (size_t)s * B.W2, u);
         const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
         waveRowsH(act, u, du, s, b, e, H, f);
@@ -192,10 +221,9 @@ __device__ __forceinline__ uint64_t farMinGroup(const SsspBatch& B, int s, int64
                                                 uint64_t mn, int32_t* lds) {
     const int l = laneId();
     const uint64_t m = ((sm >> l) & 1) ? B.farbits[(size_t)s * B.W + g * 64 + l] : 0;
-    const uint64_t* d = B.dist + (size_t)s * B.V;
     waveBits(m, g * 64 + l, lds, [&](bool act, int32_t x) {
         if (!act) return;
-        const uint64_t v = d[x];
+        const uint64_t v = dGet(B, s, x);
         if (v >= hi && v < mn) mn = v;
     });
     return mn;
@@ -260,7 +288,6 @@ __device__ __forceinline__ void splitPhase(const SsspBatch& B, int par, uint64_t
         B.ctl[C_LO + s] = (int64_t)lo;
         B.ctl[C_HI + s] = (int64_t)hi;
     }
-    const uint64_t* d = B.dist + (size_t)s * B.V;
     int32_t* q = B.qn[par ^ 1] + (size_t)s * B.V;
     uint64_t* fs = B.fsum + (size_t)s * B.W2;
     uint64_t* fbits = B.farbits + (size_t)s * B.W;
@@ -278,7 +305,7 @@ __device__ __forceinline__ void splitPhase(const SsspBatch& B, int par, uint64_t
         waveSync();
         waveBits(m, wi, lds, [&](bool act, int32_t x) {
             if (!act) return;
-            const uint64_t v = d[x];
+            const uint64_t v = dGet(B, s, x);
             uint64_t* mw = v >= hi ? keepw : (v >= lo ? takew : nullptr);
             if (mw) atomicOr((unsigned long long*)&mw[(x >> 6) & 63], 1ULL << (x & 63));
         });
@@ -366,6 +393,14 @@ __global__ void __launch_bounds__(256) kSsspPersist(RelaxF fl, RelaxF fh, const 
     if (blockIdx.x == 0 && threadIdx.x == 0) *io = par;
 }
 
+// 32-bit distances -> the caller's 64-bit rows (UINT32_MAX -> UINT64_MAX).
+__global__ void kWiden(const uint32_t* __restrict__ in, int64_t n, uint64_t* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = in[i];
+        out[i] = v == 0xffffffffu ? INF : (uint64_t)v;
+    }
+}
+
 __global__ void kReachedArcsD(const uint64_t* __restrict__ dist, const int32_t* __restrict__ beg, int32_t V,
                               int64_t* out) {
     int64_t m = 0;
@@ -382,7 +417,8 @@ constexpr int SSSP_CHUNK = 16;
 // k <= SB sources (device array dsrc) into rows of dist.
 // Batch state lives in the handle's scratch (s_* buffers, sized for SB
 // sources by the caller).
-static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, uint64_t* dist) {
+// dist: k rows of V uint64 (d32 = false) or uint32 (d32 = true) distances.
+static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, void* dist, bool d32) {
     Scratch& S = scratch(h);
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
@@ -401,7 +437,9 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
     B.dsum = S.s_dsum.p;
     B.ctl = S.s_ctl.p;
     B.dist = dist;
-    fillU64(h, dist, INF, (int64_t)k * V);
+    B.d32 = d32 ? 1 : 0;
+    if (d32) fillI32(h, (int32_t*)dist, -1, (int64_t)k * V);
+    else fillU64(h, (uint64_t*)dist, INF, (int64_t)k * V);
     fillU64(h, B.nearbits, 0, (int64_t)k * B.W);
     fillU64(h, B.farbits, 0, (int64_t)k * B.W);
     fillU64(h, B.donebits, 0, (int64_t)k * B.W);
@@ -503,14 +541,28 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         S.s_nsum.ensure((size_t)kc * W2s);
         S.s_fsum.ensure((size_t)kc * W2s);
         S.s_dsum.ensure((size_t)kc * W2s);
-        S.s_ctl.ensure(C_SLOTS + 1);  // + the persistent kernel's parity slot
+        S.s_ctl.ensure(C_SLOTS + 2);  // + the persistent kernel's parity slot and the overflow flag
         S.src.ensure((size_t)(k > 0 ? k : 1));
         const int32_t* dsrc = S.src.p;
         copyIn(h, S.src.p, s.data(), sizeof(int32_t) * (size_t)k);
+        // 32-bit distances unless weights alone could overflow them
+        const bool try32 = h->prm.sssp_dist32 != 0 && (uint64_t)maxWeight(h) < 0x7fffffffull;
+        if (try32) S.s_d32.ensure((size_t)kc * Vs);
         tBegin(h);
         for (int32_t i0 = 0; i0 < k; i0 += SB) {
             const int c = std::min<int32_t>(SB, k - i0);
-            ssspBatch(h, dsrc + i0, c, delta, out + (size_t)i0 * V);
+            uint64_t* rows = out + (size_t)i0 * V;
+            if (try32) {
+                ssspBatch(h, dsrc + i0, c, delta, S.s_d32.p, true);
+                int64_t ovf;
+                readCounters(h, S.s_ctl.p + C_OVF, 1, &ovf);
+                if (ovf == 0) {
+                    launch(h, "sssp_widen", kWiden, boundedGrid((int64_t)c * V, 256), 256, 0,
+                           (const uint32_t*)S.s_d32.p, (int64_t)c * V, rows);
+                    continue;
+                }
+            }
+            ssspBatch(h, dsrc + i0, c, delta, rows, false);
         }
         tEnd(h, "sssp");
         if (edges_out) {

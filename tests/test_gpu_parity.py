@@ -177,7 +177,7 @@ def test_tuning_knobs_do_not_change_results():
         for k, v in [("msbfs", 0), ("block", 64), ("block", 1024), ("bfs_alpha", 1), ("bfs_beta", 1000),
                      ("bfs_min_reach", 0),
                      ("wcc_rounds", 0), ("scc_trim_rounds", 0), ("scc_pivot", 0), ("sssp_delta", 1),
-                     ("persist", 1), ("chunk", 1),
+                     ("persist", 1), ("chunk", 1), ("sssp_dist32", 0),
                      ("sssp_delta", 100000)]:
             say("knob", k, v)
             old = G.get_param(k)
@@ -327,3 +327,19 @@ def test_bfs_in_component_harness():
                 for i, x in enumerate(src):
                     want, wv = oracle.bfs_component(fb, fa, bb, ba, lab, x, und)
                     assert (lv[i] == want).all() and vis[i] == wv, (und, x)
+
+
+def test_sssp_distances_beyond_32_bits():
+    # weights near 2^30: distances pass 2^32 after a few hops, so the 32-bit
+    # pass overflows and the batch is redone with 64-bit distances
+    V, fb, fa, bb, ba = girhip.grid_graph(30, 40, seed=38)
+    w = girhip.weights(len(fa), 1 << 29, (1 << 30) + 12345, 39)
+    src = girhip.sources(fb, 3, 40)
+    say("sssp 64-bit redo")
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.set_weights(w)
+        d = G.sssp(src, 1 << 30)
+        for i, x in enumerate(src):
+            want = oracle.dijkstra(fb, fa, w, x)
+            assert (d[i] == want).all()
+        assert d[d != INF].max() > (1 << 32)
