@@ -235,6 +235,9 @@ def main():
     if os.path.exists(model_path) and not args.no_extras:
         import tuning
         model, norm = tuning.load_model(model_path)
+        # reference results with the default knobs and the step's weights
+        G.sssp(sssp_src, args.delta, out=dist)
+        G.scc_dag(out=scc)
         ref_dist = dist.clone()
         ref_scc = scc.clone()
         res = G.learned_tune(model, norm, sssp_src, judge=0)
