@@ -54,7 +54,7 @@ def make_graph(cfg):
     return girhip.grid_graph(cfg["rows"], cfg["cols"], cfg["p_keep"], cfg["p_bidir"], cfg["seed"])
 
 
-def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=None):
+def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=None, reach_pairs=None):
     """Time the compiled reference (oracle/_ref/gir_ref: the reference's own
     gir.cpp / gir_query.cpp) on this host: one buildWCC + buildSCC_DAG, BFS
     from n_bfs and SSSP (its hash weights, delta 16) from n_sssp sources, and
@@ -72,6 +72,9 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
         if batch is not None:
             np.stack([np.asarray(x, np.int32) for x in batch], axis=1).tofile(os.path.join(td, "ops.bin"))
             cmd += ["--batch", os.path.join(td, "ops.bin")]
+        if reach_pairs is not None:
+            np.ascontiguousarray(reach_pairs, np.int32).tofile(os.path.join(td, "reach.bin"))
+            cmd += ["--reach", os.path.join(td, "reach.bin")]
         env = dict(os.environ)
         env.setdefault("OMP_PROC_BIND", "close")
         t0 = time.time()
@@ -283,6 +286,20 @@ def main():
         torch.cuda.empty_cache()
         log(f"queries: {queries}")
 
+    # ---- condensation-guided reachability (GIR::reachable) on uniform pairs ----
+    reach = None
+    reach_pairs = None
+    if not args.no_extras:
+        rng = np.random.default_rng(29)
+        reach_pairs = rng.integers(0, V, size=(65536, 2)).astype(np.int32)
+        G.scc_dag(out=scc)
+        G.reachable(reach_pairs[:64, 0], reach_pairs[:64, 1])  # warm
+        ans = G.reachable(reach_pairs[:, 0], reach_pairs[:, 1])
+        r_ms = G.last_ms("reachable")
+        reach = {"pairs": len(reach_pairs), "reachable": int(ans.sum()), "device_ms": round(r_ms, 3),
+                 "pairs_per_s": round(len(reach_pairs) / (r_ms * 1e-3), 1)}
+        log(f"reachable: {reach}")
+
     # ---- update batches (50% inserts of uniform pairs, 50% deletes of arcs) ----
     updates = None
     if not args.no_extras:
@@ -316,7 +333,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (compiled reference) ...")
-        r = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, batch=first_batch if updates else None)
+        r = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, batch=first_batch if updates else None,
+                         reach_pairs=reach_pairs[:4096] if reach_pairs is not None else None)
         if r is not None:
             rec, wall = r
             b = rec.get("build", {})
@@ -334,6 +352,9 @@ def main():
             if "batch" in rec:
                 cpu["batch_ops"] = rec["batch"]["ops"]
                 cpu["batch_ms"] = rec["batch"]["ms"]
+            if "reach" in rec:
+                cpu["reach_pairs"] = rec["reach"]["pairs"]
+                cpu["reach_pairs_per_s"] = round(rec["reach"]["pairs"] / (rec["reach"]["ms"] * 1e-3), 1)
             if updates and 100000 in batches:
                 # a second run for the 10^5-op batch (on the original graph,
                 # like the GPU's first batch; the GPU applies it second)
@@ -364,7 +385,10 @@ def main():
             out["speedup_batch1e3_vs_cpu"] = round(cpu["batch_ms"] / updates[0]["ms"], 2)
         if "batch1e5_ms" in cpu and updates:
             out["speedup_batch1e5_vs_cpu"] = round(cpu["batch1e5_ms"] / updates[1]["ms"], 2)
+    if cpu and reach and cpu.get("reach_pairs_per_s"):
+        out["speedup_reachable_vs_cpu"] = round(reach["pairs_per_s"] / cpu["reach_pairs_per_s"], 2)
     out["learned_vs_default"] = learned
+    out["reachable"] = reach
     out["query_batch"] = queries
     out["update_batches"] = updates
     if kernels:

@@ -61,6 +61,7 @@ _SIGS = {
     "girhip_get_ranks": (ctypes.c_int, [_vp, _vp]),
     "girhip_bfs": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, _vp]),
     "girhip_bfs_component": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
+    "girhip_reachable": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),
     "girhip_sssp": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_uint64, _vp, _vp]),
     "girhip_apply_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]),
     "girhip_ingest_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]),
@@ -410,6 +411,18 @@ class Graph:
         _check(lib().girhip_bfs_component(self._h, src.ctypes.data if k else None, k, int(bool(undirected)),
                                           _ptr(lv, np.int32, k * self.V, "levels"), vis.ctypes.data))
         return lv, vis
+
+    def reachable(self, u, v):
+        """GIR::reachable (gir_query.cpp:789-828) for pairs (u[i], v[i]) ->
+        bool array."""
+        u = np.ascontiguousarray(np.atleast_1d(np.asarray(u, dtype=np.int32)))
+        v = np.ascontiguousarray(np.atleast_1d(np.asarray(v, dtype=np.int32)))
+        if len(u) != len(v):
+            raise ValueError("reachable: u and v differ in length")
+        out = np.zeros(len(u), np.uint8)
+        _check(lib().girhip_reachable(self._h, u.ctypes.data if len(u) else None, v.ctypes.data if len(v) else None,
+                                      len(u), out.ctypes.data))
+        return out.astype(bool)
 
     def features(self, k):
         """16 graph/query features (see csrc/tune.hip) for k query sources."""

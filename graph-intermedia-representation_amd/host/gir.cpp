@@ -1,11 +1,10 @@
 // Drop-in GIR front end over the gfx950 C ABI.  See gir.h.
-#include <chrono>
-#include <iostream>
 #include "gir.h"
 
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -325,6 +324,17 @@ QueryVec<uint64_t> GIR::ssspMany(const std::vector<int>& sources, uint64_t delta
     return d;
 }
 QueryVec<int> GIR::bfs(int source) { return bfsMany({source}); }
+
+std::vector<char> GIR::reachableMany(const std::vector<int>& u, const std::vector<int>& v) {
+    if (u.size() != v.size()) throw std::invalid_argument("reachableMany: u and v differ in length");
+    if (!built_scc) buildSCC_DAG();
+    if (!synced()) sync();
+    std::vector<uint8_t> r(u.size());
+    check(girhip_reachable(h, u.data(), v.data(), (int64_t)u.size(), r.data()), "reachable");
+    return std::vector<char>(r.begin(), r.end());
+}
+
+bool GIR::reachable(int u, int v) { return reachableMany({u}, {v})[0] != 0; }
 
 QueryVec<int> GIR::bfsInComponent(int source, bool directed, long long* visited) {
     if (directed ? !built_scc : !built_wcc) {

@@ -128,6 +128,12 @@ public:
 
     girhip_graph* handle() const { return h; }
 
+    // Condensation-guided reachability (gir_query.cpp:789-828); syncs
+    // pending ingested changes first, as the reference does.
+    bool reachable(int u, int v);
+    // Many pairs in one call (out[i] for (u[i], v[i])).
+    std::vector<char> reachableMany(const std::vector<int>& u, const std::vector<int>& v);
+
     // BFS inside the source's component (the ParallelBFS harness below).
     QueryVec<int> bfsInComponent(int source, bool directed, long long* visited = nullptr);
 

@@ -129,6 +129,14 @@ int girhip_bfs(girhip_graph* h, const int32_t* src, int32_t k, int32_t* levels, 
 int girhip_bfs_component(girhip_graph* h, const int32_t* src, int32_t k, int32_t undirected, int32_t* levels,
                          int64_t* visited);
 
+/* Replaces GIR::reachable (gir_query.cpp:789-828) for n pairs: out[i] = 1
+ * if v[i] is reachable from u[i] (every vertex reaches itself; out-of-range
+ * ids give 0).  Answered on the condensation: same SCC -> 1, source SCC
+ * ranked after the target's -> 0, otherwise a rank-pruned search of the DAG
+ * (64 pairs per bit-parallel sweep).  Pending ingested changes are synced
+ * first, as the reference does.  Needs girhip_scc_dag (GIRHIP_ESTATE). */
+int girhip_reachable(girhip_graph* h, const int32_t* u, const int32_t* v, int64_t n, uint8_t* out);
+
 /* Replaces GIR::sssp (gir_query.cpp:695-703) for k sources: delta-stepping
  * with bucket width delta (0 = the handle's tuned default, 16 unless
  * changed by girhip_set_param).  dist holds k*V uint64, UINT64_MAX =

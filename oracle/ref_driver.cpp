@@ -8,7 +8,7 @@
 // results with canonical (minimum-vertex) component ids:
 //
 //   gir_ref CSR_DIR OUT_DIR [--bfs SRC.bin] [--sssp SRC.bin] [--delta D]
-//           [--batch OPS.bin] [--pairs PAIRS.bin] [--no-output] [--reps R]
+//           [--batch OPS.bin] [--pairs PAIRS.bin] [--reach PAIRS.bin] [--no-output] [--reps R]
 //
 //   wcc.bin  int32[V]   buildWCC labels (gir.cpp:593-655), canonical
 //   scc.bin  int32[V]   buildSCC_DAG node_scc (gir.cpp:798-839), canonical
@@ -16,6 +16,7 @@
 //   bfs.bin  int32[k*V] GIR::bfs levels per source (gir_query.cpp:108-294)
 //   sssp.bin uint64[k*V] GIR::sssp distances, delta D (gir_query.cpp:695-703)
 //   weight.bin uint32[k] GIR::edgeWeight of each (u, v) in PAIRS.bin
+//   reach.bin uint8[k]   GIR::reachable of each (u, v) pair of --reach (gir_query.cpp:789-828)
 //   post_{wcc,scc,dag}.bin  the same after GIR::applyBatch of OPS.bin
 //                        (int32 triples u, v, insert) with blocks on demand
 // One JSON line per timed phase goes to stdout, prefixed "@@".
@@ -109,13 +110,13 @@ void writeLayers(const std::string& out, const std::string& pre, const Layers& L
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--pairs F] [--no-output] [--reps R]\n");
+        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--pairs F] [--reach F] [--no-output] [--reps R]\n");
         return 2;
     }
     std::string dir = argv[1], out = argv[2];
     if (dir.back() != '/') dir += '/';
     if (out.back() != '/') out += '/';
-    std::string bfs_f, sssp_f, batch_f, pairs_f;
+    std::string bfs_f, sssp_f, batch_f, pairs_f, reach_f;
     uint64_t delta = 16;
     bool write = true;
     int reps = 1;
@@ -127,6 +128,7 @@ int main(int argc, char** argv) {
         else if (a == "--delta") delta = std::strtoull(next().c_str(), nullptr, 10);
         else if (a == "--batch") batch_f = next();
         else if (a == "--pairs") pairs_f = next();
+        else if (a == "--reach") reach_f = next();
         else if (a == "--no-output") write = false;
         else if (a == "--reps") reps = std::atoi(next().c_str());
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
@@ -178,6 +180,14 @@ int main(int argc, char** argv) {
         }
         std::printf("@@{\"phase\": \"sssp\", \"sources\": %zu, \"ms\": %.3f, \"keep\": %llu}\n", src.size(), msSince(t0), keep);
         if (write) writeVec(out + "sssp.bin", dv.data(), dv.size());
+    }
+    if (!reach_f.empty()) {
+        auto pr = readVec<int>(reach_f);
+        std::vector<uint8_t> r(pr.size() / 2);
+        auto t0 = Clock::now();
+        for (size_t i = 0; i < r.size(); ++i) r[i] = g.reachable(pr[2 * i], pr[2 * i + 1]) ? 1 : 0;
+        std::printf("@@{\"phase\": \"reach\", \"pairs\": %zu, \"ms\": %.3f}\n", r.size(), msSince(t0));
+        if (write) writeVec(out + "reach.bin", r.data(), r.size());
     }
     if (!batch_f.empty()) {
         // The reference's drivers build the block layer before any update

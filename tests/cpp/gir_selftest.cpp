@@ -211,6 +211,24 @@ static void checkQueries(GIR& g, int V, const Edges& E, std::mt19937& rng, const
         auto d = g.sssp(s);
         auto wd = truthDijkstra(V, E, s);
         CHECK(std::equal(d.begin(), d.end(), wd.begin()), "%s: sssp(%d)", what, s);
+        // reachable(s, x) (gir_query.cpp:789-828) == x has a BFS level
+        std::vector<int> us(V, s), vs(V);
+        for (int x = 0; x < V; ++x) vs[x] = x;
+        auto r = g.reachableMany(us, vs);
+        bool ok = true;
+        for (int x = 0; x < V; ++x) ok = ok && ((r[x] != 0) == (want[x] >= 0));
+        CHECK(ok, "%s: reachable(%d, *)", what, s);
+        // the harness: BFS inside s's SCC reaches exactly the SCC
+        long long vis = 0;
+        auto lc = g.bfsInComponent(s, true, &vis);
+        const int c = g.sccOf(s);
+        long long members = 0;
+        bool inside = true;
+        for (int x = 0; x < V; ++x) {
+            members += g.sccOf(x) == c;
+            inside = inside && ((lc[x] >= 0) == (g.sccOf(x) == c));
+        }
+        CHECK(inside && vis == members, "%s: bfsInComponent(%d)", what, s);
     }
     auto l = g.bfs(V + 5);  // out of range: all unreachable (gir_query.cpp:112)
     CHECK(std::all_of(l.begin(), l.end(), [](int x) { return x == -1; }), "%s: bfs out of range", what);

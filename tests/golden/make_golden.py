@@ -10,6 +10,7 @@ graphs and stores inputs and outputs as compressed .npz files:
   bfs_src, bfs   GIR::bfs levels per source
   sssp_src, sssp GIR::sssp distances per source (GIR::edgeWeight weights, delta 16)
   pairs, weight  GIR::edgeWeight of sampled (u, v)
+  reach          GIR::reachable of the same pairs (gir_query.cpp:789-828)
   ops            int32 (u, v, insert) batch; post_wcc / post_scc / post_dag
                  after GIR::applyBatch (blocks on demand)
 
@@ -78,7 +79,8 @@ def run_case(name, V, fb, fa, bb, ba, n_bfs, n_sssp, n_ops, seed):
         out = os.path.join(td, "out") + "/"
         os.makedirs(out)
         subprocess.run([REF, g + "/", out, "--bfs", os.path.join(td, "b.bin"), "--sssp", os.path.join(td, "s.bin"),
-                        "--pairs", os.path.join(td, "p.bin"), "--batch", os.path.join(td, "o.bin")],
+                        "--pairs", os.path.join(td, "p.bin"), "--reach", os.path.join(td, "p.bin"),
+                        "--batch", os.path.join(td, "o.bin")],
                        check=True, stdout=subprocess.DEVNULL)
 
         def rd(f, dt):
@@ -89,7 +91,7 @@ def run_case(name, V, fb, fa, bb, ba, n_bfs, n_sssp, n_ops, seed):
             wcc=rd("wcc.bin", np.int32), scc=rd("scc.bin", np.int32), dag=rd("dag.bin", np.int32).reshape(-1, 2),
             bfs_src=bsrc, bfs=rd("bfs.bin", np.int32).reshape(len(bsrc), V),
             sssp_src=ssrc, sssp=rd("sssp.bin", np.uint64).reshape(len(ssrc), V),
-            pairs=pairs, weight=rd("weight.bin", np.uint32), ops=ops,
+            pairs=pairs, weight=rd("weight.bin", np.uint32), reach=rd("reach.bin", np.uint8), ops=ops,
             post_wcc=rd("post_wcc.bin", np.int32), post_scc=rd("post_scc.bin", np.int32),
             post_dag=rd("post_dag.bin", np.int32).reshape(-1, 2))
         print(name, "V", V, "E", len(fa))

@@ -54,6 +54,8 @@ def test_golden_cases(golden):
         assert (G.bfs(g["bfs_src"]) == g["bfs"]).all()
         say("golden sssp", name)
         assert (G.sssp(g["sssp_src"], 16) == g["sssp"]).all()
+        say("golden reachable", name)
+        assert (G.reachable(g["pairs"][:, 0], g["pairs"][:, 1]) == g["reach"].astype(bool)).all()
         say("golden batch", name)
         ops = g["ops"]
         G.apply_batch(ops[:, 0], ops[:, 1], ops[:, 2])
@@ -343,3 +345,35 @@ def test_sssp_distances_beyond_32_bits():
             want = oracle.dijkstra(fb, fa, w, x)
             assert (d[i] == want).all()
         assert d[d != INF].max() > (1 << 32)
+
+
+def test_reachable_queries():
+    # GIR::reachable (gir_query.cpp:789-828) on many pairs: trivial answers
+    # (same SCC, rank order), DAG sweeps of 64 pairs, invalid ids, and the
+    # sync of pending ingested changes before answering
+    V, fb, fa, bb, ba = girhip.rmat_graph(13, edge_factor=3, seed=41)
+    rng = np.random.default_rng(42)
+    u = rng.integers(0, V, 700).astype(np.int32)
+    v = rng.integers(0, V, 700).astype(np.int32)
+    u[:3] = [-1, 5, V]
+    v[:3] = [2, 5, 1]
+    say("reachable rmat13")
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        with pytest.raises(girhip.GirError):
+            G.reachable(u[:1], v[:1])  # SCC layer not built yet
+        G.scc_dag()
+        got = G.reachable(u, v)
+        lv = {}
+        for i, (x, y) in enumerate(zip(u, v)):
+            if not (0 <= x < V and 0 <= y < V):
+                assert not got[i]
+                continue
+            if x not in lv:
+                lv[x] = oracle.bfs_textbook(fb, fa, int(x))
+            assert got[i] == (lv[x][y] >= 0), (x, y)
+        assert 0 < got.sum() < len(got)
+        # ingest an arc that makes a previously unreachable pair reachable;
+        # reachable() syncs it first
+        i = int(np.nonzero(~got[3:])[0][0]) + 3
+        G.ingest_batch([u[i]], [v[i]], [1])
+        assert G.reachable(u[i:i + 1], v[i:i + 1])[0]

@@ -192,3 +192,14 @@ def test_oracle_bfs_component_pinned(seed):
         assert vis == (wcc == wcc[x]).sum()
     lv, vis = oracle.bfs_component(fb, fa, bb, ba, scc, -1, False)
     assert vis == 0 and (lv == -1).all()
+
+
+def test_golden_reachable_matches_textbook_bfs(golden):
+    # GIR::reachable (gir_query.cpp:789-828) answers from the compiled
+    # reference against plain BFS reachability
+    name, g = golden
+    fb, fa = g["fw_beg"], g["fw_adj"]
+    V = len(fb) - 1
+    for (u, v), r in zip(g["pairs"], g["reach"]):
+        want = u == v or (0 <= u < V and oracle.bfs_textbook(fb, fa, int(u))[v] >= 0)
+        assert bool(r) == want, (name, u, v)
