@@ -121,6 +121,10 @@ struct Scratch {
     DevBuf<uint64_t> s_near, s_far, s_done, s_nsum, s_fsum, s_dsum;
     DevBuf<int64_t> s_ctl;
     DevBuf<int32_t> src;  // query sources
+    // reachability query sweeps (a group of sweeps, V words each)
+    DevBuf<uint64_t> rq_seen, rq_next, rq_tgt, rq_fb[2];
+    DevBuf<int32_t> rq_fq[2], rq_tq, rq_pairs;
+    DevBuf<int64_t> rq_ctl, rq_lim;
     // heavy-row list
     DevBuf<int64_t> hv_ctl, hv_b, hv_e, hv_off;
     DevBuf<int32_t> hv_row, hv_aux;
@@ -164,6 +168,7 @@ struct girhip_graph {
     int32_t n_scc = 0;
     gir::DevBuf<int32_t> dag_src, dag_dst, dag_mult;
     gir::DevBuf<int32_t> dag_beg;  // DAG CSR offsets by source SCC id (V+1)
+    gir::DevBuf<int32_t> dag_in_beg, dag_in_src;  // DAG in-arcs by destination SCC id (V+1, n_dag)
     int64_t n_dag = 0;
     gir::DevBuf<int64_t> rank;
 

@@ -244,6 +244,23 @@ __global__ void kIndegFromKeys(const uint64_t* __restrict__ tk, int64_t nd, int 
     indeg[t] = (int32_t)(lb((uint64_t)(t + 1) << B) - lb((uint64_t)t << B));
 }
 
+// In-arc CSR of the DAG from its arcs' keys sorted by (dst, src).
+__global__ void kDagInCsr(const uint64_t* __restrict__ tk, int64_t nd, int B, int32_t V, int32_t* __restrict__ beg,
+                          int32_t* __restrict__ src) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nd) src[i] = (int32_t)(tk[i] & ((1ULL << B) - 1));
+    if (i <= V) {
+        const uint64_t key = (uint64_t)i << B;
+        int64_t lo = 0, hi = nd;
+        while (lo < hi) {
+            const int64_t m = lo + ((hi - lo) >> 1);
+            if (tk[m] < key) lo = m + 1;
+            else hi = m;
+        }
+        beg[i] = (int32_t)lo;
+    }
+}
+
 __global__ void kRunFlags(const uint64_t* __restrict__ keys, int64_t n, int32_t* __restrict__ flag) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
@@ -506,6 +523,16 @@ void runDag(girhip_graph* h) {
                (const int32_t*)h->dag_dst.p, nd, B, S.keys[0].p);
         radixSort(h, S.keys[0].p, nullptr, nd, 2 * B);
         launch(h, "dag_indeg", kIndegFromKeys, gridFor(V, blk), blk, 0, (const uint64_t*)S.keys[0].p, nd, B, V, ideg);
+        // the same sorted keys are the in-arc CSR of the DAG
+        h->dag_in_src.ensure((size_t)nd);
+        h->dag_in_beg.ensure((size_t)V + 1);
+        launch(h, "dag_in", kDagInCsr, gridFor(std::max<int64_t>(nd, (int64_t)V + 1), blk), blk, 0,
+               (const uint64_t*)S.keys[0].p, nd, B, V, h->dag_in_beg.p, h->dag_in_src.p);
+    }
+    if (nd == 0) {
+        h->dag_in_beg.ensure((size_t)V + 1);
+        h->dag_in_src.ensure(1);
+        fillI32(h, h->dag_in_beg.p, 0, (int64_t)V + 1);
     }
     h->n_dag = nd;
     // DAG CSR offsets by source SCC (arcs are sorted by src)
