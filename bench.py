@@ -73,13 +73,16 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
             np.stack([np.asarray(x, np.int32) for x in batch], axis=1).tofile(os.path.join(td, "ops.bin"))
             cmd += ["--batch", os.path.join(td, "ops.bin")]
         if reach_pairs is not None:
-            np.ascontiguousarray(reach_pairs, np.int32).tofile(os.path.join(td, "reach.bin"))
-            cmd += ["--reach", os.path.join(td, "reach.bin")]
+            np.ascontiguousarray(reach_pairs, np.int32).tofile(os.path.join(td, "reach_pairs.bin"))
+            cmd += ["--reach", os.path.join(td, "reach_pairs.bin")]
         env = dict(os.environ)
         env.setdefault("OMP_PROC_BIND", "close")
         t0 = time.time()
         out = subprocess.run(cmd, capture_output=True, text=True, env=env, check=True).stdout
         wall = time.time() - t0
+        reach_ans = None
+        if reach_pairs is not None and os.path.exists(os.path.join(td, "reach.bin")):
+            reach_ans = np.fromfile(os.path.join(td, "reach.bin"), np.uint8)
     rec = {}
     for line in out.splitlines():
         if line.startswith("@@"):
@@ -88,6 +91,8 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
                 rec[d["phase"]] = d
             else:
                 rec.update(d)
+    if reach_ans is not None and "reach" in rec:
+        rec["reach"]["answers"] = reach_ans
     return rec, wall
 
 
@@ -294,9 +299,9 @@ def main():
         reach_pairs = rng.integers(0, V, size=(65536, 2)).astype(np.int32)
         G.scc_dag(out=scc)
         G.reachable(reach_pairs[:64, 0], reach_pairs[:64, 1])  # warm
-        ans = G.reachable(reach_pairs[:, 0], reach_pairs[:, 1])
+        reach_ans = G.reachable(reach_pairs[:, 0], reach_pairs[:, 1])
         r_ms = G.last_ms("reachable")
-        reach = {"pairs": len(reach_pairs), "reachable": int(ans.sum()), "device_ms": round(r_ms, 3),
+        reach = {"pairs": len(reach_pairs), "reachable": int(reach_ans.sum()), "device_ms": round(r_ms, 3),
                  "pairs_per_s": round(len(reach_pairs) / (r_ms * 1e-3), 1)}
         log(f"reachable: {reach}")
 
@@ -355,6 +360,9 @@ def main():
             if "reach" in rec:
                 cpu["reach_pairs"] = rec["reach"]["pairs"]
                 cpu["reach_pairs_per_s"] = round(rec["reach"]["pairs"] / (rec["reach"]["ms"] * 1e-3), 1)
+                if reach is not None and "answers" in rec["reach"]:
+                    ra = rec["reach"]["answers"]
+                    reach["matches_reference"] = bool(np.array_equal(reach_ans[:len(ra)].astype(np.uint8), ra))
             if updates and 100000 in batches:
                 # a second run for the 10^5-op batch (on the original graph,
                 # like the GPU's first batch; the GPU applies it second)

@@ -85,6 +85,7 @@ struct Params {
     double chunk = 16;   // most BFS levels / SSSP rounds queued per host readback
     double persist = 0;  // 1: tiny-frontier phases run in one persistent cooperative kernel
     double sssp_dist32 = 1;  // 1: SSSP works on 32-bit distances (64-bit redo on overflow)
+    double reach_local = 1;  // 1: reachable() first tries a per-pair LDS search
 };
 
 struct KStat {

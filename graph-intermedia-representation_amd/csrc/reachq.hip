@@ -376,10 +376,11 @@ extern "C" int girhip_reachable(girhip_graph* h, const int32_t* u, const int32_t
         uint8_t* ans = S.b_ins.p;
         launch(h, "reach_trivial", kReachTrivial, gridFor(n, blk), blk, 0, (const int32_t*)S.b_u.p,
                (const int32_t*)S.b_v.p, n, (const int32_t*)h->scc.p, (const int64_t*)h->rank.p, V, ans, a, b);
-        launch(h, "reach_local", kReachLocal, std::min<unsigned>(gridFor(n * WAVE, 256), 2048), 256, 0,
-               (const int32_t*)h->dag_beg.p, (const int32_t*)h->dag_dst.p, (const int32_t*)h->dag_in_beg.p,
-               (const int32_t*)h->dag_in_src.p, (const int64_t*)h->rank.p, n, (const int32_t*)a, (const int32_t*)b,
-               ans);
+        if (h->prm.reach_local != 0)
+            launch(h, "reach_local", kReachLocal, std::min<unsigned>(gridFor(n * WAVE, 256), 2048), 256, 0,
+                   (const int32_t*)h->dag_beg.p, (const int32_t*)h->dag_dst.p, (const int32_t*)h->dag_in_beg.p,
+                   (const int32_t*)h->dag_in_src.p, (const int64_t*)h->rank.p, n, (const int32_t*)a,
+                   (const int32_t*)b, ans);
         std::vector<uint8_t> res((size_t)n);
         std::vector<int32_t> ha((size_t)n), hb((size_t)n);
         HIPCHK(hipMemcpyAsync(res.data(), ans, (size_t)n, hipMemcpyDeviceToHost, h->st));

@@ -186,6 +186,9 @@ int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
  *   while frontiers are tiny, 1..16), "persist" (0; 1: tiny-frontier BFS
  *   levels, reach levels and SSSP rounds run inside one persistent
  *   cooperative kernel with grid barriers instead of host-driven launches),
+ *   "sssp_dist32" (1: SSSP on 32-bit distances, redone in 64 bits on
+ *   overflow), "reach_local" (1: girhip_reachable first tries one
+ *   bidirectional LDS search per pair; 0: bit-parallel sweeps only),
  *   "profile" (0: 1 times every kernel)
  * Values only change work order; results never depend on them. */
 int girhip_set_param(girhip_graph* h, const char* name, double value);

@@ -187,7 +187,7 @@ int main(int argc, char** argv) {
         auto t0 = Clock::now();
         for (size_t i = 0; i < r.size(); ++i) r[i] = g.reachable(pr[2 * i], pr[2 * i + 1]) ? 1 : 0;
         std::printf("@@{\"phase\": \"reach\", \"pairs\": %zu, \"ms\": %.3f}\n", r.size(), msSince(t0));
-        if (write) writeVec(out + "reach.bin", r.data(), r.size());
+        writeVec(out + "reach.bin", r.data(), r.size());  // small: written even with --no-output
     }
     if (!batch_f.empty()) {
         // The reference's drivers build the block layer before any update

@@ -377,3 +377,35 @@ def test_reachable_queries():
         i = int(np.nonzero(~got[3:])[0][0]) + 3
         G.ingest_batch([u[i]], [v[i]], [1])
         assert G.reachable(u[i:i + 1], v[i:i + 1])[0]
+
+
+@pytest.mark.gpu
+def test_reachable_at_scale_both_paths():
+    # GIR::reachable (gir_query.cpp:789-828) on a power-law graph whose giant
+    # SCC has DAG rows far longer than a local search expands: answers of the
+    # per-pair LDS search (with its fallback to sweeps) and of the sweeps
+    # alone both equal reachability from the oracle's BFS
+    V, fb, fa, bb, ba = girhip.rmat_graph(17, edge_factor=16, seed=43)
+    rng = np.random.default_rng(44)
+    srcs = rng.integers(0, V, 24).astype(np.int32)
+    lv = {int(s): oracle.bfs_textbook(fb, fa, int(s)) for s in srcs}
+    u, v = [], []
+    for s in srcs:
+        reached = np.nonzero(lv[int(s)] >= 0)[0]
+        u.append(np.full(256, s, np.int32))
+        v.append(np.concatenate([rng.integers(0, V, 128), rng.choice(reached, 128)]).astype(np.int32))
+    u = np.concatenate(u)
+    v = np.concatenate(v)
+    perm = rng.permutation(len(u))
+    u, v = u[perm], v[perm]
+    want = np.array([lv[int(x)][y] >= 0 for x, y in zip(u, v)])
+    say("reachable rmat17 ef16")
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.scc_dag()
+        ds, dd = G.dag()[:2]
+        deg = max(np.bincount(ds).max(), np.bincount(dd).max())
+        assert deg > 2048  # rows the local search sets aside
+        got = G.reachable(u, v)
+        assert np.array_equal(got, want)
+        G.set_param("reach_local", 0)
+        assert np.array_equal(G.reachable(u, v), want)
