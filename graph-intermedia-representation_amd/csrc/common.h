@@ -126,6 +126,11 @@ struct Scratch {
     DevBuf<uint64_t> rq_seen, rq_next, rq_tgt, rq_fb[2];
     DevBuf<int32_t> rq_fq[2], rq_tq, rq_pairs;
     DevBuf<int64_t> rq_ctl, rq_lim;
+    // condensation-guided queries: reachable-SCC bitmap, reached list,
+    // pending predecessor counts, wave lists, vertex list, control slots
+    DevBuf<uint64_t> tl_mask;
+    DevBuf<int32_t> tl_list, tl_left, tl_rdy, tl_vl;
+    DevBuf<int64_t> tl_ctl;
     // heavy-row list
     DevBuf<int64_t> hv_ctl, hv_b, hv_e, hv_off;
     DevBuf<int32_t> hv_row, hv_aux;
@@ -170,6 +175,8 @@ struct girhip_graph {
     gir::DevBuf<int32_t> dag_src, dag_dst, dag_mult;
     gir::DevBuf<int32_t> dag_beg;  // DAG CSR offsets by source SCC id (V+1)
     gir::DevBuf<int32_t> dag_in_beg, dag_in_src;  // DAG in-arcs by destination SCC id (V+1, n_dag)
+    gir::DevBuf<int32_t> mem_beg, mem_v;  // members by SCC id (V+1, V), built on demand
+    bool mem_ok = false;
     int64_t n_dag = 0;
     gir::DevBuf<int64_t> rank;
 

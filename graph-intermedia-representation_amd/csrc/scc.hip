@@ -535,6 +535,7 @@ void runDag(girhip_graph* h) {
         fillI32(h, h->dag_in_beg.p, 0, (int64_t)V + 1);
     }
     h->n_dag = nd;
+    h->mem_ok = false;
     // DAG CSR offsets by source SCC (arcs are sorted by src)
     launch(h, "dag_beg", kDagBeg, gridFor((int64_t)V + 1, blk), blk, 0, (const int32_t*)h->dag_src.p, nd, V,
            h->dag_beg.p);

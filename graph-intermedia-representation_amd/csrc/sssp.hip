@@ -117,6 +117,9 @@ __global__ void kBInit(SsspBatch B, const int32_t* __restrict__ src, uint64_t de
 // Relaxation of arc slot j of row p (owner distance dp) for source s.
 // heavy = 0: only light arcs (w <= delta, the bucket's inner rounds);
 // heavy = 1: only heavy arcs (w > delta, once per vertex and bucket).
+// scope (optional): arcs whose ends carry different labels are skipped (the
+// SCC-restricted delta-stepping of the two-level queries); unit: every arc
+// weighs 1 (BFS levels as distances).
 struct RelaxF {
     SsspBatch B;
     const int32_t* fa;
@@ -124,10 +127,16 @@ struct RelaxF {
     uint64_t delta;
     int heavy;
     int par;
+    const int32_t* scope;
+    int unit;
     __device__ int64_t operator()(int32_t p, uint64_t dp, int32_t s, int64_t j) const {
         uint64_t wt;
         int32_t x;
-        if (w) {
+        if (unit) {
+            if (heavy) return 0;
+            wt = 1;
+            x = fa[j];
+        } else if (w) {
             wt = w[j];
             if ((wt > delta) != (heavy != 0)) return 0;
             x = fa[j];
@@ -136,6 +145,7 @@ struct RelaxF {
             wt = hashWeight(p, x);
             if ((wt > delta) != (heavy != 0)) return 0;
         }
+        if (scope && scope[x] != scope[p]) return 0;
         const uint64_t nd = dp + wt;
         if (B.d32 && nd >= 0xffffffffull) {  // beyond 32 bits: the batch is redone with 64
             B.ctl[C_OVF] = 1;
@@ -414,18 +424,14 @@ __global__ void kReachedArcsD(const uint64_t* __restrict__ dist, const int32_t* 
 // round with nothing left to do is a no-op.
 constexpr int SSSP_CHUNK = 16;
 
-// k <= SB sources (device array dsrc) into rows of dist.
-// Batch state lives in the handle's scratch (s_* buffers, sized for SB
-// sources by the caller).
-// dist: k rows of V uint64 (d32 = false) or uint32 (d32 = true) distances.
-static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, void* dist, bool d32) {
+// Batch state over the handle's scratch (s_* buffers, sized for SB sources
+// by the caller); dist: k rows of V uint64 (d32 = false) or uint32 (d32 =
+// true) distances.
+static SsspBatch batchState(girhip_graph* h, int k, void* dist, bool d32) {
     Scratch& S = scratch(h);
-    const int32_t V = h->V;
-    const int blk = (int)h->prm.block;
-    const uint32_t* w = h->has_w ? h->w_fw.p : nullptr;
     SsspBatch B;
-    B.V = V;
-    B.W = ((int64_t)V + 63) / 64;
+    B.V = h->V;
+    B.W = ((int64_t)h->V + 63) / 64;
     B.W2 = (B.W + 63) / 64;
     for (int p = 0; p < 2; ++p) B.qn[p] = S.s_q[p].p;
     B.qh = S.s_qh.p;
@@ -438,20 +444,32 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
     B.ctl = S.s_ctl.p;
     B.dist = dist;
     B.d32 = d32 ? 1 : 0;
-    if (d32) fillI32(h, (int32_t*)dist, -1, (int64_t)k * V);
-    else fillU64(h, (uint64_t*)dist, INF, (int64_t)k * V);
-    fillU64(h, B.nearbits, 0, (int64_t)k * B.W);
-    fillU64(h, B.farbits, 0, (int64_t)k * B.W);
-    fillU64(h, B.donebits, 0, (int64_t)k * B.W);
-    fillU64(h, B.nsum, 0, (int64_t)k * B.W2);
-    fillU64(h, B.fsum, 0, (int64_t)k * B.W2);
-    fillU64(h, B.dsum, 0, (int64_t)k * B.W2);
     B.k = k;
-    launch(h, "sssp_init", kBInit, 1, 64, 0, B, dsrc, delta);
+    return B;
+}
+
+static void clearBitmaps(girhip_graph* h, const SsspBatch& B) {
+    fillU64(h, B.nearbits, 0, (int64_t)B.k * B.W);
+    fillU64(h, B.farbits, 0, (int64_t)B.k * B.W);
+    fillU64(h, B.donebits, 0, (int64_t)B.k * B.W);
+    fillU64(h, B.nsum, 0, (int64_t)B.k * B.W2);
+    fillU64(h, B.fsum, 0, (int64_t)B.k * B.W2);
+    fillU64(h, B.dsum, 0, (int64_t)B.k * B.W2);
+}
+
+// Bucket rounds from the state in B until every source's queue is empty
+// after an advance (all settled).  scope / unit: see RelaxF.
+static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, const int32_t* scope, bool unit) {
+    Scratch& S = scratch(h);
+    const int32_t V = h->V;
+    const int k = B.k;
+    const int blk = (int)h->prm.block;
+    const uint32_t* w = h->has_w ? h->w_fw.p : nullptr;
     // heavy arcs exist only if some weight exceeds delta
-    const int heavy = (uint64_t)maxWeight(h) > delta ? 1 : 0;
-    const RelaxF fl[2] = {{B, h->fw_adj.p, w, delta, 0, 0}, {B, h->fw_adj.p, w, delta, 0, 1}};
-    const RelaxF fh[2] = {{B, h->fw_adj.p, w, delta, 1, 0}, {B, h->fw_adj.p, w, delta, 1, 1}};
+    const int heavy = !unit && (uint64_t)maxWeight(h) > delta ? 1 : 0;
+    const int u = unit ? 1 : 0;
+    const RelaxF fl[2] = {{B, h->fw_adj.p, w, delta, 0, 0, scope, u}, {B, h->fw_adj.p, w, delta, 0, 1, scope, u}};
+    const RelaxF fh[2] = {{B, h->fw_adj.p, w, delta, 1, 0, scope, u}, {B, h->fw_adj.p, w, delta, 1, 1, scope, u}};
     const unsigned hgrid = std::min<unsigned>(gridFor(V, blk), 1024);
     const int32_t* fb = h->fw_beg.p;
     int par = 0;
@@ -507,6 +525,371 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
         R = (mx * 64 < (int64_t)V && mx <= 2 * maxn && maxn > 1) ? std::min(cmax, 2 * R) : 1;
         maxn = mx;
     }
+}
+
+// k <= SB sources (device array dsrc) into rows of dist (see batchState).
+static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, void* dist, bool d32) {
+    const SsspBatch B = batchState(h, k, dist, d32);
+    if (d32) fillI32(h, (int32_t*)dist, -1, (int64_t)k * h->V);
+    else fillU64(h, (uint64_t*)dist, INF, (int64_t)k * h->V);
+    clearBitmaps(h, B);
+    launch(h, "sssp_init", kBInit, 1, 64, 0, B, dsrc, delta);
+    ssspRounds(h, B, delta, nullptr, false);
+}
+
+// ---- condensation-guided queries --------------------------------------------
+// GIR::reachableSccs / reachInfo (gir_query.cpp:56-88) and the two-level
+// model of GIR::dagWavefront / bfsTwoLevel / ssspTwoLevel
+// (gir_query.cpp:304-350, 604-620, 705-722).
+//
+// The reachable part of the condensation is found by a level sweep over the
+// DAG CSR from the source's SCC: a claim bitmap over SCC ids and an
+// append-only list whose level segments are the frontiers.  Each reachable
+// SCC's pending count is its number of reachable DAG predecessors.  The
+// wavefront then releases SCCs in Kahn order over that part, all ready SCCs
+// of a wave at once:
+//   1. their members go to a vertex list; members of multi-vertex SCCs that
+//      already hold a distance become the seeds (far bits) of one
+//      delta-stepping run restricted to arcs inside an SCC, which finishes
+//      every ready SCC of the wave together (a single-vertex SCC is final
+//      once its predecessors are);
+//   2. the members' arcs into other SCCs are relaxed once (atomicMin);
+//   3. every DAG successor of a ready SCC loses one pending predecessor;
+//      those reaching zero form the next wave.
+// BFS levels are the same computation with unit weights and delta 1.
+enum : int { T_LO = 0, T_HI, T_TAIL, T_NV, T_NBIG, T_VERTS, T_EDGES, T_SLOTS = 8 };
+
+__global__ void kMemKeys(const int32_t* __restrict__ scc, int32_t V, int B, uint64_t* __restrict__ keys) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) keys[i] = ((uint64_t)(uint32_t)scc[i] << B) | (uint64_t)i;
+}
+
+// sorted (scc, v) keys -> member lists: mem[beg[c] .. beg[c + 1]) = SCC c's vertices
+__global__ void kMemCsr(const uint64_t* __restrict__ keys, int32_t V, int B, int32_t* __restrict__ beg,
+                        int32_t* __restrict__ mem) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) mem[i] = (int32_t)(keys[i] & ((1ULL << B) - 1));
+    if (i <= V) {
+        const uint64_t key = (uint64_t)i << B;
+        int64_t lo = 0, hi = V;
+        while (lo < hi) {
+            const int64_t m = lo + ((hi - lo) >> 1);
+            if (keys[m] < key) lo = m + 1;
+            else hi = m;
+        }
+        beg[i] = (int32_t)lo;
+    }
+}
+
+static void memberCsr(girhip_graph* h) {
+    if (h->mem_ok) return;
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    const int B = bitsFor((uint64_t)(V > 1 ? V - 1 : 1));
+    Scratch& S = scratch(h);
+    S.keys[0].ensure((size_t)std::max<int32_t>(V, 1));
+    h->mem_beg.ensure((size_t)V + 1);
+    h->mem_v.ensure((size_t)std::max<int32_t>(V, 1));
+    launch(h, "tl_mem_keys", kMemKeys, gridFor(V, blk), blk, 0, (const int32_t*)h->scc.p, V, B, S.keys[0].p);
+    radixSort(h, S.keys[0].p, nullptr, V, 2 * B);
+    launch(h, "tl_mem_csr", kMemCsr, gridFor((int64_t)V + 1, blk), blk, 0, (const uint64_t*)S.keys[0].p, V, B,
+           h->mem_beg.p, h->mem_v.p);
+    h->mem_ok = true;
+}
+
+// DAG arc j: claim its head in the bitmap, list it once.
+struct DagClaimF {
+    const int32_t* dst;
+    uint64_t* mask;
+    int32_t* list;
+    int64_t* tail;
+    __device__ int64_t operator()(int32_t, uint64_t, int32_t, int64_t j) const {
+        const int32_t z = dst[j];
+        const uint64_t bit = 1ULL << (z & 63);
+        bool nw = false;
+        if (!(mask[z >> 6] & bit))  // plain read: bits only get set
+            nw = !(atomicOr((unsigned long long*)&mask[z >> 6], (unsigned long long)bit) & bit);
+        const int64_t pos = wavePush(tail, nw);
+        if (nw) list[pos] = z;
+        return 0;
+    }
+};
+
+// DAG arc j of a reachable SCC: one more reachable predecessor of its head.
+struct PendF {
+    const int32_t* dst;
+    int32_t* left;
+    __device__ int64_t operator()(int32_t, uint64_t, int32_t, int64_t j) const {
+        atomicAdd(&left[dst[j]], 1);
+        return 0;
+    }
+};
+
+// Rows (CSR beg) of the ids list[ctl[T_LO] .. ctl[T_HI]) (bounds read on the
+// device), each with its length as payload; grid stride, one lane per id.
+template <class F>
+__global__ void kListRows(const int32_t* __restrict__ beg, const int32_t* list, const int64_t* ctl, F f, HeavyList H) {
+    const int64_t lo = ctl[T_LO], hi = ctl[T_HI];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = lo + (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < hi;
+         base += stride) {
+        const int64_t i = base + laneId();
+        const bool act = i < hi;
+        const int32_t x = act ? list[i] : 0;
+        const int64_t b = act ? beg[x] : 0, e = act ? beg[x + 1] : 0;
+        waveRowsH(act, x, (uint64_t)(e - b), 0, b, e, H, f);
+    }
+}
+
+// Next level / wave: [lo, hi) := [hi, tail); the vertex list restarts.
+__global__ void kListAdvance(int64_t* ctl) {
+    if (threadIdx.x == 0) {
+        ctl[T_LO] = ctl[T_HI];
+        ctl[T_HI] = ctl[T_TAIL];
+        ctl[T_NV] = 0;
+    }
+}
+
+// Sweep seed: the SCC of vertex src (canonical id scc[src]).
+__global__ void kSweepSeed(const int32_t* __restrict__ scc, int32_t src, uint64_t* mask, int32_t* list, int64_t* ctl) {
+    if (threadIdx.x != 0) return;
+    const int32_t c = scc[src];
+    mask[c >> 6] |= 1ULL << (c & 63);
+    list[0] = c;
+    for (int j = 0; j < T_SLOTS; ++j) ctl[j] = 0;
+    ctl[T_HI] = 1;
+    ctl[T_TAIL] = 1;
+}
+
+// Whole reached list as the current range (for the pending-count pass).
+__global__ void kListAll(int64_t* ctl) {
+    if (threadIdx.x == 0) {
+        ctl[T_LO] = 0;
+        ctl[T_HI] = ctl[T_TAIL];
+    }
+}
+
+// Vertices in reachable SCCs and their out-arcs.
+__global__ void kReachVerts(const int32_t* __restrict__ scc, const int32_t* __restrict__ fb,
+                            const uint64_t* __restrict__ mask, int32_t V, int64_t* ctl) {
+    int64_t nv = 0, ne = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = scc[v];
+        if ((mask[c >> 6] >> (c & 63)) & 1) {
+            ++nv;
+            ne += fb[v + 1] - fb[v];
+        }
+    }
+    blockAtomicAdd(nv, ctl + T_VERTS);
+    blockAtomicAdd(ne, ctl + T_EDGES);
+}
+
+__global__ void kMaskBytes(const uint64_t* __restrict__ mask, int32_t V, uint8_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) out[i] = (uint8_t)((mask[i >> 6] >> (i & 63)) & 1);
+}
+
+// Reachable part of the condensation from the SCC of vertex src: bitmap
+// tl_mask, list tl_list[0 .. n_sccs), pending counts tl_left (zero outside).
+// Returns n_sccs.
+static int64_t reachSweep(girhip_graph* h, int32_t src) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    const int64_t W = ((int64_t)V + 63) / 64;
+    S.tl_mask.ensure((size_t)std::max<int64_t>(W, 1));
+    S.tl_list.ensure((size_t)std::max<int32_t>(V, 1));
+    S.tl_left.ensure((size_t)std::max<int32_t>(V, 1));
+    S.tl_ctl.ensure(2 * T_SLOTS);
+    int64_t* ctl = S.tl_ctl.p;
+    fillU64(h, S.tl_mask.p, 0, W);
+    fillI32(h, S.tl_left.p, 0, V);
+    launch(h, "tl_sweep_seed", kSweepSeed, 1, 64, 0, (const int32_t*)h->scc.p, src, S.tl_mask.p, S.tl_list.p, ctl);
+    const DagClaimF f{h->dag_dst.p, S.tl_mask.p, S.tl_list.p, ctl + T_TAIL};
+    const unsigned g = 512;
+    int64_t r[T_SLOTS];
+    while (true) {
+        for (int t = 0; t < 16; ++t) {  // levels per readback
+            launch(h, "tl_sweep", kListRows<DagClaimF>, g, blk, 0, (const int32_t*)h->dag_beg.p,
+                   (const int32_t*)S.tl_list.p, (const int64_t*)ctl, f, heavyList(h));
+            launchHeavy(h, "tl_sweep_heavy", f);
+            launch(h, "tl_advance", kListAdvance, 1, 64, 0, ctl);
+        }
+        readCounters(h, ctl, T_SLOTS, r);
+        if (r[T_LO] == r[T_HI]) break;
+    }
+    launch(h, "tl_all", kListAll, 1, 64, 0, ctl);
+    const PendF pf{h->dag_dst.p, S.tl_left.p};
+    launch(h, "tl_pending", kListRows<PendF>, boundedGrid(r[T_TAIL], blk), blk, 0,
+           (const int32_t*)h->dag_beg.p, (const int32_t*)S.tl_list.p, (const int64_t*)ctl, pf, heavyList(h));
+    launchHeavy(h, "tl_pending_heavy", pf);
+    return r[T_TAIL];
+}
+
+// Wave step 1: member v of a ready SCC (payload: the SCC's size) joins the
+// vertex list; in a multi-vertex SCC a member with a distance is a seed.
+struct ExpandF {
+    const int32_t* mem;
+    int32_t* vl;
+    int64_t* nv;
+    SsspBatch B;
+    __device__ int64_t operator()(int32_t, uint64_t size, int32_t, int64_t j) const {
+        const int32_t v = mem[j];
+        const int64_t pos = wavePush(nv, true);
+        vl[pos] = v;
+        if (size > 1 && dGet(B, 0, v) != INF) setBit(B.farbits, B.fsum, v);
+        return 0;
+    }
+};
+
+// Wave step 2: arc j of vertex p (distance dp, SCC label sp) into another SCC.
+struct CrossF {
+    SsspBatch B;
+    const int32_t* fa;
+    const uint32_t* w;
+    const int32_t* scc;
+    int unit;
+    __device__ int64_t operator()(int32_t p, uint64_t dp, int32_t sp, int64_t j) const {
+        const int32_t x = fa[j];
+        if (scc[x] == sp) return 0;
+        const uint64_t wt = unit ? 1 : (w ? (uint64_t)w[j] : (uint64_t)hashWeight(p, x));
+        const uint64_t nd = dp + wt;
+        if (B.d32 && nd >= 0xffffffffull) {
+            B.ctl[C_OVF] = 1;
+            return 0;
+        }
+        if (nd < dGet(B, 0, x)) dMin(B, 0, x, nd);
+        return 0;
+    }
+};
+
+__global__ void kCross(CrossF f, const int32_t* __restrict__ fb, const int32_t* __restrict__ vl,
+                       const int64_t* __restrict__ ctl, HeavyList H) {
+    const int64_t n = ctl[T_NV];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+        const int64_t i = base + laneId();
+        bool act = i < n;
+        const int32_t v = act ? vl[i] : 0;
+        const uint64_t dv = act ? dGet(f.B, 0, v) : INF;
+        act = act && dv != INF;
+        const int64_t b = act ? fb[v] : 0, e = act ? fb[v + 1] : 0;
+        waveRowsH(act, v, dv, act ? f.scc[v] : 0, b, e, H, f);
+    }
+}
+
+// Wave step 3: DAG arc j of a ready SCC; the head is ready once all its
+// reachable predecessors are.
+struct ReleaseF {
+    const int32_t* dst;
+    const int32_t* mbeg;
+    int32_t* left;
+    int32_t* rdy;
+    int64_t* ctl;
+    __device__ int64_t operator()(int32_t, uint64_t, int32_t, int64_t j) const {
+        const int32_t d = dst[j];
+        const bool go = atomicSub(&left[d], 1) == 1;
+        const int64_t pos = wavePush(ctl + T_TAIL, go);
+        if (go) {
+            rdy[pos] = d;
+            if (mbeg[d + 1] - mbeg[d] > 1) atomicAdd((unsigned long long*)(ctl + T_NBIG), 1ULL);
+        }
+        return 0;
+    }
+};
+
+// Wave state: distances (source vertex at 0), the source's SCC as the first
+// wave; T_NBIG: multi-vertex SCCs in the current wave.
+__global__ void kWaveInit(SsspBatch B, const int32_t* __restrict__ scc, const int32_t* __restrict__ mbeg, int32_t src,
+                          int32_t* rdy, int64_t* ctl) {
+    if (threadIdx.x != 0) return;
+    if (B.d32) ((uint32_t*)B.dist)[src] = 0;
+    else ((uint64_t*)B.dist)[src] = 0;
+    const int32_t c = scc[src];
+    rdy[0] = c;
+    for (int j = 0; j < T_SLOTS; ++j) ctl[j] = 0;
+    ctl[T_HI] = 1;
+    ctl[T_TAIL] = 1;
+    ctl[T_NBIG] = mbeg[c + 1] - mbeg[c] > 1 ? 1 : 0;
+    B.ctl[C_OVF] = 0;
+}
+
+// Delta-stepping state for a run seeded through the far bitmap: every slot
+// zero, so the first far minimum (over distances >= hi = 0) covers every seed.
+__global__ void kBInitSeeded(SsspBatch B) {
+    const int s = threadIdx.x;
+    if (s >= SB) return;
+    for (int j = 0; j < C_SLOTS / SB; ++j) B.ctl[j * SB + s] = 0;
+}
+
+__global__ void kZeroSlot(int64_t* p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+// One source's two-level query into dist (uint32 when d32); the pending
+// counts of the sweep from src's SCC must be in tl_left.  unit: BFS levels.
+static void waveQuery(girhip_graph* h, int32_t src, uint64_t delta, bool unit, void* dist, bool d32) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    S.tl_rdy.ensure((size_t)std::max<int32_t>(V, 1));
+    S.tl_vl.ensure((size_t)std::max<int32_t>(V, 1));
+    int64_t* ctl = S.tl_ctl.p + T_SLOTS;
+    const SsspBatch B = batchState(h, 1, dist, d32);
+    if (d32) fillI32(h, (int32_t*)dist, -1, V);
+    else fillU64(h, (uint64_t*)dist, INF, V);
+    launch(h, "tl_wave_init", kWaveInit, 1, 64, 0, B, (const int32_t*)h->scc.p, (const int32_t*)h->mem_beg.p, src,
+           S.tl_rdy.p, ctl);
+    const uint32_t* w = h->has_w ? h->w_fw.p : nullptr;
+    const ExpandF ef{h->mem_v.p, S.tl_vl.p, ctl + T_NV, B};
+    const CrossF cf{B, h->fw_adj.p, w, h->scc.p, unit ? 1 : 0};
+    const ReleaseF rf{h->dag_dst.p, h->mem_beg.p, S.tl_left.p, S.tl_rdy.p, ctl};
+    int64_t r[T_SLOTS];
+    int64_t nready = 1, nbig = 0;
+    readCounters(h, ctl, T_SLOTS, r);
+    nbig = r[T_NBIG];
+    while (nready > 0) {
+        if (nbig > 0) clearBitmaps(h, B);
+        launch(h, "tl_expand", kListRows<ExpandF>, std::min<unsigned>(1024, gridFor(nready, blk)), blk,
+               0, (const int32_t*)h->mem_beg.p, (const int32_t*)S.tl_rdy.p, (const int64_t*)ctl, ef, heavyList(h));
+        launchHeavy(h, "tl_expand_heavy", ef);
+        if (nbig > 0) {
+            launch(h, "tl_seeded", kBInitSeeded, 1, 64, 0, B);
+            ssspRounds(h, B, delta, h->scc.p, unit);
+        }
+        launch(h, "tl_cross", kCross, 1024, blk, 0, cf, (const int32_t*)h->fw_beg.p, (const int32_t*)S.tl_vl.p,
+               (const int64_t*)ctl, heavyList(h));
+        launchHeavyStore(h, "tl_cross_heavy", cf);
+        launch(h, "tl_zero", kZeroSlot, 1, 64, 0, ctl + T_NBIG);
+        launch(h, "tl_release", kListRows<ReleaseF>, std::min<unsigned>(1024, gridFor(nready, blk)),
+               blk, 0, (const int32_t*)h->dag_beg.p, (const int32_t*)S.tl_rdy.p, (const int64_t*)ctl, rf,
+               heavyList(h));
+        launchHeavy(h, "tl_release_heavy", rf);
+        launch(h, "tl_advance", kListAdvance, 1, 64, 0, ctl);
+        readCounters(h, ctl, T_SLOTS, r);
+        nready = r[T_HI] - r[T_LO];
+        nbig = r[T_NBIG];
+    }
+}
+
+__global__ void kLevels32(const uint32_t* __restrict__ d, int64_t n, int32_t* __restrict__ lv) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        lv[i] = d[i] == 0xffffffffu ? -1 : (int32_t)d[i];
+}
+
+// Scratch of one SSSP batch of one source.
+static void ensureBatchScratch(girhip_graph* h, int kc) {
+    Scratch& S = scratch(h);
+    const size_t Vs = (size_t)(h->V > 0 ? h->V : 1), Ws = (Vs + 63) / 64, W2s = (Ws + 63) / 64;
+    for (int p = 0; p < 2; ++p) S.s_q[p].ensure((size_t)kc * Vs);
+    S.s_qh.ensure((size_t)kc * Vs);
+    S.s_near.ensure((size_t)kc * Ws);
+    S.s_far.ensure((size_t)kc * Ws);
+    S.s_done.ensure((size_t)kc * Ws);
+    S.s_nsum.ensure((size_t)kc * W2s);
+    S.s_fsum.ensure((size_t)kc * W2s);
+    S.s_dsum.ensure((size_t)kc * W2s);
+    S.s_ctl.ensure(C_SLOTS + 2);
 }
 
 }  // namespace gir
@@ -580,5 +963,126 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         }
         if (dist && !direct) copyOut(h, dist, out, sizeof(uint64_t) * n);
         syncStream(h);
+    });
+}
+
+// Shared checks of the condensation-guided entry points: SCC layer built,
+// pending ingested changes applied first (GIR::bfsTwoLevel / ssspTwoLevel
+// call sync(), gir_query.cpp:605, 706).
+static void twoLevelReady(girhip_graph* h, const char* what) {
+    if (!h->scc_req) raise(GIRHIP_ESTATE, std::string(what) + ": call girhip_scc_dag first");
+    if (h->pending && girhip_sync(h) != GIRHIP_OK) raise(GIRHIP_EHIP, girhip_last_error());
+}
+
+extern "C" int girhip_reach_info(girhip_graph* h, int32_t scc_id, uint8_t* mask, int32_t* pending, int64_t* n_sccs,
+                                 int64_t* n_verts, int64_t* n_edges) {
+    return guarded([&] {
+        useDevice(h);
+        twoLevelReady(h, "reach_info");
+        const int32_t V = h->V;
+        if (scc_id < 0 || scc_id >= V) raise(GIRHIP_EINVAL, "reach_info: SCC id out of range");
+        int32_t c = -1;
+        HIPCHK(hipMemcpyAsync(&c, h->scc.p + scc_id, sizeof(int32_t), hipMemcpyDeviceToHost, h->st));
+        syncStream(h);
+        if (c != scc_id) raise(GIRHIP_EINVAL, "reach_info: not an SCC id (ids are the minimum vertex of each SCC)");
+        tBegin(h);
+        const int64_t ns = reachSweep(h, scc_id);
+        Scratch& S = scratch(h);
+        int64_t* ctl = S.tl_ctl.p;
+        launch(h, "tl_verts", kReachVerts, boundedGrid(V, 256), 256, 0, (const int32_t*)h->scc.p,
+               (const int32_t*)h->fw_beg.p, (const uint64_t*)S.tl_mask.p, V, ctl);
+        int64_t r[T_SLOTS];
+        readCounters(h, ctl, T_SLOTS, r);
+        tEnd(h, "reach_info");
+        if (mask) {
+            uint8_t* out = mask;
+            const bool direct = isDevicePtr(mask);
+            if (!direct) {
+                S.stage.ensure((size_t)std::max<int32_t>(V, 1));
+                out = S.stage.p;
+            }
+            launch(h, "tl_mask_bytes", kMaskBytes, gridFor(V, 256), 256, 0, (const uint64_t*)S.tl_mask.p, V, out);
+            if (!direct) copyOut(h, mask, out, (size_t)V);
+        }
+        if (pending) copyOut(h, pending, S.tl_left.p, sizeof(int32_t) * (size_t)V);
+        syncStream(h);
+        if (n_sccs) *n_sccs = ns;
+        if (n_verts) *n_verts = r[T_VERTS];
+        if (n_edges) *n_edges = r[T_EDGES];
+    });
+}
+
+// k sources, one after another: sweep from the source's SCC, then the
+// wavefront; 32-bit distances first (64-bit redo on overflow).  unit: BFS
+// levels into lv (int32, -1 = unreached); else distances into dist.
+static void twoLevelMany(girhip_graph* h, const int32_t* src, int32_t k, uint64_t delta, bool unit, int32_t* lv,
+                         uint64_t* dist, const char* what) {
+    const int32_t V = h->V;
+    std::vector<int32_t> s = hostSources(h, src, k);
+    Scratch& S = scratch(h);
+    const size_t n = (size_t)k * (size_t)V;
+    void* outp = unit ? (void*)lv : (void*)dist;
+    const size_t esz = unit ? sizeof(int32_t) : sizeof(uint64_t);
+    const bool direct = outp && isDevicePtr(outp);
+    uint8_t* out = (uint8_t*)outp;
+    if (!direct) {
+        S.stage.ensure(esz * (n > 0 ? n : 1));
+        out = S.stage.p;
+    }
+    ensureBatchScratch(h, 1);
+    const bool try32 = unit || (h->prm.sssp_dist32 != 0 && (uint64_t)maxWeight(h) < 0x7fffffffull);
+    if (try32) S.s_d32.ensure((size_t)std::max<int32_t>(V, 1));
+    tBegin(h);
+    if (V > 0) memberCsr(h);
+    for (int32_t i = 0; i < k; ++i) {
+        uint8_t* row = out + esz * (size_t)i * V;
+        const int32_t x = s[i];
+        if (x < 0 || x >= V) {  // invalid source: nothing reached
+            if (unit) fillI32(h, (int32_t*)row, -1, V);
+            else fillU64(h, (uint64_t*)row, INF, V);
+            continue;
+        }
+        reachSweep(h, x);
+        if (try32) {
+            waveQuery(h, x, delta, unit, S.s_d32.p, true);
+            int64_t ovf;
+            readCounters(h, S.s_ctl.p + C_OVF, 1, &ovf);
+            if (ovf == 0) {
+                if (unit)
+                    launch(h, "tl_levels", kLevels32, boundedGrid(V, 256), 256, 0, (const uint32_t*)S.s_d32.p,
+                           (int64_t)V, (int32_t*)row);
+                else
+                    launch(h, "sssp_widen", kWiden, boundedGrid(V, 256), 256, 0, (const uint32_t*)S.s_d32.p,
+                           (int64_t)V, (uint64_t*)row);
+                continue;
+            }
+            // the pending counts were used up by the wavefront
+            reachSweep(h, x);
+        }
+        waveQuery(h, x, delta, false, row, false);
+    }
+    tEnd(h, what);
+    if (outp && !direct) copyOut(h, outp, out, esz * n);
+    syncStream(h);
+}
+
+extern "C" int girhip_bfs_two_level(girhip_graph* h, const int32_t* src, int32_t k, int32_t* levels) {
+    return guarded([&] {
+        useDevice(h);
+        if (k < 0 || (k > 0 && !src)) raise(GIRHIP_EINVAL, "bfs_two_level: bad source list");
+        twoLevelReady(h, "bfs_two_level");
+        twoLevelMany(h, src, k, 1, true, levels, nullptr, "bfs_two_level");
+    });
+}
+
+extern "C" int girhip_sssp_two_level(girhip_graph* h, const int32_t* src, int32_t k, uint64_t delta,
+                                     uint64_t* dist) {
+    return guarded([&] {
+        useDevice(h);
+        if (k < 0 || (k > 0 && !src)) raise(GIRHIP_EINVAL, "sssp_two_level: bad source list");
+        if (delta == 0) delta = (uint64_t)h->prm.sssp_delta;
+        if (delta == 0) raise(GIRHIP_EINVAL, "sssp_two_level: delta must be > 0");
+        twoLevelReady(h, "sssp_two_level");
+        twoLevelMany(h, src, k, delta, false, nullptr, dist, "sssp_two_level");
     });
 }

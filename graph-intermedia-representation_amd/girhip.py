@@ -62,6 +62,9 @@ _SIGS = {
     "girhip_bfs": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, _vp]),
     "girhip_bfs_component": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
     "girhip_reachable": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),
+    "girhip_reach_info": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp]),
+    "girhip_bfs_two_level": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp]),
+    "girhip_sssp_two_level": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_uint64, _vp]),
     "girhip_sssp": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_uint64, _vp, _vp]),
     "girhip_apply_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]),
     "girhip_ingest_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]),
@@ -423,6 +426,40 @@ class Graph:
         _check(lib().girhip_reachable(self._h, u.ctypes.data if len(u) else None, v.ctypes.data if len(v) else None,
                                       len(u), out.ctypes.data))
         return out.astype(bool)
+
+    def reach_info(self, scc_id):
+        """GIR::reachableSccs / reachInfo (gir_query.cpp:56-88) from the SCC
+        with canonical id scc_id -> dict: mask (bool per SCC id), pending
+        (reachable DAG predecessors per reachable SCC id), sccs, verts,
+        edges."""
+        V = self.V
+        mask = np.zeros(max(V, 1), np.uint8)
+        pend = np.zeros(max(V, 1), np.int32)
+        c = np.zeros(3, np.int64)
+        _check(lib().girhip_reach_info(self._h, int(scc_id), mask.ctypes.data, pend.ctypes.data, c[0:].ctypes.data,
+                                       c[1:].ctypes.data, c[2:].ctypes.data))
+        return {"mask": mask[:V].astype(bool), "pending": pend[:V], "sccs": int(c[0]), "verts": int(c[1]),
+                "edges": int(c[2])}
+
+    def bfs_two_level(self, sources, out=None):
+        """GIR::bfsTwoLevel (gir_query.cpp:604-620) for every source -> int32
+        (k, V) levels, -1 unreachable (equal to bfs)."""
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
+        k, V = len(src), self.V
+        lv = np.empty((k, V), np.int32) if out is None else out
+        _check(lib().girhip_bfs_two_level(self._h, src.ctypes.data if k else None, k,
+                                          _ptr(lv, np.int32, k * V, "levels")))
+        return lv
+
+    def sssp_two_level(self, sources, delta=0, out=None):
+        """GIR::ssspTwoLevel (gir_query.cpp:705-722) for every source ->
+        uint64 (k, V) distances, 2^64-1 unreachable (equal to sssp)."""
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
+        k, V = len(src), self.V
+        d = np.empty((k, V), np.uint64) if out is None else out
+        _check(lib().girhip_sssp_two_level(self._h, src.ctypes.data if k else None, k, int(delta),
+                                           _ptr(d, np.uint64, k * V, "dist")))
+        return d
 
     def features(self, k):
         """16 graph/query features (see csrc/tune.hip) for k query sources."""

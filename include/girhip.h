@@ -137,6 +137,29 @@ int girhip_bfs_component(girhip_graph* h, const int32_t* src, int32_t k, int32_t
  * first, as the reference does.  Needs girhip_scc_dag (GIRHIP_ESTATE). */
 int girhip_reachable(girhip_graph* h, const int32_t* u, const int32_t* v, int64_t n, uint8_t* out);
 
+/* Replaces GIR::reachableSccs / reachInfo (gir_query.cpp:56-88; public as
+ * reachableSccsPublic, gir.h:205): the part of the condensation reachable
+ * from SCC scc_id (a canonical id: the minimum vertex of its SCC, else
+ * GIRHIP_EINVAL).  mask (V bytes, may be NULL): 1 at the ids of reachable
+ * SCCs; pending (V int32, may be NULL): per reachable SCC id its number of
+ * reachable DAG predecessors (the two-level driver's counts), 0 elsewhere;
+ * n_sccs / n_verts / n_edges: reachable SCCs, their vertices and those
+ * vertices' out-arcs.  Pending ingested changes are synced first. */
+int girhip_reach_info(girhip_graph* h, int32_t scc_id, uint8_t* mask, int32_t* pending, int64_t* n_sccs,
+                      int64_t* n_verts, int64_t* n_edges);
+
+/* Replace GIR::bfsTwoLevel / GIR::ssspTwoLevel (gir_query.cpp:604-620,
+ * 705-722) for k sources: the condensation-guided model of GIR::dagWavefront
+ * (gir_query.cpp:304-350) - SCCs finished in waves of Kahn order over the
+ * part reachable from the source's SCC, an SCC-restricted delta-stepping
+ * inside the multi-vertex SCCs of a wave, the arcs between SCCs relaxed
+ * once.  Results equal girhip_bfs / girhip_sssp: levels k*V int32 (-1 =
+ * unreachable; all -1 for an out-of-range source), dist k*V uint64
+ * (UINT64_MAX = unreachable), delta 0 = the "sssp_delta" knob.  Pending
+ * ingested changes are synced first; needs girhip_scc_dag (GIRHIP_ESTATE). */
+int girhip_bfs_two_level(girhip_graph* h, const int32_t* src, int32_t k, int32_t* levels);
+int girhip_sssp_two_level(girhip_graph* h, const int32_t* src, int32_t k, uint64_t delta, uint64_t* dist);
+
 /* Replaces GIR::sssp (gir_query.cpp:695-703) for k sources: delta-stepping
  * with bucket width delta (0 = the handle's tuned default, 16 unless
  * changed by girhip_set_param).  dist holds k*V uint64, UINT64_MAX =

@@ -466,6 +466,156 @@ extern "C" int orc_apply_batch(int32_t V, const int32_t* fb, const int32_t* fc, 
     return 0;
 }
 
+// ---------------- condensation-guided queries ----------------
+
+namespace {
+// The condensation of canonical SCC labels: deduplicated out- and in-arcs
+// per SCC id (gir.cpp:818-831 builds dag_out / dag_in the same way).
+struct Cond {
+    std::vector<std::vector<int32_t>> out, in, mem;
+};
+Cond condensation(int32_t V, const int32_t* fb, const int32_t* fc, const int32_t* scc) {
+    Cond C;
+    C.out.assign(V, {});
+    C.in.assign(V, {});
+    C.mem.assign(V, {});
+    for (int32_t v = 0; v < V; ++v) C.mem[scc[v]].push_back(v);
+    for (int32_t u = 0; u < V; ++u)
+        for (int32_t e = fb[u]; e < fb[u + 1]; ++e) {
+            const int32_t a = scc[u], b = scc[fc[e]];
+            if (a != b) C.out[a].push_back(b);
+        }
+    for (int32_t a = 0; a < V; ++a) {
+        auto& o = C.out[a];
+        std::sort(o.begin(), o.end());
+        o.erase(std::unique(o.begin(), o.end()), o.end());
+        for (int32_t b : o) C.in[b].push_back(a);
+    }
+    return C;
+}
+
+// gir_query.cpp:56-64 (reachableSccs): queue BFS over dag_out.
+std::vector<uint8_t> reachableSccs(const Cond& C, int32_t s0) {
+    std::vector<uint8_t> r(C.out.size(), 0);
+    std::vector<int32_t> q{s0};
+    r[s0] = 1;
+    for (size_t i = 0; i < q.size(); ++i)
+        for (int32_t d : C.out[q[i]])
+            if (!r[d]) {
+                r[d] = 1;
+                q.push_back(d);
+            }
+    return r;
+}
+
+// gir_query.cpp:80-86: per reachable SCC, its reachable predecessors.
+std::vector<int32_t> pendingCounts(const Cond& C, const std::vector<uint8_t>& r) {
+    std::vector<int32_t> p(C.out.size(), 0);
+    for (size_t c = 0; c < C.out.size(); ++c)
+        if (r[c])
+            for (int32_t a : C.in[c]) p[c] += r[a] ? 1 : 0;
+    return p;
+}
+}  // namespace
+
+// gir_query.cpp:66-88 (reachInfo) from the SCC with canonical id src_scc:
+// reachable SCC mask, pending counts, counts = {SCCs, vertices, out-arcs}.
+extern "C" int orc_reach_info(int32_t V, const int32_t* fb, const int32_t* fc, const int32_t* scc, int32_t src_scc,
+                              uint8_t* mask, int32_t* pending, int64_t* counts) {
+    if (V <= 0 || !fb || !scc || src_scc < 0 || src_scc >= V || scc[src_scc] != src_scc) return 1;
+    const Cond C = condensation(V, fb, fc, scc);
+    const auto r = reachableSccs(C, src_scc);
+    const auto p = pendingCounts(C, r);
+    int64_t ns = 0, nv = 0, ne = 0;
+    for (int32_t c = 0; c < V; ++c) ns += r[c];
+    for (int32_t v = 0; v < V; ++v)
+        if (r[scc[v]]) {
+            ++nv;
+            ne += fb[v + 1] - fb[v];
+        }
+    if (mask) std::copy(r.begin(), r.end(), mask);
+    if (pending) std::copy(p.begin(), p.end(), pending);
+    if (counts) {
+        counts[0] = ns;
+        counts[1] = nv;
+        counts[2] = ne;
+    }
+    return 0;
+}
+
+// gir_query.cpp:304-350 (dagWavefront) with the inner step of
+// gir_query.cpp:604-620 (bfsTwoLevel, unit = 1: levels as distances, the
+// multi-seed BFS inside an SCC) or 705-722 (ssspTwoLevel: delta-stepping of
+// gir_query.cpp:629-693 restricted to the SCC, seeded with every member
+// that holds a distance; single-vertex SCCs skip it).  Ready SCCs are
+// processed one after another (the reference runs them as tasks; their
+// results do not interact).  w_fw: weights per forward slot, NULL for
+// orc_edge_weight.
+extern "C" int orc_two_level(int32_t V, const int32_t* fb, const int32_t* fc, const uint32_t* w_fw,
+                             const int32_t* scc, int32_t source, uint64_t delta, int32_t unit, uint64_t* dist) {
+    if (V < 0 || (V > 0 && (!fb || !scc || !dist)) || (!unit && delta == 0)) return 1;
+    for (int32_t i = 0; i < V; ++i) dist[i] = UINT64_MAX;
+    if (source < 0 || source >= V) return 0;
+    if (unit) delta = 1;
+    auto wt = [&](int32_t u, int64_t e) -> uint64_t {
+        return unit ? 1 : (w_fw ? w_fw[e] : orc_edge_weight(u, fc[e]));
+    };
+    const Cond C = condensation(V, fb, fc, scc);
+    const int32_t s0 = scc[source];
+    const auto r = reachableSccs(C, s0);
+    auto left = pendingCounts(C, r);
+    dist[source] = 0;
+    // seeded bucket rounds inside SCC c: bucket k re-processed until nothing
+    // new lands in it (gir_query.cpp:641-690, one thread)
+    auto inner = [&](int32_t c) {
+        std::vector<std::vector<int32_t>> bk;
+        auto push = [&](int32_t x) {
+            const size_t k = (size_t)(dist[x] / delta);
+            if (bk.size() <= k) bk.resize(k + 1);
+            bk[k].push_back(x);
+        };
+        for (int32_t x : C.mem[c])
+            if (dist[x] != UINT64_MAX) push(x);
+        for (size_t k = 0; k < bk.size(); ++k)
+            while (!bk[k].empty()) {
+                std::vector<int32_t> fr;
+                fr.swap(bk[k]);
+                std::sort(fr.begin(), fr.end());
+                fr.erase(std::unique(fr.begin(), fr.end()), fr.end());
+                for (int32_t u : fr) {
+                    if (dist[u] / delta != k) continue;  // stale entry
+                    for (int32_t e = fb[u]; e < fb[u + 1]; ++e) {
+                        const int32_t x = fc[e];
+                        if (scc[x] != c) continue;
+                        const uint64_t nd = dist[u] + wt(u, e);
+                        if (nd < dist[x]) {
+                            dist[x] = nd;
+                            push(x);
+                        }
+                    }
+                }
+            }
+    };
+    std::vector<int32_t> ready{s0};
+    while (!ready.empty()) {
+        std::vector<int32_t> next;
+        for (int32_t c : ready) {
+            if (C.mem[c].size() > 1) inner(c);
+            for (int32_t x : C.mem[c])
+                for (int32_t e = fb[x]; e < fb[x + 1] && dist[x] != UINT64_MAX; ++e) {
+                    const int32_t y = fc[e];
+                    if (scc[y] == c) continue;
+                    const uint64_t nd = dist[x] + wt(x, e);
+                    if (nd < dist[y]) dist[y] = nd;
+                }
+            for (int32_t d : C.out[c])
+                if (--left[d] == 0) next.push_back(d);
+        }
+        ready.swap(next);
+    }
+    return 0;
+}
+
 // ---------------- textbook checkers ----------------
 
 extern "C" int orc_bfs_textbook(int32_t V, const int32_t* fb, const int32_t* fc, int32_t src, int32_t* level) {

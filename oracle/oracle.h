@@ -27,6 +27,12 @@
  *   orc_edge_weight  golden weights from the compiled reference
  *   orc_normalize_batch / orc_apply_batch  golden post-batch partitions;
  *                  a Python set simulation of last-op-wins on tiny batches
+ *   orc_reach_info golden reachable-SCC masks (reachableSccsPublic);
+ *                  a vertex-level textbook BFS (an SCC is reachable iff
+ *                  its minimum vertex is); pending counts by brute force
+ *                  over the DAG arc list
+ *   orc_two_level  golden bfsTwoLevel / ssspTwoLevel results;
+ *                  orc_bfs_textbook and orc_dijkstra
  */
 #ifndef GIR_ORACLE_H
 #define GIR_ORACLE_H
@@ -102,6 +108,18 @@ int orc_wcc_bfs(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const i
                 const int32_t* bw_adj, int32_t* label);
 int orc_scc_kosaraju(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,
                      const int32_t* bw_adj, int32_t* scc);
+
+/* gir_query.cpp:56-88: the condensation part reachable from SCC src_scc
+ * (canonical id): mask[V] (1 at reachable SCC ids), pending[V] (reachable
+ * DAG predecessors of each reachable SCC id), counts[3] = reachable SCCs,
+ * their vertices, those vertices' out-arcs.  Either array may be NULL. */
+int orc_reach_info(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* scc, int32_t src_scc,
+                   uint8_t* mask, int32_t* pending, int64_t* counts);
+/* gir_query.cpp:304-350, 604-620, 705-722: bfsTwoLevel (unit = 1: levels in
+ * dist, UINT64_MAX = unreachable) or ssspTwoLevel (unit = 0, bucket width
+ * delta) over canonical SCC labels scc. */
+int orc_two_level(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const uint32_t* w_fw,
+                  const int32_t* scc, int32_t src, uint64_t delta, int32_t unit, uint64_t* dist);
 
 /* threads the OpenMP parts use (omp_get_max_threads) */
 int orc_num_threads(void);

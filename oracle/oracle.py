@@ -20,6 +20,8 @@ _i32, _i64, _u64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
 _SIGS = {
     "orc_edge_weight": (ctypes.c_uint32, [_i32, _i32]),
     "orc_normalize_csr": (ctypes.c_int, [_i32, _vp, _vp, ctypes.POINTER(_i64)]),
+    "orc_reach_info": (ctypes.c_int, [_i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp]),
+    "orc_two_level": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _i32, _u64, _i32, _vp]),
     "orc_wcc": (ctypes.c_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
     "orc_scc": (ctypes.c_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
     "orc_dag": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64)]),
@@ -161,6 +163,31 @@ def bfs_component(fb, fa, bb, ba, label, src, undirected):
     _chk(lib().orc_bfs_component(V, _p(fb), _p(fa), _p(bb), _p(ba), _p(label), int(bool(undirected)), int(src),
                                  _p(lv), _p(vis)), "bfs_component")
     return lv, int(vis[0])
+
+
+def reach_info(fb, fa, scc_labels, src_scc):
+    """GIR::reachableSccs / reachInfo (gir_query.cpp:56-88) from the SCC with
+    canonical id src_scc -> (mask bool[V] by SCC id, pending int32[V],
+    (sccs, verts, edges))."""
+    V = len(fb) - 1
+    lab = np.ascontiguousarray(scc_labels, np.int32)
+    mask = np.zeros(V, np.uint8)
+    pend = np.zeros(V, np.int32)
+    cnt = np.zeros(3, np.int64)
+    _chk(lib().orc_reach_info(V, _p(fb), _p(fa), _p(lab), int(src_scc), _p(mask), _p(pend), _p(cnt)), "reach_info")
+    return mask.astype(bool), pend, tuple(int(x) for x in cnt)
+
+
+def two_level(fb, fa, w, scc_labels, src, delta=16, unit=False):
+    """GIR::bfsTwoLevel (unit=True, levels as uint64) / ssspTwoLevel
+    (gir_query.cpp:304-350, 604-722) -> uint64[V], 2^64-1 unreachable."""
+    V = len(fb) - 1
+    lab = np.ascontiguousarray(scc_labels, np.int32)
+    d = np.empty(V, np.uint64)
+    wa = None if w is None else np.ascontiguousarray(w, np.uint32)
+    _chk(lib().orc_two_level(V, _p(fb), _p(fa), _p(wa), _p(lab), int(src), int(delta), int(bool(unit)), _p(d)),
+         "two_level")
+    return d
 
 
 def dijkstra(fb, fa, w, src):

@@ -8,7 +8,7 @@
 // results with canonical (minimum-vertex) component ids:
 //
 //   gir_ref CSR_DIR OUT_DIR [--bfs SRC.bin] [--sssp SRC.bin] [--delta D]
-//           [--batch OPS.bin] [--pairs PAIRS.bin] [--reach PAIRS.bin] [--no-output] [--reps R]
+//           [--batch OPS.bin] [--pairs PAIRS.bin] [--reach PAIRS.bin] [--two-level] [--no-output] [--reps R]
 //
 //   wcc.bin  int32[V]   buildWCC labels (gir.cpp:593-655), canonical
 //   scc.bin  int32[V]   buildSCC_DAG node_scc (gir.cpp:798-839), canonical
@@ -17,6 +17,11 @@
 //   sssp.bin uint64[k*V] GIR::sssp distances, delta D (gir_query.cpp:695-703)
 //   weight.bin uint32[k] GIR::edgeWeight of each (u, v) in PAIRS.bin
 //   reach.bin uint8[k]   GIR::reachable of each (u, v) pair of --reach (gir_query.cpp:789-828)
+//   with --two-level, per --bfs source: bfs2.bin int32[k*V] GIR::bfsTwoLevel
+//   (gir_query.cpp:604-620) and rsccs.bin uint8[k*V] (1 where the vertex's
+//   SCC is in GIR::reachableSccsPublic of the source's SCC, gir_query.cpp:56-64);
+//   per --sssp source: sssp2.bin uint64[k*V] GIR::ssspTwoLevel, delta D
+//   (gir_query.cpp:705-722)
 //   post_{wcc,scc,dag}.bin  the same after GIR::applyBatch of OPS.bin
 //                        (int32 triples u, v, insert) with blocks on demand
 // One JSON line per timed phase goes to stdout, prefixed "@@".
@@ -110,7 +115,7 @@ void writeLayers(const std::string& out, const std::string& pre, const Layers& L
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--pairs F] [--reach F] [--no-output] [--reps R]\n");
+        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--pairs F] [--reach F] [--two-level] [--no-output] [--reps R]\n");
         return 2;
     }
     std::string dir = argv[1], out = argv[2];
@@ -118,7 +123,7 @@ int main(int argc, char** argv) {
     if (out.back() != '/') out += '/';
     std::string bfs_f, sssp_f, batch_f, pairs_f, reach_f;
     uint64_t delta = 16;
-    bool write = true;
+    bool write = true, two_level = false;
     int reps = 1;
     for (int i = 3; i < argc; ++i) {
         std::string a = argv[i];
@@ -130,6 +135,7 @@ int main(int argc, char** argv) {
         else if (a == "--pairs") pairs_f = next();
         else if (a == "--reach") reach_f = next();
         else if (a == "--no-output") write = false;
+        else if (a == "--two-level") two_level = true;
         else if (a == "--reps") reps = std::atoi(next().c_str());
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
@@ -180,6 +186,43 @@ int main(int argc, char** argv) {
         }
         std::printf("@@{\"phase\": \"sssp\", \"sources\": %zu, \"ms\": %.3f, \"keep\": %llu}\n", src.size(), msSince(t0), keep);
         if (write) writeVec(out + "sssp.bin", dv.data(), dv.size());
+    }
+    if (two_level && !bfs_f.empty()) {
+        auto src = readVec<int>(bfs_f);
+        std::vector<int> lv;
+        std::vector<uint8_t> rs;
+        // the two-level BFS walks the source SCC by its block-cut forest: the
+        // reference's drivers build it after the SCC layer (bench_main.cpp:164)
+        g.buildBCT();
+        auto t0 = Clock::now();
+        long long keep = 0;
+        for (int s : src) {
+            auto l = g.bfsTwoLevel(s);
+            if (write) lv.insert(lv.end(), l.begin(), l.end());
+            else keep += l[0];
+        }
+        std::printf("@@{\"phase\": \"bfs2\", \"sources\": %zu, \"ms\": %.3f, \"keep\": %lld}\n", src.size(), msSince(t0), keep);
+        if (write) {
+            for (int s : src) {
+                auto m = g.reachableSccsPublic(g.sccOf(s));
+                for (int v = 0; v < V; ++v) rs.push_back((uint8_t)(m[g.sccOf(v)] != 0));
+            }
+            writeVec(out + "bfs2.bin", lv.data(), lv.size());
+            writeVec(out + "rsccs.bin", rs.data(), rs.size());
+        }
+    }
+    if (two_level && !sssp_f.empty()) {
+        auto src = readVec<int>(sssp_f);
+        std::vector<uint64_t> dv;
+        auto t0 = Clock::now();
+        unsigned long long keep = 0;
+        for (int s : src) {
+            auto d = g.ssspTwoLevel(s, delta);
+            if (write) dv.insert(dv.end(), d.begin(), d.end());
+            else keep += d[0];
+        }
+        std::printf("@@{\"phase\": \"sssp2\", \"sources\": %zu, \"ms\": %.3f, \"keep\": %llu}\n", src.size(), msSince(t0), keep);
+        if (write) writeVec(out + "sssp2.bin", dv.data(), dv.size());
     }
     if (!reach_f.empty()) {
         auto pr = readVec<int>(reach_f);

@@ -56,6 +56,15 @@ def test_golden_cases(golden):
         assert (G.sssp(g["sssp_src"], 16) == g["sssp"]).all()
         say("golden reachable", name)
         assert (G.reachable(g["pairs"][:, 0], g["pairs"][:, 1]) == g["reach"].astype(bool)).all()
+        say("golden two-level", name)
+        assert (G.bfs_two_level(g["bfs_src"]) == g["bfs2"]).all()
+        assert (G.sssp_two_level(g["sssp_src"], 16) == g["sssp2"]).all()
+        for i, x in enumerate(g["bfs_src"]):
+            ri = G.reach_info(int(s[x]))
+            assert (ri["mask"][s] == g["rsccs"][i].astype(bool)).all()
+            m, p, cnt = oracle.reach_info(fb, fa, g["scc"], int(s[x]))
+            assert (ri["mask"] == m).all() and (ri["pending"] == p).all()
+            assert (ri["sccs"], ri["verts"], ri["edges"]) == cnt
         say("golden batch", name)
         ops = g["ops"]
         G.apply_batch(ops[:, 0], ops[:, 1], ops[:, 2])
@@ -409,3 +418,65 @@ def test_reachable_at_scale_both_paths():
         assert np.array_equal(got, want)
         G.set_param("reach_local", 0)
         assert np.array_equal(G.reachable(u, v), want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["rmat16", "lattice_oneway", "lattice_mixed"])
+def test_two_level_queries_at_scale(kind):
+    # bfsTwoLevel / ssspTwoLevel (gir_query.cpp:304-350, 604-722) and
+    # reachInfo (56-88) on a power-law graph (giant SCC, DAG rows longer than
+    # a wave's share) and on lattices with deep condensations (every wave
+    # small): levels equal the oracle's textbook BFS, distances its Dijkstra
+    # (hash weights, then explicit weights at two bucket widths)
+    if kind == "rmat16":
+        V, fb, fa, bb, ba = girhip.rmat_graph(16, edge_factor=16, seed=51)
+    else:
+        V, fb, fa, bb, ba = girhip.grid_graph(150, 150, p_keep=0.9, p_bidir=0.0 if kind == "lattice_oneway" else 0.3,
+                                              seed=53)
+    srcs = girhip.sources(fb, 3, 52)
+    say("two-level", kind, "V", V, "E", len(fa))
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        scc, _, _ = G.scc_dag()
+        lv = G.bfs_two_level(srcs)
+        d = G.sssp_two_level(srcs, 16)
+        for i, x in enumerate(srcs):
+            assert (lv[i] == oracle.bfs_textbook(fb, fa, int(x))).all(), (kind, x)
+            assert (d[i] == oracle.dijkstra(fb, fa, None, int(x))).all(), (kind, x)
+        ri = G.reach_info(int(scc[srcs[0]]))
+        m, p, cnt = oracle.reach_info(fb, fa, scc, int(scc[srcs[0]]))
+        assert (ri["mask"] == m).all() and (ri["pending"] == p).all()
+        assert (ri["sccs"], ri["verts"], ri["edges"]) == cnt
+        w = np.random.default_rng(54).integers(1, 5000, len(fa)).astype(np.uint32)
+        G.set_weights(w)
+        want = oracle.dijkstra(fb, fa, w, int(srcs[0]))
+        for delta in (64, 4096):
+            assert (G.sssp_two_level(srcs[:1], delta)[0] == want).all(), (kind, delta)
+
+
+@pytest.mark.gpu
+def test_two_level_edge_cases():
+    # 64-bit redo when 32-bit distances would overflow, out-of-range
+    # sources, calls before the SCC layer, and the sync of ingested changes
+    V, fb, fa, bb, ba = girhip.rmat_graph(12, edge_factor=8, seed=55)
+    say("two-level edge cases")
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        with pytest.raises(girhip.GirError):
+            G.bfs_two_level([0])
+        scc, _, _ = G.scc_dag()
+        with pytest.raises(girhip.GirError):  # not an SCC id
+            G.reach_info(int(np.nonzero(scc != np.arange(V))[0][0]))
+        x = int(girhip.sources(fb, 1, 56)[0])
+        w = np.full(len(fa), 2**31 - 1, np.uint32)
+        G.set_weights(w)
+        d = G.sssp_two_level([x], 2**31)
+        want = oracle.dijkstra(fb, fa, w, x)
+        assert (d[0] == want).all()
+        assert want[want != INF].max() > 2**32  # beyond 32 bits: the 64-bit redo ran
+        G.set_weights(None)
+        assert (G.bfs_two_level([-1, V])[:, :] == -1).all()
+        assert (G.sssp_two_level([-1], 16) == INF).all()
+        lv = G.bfs_two_level([x])[0]
+        y = int(np.nonzero(lv < 0)[0][0])
+        G.ingest_batch([x], [y], [1])
+        fb2, fa2 = oracle.apply_batch(fb, fa, [x], [y], [1])[:2]
+        assert (G.bfs_two_level([x])[0] == oracle.bfs_textbook(fb2, fa2, x)).all()

@@ -203,3 +203,54 @@ def test_golden_reachable_matches_textbook_bfs(golden):
     for (u, v), r in zip(g["pairs"], g["reach"]):
         want = u == v or (0 <= u < V and oracle.bfs_textbook(fb, fa, int(u))[v] >= 0)
         assert bool(r) == want, (name, u, v)
+
+
+def test_oracle_reach_info_pinned(golden):
+    # reachableSccs / reachInfo (gir_query.cpp:56-88) restated: the mask
+    # equals the compiled reference's reachableSccsPublic, and (textbook) an
+    # SCC is reachable iff its minimum vertex is; pending counts are the
+    # reachable DAG predecessors counted over the golden arc list; vertices
+    # and arcs summed over the reachable SCCs.
+    name, g = golden
+    fb, fa, scc = g["fw_beg"], g["fw_adj"], g["scc"]
+    dag = g["dag"]
+    outdeg = np.diff(fb)
+    for i, s in enumerate(g["bfs_src"]):
+        c = int(scc[s])
+        mask, pend, (ns, nv, ne) = oracle.reach_info(fb, fa, scc, c)
+        assert (mask[scc] == g["rsccs"][i].astype(bool)).all(), name
+        reached = oracle.bfs_textbook(fb, fa, int(s)) >= 0
+        ids = np.nonzero(scc == np.arange(len(scc)))[0]
+        assert (mask[ids] == reached[ids]).all() and not mask[scc != np.arange(len(scc))].any()
+        want = np.zeros(len(scc), np.int32)
+        for a, b in dag:
+            if mask[a] and mask[b]:
+                want[b] += 1
+        assert (pend == want).all(), name
+        on = mask[scc]
+        assert (ns, nv, ne) == (int(mask.sum()), int(on.sum()), int(outdeg[on].sum()))
+    bad = np.nonzero(scc != np.arange(len(scc)))[0]
+    with pytest.raises(ValueError):  # not an SCC id
+        oracle.reach_info(fb, fa, scc, int(bad[0]) if len(bad) else -1)
+
+
+def test_oracle_two_level_pinned(golden):
+    # bfsTwoLevel / ssspTwoLevel (gir_query.cpp:304-350, 604-722) restated:
+    # equal to the compiled reference's results, to a textbook BFS and to
+    # Dijkstra (hash weights, and explicit weights at several bucket widths)
+    name, g = golden
+    fb, fa, scc = g["fw_beg"], g["fw_adj"], g["scc"]
+    INF = np.uint64(2**64 - 1)
+    for i, s in enumerate(g["bfs_src"]):
+        d = oracle.two_level(fb, fa, None, scc, int(s), unit=True)
+        lv = np.where(d == INF, -1, d.astype(np.int64)).astype(np.int32)
+        assert (lv == g["bfs2"][i]).all() and (lv == oracle.bfs_textbook(fb, fa, int(s))).all(), name
+    for i, s in enumerate(g["sssp_src"]):
+        d = oracle.two_level(fb, fa, None, scc, int(s), delta=16)
+        assert (d == g["sssp2"][i]).all() and (d == oracle.dijkstra(fb, fa, None, int(s))).all(), name
+    w = np.random.default_rng(7).integers(1, 2000, len(fa)).astype(np.uint32)
+    for s in g["sssp_src"][:2]:
+        want = oracle.dijkstra(fb, fa, w, int(s))
+        for delta in (1, 37, 5000):
+            assert (oracle.two_level(fb, fa, w, scc, int(s), delta=delta) == want).all(), (name, delta)
+    assert (oracle.two_level(fb, fa, None, scc, -1, unit=True) == INF).all()
