@@ -54,11 +54,13 @@ def make_graph(cfg):
     return girhip.grid_graph(cfg["rows"], cfg["cols"], cfg["p_keep"], cfg["p_bidir"], cfg["seed"])
 
 
-def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=None, reach_pairs=None):
+def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=None, reach_pairs=None,
+                 two_level=False):
     """Time the compiled reference (oracle/_ref/gir_ref: the reference's own
     gir.cpp / gir_query.cpp) on this host: one buildWCC + buildSCC_DAG, BFS
     from n_bfs and SSSP (its hash weights, delta 16) from n_sssp sources, and
-    GIR::applyBatch of batch (u, v, insert arrays) when given."""
+    GIR::applyBatch of batch (u, v, insert arrays) when given; two_level:
+    also GIR::bfsTwoLevel / ssspTwoLevel from the same sources."""
     import girhip
     ref = os.path.join(ROOT, "oracle", "_ref", "gir_ref")
     if not os.path.exists(ref):
@@ -75,6 +77,8 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
         if reach_pairs is not None:
             np.ascontiguousarray(reach_pairs, np.int32).tofile(os.path.join(td, "reach_pairs.bin"))
             cmd += ["--reach", os.path.join(td, "reach_pairs.bin")]
+        if two_level:
+            cmd += ["--two-level"]
         env = dict(os.environ)
         env.setdefault("OMP_PROC_BIND", "close")
         t0 = time.time()
@@ -291,6 +295,28 @@ def main():
         torch.cuda.empty_cache()
         log(f"queries: {queries}")
 
+    # ---- two-level queries (GIR::bfsTwoLevel / ssspTwoLevel, hash weights) ----
+    two = None
+    if not args.no_extras:
+        G.scc_dag(out=scc)
+        G.set_weights(None)
+        tb = bfs_src[:4].astype(np.int32)
+        ts = sssp_src[:2].astype(np.int32)
+        G.bfs_two_level(tb[:1])  # warm (builds the member lists)
+        t_b, t_s = [], []
+        for _ in range(2):
+            lv2 = G.bfs_two_level(tb)
+            t_b.append(G.last_ms("bfs_two_level"))
+            d2 = G.sssp_two_level(ts, 16)
+            t_s.append(G.last_ms("sssp_two_level"))
+        same = bool((lv2 == G.bfs(tb)).all() and (d2 == G.sssp(ts, 16)).all())
+        G.set_weights(d_w)
+        two = {"bfs_sources": len(tb), "bfs_ms_per_source": round(min(t_b) / len(tb), 3),
+               "sssp_sources": len(ts), "sssp_ms_per_source": round(min(t_s) / len(ts), 3), "delta": 16,
+               "equal_to_bfs_sssp": same}
+        del lv2, d2
+        log(f"two-level: {two}")
+
     # ---- condensation-guided reachability (GIR::reachable) on uniform pairs ----
     reach = None
     reach_pairs = None
@@ -339,7 +365,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (compiled reference) ...")
         r = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, batch=first_batch if updates else None,
-                         reach_pairs=reach_pairs[:4096] if reach_pairs is not None else None)
+                         reach_pairs=reach_pairs[:4096] if reach_pairs is not None else None,
+                         two_level=two is not None)
         if r is not None:
             rec, wall = r
             b = rec.get("build", {})
@@ -347,8 +374,9 @@ def main():
             cpu = {"value": round(cpu_build, 3), "unit": "ms", "cores": rec.get("threads"), "kind": "reference",
                    "sample": "compiled reference (oracle/_ref/gir_ref) on this host: one buildWCC + buildSCC_DAG "
                              f"on the same graph, BFS from {rec['bfs']['sources']} and SSSP (reference hash "
-                             f"weights, delta 16) from {rec['sssp']['sources']} of the same sources; "
-                             f"{wall:.1f} s wall",
+                             f"weights, delta 16) from {rec['sssp']['sources']} of the same sources"
+                             + ("; bfsTwoLevel / ssspTwoLevel from the same sources after an untimed buildBCT"
+                                if "bfs2" in rec else "") + f"; {wall:.1f} s wall",
                    "wcc_ms": b.get("wcc_ms"), "scc_dag_ms": b.get("scc_dag_ms"),
                    "bfs_ms_per_source": rec["bfs"]["ms"] / rec["bfs"]["sources"],
                    "sssp_ms_per_source": rec["sssp"]["ms"] / rec["sssp"]["sources"]}
@@ -357,6 +385,9 @@ def main():
             if "batch" in rec:
                 cpu["batch_ops"] = rec["batch"]["ops"]
                 cpu["batch_ms"] = rec["batch"]["ms"]
+            for ph in ("bfs2", "sssp2"):
+                if ph in rec:
+                    cpu[f"{ph}_ms_per_source"] = rec[ph]["ms"] / rec[ph]["sources"]
             if "reach" in rec:
                 cpu["reach_pairs"] = rec["reach"]["pairs"]
                 cpu["reach_pairs_per_s"] = round(rec["reach"]["pairs"] / (rec["reach"]["ms"] * 1e-3), 1)
@@ -395,6 +426,10 @@ def main():
             out["speedup_batch1e5_vs_cpu"] = round(cpu["batch1e5_ms"] / updates[1]["ms"], 2)
     if cpu and reach and cpu.get("reach_pairs_per_s"):
         out["speedup_reachable_vs_cpu"] = round(reach["pairs_per_s"] / cpu["reach_pairs_per_s"], 2)
+    if cpu and two and "bfs2_ms_per_source" in cpu:
+        two["speedup_bfs_vs_cpu"] = round(cpu["bfs2_ms_per_source"] / two["bfs_ms_per_source"], 2)
+        two["speedup_sssp_vs_cpu"] = round(cpu["sssp2_ms_per_source"] / two["sssp_ms_per_source"], 2)
+    out["two_level"] = two
     out["learned_vs_default"] = learned
     out["reachable"] = reach
     out["query_batch"] = queries

@@ -86,6 +86,7 @@ struct Params {
     double persist = 0;  // 1: tiny-frontier phases run in one persistent cooperative kernel
     double sssp_dist32 = 1;  // 1: SSSP works on 32-bit distances (64-bit redo on overflow)
     double reach_local = 1;  // 1: reachable() first tries a per-pair LDS search
+    double graph = 0;        // 1: long runs of full chunks replay captured HIP graphs
 };
 
 struct KStat {
@@ -244,6 +245,44 @@ void launch(girhip_graph* h, const char* name, K kernel, dim3 grid, dim3 block, 
         h->pend_ev.push_back({name, a, b});
     }
 }
+
+// Fixed launch sequences replayed as HIP graphs: the chunked rounds of
+// high-diameter graphs are bound by launch overhead, which a replayed graph
+// lowers.  Sequences are captured on the handle's stream the first time a
+// key is seen and destroyed with the cache.  The captured work is kernels
+// only (fills are kernels too; no memset or copy nodes).
+struct GraphCache {
+    girhip_graph* h;
+    std::map<uint64_t, hipGraphExec_t> ex;
+    explicit GraphCache(girhip_graph* hh) : h(hh) {}
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    ~GraphCache() {
+        for (auto& kv : ex) (void)hipGraphExecDestroy(kv.second);
+    }
+    template <class F>
+    void run(uint64_t key, F issue) {
+        auto it = ex.find(key);
+        if (it == ex.end()) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+            try {
+                issue();
+            } catch (...) {
+                (void)hipStreamEndCapture(h->st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(h->st, &g));
+            hipGraphExec_t e = nullptr;
+            const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(r);
+            it = ex.emplace(key, e).first;
+        }
+        HIPCHK(hipGraphLaunch(it->second, h->st));
+    }
+};
 
 // Persistent kernels: a cooperative launch of at most blocks_per_cu blocks
 // per compute unit (all blocks co-resident, so grid-wide barriers are

@@ -36,18 +36,24 @@ std::vector<int32_t> hostSources(girhip_graph* h, const int32_t* src, int32_t k)
 constexpr uint64_t INF = ~0ULL;
 constexpr int SB = 32;  // sources per batch
 
-// Per-source control block (int64 slots, SB each).
+// Per-source control slots.  Every (slot, source) pair has a 128-byte line
+// of its own (CSTR int64): the queue counters and flags take atomics and
+// stores from every wave of a pass, and counters of different sources on
+// one line would serialise each other.
 enum : int {
-    C_NEAR0 = 0,       // queue length, parity 0
-    C_NEAR1 = SB,      // queue length, parity 1
-    C_NSET0 = 2 * SB,  // 1: the round of parity 0 put a vertex into the bucket
-    C_NSET1 = 3 * SB,  //    (parity 1)
-    C_LO = 4 * SB,     // bucket [lo, hi) (uint64)
-    C_HI = 5 * SB,
-    C_FMIN = 6 * SB,   // smallest far distance >= hi (uint64)
-    C_HQ = 7 * SB,     // heavy-phase queue length
-    C_SLOTS = 8 * SB
+    C_NEAR0 = 0,  // queue length, parity 0
+    C_NEAR1,      // queue length, parity 1
+    C_NSET0,      // 1: the round of parity 0 put a vertex into the bucket
+    C_NSET1,      //    (parity 1)
+    C_LO,         // bucket [lo, hi) (uint64)
+    C_HI,
+    C_FMIN,       // smallest far distance >= hi (uint64)
+    C_HQ,         // heavy-phase queue length
+    C_NSLOT
 };
+constexpr int CSTR = 16;
+constexpr int C_SLOTS = C_NSLOT * SB * CSTR;  // first index past the slot lines
+__host__ __device__ __forceinline__ size_t cIdx(int slot, int s) { return ((size_t)slot * SB + s) * CSTR; }
 
 struct SsspBatch {
     int32_t* qn[2];      // [SB][V] queues by parity
@@ -58,7 +64,7 @@ struct SsspBatch {
     uint64_t* nsum;      // [SB][W2] summaries: bit g of word i <=> word 64i+g
     uint64_t* fsum;      //          of the bitmap may be non-zero
     uint64_t* dsum;
-    int64_t* ctl;        // C_SLOTS (+ persistent parity, overflow flag)
+    int64_t* ctl;        // slot lines (cIdx), then the persistent parity, the overflow flag, a gather area
     void* dist;          // [SB][V] distances: uint32 (d32) or uint64
     int64_t V, W, W2;
     int k;               // sources in the batch
@@ -70,6 +76,10 @@ struct SsspBatch {
 // would reach UINT32_MAX raises this flag instead, and the batch is redone
 // with 64-bit distances.
 constexpr int C_OVF = C_SLOTS + 1;
+constexpr int C_GATHER = C_SLOTS + CSTR;  // SB entries: one slot of every source, for host readback
+constexpr int C_TOTAL = C_GATHER + SB;
+
+__device__ __forceinline__ int64_t* cp(const SsspBatch& B, int slot, int s) { return B.ctl + cIdx(slot, s); }
 
 __device__ __forceinline__ uint64_t dGet(const SsspBatch& B, int s, int32_t x) {
     const size_t i = (size_t)s * B.V + x;
@@ -90,26 +100,25 @@ __device__ __forceinline__ uint64_t dMin(const SsspBatch& B, int s, int32_t x, u
 }
 
 __device__ __forceinline__ int64_t* nearCount(const SsspBatch& B, int par, int s) {
-    return B.ctl + (par ? C_NEAR1 : C_NEAR0) + s;
+    return cp(B, par ? C_NEAR1 : C_NEAR0, s);
 }
 __device__ __forceinline__ int64_t* nearSet(const SsspBatch& B, int par, int s) {
-    return B.ctl + (par ? C_NSET1 : C_NSET0) + s;
+    return cp(B, par ? C_NSET1 : C_NSET0, s);
 }
 
 __global__ void kBInit(SsspBatch B, const int32_t* __restrict__ src, uint64_t delta) {
     const int s = threadIdx.x;
     if (s >= SB) return;
-    int64_t* c = B.ctl;
-    for (int j = 0; j < C_SLOTS / SB; ++j) c[j * SB + s] = 0;
-    c[C_HI + s] = (int64_t)delta;
-    if (s == 0) c[C_OVF] = 0;
+    for (int j = 0; j < C_NSLOT; ++j) *cp(B, j, s) = 0;
+    *cp(B, C_HI, s) = (int64_t)delta;
+    if (s == 0) B.ctl[C_OVF] = 0;
     if (s < B.k) {
         const int32_t v = src[s];
         if (v >= 0 && v < B.V) {
             if (B.d32) ((uint32_t*)B.dist)[(size_t)s * B.V + v] = 0;
             else ((uint64_t*)B.dist)[(size_t)s * B.V + v] = 0;
             B.qn[0][(size_t)s * B.V] = v;
-            c[C_NEAR0 + s] = 1;
+            *cp(B, C_NEAR0, s) = 1;
         }
     }
 }
@@ -154,7 +163,7 @@ struct RelaxF {
         if (nd < dGet(B, s, x)) {  // plain read: distances only fall, so a stale value is a safe pre-check
             const uint64_t old = dMin(B, s, x, nd);
             if (nd < old) {
-                if (nd < (uint64_t)B.ctl[C_HI + s]) {
+                if (nd < (uint64_t)*cp(B, C_HI, s)) {
                     // the first bit of a word raises the flag: the near bitmap
                     // is empty at the start of every round
                     if (setBit(B.nearbits + (size_t)s * B.W, B.nsum + (size_t)s * B.W2, x)) *nearSet(B, par, s) = 1;
@@ -174,8 +183,8 @@ struct RelaxF {
 __device__ __forceinline__ void resetRound(const SsspBatch& B, int par, int s) {
     *nearCount(B, par ^ 1, s) = 0;
     *nearSet(B, par ^ 1, s) = 0;
-    B.ctl[C_FMIN + s] = (int64_t)INF;
-    B.ctl[C_HQ + s] = 0;
+    *cp(B, C_FMIN, s) = (int64_t)INF;
+    *cp(B, C_HQ, s) = 0;
 }
 
 // Relax the n queued vertices q of source s (grid stride, uniform per wave).
@@ -244,7 +253,7 @@ __device__ __forceinline__ void farMinCommit(const SsspBatch& B, int s, unsigned
         const unsigned long long t = __shfl_xor(mn, o);
         mn = t < mn ? t : mn;
     }
-    if (laneId() == 0 && mn != INF) atomicMin((unsigned long long*)&B.ctl[C_FMIN + s], mn);
+    if (laneId() == 0 && mn != INF) atomicMin((unsigned long long*)cp(B, C_FMIN, s), mn);
 }
 
 // After the relaxation of parity par, for source s: if the round put
@@ -260,10 +269,10 @@ __device__ __forceinline__ void compactPhase(const SsspBatch& B, int par, int he
         });
     } else if (heavy) {
         forSummary(B.dsum + so, B.W2, G, [&](int64_t g, uint64_t sm) {
-            drainGroup(B.donebits + bo, B.dsum + so, g, sm, B.qh + qo, B.ctl + C_HQ + s);
+            drainGroup(B.donebits + bo, B.dsum + so, g, sm, B.qh + qo, cp(B, C_HQ, s));
         });
     } else {
-        const uint64_t hi = (uint64_t)ldAcq(&B.ctl[C_HI + s]);
+        const uint64_t hi = (uint64_t)ldAcq(cp(B, C_HI, s));
         uint64_t mn = INF;
         int32_t* lds = sh.ids[threadIdx.x / WAVE];
         forSummary(B.fsum + so, B.W2, G,
@@ -275,7 +284,7 @@ __device__ __forceinline__ void compactPhase(const SsspBatch& B, int par, int he
 // Drained source s after its heavy phase: smallest far distance >= hi.
 __device__ __forceinline__ void farMinPhase(const SsspBatch& B, int par, int G, int s, FarLds& sh) {
     if (ldAcq(nearSet(B, par, s))) return;
-    const uint64_t hi = (uint64_t)ldAcq(&B.ctl[C_HI + s]);
+    const uint64_t hi = (uint64_t)ldAcq(cp(B, C_HI, s));
     uint64_t mn = INF;
     int32_t* lds = sh.ids[threadIdx.x / WAVE];
     forSummary(B.fsum + (size_t)s * B.W2, B.W2, G,
@@ -290,13 +299,13 @@ __device__ __forceinline__ void farMinPhase(const SsspBatch& B, int par, int G, 
 // A summary word is cleared once its 64 bitmap words are all empty.
 __device__ __forceinline__ void splitPhase(const SsspBatch& B, int par, uint64_t delta, int G, int s, FarLds& sh) {
     if (ldAcq(nearSet(B, par, s))) return;
-    const uint64_t mfar = (uint64_t)ldAcq(&B.ctl[C_FMIN + s]);
+    const uint64_t mfar = (uint64_t)ldAcq(cp(B, C_FMIN, s));
     if (mfar == INF) return;  // no far work left: this source is done
     const uint64_t lo = (mfar / delta) * delta;
     const uint64_t hi = lo > INF - delta ? INF : lo + delta;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        B.ctl[C_LO + s] = (int64_t)lo;
-        B.ctl[C_HI + s] = (int64_t)hi;
+        *cp(B, C_LO, s) = (int64_t)lo;
+        *cp(B, C_HI, s) = (int64_t)hi;
     }
     int32_t* q = B.qn[par ^ 1] + (size_t)s * B.V;
     uint64_t* fs = B.fsum + (size_t)s * B.W2;
@@ -350,7 +359,7 @@ __global__ void kBCompact(SsspBatch B, int par, int heavy, int G) {
 __global__ void kBRelaxHeavy(RelaxF f, const int32_t* __restrict__ fb, HeavyList H) {
     const int s = blockIdx.y;
     const SsspBatch& B = f.B;
-    relaxQueue(f, fb, H, B.qh + (size_t)s * B.V, B.ctl[C_HQ + s], s, false);
+    relaxQueue(f, fb, H, B.qh + (size_t)s * B.V, *cp(B, C_HQ, s), s, false);
 }
 
 __global__ void kBFarMin(SsspBatch B, int par, int G) {
@@ -390,7 +399,7 @@ __global__ void __launch_bounds__(256) kSsspPersist(RelaxF fl, RelaxF fh, const 
         for (int s = 0; s < k; ++s) compactPhase(B, par, heavy, G, s, sh);
         gridSync();
         if (heavy) {
-            for (int s = 0; s < k; ++s) relaxQueue(fh, fb, H, B.qh + (size_t)s * B.V, ldAcq(&B.ctl[C_HQ + s]), s,
+            for (int s = 0; s < k; ++s) relaxQueue(fh, fb, H, B.qh + (size_t)s * B.V, ldAcq(cp(B, C_HQ, s)), s,
                                                    false);
             gridSync();
             for (int s = 0; s < k; ++s) farMinPhase(B, par, G, s, sh);
@@ -401,6 +410,17 @@ __global__ void __launch_bounds__(256) kSsspPersist(RelaxF fl, RelaxF fh, const 
         par ^= 1;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *io = par;
+}
+
+// One slot of every source -> the gather area (contiguous, for one readback).
+__global__ void kGatherSlot(int64_t* ctl, int slot) {
+    const int s = threadIdx.x;
+    if (s < SB) ctl[C_GATHER + s] = ctl[cIdx(slot, s)];
+}
+
+static void readSlot(girhip_graph* h, int64_t* ctl, int slot, int64_t* out) {
+    launch(h, "sssp_gather", kGatherSlot, 1, 64, 0, ctl, slot);
+    readCounters(h, ctl + C_GATHER, SB, out);
 }
 
 // 32-bit distances -> the caller's 64-bit rows (UINT32_MAX -> UINT64_MAX).
@@ -484,6 +504,11 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
     // persistent rounds need every row below the heavy-list threshold
     bool persist_ok = h->prm.persist != 0 && maxDegree(h) <= HEAVY_T;
     const int64_t cap = std::max<int64_t>(1, (int64_t)V / 64);  // queues up to this size stay persistent
+    // after two full chunks, full chunks replay captured graphs (a run of
+    // many small rounds: high-diameter graphs, small delta)
+    const bool graphs = h->prm.graph != 0 && h->prm.profile == 0 && cmax > 1;
+    GraphCache gc(h);
+    int full = 0;
     while (true) {
         if (persist_ok && maxn <= cap) {
             int64_t* io = S.s_ctl.p + C_SLOTS;
@@ -493,7 +518,7 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
                 int64_t pr;
                 readCounters(h, io, 1, &pr);
                 par = (int)pr;
-                readCounters(h, S.s_ctl.p + (par ? C_NEAR1 : C_NEAR0), SB, r.data());
+                readSlot(h, S.s_ctl.p, par ? C_NEAR1 : C_NEAR0, r.data());
                 int64_t mx = 0;
                 for (int s = 0; s < k; ++s) mx = std::max(mx, r[s]);
                 if (mx == 0) break;
@@ -503,21 +528,33 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
             }
             persist_ok = false;  // no cooperative launch here: host-driven rounds
         }
-        const unsigned rgrid = R > 1 ? std::min<unsigned>(1024, std::max<unsigned>(64, gridFor(4 * maxn, blk)))
-                                     : gridFor(maxn, blk);
-        for (int t = 0; t < R; ++t) {
-            launch(h, "sssp_relax", kBRelax, dim3(rgrid, k), blk, 0, fl[par], fb, heavyList(h), heavy);
-            launchHeavyStore(h, "sssp_relax_rows", fl[par]);
-            launch(h, "sssp_compact", kBCompact, dim3(sgrid, k), blk, 0, B, par, heavy, G);
-            if (heavy) {
-                launch(h, "sssp_relax_heavy", kBRelaxHeavy, dim3(hgrid, k), blk, 0, fh[par], fb, heavyList(h));
-                launchHeavyStore(h, "sssp_relax_heavy_rows", fh[par]);
-                launch(h, "sssp_farmin", kBFarMin, dim3(sgrid, k), blk, 0, B, par, G);
-            }
-            launch(h, "sssp_split", kBSplit, dim3(sgrid, k), blk, 0, B, par, delta, G);
-            par ^= 1;
+        unsigned rgrid = R > 1 ? std::min<unsigned>(1024, std::max<unsigned>(64, gridFor(4 * maxn, blk)))
+                               : gridFor(maxn, blk);
+        const bool replay = graphs && R == cmax && full >= 2;
+        if (replay) {  // few distinct grids, so few captured sequences
+            unsigned q = 64;
+            while (q < rgrid) q <<= 1;
+            rgrid = q;
         }
-        readCounters(h, S.s_ctl.p + (par ? C_NEAR1 : C_NEAR0), SB, r.data());
+        auto issue = [&](int p) {
+            for (int t = 0; t < R; ++t) {
+                launch(h, "sssp_relax", kBRelax, dim3(rgrid, k), blk, 0, fl[p], fb, heavyList(h), heavy);
+                launchHeavyStore(h, "sssp_relax_rows", fl[p]);
+                launch(h, "sssp_compact", kBCompact, dim3(sgrid, k), blk, 0, B, p, heavy, G);
+                if (heavy) {
+                    launch(h, "sssp_relax_heavy", kBRelaxHeavy, dim3(hgrid, k), blk, 0, fh[p], fb, heavyList(h));
+                    launchHeavyStore(h, "sssp_relax_heavy_rows", fh[p]);
+                    launch(h, "sssp_farmin", kBFarMin, dim3(sgrid, k), blk, 0, B, p, G);
+                }
+                launch(h, "sssp_split", kBSplit, dim3(sgrid, k), blk, 0, B, p, delta, G);
+                p ^= 1;
+            }
+        };
+        if (replay) gc.run(((uint64_t)rgrid << 8) | ((uint64_t)R << 1) | (uint64_t)par, [&] { issue(par); });
+        else issue(par);
+        if (R == cmax) ++full;
+        par ^= R & 1;
+        readSlot(h, S.s_ctl.p, par ? C_NEAR1 : C_NEAR0, r.data());
         int64_t mx = 0;
         for (int s = 0; s < k; ++s) mx = std::max(mx, r[s]);
         if (mx == 0) break;  // every queue empty after the advance: all sources settled
@@ -819,7 +856,7 @@ __global__ void kWaveInit(SsspBatch B, const int32_t* __restrict__ scc, const in
 __global__ void kBInitSeeded(SsspBatch B) {
     const int s = threadIdx.x;
     if (s >= SB) return;
-    for (int j = 0; j < C_SLOTS / SB; ++j) B.ctl[j * SB + s] = 0;
+    for (int j = 0; j < C_NSLOT; ++j) *cp(B, j, s) = 0;
 }
 
 __global__ void kZeroSlot(int64_t* p) {
@@ -889,7 +926,7 @@ static void ensureBatchScratch(girhip_graph* h, int kc) {
     S.s_nsum.ensure((size_t)kc * W2s);
     S.s_fsum.ensure((size_t)kc * W2s);
     S.s_dsum.ensure((size_t)kc * W2s);
-    S.s_ctl.ensure(C_SLOTS + 2);
+    S.s_ctl.ensure(C_TOTAL);
 }
 
 }  // namespace gir
@@ -924,7 +961,7 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         S.s_nsum.ensure((size_t)kc * W2s);
         S.s_fsum.ensure((size_t)kc * W2s);
         S.s_dsum.ensure((size_t)kc * W2s);
-        S.s_ctl.ensure(C_SLOTS + 2);  // + the persistent kernel's parity slot and the overflow flag
+        S.s_ctl.ensure(C_TOTAL);  // slot lines, the persistent kernel's parity, the overflow flag, the gather area
         S.src.ensure((size_t)(k > 0 ? k : 1));
         const int32_t* dsrc = S.src.p;
         copyIn(h, S.src.p, s.data(), sizeof(int32_t) * (size_t)k);

@@ -212,6 +212,10 @@ int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
  *   "sssp_dist32" (1: SSSP on 32-bit distances, redone in 64 bits on
  *   overflow), "reach_local" (1: girhip_reachable first tries one
  *   bidirectional LDS search per pair; 0: bit-parallel sweeps only),
+ *   "graph" (0; 1: after two full chunks of SSSP rounds, further full
+ *   chunks are captured once and replayed as HIP graphs - measured no
+ *   faster on MI355X, whose rounds are bound by the kernels, not by their
+ *   launches; ignored while profiling),
  *   "profile" (0: 1 times every kernel)
  * Values only change work order; results never depend on them. */
 int girhip_set_param(girhip_graph* h, const char* name, double value);

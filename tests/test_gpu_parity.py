@@ -480,3 +480,25 @@ def test_two_level_edge_cases():
         G.ingest_batch([x], [y], [1])
         fb2, fa2 = oracle.apply_batch(fb, fa, [x], [y], [1])[:2]
         assert (G.bfs_two_level([x])[0] == oracle.bfs_textbook(fb2, fa2, x)).all()
+
+
+@pytest.mark.gpu
+def test_sssp_graph_replay_matches_eager():
+    # a lattice at delta 1 runs hundreds of chunks of rounds: with the
+    # "graph" knob the full chunks replay captured HIP graphs (many replays,
+    # readbacks between them, a second call re-captures); distances equal
+    # the eager launches' and Dijkstra's
+    V, fb, fa, bb, ba = girhip.grid_graph(120, 120, seed=61)
+    srcs = girhip.sources(fb, 3, 62)
+    say("sssp graph replay lattice 120x120")
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        want = [oracle.dijkstra(fb, fa, None, int(x)) for x in srcs]
+        for graph in (1, 0, 1):
+            G.set_param("graph", graph)
+            d = G.sssp(srcs, 1)
+            for i in range(len(srcs)):
+                assert (d[i] == want[i]).all(), (graph, i)
+        G.set_param("graph", 1)
+        G.scc_dag()
+        d2 = G.sssp_two_level(srcs[:1], 1)
+        assert (d2[0] == want[0]).all()
