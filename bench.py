@@ -55,12 +55,13 @@ def make_graph(cfg):
 
 
 def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=None, reach_pairs=None,
-                 two_level=False):
+                 two_level=False, bc_src=None):
     """Time the compiled reference (oracle/_ref/gir_ref: the reference's own
     gir.cpp / gir_query.cpp) on this host: one buildWCC + buildSCC_DAG, BFS
     from n_bfs and SSSP (its hash weights, delta 16) from n_sssp sources, and
     GIR::applyBatch of batch (u, v, insert arrays) when given; two_level:
-    also GIR::bfsTwoLevel / ssspTwoLevel from the same sources."""
+    also GIR::bfsTwoLevel / ssspTwoLevel from the same sources; bc_src:
+    GIR::bc over those sources (scores returned for comparison)."""
     import girhip
     ref = os.path.join(ROOT, "oracle", "_ref", "gir_ref")
     if not os.path.exists(ref):
@@ -79,6 +80,9 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
             cmd += ["--reach", os.path.join(td, "reach_pairs.bin")]
         if two_level:
             cmd += ["--two-level"]
+        if bc_src is not None:
+            np.ascontiguousarray(bc_src, np.int32).tofile(os.path.join(td, "bc_src.bin"))
+            cmd += ["--bc", os.path.join(td, "bc_src.bin")]
         env = dict(os.environ)
         env.setdefault("OMP_PROC_BIND", "close")
         t0 = time.time()
@@ -87,6 +91,9 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
         reach_ans = None
         if reach_pairs is not None and os.path.exists(os.path.join(td, "reach.bin")):
             reach_ans = np.fromfile(os.path.join(td, "reach.bin"), np.uint8)
+        bc_scores = None
+        if bc_src is not None and os.path.exists(os.path.join(td, "bc.bin")):
+            bc_scores = np.fromfile(os.path.join(td, "bc.bin"), np.float64)
     rec = {}
     for line in out.splitlines():
         if line.startswith("@@"):
@@ -97,7 +104,47 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
                 rec.update(d)
     if reach_ans is not None and "reach" in rec:
         rec["reach"]["answers"] = reach_ans
+    if bc_scores is not None and "bc" in rec:
+        rec["bc"]["scores"] = bc_scores
     return rec, wall
+
+
+def stream_runs(fb, fa, B=10000, NB=5, cpu=True):
+    """The streaming protocol (bench_main.cpp:280-345 hold-out: a fresh GIR on
+    half the arcs, then insert / delete / mixed streams of B-op batches, each
+    followed by BFS and SSSP from scratch) through the streaming caller with
+    the reference's command line: ours on the GPU (FMU and lazy) and, when
+    built, the reference's own experiments/baselines/gir/gir_stream on the
+    host.  -> {system: {metric: value}}."""
+    import girhip
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        d = os.path.join(td, "proto")
+        girhip.export_protocol(d, fb, fa, B * NB, n_sources=1, seed=2027, full=False)
+        runs = [("gfx950", os.path.join(PKG, "gir_stream"), ""), ("gfx950", os.path.join(PKG, "gir_stream"), "lazy")]
+        ref = os.path.join(ROOT, "oracle", "_ref", "gir_stream")
+        if cpu and os.path.exists(ref):
+            runs += [("reference", ref, ""), ("reference", ref, "lazy")]
+        for who, exe, mode in runs:
+            csv = os.path.join(td, f"{who}_{mode or 'fmu'}.csv")
+            env = dict(os.environ)
+            env.setdefault("OMP_PROC_BIND", "close")
+            t0 = time.time()
+            r = subprocess.run([exe, d, "rmat", csv, str(B), str(NB)] + ([mode] if mode else []),
+                               capture_output=True, text=True, env=env)
+            if r.returncode != 0:
+                raise RuntimeError(f"{exe} {mode}: {r.stderr[-500:]}")
+            rows = {}
+            threads = None
+            for line in open(csv).read().splitlines():
+                f = line.split(",")
+                rows[f[2]] = float(f[4])
+                threads = int(f[7])
+            key = f"{who}_{mode or 'fmu'}"
+            out[key] = {"threads": threads, "wall_s": round(time.time() - t0, 1),
+                        **{k: round(v, 3) for k, v in rows.items()}}
+            log(f"stream {key}: {out[key]}")
+    return out
 
 
 def main():
@@ -119,7 +166,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU design is described in DESIGN.md", file=sys.stderr)
+              "queries would shard by source over replicas of the graph", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -317,6 +364,22 @@ def main():
         del lv2, d2
         log(f"two-level: {two}")
 
+    # ---- betweenness centrality (GIR::bc, Brandes) from 16 of the BFS sources ----
+    bcr = None
+    bc_scores = None
+    if not args.no_extras:
+        bsrc = bfs_src[:16].astype(np.int32)
+        G.bc(bsrc[:1])  # warm
+        bc_scores = G.bc(bsrc)
+        b_ms = G.last_ms("bc")
+        bcr = {"sources": len(bsrc), "ms": round(b_ms, 3), "ms_per_source": round(b_ms / len(bsrc), 3)}
+        log(f"bc: {bcr}")
+
+    # ---- streaming protocol (gir_stream: batches of 10^4, BFS + SSSP after each) ----
+    stream = None
+    if not args.no_extras:
+        stream = stream_runs(fb, fa, cpu=not args.no_cpu)
+
     # ---- condensation-guided reachability (GIR::reachable) on uniform pairs ----
     reach = None
     reach_pairs = None
@@ -366,7 +429,7 @@ def main():
         log("cpu baseline (compiled reference) ...")
         r = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, batch=first_batch if updates else None,
                          reach_pairs=reach_pairs[:4096] if reach_pairs is not None else None,
-                         two_level=two is not None)
+                         two_level=two is not None, bc_src=bfs_src[:16] if bcr else None)
         if r is not None:
             rec, wall = r
             b = rec.get("build", {})
@@ -385,6 +448,13 @@ def main():
             if "batch" in rec:
                 cpu["batch_ops"] = rec["batch"]["ops"]
                 cpu["batch_ms"] = rec["batch"]["ms"]
+            if "bc" in rec and bcr:
+                cpu["bc_ms_per_source"] = rec["bc"]["ms"] / rec["bc"]["sources"]
+                if "scores" in rec["bc"]:
+                    ref_sc = rec["bc"]["scores"]
+                    rel = np.abs(bc_scores - ref_sc) / (1 + np.abs(ref_sc))
+                    bcr["max_rel_diff_vs_reference"] = float(rel.max())
+                bcr["speedup_vs_cpu"] = round(cpu["bc_ms_per_source"] / bcr["ms_per_source"], 2)
             for ph in ("bfs2", "sssp2"):
                 if ph in rec:
                     cpu[f"{ph}_ms_per_source"] = rec[ph]["ms"] / rec[ph]["sources"]
@@ -430,6 +500,13 @@ def main():
         two["speedup_bfs_vs_cpu"] = round(cpu["bfs2_ms_per_source"] / two["bfs_ms_per_source"], 2)
         two["speedup_sssp_vs_cpu"] = round(cpu["sssp2_ms_per_source"] / two["sssp_ms_per_source"], 2)
     out["two_level"] = two
+    if stream and "reference_fmu" in stream:
+        g, c = stream["gfx950_fmu"], stream["reference_fmu"]
+        stream["speedup_vs_cpu"] = {
+            k: round(c[k] / g[k], 2) if k.endswith("_ms") else round(g[k] / c[k], 2)
+            for k in g if k in c and k not in ("threads", "wall_s") and g[k] and c[k]}
+    out["stream"] = stream
+    out["bc"] = bcr
     out["learned_vs_default"] = learned
     out["reachable"] = reach
     out["query_batch"] = queries

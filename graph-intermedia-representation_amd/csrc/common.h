@@ -132,6 +132,11 @@ struct Scratch {
     DevBuf<uint64_t> tl_mask;
     DevBuf<int32_t> tl_list, tl_left, tl_rdy, tl_vl;
     DevBuf<int64_t> tl_ctl;
+    // betweenness centrality batch: levels, path counts, dependencies (fp64
+    // bits), visit orders, level starts, counters
+    DevBuf<int32_t> bc_lev, bc_ord, bc_lv;
+    DevBuf<uint64_t> bc_sigma, bc_delta;
+    DevBuf<int64_t> bc_ctl;
     // heavy-row list
     DevBuf<int64_t> hv_ctl, hv_b, hv_e, hv_off;
     DevBuf<int32_t> hv_row, hv_aux;
@@ -380,6 +385,27 @@ __device__ __forceinline__ int64_t wavePush(int64_t* counter, bool pred) {
     if (laneId() == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)__popcll(m));
     base = __shfl(base, leader);
     return pred ? base + __popcll(m & lanesBelow()) : -1;
+}
+
+// wavePush for lanes that may push to different counters: the lanes of one
+// counter at a time share one atomic.  Row functors need it when the heavy
+// pass (kHeavyRows) puts arcs of rows from different sources or sweeps into
+// one wave; elsewhere all lanes share a counter and the loop runs once.
+__device__ __forceinline__ int64_t wavePushAny(int64_t* counter, bool pred) {
+    int64_t pos = -1;
+    uint64_t pending = __ballot(pred);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        int64_t* lc = (int64_t*)__shfl((unsigned long long)(uintptr_t)counter, leader);
+        const bool mine = pred && pos < 0 && counter == lc;
+        const uint64_t m = __ballot(mine);
+        int64_t base = 0;
+        if (laneId() == leader) base = atomicAdd((unsigned long long*)lc, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader);
+        if (mine) pos = base + __popcll(m & lanesBelow());
+        pending &= ~m;
+    }
+    return pos;
 }
 
 // Relaxed, device-coherent loads/stores (bypass the non-coherent CU cache).

@@ -252,7 +252,7 @@ struct ReachPushF {
         const size_t o = (size_t)y * R.V + z;
         const uint64_t m = fp & allowed(R.lim + (size_t)y * 64, R.rank[z]) & ~(R.seen[o] | R.next[o]);
         const bool first = m && atomicOr((unsigned long long*)&R.next[o], (unsigned long long)m) == 0;
-        const int64_t pos = wavePush(R.ctl + (size_t)y * R_SLOTS + R_NT + par, first);
+        const int64_t pos = wavePushAny(R.ctl + (size_t)y * R_SLOTS + R_NT + par, first);  // heavy-pass waves mix sweeps
         if (first) R.tq[(size_t)y * R.V + pos] = z;
         return 0;
     }

@@ -65,6 +65,7 @@ _SIGS = {
     "girhip_reach_info": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp]),
     "girhip_bfs_two_level": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp]),
     "girhip_sssp_two_level": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_uint64, _vp]),
+    "girhip_bc": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp]),
     "girhip_sssp": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_uint64, _vp, _vp]),
     "girhip_apply_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]),
     "girhip_ingest_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]),
@@ -106,6 +107,8 @@ _GEN_SIGS = {
     "girgen_weights": (ctypes.c_int, [ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _vp]),
     "girgen_sources": (ctypes.c_int, [ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_uint64, _vp]),
     "girgen_write_csr": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, _vp]),
+    "girgen_export_protocol": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, _vp, _vp, ctypes.c_int64,
+                                              ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32]),
     "girgen_read_csr": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32),
                                        ctypes.POINTER(ctypes.c_int64), _vp, _vp, _vp, _vp]),
 }
@@ -461,6 +464,21 @@ class Graph:
                                            _ptr(d, np.uint64, k * V, "dist")))
         return d
 
+    def bc(self, sources, out=None):
+        """GIR::bc (gir_query.cpp:728-778): Brandes betweenness over the
+        sources -> float64 (V,)."""
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
+        k = len(src)
+        sc = np.empty(max(self.V, 1), np.float64) if out is None else out
+        if out is not None and not isinstance(out, np.ndarray):
+            ptr = out.data_ptr()
+        else:
+            if sc.dtype != np.float64 or not sc.flags["C_CONTIGUOUS"]:
+                raise TypeError("bc: out must be contiguous float64")
+            ptr = sc.ctypes.data
+        _check(lib().girhip_bc(self._h, src.ctypes.data if k else None, k, ptr))
+        return sc[:self.V] if out is None else out
+
     def features(self, k):
         """16 graph/query features (see csrc/tune.hip) for k query sources."""
         f = np.empty(16, np.float32)
@@ -614,6 +632,39 @@ def write_csr(dirpath, fb, fa, bb, ba):
     os.makedirs(dirpath, exist_ok=True)
     _gcheck(gen().girgen_write_csr(dirpath.encode(), len(fb) - 1, len(fa), fb.ctypes.data, fa.ctypes.data,
                                    bb.ctypes.data, ba.ctypes.data))
+
+
+def export_protocol(dirpath, fb, fa, k_ops, n_sources=8, seed=2027, full=True):
+    """Streaming-protocol inputs (loaded.el, full.el, insert/delete/mixed.ops,
+    sources.txt, meta.txt) of the reference's baseline drivers
+    (bench_main.cpp:280-345, experiments/baselines/common/protocol.h) for the
+    canonical CSR (fb, fa): a seeded 50% hold-out (full.el only if full)."""
+    os.makedirs(dirpath, exist_ok=True)
+    fb = np.ascontiguousarray(fb, np.int32)
+    fa = np.ascontiguousarray(fa, np.int32)
+    _gcheck(gen().girgen_export_protocol(dirpath.encode(), len(fb) - 1, fb.ctypes.data, fa.ctypes.data, int(k_ops),
+                                         int(n_sources), int(seed), int(bool(full))))
+
+
+def read_protocol(dirpath):
+    """The protocol's inputs back as arrays: dict(V, loaded (n, 2), insert /
+    delete / mixed: (ops (n,) uint8 1 = insert, arcs (n, 2)), sources)."""
+    meta = dict(l.split() for l in open(os.path.join(dirpath, "meta.txt")) if l.strip())
+    V = int(meta["V"])
+
+    def el(f):
+        a = np.loadtxt(os.path.join(dirpath, f), dtype=np.int64, ndmin=2)
+        return a.reshape(-1, 2).astype(np.int32)
+
+    def ops(f):
+        rows = [l.split() for l in open(os.path.join(dirpath, f)) if l.strip()]
+        ins = np.array([r[0] == "a" for r in rows], np.uint8)
+        arcs = np.array([[int(r[1]), int(r[2])] for r in rows], np.int32).reshape(-1, 2)
+        return ins, arcs
+
+    return {"V": V, "loaded": el("loaded.el"), "insert": ops("insert.ops"), "delete": ops("delete.ops"),
+            "mixed": ops("mixed.ops"),
+            "sources": np.loadtxt(os.path.join(dirpath, "sources.txt"), dtype=np.int32, ndmin=1)}
 
 
 def read_csr(dirpath):

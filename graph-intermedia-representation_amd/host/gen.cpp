@@ -300,3 +300,123 @@ extern "C" int girgen_read_csr(const char* dir, int32_t* V_out, int64_t* E_out, 
         return fail("read_csr: malformed CSR (offsets not monotone from 0 to E, or ids outside [0, V))");
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Streaming protocol inputs (the file set bench_main.cpp:280-345 exports and
+// experiments/baselines/common/protocol.h reads).
+
+namespace {
+// Text writer: "u v\n" / "a u v\n" lines formatted into a large buffer.
+struct TextOut {
+    FILE* f = nullptr;
+    std::vector<char> buf;
+    size_t n = 0;
+    bool ok = true;
+    explicit TextOut(const std::string& p) : f(std::fopen(p.c_str(), "wb")), buf(1 << 22) { ok = f != nullptr; }
+    ~TextOut() { close(); }
+    bool close() {
+        if (f) {
+            flush();
+            ok = (std::fclose(f) == 0) && ok;
+            f = nullptr;
+        }
+        return ok;
+    }
+    void flush() {
+        if (f && n && std::fwrite(buf.data(), 1, n, f) != n) ok = false;
+        n = 0;
+    }
+    void num(uint32_t x) {
+        char t[12];
+        int k = 0;
+        do {
+            t[k++] = (char)('0' + x % 10);
+            x /= 10;
+        } while (x);
+        while (k) buf[n++] = t[--k];
+    }
+    void line(char op, uint32_t u, uint32_t v) {
+        if (n + 32 > buf.size()) flush();
+        if (op) {
+            buf[n++] = op;
+            buf[n++] = ' ';
+        }
+        num(u);
+        buf[n++] = ' ';
+        num(v);
+        buf[n++] = '\n';
+    }
+};
+}  // namespace
+
+extern "C" int girgen_export_protocol(const char* dir, int32_t V, const int32_t* fw_beg, const int32_t* fw_adj,
+                                      int64_t k_ops, int32_t n_sources, uint64_t seed, int32_t write_full) {
+    if (!dir || V < 1 || !fw_beg || k_ops < 0 || n_sources < 0) return fail("export_protocol: bad argument");
+    const int64_t E = fw_beg[V];
+    if (E < 4) return fail("export_protocol: too few arcs");
+    std::string d(dir);
+    if (!d.empty() && d.back() != '/') d += '/';
+    // every arc once, then a seeded Fisher-Yates shuffle (counter-based draws)
+    std::vector<uint64_t> all((size_t)E);
+    for (int32_t u = 0; u < V; ++u)
+        for (int64_t i = fw_beg[u]; i < fw_beg[u + 1]; ++i) all[(size_t)i] = ((uint64_t)(uint32_t)u << 32) | (uint32_t)fw_adj[i];
+    for (int64_t i = E - 1; i > 0; --i) {
+        const int64_t j = (int64_t)(draw(seed, 7, (uint64_t)i) % (uint64_t)(i + 1));
+        std::swap(all[(size_t)i], all[(size_t)j]);
+    }
+    const int64_t half = E / 2;
+    const int64_t k = std::min<int64_t>(k_ops, half / 2);
+    auto U = [&](int64_t i) { return (uint32_t)(all[(size_t)i] >> 32); };
+    auto W = [&](int64_t i) { return (uint32_t)all[(size_t)i]; };
+    {
+        TextOut o(d + "loaded.el");
+        for (int64_t i = 0; i < half; ++i) o.line(0, U(i), W(i));
+        if (!o.close()) return fail("export_protocol: cannot write " + d + "loaded.el");
+        if (write_full) {
+            TextOut f(d + "full.el");
+            for (int64_t i = 0; i < E; ++i) f.line(0, U(i), W(i));
+            if (!f.close()) return fail("export_protocol: cannot write " + d + "full.el");
+        }
+    }
+    {
+        // insert-only: k held-out arcs; delete-only: k arcs of the loaded half
+        TextOut a(d + "insert.ops"), b(d + "delete.ops");
+        for (int64_t i = 0; i < k; ++i) {
+            a.line('a', U(half + i), W(half + i));
+            b.line('d', U(i), W(i));
+        }
+        if (!a.close() || !b.close()) return fail("export_protocol: cannot write the op streams under " + d);
+    }
+    {
+        // mixed: k/2 further held-out inserts and k/2 further loaded deletes,
+        // interleaved in a seeded shuffle
+        const int64_t h2 = k / 2;
+        std::vector<int64_t> m;
+        m.reserve((size_t)(2 * h2));
+        for (int64_t i = 0; i < h2; ++i) m.push_back(half + k + i);  // insert
+        for (int64_t i = 0; i < h2; ++i) m.push_back(-(k + i) - 1);  // delete (encoded)
+        for (int64_t i = (int64_t)m.size() - 1; i > 0; --i) {
+            const int64_t j = (int64_t)(draw(seed, 8, (uint64_t)i) % (uint64_t)(i + 1));
+            std::swap(m[(size_t)i], m[(size_t)j]);
+        }
+        TextOut o(d + "mixed.ops");
+        for (int64_t x : m) {
+            if (x >= 0) o.line('a', U(x), W(x));
+            else o.line('d', U(-x - 1), W(-x - 1));
+        }
+        if (!o.close()) return fail("export_protocol: cannot write " + d + "mixed.ops");
+    }
+    // query sources over vertices with out-arcs in the full graph
+    std::vector<int32_t> src((size_t)std::max(n_sources, 1));
+    if (n_sources > 0 && girgen_sources(V, fw_beg, n_sources, seed ^ 0x5eed, src.data()) != 0) return 1;
+    FILE* f = std::fopen((d + "sources.txt").c_str(), "w");
+    if (!f) return fail("export_protocol: cannot write " + d + "sources.txt");
+    for (int32_t i = 0; i < n_sources; ++i) std::fprintf(f, "%d\n", src[(size_t)i]);
+    std::fclose(f);
+    f = std::fopen((d + "meta.txt").c_str(), "w");
+    if (!f) return fail("export_protocol: cannot write " + d + "meta.txt");
+    std::fprintf(f, "V %d\nE %lld\nloaded %lld\nops_per_phase %lld\nmax_weight %d\nsymmetric 0\n", V, (long long)E,
+                 (long long)half, (long long)k, 100);
+    std::fclose(f);
+    return 0;
+}

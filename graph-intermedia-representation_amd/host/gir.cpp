@@ -325,6 +325,38 @@ QueryVec<uint64_t> GIR::ssspMany(const std::vector<int>& sources, uint64_t delta
 }
 QueryVec<int> GIR::bfs(int source) { return bfsMany({source}); }
 
+QueryVec<int> GIR::bfsTwoLevel(int source) {
+    if (!built_scc) buildSCC_DAG();
+    QueryVec<int> lv((size_t)V);
+    const int32_t s = source;
+    check(girhip_bfs_two_level(h, &s, 1, lv.data()), "bfsTwoLevel");
+    stale = true;  // a pending batch may have been synced
+    return lv;
+}
+
+QueryVec<uint64_t> GIR::ssspTwoLevel(int source, uint64_t delta) {
+    if (!built_scc) buildSCC_DAG();
+    QueryVec<uint64_t> d((size_t)V);
+    const int32_t s = source;
+    check(girhip_sssp_two_level(h, &s, 1, delta, d.data()), "ssspTwoLevel");
+    stale = true;
+    return d;
+}
+
+std::vector<char> GIR::reachableSccsPublic(int c) {
+    if (!built_scc) buildSCC_DAG();
+    std::vector<uint8_t> m((size_t)V);
+    check(girhip_reach_info(h, c, m.data(), nullptr, nullptr, nullptr, nullptr), "reachableSccs");
+    stale = true;
+    return std::vector<char>(m.begin(), m.end());
+}
+
+QueryVec<double> GIR::bc(const std::vector<int>& sources) {
+    QueryVec<double> sc((size_t)V);
+    check(girhip_bc(h, sources.data(), (int32_t)sources.size(), sc.data()), "bc");
+    return sc;
+}
+
 std::vector<char> GIR::reachableMany(const std::vector<int>& u, const std::vector<int>& v) {
     if (u.size() != v.size()) throw std::invalid_argument("reachableMany: u and v differ in length");
     if (!built_scc) buildSCC_DAG();

@@ -116,6 +116,17 @@ public:
     // Many sources in one call (k x V, row i for sources[i]).
     QueryVec<int> bfsMany(const std::vector<int>& sources);
     QueryVec<uint64_t> ssspMany(const std::vector<int>& sources, uint64_t delta = 16);
+    // Condensation-guided two-level queries (gir_query.cpp:604-620,
+    // 705-722): equal to bfs / sssp; pending ingested changes are synced
+    // first, as the reference does.
+    QueryVec<int> bfsTwoLevel(int source);
+    QueryVec<uint64_t> ssspTwoLevel(int source, uint64_t delta = 16);
+    // SCCs reachable from SCC c in the condensation (gir.h:205,
+    // gir_query.cpp:56-64), indexed by SCC id; ids here are canonical (the
+    // minimum vertex of each SCC), so the mask has vertexCount() entries.
+    std::vector<char> reachableSccsPublic(int c);
+    // Brandes betweenness over the sources (gir_query.cpp:728-778).
+    QueryVec<double> bc(const std::vector<int>& sources);
     // Explicit per-arc weights aligned to the forward CSR (nullptr restores
     // edgeWeight); not in the reference, whose weights are always the hash.
     void setWeights(const uint32_t* w_fw);

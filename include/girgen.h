@@ -70,6 +70,20 @@ int girgen_write_csr(const char* dir, int32_t V, int64_t E, const int32_t* fw_be
 int girgen_read_csr(const char* dir, int32_t* V_out, int64_t* E_out, int32_t* fw_beg, int32_t* fw_adj,
                     int32_t* bw_beg, int32_t* bw_adj);
 
+/* Streaming-protocol inputs in the text formats of the reference's
+ * baseline drivers (written by bench_main.cpp:280-345 --export, read by
+ * experiments/baselines/common/protocol.h): the arcs of the canonical CSR
+ * shuffled with a seeded draw; loaded.el = the first half ("u v" lines),
+ * full.el = all arcs, insert.ops = k held-out arcs ("a u v"), delete.ops =
+ * k arcs of the loaded half ("d u v"), mixed.ops = k/2 further held-out
+ * inserts and k/2 further loaded deletes in a seeded shuffle, with
+ * k = min(k_ops, E/4); sources.txt = n_sources query sources (out-degree
+ * > 0); meta.txt = V, E, loaded, ops_per_phase, max_weight, symmetric.
+ * full.el only when write_full != 0 (the streaming callers do not read
+ * it).  dir must exist. */
+int girgen_export_protocol(const char* dir, int32_t V, const int32_t* fw_beg, const int32_t* fw_adj,
+                           int64_t k_ops, int32_t n_sources, uint64_t seed, int32_t write_full);
+
 #ifdef __cplusplus
 }
 #endif

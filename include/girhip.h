@@ -160,6 +160,15 @@ int girhip_reach_info(girhip_graph* h, int32_t scc_id, uint8_t* mask, int32_t* p
 int girhip_bfs_two_level(girhip_graph* h, const int32_t* src, int32_t k, int32_t* levels);
 int girhip_sssp_two_level(girhip_graph* h, const int32_t* src, int32_t k, uint64_t delta, uint64_t* dist);
 
+/* Replaces GIR::bc (gir_query.cpp:728-778): Brandes betweenness over the k
+ * sources (unit-length directed arcs): score[v] (V doubles, host or device)
+ * = sum over sources s != v of the dependency of s on v.  Out-of-range
+ * sources are skipped; a repeated source counts each time.  Sums of
+ * fractions are accumulated in another order than the reference's, so
+ * scores agree to rounding (relative 1e-9 or better on test graphs), not
+ * bit for bit. */
+int girhip_bc(girhip_graph* h, const int32_t* src, int32_t k, double* score);
+
 /* Replaces GIR::sssp (gir_query.cpp:695-703) for k sources: delta-stepping
  * with bucket width delta (0 = the handle's tuned default, 16 unless
  * changed by girhip_set_param).  dist holds k*V uint64, UINT64_MAX =

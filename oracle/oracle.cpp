@@ -616,6 +616,53 @@ extern "C" int orc_two_level(int32_t V, const int32_t* fb, const int32_t* fc, co
     return 0;
 }
 
+// gir_query.cpp:728-778 (GIR::bc): per source a queue BFS in visit order
+// that counts shortest paths (sigma), then the order in reverse, each
+// vertex w giving every in-neighbour u one level up sigma[u] * (1 +
+// delta[w]) / sigma[w]; score[w] += delta[w] for w != s.  Sources run one
+// after another (the reference runs them as tasks and adds with atomics).
+extern "C" int orc_bc(int32_t V, const int32_t* fb, const int32_t* fc, const int32_t* bb, const int32_t* bc,
+                      const int32_t* src, int32_t k, double* score) {
+    if (V < 0 || k < 0 || (V > 0 && (!fb || !bb || !score)) || (k > 0 && !src)) return 1;
+    for (int32_t i = 0; i < V; ++i) score[i] = 0.0;
+    std::vector<int32_t> dist(V, -1), order;
+    std::vector<double> sigma(V, 0.0), delta(V, 0.0);
+    for (int32_t si = 0; si < k; ++si) {
+        const int32_t s = src[si];
+        if (s < 0 || s >= V) continue;
+        order.clear();
+        dist[s] = 0;
+        sigma[s] = 1;
+        order.push_back(s);
+        for (size_t i = 0; i < order.size(); ++i) {
+            const int32_t u = order[i];
+            for (int32_t e = fb[u]; e < fb[u + 1]; ++e) {
+                const int32_t w = fc[e];
+                if (dist[w] < 0) {
+                    dist[w] = dist[u] + 1;
+                    order.push_back(w);
+                }
+                if (dist[w] == dist[u] + 1) sigma[w] += sigma[u];
+            }
+        }
+        for (size_t i = order.size(); i-- > 0;) {
+            const int32_t w = order[i];
+            const double coef = (1.0 + delta[w]) / sigma[w];
+            for (int32_t e = bb[w]; e < bb[w + 1]; ++e) {
+                const int32_t u = bc[e];
+                if (dist[u] >= 0 && dist[u] == dist[w] - 1) delta[u] += sigma[u] * coef;
+            }
+            if (w != s) score[w] += delta[w];
+        }
+        for (int32_t x : order) {
+            dist[x] = -1;
+            sigma[x] = 0;
+            delta[x] = 0;
+        }
+    }
+    return 0;
+}
+
 // ---------------- textbook checkers ----------------
 
 extern "C" int orc_bfs_textbook(int32_t V, const int32_t* fb, const int32_t* fc, int32_t src, int32_t* level) {

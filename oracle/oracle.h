@@ -33,6 +33,9 @@
  *                  over the DAG arc list
  *   orc_two_level  golden bfsTwoLevel / ssspTwoLevel results;
  *                  orc_bfs_textbook and orc_dijkstra
+ *   orc_bc         golden GIR::bc scores; the definition (sums over pairs
+ *                  of sigma_sv * sigma_vt / sigma_st from all-pairs BFS)
+ *                  on tiny graphs
  */
 #ifndef GIR_ORACLE_H
 #define GIR_ORACLE_H
@@ -120,6 +123,11 @@ int orc_reach_info(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, cons
  * delta) over canonical SCC labels scc. */
 int orc_two_level(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const uint32_t* w_fw,
                   const int32_t* scc, int32_t src, uint64_t delta, int32_t unit, uint64_t* dist);
+
+/* gir_query.cpp:728-778: Brandes betweenness over the k sources
+ * (out-of-range ones skipped), score[V]. */
+int orc_bc(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg, const int32_t* bw_adj,
+           const int32_t* src, int32_t k, double* score);
 
 /* threads the OpenMP parts use (omp_get_max_threads) */
 int orc_num_threads(void);

@@ -22,6 +22,7 @@ _SIGS = {
     "orc_normalize_csr": (ctypes.c_int, [_i32, _vp, _vp, ctypes.POINTER(_i64)]),
     "orc_reach_info": (ctypes.c_int, [_i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp]),
     "orc_two_level": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _i32, _u64, _i32, _vp]),
+    "orc_bc": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
     "orc_wcc": (ctypes.c_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
     "orc_scc": (ctypes.c_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
     "orc_dag": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64)]),
@@ -188,6 +189,15 @@ def two_level(fb, fa, w, scc_labels, src, delta=16, unit=False):
     _chk(lib().orc_two_level(V, _p(fb), _p(fa), _p(wa), _p(lab), int(src), int(delta), int(bool(unit)), _p(d)),
          "two_level")
     return d
+
+
+def bc(fb, fa, bb, ba, sources):
+    """GIR::bc (gir_query.cpp:728-778) -> float64[V]."""
+    V = len(fb) - 1
+    src = np.ascontiguousarray(sources, np.int32)
+    sc = np.empty(V, np.float64)
+    _chk(lib().orc_bc(V, _p(fb), _p(fa), _p(bb), _p(ba), _p(src), len(src), _p(sc)), "bc")
+    return sc
 
 
 def dijkstra(fb, fa, w, src):

@@ -8,7 +8,8 @@
 // results with canonical (minimum-vertex) component ids:
 //
 //   gir_ref CSR_DIR OUT_DIR [--bfs SRC.bin] [--sssp SRC.bin] [--delta D]
-//           [--batch OPS.bin] [--pairs PAIRS.bin] [--reach PAIRS.bin] [--two-level] [--no-output] [--reps R]
+//           [--batch OPS.bin] [--pairs PAIRS.bin] [--reach PAIRS.bin] [--two-level] [--bc SRC.bin]
+//           [--no-output] [--reps R]
 //
 //   wcc.bin  int32[V]   buildWCC labels (gir.cpp:593-655), canonical
 //   scc.bin  int32[V]   buildSCC_DAG node_scc (gir.cpp:798-839), canonical
@@ -22,6 +23,7 @@
 //   SCC is in GIR::reachableSccsPublic of the source's SCC, gir_query.cpp:56-64);
 //   per --sssp source: sssp2.bin uint64[k*V] GIR::ssspTwoLevel, delta D
 //   (gir_query.cpp:705-722)
+//   bc.bin   double[V] GIR::bc over the sources of --bc (gir_query.cpp:728-778)
 //   post_{wcc,scc,dag}.bin  the same after GIR::applyBatch of OPS.bin
 //                        (int32 triples u, v, insert) with blocks on demand
 // One JSON line per timed phase goes to stdout, prefixed "@@".
@@ -115,13 +117,13 @@ void writeLayers(const std::string& out, const std::string& pre, const Layers& L
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--pairs F] [--reach F] [--two-level] [--no-output] [--reps R]\n");
+        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--pairs F] [--reach F] [--two-level] [--bc F] [--no-output] [--reps R]\n");
         return 2;
     }
     std::string dir = argv[1], out = argv[2];
     if (dir.back() != '/') dir += '/';
     if (out.back() != '/') out += '/';
-    std::string bfs_f, sssp_f, batch_f, pairs_f, reach_f;
+    std::string bfs_f, sssp_f, batch_f, pairs_f, reach_f, bc_f;
     uint64_t delta = 16;
     bool write = true, two_level = false;
     int reps = 1;
@@ -136,6 +138,7 @@ int main(int argc, char** argv) {
         else if (a == "--reach") reach_f = next();
         else if (a == "--no-output") write = false;
         else if (a == "--two-level") two_level = true;
+        else if (a == "--bc") bc_f = next();
         else if (a == "--reps") reps = std::atoi(next().c_str());
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
@@ -223,6 +226,13 @@ int main(int argc, char** argv) {
         }
         std::printf("@@{\"phase\": \"sssp2\", \"sources\": %zu, \"ms\": %.3f, \"keep\": %llu}\n", src.size(), msSince(t0), keep);
         if (write) writeVec(out + "sssp2.bin", dv.data(), dv.size());
+    }
+    if (!bc_f.empty()) {
+        auto src = readVec<int>(bc_f);
+        auto t0 = Clock::now();
+        auto sc = g.bc(src);
+        std::printf("@@{\"phase\": \"bc\", \"sources\": %zu, \"ms\": %.3f}\n", src.size(), msSince(t0));
+        writeVec(out + "bc.bin", sc.data(), sc.size());  // written even with --no-output (compared by the bench)
     }
     if (!reach_f.empty()) {
         auto pr = readVec<int>(reach_f);

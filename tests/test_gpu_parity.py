@@ -65,6 +65,8 @@ def test_golden_cases(golden):
             m, p, cnt = oracle.reach_info(fb, fa, g["scc"], int(s[x]))
             assert (ri["mask"] == m).all() and (ri["pending"] == p).all()
             assert (ri["sccs"], ri["verts"], ri["edges"]) == cnt
+        say("golden bc", name)
+        assert np.allclose(G.bc(g["bfs_src"]), g["bc"], rtol=1e-9, atol=1e-9)
         say("golden batch", name)
         ops = g["ops"]
         G.apply_batch(ops[:, 0], ops[:, 1], ops[:, 2])
@@ -502,3 +504,25 @@ def test_sssp_graph_replay_matches_eager():
         G.scc_dag()
         d2 = G.sssp_two_level(srcs[:1], 1)
         assert (d2[0] == want[0]).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["rmat16", "lattice"])
+def test_bc_matches_oracle(kind):
+    # GIR::bc (gir_query.cpp:728-778): Brandes over more sources than one
+    # batch holds, with an invalid and a repeated source, on a power-law
+    # graph (hub rows through the heavy list) and a lattice (hundreds of
+    # levels, many equal-length paths); fp64 sums in another order than the
+    # oracle's, so within 1e-9 relative
+    if kind == "rmat16":
+        V, fb, fa, bb, ba = girhip.rmat_graph(16, edge_factor=16, seed=81)
+    else:
+        V, fb, fa, bb, ba = girhip.grid_graph(200, 200, p_keep=0.9, p_bidir=0.5, seed=82)
+    src = list(girhip.sources(fb, 20, 83)) + [-1, V + 5]
+    src += src[:2]
+    say("bc", kind, "V", V, "sources", len(src))
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        got = G.bc(src)
+    want = oracle.bc(fb, fa, bb, ba, src)
+    assert np.allclose(got, want, rtol=1e-9, atol=1e-9), float(np.max(np.abs(got - want) / (1 + np.abs(want))))
+    assert want.max() > 0

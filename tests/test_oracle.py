@@ -254,3 +254,51 @@ def test_oracle_two_level_pinned(golden):
         for delta in (1, 37, 5000):
             assert (oracle.two_level(fb, fa, w, scc, int(s), delta=delta) == want).all(), (name, delta)
     assert (oracle.two_level(fb, fa, None, scc, -1, unit=True) == INF).all()
+
+
+def _bc_by_definition(fb, fa, sources):
+    """Betweenness from its definition: sum over sources s and targets t of
+    sigma_sv * sigma_vt / sigma_st over the vertices v != s, t on shortest
+    s-t paths (all-pairs BFS path counts; tiny graphs only)."""
+    V = len(fb) - 1
+    dist = np.full((V, V), -1, np.int64)
+    sig = np.zeros((V, V))
+    for a in range(V):
+        dist[a, a], sig[a, a] = 0, 1
+        q = [a]
+        for u in q:
+            for w in fa[fb[u]:fb[u + 1]]:
+                if dist[a, w] < 0:
+                    dist[a, w] = dist[a, u] + 1
+                    q.append(w)
+                if dist[a, w] == dist[a, u] + 1:
+                    sig[a, w] += sig[a, u]
+    sc = np.zeros(V)
+    for s in sources:
+        if not 0 <= s < V:
+            continue
+        for t in range(V):
+            if t == s or dist[s, t] < 0:
+                continue
+            on = (dist[s] >= 0) & (dist[:, t] >= 0) & (dist[s] + dist[:, t] == dist[s, t])
+            on[[s, t]] = False
+            sc[on] += sig[s, on] * sig[on, t] / sig[s, t]
+    return sc
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_oracle_bc_matches_definition(seed):
+    # GIR::bc (gir_query.cpp:728-778) restated; pinned by the definition of
+    # betweenness on tiny graphs with many equal-length paths
+    V, fb, fa, bb, ba = girhip.grid_graph(6, 7, p_keep=0.9, p_bidir=0.6, seed=seed)
+    src = list(range(0, V, 3)) + [-1, 4, 4]  # invalid skipped, repeats count twice
+    got = oracle.bc(fb, fa, bb, ba, src)
+    want = _bc_by_definition(fb, fa, src)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+    assert want.max() > 0
+
+
+def test_oracle_bc_matches_reference_golden(golden):
+    name, g = golden
+    got = oracle.bc(g["fw_beg"], g["fw_adj"], g["bw_beg"], g["bw_adj"], g["bfs_src"])
+    assert np.allclose(got, g["bc"], rtol=1e-12, atol=1e-9), name
