@@ -213,11 +213,19 @@ extern "C" int girhip_bc(girhip_graph* h, const int32_t* src, int32_t k, double*
             launch(h, "bc_seed", kBcSeed, 1, 64, 0, B, dsrc);
             // forward levels, a chunk per readback
             int32_t d = 0;
-            int64_t width = 1;  // largest level of the last readback
+            int64_t width = 1, prev = 0;  // largest level of the last two readbacks
+            int Cp = 1;
             std::vector<int64_t> last(BB);
             while (true) {
-                const int C = width * 64 < (int64_t)V ? BC_CHUNK : 1;
-                const unsigned g = std::min<unsigned>(1024, std::max<unsigned>(16, gridFor(4 * width, blk)));
+                // a chunk of levels only while levels stay small and are not
+                // growing fast (its grid is sized at the chunk's start)
+                const int C = (prev > 0 && width * 64 < (int64_t)V && width <= 2 * prev)
+                                  ? std::min(BC_CHUNK, 2 * Cp)
+                                  : 1;
+                Cp = C;
+                prev = width;
+                const unsigned g = C > 1 ? std::min<unsigned>(1024, std::max<unsigned>(16, gridFor(8 * width, blk)))
+                                         : std::min<unsigned>(1024, std::max<unsigned>(1, gridFor(width, blk)));
                 for (int t = 0; t < C; ++t, ++d) {
                     const BcForwardF f{B, h->fw_adj.p, d};
                     launch(h, "bc_forward", kBcForward, dim3(g, c), blk, 0, f, (const int32_t*)h->fw_beg.p,

@@ -231,7 +231,8 @@ int girhip_set_param(girhip_graph* h, const char* name, double value);
 int girhip_get_param(const girhip_graph* h, const char* name, double* value);
 
 /* Device time (ms, HIP events) of the kernels the last call of each kind
- * launched: kind "wcc", "scc_dag", "bfs", "sssp", "batch". */
+ * launched: kind "wcc", "scc_dag", "bfs", "sssp", "batch", "reachable",
+ * "reach_info", "bfs_two_level", "sssp_two_level", "bc", "query_batch". */
 int girhip_last_ms(const girhip_graph* h, const char* kind, double* ms);
 
 /* Per-kernel device time when the "profile" parameter is 1: every launch is
