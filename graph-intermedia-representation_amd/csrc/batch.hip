@@ -375,6 +375,8 @@ void applyStore(girhip_graph* h, const int32_t* u, const int32_t* v, const uint8
              B, newE);
     mergeCsr(h, h->bw_beg, h->bw_adj, h->bw_adj2, nullptr, nullptr, T, B, newE);
     h->E = newE;
+    incLog(h, C.ik, C.ni);
+    incLog(h, C.dk, C.nd);
     h->src_ok = false;
     h->max_deg = -1;
     h->max_w = -1;

@@ -137,6 +137,13 @@ struct Scratch {
     DevBuf<int32_t> bc_lev, bc_ord, bc_lv;
     DevBuf<uint64_t> bc_sigma, bc_delta;
     DevBuf<int64_t> bc_ctl;
+    // incremental queries: from-scratch distances / levels, sorted and unique
+    // log keys, presence, invalid list, round queues, flags (zero between
+    // updates), counters
+    DevBuf<uint64_t> inc_d, inc_keys, inc_u;
+    DevBuf<int32_t> inc_lv, inc_list, inc_q[2], inc_flag;
+    DevBuf<uint8_t> inc_pres;
+    DevBuf<int64_t> inc_ctl;
     // heavy-row list
     DevBuf<int64_t> hv_ctl, hv_b, hv_e, hv_off;
     DevBuf<int32_t> hv_row, hv_aux;
@@ -183,6 +190,16 @@ struct girhip_graph {
     gir::DevBuf<int32_t> dag_in_beg, dag_in_src;  // DAG in-arcs by destination SCC id (V+1, n_dag)
     gir::DevBuf<int32_t> mem_beg, mem_v;  // members by SCC id (V+1, V), built on demand
     bool mem_ok = false;
+    // incremental single-source queries, [0] BFS levels, [1] SSSP distances:
+    // packed (distance << 32 | parent) per vertex, and the store's changes
+    // since the last update (packed (u, v) keys, as the batch path sorts them)
+    struct IncTrack {
+        bool on = false;
+        int32_t source = -1;
+        gir::DevBuf<uint64_t> st;
+        gir::DevBuf<uint64_t> log;
+        int64_t nlog = 0;
+    } inc[2];
     int64_t n_dag = 0;
     gir::DevBuf<int64_t> rank;
 
@@ -821,6 +838,9 @@ void launchHeavy(girhip_graph* h, const char* name, const F& f, int64_t* sum = n
 // changes); the undirected graph of the reference's buildWCC
 // (gir.cpp:593-620).  Returns its arc count.
 int64_t symCsr(girhip_graph* h);
+// Incremental queries: record n changed arcs (packed (u << B) | v keys,
+// B = bitsFor(V - 1)) for every tracked kind.
+void incLog(girhip_graph* h, const uint64_t* keys, int64_t n);
 // Largest row of the edge store's fw and bw CSR (cached until the store
 // changes).
 int64_t maxDegree(girhip_graph* h);

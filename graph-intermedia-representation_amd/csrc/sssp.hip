@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "common.h"
+#include "lists.h"
 
 namespace gir {
 
@@ -594,7 +595,6 @@ static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delt
 //   3. every DAG successor of a ready SCC loses one pending predecessor;
 //      those reaching zero form the next wave.
 // BFS levels are the same computation with unit weights and delta 1.
-enum : int { T_LO = 0, T_HI, T_TAIL, T_NV, T_NBIG, T_VERTS, T_EDGES, T_SLOTS = 8 };
 
 __global__ void kMemKeys(const int32_t* __restrict__ scc, int32_t V, int B, uint64_t* __restrict__ keys) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -662,31 +662,6 @@ struct PendF {
     }
 };
 
-// Rows (CSR beg) of the ids list[ctl[T_LO] .. ctl[T_HI]) (bounds read on the
-// device), each with its length as payload; grid stride, one lane per id.
-template <class F>
-__global__ void kListRows(const int32_t* __restrict__ beg, const int32_t* list, const int64_t* ctl, F f, HeavyList H) {
-    const int64_t lo = ctl[T_LO], hi = ctl[T_HI];
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = lo + (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < hi;
-         base += stride) {
-        const int64_t i = base + laneId();
-        const bool act = i < hi;
-        const int32_t x = act ? list[i] : 0;
-        const int64_t b = act ? beg[x] : 0, e = act ? beg[x + 1] : 0;
-        waveRowsH(act, x, (uint64_t)(e - b), 0, b, e, H, f);
-    }
-}
-
-// Next level / wave: [lo, hi) := [hi, tail); the vertex list restarts.
-__global__ void kListAdvance(int64_t* ctl) {
-    if (threadIdx.x == 0) {
-        ctl[T_LO] = ctl[T_HI];
-        ctl[T_HI] = ctl[T_TAIL];
-        ctl[T_NV] = 0;
-    }
-}
-
 // Sweep seed: the SCC of vertex src (canonical id scc[src]).
 __global__ void kSweepSeed(const int32_t* __restrict__ scc, int32_t src, uint64_t* mask, int32_t* list, int64_t* ctl) {
     if (threadIdx.x != 0) return;
@@ -696,14 +671,6 @@ __global__ void kSweepSeed(const int32_t* __restrict__ scc, int32_t src, uint64_
     for (int j = 0; j < T_SLOTS; ++j) ctl[j] = 0;
     ctl[T_HI] = 1;
     ctl[T_TAIL] = 1;
-}
-
-// Whole reached list as the current range (for the pending-count pass).
-__global__ void kListAll(int64_t* ctl) {
-    if (threadIdx.x == 0) {
-        ctl[T_LO] = 0;
-        ctl[T_HI] = ctl[T_TAIL];
-    }
 }
 
 // Vertices in reachable SCCs and their out-arcs.

@@ -66,6 +66,9 @@ _SIGS = {
     "girhip_bfs_two_level": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp]),
     "girhip_sssp_two_level": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_uint64, _vp]),
     "girhip_bc": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp]),
+    "girhip_inc_track": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
+    "girhip_inc_update": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "girhip_inc_dist": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "girhip_sssp": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_uint64, _vp, _vp]),
     "girhip_apply_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]),
     "girhip_ingest_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]),
@@ -478,6 +481,21 @@ class Graph:
             ptr = sc.ctypes.data
         _check(lib().girhip_bc(self._h, src.ctypes.data if k else None, k, ptr))
         return sc[:self.V] if out is None else out
+
+    def track_incremental(self, source, weighted):
+        """GIR::trackIncremental (gir_query.cpp:867-891)."""
+        _check(lib().girhip_inc_track(self._h, int(source), int(bool(weighted))))
+
+    def update_incremental(self, weighted):
+        """GIR::updateIncremental (gir_query.cpp:893-996)."""
+        _check(lib().girhip_inc_update(self._h, int(bool(weighted))))
+
+    def inc_dist(self, weighted):
+        """GIR::incDist for every vertex (gir.h:229-233) -> uint64 (V,),
+        2^64-1 unreachable (BFS: levels)."""
+        d = np.empty(max(self.V, 1), np.uint64)
+        _check(lib().girhip_inc_dist(self._h, int(bool(weighted)), d.ctypes.data))
+        return d[:self.V]
 
     def features(self, k):
         """16 graph/query features (see csrc/tune.hip) for k query sources."""

@@ -351,6 +351,25 @@ std::vector<char> GIR::reachableSccsPublic(int c) {
     return std::vector<char>(m.begin(), m.end());
 }
 
+void GIR::trackIncremental(int source, bool weighted) {
+    check(girhip_inc_track(h, source, weighted ? 1 : 0), "trackIncremental");
+    inc_fresh[weighted] = false;
+}
+
+void GIR::updateIncremental(bool weighted) {
+    check(girhip_inc_update(h, weighted ? 1 : 0), "updateIncremental");
+    inc_fresh[weighted] = false;
+}
+
+uint64_t GIR::incDist(bool weighted, int v) const {
+    if (!inc_fresh[weighted]) {
+        inc_host[weighted].resize((size_t)V);
+        check(girhip_inc_dist(h, weighted ? 1 : 0, inc_host[weighted].data()), "incDist");
+        inc_fresh[weighted] = true;
+    }
+    return (v >= 0 && v < V) ? inc_host[weighted][(size_t)v] : UINT64_MAX;
+}
+
 QueryVec<double> GIR::bc(const std::vector<int>& sources) {
     QueryVec<double> sc((size_t)V);
     check(girhip_bc(h, sources.data(), (int32_t)sources.size(), sc.data()), "bc");

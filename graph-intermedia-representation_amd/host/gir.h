@@ -127,6 +127,13 @@ public:
     std::vector<char> reachableSccsPublic(int c);
     // Brandes betweenness over the sources (gir_query.cpp:728-778).
     QueryVec<double> bc(const std::vector<int>& sources);
+    // Incremental single-source BFS (weighted = false) / SSSP (true)
+    // (gir.h:222-233, gir_query.cpp:867-996): track once, then every edge
+    // change is logged; updateIncremental corrects the result; incDist reads
+    // it (UINT64_MAX = unreachable; BFS: the level).
+    void trackIncremental(int source, bool weighted);
+    void updateIncremental(bool weighted);
+    uint64_t incDist(bool weighted, int v) const;
     // Explicit per-arc weights aligned to the forward CSR (nullptr restores
     // edgeWeight); not in the reference, whose weights are always the hash.
     void setWeights(const uint32_t* w_fw);
@@ -162,6 +169,8 @@ private:
     mutable std::vector<std::vector<int>> sccs;
     mutable bool sccs_ok = false;
     mutable std::unique_ptr<graph> g_copy;
+    mutable std::vector<uint64_t> inc_host[2];  // incDist values, refreshed after a track / update
+    mutable bool inc_fresh[2] = {false, false};
 };
 
 // The reference's ParallelBFS harness (gir_test.cpp:143-343): a

@@ -169,6 +169,23 @@ int girhip_sssp_two_level(girhip_graph* h, const int32_t* src, int32_t k, uint64
  * bit for bit. */
 int girhip_bc(girhip_graph* h, const int32_t* src, int32_t k, double* score);
 
+/* Replace GIR::trackIncremental / updateIncremental / incDist
+ * (gir_query.cpp:867-996, gir.h:222-233): single-source BFS (weighted = 0)
+ * or SSSP (weighted = 1, the weights girhip_sssp uses) kept up to date
+ * across edge-store changes.  inc_track computes the result for source once
+ * and from then on every change the store takes (apply / ingest batches,
+ * single updates) is logged; inc_update corrects the result for the logged
+ * changes (deleted tree arcs invalidate the dependence subtree below them,
+ * which restarts from its valid in-neighbours; present logged arcs relax;
+ * then frontier Bellman-Ford); inc_dist writes V distances (BFS: levels),
+ * UINT64_MAX = unreachable, equal to a from-scratch query after every
+ * update.  Distances are packed in 32 bits with the parent, as in the
+ * reference: a larger one gives GIRHIP_EINVAL.  inc_update / inc_dist before
+ * inc_track give GIRHIP_ESTATE. */
+int girhip_inc_track(girhip_graph* h, int32_t source, int32_t weighted);
+int girhip_inc_update(girhip_graph* h, int32_t weighted);
+int girhip_inc_dist(girhip_graph* h, int32_t weighted, uint64_t* dist);
+
 /* Replaces GIR::sssp (gir_query.cpp:695-703) for k sources: delta-stepping
  * with bucket width delta (0 = the handle's tuned default, 16 unless
  * changed by girhip_set_param).  dist holds k*V uint64, UINT64_MAX =

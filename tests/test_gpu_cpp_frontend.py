@@ -19,7 +19,7 @@ def test_cpp_gir_selftest():
     assert r.returncode == 0, r.stderr[-4000:]
 
 
-@pytest.mark.parametrize("mode", ["", "lazy"])
+@pytest.mark.parametrize("mode", ["", "lazy", "inc", "lazy-inc"])
 def test_stream_caller_matches_oracle(tmp_path, mode):
     # the streaming caller (the reference's gir_stream.cpp:21-88 command line)
     # on the GPU: after each stream the layers and the last BFS / SSSP equal
@@ -44,9 +44,12 @@ def test_stream_caller_matches_oracle(tmp_path, mode):
     assert r.returncode == 0, r.stderr[-2000:]
     rows = [l.split(",") for l in open(csv).read().splitlines()]
     metrics = {x[2] for x in rows}
+    vals = {x[2]: float(x[4]) for x in rows}
     for s in ("insert", "delete", "mixed"):
         assert {f"{s}_mutation_MEPS", f"bfs_{s}_recompute_ms", f"sssp_{s}_endtoend_ms"} <= metrics
-        assert (f"sync_{s}_ms" in metrics) == (mode == "lazy")
+        assert (f"sync_{s}_ms" in metrics) == mode.startswith("lazy")
+        if mode.endswith("inc"):
+            assert vals[f"inc_mismatch_{s}"] == 0
     lo = P["loaded"]
     fb0, fa0, _, _ = girhip.csr_from_edges(V, lo[:, 0], lo[:, 1])
     src = int(P["sources"][0])

@@ -526,3 +526,86 @@ def test_bc_matches_oracle(kind):
     want = oracle.bc(fb, fa, bb, ba, src)
     assert np.allclose(got, want, rtol=1e-9, atol=1e-9), float(np.max(np.abs(got - want) / (1 + np.abs(want))))
     assert want.max() > 0
+
+
+def _sp_arcs(fb, fa, d, wfun):
+    """Arcs (u, v) on shortest paths (d[v] == d[u] + w(u, v)) - the arcs whose
+    deletion can invalidate a dependence subtree."""
+    V = len(fb) - 1
+    src = np.repeat(np.arange(V, dtype=np.int64), np.diff(fb))
+    ok = (d[src] != INF) & (d[fa] != INF)
+    w = wfun(src, fa)
+    on = ok & (d[fa].astype(np.float64) == d[src].astype(np.float64) + w)
+    return src[on].astype(np.int32), fa[on].astype(np.int32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("weighted", [0, 1])
+def test_incremental_matches_from_scratch(weighted):
+    # GIR::trackIncremental / updateIncremental / incDist
+    # (gir_query.cpp:867-996): after every update the corrected distances
+    # equal a from-scratch BFS / Dijkstra (the reference's own check,
+    # gir_stream.cpp:54-67); batches delete shortest-path arcs (dependence
+    # subtrees invalidated), insert random arcs, go through apply, ingest
+    # and single updates, and updates may pile up before a correction
+    V, fb, fa, bb, ba = girhip.rmat_graph(14, edge_factor=8, seed=91)
+    rng = np.random.default_rng(92)
+    x = int(girhip.sources(fb, 1, 93)[0])
+    wf = (lambda u, v: np.array([oracle.edge_weight(int(a), int(b)) for a, b in zip(u, v)], np.float64)) \
+        if weighted else (lambda u, v: np.ones(len(u)))
+    ref = (lambda f, a: oracle.dijkstra(f, a, None, x)) if weighted else \
+        (lambda f, a: np.where(oracle.bfs_textbook(f, a, x) < 0, INF, oracle.bfs_textbook(f, a, x)).astype(np.uint64))
+    say("incremental", "sssp" if weighted else "bfs", "rmat14")
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        with pytest.raises(girhip.GirError):
+            G.update_incremental(weighted)
+        G.track_incremental(x, weighted)
+        assert (G.inc_dist(weighted) == ref(fb, fa)).all()
+        f1, a1 = fb, fa
+        for r in range(6):
+            d = ref(f1, a1)
+            su, sv = _sp_arcs(f1, a1, d, wf)
+            pick = rng.choice(len(su), size=min(len(su), 300), replace=False)
+            u = np.concatenate([su[pick], rng.integers(0, V, 300).astype(np.int32)])
+            v = np.concatenate([sv[pick], rng.integers(0, V, 300).astype(np.int32)])
+            ins = np.concatenate([np.zeros(len(pick), np.uint8), np.ones(300, np.uint8)])
+            if r % 3 == 0:
+                G.apply_batch(u, v, ins)
+            elif r % 3 == 1:
+                G.ingest_batch(u, v, ins)
+            else:
+                for i in range(0, 40):
+                    G.update_edge_directed(int(u[i]), int(v[i]), bool(ins[i]))
+                u, v, ins = u[:40], v[:40], ins[:40]
+            f1, a1 = oracle.apply_batch(f1, a1, u, v, ins)[:2]
+            if r == 3:
+                continue  # two batches before one correction
+            G.update_incremental(weighted)
+            assert (G.inc_dist(weighted) == ref(f1, a1)).all(), r
+
+
+@pytest.mark.gpu
+def test_incremental_edge_cases():
+    # out-of-range source (everything unreachable, updates keep it so), the
+    # 32-bit packed distance limit, and both kinds tracked at once
+    V, fb, fa, bb, ba = girhip.rmat_graph(11, edge_factor=6, seed=94)
+    say("incremental edge cases")
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.track_incremental(-1, 0)
+        G.apply_batch([1, 2], [3, 4], [1, 1])
+        G.update_incremental(0)
+        assert (G.inc_dist(0) == INF).all()
+        x = int(girhip.sources(fb, 1, 95)[0])
+        G.track_incremental(x, 0)
+        G.track_incremental(x, 1)
+        G.apply_batch([x], [int(np.nonzero(oracle.bfs_textbook(fb, fa, x) < 0)[0][0])], [1])
+        f1, a1 = oracle.apply_batch(fb, fa, [x], [int(np.nonzero(oracle.bfs_textbook(fb, fa, x) < 0)[0][0])], [1])[:2]
+        G.update_incremental(0)
+        G.update_incremental(1)
+        lv = oracle.bfs_textbook(f1, a1, x)
+        assert (G.inc_dist(0) == np.where(lv < 0, INF, lv).astype(np.uint64)).all()
+        assert (G.inc_dist(1) == oracle.dijkstra(f1, a1, None, x)).all()
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.set_weights(np.full(len(fa), 2**31 - 1, np.uint32))
+        with pytest.raises(girhip.GirError):
+            G.track_incremental(x, 1)  # distances beyond 32 bits

@@ -55,5 +55,5 @@ def test_stream_caller_command_line():
     assert os.path.exists(exe), "build the package first"
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 2 and "usage" in r.stderr
-    r = subprocess.run([exe, "/nonexistent", "x", "/dev/null", "100", "1", "inc"], capture_output=True, text=True)
-    assert r.returncode == 2 and "incremental" in r.stderr
+    r = subprocess.run([exe, "/nonexistent", "x", "/dev/null", "100", "1", "eager"], capture_output=True, text=True)
+    assert r.returncode == 2 and "unknown mode" in r.stderr
