@@ -137,10 +137,10 @@ struct Scratch {
     DevBuf<int32_t> bc_lev, bc_ord, bc_lv;
     DevBuf<uint64_t> bc_sigma, bc_delta;
     DevBuf<int64_t> bc_ctl;
-    // incremental queries: from-scratch distances / levels, sorted and unique
-    // log keys, presence, invalid list, round queues, flags (zero between
-    // updates), counters
-    DevBuf<uint64_t> inc_d, inc_keys, inc_u;
+    // incremental queries: from-scratch distances / levels, presence of the
+    // logged arcs, invalid list, round queues, flags (zero between updates),
+    // counters
+    DevBuf<uint64_t> inc_d;
     DevBuf<int32_t> inc_lv, inc_list, inc_q[2], inc_flag;
     DevBuf<uint8_t> inc_pres;
     DevBuf<int64_t> inc_ctl;

@@ -6,9 +6,10 @@
 // girhip_sssp uses): st[v] = (distance << 32) | parent, ~0 = unreachable,
 // the parent being an in-neighbour on a shortest path.  Every change the
 // edge store takes while a kind is tracked is appended to its log.  An
-// update:
-//   1. sorts and dedups the log; an arc absent now whose head's parent is
-//      its tail (a deleted tree arc) invalidates the head;
+// update (repeated log entries are harmless: invalidation and queueing go
+// through flag CAS, relaxation through atomicMin):
+//   1. an arc of the log absent now whose head's parent is its tail (a
+//      deleted tree arc) invalidates the head;
 //   2. invalidation spreads down the dependence tree (children: out-arcs
 //      whose head names the vertex as its parent), level by level;
 //   3. every invalid vertex restarts from its best valid in-neighbour;
@@ -34,7 +35,7 @@ constexpr int INC_CHUNK = 8;   // rounds / levels per host readback
 // control slots: [0, T_SLOTS) the invalidation list, then the round ring
 constexpr int C_RING = 16;
 constexpr int C_OVF = C_RING + INC_RING;
-constexpr int C_NU = C_OVF + 1;  // unique log keys
+constexpr int C_NU = C_OVF + 1;  // log length (device copy)
 constexpr int C_TOTAL = C_NU + 16;
 
 __device__ __forceinline__ int32_t parentOf(uint64_t k) { return (int32_t)(uint32_t)(k & 0xffffffffULL); }
@@ -102,17 +103,6 @@ __global__ void kIncParents(IncG g, const int32_t* __restrict__ bb, const int32_
             }
         }
         st[v] = pack(d, par);
-    }
-}
-
-// Sorted log -> unique keys (append order does not matter afterwards).
-__global__ void kIncUnique(const uint64_t* __restrict__ keys, int64_t n, uint64_t* __restrict__ out, int64_t* cnt) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
-        const int64_t i = base + laneId();
-        const bool keep = i < n && (i == 0 || keys[i] != keys[i - 1]);
-        const int64_t pos = wavePush(cnt, keep);
-        if (keep) out[pos] = keys[i];
     }
 }
 
@@ -379,8 +369,6 @@ extern "C" int girhip_inc_update(girhip_graph* h, int32_t weighted) {
         const int B = bitsFor((uint64_t)(V > 1 ? V - 1 : 1));
         Scratch& S = scratch(h);
         const int64_t n = T.nlog;
-        S.inc_keys.ensure((size_t)n);
-        S.inc_u.ensure((size_t)n);
         S.inc_pres.ensure((size_t)n);
         S.inc_list.ensure((size_t)std::max<int32_t>(V, 1));
         for (int p = 0; p < 2; ++p) S.inc_q[p].ensure((size_t)std::max<int32_t>(V, 1));
@@ -392,21 +380,21 @@ extern "C" int girhip_inc_update(girhip_graph* h, int32_t weighted) {
         int64_t* ctl = S.inc_ctl.p;
         tBegin(h);
         fillI64(h, ctl, 0, C_TOTAL);
-        // 0. sort + dedup the log
-        launch(h, "inc_copy", kIncAppend, boundedGrid(n, 256), 256, 0, (const uint64_t*)T.log.p, n, S.inc_keys.p);
-        radixSort(h, S.inc_keys.p, nullptr, n, 2 * B);
-        launch(h, "inc_unique", kIncUnique, boundedGrid(n, blk), blk, 0, (const uint64_t*)S.inc_keys.p, n, S.inc_u.p,
-               ctl + C_NU);
+        fillI64(h, ctl + C_NU, n, 1);
+        const uint64_t* keys = T.log.p;
         const IncG g = incGraph(h, weighted);
         const unsigned ng = boundedGrid(n, blk);
         // 1. deleted tree arcs
-        launch(h, "inc_deleted", kIncDeleted, ng, blk, 0, g, (const uint64_t*)S.inc_u.p, (const int64_t*)(ctl + C_NU),
-               B, T.source, (const uint64_t*)T.st.p, S.inc_flag.p, S.inc_pres.p, S.inc_list.p, ctl);
+        launch(h, "inc_deleted", kIncDeleted, ng, blk, 0, g, keys, (const int64_t*)(ctl + C_NU), B, T.source,
+               (const uint64_t*)T.st.p, S.inc_flag.p, S.inc_pres.p, S.inc_list.p, ctl);
         launch(h, "inc_advance", kListAdvance, 1, 64, 0, ctl);
-        // 2. spread down the dependence tree
+        // 2. spread down the dependence tree (nothing to do without a
+        // deleted tree arc)
         const IncSpreadF sf{h->fw_adj.p, T.st.p, S.inc_flag.p, S.inc_list.p, ctl + T_TAIL, T.source};
         int64_t r[T_SLOTS];
-        while (true) {
+        readCounters(h, ctl, T_SLOTS, r);
+        const bool any_invalid = r[T_TAIL] > 0;
+        while (any_invalid) {
             for (int t = 0; t < INC_CHUNK; ++t) {
                 launch(h, "inc_spread", kListRows<IncSpreadF>, 1024, blk, 0, (const int32_t*)h->fw_beg.p,
                        (const int32_t*)S.inc_list.p, (const int64_t*)ctl, sf, heavyList(h));
@@ -417,20 +405,21 @@ extern "C" int girhip_inc_update(girhip_graph* h, int32_t weighted) {
             if (r[T_LO] == r[T_HI]) break;
         }
         // 3. restart the invalid vertices from valid in-neighbours
-        launch(h, "inc_clear", kIncClear, 1024, 256, 0, (const int32_t*)S.inc_list.p, (const int64_t*)ctl, T.st.p);
-        launch(h, "inc_all", kListAll, 1, 64, 0, ctl);
-        const IncRestartF rf{g, h->bw_adj.p, S.inc_flag.p, T.st.p, ctl + C_OVF};
-        launch(h, "inc_restart", kListRows<IncRestartF>, 1024, blk, 0, (const int32_t*)h->bw_beg.p,
-               (const int32_t*)S.inc_list.p, (const int64_t*)ctl, rf, heavyList(h));
-        launchHeavyStore(h, "inc_restart_rows", rf);
+        if (any_invalid) {
+            launch(h, "inc_clear", kIncClear, 1024, 256, 0, (const int32_t*)S.inc_list.p, (const int64_t*)ctl, T.st.p);
+            launch(h, "inc_all", kListAll, 1, 64, 0, ctl);
+            const IncRestartF rf{g, h->bw_adj.p, S.inc_flag.p, T.st.p, ctl + C_OVF};
+            launch(h, "inc_restart", kListRows<IncRestartF>, 1024, blk, 0, (const int32_t*)h->bw_beg.p,
+                   (const int32_t*)S.inc_list.p, (const int64_t*)ctl, rf, heavyList(h));
+            launchHeavyStore(h, "inc_restart_rows", rf);
+        }
         // 4. seeds into the round-0 queue (its length in ring slot INC_RING - 1)
         int64_t* ring = ctl + C_RING;
         auto slot = [&](int64_t rr) { return ring + ((rr % INC_RING) + INC_RING) % INC_RING; };
         launch(h, "inc_reseed", kIncReseed, 1024, blk, 0, (const int32_t*)S.inc_list.p, (const int64_t*)ctl,
                (const uint64_t*)T.st.p, S.inc_flag.p, S.inc_q[0].p, slot(-1));
-        launch(h, "inc_inserted", kIncInserted, ng, blk, 0, g, (const uint64_t*)S.inc_u.p,
-               (const int64_t*)(ctl + C_NU), B, (const uint8_t*)S.inc_pres.p, T.st.p, S.inc_flag.p, S.inc_q[0].p,
-               slot(-1), ctl + C_OVF);
+        launch(h, "inc_inserted", kIncInserted, ng, blk, 0, g, keys, (const int64_t*)(ctl + C_NU), B,
+               (const uint8_t*)S.inc_pres.p, T.st.p, S.inc_flag.p, S.inc_q[0].p, slot(-1), ctl + C_OVF);
         // 5. rounds of relaxation
         int64_t rr = 0;
         std::vector<int64_t> rv(C_TOTAL);

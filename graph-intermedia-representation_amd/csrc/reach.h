@@ -91,6 +91,8 @@ __global__ void kTopDown(const int32_t* __restrict__ qin, const int64_t* nin, To
     blockAtomicAdd(m, &f.cnt[1]);
 }
 
+constexpr int32_t BU_SMALL = 8;
+
 template <class Flt, class Vis>
 __global__ void kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __restrict__ radj,
                           const int32_t* __restrict__ pbeg, uint64_t* vis, const uint64_t* __restrict__ fbits,
@@ -103,15 +105,38 @@ __global__ void kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __res
     const uint64_t w0 = word_ok ? vis[word] : ~0ULL;
     const bool act = v < V && !((w0 >> laneId()) & 1) && flt.self((int32_t)v);
     bool found = false;
-    if (act) {
-        const int32_t b = rbeg[v], e = rbeg[v + 1];
-        for (int32_t j = b; j < e; ++j) {
-            const int32_t p = radj[j];
-            if (((fbits[p >> 6] >> (p & 63)) & 1) && flt((int32_t)v, p)) {
-                found = true;
+    // each lane scans the first BU_SMALL slots of its own row; rows with
+    // slots left and no parent yet are then scanned by the whole wave, one
+    // row at a time, 64 coalesced slots per step with a ballot exit (a hub's
+    // long row no longer holds the other 63 lanes)
+    const int32_t b = act ? rbeg[v] : 0, e = act ? rbeg[v + 1] : 0;
+    const int32_t lim = e - b > BU_SMALL ? b + BU_SMALL : e;
+    for (int32_t j = b; j < lim; ++j) {
+        const int32_t p = radj[j];
+        if (((fbits[p >> 6] >> (p & 63)) & 1) && flt((int32_t)v, p)) {
+            found = true;
+            break;
+        }
+    }
+    uint64_t rest = __ballot(act && !found && lim < e);
+    while (rest) {
+        const int l = __ffsll((long long)rest) - 1;
+        rest &= rest - 1;
+        const int32_t rv = __shfl((int32_t)v, l), rb = __shfl(lim, l), re = __shfl(e, l);
+        bool hit = false;
+        for (int32_t j0 = rb; j0 < re; j0 += WAVE) {
+            const int32_t j = j0 + laneId();
+            bool hj = false;
+            if (j < re) {
+                const int32_t p = radj[j];
+                hj = ((fbits[p >> 6] >> (p & 63)) & 1) && flt(rv, p);
+            }
+            if (__ballot(hj)) {
+                hit = true;
                 break;
             }
         }
+        if (laneId() == l && hit) found = true;
     }
     const uint64_t nb = __ballot(found);
     if (laneId() == 0 && word_ok) {
