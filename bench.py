@@ -54,8 +54,27 @@ def make_graph(cfg):
     return girhip.grid_graph(cfg["rows"], cfg["cols"], cfg["p_keep"], cfg["p_bidir"], cfg["seed"])
 
 
+def block_partition(bid, v):
+    """A block partition given as (block, vertex) membership pairs, in a
+    canonical form: each block named by its two smallest vertices (two
+    blocks share at most one vertex), pairs sorted."""
+    bid = np.asarray(bid, np.int64)
+    v = np.asarray(v, np.int64)
+    o = np.lexsort((v, bid))
+    b, x = bid[o], v[o]
+    keep = np.r_[True, (np.diff(b) != 0) | (np.diff(x) != 0)]
+    b, x = b[keep], x[keep]
+    starts = np.r_[0, np.nonzero(np.diff(b))[0] + 1]
+    ends = np.r_[starts[1:], len(b)]
+    second = np.where(ends - starts > 1, x[np.minimum(starts + 1, len(x) - 1)], -1)
+    key = (x[starts] << 32) | (second & 0xffffffff)
+    lab = np.repeat(key, ends - starts)
+    o = np.lexsort((x, lab))
+    return lab[o], x[o]
+
+
 def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=None, reach_pairs=None,
-                 two_level=False, bc_src=None):
+                 two_level=False, bc_src=None, bicc=False):
     """Time the compiled reference (oracle/_ref/gir_ref: the reference's own
     gir.cpp / gir_query.cpp) on this host: one buildWCC + buildSCC_DAG, BFS
     from n_bfs and SSSP (its hash weights, delta 16) from n_sssp sources, and
@@ -83,6 +102,8 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
         if bc_src is not None:
             np.ascontiguousarray(bc_src, np.int32).tofile(os.path.join(td, "bc_src.bin"))
             cmd += ["--bc", os.path.join(td, "bc_src.bin")]
+        if bicc:
+            cmd += ["--bicc"]
         env = dict(os.environ)
         env.setdefault("OMP_PROC_BIND", "close")
         t0 = time.time()
@@ -94,6 +115,9 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
         bc_scores = None
         if bc_src is not None and os.path.exists(os.path.join(td, "bc.bin")):
             bc_scores = np.fromfile(os.path.join(td, "bc.bin"), np.float64)
+        bicc_pairs = None
+        if bicc and os.path.exists(os.path.join(td, "bicc.bin")):
+            bicc_pairs = np.fromfile(os.path.join(td, "bicc.bin"), np.int32).reshape(-1, 2)
     rec = {}
     for line in out.splitlines():
         if line.startswith("@@"):
@@ -106,6 +130,8 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
         rec["reach"]["answers"] = reach_ans
     if bc_scores is not None and "bc" in rec:
         rec["bc"]["scores"] = bc_scores
+    if bicc_pairs is not None and "bicc" in rec:
+        rec["bicc"]["pairs"] = bicc_pairs
     return rec, wall
 
 
@@ -121,10 +147,11 @@ def stream_runs(fb, fa, B=10000, NB=5, cpu=True):
     with tempfile.TemporaryDirectory() as td:
         d = os.path.join(td, "proto")
         girhip.export_protocol(d, fb, fa, B * NB, n_sources=1, seed=2027, full=False)
-        runs = [("gfx950", os.path.join(PKG, "gir_stream"), ""), ("gfx950", os.path.join(PKG, "gir_stream"), "lazy")]
+        mine = os.path.join(PKG, "gir_stream")
+        runs = [("gfx950", mine, ""), ("gfx950", mine, "lazy"), ("gfx950", mine, "inc")]
         ref = os.path.join(ROOT, "oracle", "_ref", "gir_stream")
         if cpu and os.path.exists(ref):
-            runs += [("reference", ref, ""), ("reference", ref, "lazy")]
+            runs += [("reference", ref, ""), ("reference", ref, "lazy"), ("reference", ref, "inc")]
         for who, exe, mode in runs:
             csv = os.path.join(td, f"{who}_{mode or 'fmu'}.csv")
             env = dict(os.environ)
@@ -375,6 +402,24 @@ def main():
         bcr = {"sources": len(bsrc), "ms": round(b_ms, 3), "ms_per_source": round(b_ms / len(bsrc), 3)}
         log(f"bc: {bcr}")
 
+    # ---- biconnected blocks of every SCC (buildBCT's block layer) ----
+    bic = None
+    bic_part = None
+    if not args.no_extras:
+        G.scc_dag(out=scc)
+        G.bicc()  # warm
+        r = G.bicc()
+        bic = {"ms": round(G.last_ms("bicc"), 3), "blocks": int(r["n_blocks"]),
+               "articulation_points": int(r["ap"].sum())}
+        par, blk = r["parent"], r["block"]
+        nonroot = np.nonzero(par >= 0)[0]
+        has_child = np.zeros(V, bool)
+        has_child[par[nonroot]] = True
+        lone = np.nonzero((par < 0) & ~has_child)[0]
+        bic_part = block_partition(np.r_[blk[nonroot], blk[nonroot], -1 - lone],
+                                   np.r_[nonroot, par[nonroot], lone])
+        log(f"bicc: {bic}")
+
     # ---- streaming protocol (gir_stream: batches of 10^4, BFS + SSSP after each) ----
     stream = None
     if not args.no_extras:
@@ -429,7 +474,8 @@ def main():
         log("cpu baseline (compiled reference) ...")
         r = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, batch=first_batch if updates else None,
                          reach_pairs=reach_pairs[:4096] if reach_pairs is not None else None,
-                         two_level=two is not None, bc_src=bfs_src[:16] if bcr else None)
+                         two_level=two is not None, bc_src=bfs_src[:16] if bcr else None,
+                         bicc=bic is not None)
         if r is not None:
             rec, wall = r
             b = rec.get("build", {})
@@ -439,7 +485,11 @@ def main():
                              f"on the same graph, BFS from {rec['bfs']['sources']} and SSSP (reference hash "
                              f"weights, delta 16) from {rec['sssp']['sources']} of the same sources"
                              + ("; bfsTwoLevel / ssspTwoLevel from the same sources after an untimed buildBCT"
-                                if "bfs2" in rec else "") + f"; {wall:.1f} s wall",
+                                if "bfs2" in rec else "")
+                             + ("; buildBCT (blocks of every SCC)" if "bicc" in rec else "")
+                             + (f"; bc from {rec['bc']['sources']} sources" if "bc" in rec else "")
+                             + ("; reachable on 4,096 pairs" if "reach" in rec else "")
+                             + ("; applyBatch of 10^3 ops" if "batch" in rec else "") + f"; {wall:.1f} s wall",
                    "wcc_ms": b.get("wcc_ms"), "scc_dag_ms": b.get("scc_dag_ms"),
                    "bfs_ms_per_source": rec["bfs"]["ms"] / rec["bfs"]["sources"],
                    "sssp_ms_per_source": rec["sssp"]["ms"] / rec["sssp"]["sources"]}
@@ -448,6 +498,14 @@ def main():
             if "batch" in rec:
                 cpu["batch_ops"] = rec["batch"]["ops"]
                 cpu["batch_ms"] = rec["batch"]["ms"]
+            if "bicc" in rec and bic:
+                cpu["bicc_ms"] = rec["bicc"]["ms"]
+                bic["speedup_vs_cpu"] = round(rec["bicc"]["ms"] / bic["ms"], 2)
+                if "pairs" in rec["bicc"]:
+                    rp = rec["bicc"]["pairs"]
+                    want = block_partition(rp[:, 0], rp[:, 1])
+                    bic["matches_reference"] = bool(np.array_equal(want[0], bic_part[0])
+                                                    and np.array_equal(want[1], bic_part[1]))
             if "bc" in rec and bcr:
                 cpu["bc_ms_per_source"] = rec["bc"]["ms"] / rec["bc"]["sources"]
                 if "scores" in rec["bc"]:
@@ -501,12 +559,17 @@ def main():
         two["speedup_sssp_vs_cpu"] = round(cpu["sssp2_ms_per_source"] / two["sssp_ms_per_source"], 2)
     out["two_level"] = two
     if stream and "reference_fmu" in stream:
-        g, c = stream["gfx950_fmu"], stream["reference_fmu"]
-        stream["speedup_vs_cpu"] = {
-            k: round(c[k] / g[k], 2) if k.endswith("_ms") else round(g[k] / c[k], 2)
-            for k in g if k in c and k not in ("threads", "wall_s") and g[k] and c[k]}
+        for m in ("fmu", "lazy", "inc"):
+            g, c = stream.get(f"gfx950_{m}"), stream.get(f"reference_{m}")
+            if not g or not c:
+                continue
+            stream[f"speedup_vs_cpu_{m}"] = {
+                k: round(c[k] / g[k], 2) if k.endswith("_ms") else round(g[k] / c[k], 2)
+                for k in g if k in c and k not in ("threads", "wall_s") and not k.startswith("inc_mismatch")
+                and g[k] and c[k]}
     out["stream"] = stream
     out["bc"] = bcr
+    out["bicc"] = bic
     out["learned_vs_default"] = learned
     out["reachable"] = reach
     out["query_batch"] = queries

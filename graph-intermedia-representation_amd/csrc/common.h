@@ -144,6 +144,13 @@ struct Scratch {
     DevBuf<int32_t> inc_lv, inc_list, inc_q[2], inc_flag;
     DevBuf<uint8_t> inc_pres;
     DevBuf<int64_t> inc_ctl;
+    // biconnected blocks: BFS forest (levels, parents), vertices by level,
+    // children by parent, subtree sizes, preorder, low / high, union-find,
+    // block ids, articulation flags, scans, counters
+    DevBuf<int32_t> bi_lev, bi_par, bi_order, bi_lbeg, bi_cadj, bi_cbeg, bi_cpos, bi_sz, bi_pre, bi_low, bi_high,
+        bi_uf, bi_blk, bi_tmp;
+    DevBuf<uint8_t> bi_ap;
+    DevBuf<int64_t> bi_scan, bi_cscan, bi_ctl;
     // heavy-row list
     DevBuf<int64_t> hv_ctl, hv_b, hv_e, hv_off;
     DevBuf<int32_t> hv_row, hv_aux;
@@ -847,10 +854,11 @@ int64_t maxDegree(girhip_graph* h);
 // Largest arc weight SSSP uses: the stored weights' maximum (cached until
 // they change), or the hash weights' bound 100.
 int64_t maxWeight(girhip_graph* h);
-// launchHeavy for row kernels over the edge store's CSR: skipped when no
-// row of the store can reach the heavy list.
+// launchHeavy for row kernels over the edge store's CSR or its symmetric
+// closure: skipped when no such row can reach the heavy list (a closure row
+// holds at most a vertex's out- plus in-arcs).
 template <class F>
 void launchHeavyStore(girhip_graph* h, const char* name, const F& f, int64_t* sum = nullptr) {
-    if (maxDegree(h) > HEAVY_T) launchHeavy(h, name, f, sum);
+    if (2 * maxDegree(h) > HEAVY_T) launchHeavy(h, name, f, sum);
 }
 }  // namespace gir

@@ -261,7 +261,7 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
     int c_prev = 1;
     int cur = 0;
     // persistent levels need every row below the heavy-list threshold
-    bool persist_ok = h->prm.persist != 0 && maxDegree(h) <= HEAVY_T;
+    bool persist_ok = h->prm.persist != 0 && 2 * maxDegree(h) <= HEAVY_T;  // (closure rows too)
     int32_t L = 0;
     st.reached = n0;
     while (nf > 0) {

@@ -66,6 +66,7 @@ _SIGS = {
     "girhip_bfs_two_level": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp]),
     "girhip_sssp_two_level": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_uint64, _vp]),
     "girhip_bc": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp]),
+    "girhip_bicc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "girhip_inc_track": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     "girhip_inc_update": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "girhip_inc_dist": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
@@ -482,6 +483,18 @@ class Graph:
         _check(lib().girhip_bc(self._h, src.ctypes.data if k else None, k, ptr))
         return sc[:self.V] if out is None else out
 
+    def bicc(self):
+        """Blocks of every SCC's undirected view (GIR::buildBCT /
+        getBCTComponents / articulationPoint, gir.cpp:2063-2110, 4140-4149)
+        -> dict: parent, block (int32 per vertex, see girhip_bicc), ap (bool
+        per vertex), n_blocks."""
+        V = max(self.V, 1)
+        par, blk = np.empty(V, np.int32), np.empty(V, np.int32)
+        ap = np.empty(V, np.uint8)
+        nb = ctypes.c_int64()
+        _check(lib().girhip_bicc(self._h, par.ctypes.data, blk.ctypes.data, ap.ctypes.data, ctypes.byref(nb)))
+        return {"parent": par[:self.V], "block": blk[:self.V], "ap": ap[:self.V].astype(bool), "n_blocks": nb.value}
+
     def track_incremental(self, source, weighted):
         """GIR::trackIncremental (gir_query.cpp:867-891)."""
         _check(lib().girhip_inc_track(self._h, int(source), int(bool(weighted))))
@@ -650,6 +663,26 @@ def write_csr(dirpath, fb, fa, bb, ba):
     os.makedirs(dirpath, exist_ok=True)
     _gcheck(gen().girgen_write_csr(dirpath.encode(), len(fb) - 1, len(fa), fb.ctypes.data, fa.ctypes.data,
                                    bb.ctypes.data, ba.ctypes.data))
+
+
+def bicc_blocks(parent, block):
+    """Blocks as vertex sets from girhip_bicc's forest arrays: { v, parent[v]
+    : block[v] = b } per block, plus every root without children (a
+    single-vertex SCC) on its own -> sorted list of sorted tuples."""
+    parent = np.asarray(parent)
+    block = np.asarray(block)
+    nonroot = parent >= 0
+    v = np.nonzero(nonroot)[0]
+    groups = {}
+    for b, x, p in zip(block[v].tolist(), v.tolist(), parent[v].tolist()):
+        g = groups.setdefault(b, set())
+        g.add(x)
+        g.add(p)
+    has_child = np.zeros(len(parent), bool)
+    has_child[parent[nonroot]] = True
+    out = [tuple(sorted(g)) for g in groups.values()]
+    out += [(int(x),) for x in np.nonzero(~nonroot & ~has_child)[0]]
+    return sorted(out)
 
 
 def export_protocol(dirpath, fb, fa, k_ops, n_sources=8, seed=2027, full=True):

@@ -140,13 +140,13 @@ GIR::~GIR() { girhip_destroy(h); }
 void GIR::buildWCC() {
     check(girhip_wcc(h, nullptr, nullptr), "buildWCC");
     built_wcc = true;
-    stale = true;
+    stale = true, ++epoch;
 }
 
 void GIR::buildSCC_DAG() {
     check(girhip_scc_dag(h, nullptr, nullptr, nullptr), "buildSCC_DAG");
     built_scc = true;
-    stale = true;
+    stale = true, ++epoch;
 }
 
 void GIR::refresh() const {
@@ -179,7 +179,7 @@ void GIR::applyOps(const std::vector<Update>& batch, bool repair) {
     check(repair ? girhip_apply_batch(h, u.data(), v.data(), ins.data(), nullptr, (int64_t)n, &st)
                  : girhip_ingest_batch(h, u.data(), v.data(), ins.data(), nullptr, (int64_t)n, &st),
           repair ? "applyBatch" : "ingestBatch");
-    stale = true;
+    stale = true, ++epoch;
 }
 
 void GIR::insertEdge(int u, int v) { applyOps({{u, v, true}}, true); }
@@ -190,7 +190,7 @@ void GIR::ingestBatch(const std::vector<Update>& batch) { applyOps(batch, false)
 
 void GIR::sync() {
     check(girhip_sync(h), "sync");
-    stale = true;
+    stale = true, ++epoch;
 }
 
 bool GIR::synced() const {
@@ -330,7 +330,7 @@ QueryVec<int> GIR::bfsTwoLevel(int source) {
     QueryVec<int> lv((size_t)V);
     const int32_t s = source;
     check(girhip_bfs_two_level(h, &s, 1, lv.data()), "bfsTwoLevel");
-    stale = true;  // a pending batch may have been synced
+    stale = true, ++epoch;  // a pending batch may have been synced
     return lv;
 }
 
@@ -339,7 +339,7 @@ QueryVec<uint64_t> GIR::ssspTwoLevel(int source, uint64_t delta) {
     QueryVec<uint64_t> d((size_t)V);
     const int32_t s = source;
     check(girhip_sssp_two_level(h, &s, 1, delta, d.data()), "ssspTwoLevel");
-    stale = true;
+    stale = true, ++epoch;
     return d;
 }
 
@@ -347,7 +347,7 @@ std::vector<char> GIR::reachableSccsPublic(int c) {
     if (!built_scc) buildSCC_DAG();
     std::vector<uint8_t> m((size_t)V);
     check(girhip_reach_info(h, c, m.data(), nullptr, nullptr, nullptr, nullptr), "reachableSccs");
-    stale = true;
+    stale = true, ++epoch;
     return std::vector<char>(m.begin(), m.end());
 }
 
@@ -368,6 +368,84 @@ uint64_t GIR::incDist(bool weighted, int v) const {
         inc_fresh[weighted] = true;
     }
     return (v >= 0 && v < V) ? inc_host[weighted][(size_t)v] : UINT64_MAX;
+}
+
+void GIR::ensureBlocks() {
+    if (!built_scc) buildSCC_DAG();
+    if (!synced()) sync();
+    if (blk_ok && blk_epoch == epoch) return;
+    bc_parent.resize((size_t)V);
+    bc_block.resize((size_t)V);
+    std::vector<uint8_t> ap((size_t)V);
+    int64_t nb = 0;
+    check(girhip_bicc(h, bc_parent.data(), bc_block.data(), ap.data(), &nb), "buildBCT");
+    bc_ap.assign(ap.begin(), ap.end());
+    // children of every forest vertex (counting sort by parent)
+    bc_cbeg.assign((size_t)V + 1, 0);
+    for (int v = 0; v < V; ++v)
+        if (bc_parent[v] >= 0) ++bc_cbeg[bc_parent[v] + 1];
+    for (int v = 0; v < V; ++v) bc_cbeg[v + 1] += bc_cbeg[v];
+    bc_cadj.resize((size_t)bc_cbeg[V]);
+    std::vector<int> fill(bc_cbeg.begin(), bc_cbeg.end() - 1);
+    for (int v = 0; v < V; ++v)
+        if (bc_parent[v] >= 0) bc_cadj[fill[bc_parent[v]]++] = v;
+    n_blocks = nb;
+    blk_ok = true;
+    blk_epoch = epoch;
+}
+
+std::vector<int> GIR::blocksOf(int x) const {
+    std::vector<int> b;
+    if (bc_block[x] >= 0) b.push_back(bc_block[x]);
+    for (int i = bc_cbeg[x]; i < bc_cbeg[x + 1]; ++i) b.push_back(bc_block[bc_cadj[i]]);
+    std::sort(b.begin(), b.end());
+    b.erase(std::unique(b.begin(), b.end()), b.end());
+    return b;
+}
+
+bool GIR::articulationPoint(int v) {
+    ensureBlocks();
+    return v >= 0 && v < V && bc_ap[v] != 0;
+}
+
+bool GIR::biconnected(int u, int v) {
+    if (u == v) return true;
+    ensureBlocks();
+    if (u < 0 || v < 0 || u >= V || v >= V) return false;
+    const auto a = blocksOf(u), b = blocksOf(v);
+    std::vector<int> c;
+    std::set_intersection(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(c));
+    return !c.empty();
+}
+
+std::vector<std::vector<int>> GIR::getBCTComponents(int scc_id) {
+    std::vector<std::vector<int>> r;
+    ensureBlocks();
+    refresh();
+    if (scc_id < 0 || scc_id >= V || node_scc[scc_id] != scc_id) return r;
+    const auto& comps = getSCCs();
+    const std::vector<int>* mem = nullptr;
+    for (const auto& c : comps)
+        if (!c.empty() && node_scc[c[0]] == scc_id) {
+            mem = &c;
+            break;
+        }
+    if (!mem) return r;
+    if (mem->size() == 1) return {{(*mem)[0]}};
+    std::map<int, std::vector<int>> blocks;
+    for (int x : *mem)
+        if (bc_parent[x] >= 0) {
+            auto& bl = blocks[bc_block[x]];
+            bl.push_back(x);
+            bl.push_back(bc_parent[x]);
+        }
+    for (auto& kv : blocks) {
+        auto& bl = kv.second;
+        std::sort(bl.begin(), bl.end());
+        bl.erase(std::unique(bl.begin(), bl.end()), bl.end());
+        r.push_back(std::move(bl));
+    }
+    return r;
 }
 
 QueryVec<double> GIR::bc(const std::vector<int>& sources) {

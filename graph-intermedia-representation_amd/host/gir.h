@@ -71,7 +71,10 @@ public:
     // Construction (gir.h:98-101).
     void buildWCC();
     void buildSCC_DAG();
-    void buildBCT() {}  // the block layer is not part of this path
+    // Biconnected blocks of every SCC's undirected view (gir.cpp:2063,
+    // computed from scratch on the GPU when asked for after a change).
+    void buildBCT() { ensureBlocks(); }
+    void ensureBlocks();
 
     // Updates (gir.h:104-119).
     void insertEdge(int u, int v);
@@ -140,8 +143,15 @@ public:
 
     // Knobs (gir.h:170-240) that exist on this path.
     void setQueryGranularity(size_t par_frontier, size_t large_unit);
-    void setBlocksOnDemand(bool) {}
+    void setBlocksOnDemand(bool) {}  // blocks are always built on demand here
     bool blocksOnDemand() const { return true; }
+    // Block queries (gir.h:133-148): v lies in two or more blocks; u and v
+    // share a block; the blocks (vertex sets) of the SCC with canonical id
+    // scc_id.
+    bool articulationPoint(int v);
+    bool biconnected(int u, int v);
+    std::vector<std::vector<int>> getBCTComponents(int scc_id);
+    long long blockCount() { ensureBlocks(); return n_blocks; }
     void setParam(const char* name, double value);
 
     girhip_graph* handle() const { return h; }
@@ -169,6 +179,15 @@ private:
     mutable std::vector<std::vector<int>> sccs;
     mutable bool sccs_ok = false;
     mutable std::unique_ptr<graph> g_copy;
+    mutable uint64_t epoch = 0;  // bumped whenever the store or a layer changes
+    // block layer (host copy): forest parent, block of each tree edge,
+    // articulation flags, children CSR
+    std::vector<int> bc_parent, bc_block, bc_cbeg, bc_cadj;
+    std::vector<char> bc_ap;
+    long long n_blocks = 0;
+    bool blk_ok = false;
+    uint64_t blk_epoch = 0;
+    std::vector<int> blocksOf(int x) const;
     mutable std::vector<uint64_t> inc_host[2];  // incDist values, refreshed after a track / update
     mutable bool inc_fresh[2] = {false, false};
 };

@@ -169,6 +169,21 @@ int girhip_sssp_two_level(girhip_graph* h, const int32_t* src, int32_t k, uint64
  * bit for bit. */
 int girhip_bc(girhip_graph* h, const int32_t* src, int32_t k, double* score);
 
+/* Replaces the block layer of GIR::buildBCT / getBCTComponents /
+ * articulationPoint / biconnected (gir.cpp:2063-2110, 4140-4149,
+ * gir.h:133-150): the biconnected blocks of every SCC's undirected view
+ * (its vertices, an edge {u, w} wherever u -> w or w -> u, as
+ * gir_test.cpp:448-465 derives it), computed from scratch by Tarjan-Vishkin
+ * over a BFS spanning forest rooted at the canonical SCC ids.  Outputs (V
+ * entries each, any may be NULL): parent[v] = v's parent in that forest (-1
+ * at a root); block[v] = the block holding the tree edge (v, parent[v])
+ * (named by the smallest child vertex among its tree edges; -1 at a root);
+ * a block's vertices are { v, parent[v] : block[v] = b }, and a vertex of a
+ * single-vertex SCC is a block of its own; ap[v] = 1 if v lies in two or
+ * more blocks; n_blocks counts all blocks.  Pending ingested changes are
+ * synced first; needs girhip_scc_dag (GIRHIP_ESTATE). */
+int girhip_bicc(girhip_graph* h, int32_t* parent, int32_t* block, uint8_t* ap, int64_t* n_blocks);
+
 /* Replace GIR::trackIncremental / updateIncremental / incDist
  * (gir_query.cpp:867-996, gir.h:222-233): single-source BFS (weighted = 0)
  * or SSSP (weighted = 1, the weights girhip_sssp uses) kept up to date

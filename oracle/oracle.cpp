@@ -663,6 +663,111 @@ extern "C" int orc_bc(int32_t V, const int32_t* fb, const int32_t* fc, const int
     return 0;
 }
 
+// Biconnected blocks of every SCC's undirected view: the view of
+// gir_test.cpp:448-465 (out- and in-neighbours inside the SCC, no self
+// loops, no repeats) and the iterative Hopcroft-Tarjan of
+// dynamic_repair.h:96-170 (discovery times, low points, an edge stack popped
+// into a block when low[child] >= disc[parent]); vertices no block covers
+// are blocks of their own (gir_test.cpp:462-464).  Output: membership pairs
+// (block index, vertex), blocks numbered in emission order; cap = room for
+// pairs.
+extern "C" int orc_bicc(int32_t V, const int32_t* fb, const int32_t* fc, const int32_t* bb, const int32_t* bc,
+                        const int32_t* scc, int32_t* mem_block, int32_t* mem_vertex, int64_t cap, int64_t* n_mem,
+                        int64_t* n_blocks) {
+    if (V < 0 || !n_mem || !n_blocks || (V > 0 && (!fb || !bb || !scc))) return 1;
+    auto nbrs = [&](int32_t x, std::vector<int32_t>& out) {
+        out.clear();
+        int32_t i = fb[x], j = bb[x];
+        const int32_t ie = fb[x + 1], je = bb[x + 1];
+        while (i < ie || j < je) {
+            int32_t w;
+            if (j >= je || (i < ie && fc[i] <= bc[j])) {
+                w = fc[i++];
+                if (j < je && bc[j] == w) ++j;
+            } else {
+                w = bc[j++];
+            }
+            if (w != x && scc[w] == scc[x]) out.push_back(w);
+        }
+    };
+    std::vector<int32_t> disc(V, -1), low(V, 0), par(V, -1);
+    std::vector<uint8_t> covered(V, 0);
+    std::vector<std::pair<int32_t, int32_t>> estack;
+    struct Frame {
+        int32_t v;
+        std::vector<int32_t> nb;
+        size_t next;
+    };
+    int64_t nm = 0, nbk = 0;
+    bool overflow = false;
+    auto emit = [&](std::vector<int32_t>& verts) {
+        std::sort(verts.begin(), verts.end());
+        verts.erase(std::unique(verts.begin(), verts.end()), verts.end());
+        for (int32_t x : verts) {
+            if (nm < cap) {
+                mem_block[nm] = (int32_t)nbk;
+                mem_vertex[nm] = x;
+            } else {
+                overflow = true;
+            }
+            ++nm;
+            covered[x] = 1;
+        }
+        ++nbk;
+    };
+    int32_t timer = 0;
+    std::vector<int32_t> blockv;
+    for (int32_t root = 0; root < V; ++root) {
+        if (disc[root] >= 0) continue;
+        std::vector<Frame> st;
+        disc[root] = low[root] = timer++;
+        st.push_back({root, {}, 0});
+        nbrs(root, st.back().nb);
+        while (!st.empty()) {
+            Frame& f = st.back();
+            const int32_t v = f.v;
+            if (f.next < f.nb.size()) {
+                const int32_t w = f.nb[f.next++];
+                if (w == par[v]) continue;  // the tree edge (a simple view has one)
+                if (disc[w] < 0) {
+                    disc[w] = low[w] = timer++;
+                    par[w] = v;
+                    estack.push_back({v, w});
+                    st.push_back({w, {}, 0});
+                    nbrs(w, st.back().nb);
+                } else if (disc[w] < disc[v]) {
+                    low[v] = std::min(low[v], disc[w]);
+                    estack.push_back({v, w});
+                }
+            } else {
+                st.pop_back();
+                if (st.empty()) break;
+                const int32_t p = st.back().v;
+                low[p] = std::min(low[p], low[v]);
+                if (low[v] >= disc[p]) {
+                    blockv.clear();
+                    while (!estack.empty()) {
+                        const auto e = estack.back();
+                        estack.pop_back();
+                        blockv.push_back(e.first);
+                        blockv.push_back(e.second);
+                        if (e.first == p && e.second == v) break;
+                    }
+                    emit(blockv);
+                }
+            }
+        }
+    }
+    for (int32_t x = 0; x < V; ++x)
+        if (!covered[x]) {
+            blockv.assign(1, x);
+            emit(blockv);
+        }
+    *n_mem = nm;
+    *n_blocks = nbk;
+    return overflow ? 2 : 0;
+}
+
 // ---------------- textbook checkers ----------------
 
 extern "C" int orc_bfs_textbook(int32_t V, const int32_t* fb, const int32_t* fc, int32_t src, int32_t* level) {

@@ -33,6 +33,8 @@
  *                  over the DAG arc list
  *   orc_two_level  golden bfsTwoLevel / ssspTwoLevel results;
  *                  orc_bfs_textbook and orc_dijkstra
+ *   orc_bicc       golden getBCTComponents blocks of every SCC; networkx's
+ *                  biconnected_components on each SCC's undirected view
  *   orc_bc         golden GIR::bc scores; the definition (sums over pairs
  *                  of sigma_sv * sigma_vt / sigma_st from all-pairs BFS)
  *                  on tiny graphs
@@ -128,6 +130,12 @@ int orc_two_level(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const
  * (out-of-range ones skipped), score[V]. */
 int orc_bc(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg, const int32_t* bw_adj,
            const int32_t* src, int32_t k, double* score);
+
+/* Blocks of every SCC's undirected view (gir_test.cpp:448-465,
+ * dynamic_repair.h:96-170) as (block, vertex) membership pairs. */
+int orc_bicc(int32_t V, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg, const int32_t* bw_adj,
+             const int32_t* scc, int32_t* mem_block, int32_t* mem_vertex, int64_t cap, int64_t* n_mem,
+             int64_t* n_blocks);
 
 /* threads the OpenMP parts use (omp_get_max_threads) */
 int orc_num_threads(void);

@@ -23,6 +23,8 @@ _SIGS = {
     "orc_reach_info": (ctypes.c_int, [_i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp]),
     "orc_two_level": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _i32, _u64, _i32, _vp]),
     "orc_bc": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
+    "orc_bicc": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, ctypes.POINTER(_i64),
+                                ctypes.POINTER(_i64)]),
     "orc_wcc": (ctypes.c_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
     "orc_scc": (ctypes.c_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
     "orc_dag": (ctypes.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64)]),
@@ -198,6 +200,27 @@ def bc(fb, fa, bb, ba, sources):
     sc = np.empty(V, np.float64)
     _chk(lib().orc_bc(V, _p(fb), _p(fa), _p(bb), _p(ba), _p(src), len(src), _p(sc)), "bc")
     return sc
+
+
+def bicc(fb, fa, bb, ba, scc_labels):
+    """Blocks of every SCC's undirected view (gir_test.cpp:448-465,
+    dynamic_repair.h:96-170) -> sorted list of sorted vertex tuples."""
+    V = len(fb) - 1
+    lab = np.ascontiguousarray(scc_labels, np.int32)
+    cap = V + 2 * len(fa) + 1
+    mb, mv = np.empty(cap, np.int32), np.empty(cap, np.int32)
+    nm, nb = _i64(), _i64()
+    _chk(lib().orc_bicc(V, _p(fb), _p(fa), _p(bb), _p(ba), _p(lab), _p(mb), _p(mv), cap, ctypes.byref(nm),
+                        ctypes.byref(nb)), "bicc")
+    return partition(mb[:nm.value], mv[:nm.value])
+
+
+def partition(block_ids, vertices):
+    """(block, vertex) membership pairs -> sorted list of sorted tuples."""
+    groups = {}
+    for b, v in zip(np.asarray(block_ids).tolist(), np.asarray(vertices).tolist()):
+        groups.setdefault(b, set()).add(v)
+    return sorted(tuple(sorted(g)) for g in groups.values())
 
 
 def dijkstra(fb, fa, w, src):

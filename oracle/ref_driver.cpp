@@ -9,7 +9,7 @@
 //
 //   gir_ref CSR_DIR OUT_DIR [--bfs SRC.bin] [--sssp SRC.bin] [--delta D]
 //           [--batch OPS.bin] [--pairs PAIRS.bin] [--reach PAIRS.bin] [--two-level] [--bc SRC.bin]
-//           [--no-output] [--reps R]
+//           [--bicc] [--no-output] [--reps R]
 //
 //   wcc.bin  int32[V]   buildWCC labels (gir.cpp:593-655), canonical
 //   scc.bin  int32[V]   buildSCC_DAG node_scc (gir.cpp:798-839), canonical
@@ -24,6 +24,8 @@
 //   per --sssp source: sssp2.bin uint64[k*V] GIR::ssspTwoLevel, delta D
 //   (gir_query.cpp:705-722)
 //   bc.bin   double[V] GIR::bc over the sources of --bc (gir_query.cpp:728-778)
+//   bicc.bin int32[2*k] (block, vertex) pairs of GIR::getBCTComponents over
+//            every SCC after buildBCT (gir.cpp:2063, 4140-4149); with --bicc
 //   post_{wcc,scc,dag}.bin  the same after GIR::applyBatch of OPS.bin
 //                        (int32 triples u, v, insert) with blocks on demand
 // One JSON line per timed phase goes to stdout, prefixed "@@".
@@ -117,7 +119,7 @@ void writeLayers(const std::string& out, const std::string& pre, const Layers& L
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--pairs F] [--reach F] [--two-level] [--bc F] [--no-output] [--reps R]\n");
+        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--pairs F] [--reach F] [--two-level] [--bc F] [--bicc] [--no-output] [--reps R]\n");
         return 2;
     }
     std::string dir = argv[1], out = argv[2];
@@ -125,7 +127,7 @@ int main(int argc, char** argv) {
     if (out.back() != '/') out += '/';
     std::string bfs_f, sssp_f, batch_f, pairs_f, reach_f, bc_f;
     uint64_t delta = 16;
-    bool write = true, two_level = false;
+    bool write = true, two_level = false, bicc = false;
     int reps = 1;
     for (int i = 3; i < argc; ++i) {
         std::string a = argv[i];
@@ -139,6 +141,7 @@ int main(int argc, char** argv) {
         else if (a == "--no-output") write = false;
         else if (a == "--two-level") two_level = true;
         else if (a == "--bc") bc_f = next();
+        else if (a == "--bicc") bicc = true;
         else if (a == "--reps") reps = std::atoi(next().c_str());
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
@@ -226,6 +229,25 @@ int main(int argc, char** argv) {
         }
         std::printf("@@{\"phase\": \"sssp2\", \"sources\": %zu, \"ms\": %.3f, \"keep\": %llu}\n", src.size(), msSince(t0), keep);
         if (write) writeVec(out + "sssp2.bin", dv.data(), dv.size());
+    }
+    if (bicc) {
+        auto t0 = Clock::now();
+        g.buildBCT();
+        std::printf("@@{\"phase\": \"bicc\", \"ms\": %.3f}\n", msSince(t0));
+        std::vector<int> pairs;
+        const auto& sccs = g.getSCCs();
+        int nb = 0;
+        for (size_t s2 = 0; s2 < sccs.size(); ++s2) {
+            if (sccs[s2].empty()) continue;
+            for (auto& blk : g.getBCTComponents((int)s2)) {
+                for (int x : blk) {
+                    pairs.push_back(nb);
+                    pairs.push_back(x);
+                }
+                ++nb;
+            }
+        }
+        writeVec(out + "bicc.bin", pairs.data(), pairs.size());  // (compared by the bench)
     }
     if (!bc_f.empty()) {
         auto src = readVec<int>(bc_f);

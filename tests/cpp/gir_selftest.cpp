@@ -306,6 +306,20 @@ int main(int argc, char** argv) {
         g->buildSCC_DAG();
         CHECK(g->getWCCs().size() == 2, "example: 2 WCCs");
         CHECK(normalise(g->getSCCs()).size() == 7, "example: 7 SCCs");
+        // block layer (blocks of the reference's own build on this graph)
+        CHECK(normalise(g->getBCTComponents(0)) ==
+                  std::vector<std::vector<int>>({{0, 1, 2}, {2, 9}, {9, 10, 11}}),
+              "example: blocks of SCC {0,1,2,9,10,11}");
+        CHECK(normalise(g->getBCTComponents(6)) == std::vector<std::vector<int>>({{6, 7, 8}}),
+              "example: blocks of SCC {6,7,8}");
+        CHECK(normalise(g->getBCTComponents(5)) == std::vector<std::vector<int>>({{5}}),
+              "example: single-vertex SCC is a block");
+        CHECK(g->articulationPoint(2) && g->articulationPoint(9) && !g->articulationPoint(0) &&
+                  !g->articulationPoint(3),
+              "example: articulation points 2 and 9");
+        CHECK(g->biconnected(0, 1) && g->biconnected(2, 9) && !g->biconnected(0, 9) && !g->biconnected(0, 3),
+              "example: biconnected pairs");
+        CHECK(g->blockCount() == 9, "example: 9 blocks");
         g->insertEdge(3, 12);
         CHECK(g->wccOf(3) == g->wccOf(12), "example: insert 3->12 merges WCCs");
         g->deleteEdge(3, 12);

@@ -17,6 +17,7 @@ graphs and stores inputs and outputs as compressed .npz files:
   sssp2          GIR::ssspTwoLevel distances per sssp source (delta 16,
                  gir_query.cpp:705-722)
   bc             GIR::bc over the bfs sources (gir_query.cpp:728-778)
+  bicc           (block, vertex) pairs of GIR::getBCTComponents over every SCC
   ops            int32 (u, v, insert) batch; post_wcc / post_scc / post_dag
                  after GIR::applyBatch (blocks on demand)
 
@@ -85,7 +86,7 @@ def run_case(name, V, fb, fa, bb, ba, n_bfs, n_sssp, n_ops, seed):
         out = os.path.join(td, "out") + "/"
         os.makedirs(out)
         subprocess.run([REF, g + "/", out, "--bfs", os.path.join(td, "b.bin"), "--sssp", os.path.join(td, "s.bin"),
-                        "--pairs", os.path.join(td, "p.bin"), "--reach", os.path.join(td, "p.bin"), "--two-level", "--bc", os.path.join(td, "b.bin"),
+                        "--pairs", os.path.join(td, "p.bin"), "--reach", os.path.join(td, "p.bin"), "--two-level", "--bc", os.path.join(td, "b.bin"), "--bicc",
                         "--batch", os.path.join(td, "o.bin")],
                        check=True, stdout=subprocess.DEVNULL)
 
@@ -99,7 +100,7 @@ def run_case(name, V, fb, fa, bb, ba, n_bfs, n_sssp, n_ops, seed):
             sssp_src=ssrc, sssp=rd("sssp.bin", np.uint64).reshape(len(ssrc), V),
             pairs=pairs, weight=rd("weight.bin", np.uint32), reach=rd("reach.bin", np.uint8), ops=ops,
             bfs2=rd("bfs2.bin", np.int32).reshape(len(bsrc), V), rsccs=rd("rsccs.bin", np.uint8).reshape(len(bsrc), V),
-            sssp2=rd("sssp2.bin", np.uint64).reshape(len(ssrc), V), bc=rd("bc.bin", np.float64),
+            sssp2=rd("sssp2.bin", np.uint64).reshape(len(ssrc), V), bc=rd("bc.bin", np.float64), bicc=rd("bicc.bin", np.int32).reshape(-1, 2),
             post_wcc=rd("post_wcc.bin", np.int32), post_scc=rd("post_scc.bin", np.int32),
             post_dag=rd("post_dag.bin", np.int32).reshape(-1, 2))
         print(name, "V", V, "E", len(fa))

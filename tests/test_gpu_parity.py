@@ -609,3 +609,43 @@ def test_incremental_edge_cases():
         G.set_weights(np.full(len(fa), 2**31 - 1, np.uint32))
         with pytest.raises(girhip.GirError):
             G.track_incremental(x, 1)  # distances beyond 32 bits
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["golden", "rmat16", "lattice"])
+def test_bicc_matches_oracle(kind):
+    # blocks of every SCC's undirected view (buildBCT / getBCTComponents /
+    # articulationPoint, gir.cpp:2063-2110, 4140-4149): the same vertex-set
+    # partition as the oracle (itself pinned to the compiled reference and
+    # networkx), articulation points = vertices in two or more blocks
+    cases = []
+    if kind == "golden":
+        for name in ("example_graph", "rmat10", "rmat12", "grid40", "er3000"):
+            g = load_golden(name)
+            cases.append((name, g["fw_beg"], g["fw_adj"], g["bw_beg"], g["bw_adj"],
+                          oracle.partition(g["bicc"][:, 0], g["bicc"][:, 1])))
+    elif kind == "rmat16":
+        V, fb, fa, bb, ba = girhip.rmat_graph(16, edge_factor=16, seed=111)
+        cases.append((kind, fb, fa, bb, ba, None))
+    else:
+        V, fb, fa, bb, ba = girhip.grid_graph(150, 150, p_keep=0.9, p_bidir=0.6, seed=112)
+        cases.append((kind, fb, fa, bb, ba, None))
+    for name, fb, fa, bb, ba, want in cases:
+        say("bicc", name, "V", len(fb) - 1)
+        with girhip.Graph(fb, fa, bb, ba) as G:
+            with pytest.raises(girhip.GirError):
+                G.bicc()  # SCC layer not built
+            scc, _, _ = G.scc_dag()
+            r = G.bicc()
+        if want is None:
+            want = oracle.bicc(fb, fa, bb, ba, scc)
+        got = girhip.bicc_blocks(r["parent"], r["block"])
+        assert got == want, name
+        assert r["n_blocks"] == len(want)
+        count = np.zeros(len(fb) - 1, np.int64)
+        for b in want:
+            count[list(b)] += 1
+        assert (r["ap"] == (count >= 2)).all(), name
+        p = r["parent"]
+        assert ((p < 0) == (scc == np.arange(len(scc)))).all()  # roots = canonical ids
+        assert (scc[p[p >= 0]] == scc[p >= 0]).all()

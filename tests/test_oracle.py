@@ -302,3 +302,47 @@ def test_oracle_bc_matches_reference_golden(golden):
     name, g = golden
     got = oracle.bc(g["fw_beg"], g["fw_adj"], g["bw_beg"], g["bw_adj"], g["bfs_src"])
     assert np.allclose(got, g["bc"], rtol=1e-12, atol=1e-9), name
+
+
+def _nx_bicc(fb, fa, scc):
+    """Blocks of every SCC's undirected view by networkx (its
+    biconnected_components; vertices in no block on their own)."""
+    nx = pytest.importorskip("networkx")
+    V = len(fb) - 1
+    G = nx.Graph()
+    G.add_nodes_from(range(V))
+    src = np.repeat(np.arange(V), np.diff(fb))
+    for u, w in zip(src.tolist(), fa.tolist()):
+        if u != w and scc[u] == scc[w]:
+            G.add_edge(u, w)
+    blocks = [tuple(sorted(c)) for c in nx.biconnected_components(G)]
+    covered = {x for b in blocks for x in b}
+    blocks += [(x,) for x in range(V) if x not in covered]
+    return sorted(blocks)
+
+
+@pytest.mark.parametrize("kind", ["rmat", "lattice", "er"])
+def test_oracle_bicc_matches_networkx(kind):
+    # blocks of every SCC's undirected view (gir_test.cpp:448-465,
+    # dynamic_repair.h:96-170) restated; pinned by networkx's biconnected
+    # components on the same views
+    if kind == "rmat":
+        V, fb, fa, bb, ba = girhip.rmat_graph(11, edge_factor=6, seed=101)
+    elif kind == "lattice":
+        V, fb, fa, bb, ba = girhip.grid_graph(30, 30, p_keep=0.9, p_bidir=0.5, seed=102)
+    else:
+        s, d = girhip.uniform_edges(2000, 5000, seed=103)
+        fb, fa, bb, ba = girhip.csr_from_edges(2000, s, d)
+    scc, _ = oracle.scc(fb, fa)
+    got = oracle.bicc(fb, fa, bb, ba, scc)
+    want = _nx_bicc(fb, fa, scc)
+    assert got == want
+    assert any(len(b) > 2 for b in want)
+    if kind != "er":  # bridges (two-vertex blocks) too
+        assert any(len(b) == 2 for b in want)
+
+
+def test_oracle_bicc_matches_reference_golden(golden):
+    name, g = golden
+    got = oracle.bicc(g["fw_beg"], g["fw_adj"], g["bw_beg"], g["bw_adj"], g["scc"])
+    assert got == oracle.partition(g["bicc"][:, 0], g["bicc"][:, 1]), name
