@@ -87,6 +87,7 @@ struct Params {
     double sssp_dist32 = 1;  // 1: SSSP works on 32-bit distances (64-bit redo on overflow)
     double reach_local = 1;  // 1: reachable() first tries a per-pair LDS search
     double graph = 0;        // 1: long runs of full chunks replay captured HIP graphs
+    double xcd = 0;          // 1: SSSP source rows pinned to XCDs (block b -> source b mod k)
 };
 
 struct KStat {
@@ -653,10 +654,13 @@ __device__ __forceinline__ bool setBit(uint64_t* bits, uint64_t* sum, int32_t x)
 // call body(g, sm) for every non-zero word g, sm = its value (wave-uniform
 // arguments: the whole wave runs the body).  Small G spreads dense bitmaps
 // over many waves; large G lets few waves sweep a sparse one.
+// (b, nb: this block's index and the block count of the walk, for kernels
+// that split one grid among several rows of work)
 template <class F>
-__device__ __forceinline__ void forSummary(const uint64_t* __restrict__ sum, int64_t W2, int G, F&& body) {
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / WAVE;
+__device__ __forceinline__ void forSummaryB(const uint64_t* __restrict__ sum, int64_t W2, int G, int64_t b, int64_t nb,
+                                            F&& body) {
+    const int64_t wave = (b * blockDim.x + threadIdx.x) / WAVE;
+    const int64_t nw = (nb * blockDim.x) / WAVE;
     const int l = laneId();
     for (int64_t t0 = wave * G; t0 < W2; t0 += nw * G) {
         const int64_t gi = t0 + l;
@@ -668,6 +672,10 @@ __device__ __forceinline__ void forSummary(const uint64_t* __restrict__ sum, int
             body(t0 + j, (uint64_t)__shfl((unsigned long long)mine, j));
         }
     }
+}
+template <class F>
+__device__ __forceinline__ void forSummary(const uint64_t* __restrict__ sum, int64_t W2, int G, F&& body) {
+    forSummaryB(sum, W2, G, (int64_t)blockIdx.x, (int64_t)gridDim.x, body);
 }
 
 // Wave-level synchronisation of LDS traffic between the lanes of one wave.

@@ -565,7 +565,8 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"scc_color_min", &Params::scc_color_min}, {"profile", &Params::profile},
                {"msbfs", &Params::msbfs},         {"chunk", &Params::chunk},
                {"persist", &Params::persist},     {"sssp_dist32", &Params::sssp_dist32},
-               {"reach_local", &Params::reach_local}, {"graph", &Params::graph}};
+               {"reach_local", &Params::reach_local}, {"graph", &Params::graph},
+               {"xcd", &Params::xcd}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;

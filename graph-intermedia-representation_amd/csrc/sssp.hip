@@ -3,7 +3,7 @@
 //
 // Delta-stepping with device-resident buckets.  All sources of a call (up to
 // SB per batch) advance together in rounds,
-// one grid row (blockIdx.y) per source.  Per source:
+// one row of the grid's blocks per source (gridRow).  Per source:
 //   - the current bucket is [lo, lo + delta); its work list is a queue;
 //   - a relaxation that improves a distance sets the head's bit in the near
 //     bitmap (lands inside the bucket) or in the far bitmap (beyond it), so
@@ -70,7 +70,24 @@ struct SsspBatch {
     int64_t V, W, W2;
     int k;               // sources in the batch
     int d32;             // 1: 32-bit distances, UINT32_MAX = unreachable
+    int xcd;             // 1: block b serves source b mod k (all of a source's blocks on the same XCDs)
 };
+
+// The part of a 1D grid of (blocks per source) x k blocks that serves one
+// source: its row s, this block's index b among the row's nb blocks.  By
+// default the rows are contiguous.  With xcd, block b serves source b mod k:
+// blocks are dealt to the 8 XCDs round robin, so a source's blocks share
+// XCDs and its distances stay in those XCDs' L2 (measured slower at RMAT-22:
+// the sources' frontiers differ in size, and each is confined to its XCDs).
+struct Row {
+    int s;
+    int64_t b, nb;
+};
+__device__ __forceinline__ Row gridRow(const SsspBatch& B) {
+    const int64_t nb = gridDim.x / B.k;
+    if (B.xcd) return Row{(int)(blockIdx.x % B.k), (int64_t)(blockIdx.x / B.k), nb};
+    return Row{(int)(blockIdx.x / nb), (int64_t)(blockIdx.x % nb), nb};
+}
 
 // 32-bit distances halve the array the random relaxations hit (a batch of
 // RMAT-22 distances then fits the 256 MB Infinity Cache).  A relaxation that
@@ -192,10 +209,11 @@ __device__ __forceinline__ void resetRound(const SsspBatch& B, int par, int s) {
 // track_done: mark them processed (their heavy arcs are due when the bucket
 // drains).
 __device__ __forceinline__ void relaxQueue(const RelaxF& f, const int32_t* __restrict__ fb, HeavyList H,
-                                           const int32_t* q, int64_t n, int s, bool track_done) {
+                                           const int32_t* q, int64_t n, int s, bool track_done, int64_t b0,
+                                           int64_t nb) {
     const SsspBatch& B = f.B;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+    const int64_t stride = nb * blockDim.x;
+    for (int64_t base = b0 * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
         const int64_t i = base + laneId();
         const bool act = i < n;
         const int32_t u = act ? q[i] : 0;
@@ -262,34 +280,36 @@ __device__ __forceinline__ void farMinCommit(const SsspBatch& B, int s, unsigned
 // bucket has drained: with heavy arcs pending (heavy = 1) the processed
 // vertices -> heavy-phase queue, else the smallest far distance is found
 // right here.
-__device__ __forceinline__ void compactPhase(const SsspBatch& B, int par, int heavy, int G, int s, FarLds& sh) {
+__device__ __forceinline__ void compactPhase(const SsspBatch& B, int par, int heavy, int G, int s, FarLds& sh,
+                                             int64_t bb, int64_t nbb) {
     const size_t bo = (size_t)s * B.W, so = (size_t)s * B.W2, qo = (size_t)s * B.V;
     if (ldAcq(nearSet(B, par, s))) {
-        forSummary(B.nsum + so, B.W2, G, [&](int64_t g, uint64_t sm) {
+        forSummaryB(B.nsum + so, B.W2, G, bb, nbb, [&](int64_t g, uint64_t sm) {
             drainGroup(B.nearbits + bo, B.nsum + so, g, sm, B.qn[par ^ 1] + qo, nearCount(B, par ^ 1, s));
         });
     } else if (heavy) {
-        forSummary(B.dsum + so, B.W2, G, [&](int64_t g, uint64_t sm) {
+        forSummaryB(B.dsum + so, B.W2, G, bb, nbb, [&](int64_t g, uint64_t sm) {
             drainGroup(B.donebits + bo, B.dsum + so, g, sm, B.qh + qo, cp(B, C_HQ, s));
         });
     } else {
         const uint64_t hi = (uint64_t)ldAcq(cp(B, C_HI, s));
         uint64_t mn = INF;
         int32_t* lds = sh.ids[threadIdx.x / WAVE];
-        forSummary(B.fsum + so, B.W2, G,
-                   [&](int64_t g, uint64_t sm) { mn = farMinGroup(B, s, g, sm, hi, mn, lds); });
+        forSummaryB(B.fsum + so, B.W2, G, bb, nbb,
+                    [&](int64_t g, uint64_t sm) { mn = farMinGroup(B, s, g, sm, hi, mn, lds); });
         farMinCommit(B, s, mn);
     }
 }
 
 // Drained source s after its heavy phase: smallest far distance >= hi.
-__device__ __forceinline__ void farMinPhase(const SsspBatch& B, int par, int G, int s, FarLds& sh) {
+__device__ __forceinline__ void farMinPhase(const SsspBatch& B, int par, int G, int s, FarLds& sh, int64_t bb,
+                                            int64_t nbb) {
     if (ldAcq(nearSet(B, par, s))) return;
     const uint64_t hi = (uint64_t)ldAcq(cp(B, C_HI, s));
     uint64_t mn = INF;
     int32_t* lds = sh.ids[threadIdx.x / WAVE];
-    forSummary(B.fsum + (size_t)s * B.W2, B.W2, G,
-               [&](int64_t g, uint64_t sm) { mn = farMinGroup(B, s, g, sm, hi, mn, lds); });
+    forSummaryB(B.fsum + (size_t)s * B.W2, B.W2, G, bb, nbb,
+                [&](int64_t g, uint64_t sm) { mn = farMinGroup(B, s, g, sm, hi, mn, lds); });
     farMinCommit(B, s, mn);
 }
 
@@ -298,13 +318,14 @@ __device__ __forceinline__ void farMinPhase(const SsspBatch& B, int par, int G, 
 // stores it for the next round).  Far entries inside it -> next queue,
 // below it -> settled (dropped), both leave the far bitmap; the rest stay.
 // A summary word is cleared once its 64 bitmap words are all empty.
-__device__ __forceinline__ void splitPhase(const SsspBatch& B, int par, uint64_t delta, int G, int s, FarLds& sh) {
+__device__ __forceinline__ void splitPhase(const SsspBatch& B, int par, uint64_t delta, int G, int s, FarLds& sh,
+                                           int64_t bb, int64_t nbb) {
     if (ldAcq(nearSet(B, par, s))) return;
     const uint64_t mfar = (uint64_t)ldAcq(cp(B, C_FMIN, s));
     if (mfar == INF) return;  // no far work left: this source is done
     const uint64_t lo = (mfar / delta) * delta;
     const uint64_t hi = lo > INF - delta ? INF : lo + delta;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bb == 0 && threadIdx.x == 0) {
         *cp(B, C_LO, s) = (int64_t)lo;
         *cp(B, C_HI, s) = (int64_t)hi;
     }
@@ -316,7 +337,7 @@ __device__ __forceinline__ void splitPhase(const SsspBatch& B, int par, uint64_t
     int32_t* lds = sh.ids[wv];
     uint64_t* keepw = sh.w[wv];
     uint64_t* takew = sh.t[wv];
-    forSummary(fs, B.W2, G, [&](int64_t g, uint64_t sm) {
+    forSummaryB(fs, B.W2, G, bb, nbb, [&](int64_t g, uint64_t sm) {
         const int l = laneId();
         const int64_t wi = g * 64 + l;
         const uint64_t m = ((sm >> l) & 1) ? fbits[wi] : 0;
@@ -346,31 +367,35 @@ __device__ __forceinline__ void splitPhase(const SsspBatch& B, int par, uint64_t
 // ---- one kernel per phase (grid row y = source) ----------------------------
 
 __global__ void kBRelax(RelaxF f, const int32_t* __restrict__ fb, HeavyList H, int track_done) {
-    const int s = blockIdx.y;
     const SsspBatch& B = f.B;
-    if (blockIdx.x == 0 && threadIdx.x == 0) resetRound(B, f.par, s);
-    relaxQueue(f, fb, H, B.qn[f.par] + (size_t)s * B.V, *nearCount(B, f.par, s), s, track_done != 0);
+    const Row r = gridRow(B);
+    if (r.b == 0 && threadIdx.x == 0) resetRound(B, f.par, r.s);
+    relaxQueue(f, fb, H, B.qn[f.par] + (size_t)r.s * B.V, *nearCount(B, f.par, r.s), r.s, track_done != 0, r.b,
+               r.nb);
 }
 
 __global__ void kBCompact(SsspBatch B, int par, int heavy, int G) {
     __shared__ FarLds sh;
-    compactPhase(B, par, heavy, G, blockIdx.y, sh);
+    const Row r = gridRow(B);
+    compactPhase(B, par, heavy, G, r.s, sh, r.b, r.nb);
 }
 
 __global__ void kBRelaxHeavy(RelaxF f, const int32_t* __restrict__ fb, HeavyList H) {
-    const int s = blockIdx.y;
     const SsspBatch& B = f.B;
-    relaxQueue(f, fb, H, B.qh + (size_t)s * B.V, *cp(B, C_HQ, s), s, false);
+    const Row r = gridRow(B);
+    relaxQueue(f, fb, H, B.qh + (size_t)r.s * B.V, *cp(B, C_HQ, r.s), r.s, false, r.b, r.nb);
 }
 
 __global__ void kBFarMin(SsspBatch B, int par, int G) {
     __shared__ FarLds sh;
-    farMinPhase(B, par, G, blockIdx.y, sh);
+    const Row r = gridRow(B);
+    farMinPhase(B, par, G, r.s, sh, r.b, r.nb);
 }
 
 __global__ void kBSplit(SsspBatch B, int par, uint64_t delta, int G) {
     __shared__ FarLds sh;
-    splitPhase(B, par, delta, G, blockIdx.y, sh);
+    const Row r = gridRow(B);
+    splitPhase(B, par, delta, G, r.s, sh, r.b, r.nb);
 }
 
 // ---- persistent rounds -----------------------------------------------------
@@ -395,18 +420,18 @@ __global__ void __launch_bounds__(256) kSsspPersist(RelaxF fl, RelaxF fh, const 
         fl.par = par;
         fh.par = par;
         for (int s = 0; s < k; ++s) relaxQueue(fl, fb, H, B.qn[par] + (size_t)s * B.V, ldAcq(nearCount(B, par, s)),
-                                               s, heavy != 0);
+                                               s, heavy != 0, blockIdx.x, gridDim.x);
         gridSync();
-        for (int s = 0; s < k; ++s) compactPhase(B, par, heavy, G, s, sh);
+        for (int s = 0; s < k; ++s) compactPhase(B, par, heavy, G, s, sh, blockIdx.x, gridDim.x);
         gridSync();
         if (heavy) {
             for (int s = 0; s < k; ++s) relaxQueue(fh, fb, H, B.qh + (size_t)s * B.V, ldAcq(cp(B, C_HQ, s)), s,
-                                                   false);
+                                                   false, blockIdx.x, gridDim.x);
             gridSync();
-            for (int s = 0; s < k; ++s) farMinPhase(B, par, G, s, sh);
+            for (int s = 0; s < k; ++s) farMinPhase(B, par, G, s, sh, blockIdx.x, gridDim.x);
             gridSync();
         }
-        for (int s = 0; s < k; ++s) splitPhase(B, par, delta, G, s, sh);
+        for (int s = 0; s < k; ++s) splitPhase(B, par, delta, G, s, sh, blockIdx.x, gridDim.x);
         gridSync();
         par ^= 1;
     }
@@ -465,6 +490,7 @@ static SsspBatch batchState(girhip_graph* h, int k, void* dist, bool d32) {
     B.ctl = S.s_ctl.p;
     B.dist = dist;
     B.d32 = d32 ? 1 : 0;
+    B.xcd = h->prm.xcd != 0 ? 1 : 0;
     B.k = k;
     return B;
 }
@@ -539,15 +565,15 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
         }
         auto issue = [&](int p) {
             for (int t = 0; t < R; ++t) {
-                launch(h, "sssp_relax", kBRelax, dim3(rgrid, k), blk, 0, fl[p], fb, heavyList(h), heavy);
+                launch(h, "sssp_relax", kBRelax, rgrid * k, blk, 0, fl[p], fb, heavyList(h), heavy);
                 launchHeavyStore(h, "sssp_relax_rows", fl[p]);
-                launch(h, "sssp_compact", kBCompact, dim3(sgrid, k), blk, 0, B, p, heavy, G);
+                launch(h, "sssp_compact", kBCompact, sgrid * k, blk, 0, B, p, heavy, G);
                 if (heavy) {
-                    launch(h, "sssp_relax_heavy", kBRelaxHeavy, dim3(hgrid, k), blk, 0, fh[p], fb, heavyList(h));
+                    launch(h, "sssp_relax_heavy", kBRelaxHeavy, hgrid * k, blk, 0, fh[p], fb, heavyList(h));
                     launchHeavyStore(h, "sssp_relax_heavy_rows", fh[p]);
-                    launch(h, "sssp_farmin", kBFarMin, dim3(sgrid, k), blk, 0, B, p, G);
+                    launch(h, "sssp_farmin", kBFarMin, sgrid * k, blk, 0, B, p, G);
                 }
-                launch(h, "sssp_split", kBSplit, dim3(sgrid, k), blk, 0, B, p, delta, G);
+                launch(h, "sssp_split", kBSplit, sgrid * k, blk, 0, B, p, delta, G);
                 p ^= 1;
             }
         };
