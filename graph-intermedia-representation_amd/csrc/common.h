@@ -489,8 +489,9 @@ __device__ __forceinline__ void waveRowsP(bool act, int32_t v, P pay, int64_t b,
 // Heavy rows.  RMAT hubs hold 10^5 arcs; a row walked by one wave costs
 // thousands of dependent steps while the rest of the GPU idles.  A row kernel
 // therefore hands every row longer than HEAVY_T arcs to a heavy list instead
-// of walking it, and a follow-up launch of kHeavyRows walks each listed row
-// with the whole grid (grid-stride over the row's arcs).  The row work is a
+// of walking it, and a follow-up launch of kHeavyRows walks the listed rows
+// with the whole grid (the rows' arcs concatenated, one contiguous range per
+// block).  The row work is a
 // functor shared by both kernels: f(v, pay, aux, e) -> int64 contribution
 // (summed per lane; the kernels reduce it per wave).  The list is reset by
 // the last block of kHeavyRows, so consecutive launches need no host work.
@@ -776,10 +777,14 @@ template <class F>
 __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
     const int64_t n = ldAcq(&H.ctl[0]);
     const int64_t T = n ? H.off[n] : 0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // each block takes a contiguous range of the concatenated arcs, so a
+    // thread's next arc is usually in the same row and the row search runs
+    // only at row boundaries
+    const int64_t per = (T + gridDim.x - 1) / gridDim.x;
+    const int64_t p0 = (int64_t)blockIdx.x * per, p1 = p0 + per < T ? p0 + per : T;
     int64_t acc = 0;
     int64_t k = 0;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < T; p += stride) {
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
         if (!(p >= H.off[k] && p < H.off[k + 1])) {
             int64_t lo = 0, hi = n - 1;  // last k with off[k] <= p
             while (lo < hi) {
