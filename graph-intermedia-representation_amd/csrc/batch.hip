@@ -381,6 +381,7 @@ void applyStore(girhip_graph* h, const int32_t* u, const int32_t* v, const uint8
     h->max_deg = -1;
     h->max_w = -1;
     h->sym_ok = false;
+    for (auto& sv : h->split) sv.delta = 0;
 }
 
 // Bring every requested layer up to date.  C = the changes just applied to a

@@ -88,6 +88,7 @@ struct Params {
     double reach_local = 1;  // 1: reachable() first tries a per-pair LDS search
     double graph = 0;        // 1: long runs of full chunks replay captured HIP graphs
     double xcd = 0;          // 1: SSSP source rows pinned to XCDs (block b -> source b mod k)
+    double split = 1;        // 1: weighted SSSP walks rows split into light and heavy arcs
 };
 
 struct KStat {
@@ -184,6 +185,18 @@ struct girhip_graph {
     int64_t sym_E = 0;
     int64_t max_deg = -1;  // largest fw/bw row length; -1 = not yet computed
     int64_t max_w = -1;    // largest stored weight; -1 = not yet computed
+    // SSSP views of the forward rows for one bucket width delta (the two
+    // widths used last): each row's arcs reordered so that those of weight
+    // <= delta come first, mid[u] = where the heavier ones start, weights
+    // (stored or hash) alongside.  delta 0 = empty; cleared when the store
+    // or the weights change.
+    struct SplitView {
+        uint64_t delta = 0;
+        uint64_t used = 0;
+        gir::DevBuf<int32_t> adj, mid;
+        gir::DevBuf<uint32_t> w;
+    } split[2];
+    uint64_t split_clock = 0;
 
     // layers
     bool wcc_req = false, scc_req = false;  // built at least once (repair on update)

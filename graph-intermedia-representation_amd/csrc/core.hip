@@ -531,6 +531,7 @@ __global__ void kCheckWeights(const uint32_t* __restrict__ w, int64_t n, int64_t
 extern "C" int girhip_set_weights(girhip_graph* h, const uint32_t* w_fw) {
     return guarded([&] {
         useDevice(h);
+        for (auto& sv : h->split) sv.delta = 0;
         if (!w_fw) {
             h->has_w = false;
             h->w_fw.release();
@@ -566,7 +567,7 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"msbfs", &Params::msbfs},         {"chunk", &Params::chunk},
                {"persist", &Params::persist},     {"sssp_dist32", &Params::sssp_dist32},
                {"reach_local", &Params::reach_local}, {"graph", &Params::graph},
-               {"xcd", &Params::xcd}};
+               {"xcd", &Params::xcd}, {"split", &Params::split}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;

@@ -144,6 +144,8 @@ __global__ void kBInit(SsspBatch B, const int32_t* __restrict__ src, uint64_t de
 // Relaxation of arc slot j of row p (owner distance dp) for source s.
 // heavy = 0: only light arcs (w <= delta, the bucket's inner rounds);
 // heavy = 1: only heavy arcs (w > delta, once per vertex and bucket).
+// With the split view (mid) each pass walks only its own segment of a row,
+// so the class test drops out and no pass reads the other class's arcs.
 // scope (optional): arcs whose ends carry different labels are skipped (the
 // SCC-restricted delta-stepping of the two-level queries); unit: every arc
 // weighs 1 (BFS levels as distances).
@@ -156,12 +158,16 @@ struct RelaxF {
     int par;
     const int32_t* scope;
     int unit;
+    const int32_t* mid;  // split view (fa / w are its arrays): rows walk only their class's arcs
     __device__ int64_t operator()(int32_t p, uint64_t dp, int32_t s, int64_t j) const {
         uint64_t wt;
         int32_t x;
         if (unit) {
             if (heavy) return 0;
             wt = 1;
+            x = fa[j];
+        } else if (mid) {
+            wt = w[j];
             x = fa[j];
         } else if (w) {
             wt = w[j];
@@ -194,6 +200,102 @@ struct RelaxF {
     }
 };
 
+// ---- light/heavy split view -------------------------------------------------
+// For bucket width delta, every forward row reordered so that its arcs of
+// weight <= delta come first; mid[u] = where the heavier ones start.  Rows of
+// at most SPLIT_SHORT arcs take one thread each (light arcs forward from the
+// row start, heavy ones backward from its end); longer rows are listed for
+// kSplitLong, one block per row (block-wide ranks of the light arcs).  The
+// order inside a class is free: distances do not depend on the order in
+// which arcs are relaxed.
+constexpr int SPLIT_SHORT = 64;
+
+struct SplitArgs {
+    const int32_t* beg;
+    const int32_t* adj;
+    const uint32_t* w;  // null: the reference's hash weights
+    int32_t V;
+    uint64_t delta;
+    int32_t* oadj;
+    uint32_t* ow;
+    int32_t* mid;
+    int32_t* longq;
+    int64_t* nlong;
+    __device__ uint32_t weight(int32_t u, int64_t j, int32_t x) const { return w ? w[j] : hashWeight(u, x); }
+};
+
+__global__ void kSplitShort(SplitArgs A) {
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < A.V; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t u = base + threadIdx.x;
+        const bool act = u < A.V;
+        const int64_t b = act ? A.beg[u] : 0, e = act ? A.beg[u + 1] : 0;
+        const bool lng = act && e - b > SPLIT_SHORT;
+        const int64_t pos = wavePush(A.nlong, lng);
+        if (lng) A.longq[pos] = (int32_t)u;
+        if (act && !lng) {
+            int64_t lo = b, hi = e;
+            for (int64_t j = b; j < e; ++j) {
+                const int32_t x = A.adj[j];
+                const uint32_t wt = A.weight((int32_t)u, j, x);
+                const int64_t o = wt <= A.delta ? lo++ : --hi;
+                A.oadj[o] = x;
+                A.ow[o] = wt;
+            }
+            A.mid[u] = (int32_t)lo;
+        }
+    }
+}
+
+__global__ void kSplitLong(SplitArgs A) {
+    __shared__ int32_t wcnt[1024 / WAVE];
+    __shared__ int64_t tot;
+    const int64_t n = *A.nlong;
+    const int lane = laneId(), wv = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int32_t u = A.longq[i];
+        const int64_t b = A.beg[u], e = A.beg[u + 1];
+        int64_t c = 0;  // light arcs of the row
+        for (int64_t j = b + threadIdx.x; j < e; j += blockDim.x) c += A.weight(u, j, A.adj[j]) <= A.delta;
+        c = waveSum(c);
+        if (lane == 0) wcnt[wv] = (int32_t)c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t t = 0;
+            for (int k = 0; k < nw; ++k) t += wcnt[k];
+            tot = t;
+        }
+        __syncthreads();
+        const int64_t m = b + tot;
+        int64_t lc = 0;  // light arcs of the earlier chunks
+        for (int64_t c0 = b; c0 < e; c0 += blockDim.x) {
+            const int64_t j = c0 + threadIdx.x;
+            const bool val = j < e;
+            const int32_t x = val ? A.adj[j] : 0;
+            const uint32_t wt = val ? A.weight(u, j, x) : 0;
+            const bool light = val && wt <= A.delta;
+            const uint64_t bal = __ballot(light);
+            __syncthreads();  // wcnt: the previous chunk's (or the count's) reads are done
+            if (lane == 0) wcnt[wv] = __popcll(bal);
+            __syncthreads();
+            int64_t before = 0, all = 0;
+            for (int k = 0; k < nw; ++k) {
+                const int t = wcnt[k];
+                if (k < wv) before += t;
+                all += t;
+            }
+            if (val) {
+                const int64_t lr = lc + before + __popcll(bal & lanesBelow());  // light arcs before j in the row
+                const int64_t o = light ? b + lr : m + (j - b) - lr;
+                A.oadj[o] = x;
+                A.ow[o] = wt;
+            }
+            lc += all;
+        }
+        if (threadIdx.x == 0) A.mid[u] = (int32_t)m;
+        __syncthreads();  // tot and wcnt are rewritten for the next row
+    }
+}
+
 // ---- round phases (per source s; the grid strides over x) -----------------
 
 // Per-round slots of source s that the round fills later, and the next
@@ -219,7 +321,12 @@ __device__ __forceinline__ void relaxQueue(const RelaxF& f, const int32_t* __res
         const int32_t u = act ? q[i] : 0;
         const uint64_t du = act ? dGet(B, s, u) : 0;
         if (act && track_done) setBit(B.donebits + (size_t)s * B.W, B.dsum + (size_t)s * B.W2, u);
-        const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+        int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+        if (f.mid && act) {
+            const int32_t m = f.mid[u];
+            if (f.heavy) b = m;
+            else e = m;
+        }
         waveRowsH(act, u, du, s, b, e, H, f);
     }
 }
@@ -504,19 +611,56 @@ static void clearBitmaps(girhip_graph* h, const SsspBatch& B) {
     fillU64(h, B.dsum, 0, (int64_t)B.k * B.W2);
 }
 
+using SplitView = girhip_graph::SplitView;
+
+// The split view of width delta (null when the split knob is off); built on
+// first use and kept for the two widths used last.  Its build time is
+// reported as "sssp_split".
+static const SplitView* splitView(girhip_graph* h, uint64_t delta) {
+    if (h->prm.split == 0 || h->E == 0) return nullptr;
+    for (auto& sv : h->split)
+        if (sv.delta == delta) {
+            sv.used = ++h->split_clock;
+            return &sv;
+        }
+    SplitView& sv = h->split[0].used <= h->split[1].used ? h->split[0] : h->split[1];
+    sv.delta = 0;
+    const int32_t V = h->V;
+    sv.adj.ensure((size_t)h->E);
+    sv.w.ensure((size_t)h->E);
+    sv.mid.ensure((size_t)(V > 0 ? V : 1));
+    DevBuf<int32_t> longq;
+    longq.ensure((size_t)(V > 0 ? V : 1));
+    fillI64(h, h->dcnt.p, 0, 1);
+    const SplitArgs A{h->fw_beg.p, h->fw_adj.p, h->has_w ? h->w_fw.p : nullptr, V, delta, sv.adj.p, sv.w.p,
+                      sv.mid.p, longq.p, h->dcnt.p};
+    tBegin(h);
+    launch(h, "sssp_split_short", kSplitShort, boundedGrid(V, 256), 256, 0, A);
+    launch(h, "sssp_split_long", kSplitLong, 2048, 256, 0, A);
+    tEnd(h, "sssp_split");
+    sv.delta = delta;
+    sv.used = ++h->split_clock;
+    return &sv;
+}
+
 // Bucket rounds from the state in B until every source's queue is empty
-// after an advance (all settled).  scope / unit: see RelaxF.
-static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, const int32_t* scope, bool unit) {
+// after an advance (all settled).  scope / unit: see RelaxF; sv: the split
+// view of delta, or null (rows classify their arcs by weight on the fly).
+static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, const int32_t* scope, bool unit,
+                       const SplitView* sv) {
     Scratch& S = scratch(h);
     const int32_t V = h->V;
     const int k = B.k;
     const int blk = (int)h->prm.block;
-    const uint32_t* w = h->has_w ? h->w_fw.p : nullptr;
+    if (unit) sv = nullptr;
+    const int32_t* fa = sv ? sv->adj.p : h->fw_adj.p;
+    const uint32_t* w = sv ? sv->w.p : (h->has_w ? h->w_fw.p : nullptr);
+    const int32_t* mid = sv ? sv->mid.p : nullptr;
     // heavy arcs exist only if some weight exceeds delta
     const int heavy = !unit && (uint64_t)maxWeight(h) > delta ? 1 : 0;
     const int u = unit ? 1 : 0;
-    const RelaxF fl[2] = {{B, h->fw_adj.p, w, delta, 0, 0, scope, u}, {B, h->fw_adj.p, w, delta, 0, 1, scope, u}};
-    const RelaxF fh[2] = {{B, h->fw_adj.p, w, delta, 1, 0, scope, u}, {B, h->fw_adj.p, w, delta, 1, 1, scope, u}};
+    const RelaxF fl[2] = {{B, fa, w, delta, 0, 0, scope, u, mid}, {B, fa, w, delta, 0, 1, scope, u, mid}};
+    const RelaxF fh[2] = {{B, fa, w, delta, 1, 0, scope, u, mid}, {B, fa, w, delta, 1, 1, scope, u, mid}};
     const unsigned hgrid = std::min<unsigned>(gridFor(V, blk), 1024);
     const int32_t* fb = h->fw_beg.p;
     int par = 0;
@@ -592,13 +736,14 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
 }
 
 // k <= SB sources (device array dsrc) into rows of dist (see batchState).
-static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, void* dist, bool d32) {
+static void ssspBatch(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, void* dist, bool d32,
+                      const SplitView* sv) {
     const SsspBatch B = batchState(h, k, dist, d32);
     if (d32) fillI32(h, (int32_t*)dist, -1, (int64_t)k * h->V);
     else fillU64(h, (uint64_t*)dist, INF, (int64_t)k * h->V);
     clearBitmaps(h, B);
     launch(h, "sssp_init", kBInit, 1, 64, 0, B, dsrc, delta);
-    ssspRounds(h, B, delta, nullptr, false);
+    ssspRounds(h, B, delta, nullptr, false, sv);
 }
 
 // ---- condensation-guided queries --------------------------------------------
@@ -858,7 +1003,8 @@ __global__ void kZeroSlot(int64_t* p) {
 
 // One source's two-level query into dist (uint32 when d32); the pending
 // counts of the sweep from src's SCC must be in tl_left.  unit: BFS levels.
-static void waveQuery(girhip_graph* h, int32_t src, uint64_t delta, bool unit, void* dist, bool d32) {
+static void waveQuery(girhip_graph* h, int32_t src, uint64_t delta, bool unit, void* dist, bool d32,
+                      const SplitView* sv) {
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
     Scratch& S = scratch(h);
@@ -885,7 +1031,7 @@ static void waveQuery(girhip_graph* h, int32_t src, uint64_t delta, bool unit, v
         launchHeavy(h, "tl_expand_heavy", ef);
         if (nbig > 0) {
             launch(h, "tl_seeded", kBInitSeeded, 1, 64, 0, B);
-            ssspRounds(h, B, delta, h->scc.p, unit);
+            ssspRounds(h, B, delta, h->scc.p, unit, sv);
         }
         launch(h, "tl_cross", kCross, 1024, blk, 0, cf, (const int32_t*)h->fw_beg.p, (const int32_t*)S.tl_vl.p,
                (const int64_t*)ctl, heavyList(h));
@@ -961,12 +1107,13 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         // 32-bit distances unless weights alone could overflow them
         const bool try32 = h->prm.sssp_dist32 != 0 && (uint64_t)maxWeight(h) < 0x7fffffffull;
         if (try32) S.s_d32.ensure((size_t)kc * Vs);
+        const SplitView* sv = k > 0 ? splitView(h, delta) : nullptr;
         tBegin(h);
         for (int32_t i0 = 0; i0 < k; i0 += SB) {
             const int c = std::min<int32_t>(SB, k - i0);
             uint64_t* rows = out + (size_t)i0 * V;
             if (try32) {
-                ssspBatch(h, dsrc + i0, c, delta, S.s_d32.p, true);
+                ssspBatch(h, dsrc + i0, c, delta, S.s_d32.p, true, sv);
                 int64_t ovf;
                 readCounters(h, S.s_ctl.p + C_OVF, 1, &ovf);
                 if (ovf == 0) {
@@ -975,7 +1122,7 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
                     continue;
                 }
             }
-            ssspBatch(h, dsrc + i0, c, delta, rows, false);
+            ssspBatch(h, dsrc + i0, c, delta, rows, false, sv);
         }
         tEnd(h, "sssp");
         if (edges_out) {
@@ -1062,6 +1209,7 @@ static void twoLevelMany(girhip_graph* h, const int32_t* src, int32_t k, uint64_
     ensureBatchScratch(h, 1);
     const bool try32 = unit || (h->prm.sssp_dist32 != 0 && (uint64_t)maxWeight(h) < 0x7fffffffull);
     if (try32) S.s_d32.ensure((size_t)std::max<int32_t>(V, 1));
+    const SplitView* sv = unit || k == 0 ? nullptr : splitView(h, delta);
     tBegin(h);
     if (V > 0) memberCsr(h);
     for (int32_t i = 0; i < k; ++i) {
@@ -1074,7 +1222,7 @@ static void twoLevelMany(girhip_graph* h, const int32_t* src, int32_t k, uint64_
         }
         reachSweep(h, x);
         if (try32) {
-            waveQuery(h, x, delta, unit, S.s_d32.p, true);
+            waveQuery(h, x, delta, unit, S.s_d32.p, true, sv);
             int64_t ovf;
             readCounters(h, S.s_ctl.p + C_OVF, 1, &ovf);
             if (ovf == 0) {
@@ -1089,7 +1237,7 @@ static void twoLevelMany(girhip_graph* h, const int32_t* src, int32_t k, uint64_
             // the pending counts were used up by the wavefront
             reachSweep(h, x);
         }
-        waveQuery(h, x, delta, false, row, false);
+        waveQuery(h, x, delta, false, row, false, sv);
     }
     tEnd(h, what);
     if (outp && !direct) copyOut(h, outp, out, esz * n);

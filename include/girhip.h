@@ -261,14 +261,22 @@ int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
  *   chunks are captured once and replayed as HIP graphs - measured no
  *   faster on MI355X, whose rounds are bound by the kernels, not by their
  *   launches; ignored while profiling),
+ *   "split" (1: weighted SSSP walks a split view of the forward rows, each
+ *   row's arcs of weight <= delta first, so the light rounds and the heavy
+ *   pass each read only their own arcs; built on the first call with a
+ *   given delta, kept for the two deltas used last, E x 8 bytes each,
+ *   rebuilt after the store or the weights change; 0: rows classify their
+ *   arcs on the fly),
  *   "profile" (0: 1 times every kernel)
  * Values only change work order; results never depend on them. */
 int girhip_set_param(girhip_graph* h, const char* name, double value);
 int girhip_get_param(const girhip_graph* h, const char* name, double* value);
 
 /* Device time (ms, HIP events) of the kernels the last call of each kind
- * launched: kind "wcc", "scc_dag", "bfs", "sssp", "batch", "reachable",
- * "reach_info", "bfs_two_level", "sssp_two_level", "bc", "query_batch". */
+ * launched: kind "wcc", "scc_dag", "bfs", "bfs_component", "sssp",
+ * "sssp_split" (the last build of a split view, outside "sssp"), "batch",
+ * "reachable", "reach_info", "bfs_two_level", "sssp_two_level", "bc",
+ * "bicc", "inc_update_bfs", "inc_update_sssp", "query_batch". */
 int girhip_last_ms(const girhip_graph* h, const char* kind, double* ms);
 
 /* Per-kernel device time when the "profile" parameter is 1: every launch is
