@@ -185,15 +185,19 @@ struct RelaxF {
             return 0;
         }
         if (nd < dGet(B, s, x)) {  // plain read: distances only fall, so a stale value is a safe pre-check
-            const uint64_t old = dMin(B, s, x, nd);
-            if (nd < old) {
-                if (nd < (uint64_t)*cp(B, C_HI, s)) {
-                    // the first bit of a word raises the flag: the near bitmap
-                    // is empty at the start of every round
-                    if (setBit(B.nearbits + (size_t)s * B.W, B.nsum + (size_t)s * B.W2, x)) *nearSet(B, par, s) = 1;
-                } else {
-                    setBit(B.farbits + (size_t)s * B.W, B.fsum + (size_t)s * B.W2, x);
-                }
+            // The atomicMin's old value is not awaited: every relaxation that
+            // lowers a distance passed this pre-check and marks x below, so
+            // marking on the pre-check alone loses nothing.  A relaxation that
+            // lost the race marks x too, which costs at most a redundant visit:
+            // a near mark makes the rounds visit x with its current distance,
+            // a far mark of a vertex settled lower is dropped by the split.
+            dMin(B, s, x, nd);
+            if (nd < (uint64_t)*cp(B, C_HI, s)) {
+                // the first bit of a word raises the flag: the near bitmap
+                // is empty at the start of every round
+                if (setBit(B.nearbits + (size_t)s * B.W, B.nsum + (size_t)s * B.W2, x)) *nearSet(B, par, s) = 1;
+            } else {
+                setBit(B.farbits + (size_t)s * B.W, B.fsum + (size_t)s * B.W2, x);
             }
         }
         return 0;
