@@ -228,19 +228,20 @@ __global__ void kMsFinishS(MsBits M, uint64_t* seen, uint64_t* front, uint64_t* 
 // together.  Used only when no row of the store can reach the heavy list.
 __global__ void __launch_bounds__(256) kMsPersist(MsPushSF f, const int32_t* __restrict__ fb, uint64_t* front,
                                                   int32_t* __restrict__ levels, int32_t V, MsCnt* ring, int64_t* Lio,
-                                                  int64_t E, int64_t alpha, int32_t Lmax, HeavyList H) {
+                                                  int64_t E, int64_t alpha, int32_t Lmax, HeavyList H,
+                                                  uint32_t* bar) {
     __shared__ MsLds sh;
     const int wv = threadIdx.x / WAVE;
     const MsBits& M = f.M;
     int32_t L = (int32_t)ldAcq(Lio);
-    while (L < Lmax) {
+    while (L < Lmax && !gridFailed(bar)) {
         MsCnt* cur = &ring[L % MS_RING];
         const int64_t nf = ldAcq(&cur->nf), mf = ldAcq(&cur->mf);
         if (nf == 0 || nf * 4 >= (int64_t)V || (alpha > 0 && mf * alpha > E)) break;
         MsCnt* nxt = &ring[(L + 1) % MS_RING];
         if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = MsCnt{0, 0};
         forSummary(M.FS, M.W2, 1, [&](int64_t g, uint64_t sm) { msPushGroup(fb, front, f, H, g, sm, sh.ids[wv]); });
-        gridSync();
+        gridSync(bar);
         int64_t a = 0, b = 0;
         forSummary(M.TS, M.W2, 1, [&](int64_t g, uint64_t sm) {
             msFinishGroup(M, const_cast<uint64_t*>(f.seen), front, f.next, levels, V, L, fb, g, sm, sh.ids[wv],
@@ -248,7 +249,7 @@ __global__ void __launch_bounds__(256) kMsPersist(MsPushSF f, const int32_t* __r
         });
         blockAtomicAdd(a, &nxt->nf);
         blockAtomicAdd(b, &nxt->mf);
-        gridSync();
+        gridSync(bar);
         ++L;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *Lio = L;
@@ -420,6 +421,7 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
                                  heavyList(h))) {
                 int64_t r[2 * MS_RING + 1];
                 readCounters(h, S.mscnt.p, 2 * MS_RING + 1, r);
+                checkGridBar(h);
                 L = (int32_t)r[2 * MS_RING];
                 nf = r[2 * (L % MS_RING)];
                 mf = r[2 * (L % MS_RING) + 1];

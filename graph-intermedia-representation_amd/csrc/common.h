@@ -1,7 +1,6 @@
 // Internal header of the gfx950 GIR library: the handle, device buffers,
 // error handling, kernel launch wrapper and wave64 helpers.
 #pragma once
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -164,6 +163,16 @@ struct Scratch {
 
 constexpr int HCNT_SLOTS = 256;
 
+namespace gir {
+// Grid barrier words: eight group arrival counters, the top counter, the
+// generation and an error flag, each 4 KB apart (own cache line and memory
+// channel).
+constexpr int BAR_STRIDE = 1024;
+constexpr int BAR_TOP = 8 * BAR_STRIDE, BAR_GEN = 9 * BAR_STRIDE, BAR_ERR = 10 * BAR_STRIDE;
+constexpr int BAR_WORDS = BAR_ERR + 1;
+constexpr unsigned BAR_SPIN = 1u << 22;  // polls (about a microsecond each) before a wait gives up
+}  // namespace gir
+
 struct girhip_graph {
     int device = 0;
     int n_cu = 0;        // compute units of the device
@@ -227,6 +236,7 @@ struct girhip_graph {
     // host-pinned counters for small readbacks (HCNT_SLOTS entries)
     int64_t* hcnt = nullptr;
     gir::DevBuf<int64_t> dcnt;  // device counters (64 slots)
+    gir::DevBuf<uint32_t> gbar;  // grid barrier of the persistent kernels (gir::BAR_WORDS)
 
     gir::Params prm;
     std::map<std::string, double> last_ms;
@@ -332,11 +342,18 @@ struct GraphCache {
 // safe).  Returns false, launching nothing, where cooperative launch is
 // unavailable or the kernel cannot keep that many blocks resident.
 bool persistentGrid(girhip_graph* h, const void* kernel, int block, int blocks_per_cu, unsigned* grid);
+void fillU32(girhip_graph* h, uint32_t* p, uint32_t v, int64_t n);
+// The kernel takes the grid barrier's words (uint32_t*, cleared here before
+// every launch) as its last argument, after args; the caller checks the
+// barrier with checkGridBar after reading the kernel's results back.
 template <class K, class... Args>
 bool launchPersistent(girhip_graph* h, const char* name, K kernel, int block, int blocks_per_cu, Args... args) {
     unsigned grid = 0;
     if (!persistentGrid(h, (const void*)kernel, block, blocks_per_cu, &grid)) return false;
-    void* ptrs[] = {(void*)&args...};
+    h->gbar.ensure(BAR_WORDS);
+    fillU32(h, h->gbar.p, 0, BAR_WORDS);
+    uint32_t* bar = h->gbar.p;
+    void* ptrs[] = {(void*)&args..., (void*)&bar};
     hipEvent_t a = nullptr;
     if (h->prm.profile != 0) {
         a = takeEvent(h);
@@ -351,12 +368,9 @@ bool launchPersistent(girhip_graph* h, const char* name, K kernel, int block, in
     return true;
 }
 
-// Grid-wide barrier of a persistent kernel, then an agent-scope acquire so
-// no stale cached line survives it.
-__device__ __forceinline__ void gridSync() {
-    cooperative_groups::this_grid().sync();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
+// Raises GIRHIP_EHIP if a persistent kernel's barrier gave up waiting
+// (call after the stream has synchronised on the kernel).
+void checkGridBar(girhip_graph* h);
 
 void syncStream(girhip_graph* h);
 // Begin/end an entry point's timed region (HIP events on the handle's stream).
@@ -409,6 +423,51 @@ inline Scratch& scratch(girhip_graph* h) { return h->ws; }
 
 // ---- device helpers ----
 namespace gir {
+// Grid-wide barrier of a persistent kernel (every block resident: the
+// launch is cooperative).  Each wave first waits for its own stores to
+// complete, so after the block barrier they are all in this XCD's L2; thread
+// 0 then writes the L2 back (agent-scope release) and arrives: block b on
+// group counter b % 8, the last of a group on the top counter, and the last
+// of those publishes the next generation, which the others poll with plain
+// device-coherent loads (no cache maintenance per poll).  A block leaves
+// with an agent-scope acquire (L2 and L1 invalidated), so no stale line
+// survives the barrier.  Counters only grow (cleared before each launch):
+// in barrier g the last arriver is the one that brings a counter to (g + 1)
+// times its arrivals.  A wait longer than BAR_SPIN polls, or one that finds
+// the error flag raised, raises the flag and leaves, so a broken barrier
+// ends the kernel instead of hanging the device (the host then reports it).
+__device__ __forceinline__ void gridSync(uint32_t* bar) {
+    __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0): this wave's stores have completed
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned nb = gridDim.x, grp = blockIdx.x & 7;
+        const unsigned ng = nb / 8 + (grp < nb % 8 ? 1u : 0u), ntop = nb < 8 ? nb : 8;
+        uint32_t* gen = &bar[BAR_GEN];
+        const unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned a =
+            __hip_atomic_fetch_add(&bar[grp * BAR_STRIDE], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+        if (a == (g + 1) * ng) {
+            const unsigned t = __hip_atomic_fetch_add(&bar[BAR_TOP], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+            if (t == (g + 1) * ntop) __hip_atomic_store(gen, g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        for (unsigned it = 0; __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g; ++it) {
+            if (it >= BAR_SPIN || __hip_atomic_load(&bar[BAR_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_store(&bar[BAR_ERR], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+// True once a barrier wait has given up: persistent loops test it at their
+// head and leave (the results are then discarded by the host's check).
+__device__ __forceinline__ bool gridFailed(const uint32_t* bar) {
+    return __hip_atomic_load(&bar[BAR_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
 __device__ __forceinline__ int laneId() { return (int)__lane_id(); }
 __device__ __forceinline__ uint64_t lanesBelow() { return (1ULL << laneId()) - 1ULL; }
 

@@ -330,6 +330,13 @@ bool persistentGrid(girhip_graph* h, const void* kernel, int block, int blocks_p
     return true;
 }
 
+void checkGridBar(girhip_graph* h) {
+    uint32_t err = 0;
+    HIPCHK(hipMemcpyAsync(&err, h->gbar.p + BAR_ERR, sizeof(err), hipMemcpyDeviceToHost, h->st));
+    syncStream(h);
+    if (err) raise(GIRHIP_EHIP, "persistent kernel: a grid barrier wait gave up");
+}
+
 int64_t maxWeight(girhip_graph* h) {
     if (!h->has_w) return 100;
     if (h->max_w < 0) {

@@ -194,13 +194,14 @@ static_assert(REACH_RING >= REACH_CHUNK + 3, "ring must hold a chunk and its nei
 template <class Flt, class Vis>
 __global__ void __launch_bounds__(256) kReachPersist(const int32_t* __restrict__ pbeg, const int32_t* __restrict__ padj,
                                                      uint64_t* vis, int32_t* q0, int32_t* q1, int64_t* ring,
-                                                     int64_t* io, int64_t alpha, Flt flt, Vis on, HeavyList H) {
+                                                     int64_t* io, int64_t alpha, Flt flt, Vis on, HeavyList H,
+                                                     uint32_t* bar) {
     int32_t L = (int32_t)ldAcq(&io[0]);
     int cur = (int)ldAcq(&io[1]);
     int64_t reached = ldAcq(&io[2]), mun = ldAcq(&io[3]);
     auto pair = [&](int64_t l) { return ring + 2 * ((l % REACH_RING + REACH_RING) % REACH_RING); };
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    while (true) {
+    while (!gridFailed(bar)) {
         const int64_t* pin = pair(L - 1);
         const int64_t nf = ldAcq(&pin[0]), mf = ldAcq(&pin[1]);
         if (nf == 0 || (alpha > 0 && mf * alpha > mun)) break;
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(256) kReachPersist(const int32_t* __restrict__
             m += waveRowsH(act, u, 0, 0, b, e, H, f);
         }
         blockAtomicAdd(m, &cn[1]);
-        gridSync();
+        gridSync(bar);
         reached += ldAcq(&cn[0]);
         mun -= ldAcq(&cn[1]);
         cur ^= 1;
@@ -303,6 +304,7 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
                                  c, io, a, flt, on, heavyList(h))) {
                 int64_t r[2 * REACH_RING + 4];
                 readCounters(h, c, 2 * REACH_RING + 4, r);
+                checkGridBar(h);
                 const int64_t* o = r + 2 * REACH_RING;
                 L = (int32_t)o[0];
                 cur = (int)o[1];

@@ -518,12 +518,12 @@ __global__ void kBSplit(SsspBatch B, int par, uint64_t delta, int G) {
 // when no row of the store can reach the heavy list.
 __global__ void __launch_bounds__(256) kSsspPersist(RelaxF fl, RelaxF fh, const int32_t* __restrict__ fb, HeavyList H,
                                                     int heavy, uint64_t delta, int G, int64_t cap, int32_t max_rounds,
-                                                    int64_t* io) {
+                                                    int64_t* io, uint32_t* bar) {
     __shared__ FarLds sh;
     const SsspBatch& B = fl.B;
     const int k = B.k;
     int par = (int)ldAcq(io);
-    for (int32_t r = 0; r < max_rounds; ++r) {
+    for (int32_t r = 0; r < max_rounds && !gridFailed(bar); ++r) {
         int64_t mx = 0;
         for (int s = 0; s < k; ++s) mx = std::max<int64_t>(mx, ldAcq(nearCount(B, par, s)));
         if (mx == 0 || mx > cap) break;
@@ -532,18 +532,18 @@ __global__ void __launch_bounds__(256) kSsspPersist(RelaxF fl, RelaxF fh, const 
         fh.par = par;
         for (int s = 0; s < k; ++s) relaxQueue(fl, fb, H, B.qn[par] + (size_t)s * B.V, ldAcq(nearCount(B, par, s)),
                                                s, heavy != 0, blockIdx.x, gridDim.x);
-        gridSync();
+        gridSync(bar);
         for (int s = 0; s < k; ++s) compactPhase(B, par, heavy, G, s, sh, blockIdx.x, gridDim.x);
-        gridSync();
+        gridSync(bar);
         if (heavy) {
             for (int s = 0; s < k; ++s) relaxQueue(fh, fb, H, B.qh + (size_t)s * B.V, ldAcq(cp(B, C_HQ, s)), s,
                                                    false, blockIdx.x, gridDim.x);
-            gridSync();
+            gridSync(bar);
             for (int s = 0; s < k; ++s) farMinPhase(B, par, G, s, sh, blockIdx.x, gridDim.x);
-            gridSync();
+            gridSync(bar);
         }
         for (int s = 0; s < k; ++s) splitPhase(B, par, delta, G, s, sh, blockIdx.x, gridDim.x);
-        gridSync();
+        gridSync(bar);
         par ^= 1;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *io = par;
@@ -692,6 +692,7 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
                                  4, cap, (int32_t)0x7fffffff, io)) {
                 int64_t pr;
                 readCounters(h, io, 1, &pr);
+                checkGridBar(h);
                 par = (int)pr;
                 readSlot(h, S.s_ctl.p, par ? C_NEAR1 : C_NEAR0, r.data());
                 int64_t mx = 0;
