@@ -691,6 +691,43 @@ __device__ __forceinline__ int64_t blockReserve(int n, int64_t* cnt) {
     return off;
 }
 
+// blockReserve of one slot per thread with pred, together with the block's
+// sum of m: one atomicAdd of (slots << 32) + sum(m) on *packed, so a launch
+// over millions of threads hits its counter once per block (same-address
+// atomics serialise at the memory side).  Both halves must stay below 2^32
+// over the counter's life.  Returns this thread's slot (valid when pred).
+// Every thread of the block must call it.
+__device__ __forceinline__ int64_t blockReservePacked(bool pred, int64_t m, int64_t* packed) {
+    __shared__ int64_t wcnt[1024 / WAVE], wsum[1024 / WAVE];
+    __shared__ int64_t base;
+    const int lane = laneId(), w = threadIdx.x / WAVE, nw = (blockDim.x + WAVE - 1) / WAVE;
+    const uint64_t bal = __ballot(pred);
+    int64_t ms = m;
+    for (int d = WAVE / 2; d > 0; d >>= 1) ms += __shfl_xor(ms, d);
+    if (lane == 0) {
+        wcnt[w] = __popcll(bal);
+        wsum[w] = ms;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0, sm = 0;
+        for (int i = 0; i < nw; ++i) {
+            const int64_t x = wcnt[i];
+            wcnt[i] = t;
+            t += x;
+            sm += wsum[i];
+        }
+        base = (t || sm) ? (int64_t)(atomicAdd((unsigned long long*)packed,
+                                               ((unsigned long long)t << 32) + (unsigned long long)sm) >>
+                                     32)
+                         : 0;
+    }
+    __syncthreads();
+    const int64_t off = base + wcnt[w] + __popcll(bal & lanesBelow());
+    __syncthreads();
+    return off;
+}
+
 // Wave-wide exclusive scan of n per lane; reserves the wave's range with one
 // atomicAdd on *cnt.  Every lane of the wave must call it.
 __device__ __forceinline__ int64_t waveReserve(int n, int64_t* cnt) {

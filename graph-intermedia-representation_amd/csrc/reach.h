@@ -143,14 +143,14 @@ __global__ void kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __res
         nbits[word] = nb;
         if (nb) vis[word] = w0 | nb;
     }
-    int64_t m = 0;
-    const int64_t pos = wavePush(&cnt[0], found);
+    // cnt[0] gets (found << 32) + their push-arc count, one atomic per block;
+    // the host splits it into the level's (nf, mf) pair
+    const int64_t m = found ? pbeg[v + 1] - pbeg[v] : 0;
+    const int64_t pos = blockReservePacked(found, m, &cnt[0]);
     if (found) {
         qout[pos] = (int32_t)v;
         on((int32_t)v, L + 1);
-        m = pbeg[v + 1] - pbeg[v];
     }
-    blockAtomicAdd(m, &cnt[1]);
 }
 
 static __global__ void kQueueToBits(const int32_t* __restrict__ q, int64_t n, uint64_t* bits) {
@@ -283,8 +283,10 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
             bits_valid = true;
             int64_t r[2];
             readCounters(h, cn, 2, r);
-            nf = r[0];
-            mf = r[1];
+            nf = (int64_t)((uint64_t)r[0] >> 32);
+            mf = (int64_t)((uint64_t)r[0] & 0xffffffffULL);
+            fillI64(h, cn, nf, 1);  // the level's pair as the other kernels read it
+            fillI64(h, cn + 1, mf, 1);
             m_unexplored -= mf;
             st.reached += nf;
             cur ^= 1;
