@@ -728,6 +728,22 @@ __device__ __forceinline__ int64_t blockReservePacked(bool pred, int64_t m, int6
     return off;
 }
 
+// Block-wide maximum (min: pass the bitwise complement) of every thread's
+// x, returned to every thread.  Every thread of the block must call it.
+__device__ __forceinline__ uint64_t blockMaxU64(uint64_t x) {
+    __shared__ uint64_t part[1024 / WAVE];
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        const uint64_t o = __shfl_xor((unsigned long long)x, d);
+        x = o > x ? o : x;
+    }
+    const int w = threadIdx.x / WAVE, nw = (blockDim.x + WAVE - 1) / WAVE;
+    if (laneId() == 0) part[w] = x;
+    __syncthreads();
+    for (int i = 0; i < nw; ++i) x = part[i] > x ? part[i] : x;
+    __syncthreads();
+    return x;
+}
+
 // Wave-wide exclusive scan of n per lane; reserves the wave's range with one
 // atomicAdd on *cnt.  Every lane of the wave must call it.
 __device__ __forceinline__ int64_t waveReserve(int n, int64_t* cnt) {

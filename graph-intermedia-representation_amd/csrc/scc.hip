@@ -46,17 +46,34 @@ struct SameColour {
     __device__ bool self(int32_t v) const { return scc[v] == -1; }
 };
 
+// List kernels take CA_ITEMS entries per thread (coalesced, blockDim apart)
+// and reserve their output once per block: a list of millions then costs a
+// few thousand atomics on its counter instead of one per block of 256.
+constexpr int CA_ITEMS = 8;
+__host__ __device__ constexpr int64_t caSpan(int block) { return (int64_t)block * CA_ITEMS; }
+
 __global__ void kCompactActive(const int32_t* __restrict__ in, int64_t n, bool all, const int32_t* __restrict__ scc,
                                int32_t* __restrict__ out, int64_t* cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int32_t v = 0;
-    bool keep = false;
-    if (i < n) {
-        v = all ? (int32_t)i : in[i];
-        keep = scc[v] == -1;
+    const int64_t i0 = (int64_t)blockIdx.x * caSpan(blockDim.x) + threadIdx.x;
+    int32_t v[CA_ITEMS];
+    int k = 0;
+    unsigned keep = 0;
+#pragma unroll
+    for (int t = 0; t < CA_ITEMS; ++t) {
+        const int64_t i = i0 + (int64_t)t * blockDim.x;
+        v[t] = 0;
+        if (i < n) {
+            v[t] = all ? (int32_t)i : in[i];
+            if (scc[v[t]] == -1) {
+                keep |= 1u << t;
+                ++k;
+            }
+        }
     }
-    const int64_t pos = blockReserve(keep, cnt);
-    if (keep) out[pos] = v;
+    int64_t pos = blockReserve(k, cnt);
+#pragma unroll
+    for (int t = 0; t < CA_ITEMS; ++t)
+        if ((keep >> t) & 1) out[pos++] = v[t];
 }
 
 __device__ __forceinline__ bool anyActive(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj,
@@ -81,44 +98,41 @@ __global__ void kTrim1(const int32_t* __restrict__ list, int64_t n, const int32_
             if (trim) stRel(&scc[v], v);
         }
     }
-    const uint64_t m = __ballot(trim);
-    if (m && laneId() == __ffsll((long long)m) - 1) atomicAdd((unsigned long long*)cnt, (unsigned long long)__popcll(m));
+    // only "any trimmed" matters to the host: a plain store of 1 per block
+    // that trimmed (every writer stores the same value; the kernel's end makes
+    // it visible), not a contended atomic per wave
+    if (__syncthreads_or(trim) && threadIdx.x == 0) *cnt = 1;
 }
 
 // argmax over the active list of min(outdeg * indeg, 2^33 - 1), ties to the
 // smaller vertex id: key = (score << 31) | (2^31 - 1 - v).
+// (grid-stride over a bounded grid; one atomic per block)
 __global__ void kPivot(const int32_t* __restrict__ list, int64_t n, const int32_t* __restrict__ fb,
                        const int32_t* __restrict__ bb, unsigned long long* best) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long key = 0;
-    if (i < n) {
+    uint64_t key = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t v = list[i];
         unsigned long long s = (unsigned long long)(fb[v + 1] - fb[v]) * (unsigned long long)(bb[v + 1] - bb[v]);
         if (s > (1ULL << 33) - 1) s = (1ULL << 33) - 1;
-        key = (s << 31) | (unsigned long long)(0x7fffffff - v);
+        const uint64_t k = (s << 31) | (unsigned long long)(0x7fffffff - v);
+        key = k > key ? k : key;
     }
-    for (int d = WAVE / 2; d > 0; d >>= 1) {
-        unsigned long long o = __shfl_xor(key, d);
-        key = o > key ? o : key;
-    }
-    if (laneId() == 0) atomicMax(best, key);
+    key = blockMaxU64(key);
+    if (threadIdx.x == 0 && key) atomicMax(best, (unsigned long long)key);
 }
 
 __device__ __forceinline__ bool bitOf(const uint64_t* b, int32_t v) { return (b[v >> 6] >> (v & 63)) & 1; }
 
+// (grid-stride over a bounded grid; one atomic per block)
 __global__ void kFwBwMin(const int32_t* __restrict__ list, int64_t n, const uint64_t* __restrict__ fwb,
                          const uint64_t* __restrict__ bwb, int64_t* minv) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t x = INT64_MAX;
-    if (i < n) {
+    uint64_t x = 0;  // complement of the minimum so far
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t v = list[i];
-        if (bitOf(fwb, v) && bitOf(bwb, v)) x = v;
+        if (bitOf(fwb, v) && bitOf(bwb, v)) x = ~(uint64_t)v > x ? ~(uint64_t)v : x;
     }
-    for (int d = WAVE / 2; d > 0; d >>= 1) {
-        int64_t o = __shfl_xor(x, d);
-        x = o < x ? o : x;
-    }
-    if (laneId() == 0 && x != INT64_MAX) atomicMin((long long*)minv, (long long)x);
+    x = blockMaxU64(x);
+    if (threadIdx.x == 0 && x) atomicMin((long long*)minv, (long long)~x);
 }
 
 __global__ void kFwBwAssign(const int32_t* __restrict__ list, int64_t n, const uint64_t* __restrict__ fwb,
@@ -301,14 +315,24 @@ __global__ void kDagBeg(const int32_t* __restrict__ src, int64_t nd, int32_t V, 
 // ---- ranks (Kahn levels) ----
 __global__ void kRankSeeds(const int32_t* __restrict__ scc, const int32_t* __restrict__ indeg, int32_t V,
                            int64_t* rank, int32_t* __restrict__ out, int64_t* cnt) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool seed = false;
-    if (v < V) {
-        rank[v] = -1;
-        seed = scc[v] == (int32_t)v && indeg[v] == 0;
+    const int64_t v0 = (int64_t)blockIdx.x * caSpan(blockDim.x) + threadIdx.x;
+    unsigned seed = 0;
+    int k = 0;
+#pragma unroll
+    for (int t = 0; t < CA_ITEMS; ++t) {
+        const int64_t v = v0 + (int64_t)t * blockDim.x;
+        if (v < V) {
+            rank[v] = -1;
+            if (scc[v] == (int32_t)v && indeg[v] == 0) {
+                seed |= 1u << t;
+                ++k;
+            }
+        }
     }
-    const int64_t pos = blockReserve(seed, cnt);
-    if (seed) out[pos] = (int32_t)v;
+    int64_t pos = blockReserve(k, cnt);
+#pragma unroll
+    for (int t = 0; t < CA_ITEMS; ++t)
+        if ((seed >> t) & 1) out[pos++] = (int32_t)(v0 + (int64_t)t * blockDim.x);
 }
 
 // Kahn step work on DAG arc j: one in-degree decrement of its head; the
@@ -365,8 +389,8 @@ namespace {
 int64_t compactActive(girhip_graph* h, const int32_t* in, int64_t n, bool all, int32_t* out) {
     const int blk = (int)h->prm.block;
     fillI64(h, h->dcnt.p + 2, 0, 1);
-    launch(h, "scc_compact", kCompactActive, gridFor(n, blk), blk, 0, in, n, all, (const int32_t*)h->scc.p, out,
-           h->dcnt.p + 2);
+    launch(h, "scc_compact", kCompactActive, (unsigned)((n + caSpan(blk) - 1) / caSpan(blk)), blk, 0, in, n, all,
+           (const int32_t*)h->scc.p, out, h->dcnt.p + 2);
     int64_t c;
     readCounters(h, h->dcnt.p + 2, 1, &c);
     return c;
@@ -417,8 +441,8 @@ void runScc(girhip_graph* h) {
     // FW-BW from the pivot
     if (n > 0 && h->prm.scc_pivot != 0 && n > (int64_t)h->prm.scc_color_min) {
         fillI64(h, h->dcnt.p + 4, 0, 1);
-        launch(h, "scc_pivot", kPivot, gridFor(n, blk), blk, 0, (const int32_t*)list, n, (const int32_t*)h->fw_beg.p,
-               (const int32_t*)h->bw_beg.p, (unsigned long long*)(h->dcnt.p + 4));
+        launch(h, "scc_pivot", kPivot, boundedGrid(n, blk), blk, 0, (const int32_t*)list, n,
+               (const int32_t*)h->fw_beg.p, (const int32_t*)h->bw_beg.p, (unsigned long long*)(h->dcnt.p + 4));
         int64_t key;
         readCounters(h, h->dcnt.p + 4, 1, &key);
         const int32_t pivot = (int32_t)(0x7fffffff - (key & 0x7fffffffLL));
@@ -431,7 +455,7 @@ void runScc(girhip_graph* h) {
         fillI32(h, B.q[0], pivot, 1);
         runReach(h, h->bw_beg.p, h->bw_adj.p, h->fw_beg.p, h->fw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
         fillI64(h, h->dcnt.p + 5, INT64_MAX, 1);
-        launch(h, "scc_fwbw_min", kFwBwMin, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
+        launch(h, "scc_fwbw_min", kFwBwMin, boundedGrid(n, blk), blk, 0, (const int32_t*)list, n,
                (const uint64_t*)S.bits[3].p, (const uint64_t*)B.vis, h->dcnt.p + 5);
         launch(h, "scc_fwbw_assign", kFwBwAssign, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
                (const uint64_t*)S.bits[3].p, (const uint64_t*)B.vis, (const int64_t*)h->dcnt.p + 5, h->scc.p);
@@ -547,7 +571,7 @@ void runDag(girhip_graph* h) {
     readCounters(h, h->dcnt.p + 12, 1, &hubk);
     const int32_t hub = hubk ? (int32_t)(hubk & 0xffffffffLL) : -1;
     fillI64(h, h->dcnt.p + 10, 0, 1);
-    launch(h, "rank_seeds", kRankSeeds, gridFor(V, blk), blk, 0, (const int32_t*)h->scc.p, (const int32_t*)ideg, V,
+    launch(h, "rank_seeds", kRankSeeds, (unsigned)(((int64_t)V + caSpan(blk) - 1) / caSpan(blk)), blk, 0, (const int32_t*)h->scc.p, (const int32_t*)ideg, V,
            h->rank.p, R.q[0], h->dcnt.p + 10);
     int64_t nq;
     readCounters(h, h->dcnt.p + 10, 1, &nq);
