@@ -277,6 +277,7 @@ __global__ void kMsPush(const int32_t* __restrict__ fb, const uint64_t* __restri
     waveRowsH(act, (int32_t)v, fr, 0, b, e, H, f);
 }
 
+constexpr int PULL_BATCH = 8;
 __global__ void kMsPull(const int32_t* __restrict__ bb, const int32_t* __restrict__ ba,
                         const uint64_t* __restrict__ front, const uint64_t* __restrict__ seen,
                         uint64_t* __restrict__ next, int32_t V, uint64_t all, MsCnt* nxt) {
@@ -286,9 +287,18 @@ __global__ void kMsPull(const int32_t* __restrict__ bb, const int32_t* __restric
     const uint64_t sw = seen[w];
     uint64_t acc = 0;
     if (sw != all) {
+        // PULL_BATCH in-arcs at a time, all their loads issued together: one
+        // memory latency per batch instead of a dependent chain per arc
         const int32_t b = bb[w], e = bb[w + 1];
-        for (int32_t j = b; j < e; ++j) {
-            acc |= front[ba[j]];
+        for (int32_t j0 = b; j0 < e; j0 += PULL_BATCH) {
+            int32_t p[PULL_BATCH];
+            uint64_t fw[PULL_BATCH];
+#pragma unroll
+            for (int k = 0; k < PULL_BATCH; ++k) p[k] = j0 + k < e ? ba[j0 + k] : -1;
+#pragma unroll
+            for (int k = 0; k < PULL_BATCH; ++k) fw[k] = p[k] >= 0 ? front[p[k]] : 0;
+#pragma unroll
+            for (int k = 0; k < PULL_BATCH; ++k) acc |= fw[k];
             if ((acc | sw) == all) break;
         }
     }
@@ -319,8 +329,9 @@ __global__ void kMsFinish(uint64_t* seen, uint64_t* front, uint64_t* next, int32
         M.F[word] = fw;
         if (fw) atomicOr((unsigned long long*)&M.FS[word >> 6], 1ULL << (word & 63));
     }
-    blockAtomicAdd(nb != 0, &nxt->nf);
-    blockAtomicAdd(nb != 0 ? (int64_t)(fb[w + 1] - fb[w]) : 0, &nxt->mf);
+    // nxt->nf gets (new vertices << 32) + their push-arc count, one atomic
+    // per block; the host splits it into (nf, mf)
+    blockReservePacked(nb != 0, nb != 0 ? (int64_t)(fb[w + 1] - fb[w]) : 0, &nxt->nf);
 }
 
 __global__ void kMsUnreached(const uint64_t* __restrict__ seen, int32_t* __restrict__ levels, int32_t V,
@@ -405,8 +416,10 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
             launch(h, "msbfs_finish", kMsFinish, g, blk, 0, seen, front, next, levels, V, L, fb, M, nxt);
             int64_t r[2];
             readCounters(h, (const int64_t*)nxt, 2, r);
-            nf = r[0];
-            mf = r[1];
+            nf = (int64_t)((uint64_t)r[0] >> 32);
+            mf = (int64_t)((uint64_t)r[0] & 0xffffffffULL);
+            fillI64(h, &nxt->nf, nf, 1);  // the level's pair as the other kernels read it
+            fillI64(h, &nxt->mf, mf, 1);
             mf_prev = 0;
             ++L;
             continue;
