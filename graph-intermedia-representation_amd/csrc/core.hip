@@ -281,12 +281,13 @@ __global__ void kRowOfSlot(const int32_t* __restrict__ beg, int32_t V, int32_t* 
     waveRows(act, (int32_t)v, b, e, [&](int32_t u, int64_t j) { src[j] = u; });
 }
 
+// (bounded grids; one atomic per block)
 __global__ void kMaxRow(const int32_t* __restrict__ beg, int32_t V, int64_t* out) {
-    int64_t m = 0;
+    uint64_t m = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
-        m = std::max<int64_t>(m, beg[v + 1] - beg[v]);
-    for (int o = WAVE / 2; o > 0; o >>= 1) m = std::max<int64_t>(m, __shfl_xor(m, o));
-    if (laneId() == 0 && m) atomicMax((unsigned long long*)out, (unsigned long long)m);
+        m = std::max<uint64_t>(m, (uint64_t)(beg[v + 1] - beg[v]));
+    m = blockMaxU64(m);
+    if (threadIdx.x == 0 && m) atomicMax((unsigned long long*)out, (unsigned long long)m);
 }
 
 int64_t maxDegree(girhip_graph* h) {
@@ -301,11 +302,11 @@ int64_t maxDegree(girhip_graph* h) {
 }
 
 __global__ void kMaxWeight(const uint32_t* __restrict__ w, int64_t E, int64_t* out) {
-    uint32_t m = 0;
+    uint64_t m = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E; i += (int64_t)gridDim.x * blockDim.x)
-        m = std::max(m, w[i]);
-    for (int o = WAVE / 2; o > 0; o >>= 1) m = std::max(m, (uint32_t)__shfl_xor(m, o));
-    if (laneId() == 0 && m) atomicMax((unsigned long long*)out, (unsigned long long)m);
+        m = std::max<uint64_t>(m, w[i]);
+    m = blockMaxU64(m);
+    if (threadIdx.x == 0 && m) atomicMax((unsigned long long*)out, (unsigned long long)m);
 }
 
 bool persistentGrid(girhip_graph* h, const void* kernel, int block, int blocks_per_cu, unsigned* grid) {

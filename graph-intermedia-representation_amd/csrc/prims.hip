@@ -175,8 +175,61 @@ __global__ void kRadixScatter(const uint64_t* __restrict__ kin, const uint32_t* 
     }
 }
 
+// Arrays of at most SS_N pairs (update batches of 10^3 ops, small DAG
+// changes) are sorted by one block in LDS instead of key_bits / 8 radix
+// passes of several launches each: a bitonic network over (masked key,
+// original position), so the order is the stable one the radix sort gives.
+constexpr int SS_N = 2048, SS_T = 1024;
+__global__ void __launch_bounds__(SS_T) kSmallSort(uint64_t* keys, uint32_t* vals, int n, uint64_t mask) {
+    __shared__ uint64_t sk[SS_N];
+    __shared__ uint32_t sv[SS_N];
+    __shared__ uint32_t si[SS_N];
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        sk[i] = i < n ? keys[i] : ~0ULL;
+        sv[i] = (i < n && vals) ? vals[i] : 0;
+        si[i] = (uint32_t)i;
+    }
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    // padding (position >= n) sorts after every real entry
+                    const uint64_t a = si[i] < (uint32_t)n ? sk[i] & mask : ~0ULL;
+                    const uint64_t b = si[l] < (uint32_t)n ? sk[l] & mask : ~0ULL;
+                    const bool gt = a > b || (a == b && si[i] > si[l]);
+                    if (gt == ((i & k) == 0)) {
+                        const uint64_t tk = sk[i];
+                        sk[i] = sk[l];
+                        sk[l] = tk;
+                        const uint32_t tv = sv[i];
+                        sv[i] = sv[l];
+                        sv[l] = tv;
+                        const uint32_t ti = si[i];
+                        si[i] = si[l];
+                        si[l] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        keys[i] = sk[i];
+        if (vals) vals[i] = sv[i];
+    }
+}
+
 void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits) {
     if (n <= 1) return;
+    if (n <= SS_N) {
+        const uint64_t mask = key_bits >= 64 ? ~0ULL : (1ULL << key_bits) - 1;
+        launch(h, "small_sort", kSmallSort, 1, SS_T, 0, keys, vals, (int)n, mask);
+        return;
+    }
     Scratch& S = scratch(h);
     const int nb = (int)((n + RS_TILE - 1) / RS_TILE);
     S.keys[1].ensure(n);

@@ -370,8 +370,8 @@ __global__ void kHub(const int32_t* __restrict__ indeg, int32_t V, int64_t* out)
     unsigned long long m = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
         m = std::max<unsigned long long>(m, ((unsigned long long)(uint32_t)indeg[v] << 32) | (uint32_t)v);
-    for (int o = WAVE / 2; o > 0; o >>= 1) m = std::max<unsigned long long>(m, __shfl_xor(m, o));
-    if (laneId() == 0 && m) atomicMax((unsigned long long*)out, m);
+    m = blockMaxU64(m);
+    if (threadIdx.x == 0 && m) atomicMax((unsigned long long*)out, m);
 }
 
 __global__ void kKahnStep(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ dbeg,
