@@ -207,12 +207,45 @@ __global__ void kColorAssign(const int32_t* __restrict__ list, int64_t n, const 
 
 // ---- DAG ----
 // Edge-parallel: DK_ITEMS coalesced slots per thread and block step, one
-// reservation per block step for the crossing arcs' packed keys.
+// reservation per block step for the crossing arcs' packed keys.  A slot's
+// row (the arc's tail) comes from the block's window of row starts and
+// labels staged in LDS (a span of 4096 slots covers a few hundred rows), so
+// no per-slot row array has to be built after every change of the store.
 constexpr int DK_ITEMS = 16;
-__global__ void kDagKeysEdge(const int32_t* __restrict__ src, const int32_t* __restrict__ adj, int64_t E,
+constexpr int DK_ROWS = 2048;  // rows a span may cover through LDS (else global searches)
+
+// last r in [lo, hi] with beg[r] <= x (beg[lo] <= x)
+__device__ __forceinline__ int32_t rowOf(const int32_t* beg, int32_t lo, int32_t hi, int64_t x) {
+    while (lo < hi) {
+        const int32_t m = lo + ((hi - lo + 1) >> 1);
+        if (beg[m] <= x) lo = m;
+        else hi = m - 1;
+    }
+    return lo;
+}
+
+__global__ void kDagKeysEdge(const int32_t* __restrict__ beg, int32_t V, const int32_t* __restrict__ adj, int64_t E,
                              const int32_t* __restrict__ scc, int B, uint64_t* __restrict__ keys, int64_t* cnt) {
+    __shared__ int32_t rb[DK_ROWS + 1];
+    __shared__ int32_t rl[DK_ROWS];
+    __shared__ int32_t win[2];
     const int64_t span = (int64_t)blockDim.x * DK_ITEMS;
     for (int64_t base = (int64_t)blockIdx.x * span; base < E; base += (int64_t)gridDim.x * span) {
+        const int64_t last = (base + span < E ? base + span : E) - 1;
+        if (threadIdx.x == 0) {
+            const int32_t r0 = rowOf(beg, 0, V - 1, base);
+            win[0] = r0;
+            win[1] = rowOf(beg, r0, V - 1, last) - r0 + 1;
+        }
+        __syncthreads();
+        const int32_t r0 = win[0], nr = win[1];
+        const bool staged = nr <= DK_ROWS;
+        if (staged)
+            for (int i = threadIdx.x; i <= nr; i += blockDim.x) {
+                rb[i] = beg[r0 + i];
+                if (i < nr) rl[i] = scc[r0 + i];
+            }
+        __syncthreads();
         uint64_t kv[DK_ITEMS];
         bool cr[DK_ITEMS];
         int n = 0;
@@ -222,7 +255,8 @@ __global__ void kDagKeysEdge(const int32_t* __restrict__ src, const int32_t* __r
             cr[t] = false;
             kv[t] = 0;
             if (e < E) {
-                const int32_t a = scc[src[e]], c = scc[adj[e]];
+                const int32_t a = staged ? rl[rowOf(rb, 0, nr - 1, e)] : scc[rowOf(beg, r0, r0 + nr - 1, e)];
+                const int32_t c = scc[adj[e]];
                 cr[t] = a != c;
                 kv[t] = ((uint64_t)(uint32_t)a << B) | (uint32_t)c;
             }
@@ -519,10 +553,9 @@ void runDag(girhip_graph* h) {
     if (h->E > 0) {
         S.keys[0].ensure((size_t)h->E);
         fillI64(h, h->dcnt.p + 9, 0, 1);
-        const int32_t* src = fwSrc(h);
         const unsigned g = std::min<unsigned>(gridFor(h->E, blk * DK_ITEMS), 8192);
-        launch(h, "dag_keys", kDagKeysEdge, g, blk, 0, src, (const int32_t*)h->fw_adj.p, h->E,
-               (const int32_t*)h->scc.p, B, S.keys[0].p, h->dcnt.p + 9);
+        launch(h, "dag_keys", kDagKeysEdge, g, blk, 0, (const int32_t*)h->fw_beg.p, V, (const int32_t*)h->fw_adj.p,
+               h->E, (const int32_t*)h->scc.p, B, S.keys[0].p, h->dcnt.p + 9);
         readCounters(h, h->dcnt.p + 9, 1, &ncross);
     }
     int64_t nd = 0;
