@@ -444,6 +444,22 @@ def main():
     # ---- update batches (50% inserts of uniform pairs, 50% deletes of arcs) ----
     updates = None
     if not args.no_extras:
+        # warm start, as a resident service would be: one untimed batch of
+        # 10^3 distinct changes (deletes of present arcs, inserts of absent
+        # pairs), then its inverse, so the store's edge set is the original
+        # one again when the timed batches begin
+        wrng = np.random.default_rng(99)
+        cfb, cfa, _, _ = G.csr()
+        keys = (np.repeat(np.arange(V, dtype=np.int64), np.diff(cfb)) << 32) | cfa.astype(np.int64)
+        dk = np.unique(keys[wrng.integers(0, len(keys), 500)])
+        ck = np.unique((wrng.integers(0, V, 600).astype(np.int64) << 32) | wrng.integers(0, V, 600))
+        ik = ck[~np.isin(ck, keys) & ((ck >> 32) != (ck & 0xffffffff))][:500]
+        both = np.concatenate([dk, ik])
+        wu, wv = (both >> 32).astype(np.int32), (both & 0xffffffff).astype(np.int32)
+        wins = np.concatenate([np.zeros(len(dk), np.uint8), np.ones(len(ik), np.uint8)])
+        G.apply_batch(wu, wv, wins)
+        G.apply_batch(wu, wv, (1 - wins).astype(np.uint8))
+        del cfb, cfa, keys
         rng = np.random.default_rng(23)
         updates = []
         rows = None

@@ -41,15 +41,29 @@ struct SameScc {
     __device__ bool self(int32_t) const { return true; }
 };
 
-// one thread per vertex (grid covers V): canonical ids -> queue, their
-// closure degrees summed (one atomic per block each)
+// BR_ITEMS vertices per thread (coalesced, blockDim apart): canonical ids ->
+// queue, their closure degrees summed; one atomic per block on each counter
+constexpr int BR_ITEMS = 8;
 __global__ void kBiRoots(const int32_t* __restrict__ scc, const int32_t* __restrict__ sb, int32_t V, int32_t* q,
                          int64_t* cnt) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool root = v < V && scc[v] == (int32_t)v;
-    const int64_t pos = blockReserve(root ? 1 : 0, &cnt[0]);
-    if (root) q[pos] = (int32_t)v;
-    blockAtomicAdd(root ? sb[v + 1] - sb[v] : 0, &cnt[1]);
+    const int64_t v0 = (int64_t)blockIdx.x * blockDim.x * BR_ITEMS + threadIdx.x;
+    unsigned root = 0;
+    int n = 0;
+    int64_t m = 0;
+#pragma unroll
+    for (int t = 0; t < BR_ITEMS; ++t) {
+        const int64_t v = v0 + (int64_t)t * blockDim.x;
+        if (v < V && scc[v] == (int32_t)v) {
+            root |= 1u << t;
+            ++n;
+            m += sb[v + 1] - sb[v];
+        }
+    }
+    int64_t pos = blockReserve(n, &cnt[0]);
+#pragma unroll
+    for (int t = 0; t < BR_ITEMS; ++t)
+        if ((root >> t) & 1) q[pos++] = (int32_t)(v0 + (int64_t)t * blockDim.x);
+    blockAtomicAdd(m, &cnt[1]);
 }
 
 // parent = first closure neighbour in the same SCC one level up
@@ -73,11 +87,8 @@ __global__ void kBiParents(const int32_t* __restrict__ sb, const int32_t* __rest
         }
         par[v] = p;
     }
-    for (int d = WAVE / 2; d > 0; d >>= 1) {
-        const int32_t o = __shfl_xor(ml, d);
-        ml = o > ml ? o : ml;
-    }
-    if (laneId() == 0) atomicMax(maxlev, ml);
+    const uint64_t mb = blockMaxU64((uint64_t)(uint32_t)ml);  // one atomic per block
+    if (threadIdx.x == 0) atomicMax(maxlev, (int32_t)mb);
 }
 
 __global__ void kBiLevelKeys(const int32_t* __restrict__ lev, int32_t V, int B, uint64_t* __restrict__ keys) {
@@ -442,7 +453,7 @@ extern "C" int girhip_bicc(girhip_graph* h, int32_t* parent, int32_t* block, uin
         fillU64(h, R.vis, 0, R.W);
         fillI32(h, S.bi_lev.p, -1, V);
         fillI64(h, ctl, 0, 8);
-        launch(h, "bi_roots", kBiRoots, gridFor(V, blk), blk, 0, scc, sb, V, R.q[0], ctl);
+        launch(h, "bi_roots", kBiRoots, gridFor(V, blk * BR_ITEMS), blk, 0, scc, sb, V, R.q[0], ctl);
         int64_t c2[2];
         readCounters(h, ctl, 2, c2);
         runReach(h, sb, sa, sb, sa, R, c2[0], c2[1], (int64_t)V, m_total, SameScc{scc}, WriteLevel{S.bi_lev.p});
