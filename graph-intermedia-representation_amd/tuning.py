@@ -214,13 +214,20 @@ def evaluate(model_path, which=("rmat23", "grid4800"), n_src=8, reps=2):
         G.set_param("scc_trim_rounds", 3)
         sd, cd = run()
         ref_d, ref_s = dist.clone(), G.get_scc()[0].copy()
-        res = G.learned_tune(model, norm, src, judge=0)
+        res = G.learned_tune(model, norm, src, judge=0)  # the raw prediction
         sl, cl = run()
         same = bool(torch.equal(ref_d, dist)) and bool((G.get_scc()[0] == ref_s).all())
+        G.set_param("sssp_delta", 16)
+        G.set_param("scc_trim_rounds", 3)
+        rj = G.learned_tune(model, norm, src, judge=n_src)  # judged by the default on the same sources
+        sj, _ = run()
+        same = same and bool(torch.equal(ref_d, dist))
         rec = {"graph": name, "V": V, "E": len(fa), "sources": n_src, "learned_delta": res["delta"],
                "learned_trim": res["trim"], "model_used": res["used"], "sssp_ms_default": round(sd, 3),
                "sssp_ms_learned": round(sl, 3), "sssp_gain": round(sd / sl, 3), "scc_dag_ms_default": round(cd, 3),
-               "scc_dag_ms_learned": round(cl, 3), "scc_dag_gain": round(cd / cl, 3), "results_identical": same}
+               "scc_dag_ms_learned": round(cl, 3), "scc_dag_gain": round(cd / cl, 3),
+               "judged_delta": rj["delta"], "judged_kept_prediction": rj["used"], "sssp_ms_judged": round(sj, 3),
+               "sssp_gain_judged": round(sd / sj, 3), "results_identical": same}
         log(rec)
         out.append(rec)
         G.close()
