@@ -277,16 +277,37 @@ __global__ void kMsPush(const int32_t* __restrict__ fb, const uint64_t* __restri
     waveRowsH(act, (int32_t)v, fr, 0, b, e, H, f);
 }
 
+// *act = OR of every frontier word: the sources still expanding (bounded
+// grid, one atomic per block; *act cleared before the launch).
+__global__ void kMsActive(const uint64_t* __restrict__ front, int32_t V, unsigned long long* act) {
+    __shared__ uint64_t part[1024 / WAVE];
+    uint64_t m = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
+        m |= front[v];
+    for (int d = WAVE / 2; d > 0; d >>= 1) m |= __shfl_xor((unsigned long long)m, d);
+    if (laneId() == 0) part[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x + WAVE - 1) / WAVE; ++i) m |= part[i];
+        if (m) atomicOr(act, (unsigned long long)m);
+    }
+}
+
+// Pull level: a vertex collects its in-neighbours' frontier words until it
+// holds every source still expanding (*act): a source whose search has
+// stopped can add no bit, so sources of small reach (a random query source
+// with no out-arcs, say) do not force every vertex to scan its whole in-row.
 constexpr int PULL_BATCH = 8;
 __global__ void kMsPull(const int32_t* __restrict__ bb, const int32_t* __restrict__ ba,
                         const uint64_t* __restrict__ front, const uint64_t* __restrict__ seen,
-                        uint64_t* __restrict__ next, int32_t V, uint64_t all, MsCnt* nxt) {
+                        uint64_t* __restrict__ next, int32_t V, const uint64_t* __restrict__ actp, MsCnt* nxt) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = MsCnt{0, 0};
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= V) return;
+    const uint64_t all = *actp;
     const uint64_t sw = seen[w];
     uint64_t acc = 0;
-    if (sw != all) {
+    if ((sw & all) != all) {
         // PULL_BATCH in-arcs at a time, all their loads issued together: one
         // memory latency per batch instead of a dependent chain per arc
         const int32_t b = bb[w], e = bb[w + 1];
@@ -299,7 +320,7 @@ __global__ void kMsPull(const int32_t* __restrict__ bb, const int32_t* __restric
             for (int k = 0; k < PULL_BATCH; ++k) fw[k] = p[k] >= 0 ? front[p[k]] : 0;
 #pragma unroll
             for (int k = 0; k < PULL_BATCH; ++k) acc |= fw[k];
-            if ((acc | sw) == all) break;
+            if (((acc | sw) & all) == all) break;
         }
     }
     next[w] = acc & ~sw;
@@ -371,7 +392,7 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
     M.FS = S.msbm[1].p;
     M.T = S.msbm[2].p;
     M.TS = S.msbm[3].p;
-    S.mscnt.ensure(2 * MS_RING + 1);  // + the persistent kernel's level slot
+    S.mscnt.ensure(2 * MS_RING + 2);  // + the persistent kernel's level slot, the pull's active mask
     uint64_t *seen = S.ms[0].p, *front = S.ms[1].p, *next = S.ms[2].p;
     MsCnt* ring = (MsCnt*)S.mscnt.p;
     const uint64_t all = k >= 64 ? ~0ULL : ((1ULL << k) - 1);
@@ -404,9 +425,14 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
         const bool sparse = !pull && nf * 4 < (int64_t)V;
         if (!sparse) {
             MsCnt* nxt = &ring[(L + 1) % MS_RING];
-            if (pull)
+            if (pull) {
+                uint64_t* act = (uint64_t*)(S.mscnt.p + 2 * MS_RING + 1);
+                fillU64(h, act, 0, 1);
+                launch(h, "msbfs_active", kMsActive, boundedGrid(V, blk), blk, 0, (const uint64_t*)front, V,
+                       (unsigned long long*)act);
                 launch(h, "msbfs_pull", kMsPull, g, blk, 0, (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p,
-                       (const uint64_t*)front, (const uint64_t*)seen, next, V, all, nxt);
+                       (const uint64_t*)front, (const uint64_t*)seen, next, V, (const uint64_t*)act, nxt);
+            }
             else {
                 const MsPushF f{h->fw_adj.p, seen, next};
                 launch(h, "msbfs_push", kMsPush, g, blk, 0, fb, (const uint64_t*)front, V, f, nxt, heavyList(h));

@@ -181,6 +181,20 @@ def test_bfs_many_levels_bit_parallel():
         assert (G.bfs(src) == lv).all()
 
 
+def test_bfs_sources_of_small_reach():
+    # uniform random sources (many with no out-arcs or a tiny reach) beside
+    # giant-component ones in one 64-source sweep: the pull levels stop
+    # scanning once a vertex holds every source still expanding
+    V, fb, fa, bb, ba = girhip.rmat_graph(14, seed=37)
+    rng = np.random.default_rng(5)
+    src = np.concatenate([rng.integers(0, V, 32).astype(np.int32), girhip.sources(fb, 32, 38)])
+    say("bfs small-reach sources", V)
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        lv = G.bfs(src)
+        for i, x in enumerate(src):
+            assert (lv[i] == oracle.bfs_textbook(fb, fa, int(x))).all(), f"bfs src {x}"
+
+
 def test_tuning_knobs_do_not_change_results():
     V, fb, fa, bb, ba = girhip.rmat_graph(12, seed=34)
     say("knobs rmat12")
