@@ -569,8 +569,12 @@ __device__ __forceinline__ void waveRowsP(bool act, int32_t v, P pay, int64_t b,
 // the last block of kHeavyRows, so consecutive launches need no host work.
 constexpr int64_t HEAVY_T = 2048;
 
+// ctl[0] packs the listed rows (bits 40..) and their total arc count (bits
+// 0..39); off[k] = arcs of the rows listed before row k.
+constexpr int HEAVY_SHIFT = 40;
+constexpr uint64_t HEAVY_MASK = (1ULL << HEAVY_SHIFT) - 1;
 struct HeavyList {
-    int64_t* ctl;  // [0] count, [1] blocks done
+    int64_t* ctl;  // [0] rows << 40 | arcs, [1] blocks done
     int32_t* row;
     int32_t* aux;
     int64_t* b;
@@ -616,13 +620,34 @@ template <int SMALL = 16, class F>
 __device__ __forceinline__ int64_t waveRowsH(bool act, int32_t v, uint64_t pay, int32_t aux, int64_t b, int64_t end,
                                              const HeavyList& H, const F& f) {
     const bool heavy = act && (end - b) > HEAVY_T;
-    const int64_t pos = wavePush(&H.ctl[0], heavy);
-    if (heavy) {
-        H.row[pos] = v;
-        H.aux[pos] = aux;
-        H.b[pos] = b;
-        H.e[pos] = end;
-        H.pay[pos] = pay;
+    const uint64_t hm = __ballot(heavy);
+    if (hm) {
+        // list slots and arc offsets are reserved together, (rows << 40) +
+        // arcs in one atomic, so offsets grow with slots and the heavy pass
+        // needs no prefix launch
+        const int lane = laneId();
+        const int64_t len = heavy ? end - b : 0;
+        int64_t incl = len;
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int64_t o = __shfl_up(incl, d);
+            if (lane >= d) incl += o;
+        }
+        const int64_t tot = __shfl(incl, WAVE - 1);
+        const int leader = __ffsll((long long)hm) - 1;
+        unsigned long long old = 0;
+        if (lane == leader)
+            old = atomicAdd((unsigned long long*)&H.ctl[0], ((unsigned long long)__popcll(hm) << HEAVY_SHIFT) +
+                                                                 (unsigned long long)tot);
+        old = __shfl(old, leader);
+        if (heavy) {
+            const int64_t pos = (int64_t)(old >> HEAVY_SHIFT) + __popcll(hm & lanesBelow());
+            H.row[pos] = v;
+            H.aux[pos] = aux;
+            H.b[pos] = b;
+            H.e[pos] = end;
+            H.pay[pos] = pay;
+            H.off[pos] = (int64_t)(old & HEAVY_MASK) + incl - len;
+        }
     }
     const bool mine = act && !heavy;
     const int first = mine ? (int)(end - b < SMALL ? end - b : SMALL) : 0;
@@ -869,39 +894,13 @@ inline unsigned summaryGrid(int64_t W2, int G, int64_t reps, int block) {
 }
 
 // One block: off[k] = sum of the lengths of listed rows before k.
-static __global__ void kHeavyPrefix(HeavyList H) {
-    __shared__ int64_t wtot[1024 / WAVE];
-    const int64_t n = ldAcq(&H.ctl[0]);
-    const int lane = laneId(), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
-    int64_t carry = 0;
-    for (int64_t base = 0; base < n; base += blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        const int64_t len = i < n ? H.e[i] - H.b[i] : 0;
-        int64_t v = len;
-        for (int d = 1; d < WAVE; d <<= 1) {
-            const int64_t o = __shfl_up(v, d);
-            if (lane >= d) v += o;
-        }
-        if (lane == WAVE - 1) wtot[w] = v;
-        __syncthreads();
-        int64_t pre = 0, tot = 0;
-        for (int k = 0; k < nw; ++k) {
-            if (k < w) pre += wtot[k];
-            tot += wtot[k];
-        }
-        if (i < n) H.off[i] = carry + pre + v - len;
-        carry += tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) H.off[n] = carry;
-}
 
 // The listed rows' arcs as one range [0, off[n]): every thread strides over
 // it and finds the owning row by binary search in off.
 template <class F>
 __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
-    const int64_t n = ldAcq(&H.ctl[0]);
-    const int64_t T = n ? H.off[n] : 0;
+    const uint64_t c = (uint64_t)ldAcq(&H.ctl[0]);
+    const int64_t n = (int64_t)(c >> HEAVY_SHIFT), T = (int64_t)(c & HEAVY_MASK);
     // each block takes a contiguous range of the concatenated arcs, so a
     // thread's next arc is usually in the same row and the row search runs
     // only at row boundaries
@@ -910,7 +909,7 @@ __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
     int64_t acc = 0;
     int64_t k = 0;
     for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        if (!(p >= H.off[k] && p < H.off[k + 1])) {
+        if (!(p >= H.off[k] && p < H.off[k] + (H.e[k] - H.b[k]))) {
             int64_t lo = 0, hi = n - 1;  // last k with off[k] <= p
             while (lo < hi) {
                 const int64_t m = (lo + hi + 1) >> 1;
@@ -975,7 +974,6 @@ HeavyList heavyList(girhip_graph* h);
 template <class F>
 void launchHeavy(girhip_graph* h, const char* name, const F& f, int64_t* sum = nullptr) {
     const HeavyList H = heavyList(h);
-    launch(h, "heavy_prefix", kHeavyPrefix, 1, 1024, 0, H);
     launch(h, name, kHeavyRows<F>, 1024, 256, 0, H, f, sum);
 }
 // Symmetric closure of the edge store (both directions, deduplicated, no
