@@ -98,10 +98,10 @@ __global__ void kTrim1(const int32_t* __restrict__ list, int64_t n, const int32_
             if (trim) stRel(&scc[v], v);
         }
     }
-    // only "any trimmed" matters to the host: a plain store of 1 per block
-    // that trimmed (every writer stores the same value; the kernel's end makes
-    // it visible), not a contended atomic per wave
-    if (__syncthreads_or(trim) && threadIdx.x == 0) *cnt = 1;
+    // only "any trimmed" matters to the host: a block that trimmed raises the
+    // flag with a device-coherent store unless it reads it raised already
+    // (not a contended atomic per wave)
+    if (__syncthreads_or(trim) && threadIdx.x == 0 && ldAcq(cnt) == 0) stRel(cnt, (int64_t)1);
 }
 
 // argmax over the active list of min(outdeg * indeg, 2^33 - 1), ties to the
