@@ -408,14 +408,23 @@ __global__ void kHub(const int32_t* __restrict__ indeg, int32_t V, int64_t* out)
     if (threadIdx.x == 0 && m) atomicMax((unsigned long long*)out, m);
 }
 
-__global__ void kKahnStep(const int32_t* __restrict__ qin, int64_t nin, const int32_t* __restrict__ dbeg,
-                          int64_t* rank, int32_t L, KahnF f, HeavyList H) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = i < nin;
-    const int32_t s = act ? qin[i] : 0;
-    if (act) rank[s] = ((int64_t)L << 32) | (int64_t)(uint32_t)s;
-    const int32_t b = act ? dbeg[s] : 0, e = act ? dbeg[s + 1] : 0;
-    waveRowsH(act, s, 0, 0, b, e, H, f);
+// One Kahn level over the queue qin, whose length the previous level left
+// in *nin (read on the device, so several levels run per host readback);
+// block 0 clears *cz, the counter of the level after next.  The grid strides
+// over the queue, uniformly per wave.
+__global__ void kKahnStep(const int32_t* __restrict__ qin, const int64_t* nin, const int32_t* __restrict__ dbeg,
+                          int64_t* rank, int32_t L, KahnF f, HeavyList H, int64_t* cz) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cz = 0;
+    const int64_t n = *nin;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+        const int64_t i = base + laneId();
+        const bool act = i < n;
+        const int32_t s = act ? qin[i] : 0;
+        if (act) rank[s] = ((int64_t)L << 32) | (int64_t)(uint32_t)s;
+        const int32_t b = act ? dbeg[s] : 0, e = act ? dbeg[s + 1] : 0;
+        waveRowsH(act, s, 0, 0, b, e, H, f);
+    }
 }
 
 namespace {
@@ -603,22 +612,29 @@ void runDag(girhip_graph* h) {
     int64_t hubk;
     readCounters(h, h->dcnt.p + 12, 1, &hubk);
     const int32_t hub = hubk ? (int32_t)(hubk & 0xffffffffLL) : -1;
-    fillI64(h, h->dcnt.p + 10, 0, 1);
-    launch(h, "rank_seeds", kRankSeeds, (unsigned)(((int64_t)V + caSpan(blk) - 1) / caSpan(blk)), blk, 0, (const int32_t*)h->scc.p, (const int32_t*)ideg, V,
-           h->rank.p, R.q[0], h->dcnt.p + 10);
-    int64_t nq;
-    readCounters(h, h->dcnt.p + 10, 1, &nq);
+    // level counters: a ring of KAHN_RING (level L reads slot L, fills
+    // slot L + 1, clears slot L + 2); KAHN_CHUNK levels per readback
+    constexpr int KAHN_RING = 4, KAHN_CHUNK = 4;
+    Scratch& S2 = scratch(h);
+    S2.rcnt.ensure(KAHN_RING);
+    int64_t* kc = S2.rcnt.p;
+    fillI64(h, kc, 0, KAHN_RING);
+    launch(h, "rank_seeds", kRankSeeds, (unsigned)(((int64_t)V + caSpan(blk) - 1) / caSpan(blk)), blk, 0,
+           (const int32_t*)h->scc.p, (const int32_t*)ideg, V, h->rank.p, R.q[0], kc);
     int cur = 0;
     int32_t L = 0;
-    while (nq > 0) {
-        fillI64(h, h->dcnt.p + 11, 0, 1);
-        const KahnF f{h->dag_dst.p, ideg, R.q[cur ^ 1], h->dcnt.p + 11, hub};
-        launch(h, "rank_kahn", kKahnStep, gridFor(nq, blk), blk, 0, (const int32_t*)R.q[cur], nq,
-               (const int32_t*)h->dag_beg.p, h->rank.p, L, f, heavyList(h));
-        launchHeavy(h, "rank_kahn_heavy", f);
-        readCounters(h, h->dcnt.p + 11, 1, &nq);
-        cur ^= 1;
-        ++L;
+    while (true) {
+        for (int t = 0; t < KAHN_CHUNK; ++t, ++L) {
+            int64_t* cin = kc + L % KAHN_RING;
+            const KahnF f{h->dag_dst.p, ideg, R.q[cur ^ 1], kc + (L + 1) % KAHN_RING, hub};
+            launch(h, "rank_kahn", kKahnStep, 1024, blk, 0, (const int32_t*)R.q[cur], (const int64_t*)cin,
+                   (const int32_t*)h->dag_beg.p, h->rank.p, L, f, heavyList(h), kc + (L + 2) % KAHN_RING);
+            launchHeavy(h, "rank_kahn_heavy", f);
+            cur ^= 1;
+        }
+        int64_t nq;
+        readCounters(h, kc + L % KAHN_RING, 1, &nq);
+        if (nq == 0) break;
     }
 }
 
