@@ -394,8 +394,12 @@ extern "C" int girhip_inc_update(girhip_graph* h, int32_t weighted) {
         int64_t r[T_SLOTS];
         readCounters(h, ctl, T_SLOTS, r);
         const bool any_invalid = r[T_TAIL] > 0;
+        // chunks of 2, 4, then INC_CHUNK levels per readback: a small batch's
+        // invalidation is a few levels deep, so a full first chunk would be
+        // mostly empty launches
+        int chunk = 2;
         while (any_invalid) {
-            for (int t = 0; t < INC_CHUNK; ++t) {
+            for (int t = 0; t < chunk; ++t) {
                 launch(h, "inc_spread", kListRows<IncSpreadF>, 1024, blk, 0, (const int32_t*)h->fw_beg.p,
                        (const int32_t*)S.inc_list.p, (const int64_t*)ctl, sf, heavyList(h));
                 launchHeavyStore(h, "inc_spread_rows", sf);
@@ -403,6 +407,7 @@ extern "C" int girhip_inc_update(girhip_graph* h, int32_t weighted) {
             }
             readCounters(h, ctl, T_SLOTS, r);
             if (r[T_LO] == r[T_HI]) break;
+            chunk = std::min(INC_CHUNK, 2 * chunk);
         }
         // 3. restart the invalid vertices from valid in-neighbours
         if (any_invalid) {
@@ -423,8 +428,9 @@ extern "C" int girhip_inc_update(girhip_graph* h, int32_t weighted) {
         // 5. rounds of relaxation
         int64_t rr = 0;
         std::vector<int64_t> rv(C_TOTAL);
+        int rchunk = 2;  // as for the spread: 2, 4, then INC_CHUNK rounds per readback
         while (true) {
-            for (int t = 0; t < INC_CHUNK; ++t, ++rr) {
+            for (int t = 0; t < rchunk; ++t, ++rr) {
                 const IncRelaxF f{g, T.st.p, S.inc_flag.p, S.inc_q[(rr + 1) & 1].p, slot(rr), ctl + C_OVF};
                 launch(h, "inc_round", kIncRound, 1024, blk, 0, f, (const int32_t*)h->fw_beg.p,
                        (const int32_t*)S.inc_q[rr & 1].p, (const int64_t*)slot(rr - 1), slot(rr + 1), heavyList(h));
@@ -432,6 +438,7 @@ extern "C" int girhip_inc_update(girhip_graph* h, int32_t weighted) {
             }
             readCounters(h, ctl, C_TOTAL, rv.data());
             if (rv[slot(rr - 1) - ctl] == 0) break;
+            rchunk = std::min(INC_CHUNK, 2 * rchunk);
         }
         tEnd(h, weighted ? "inc_update_sssp" : "inc_update_bfs");
         if (rv[C_OVF]) raise(GIRHIP_EINVAL, "inc_update: a distance does not fit the 32-bit packed word");
