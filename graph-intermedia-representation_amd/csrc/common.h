@@ -88,6 +88,7 @@ struct Params {
     double graph = 0;        // 1: long runs of full chunks replay captured HIP graphs
     double xcd = 0;          // 1: SSSP source rows pinned to XCDs (block b -> source b mod k)
     double split = 1;        // 1: weighted SSSP walks rows split into light and heavy arcs
+    double scc_block = 512;  // threads per block of the SCC/DAG build's kernels
 };
 
 struct KStat {

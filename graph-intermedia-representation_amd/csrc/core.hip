@@ -575,7 +575,8 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"msbfs", &Params::msbfs},         {"chunk", &Params::chunk},
                {"persist", &Params::persist},     {"sssp_dist32", &Params::sssp_dist32},
                {"reach_local", &Params::reach_local}, {"graph", &Params::graph},
-               {"xcd", &Params::xcd}, {"split", &Params::split}};
+               {"xcd", &Params::xcd}, {"split", &Params::split},
+               {"scc_block", &Params::scc_block}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;

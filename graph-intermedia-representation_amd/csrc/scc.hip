@@ -458,7 +458,23 @@ int64_t trimRounds(girhip_graph* h, int32_t*& list, int32_t*& alt, int64_t n, in
 
 }  // namespace
 
+namespace {
+// The SCC/DAG build runs its kernels (and the shared reachability engine's)
+// at the scc_block launch size: measured 3.8 ms vs 4.1 ms at 256 threads for
+// RMAT-22 (BFS and SSSP are best at 256).
+struct SccBlock {
+    girhip_graph* h;
+    double old;
+    explicit SccBlock(girhip_graph* g) : h(g), old(g->prm.block) {
+        const double b = g->prm.scc_block;
+        if (b >= 64 && b <= 1024) g->prm.block = b;
+    }
+    ~SccBlock() { h->prm.block = old; }
+};
+}  // namespace
+
 void runScc(girhip_graph* h) {
+    const SccBlock sb(h);
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
     Scratch& S = scratch(h);
@@ -549,6 +565,7 @@ void runScc(girhip_graph* h) {
 }
 
 void runDag(girhip_graph* h) {
+    const SccBlock sb(h);
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
     Scratch& S = scratch(h);

@@ -261,6 +261,8 @@ int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
  *   chunks are captured once and replayed as HIP graphs - measured no
  *   faster on MI355X, whose rounds are bound by the kernels, not by their
  *   launches; ignored while profiling),
+ *   "scc_block" (512: threads per block of the SCC/DAG build's kernels,
+ *   64..1024; "block" applies elsewhere),
  *   "split" (1: weighted SSSP walks a split view of the forward rows, each
  *   row's arcs of weight <= delta first, so the light rounds and the heavy
  *   pass each read only their own arcs; built on the first call with a

@@ -204,7 +204,7 @@ def test_tuning_knobs_do_not_change_results():
         for k, v in [("msbfs", 0), ("block", 64), ("block", 1024), ("bfs_alpha", 1), ("bfs_beta", 1000),
                      ("bfs_min_reach", 0),
                      ("wcc_rounds", 0), ("scc_trim_rounds", 0), ("scc_pivot", 0), ("sssp_delta", 1),
-                     ("persist", 1), ("chunk", 1), ("sssp_dist32", 0), ("xcd", 1), ("split", 0),
+                     ("persist", 1), ("chunk", 1), ("sssp_dist32", 0), ("xcd", 1), ("split", 0), ("scc_block", 256), ("scc_block", 1024),
                      ("sssp_delta", 100000)]:
             say("knob", k, v)
             old = G.get_param(k)
