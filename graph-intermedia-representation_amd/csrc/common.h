@@ -901,6 +901,7 @@ inline unsigned summaryGrid(int64_t W2, int G, int64_t reps, int block) {
 template <class F>
 __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
     const uint64_t c = (uint64_t)ldAcq(&H.ctl[0]);
+    if (c == 0) return;  // empty list (the usual case): nothing to walk or reset
     const int64_t n = (int64_t)(c >> HEAVY_SHIFT), T = (int64_t)(c & HEAVY_MASK);
     // each block takes a contiguous range of the concatenated arcs, so a
     // thread's next arc is usually in the same row and the row search runs
@@ -925,14 +926,13 @@ __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
         acc = waveSum(acc);
         if (laneId() == 0 && acc) atomicAdd((unsigned long long*)sum, (unsigned long long)acc);
     }
+    // the last block to finish clears the list for the next row kernel: every
+    // block has read ctl[0] by then (no fence needed: the list's readers are
+    // this launch's blocks, and the next writer starts after the launch)
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd((unsigned long long*)&H.ctl[1], 1ULL) == gridDim.x - 1) {
-            H.ctl[0] = 0;
-            H.ctl[1] = 0;
-            __threadfence();
-        }
+    if (threadIdx.x == 0 && atomicAdd((unsigned long long*)&H.ctl[1], 1ULL) == gridDim.x - 1) {
+        H.ctl[0] = 0;
+        H.ctl[1] = 0;
     }
 }
 
