@@ -195,6 +195,40 @@ def test_bfs_sources_of_small_reach():
             assert (lv[i] == oracle.bfs_textbook(fb, fa, int(x))).all(), f"bfs src {x}"
 
 
+def test_sssp_split_views_across_deltas_weights_and_updates():
+    # the per-delta split views (two kept, least recently used evicted) must
+    # follow delta changes, weight changes and store changes
+    V, fb, fa, bb, ba = girhip.rmat_graph(12, seed=39)
+    src = girhip.sources(fb, 3, 40)
+    w = girhip.weights(len(fa), 1, 64, 41)
+    say("split views", V)
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        want_hash = [oracle.sssp(fb, fa, None, int(x), 16) for x in src]
+        for delta in (16, 32, 64, 16, 1, 100000):  # 64 evicts 16; 16 is rebuilt
+            d = G.sssp(src, delta)
+            for i in range(len(src)):
+                assert (d[i] == want_hash[i]).all(), f"hash delta {delta}"
+        G.set_weights(w)
+        want_w = [oracle.dijkstra(fb, fa, w, int(x)) for x in src]
+        for delta in (16, 8, 16):
+            d = G.sssp(src, delta)
+            for i in range(len(src)):
+                assert (d[i] == want_w[i]).all(), f"weights delta {delta}"
+        # a store change invalidates the views; inserted arcs carry the hash weight
+        u = np.array([int(src[0])], np.int32)
+        v = np.array([(int(src[0]) + 1) % V], np.int32)
+        G.apply_batch(u, v, np.ones(1, np.uint8))
+        fb2, fa2, _, _ = G.csr()
+        rows = np.repeat(np.arange(V), np.diff(fb))
+        wmap = {(int(a), int(b)): int(c) for a, b, c in zip(rows, fa, w)}
+        rows2 = np.repeat(np.arange(V), np.diff(fb2))
+        w2 = np.array([wmap.get((int(a), int(b)), oracle.edge_weight(int(a), int(b))) for a, b in zip(rows2, fa2)],
+                      np.uint32)
+        d = G.sssp(src, 16)
+        for i, x in enumerate(src):
+            assert (d[i] == oracle.dijkstra(fb2, fa2, w2, int(x))).all(), "after update"
+
+
 def test_tuning_knobs_do_not_change_results():
     V, fb, fa, bb, ba = girhip.rmat_graph(12, seed=34)
     say("knobs rmat12")
