@@ -229,6 +229,38 @@ def test_sssp_split_views_across_deltas_weights_and_updates():
             assert (d[i] == oracle.dijkstra(fb2, fa2, w2, int(x))).all(), "after update"
 
 
+def test_many_heavy_rows_in_one_wave():
+    # 48 hubs with ids 0..47 (one wave of row owners) and 2,400-3,200 out- and
+    # in-arcs each: every pass walks their rows through the heavy list, whose
+    # slots and arc offsets a wave reserves with one packed atomic
+    rng = np.random.default_rng(42)
+    V = 20000
+    s_bg = rng.integers(0, V, 60000)
+    d_bg = rng.integers(0, V, 60000)
+    hubs = np.arange(48)
+    deg = rng.integers(2400, 3200, len(hubs))
+    s_out = np.repeat(hubs, deg)
+    d_out = rng.integers(0, V, deg.sum())
+    s_in = rng.integers(0, V, deg.sum())
+    d_in = np.repeat(hubs, deg)
+    src = np.concatenate([s_bg, s_out, s_in]).astype(np.int32)
+    dst = np.concatenate([d_bg, d_out, d_in]).astype(np.int32)
+    fb, fa, bb, ba = girhip.csr_from_edges(V, src, dst)
+    say("many heavy rows: V", V, "E", len(fa), "max row", int(np.diff(fb).max()))
+    assert int(np.diff(fb)[:48].min()) > 2048
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        check_layers(G, fb, fa, bb, ba, "heavy")
+        srcs = girhip.sources(fb, 64, 43)
+        lv = G.bfs(srcs)
+        for i in range(0, 64, 9):
+            assert (lv[i] == oracle.bfs_textbook(fb, fa, int(srcs[i]))).all(), f"bfs {srcs[i]}"
+        w = girhip.weights(len(fa), 1, 64, 44)
+        G.set_weights(w)
+        d = G.sssp(srcs[:8], 16)
+        for i in range(8):
+            assert (d[i] == oracle.dijkstra(fb, fa, w, int(srcs[i]))).all(), f"sssp {srcs[i]}"
+
+
 def test_tuning_knobs_do_not_change_results():
     V, fb, fa, bb, ba = girhip.rmat_graph(12, seed=34)
     say("knobs rmat12")
