@@ -665,7 +665,7 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
     const int u = unit ? 1 : 0;
     const RelaxF fl[2] = {{B, fa, w, delta, 0, 0, scope, u, mid}, {B, fa, w, delta, 0, 1, scope, u, mid}};
     const RelaxF fh[2] = {{B, fa, w, delta, 1, 0, scope, u, mid}, {B, fa, w, delta, 1, 1, scope, u, mid}};
-    const unsigned hgrid = std::min<unsigned>(gridFor(V, blk), 1024);
+    const unsigned hgrid = std::min<unsigned>(gridFor(V, blk), 4096);
     const int32_t* fb = h->fw_beg.p;
     int par = 0;
     std::vector<int64_t> r(SB);
