@@ -344,6 +344,9 @@ def main():
         same = bool(torch.equal(dist, ref_dist) and torch.equal(scc, ref_scc))
         learned = {"delta": res["delta"], "trim_rounds": res["trim"], "raw_delta": round(res["raw_delta"], 2),
                    "prediction_kept": res["used"], "judge_sources": N_SSSP,
+                   # with the prediction rejected the SSSP runs the default knobs, so
+                   # its gain is 1 up to timing noise
+                   "sssp_setting_is_default": float(res["delta"]) == float(args.delta),
                    "sssp_ms_default": round(med["sssp"], 3), "sssp_ms_learned": round(s_ms, 3),
                    "sssp_gain": round(med["sssp"] / s_ms, 3),
                    "scc_dag_ms_default": round(med["scc_dag"], 3), "scc_dag_ms_learned": round(c_ms, 3),
