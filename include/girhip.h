@@ -255,8 +255,9 @@ int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
  *   bidirectional LDS search per pair; 0: bit-parallel sweeps only),
  *   "xcd" (0: contiguous rows of blocks per source in a multi-source SSSP
  *   launch; 1: block b serves source b mod k, so each source's blocks share
- *   XCDs and their L2 - measured slower at RMAT-22, 64 vs 44 ms for 8
- *   sources, as a source's work is then confined to its XCDs),
+ *   XCDs and their L2 - measured slower at RMAT-22 (8 sources: 64 vs 44 ms
+ *   with that version of the kernels), as a source's work is then confined
+ *   to its XCDs),
  *   "graph" (0; 1: after two full chunks of SSSP rounds, further full
  *   chunks are captured once and replayed as HIP graphs - measured no
  *   faster on MI355X, whose rounds are bound by the kernels, not by their
