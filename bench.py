@@ -74,13 +74,16 @@ def block_partition(bid, v):
 
 
 def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=None, reach_pairs=None,
-                 two_level=False, bc_src=None, bicc=False):
+                 two_level=False, bc_src=None, bicc=False, reps=1, outputs=False):
     """Time the compiled reference (oracle/_ref/gir_ref: the reference's own
     gir.cpp / gir_query.cpp) on this host: one buildWCC + buildSCC_DAG, BFS
     from n_bfs and SSSP (its hash weights, delta 16) from n_sssp sources, and
     GIR::applyBatch of batch (u, v, insert arrays) when given; two_level:
     also GIR::bfsTwoLevel / ssspTwoLevel from the same sources; bc_src:
-    GIR::bc over those sources (scores returned for comparison)."""
+    GIR::bc over those sources (scores returned for comparison).  reps: the
+    build and the BFS / SSSP passes are timed that many times (medians
+    reported); outputs: the reference's canonical WCC / SCC labels, DAG arcs,
+    BFS levels and SSSP distances are returned under "outputs"."""
     import girhip
     ref = os.path.join(ROOT, "oracle", "_ref", "gir_ref")
     if not os.path.exists(ref):
@@ -90,7 +93,7 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
         bfs_src[:n_bfs].astype(np.int32).tofile(os.path.join(td, "b.bin"))
         sssp_src[:n_sssp].astype(np.int32).tofile(os.path.join(td, "s.bin"))
         cmd = [ref, os.path.join(td, "g") + "/", td + "/", "--bfs", os.path.join(td, "b.bin"),
-               "--sssp", os.path.join(td, "s.bin"), "--no-output"]
+               "--sssp", os.path.join(td, "s.bin"), "--reps", str(reps)] + ([] if outputs else ["--no-output"])
         if batch is not None:
             np.stack([np.asarray(x, np.int32) for x in batch], axis=1).tofile(os.path.join(td, "ops.bin"))
             cmd += ["--batch", os.path.join(td, "ops.bin")]
@@ -118,14 +121,31 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
         bicc_pairs = None
         if bicc and os.path.exists(os.path.join(td, "bicc.bin")):
             bicc_pairs = np.fromfile(os.path.join(td, "bicc.bin"), np.int32).reshape(-1, 2)
+        res = None
+        if outputs:
+            rd = lambda f, t: np.fromfile(os.path.join(td, f), t)
+            V = len(fb) - 1
+            res = {"wcc": rd("wcc.bin", np.int32), "scc": rd("scc.bin", np.int32),
+                   "dag": rd("dag.bin", np.int32).reshape(-1, 2),
+                   "bfs": rd("bfs.bin", np.int32).reshape(-1, V), "sssp": rd("sssp.bin", np.uint64).reshape(-1, V)}
     rec = {}
+    multi = {}
     for line in out.splitlines():
         if line.startswith("@@"):
             d = json.loads(line[2:])
             if "phase" in d:
-                rec[d["phase"]] = d
+                multi.setdefault(d["phase"], []).append(d)
             else:
                 rec.update(d)
+    for ph, lst in multi.items():  # medians over the timed repetitions
+        r = dict(lst[-1])
+        for key in ("ms", "wcc_ms", "scc_dag_ms"):
+            if key in r:
+                r[key] = float(np.median([x[key] for x in lst]))
+        r["reps"] = len(lst)
+        rec[ph] = r
+    if res is not None:
+        rec["outputs"] = res
     if reach_ans is not None and "reach" in rec:
         rec["reach"]["answers"] = reach_ans
     if bc_scores is not None and "bc" in rec:
@@ -138,16 +158,21 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
 def stream_runs(fb, fa, B=10000, NB=5, cpu=True):
     """The streaming protocol (bench_main.cpp:280-345 hold-out: a fresh GIR on
     half the arcs, then insert / delete / mixed streams of B-op batches, each
-    followed by BFS and SSSP from scratch) through the streaming caller with
-    the reference's command line: ours on the GPU (FMU and lazy) and, when
-    built, the reference's own experiments/baselines/gir/gir_stream on the
-    host.  -> {system: {metric: value}}."""
+    followed by BFS and SSSP from scratch) through the reference's own
+    streaming caller, experiments/baselines/gir/gir_stream.cpp: compiled
+    against the drop-in GIR class on the GPU (oracle/_ref/gir_stream_gpu) and
+    against the reference's GIR on the host (oracle/_ref/gir_stream).
+    -> {system: {metric: value}}."""
     import girhip
     out = {}
     with tempfile.TemporaryDirectory() as td:
         d = os.path.join(td, "proto")
         girhip.export_protocol(d, fb, fa, B * NB, n_sources=1, seed=2027, full=False)
-        mine = os.path.join(PKG, "gir_stream")
+        # the reference's caller compiled unchanged against the drop-in GIR class
+        mine = os.path.join(ROOT, "oracle", "_ref", "gir_stream_gpu")
+        if not os.path.exists(mine):
+            log("stream: oracle/_ref/gir_stream_gpu is not built (needs the reference sources); skipped")
+            return None
         runs = [("gfx950", mine, ""), ("gfx950", mine, "lazy"), ("gfx950", mine, "inc")]
         ref = os.path.join(ROOT, "oracle", "_ref", "gir_stream")
         if cpu and os.path.exists(ref):
@@ -269,6 +294,10 @@ def main():
     n_bfs = (lv >= 0).sum(dim=1).cpu().numpy()
     _, m_sssp = G.sssp(sssp_src, args.delta, out=dist, edges=True)
     n_sssp = (dist != -1).sum(dim=1).cpu().numpy()
+    # the step's outputs, kept for the comparison with the compiled reference
+    _s, _d, _ = G.dag()
+    gpu_out = {"wcc": lab.cpu().numpy(), "scc": scc.cpu().numpy(), "dag": np.stack([_s, _d], 1),
+               "bfs": lv[:4].cpu().numpy()}
     bfs_gteps = float(m_bfs.sum()) / (med["bfs"] * 1e-3) / 1e9
     sssp_gteps = float(m_sssp.sum()) / (med["sssp"] * 1e-3) / 1e9
     build_ms = med["wcc"] + med["scc_dag"]
@@ -279,6 +308,7 @@ def main():
         G.sssp(sssp_src, 16, out=dist)
         hash_ms.append(G.last_ms("sssp"))
     sssp_hash_ms_per_source = min(hash_ms) / N_SSSP
+    gpu_out["sssp_hash"] = dist[:2].cpu().numpy().astype(np.uint64)
     G.set_weights(d_w)
 
     # algorithmic bytes (per unit: one build, one source) -> achieved GB/s
@@ -496,14 +526,25 @@ def main():
         log(f"updates: {updates}")
 
     cpu = None
+    ref_match = None
     if not args.no_cpu:
         log("cpu baseline (compiled reference) ...")
         r = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, batch=first_batch if updates else None,
                          reach_pairs=reach_pairs[:4096] if reach_pairs is not None else None,
                          two_level=two is not None, bc_src=bfs_src[:16] if bcr else None,
-                         bicc=bic is not None)
+                         bicc=bic is not None, reps=5, outputs=True)
         if r is not None:
             rec, wall = r
+            o = rec.get("outputs")
+            if o is not None:
+                ref_match = {
+                    "build_matches_reference": bool(np.array_equal(o["wcc"], gpu_out["wcc"])
+                                                    and np.array_equal(o["scc"], gpu_out["scc"])
+                                                    and np.array_equal(o["dag"], gpu_out["dag"])),
+                    "bfs_matches_reference": bool(np.array_equal(o["bfs"], gpu_out["bfs"][:len(o["bfs"])])),
+                    "sssp_matches_reference": bool(np.array_equal(o["sssp"], gpu_out["sssp_hash"][:len(o["sssp"])])),
+                }
+                log(f"outputs vs compiled reference: {ref_match}")
             b = rec.get("build", {})
             cpu_build = b.get("wcc_ms", 0) + b.get("scc_dag_ms", 0)
             cpu = {"value": round(cpu_build, 3), "unit": "ms", "cores": rec.get("threads"), "kind": "reference",
@@ -518,6 +559,8 @@ def main():
                              + ("; applyBatch of 10^3 ops" if "batch" in rec else "") + f"; {wall:.1f} s wall",
                    "wcc_ms": b.get("wcc_ms"), "scc_dag_ms": b.get("scc_dag_ms"),
                    "bfs_ms_per_source": rec["bfs"]["ms"] / rec["bfs"]["sources"],
+                   "reps": rec["build"].get("reps"), "timing": "median of the repetitions",
+                   "nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(),
                    "sssp_ms_per_source": rec["sssp"]["ms"] / rec["sssp"]["sources"]}
             nb = rec["bfs"]["sources"]
             cpu["bfs_gteps"] = float(m_bfs[:nb].sum()) / (rec["bfs"]["ms"] * 1e-3) / 1e9
@@ -570,6 +613,10 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu,
     }
     out["sssp_hash_ms_per_source"] = round(sssp_hash_ms_per_source, 3)
+    # the step's canonical WCC / SCC labels and DAG arcs, BFS levels (4
+    # sources) and hash-weight SSSP distances (2 sources), compared with the
+    # compiled reference's on the same graph and sources
+    out.update(ref_match or {})
     out["sssp_split_build_ms"] = None if split_ms is None else round(split_ms, 3)
     if cpu:
         out["speedup_build_vs_cpu"] = round(cpu["value"] / build_ms, 2)

@@ -393,6 +393,9 @@ int64_t symCsr(girhip_graph* h) {
     launch(h, "sym_count", kCountBelow, 1, 64, 0, (const uint64_t*)S.keys[0].p, 2 * E, 1ULL << (2 * B), h->dcnt.p);
     int64_t n2;
     readCounters(h, h->dcnt.p, 1, &n2);
+    // offsets are int32 (as in the forward / backward CSR)
+    if (n2 > (int64_t)INT32_MAX)
+        raise(GIRHIP_EINVAL, "symmetric closure holds more than 2^31 - 1 arcs (int32 offsets)");
     h->sym_E = n2 > 0 ? keysToCsr(h, n2, B, h->sym_beg, h->sym_adj) : 0;
     if (n2 == 0) {
         h->sym_beg.ensure((size_t)V + 1);
@@ -540,9 +543,17 @@ extern "C" int girhip_set_weights(girhip_graph* h, const uint32_t* w_fw) {
     return guarded([&] {
         useDevice(h);
         for (auto& sv : h->split) sv.delta = 0;
+        // a tracked incremental SSSP was computed with the old weights: track
+        // its source again with the new ones
+        auto retrack = [&] {
+            if (!h->inc[1].on) return;
+            const int rc = girhip_inc_track(h, h->inc[1].source, 1);
+            if (rc != GIRHIP_OK) raise(rc, girhip_last_error());
+        };
         if (!w_fw) {
             h->has_w = false;
             h->w_fw.release();
+            retrack();
             return;
         }
         h->max_w = -1;
@@ -559,6 +570,7 @@ extern "C" int girhip_set_weights(girhip_graph* h, const uint32_t* w_fw) {
             raise(GIRHIP_EINVAL, "set_weights: weights must be > 0");
         }
         h->has_w = true;
+        retrack();
     });
 }
 

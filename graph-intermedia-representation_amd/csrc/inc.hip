@@ -107,7 +107,9 @@ __global__ void kIncParents(IncG g, const int32_t* __restrict__ bb, const int32_
 }
 
 // Step 1: presence of every logged arc now; deleted tree arcs invalidate
-// their heads (flag 1, appended to the list).
+// their heads (flag 1, appended to the list), and so do tree arcs present
+// now with another weight than the tree's distances imply (an arc deleted
+// and inserted again with a new explicit weight between two updates).
 __global__ void kIncDeleted(IncG g, const uint64_t* __restrict__ keys, const int64_t* __restrict__ nkeys, int B,
                             int32_t src, const uint64_t* __restrict__ st, int32_t* flag, uint8_t* __restrict__ pres,
                             int32_t* list, int64_t* ctl) {
@@ -119,11 +121,17 @@ __global__ void kIncDeleted(IncG g, const uint64_t* __restrict__ keys, const int
         bool inv = false;
         if (i < n) {
             const int32_t u = (int32_t)(keys[i] >> B), v = (int32_t)(keys[i] & mask);
-            const bool present = g.slot(u, v) >= 0;
+            const int64_t j = g.slot(u, v);
+            const bool present = j >= 0;
             pres[i] = present ? 1 : 0;
-            if (!present && v != src) {
+            if (v != src) {
                 const uint64_t k = st[v];
-                if (k != NONE && parentOf(k) == u) inv = atomicCAS(&flag[v], 0, 1) == 0;
+                bool stale = false;
+                if (k != NONE && parentOf(k) == u) {
+                    const uint64_t ku = st[u];
+                    stale = !present || ku == NONE || (ku >> 32) + g.weight(u, v, j) != (k >> 32);
+                }
+                if (stale) inv = atomicCAS(&flag[v], 0, 1) == 0;
             }
         }
         const int64_t pos = wavePush(ctl + T_TAIL, inv);

@@ -230,16 +230,19 @@ extern "C" int girhip_query_batch(girhip_graph* h, const int32_t* src, const uin
             HIPCHK(hipMemcpy(kd.data(), kind, (size_t)n, hipMemcpyDefault));
         }
         std::vector<int32_t> b, p;
-        for (int32_t i = 0; i < n; ++i) (kd[i] ? p : b).push_back(s[i]);
+        for (int32_t i = 0; i < n; ++i) {
+            if (kd[i] > 1) raise(GIRHIP_EINVAL, "query_batch: kind must be 0 (BFS) or 1 (SSSP)");
+            (kd[i] ? p : b).push_back(s[i]);
+        }
         double ms = 0;
         if (!b.empty()) {
-            if (girhip_bfs(h, b.data(), (int32_t)b.size(), levels, nullptr) != GIRHIP_OK)
-                raise(GIRHIP_EHIP, girhip_last_error());
+            const int rc = girhip_bfs(h, b.data(), (int32_t)b.size(), levels, nullptr);
+            if (rc != GIRHIP_OK) raise(rc, girhip_last_error());
             ms += h->last_ms["bfs"];
         }
         if (!p.empty()) {
-            if (girhip_sssp(h, p.data(), (int32_t)p.size(), delta, dist, nullptr) != GIRHIP_OK)
-                raise(GIRHIP_EHIP, girhip_last_error());
+            const int rc = girhip_sssp(h, p.data(), (int32_t)p.size(), delta, dist, nullptr);
+            if (rc != GIRHIP_OK) raise(rc, girhip_last_error());
             ms += h->last_ms["sssp"];
         }
         h->last_ms["query_batch"] = ms;

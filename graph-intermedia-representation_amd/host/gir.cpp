@@ -156,10 +156,22 @@ void GIR::refresh() const {
         check(girhip_get_wcc(h, wcc_label.data(), nullptr), "getNodeToWcc");
     }
     if (built_scc) {
-        node_scc.resize(V);
-        scc_rank.resize(V);
-        check(girhip_get_scc(h, node_scc.data(), nullptr, nullptr), "getNodeToScc");
-        check(girhip_get_ranks(h, reinterpret_cast<int64_t*>(scc_rank.data())), "sccRank");
+        // canonical labels (minimum vertex) -> dense ids in order of that vertex
+        std::vector<int> canon((size_t)V);
+        std::vector<long long> rk((size_t)V);
+        check(girhip_get_scc(h, canon.data(), nullptr, nullptr), "getNodeToScc");
+        check(girhip_get_ranks(h, reinterpret_cast<int64_t*>(rk.data())), "sccRank");
+        dense_of.assign((size_t)V, -1);
+        canon_of.clear();
+        for (int v = 0; v < V; ++v)
+            if (canon[v] == v) {
+                dense_of[v] = (int)canon_of.size();
+                canon_of.push_back(v);
+            }
+        node_scc.resize((size_t)V);
+        for (int v = 0; v < V; ++v) node_scc[v] = dense_of[canon[v]];
+        scc_rank.resize(canon_of.size());
+        for (size_t d = 0; d < canon_of.size(); ++d) scc_rank[d] = rk[canon_of[d]];
     }
     sccs_ok = false;
     g_copy.reset();
@@ -209,7 +221,7 @@ int GIR::wccOf(int v) const {
 }
 long long GIR::sccRank(int s) const {
     refresh();
-    return scc_rank[s];
+    return (s >= 0 && (size_t)s < scc_rank.size()) ? scc_rank[s] : -1;
 }
 bool GIR::sameScc(int u, int v) {
     sync();
@@ -236,7 +248,7 @@ std::map<int, int> GIR::getNodeToWcc() const {
 const std::vector<std::vector<int>>& GIR::getSCCs() const {
     refresh();
     if (!sccs_ok) {
-        sccs.assign(built_scc ? V : 0, {});
+        sccs.assign(built_scc ? canon_of.size() : 0, {});
         for (int v = 0; v < (built_scc ? V : 0); ++v) sccs[node_scc[v]].push_back(v);
         sccs_ok = true;
     }
@@ -244,14 +256,15 @@ const std::vector<std::vector<int>>& GIR::getSCCs() const {
 }
 
 std::vector<std::vector<std::pair<int, int>>> GIR::getDAGWithCounts() const {
-    std::vector<std::vector<std::pair<int, int>>> r(V);
+    refresh();
+    std::vector<std::vector<std::pair<int, int>>> r(built_scc ? canon_of.size() : 0);
     if (!built_scc) return r;
     int32_t ns;
     int64_t nd;
     check(girhip_get_scc(h, nullptr, &ns, &nd), "getDAG");
     std::vector<int32_t> s(nd), d(nd), m(nd);
     check(girhip_get_dag(h, s.data(), d.data(), m.data()), "getDAG");
-    for (int64_t i = 0; i < nd; ++i) r[s[i]].push_back({d[i], m[i]});
+    for (int64_t i = 0; i < nd; ++i) r[dense_of[s[i]]].push_back({dense_of[d[i]], m[i]});
     return r;
 }
 
@@ -345,10 +358,14 @@ QueryVec<uint64_t> GIR::ssspTwoLevel(int source, uint64_t delta) {
 
 std::vector<char> GIR::reachableSccsPublic(int c) {
     if (!built_scc) buildSCC_DAG();
+    if (!synced()) sync();
+    refresh();
+    std::vector<char> r(canon_of.size(), 0);
+    if (c < 0 || (size_t)c >= canon_of.size()) return r;
     std::vector<uint8_t> m((size_t)V);
-    check(girhip_reach_info(h, c, m.data(), nullptr, nullptr, nullptr, nullptr), "reachableSccs");
-    stale = true, ++epoch;
-    return std::vector<char>(m.begin(), m.end());
+    check(girhip_reach_info(h, canon_of[c], m.data(), nullptr, nullptr, nullptr, nullptr), "reachableSccs");
+    for (size_t d = 0; d < canon_of.size(); ++d) r[d] = (char)m[canon_of[d]];
+    return r;
 }
 
 void GIR::trackIncremental(int source, bool weighted) {
@@ -421,16 +438,9 @@ bool GIR::biconnected(int u, int v) {
 std::vector<std::vector<int>> GIR::getBCTComponents(int scc_id) {
     std::vector<std::vector<int>> r;
     ensureBlocks();
-    refresh();
-    if (scc_id < 0 || scc_id >= V || node_scc[scc_id] != scc_id) return r;
     const auto& comps = getSCCs();
-    const std::vector<int>* mem = nullptr;
-    for (const auto& c : comps)
-        if (!c.empty() && node_scc[c[0]] == scc_id) {
-            mem = &c;
-            break;
-        }
-    if (!mem) return r;
+    if (scc_id < 0 || (size_t)scc_id >= comps.size() || comps[scc_id].empty()) return r;
+    const std::vector<int>* mem = &comps[scc_id];
     if (mem->size() == 1) return {{(*mem)[0]}};
     std::map<int, std::vector<int>> blocks;
     for (int x : *mem)

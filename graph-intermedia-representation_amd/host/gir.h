@@ -8,14 +8,20 @@
 // lazily refreshed copies of the labels for the per-vertex getters.
 //
 // Differences a caller can observe:
-//   - component ids are canonical (minimum vertex id of the component); the
-//     reference uses union-find roots / Tarjan emission indices.  sccOf(v)
-//     indexes getSCCs() and getDAG() exactly as in the reference (entries of
-//     ids that name no SCC are empty, like the reference's retired ids);
-//   - ranks are Kahn levels (level << 32 | id) instead of (S - s) << 32; the
-//     contract rank(a) < rank(b) for every condensation arc is the same;
-//   - the block-cut (BiCC) layer is not built (setBlocksOnDemand is accepted
-//     and ignored).
+//   - SCC ids are dense, 0 .. S-1, numbered in order of each SCC's minimum
+//     vertex (the reference numbers SCCs by Tarjan emission, gir.cpp:809-816,
+//     and reuses retired ids after merges, gir.cpp:709-737).  sccOf(v) indexes
+//     getSCCs(), getDAG() and sccRank() as in the reference, and
+//     getSCCs().size() is the SCC count after every build or update;
+//   - WCC ids are canonical: the minimum vertex of the component (the
+//     reference uses union-find roots, gir.cpp:649-652);
+//   - ranks are Kahn levels (level << 32 | canonical id) instead of
+//     (S - s) << 32; the contract rank(a) < rank(b) for every condensation
+//     arc is the same;
+//   - the block layer (buildBCT, getBCTComponents, articulationPoint,
+//     biconnected) is computed from scratch on the GPU when first asked for
+//     after a change, which is the reference's on-demand mode;
+//     setBlocksOnDemand is accepted and has no further effect.
 #ifndef GIRHIP_GIR_H
 #define GIRHIP_GIR_H
 #include <cstdint>
@@ -125,8 +131,7 @@ public:
     QueryVec<int> bfsTwoLevel(int source);
     QueryVec<uint64_t> ssspTwoLevel(int source, uint64_t delta = 16);
     // SCCs reachable from SCC c in the condensation (gir.h:205,
-    // gir_query.cpp:56-64), indexed by SCC id; ids here are canonical (the
-    // minimum vertex of each SCC), so the mask has vertexCount() entries.
+    // gir_query.cpp:56-64), indexed by SCC id (getSCCs().size() entries).
     std::vector<char> reachableSccsPublic(int c);
     // Brandes betweenness over the sources (gir_query.cpp:728-778).
     QueryVec<double> bc(const std::vector<int>& sources);
@@ -146,8 +151,7 @@ public:
     void setBlocksOnDemand(bool) {}  // blocks are always built on demand here
     bool blocksOnDemand() const { return true; }
     // Block queries (gir.h:133-148): v lies in two or more blocks; u and v
-    // share a block; the blocks (vertex sets) of the SCC with canonical id
-    // scc_id.
+    // share a block; the blocks (vertex sets) of SCC scc_id (an sccOf id).
     bool articulationPoint(int v);
     bool biconnected(int u, int v);
     std::vector<std::vector<int>> getBCTComponents(int scc_id);
@@ -174,8 +178,10 @@ private:
     int V = 0;
     bool built_wcc = false, built_scc = false;
     mutable bool stale = true;
-    mutable std::vector<int> wcc_label, node_scc;
-    mutable std::vector<long long> scc_rank;
+    mutable std::vector<int> wcc_label, node_scc;  // node_scc: dense SCC ids
+    mutable std::vector<int> canon_of;              // dense SCC id -> canonical id (minimum vertex)
+    mutable std::vector<int> dense_of;              // canonical id -> dense id (-1 elsewhere)
+    mutable std::vector<long long> scc_rank;        // by dense id
     mutable std::vector<std::vector<int>> sccs;
     mutable bool sccs_ok = false;
     mutable std::unique_ptr<graph> g_copy;

@@ -11,6 +11,9 @@
 //           [--batch OPS.bin] [--pairs PAIRS.bin] [--reach PAIRS.bin] [--two-level] [--bc SRC.bin]
 //           [--bicc] [--no-output] [--reps R]
 //
+//   --reps R: the build and the BFS / SSSP passes are timed R times (one
+//   "@@" line each); results are those of the last pass
+//
 //   wcc.bin  int32[V]   buildWCC labels (gir.cpp:593-655), canonical
 //   scc.bin  int32[V]   buildSCC_DAG node_scc (gir.cpp:798-839), canonical
 //   dag.bin  int32[2*k] getDAG() arcs (src, dst) in canonical ids, sorted
@@ -166,32 +169,37 @@ int main(int argc, char** argv) {
     Layers L = snapshot(g);
     std::printf("@@{\"rank_ok\": %d, \"n_dag\": %zu}\n", (int)L.rank_ok, L.dag.size() / 2);
     if (write) writeLayers(out, "", L);
+    // queries: reps timed passes over the sources (results are moved out of
+    // the calls, so the timing holds no copies); the last pass is written
     if (!bfs_f.empty()) {
         auto src = readVec<int>(bfs_f);
-        std::vector<int> lv;
-        if (write) lv.reserve((size_t)src.size() * V);
-        auto t0 = Clock::now();
-        long long edges = 0;
-        for (int s : src) {
-            auto l = g.bfs(s);
-            if (write) lv.insert(lv.end(), l.begin(), l.end());
-            else edges += l[0];  // keep the result live
+        std::vector<QueryVec<int>> res;
+        for (int r = 0; r < reps; ++r) {
+            res.clear();
+            auto t0 = Clock::now();
+            for (int s : src) res.push_back(g.bfs(s));
+            std::printf("@@{\"phase\": \"bfs\", \"sources\": %zu, \"ms\": %.3f}\n", src.size(), msSince(t0));
         }
-        std::printf("@@{\"phase\": \"bfs\", \"sources\": %zu, \"ms\": %.3f, \"keep\": %lld}\n", src.size(), msSince(t0), edges);
-        if (write) writeVec(out + "bfs.bin", lv.data(), lv.size());
+        if (write) {
+            std::vector<int> lv;
+            for (auto& l : res) lv.insert(lv.end(), l.begin(), l.end());
+            writeVec(out + "bfs.bin", lv.data(), lv.size());
+        }
     }
     if (!sssp_f.empty()) {
         auto src = readVec<int>(sssp_f);
-        std::vector<uint64_t> dv;
-        auto t0 = Clock::now();
-        unsigned long long keep = 0;
-        for (int s : src) {
-            auto d = g.sssp(s, delta);
-            if (write) dv.insert(dv.end(), d.begin(), d.end());
-            else keep += d[0];
+        std::vector<QueryVec<uint64_t>> res;
+        for (int r = 0; r < reps; ++r) {
+            res.clear();
+            auto t0 = Clock::now();
+            for (int s : src) res.push_back(g.sssp(s, delta));
+            std::printf("@@{\"phase\": \"sssp\", \"sources\": %zu, \"ms\": %.3f}\n", src.size(), msSince(t0));
         }
-        std::printf("@@{\"phase\": \"sssp\", \"sources\": %zu, \"ms\": %.3f, \"keep\": %llu}\n", src.size(), msSince(t0), keep);
-        if (write) writeVec(out + "sssp.bin", dv.data(), dv.size());
+        if (write) {
+            std::vector<uint64_t> dv;
+            for (auto& d : res) dv.insert(dv.end(), d.begin(), d.end());
+            writeVec(out + "sssp.bin", dv.data(), dv.size());
+        }
     }
     if (two_level && !bfs_f.empty()) {
         auto src = readVec<int>(bfs_f);

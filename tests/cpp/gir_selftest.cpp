@@ -18,6 +18,7 @@
 #include <queue>
 #include <random>
 #include <set>
+#include <string>
 #include <vector>
 
 #include "gir.h"
@@ -174,18 +175,38 @@ static std::vector<uint64_t> truthDijkstra(int V, const Edges& E, int s) {
 static void checkConsistency(GIR& g, int V, const Edges& E, const char* what) {
     // WCC partition
     CHECK(normalise(g.getWCCs()) == normalise(truthWcc(V, E)), "%s: WCC partition", what);
-    // SCC labels (canonical) and getSCCs indexed by sccOf
-    auto lab = truthSccLabels(V, E);
+    // SCC ids: dense, in order of each SCC's minimum vertex; getSCCs() has
+    // one entry per SCC and is indexed by sccOf
+    auto lab = truthSccLabels(V, E);  // canonical (minimum vertex)
+    std::vector<int> dense(V, -1);
+    int ns = 0;
+    for (int v = 0; v < V; ++v)
+        if (lab[v] == v) dense[v] = ns++;
     const auto& sccs = g.getSCCs();
+    CHECK((int)sccs.size() == ns, "%s: getSCCs().size() = %zu, want %d", what, sccs.size(), ns);
+    auto n2s = g.getNodeToScc();
     for (int v = 0; v < V; ++v) {
-        CHECK(g.sccOf(v) == lab[v], "%s: sccOf(%d) = %d, want %d", what, v, g.sccOf(v), lab[v]);
+        const int want_id = dense[lab[v]];
+        CHECK(g.sccOf(v) == want_id, "%s: sccOf(%d) = %d, want %d", what, v, g.sccOf(v), want_id);
+        CHECK(n2s[v] == want_id, "%s: getNodeToScc()[%d]", what, v);
+        if (g.sccOf(v) < 0 || g.sccOf(v) >= (int)sccs.size()) continue;
         const auto& grp = sccs[g.sccOf(v)];
         CHECK(std::binary_search(grp.begin(), grp.end(), v), "%s: %d not in getSCCs()[sccOf]", what, v);
     }
-    // DAG arcs with multiplicities
+    auto n2w = g.getNodeToWcc();
+    for (int v = 0; v < V; ++v) CHECK(n2w[v] == g.wccOf(v), "%s: getNodeToWcc()[%d]", what, v);
+    // DAG arcs with multiplicities (getDAGWithCounts) and without (getDAG)
     std::map<std::pair<int, int>, int> want;
     for (auto& e : E)
-        if (lab[e.first] != lab[e.second]) want[{lab[e.first], lab[e.second]}]++;
+        if (lab[e.first] != lab[e.second]) want[{dense[lab[e.first]], dense[lab[e.second]]}]++;
+    {
+        auto plain = g.getDAG();
+        std::set<std::pair<int, int>> got_plain, want_plain;
+        for (int a = 0; a < (int)plain.size(); ++a)
+            for (int b : plain[a]) got_plain.insert({a, b});
+        for (auto& kv : want) want_plain.insert(kv.first);
+        CHECK(got_plain == want_plain && (int)plain.size() == ns, "%s: getDAG", what);
+    }
     std::map<std::pair<int, int>, int> got;
     auto dag = g.getDAGWithCounts();
     for (int a = 0; a < (int)dag.size(); ++a)
@@ -292,8 +313,56 @@ static void runDifferential(const char* name, int V, Edges E, int iters, unsigne
     delete g;
 }
 
+// GIR(path, path) on the reference's data/example_graph four-file CSR
+// (graph.cpp:2-120, gir.cpp:38-44), checked against the hand-derived answers:
+// 2 WCCs, 7 SCCs, the condensation arcs with multiplicities, BFS levels.
+static void runOnDiskExample(const std::string& dir) {
+    GIR g(dir, dir);
+    CHECK(g.vertexCount() == 18, "on-disk example: V = %d", g.vertexCount());
+    g.buildWCC();
+    g.buildSCC_DAG();
+    auto n2w = g.getNodeToWcc();
+    for (int v = 0; v < 18; ++v) CHECK(n2w[v] == (v < 12 ? 0 : 12), "on-disk example: wcc of %d", v);
+    CHECK(g.getWCCs().size() == 2, "on-disk example: 2 WCCs");
+    // canonical SCC labels 0,0,0,3,3,5,6,6,6,0,0,0,12,12,12,12,16,17 -> dense ids
+    const int want_scc[18] = {0, 0, 0, 1, 1, 2, 3, 3, 3, 0, 0, 0, 4, 4, 4, 4, 5, 6};
+    auto n2s = g.getNodeToScc();
+    for (int v = 0; v < 18; ++v) CHECK(n2s[v] == want_scc[v], "on-disk example: scc of %d = %d", v, n2s[v]);
+    CHECK(g.getSCCs().size() == 7, "on-disk example: 7 SCCs");
+    std::map<std::pair<int, int>, int> want = {{{0, 1}, 3}, {{0, 2}, 1}, {{1, 2}, 1}, {{1, 3}, 1},
+                                               {{2, 3}, 1}, {{4, 5}, 1}, {{5, 6}, 1}};
+    std::map<std::pair<int, int>, int> got;
+    auto dag = g.getDAGWithCounts();
+    for (int a = 0; a < (int)dag.size(); ++a)
+        for (auto& p : dag[a]) got[{a, p.first}] = p.second;
+    CHECK(got == want, "on-disk example: DAG arcs with multiplicities");
+    auto plain = g.getDAG();
+    CHECK(plain.size() == 7 && plain[0] == std::vector<int>({1, 2}) && plain[4] == std::vector<int>({5}),
+          "on-disk example: getDAG");
+    const int from9[18] = {2, 3, 1, 2, 3, 1, 2, 3, 4, 0, 1, 2, -1, -1, -1, -1, -1, -1};
+    auto l = g.bfs(9);
+    CHECK(std::equal(l.begin(), l.end(), from9), "on-disk example: bfs(9)");
+    // case 5b (gir_test.cpp:736-753): deleting 9->2 splits one SCC in two
+    const size_t before = g.getSCCs().size();
+    g.deleteEdge(9, 2);
+    CHECK(g.getSCCs().size() == before + 1, "on-disk example: case 5b adds one SCC (%zu -> %zu)", before,
+          g.getSCCs().size());
+    CHECK(g.sccOf(0) == g.sccOf(2) && g.sccOf(9) == g.sccOf(11) && g.sccOf(0) != g.sccOf(9),
+          "on-disk example: case 5b split");
+}
+
 int main(int argc, char** argv) {
     const int iters = argc > 1 ? std::atoi(argv[1]) : 24;
+    {
+        // tests/golden/example_graph next to this binary's tests/cpp/ folder
+        std::string dir = argc > 2 ? argv[2] : std::string(argv[0]);
+        if (argc <= 2) {
+            const size_t sl = dir.find_last_of('/');
+            dir = (sl == std::string::npos ? std::string(".") : dir.substr(0, sl)) + "/../golden/example_graph/";
+        }
+        if (dir.back() != '/') dir += '/';
+        runOnDiskExample(dir);
+    }
     // targeted test on the reference's 18-vertex example (gir_test.cpp:704-753)
     {
         const int ex[][2] = {{0, 1}, {0, 3}, {1, 2}, {1, 3}, {2, 0}, {2, 3}, {2, 9}, {3, 4}, {3, 5},
@@ -307,12 +376,12 @@ int main(int argc, char** argv) {
         CHECK(g->getWCCs().size() == 2, "example: 2 WCCs");
         CHECK(normalise(g->getSCCs()).size() == 7, "example: 7 SCCs");
         // block layer (blocks of the reference's own build on this graph)
-        CHECK(normalise(g->getBCTComponents(0)) ==
+        CHECK(normalise(g->getBCTComponents(g->sccOf(0))) ==
                   std::vector<std::vector<int>>({{0, 1, 2}, {2, 9}, {9, 10, 11}}),
               "example: blocks of SCC {0,1,2,9,10,11}");
-        CHECK(normalise(g->getBCTComponents(6)) == std::vector<std::vector<int>>({{6, 7, 8}}),
+        CHECK(normalise(g->getBCTComponents(g->sccOf(6))) == std::vector<std::vector<int>>({{6, 7, 8}}),
               "example: blocks of SCC {6,7,8}");
-        CHECK(normalise(g->getBCTComponents(5)) == std::vector<std::vector<int>>({{5}}),
+        CHECK(normalise(g->getBCTComponents(g->sccOf(5))) == std::vector<std::vector<int>>({{5}}),
               "example: single-vertex SCC is a block");
         CHECK(g->articulationPoint(2) && g->articulationPoint(9) && !g->articulationPoint(0) &&
                   !g->articulationPoint(3),

@@ -665,6 +665,41 @@ def test_incremental_matches_from_scratch(weighted):
 
 
 @pytest.mark.gpu
+def test_incremental_weight_changes():
+    # A tracked SSSP follows weight changes: set_weights re-tracks it, and an
+    # arc on a shortest path deleted in one batch and inserted again with a
+    # larger explicit weight in the next, both before one correction, leaves
+    # no head with its old, too small distance.
+    V, fb, fa, bb, ba = girhip.rmat_graph(12, edge_factor=8, seed=96)
+    x = int(girhip.sources(fb, 1, 97)[0])
+    w = girhip.weights(len(fa), 1, 64, 98)
+    rows = np.repeat(np.arange(V, dtype=np.int32), np.diff(fb))
+    say("incremental weight changes")
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.track_incremental(x, 1)
+        assert (G.inc_dist(1) == oracle.dijkstra(fb, fa, None, x)).all()
+        G.set_weights(w)  # hash weights -> explicit ones
+        d = oracle.dijkstra(fb, fa, w, x)
+        assert (G.inc_dist(1) == d).all()
+        # tree arcs: heads with exactly one shortest-path in-arc
+        wmap = dict(zip(zip(rows.tolist(), fa.tolist()), w.tolist()))
+        su, sv = _sp_arcs(fb, fa, d, lambda u, v: np.array([wmap[(int(a), int(b))] for a, b in zip(u, v)], float))
+        heads, cnt = np.unique(sv, return_counts=True)
+        only = np.isin(sv, heads[cnt == 1]) & (sv != x)
+        su, sv = su[only][:50], sv[only][:50]
+        assert len(su) > 0
+        G.apply_batch(su, sv, np.zeros(len(su), np.uint8))
+        G.apply_batch(su, sv, np.ones(len(su), np.uint8), w=np.full(len(su), 1000, np.uint32))
+        for a, b in zip(su.tolist(), sv.tolist()):
+            wmap[(a, b)] = 1000
+        G.update_incremental(1)
+        wv = np.array([wmap[(int(a), int(b))] for a, b in zip(rows, fa)], np.uint32)
+        assert (G.inc_dist(1) == oracle.dijkstra(fb, fa, wv, x)).all()
+        G.set_weights(None)  # back to the hash: tracked again
+        assert (G.inc_dist(1) == oracle.dijkstra(fb, fa, None, x)).all()
+
+
+@pytest.mark.gpu
 def test_incremental_edge_cases():
     # out-of-range source (everything unreachable, updates keep it so), the
     # 32-bit packed distance limit, and both kinds tracked at once
