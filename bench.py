@@ -215,6 +215,8 @@ def main():
                     help="first run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the learned-tuning, query-batch and update-batch sections")
+    ap.add_argument("--set", action="append", default=[], metavar="KNOB=VALUE",
+                    help="set a tuning knob (girhip_set_param) before the timed steps; repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -249,6 +251,9 @@ def main():
     d_w = torch.from_numpy(w.view(np.int32)).to(dev)
     G = girhip.Graph(*d_csr)
     G.set_weights(d_w)
+    for kv in args.set:
+        name, val = kv.split("=", 1)
+        G.set_param(name, float(val))
     lab = torch.empty(V, dtype=torch.int32, device=dev)
     scc = torch.empty(V, dtype=torch.int32, device=dev)
     lv = torch.empty((N_BFS, V), dtype=torch.int32, device=dev)
@@ -268,7 +273,7 @@ def main():
         for i in range(args.warmup):
             step()
         ph = [step() for _ in range(args.steps)]
-        print(json.dumps({"phase": args.phase, "workload": args.workload, "V": V, "E": E,
+        print(json.dumps({"phase": args.phase, "workload": args.workload, "V": V, "E": E, "set": args.set,
                           "ms": [p[args.phase] for p in ph]}), flush=True)
         G.close()
         return

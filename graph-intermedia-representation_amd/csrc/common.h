@@ -89,6 +89,9 @@ struct Params {
     double xcd = 0;          // 1: SSSP source rows pinned to XCDs (block b -> source b mod k)
     double split = 1;        // 1: weighted SSSP walks rows split into light and heavy arcs
     double scc_block = 512;  // threads per block of the SCC/DAG build's kernels
+    double sssp_interleave = 2;  // 1: k-source SSSP batches share buckets and interleave distances;
+                                 // 0: one layout per source; 2: 1 where it measured faster
+    double sssp_ms_batch = 8;    // sources per interleaved batch (1..32)
 };
 
 struct KStat {
@@ -124,6 +127,15 @@ struct Scratch {
     DevBuf<uint32_t> s_d32;  // 32-bit distances of a batch
     DevBuf<uint64_t> s_near, s_far, s_done, s_nsum, s_fsum, s_dsum;
     DevBuf<int64_t> s_ctl;
+    // interleaved multi-source delta-stepping: distances [V][KS], near / far /
+    // done source masks, their vertex bitmaps and summaries, round queues
+    // (vertex, mask) by parity and the heavy queue, control slots
+    DevBuf<uint8_t> ms_dist;
+    DevBuf<uint32_t> ms_mask[3];
+    DevBuf<uint64_t> ms_bits[3], ms_sum[3];
+    DevBuf<int32_t> ms_q[3];
+    DevBuf<uint32_t> ms_qm[3];
+    DevBuf<int64_t> ms_ctl;
     DevBuf<int32_t> src;  // query sources
     // reachability query sweeps (a group of sweeps, V words each)
     DevBuf<uint64_t> rq_seen, rq_next, rq_tgt, rq_fb[2];

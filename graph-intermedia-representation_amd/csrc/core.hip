@@ -588,7 +588,8 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"persist", &Params::persist},     {"sssp_dist32", &Params::sssp_dist32},
                {"reach_local", &Params::reach_local}, {"graph", &Params::graph},
                {"xcd", &Params::xcd}, {"split", &Params::split},
-               {"scc_block", &Params::scc_block}};
+               {"scc_block", &Params::scc_block}, {"sssp_interleave", &Params::sssp_interleave},
+               {"sssp_ms_batch", &Params::sssp_ms_batch}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;
@@ -604,6 +605,10 @@ extern "C" int girhip_set_param(girhip_graph* h, const char* name, double value)
             raise(GIRHIP_EINVAL, "set_param: block must be 64, 128, 256, 512 or 1024");
         if (std::strcmp(name, "chunk") == 0 && !(value >= 1 && value <= 16))
             raise(GIRHIP_EINVAL, "set_param: chunk must be in [1, 16]");
+        if (std::strcmp(name, "sssp_ms_batch") == 0 && !(value >= 1 && value <= 32))
+            raise(GIRHIP_EINVAL, "set_param: sssp_ms_batch must be in [1, 32]");
+        if (std::strcmp(name, "sssp_interleave") == 0 && !(value == 0 || value == 1 || value == 2))
+            raise(GIRHIP_EINVAL, "set_param: sssp_interleave must be 0, 1 or 2");
         if (!(value == value)) raise(GIRHIP_EINVAL, "set_param: NaN");
         *s = value;
     });

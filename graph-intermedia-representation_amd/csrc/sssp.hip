@@ -29,6 +29,7 @@
 
 #include "common.h"
 #include "lists.h"
+#include "msssp.h"
 
 namespace gir {
 
@@ -1095,39 +1096,64 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
             out = (uint64_t*)S.stage.p;
         }
         const size_t Vs = (size_t)(V > 0 ? V : 1), Ws = (Vs + 63) / 64;
-        const int kc = std::max(1, std::min<int32_t>(k, SB));
-        for (int p = 0; p < 2; ++p) S.s_q[p].ensure((size_t)kc * Vs);
-        S.s_qh.ensure((size_t)kc * Vs);
-        S.s_near.ensure((size_t)kc * Ws);
-        S.s_far.ensure((size_t)kc * Ws);
-        S.s_done.ensure((size_t)kc * Ws);
-        const size_t W2s = (Ws + 63) / 64;
-        S.s_nsum.ensure((size_t)kc * W2s);
-        S.s_fsum.ensure((size_t)kc * W2s);
-        S.s_dsum.ensure((size_t)kc * W2s);
-        S.s_ctl.ensure(C_TOTAL);  // slot lines, the persistent kernel's parity, the overflow flag, the gather area
         S.src.ensure((size_t)(k > 0 ? k : 1));
         const int32_t* dsrc = S.src.p;
         copyIn(h, S.src.p, s.data(), sizeof(int32_t) * (size_t)k);
         // 32-bit distances unless weights alone could overflow them
         const bool try32 = h->prm.sssp_dist32 != 0 && (uint64_t)maxWeight(h) < 0x7fffffffull;
-        if (try32) S.s_d32.ensure((size_t)kc * Vs);
         const SplitView* sv = k > 0 ? splitView(h, delta) : nullptr;
-        tBegin(h);
-        for (int32_t i0 = 0; i0 < k; i0 += SB) {
-            const int c = std::min<int32_t>(SB, k - i0);
-            uint64_t* rows = out + (size_t)i0 * V;
-            if (try32) {
-                ssspBatch(h, dsrc + i0, c, delta, S.s_d32.p, true, sv);
-                int64_t ovf;
-                readCounters(h, S.s_ctl.p + C_OVF, 1, &ovf);
-                if (ovf == 0) {
-                    launch(h, "sssp_widen", kWiden, boundedGrid((int64_t)c * V, 256), 256, 0,
-                           (const uint32_t*)S.s_d32.p, (int64_t)c * V, rows);
-                    continue;
-                }
+        // Engine: the interleaved one shares arcs between sources whose
+        // distances fall in the same bucket, which pays once buckets are
+        // wide; with narrow buckets and a heavy phase per bucket the
+        // per-source layout measured faster (RMAT-22, 8 sources, weights
+        // [1, 64]: delta 16 26 vs 28 ms, delta 32 27 vs 19 ms, delta 128 30 vs
+        // 14 ms; one source: interleaved at every delta).  Auto (2): the
+        // interleaved engine for one source or when 2 delta >= the largest
+        // weight.
+        const double mode = h->prm.sssp_interleave;
+        const bool inter = mode == 1 || (mode == 2 && (k == 1 || 2 * delta >= (uint64_t)maxWeight(h)));
+        if (inter) {
+            // batches of up to sssp_ms_batch sources sharing buckets, distances
+            // interleaved; the batches are made even (10 -> 5 + 5, not 8 + 2)
+            tBegin(h);
+            const int mb = std::max(1, std::min(MS_MAX, (int)h->prm.sssp_ms_batch));
+            const int nb = (k + mb - 1) / mb;
+            int32_t i0 = 0;
+            while (i0 < k) {
+                const int c = std::min<int32_t>((k + nb - 1) / nb, k - i0);
+                msBatch(h, dsrc + i0, c, delta, sv, out + (size_t)i0 * V, try32);
+                i0 += c;
             }
-            ssspBatch(h, dsrc + i0, c, delta, rows, false, sv);
+        } else {
+            // one layout per source (the engine below)
+            const int kc = std::max(1, std::min<int32_t>(k, SB));
+            for (int p = 0; p < 2; ++p) S.s_q[p].ensure((size_t)kc * Vs);
+            S.s_qh.ensure((size_t)kc * Vs);
+            S.s_near.ensure((size_t)kc * Ws);
+            S.s_far.ensure((size_t)kc * Ws);
+            S.s_done.ensure((size_t)kc * Ws);
+            const size_t W2s = (Ws + 63) / 64;
+            S.s_nsum.ensure((size_t)kc * W2s);
+            S.s_fsum.ensure((size_t)kc * W2s);
+            S.s_dsum.ensure((size_t)kc * W2s);
+            S.s_ctl.ensure(C_TOTAL);  // slot lines, the persistent kernel's parity, the overflow flag, the gather area
+            if (try32) S.s_d32.ensure((size_t)kc * Vs);
+            tBegin(h);
+            for (int32_t i0 = 0; i0 < k; i0 += SB) {
+                const int c = std::min<int32_t>(SB, k - i0);
+                uint64_t* rows = out + (size_t)i0 * V;
+                if (try32) {
+                    ssspBatch(h, dsrc + i0, c, delta, S.s_d32.p, true, sv);
+                    int64_t ovf;
+                    readCounters(h, S.s_ctl.p + C_OVF, 1, &ovf);
+                    if (ovf == 0) {
+                        launch(h, "sssp_widen", kWiden, boundedGrid((int64_t)c * V, 256), 256, 0,
+                               (const uint32_t*)S.s_d32.p, (int64_t)c * V, rows);
+                        continue;
+                    }
+                }
+                ssspBatch(h, dsrc + i0, c, delta, rows, false, sv);
+            }
         }
         tEnd(h, "sssp");
         if (edges_out) {

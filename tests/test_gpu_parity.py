@@ -271,7 +271,8 @@ def test_tuning_knobs_do_not_change_results():
                      ("bfs_min_reach", 0),
                      ("wcc_rounds", 0), ("scc_trim_rounds", 0), ("scc_pivot", 0), ("sssp_delta", 1),
                      ("persist", 1), ("chunk", 1), ("sssp_dist32", 0), ("xcd", 1), ("split", 0), ("scc_block", 256), ("scc_block", 1024),
-                     ("sssp_delta", 100000)]:
+                     ("sssp_delta", 100000), ("sssp_interleave", 0), ("sssp_interleave", 1),
+                     ("sssp_ms_batch", 3), ("sssp_ms_batch", 32)]:
             say("knob", k, v)
             old = G.get_param(k)
             G.set_param(k, v)
@@ -279,6 +280,41 @@ def test_tuning_knobs_do_not_change_results():
             for a, b in zip(base, got):
                 assert (a == b).all(), (k, v)
             G.set_param(k, old)
+
+
+def test_sssp_interleaved_batch_sizes():
+    # the shared-bucket engine at every interleave width (k -> 1, 2, 4, 8,
+    # 16, 32 distances per vertex; 33 and 40 sources split into even
+    # batches), repeated and invalid sources, split views on and off, and
+    # 64-bit distances, against Dijkstra
+    V, fb, fa, bb, ba = girhip.rmat_graph(12, edge_factor=8, seed=38)
+    w = girhip.weights(len(fa), 1, 64, 39)
+    base = girhip.sources(fb, 40, 40)
+    want = {int(x): oracle.dijkstra(fb, fa, w, int(x)) for x in base}
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.set_weights(w)
+        G.set_param("sssp_interleave", 1)
+        G.set_param("sssp_ms_batch", 32)
+        for k in (1, 2, 3, 5, 8, 13, 32, 33, 40):
+            src = base[:k].copy()
+            if k >= 3:
+                src[1] = src[0]  # the same vertex twice in one batch
+                src[2] = -1 if k % 2 else V  # out of range: all unreachable
+            for split in (1, 0):
+                say("interleaved sssp k", k, "split", split)
+                G.set_param("split", split)
+                d = G.sssp(src, 7)
+                for i, x in enumerate(src):
+                    ref = want[int(x)] if 0 <= x < V else np.full(V, INF, np.uint64)
+                    assert (d[i] == ref).all(), (k, split, i)
+        G.set_param("split", 1)
+        # weights near 2^31: 32-bit distances overflow, the batch is redone in 64 bits
+        big = (w.astype(np.uint64) << 25).clip(1, 2**32 - 1).astype(np.uint32)
+        G.set_weights(big)
+        say("interleaved sssp 64-bit")
+        d = G.sssp(base[:8], 1 << 30)
+        for i, x in enumerate(base[:8]):
+            assert (d[i] == oracle.dijkstra(fb, fa, big, int(x))).all()
 
 
 def test_unsorted_duplicate_input_is_normalised():
