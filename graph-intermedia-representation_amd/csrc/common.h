@@ -251,6 +251,7 @@ struct girhip_graph {
     gir::DevBuf<int64_t> dcnt;  // device counters (64 slots)
     gir::DevBuf<uint32_t> gbar;  // grid barrier of the persistent kernels (gir::BAR_WORDS)
 
+    void* na_scratch = nullptr;  // this device's allocation-free scratch array (girhip_bfs_noalloc)
     gir::Params prm;
     std::map<std::string, double> last_ms;
     std::map<std::string, gir::KStat> kstats;
@@ -431,6 +432,9 @@ void runScc(girhip_graph* h);
 void runDag(girhip_graph* h);
 
 inline Scratch& scratch(girhip_graph* h) { return h->ws; }
+// Address of the allocation-free entry point's scratch array on device
+// (resolved once per device, with that device current; noalloc.hip).
+void* noAllocScratch(int device);
 
 }  // namespace gir
 
@@ -1009,5 +1013,10 @@ int64_t maxWeight(girhip_graph* h);
 template <class F>
 void launchHeavyStore(girhip_graph* h, const char* name, const F& f, int64_t* sum = nullptr) {
     if (2 * maxDegree(h) > HEAVY_T) launchHeavy(h, name, f, sum);
+}
+// launchHeavyStore over a given heavy list.
+template <class F>
+void launchHeavyStoreH(girhip_graph* h, const HeavyList& H, const char* name, const F& f, int64_t* sum = nullptr) {
+    if (2 * maxDegree(h) > HEAVY_T) launch(h, name, kHeavyRows<F>, 1024, 256, 0, H, f, sum);
 }
 }  // namespace gir

@@ -440,6 +440,13 @@ extern "C" int girhip_create(const int32_t* fw_beg, const int32_t* fw_adj, const
         int rc = guarded([&] {
             HIPCHK(hipGetDevice(&h->device));
             HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+            // the allocation-free entry point's array: a failure here only
+            // disables that entry point (it then reports GIRHIP_ESTATE)
+            try {
+                h->na_scratch = noAllocScratch(h->device);
+            } catch (const Error&) {
+                h->na_scratch = nullptr;
+            }
             HIPCHK(hipEventCreate(&h->t0));
             HIPCHK(hipEventCreate(&h->t1));
             HIPCHK(hipHostMalloc((void**)&h->hcnt, sizeof(int64_t) * HCNT_SLOTS, hipHostMallocDefault));

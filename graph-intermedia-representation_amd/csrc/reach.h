@@ -242,15 +242,22 @@ __global__ void __launch_bounds__(256) kReachPersist(const int32_t* __restrict__
 template <class Flt, class Vis>
 ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, const int32_t* rbeg,
                     const int32_t* radj, ReachBufs& B, int64_t n0, int64_t m0, int64_t n_reach, int64_t m_total,
-                    Flt flt, Vis on, bool allow_bottom_up = true) {
+                    Flt flt, Vis on, bool allow_bottom_up = true, int64_t* ring = nullptr,
+                    const HeavyList* hlist = nullptr) {
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
     ReachStats st;
     if (n0 <= 0) return st;
     launch(h, "reach_seed", kReachSeed<Vis>, gridFor(n0, blk), blk, 0, (const int32_t*)B.q[0], n0, B.vis, on);
+    // level counter ring and heavy list: the caller's (2 * REACH_RING int64;
+    // no persistent kernel then), else the handle's scratch
     Scratch& S = scratch(h);
-    S.rcnt.ensure(2 * REACH_RING + 4);  // + the persistent kernel's state
-    int64_t* c = S.rcnt.p;
+    int64_t* c = ring;
+    if (!c) {
+        S.rcnt.ensure(2 * REACH_RING + 4);  // + the persistent kernel's state
+        c = S.rcnt.p;
+    }
+    const HeavyList HL = hlist ? *hlist : heavyList(h);
     auto pair = [&](int64_t L) { return c + 2 * ((L % REACH_RING + REACH_RING) % REACH_RING); };
     fillI64(h, c, 0, 2 * REACH_RING);
     fillI64(h, pair(-1), n0, 1);  // input length of level 0
@@ -262,7 +269,7 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
     int c_prev = 1;
     int cur = 0;
     // persistent levels need every row below the heavy-list threshold
-    bool persist_ok = h->prm.persist != 0 && 2 * maxDegree(h) <= HEAVY_T;  // (closure rows too)
+    bool persist_ok = !ring && h->prm.persist != 0 && 2 * maxDegree(h) <= HEAVY_T;  // (closure rows too)
     int32_t L = 0;
     st.reached = n0;
     while (nf > 0) {
@@ -303,7 +310,7 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
             fillI64(h, io + 3, m_unexplored, 1);
             const int64_t a = (allow_bottom_up && n_reach >= min_reach) ? alpha : 0;
             if (launchPersistent(h, "reach_persist", kReachPersist<Flt, Vis>, 256, 2, pbeg, padj, B.vis, B.q[0], B.q[1],
-                                 c, io, a, flt, on, heavyList(h))) {
+                                 c, io, a, flt, on, HL)) {
                 int64_t r[2 * REACH_RING + 4];
                 readCounters(h, c, 2 * REACH_RING + 4, r);
                 checkGridBar(h);
@@ -333,8 +340,8 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
         for (int t = 0; t < C; ++t) {
             const TopDownF<Flt, Vis> f{pbeg, padj, B.vis, L + t, B.q[cur ^ 1], pair(L + t), flt, on};
             launch(h, "reach_topdown", kTopDown<Flt, Vis>, g, blk, 0, (const int32_t*)B.q[cur],
-                   (const int64_t*)pair(L + t - 1), f, pair(L + t + 1), heavyList(h));
-            launchHeavyStore(h, "reach_topdown_heavy", f, pair(L + t) + 1);
+                   (const int64_t*)pair(L + t - 1), f, pair(L + t + 1), HL);
+            launchHeavyStoreH(h, HL, "reach_topdown_heavy", f, pair(L + t) + 1);
             cur ^= 1;
         }
         int64_t r[2 * REACH_RING];

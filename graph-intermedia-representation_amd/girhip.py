@@ -60,6 +60,7 @@ _SIGS = {
     "girhip_get_dag": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "girhip_get_ranks": (ctypes.c_int, [_vp, _vp]),
     "girhip_bfs": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, _vp]),
+    "girhip_bfs_noalloc": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "girhip_bfs_component": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
     "girhip_reachable": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),
     "girhip_reach_info": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp]),
@@ -189,6 +190,11 @@ def _ptr(x, dtype, n=None, name="array"):
         # tensor (a pending clone, fill or read) must finish first
         torch.cuda.current_stream(x.device).synchronize()
     return x.data_ptr()
+
+
+def _torch_int32():
+    import torch
+    return torch.int32
 
 
 class Graph:
@@ -323,6 +329,21 @@ class Graph:
         _check(lib().girhip_bfs(self._h, src.ctypes.data, k, _ptr(lv, np.int32, k * V, "levels"),
                                 e.ctypes.data if edges else None))
         return (lv, e[:k]) if edges else lv
+
+    def bfs_noalloc(self, source, out):
+        """girhip_bfs_noalloc: GIR::bfs for one source with no device
+        allocation.  out: a device int32 tensor of V entries, or the address
+        (int) of V int32 of device memory."""
+        if isinstance(out, int):
+            ptr = out
+        else:
+            if not hasattr(out, "data_ptr") or not out.is_cuda:
+                raise ValueError("bfs_noalloc: out must be device memory")
+            if out.dtype != _torch_int32() or out.numel() < self.V or not out.is_contiguous():
+                raise ValueError("bfs_noalloc: out must be a contiguous int32 tensor of V entries")
+            ptr = out.data_ptr()
+        _check(lib().girhip_bfs_noalloc(self._h, int(source), ptr))
+        return out
 
     def sssp(self, sources, delta=0, out=None, edges=False):
         """GIR::sssp (gir_query.cpp:695-703) for every source -> uint64 (k, V)

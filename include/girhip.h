@@ -108,6 +108,17 @@ int girhip_get_dag(girhip_graph* h, int32_t* src, int32_t* dst, int32_t* mult);
  * rank[a] < rank[b] for every DAG arc a -> b; -1 at non-representatives. */
 int girhip_get_ranks(girhip_graph* h, int64_t* rank);
 
+/* GIR::bfs for one source with no device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops).  Same levels as girhip_bfs with k = 1 (-1
+ * unreachable; all -1 for an out-of-range source).  levels must be device
+ * memory of the handle's device (V int32).  The work buffers are one
+ * file-scope device array of about 36 MB per device (resolved by
+ * girhip_create, held until the process ends), so the graph may hold at most
+ * 2^22 vertices and 2^27 arcs (GIRHIP_EINVAL beyond that); every call on a
+ * device shares that array, so calls that overlap on one device race. */
+int girhip_bfs_noalloc(girhip_graph* h, int32_t source, int32_t* levels);
+
 /* Replaces GIR::bfs (gir_query.cpp:108-294) for k sources.  levels holds
  * k*V int32 (row i = hops from src[i], -1 unreachable).  May be NULL: the
  * levels are then computed into scratch and not returned (timing, or
