@@ -391,29 +391,7 @@ def main():
         del ref_dist, ref_scc
         log(f"learned: {learned}")
 
-    # ---- batched queries (1,024 mixed BFS/SSSP from uniform random sources) ----
     queries = None
-    if not args.no_extras:
-        rng = np.random.default_rng(17)
-        nq = 1024
-        qsrc = rng.integers(0, V, nq).astype(np.int32)
-        qkind = (np.arange(nq) % 2).astype(np.uint8)
-        del lv, dist
-        torch.cuda.empty_cache()
-        qlv = torch.empty((nq // 2, V), dtype=torch.int32, device=dev)
-        qd = torch.empty((nq // 2, V), dtype=torch.int64, device=dev)
-        G.query_batch(qsrc[:64], qkind[:64], 16, levels=qlv, dist=qd)  # warm
-        t0 = time.perf_counter()
-        G.query_batch(qsrc, qkind, 16, levels=qlv, dist=qd)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        dev_ms = G.last_ms("query_batch")
-        queries = {"n": nq, "bfs": nq // 2, "sssp": nq // 2, "device_ms": round(dev_ms, 3),
-                   "queries_per_s": round(nq / (dev_ms * 1e-3), 1), "wall_ms": round(wall * 1e3, 3)}
-        del qlv, qd
-        torch.cuda.empty_cache()
-        log(f"queries: {queries}")
-
     # ---- two-level queries (GIR::bfsTwoLevel / ssspTwoLevel, hash weights) ----
     two = None
     if not args.no_extras:
@@ -530,6 +508,70 @@ def main():
                             "rebuild_wcc_scc_ms": round(rebuild, 3)})
         log(f"updates: {updates}")
 
+    # ---- batched form (girhip_run_batch): 1,024 mixed BFS/SSSP queries from
+    #      uniform random sources interleaved with update batches of 10^3,
+    #      10^5 and 10^6 ops, one call; the reference's hash weights, as its
+    #      GIR::sssp uses; delta picked by the autotuner ----
+    if not args.no_extras:
+        G.set_weights(None)
+        rng = np.random.default_rng(17)
+        nq = 1024
+        qsrc = rng.integers(0, V, nq).astype(np.int32)
+        qkind = (np.arange(nq) % 2).astype(np.int32)
+        cfb, cfa, _, _ = G.csr()
+        rows = np.repeat(np.arange(V, dtype=np.int32), np.diff(cfb))
+        kinds, A, B = [], [], []
+        qrun = 0
+        ub = {}
+        for part, nops in enumerate([1000, 100000, 1000000, None]):
+            sl = slice(part * nq // 4, (part + 1) * nq // 4)
+            kinds.append(qkind[sl])
+            A.append(qsrc[sl])
+            B.append(np.zeros(nq // 4, np.int32))
+            if nops is None:
+                break
+            nd = nops // 2
+            e = rng.integers(0, len(cfa), nd)
+            u = np.concatenate([rows[e], rng.integers(0, V, nops - nd).astype(np.int32)])
+            v = np.concatenate([cfa[e], rng.integers(0, V, nops - nd).astype(np.int32)])
+            k2 = np.concatenate([np.full(nd, 3, np.int32), np.full(nops - nd, 2, np.int32)])
+            perm = rng.permutation(nops)
+            kinds.append(k2[perm])
+            A.append(u[perm])
+            B.append(v[perm])
+            ub[nops] = (u[perm], v[perm], (k2[perm] == 2).astype(np.uint8))
+        kinds, A, B = (np.concatenate(x) for x in (kinds, A, B))
+        del cfb, cfa, rows
+        tuned, _ = G.autotune("sssp_delta", qsrc[qkind == 1][:8], [16, 32, 64, 128, 256], reps=1)
+        qlv = torch.empty((int((kinds == 0).sum()), V), dtype=torch.int32, device=dev)
+        qd = torch.empty((int((kinds == 1).sum()), V), dtype=torch.int64, device=dev)
+        G.run_batch(qkind[:32], qsrc[:32], None, 0, levels=qlv, dist=qd)  # warm
+        t0 = time.perf_counter()
+        _, _, st = G.run_batch(kinds, A, B, 0, levels=qlv, dist=qd)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        q_ms, u_ms = G.last_ms("run_batch_queries"), G.last_ms("run_batch_updates")
+        # the last quarter of the queries saw the final graph: a sample of its
+        # rows against single calls
+        lb = np.nonzero(kinds == 0)[0]
+        ls = np.nonzero(kinds == 1)[0]
+        tail_b = [i for i in range(len(lb)) if lb[i] > len(kinds) - nq // 4][-4:]
+        tail_s = [i for i in range(len(ls)) if ls[i] > len(kinds) - nq // 4][-4:]
+        same = bool((G.bfs(A[lb[tail_b]]) == qlv[tail_b].cpu().numpy()).all()
+                    and (G.sssp(A[ls[tail_s]], 0) == qd[tail_s].cpu().numpy().astype(np.uint64)).all())
+        queries = {"n_queries": nq, "bfs": int(len(lb)), "sssp": int(len(ls)), "update_ops": int(sum(ub)),
+                   "layout": "256 queries, 10^3 updates, 256 queries, 10^5 updates, 256 queries, 10^6 updates, "
+                             "256 queries",
+                   "weights": "GIR::edgeWeight hash", "delta_autotuned": tuned,
+                   "queries_ms": round(q_ms, 3), "updates_ms": round(u_ms, 3), "wall_ms": round(wall * 1e3, 3),
+                   "queries_per_s": round(nq / (q_ms * 1e-3), 1),
+                   "update_meps": round(sum(ub) / (u_ms * 1e-3) / 1e6, 3),
+                   "inserted": st["inserted"], "deleted": st["deleted"], "sample_rows_match_single_calls": same}
+        del qlv, qd
+        torch.cuda.empty_cache()
+        G.set_param("sssp_delta", args.delta)
+        log(f"batched form: {queries}")
+
     cpu = None
     ref_match = None
     if not args.no_cpu:
@@ -602,6 +644,18 @@ def main():
                 r2 = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=0, n_sssp=0, batch=batches[100000])
                 if r2 is not None and "batch" in r2[0]:
                     cpu["batch1e5_ms"] = r2[0]["batch"]["ms"]
+            if updates and 1000000 in batches:
+                r3 = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=0, n_sssp=0, batch=batches[1000000])
+                if r3 is not None and "batch" in r3[0]:
+                    cpu["batch1e6_ms"] = r3[0]["batch"]["ms"]
+            if queries and all(k in cpu for k in ("batch_ms", "batch1e5_ms", "batch1e6_ms")):
+                # the reference answering the batched form one call at a time:
+                # per-source means of this run (its SSSP uses the same hash
+                # weights) plus its applyBatch of batches of the same sizes
+                est = (queries["bfs"] * cpu["bfs_ms_per_source"] + queries["sssp"] * cpu["sssp_ms_per_source"]
+                       + cpu["batch_ms"] + cpu["batch1e5_ms"] + cpu["batch1e6_ms"])
+                queries["reference_one_call_at_a_time_ms_est"] = round(est, 1)
+                queries["speedup_vs_cpu_est"] = round(est / (queries["queries_ms"] + queries["updates_ms"]), 2)
             log(f"cpu build {cpu_build:.1f} ms")
 
     out = {

@@ -137,6 +137,10 @@ struct Scratch {
     DevBuf<uint32_t> ms_qm[3];
     DevBuf<int64_t> ms_ctl;
     DevBuf<int32_t> src;  // query sources
+    // mixed query / update lists: source degrees, output rows, trivial
+    // sources, live-source rows, host-output staging
+    DevBuf<int32_t> rb_deg, rb_rows, rb_tsrc;
+    DevBuf<uint8_t> rb_live, rb_out;
     // reachability query sweeps (a group of sweeps, V words each)
     DevBuf<uint64_t> rq_seen, rq_next, rq_tgt, rq_fb[2];
     DevBuf<int32_t> rq_fq[2], rq_tq, rq_pairs;

@@ -61,6 +61,7 @@ _SIGS = {
     "girhip_get_ranks": (ctypes.c_int, [_vp, _vp]),
     "girhip_bfs": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, _vp]),
     "girhip_bfs_noalloc": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "girhip_run_batch": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_uint64, _vp, _vp, _vp]),
     "girhip_bfs_component": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
     "girhip_reachable": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),
     "girhip_reach_info": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp]),
@@ -429,6 +430,28 @@ class Graph:
                                         _ptr(levels, np.int32, nb * V, "levels") if nb else None,
                                         _ptr(dist, np.uint64, ns * V, "dist") if ns else None))
         return levels, dist
+
+    def run_batch(self, kinds, a, b=None, delta=0, levels=None, dist=None):
+        """Mixed queries and updates in list order (girhip_run_batch): kind
+        0 BFS from a[i], 1 SSSP from a[i], 2 insert a[i] -> b[i], 3 delete
+        it.  -> (levels rows of the BFS queries, dist rows of the SSSP
+        queries, summed update stats)."""
+        kd = np.ascontiguousarray(np.asarray(kinds, dtype=np.int32))
+        av = np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+        bv = None if b is None else np.ascontiguousarray(np.asarray(b, dtype=np.int32))
+        if len(av) != len(kd) or (bv is not None and len(bv) != len(kd)):
+            raise ValueError("run_batch: kinds, a, b must have equal length")
+        nb, ns, V = int((kd == 0).sum()), int((kd == 1).sum()), self.V
+        if levels is None and nb:
+            levels = np.empty((nb, V), np.int32)
+        if dist is None and ns:
+            dist = np.empty((ns, V), np.uint64)
+        st = BatchStats()
+        _check(lib().girhip_run_batch(self._h, kd.ctypes.data, av.ctypes.data, None if bv is None else bv.ctypes.data,
+                                      len(kd), int(delta),
+                                      _ptr(levels, np.int32, nb * V, "levels") if nb else None,
+                                      _ptr(dist, np.uint64, ns * V, "dist") if ns else None, ctypes.byref(st)))
+        return levels, dist, st.as_dict()
 
     # ---- tuning ----
     def bfs_component(self, sources, undirected=False, out=None):

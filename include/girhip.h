@@ -314,6 +314,26 @@ int girhip_kernel_stat_names(girhip_graph* h, char* buf, int64_t cap, int64_t* n
 int girhip_query_batch(girhip_graph* h, const int32_t* src, const uint8_t* kind, int32_t n, uint64_t delta,
                        int32_t* levels, uint64_t* dist);
 
+/* Mixed list of n queries and edge updates, answered in list order: item i
+ * is a BFS query from a[i] (kind[i] == 0), an SSSP query from a[i] (kind 1,
+ * bucket width delta, 0 = tuned), an insert of arc a[i] -> b[i] (kind 2) or
+ * a delete of it (kind 3).  Every query sees the graph after all updates
+ * listed before it.  A maximal run of updates is applied as one batch (last
+ * op per arc wins, as applying them one at a time leaves the edge set;
+ * every built layer is repaired, as girhip_apply_batch); a maximal run of
+ * queries shares its sweeps (BFS: 64 sources per bit-parallel sweep; SSSP:
+ * multi-source batches), and sources with no out-arc take no sweep slot.
+ * levels receives one V row per BFS query and dist one V row per SSSP query,
+ * each in list order (host or device memory; either may be NULL when the
+ * list holds no query of its kind).  stats (may be NULL) sums the update
+ * runs.  b may be NULL when the list holds no update.  The reference answers
+ * such a workload one GIR::bfs / GIR::sssp / GIR::updateEdgeDirected call at
+ * a time (bench_main.cpp:538-542, gir.cpp:3043-3133); this call has no
+ * single counterpart.  girhip_last_ms kinds "run_batch_queries" /
+ * "run_batch_updates": host wall time of the query and update runs. */
+int girhip_run_batch(girhip_graph* h, const int32_t* kind, const int32_t* a, const int32_t* b, int64_t n,
+                     uint64_t delta, int32_t* levels, uint64_t* dist, girhip_batch_stats* stats);
+
 /* ---------------- tuning (no counterpart in the reference) ----------------
  * The reference fixes its knobs as constants: delta 16 (gir.h:215), BFS
  * alpha 15 / beta 18 (gir_query.cpp:161), iSpan TRIM_TIMES 3 (util.h:34).

@@ -87,3 +87,54 @@ def test_apply_large_batch(nops):
             lv = G.bfs(x)
             for i, s in enumerate(x):
                 assert np.array_equal(lv[i], oracle.bfs_textbook(fb, fa, int(s)))
+
+
+def test_run_batch_interleaved_queries_and_updates():
+    # girhip_run_batch: queries and update runs in list order; every query
+    # answers on the graph as the updates listed before it leave it, and the
+    # layers end up as a from-scratch build on the final edge set
+    V, fb, fa, bb, ba = girhip.rmat_graph(13, edge_factor=8, seed=51)
+    rng = np.random.default_rng(52)
+    deg = np.diff(fb)
+    kinds, A, B = [], [], []
+    plan = [("q", 70), ("u", 500), ("q", 40), ("u", 3000), ("q", 30), ("u", 7), ("q", 25)]
+    f1, a1, g1, h1 = fb, fa, bb, ba
+    expect = []  # per query: (kind, source, the graph it sees)
+    for what, m in plan:
+        if what == "q":
+            src = rng.integers(0, V, m).astype(np.int32)
+            src[:3] = [np.nonzero(deg == 0)[0][0], -1, V]  # no out-arc, out of range
+            kd = (rng.random(m) < 0.45).astype(np.int32)
+            kd[:3] = [0, 1, 0]
+            for k_, s_ in zip(kd, src):
+                expect.append((int(k_), int(s_), f1, a1))
+            kinds += kd.tolist()
+            A += src.tolist()
+            B += [0] * m
+        else:
+            u, v, ins = _mixed_batch(rng, f1, a1, m)
+            kinds += np.where(ins == 1, 2, 3).tolist()
+            A += u.tolist()
+            B += v.tolist()
+            f1, a1, g1, h1 = oracle.apply_batch(f1, a1, u, v, ins)
+    say("run_batch rmat13", len(kinds), "items")
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.wcc()
+        G.scc_dag()
+        lv, d, st = G.run_batch(kinds, A, B, 16)
+        ib = isp = 0
+        for k_, s_, f_, a_ in expect:
+            ok = 0 <= s_ < V
+            if k_ == 0:
+                want = oracle.bfs_textbook(f_, a_, s_) if ok else np.full(V, -1, np.int32)
+                assert np.array_equal(lv[ib], want), ("bfs", ib, s_)
+                ib += 1
+            else:
+                want = oracle.dijkstra(f_, a_, None, s_) if ok else np.full(V, INF, np.uint64)
+                assert np.array_equal(d[isp], want), ("sssp", isp, s_)
+                isp += 1
+        assert st["inserted"] + st["deleted"] > 0
+        check_layers(G, f1, a1, g1, h1, "after run_batch", rebuild=False)
+        with pytest.raises(girhip.GirError) as e:
+            G.run_batch([0, 5], [0, 0], [0, 0])
+        assert e.value.code == 1
