@@ -397,9 +397,12 @@ void repairLayers(girhip_graph* h, const Changes* C, int B) {
         h->wcc_ok = true;
     }
     if (h->scc_req) {
-        if (!h->scc_ok || changed) {
+        if (h->scc_ok && C && changed && repairScc(h, C->ik, C->ni, C->dk, C->nd, B)) {
+            // repaired from the classified changes
+        } else if (!h->scc_ok || changed) {
             runScc(h);
             runDag(h);
+            h->last_ms["batch_scc_path"] = 4;
         }
         h->scc_ok = true;
     }

@@ -24,12 +24,16 @@ struct Error {
 [[noreturn]] void raise(int code, const std::string& msg);
 void setLastError(const std::string& m);
 
+// (a failed call's error is also taken off the runtime's last-error state,
+// so that it is not reported again by a later, unrelated check)
 #define HIPCHK(x)                                                                                        \
     do {                                                                                                 \
         hipError_t e_ = (x);                                                                             \
-        if (e_ != hipSuccess)                                                                            \
+        if (e_ != hipSuccess) {                                                                          \
+            (void)hipGetLastError();                                                                     \
             ::gir::raise(e_ == hipErrorOutOfMemory ? GIRHIP_ENOMEM : GIRHIP_EHIP,                        \
                          std::string(#x) + ": " + hipGetErrorString(e_));                                \
+        }                                                                                                \
     } while (0)
 
 // Grow-only device buffer.
@@ -92,6 +96,8 @@ struct Params {
     double sssp_interleave = 2;  // 1: k-source SSSP batches share buckets and interleave distances;
                                  // 0: one layout per source; 2: 1 where it measured faster
     double sssp_ms_batch = 8;    // sources per interleaved batch (1..32)
+    double scc_trim2 = 0;        // 1: each SCC trim round also runs trim-2 (pairs)
+    double scc_incremental = 1;  // 1: update batches repair the SCC layer; 0: rebuild it
 };
 
 struct KStat {
@@ -141,6 +147,14 @@ struct Scratch {
     // sources, live-source rows, host-output staging
     DevBuf<int32_t> rb_deg, rb_rows, rb_tsrc;
     DevBuf<uint8_t> rb_live, rb_out;
+    // incremental SCC repair: counters, per-SCC "lost an inner arc" flags,
+    // condensation keys / signed counts, rank-violating inserts (SCC ends)
+    DevBuf<int64_t> ir_cnt;
+    DevBuf<uint8_t> ir_dirty;
+    DevBuf<uint64_t> ir_keys;
+    DevBuf<uint32_t> ir_vals;
+    DevBuf<int32_t> ir_bsu, ir_bsv;
+    DevBuf<uint8_t> ir_seed;  // distinct SCC ids of a sweep's seeds
     // reachability query sweeps (a group of sweeps, V words each)
     DevBuf<uint64_t> rq_seen, rq_next, rq_tgt, rq_fb[2];
     DevBuf<int32_t> rq_fq[2], rq_tq, rq_pairs;
@@ -433,7 +447,12 @@ const int32_t* fwSrc(girhip_graph* h);
 // ---- layers ----
 void runWcc(girhip_graph* h);
 void runScc(girhip_graph* h);
+void runSccActive(girhip_graph* h);
 void runDag(girhip_graph* h);
+void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks);
+// Repair the SCC layer after a batch whose sorted change keys are ik / dk
+// (scc.hip); false: rebuild it instead.
+bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* dk, int64_t nd, int B);
 
 inline Scratch& scratch(girhip_graph* h) { return h->ws; }
 // Address of the allocation-free entry point's scratch array on device

@@ -243,14 +243,16 @@ template <class Flt, class Vis>
 ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, const int32_t* rbeg,
                     const int32_t* radj, ReachBufs& B, int64_t n0, int64_t m0, int64_t n_reach, int64_t m_total,
                     Flt flt, Vis on, bool allow_bottom_up = true, int64_t* ring = nullptr,
-                    const HeavyList* hlist = nullptr) {
+                    const HeavyList* hlist = nullptr, bool any_row_heavy = false) {
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
     ReachStats st;
     if (n0 <= 0) return st;
     launch(h, "reach_seed", kReachSeed<Vis>, gridFor(n0, blk), blk, 0, (const int32_t*)B.q[0], n0, B.vis, on);
     // level counter ring and heavy list: the caller's (2 * REACH_RING int64;
-    // no persistent kernel then), else the handle's scratch
+    // no persistent kernel then), else the handle's scratch.  any_row_heavy:
+    // the rows are not the edge store's (the condensation's: a row there can
+    // outgrow every vertex's), so the heavy-row pass always runs.
     Scratch& S = scratch(h);
     int64_t* c = ring;
     if (!c) {
@@ -269,7 +271,7 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
     int c_prev = 1;
     int cur = 0;
     // persistent levels need every row below the heavy-list threshold
-    bool persist_ok = !ring && h->prm.persist != 0 && 2 * maxDegree(h) <= HEAVY_T;  // (closure rows too)
+    bool persist_ok = !ring && !any_row_heavy && h->prm.persist != 0 && 2 * maxDegree(h) <= HEAVY_T;  // (closure rows too)
     int32_t L = 0;
     st.reached = n0;
     while (nf > 0) {
@@ -341,7 +343,10 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
             const TopDownF<Flt, Vis> f{pbeg, padj, B.vis, L + t, B.q[cur ^ 1], pair(L + t), flt, on};
             launch(h, "reach_topdown", kTopDown<Flt, Vis>, g, blk, 0, (const int32_t*)B.q[cur],
                    (const int64_t*)pair(L + t - 1), f, pair(L + t + 1), HL);
-            launchHeavyStoreH(h, HL, "reach_topdown_heavy", f, pair(L + t) + 1);
+            if (any_row_heavy)
+                launch(h, "reach_topdown_heavy", kHeavyRows<TopDownF<Flt, Vis>>, 1024, 256, 0, HL, f, pair(L + t) + 1);
+            else
+                launchHeavyStoreH(h, HL, "reach_topdown_heavy", f, pair(L + t) + 1);
             cur ^= 1;
         }
         int64_t r[2 * REACH_RING];

@@ -7,6 +7,10 @@
 //
 //   trim-1   a vertex with no active out-neighbour or no active in-neighbour
 //            (self loops ignored) is a singleton SCC; repeated;
+//   trim-2   (knob scc_trim2) two vertices whose only active out-neighbours
+//            (or only active in-neighbours) are each other form an SCC of two
+//            (the reference's iSpan trim_2_from_fq, includes/scc_cpu/
+//            trim_2_3.h:161-308); run in each trim round after trim-1;
 //   FW-BW    from the pivot maximising outdeg x indeg (the rule of the
 //            reference's iSpan-style side tree, includes/scc_cpu/
 //            scc_common.h:98-158), forward and backward reachability over the
@@ -102,6 +106,52 @@ __global__ void kTrim1(const int32_t* __restrict__ list, int64_t n, const int32_
     // flag with a device-coherent store unless it reads it raised already
     // (not a contended atomic per wave)
     if (__syncthreads_or(trim) && threadIdx.x == 0 && ldAcq(cnt) == 0) stRel(cnt, (int64_t)1);
+}
+
+// The only active neighbour of v other than v in its row (beg/adj): -1 if
+// none, -2 if two or more.
+__device__ __forceinline__ int32_t onlyActive(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj,
+                                              const int32_t* scc, int32_t v) {
+    int32_t one = -1;
+    const int32_t b = beg[v], e = beg[v + 1];
+    for (int32_t j = b; j < e; ++j) {
+        const int32_t w = adj[j];
+        if (w == v || scc[w] != -1) continue;
+        if (one >= 0 && w != one) return -2;
+        one = w;
+    }
+    return one;
+}
+
+// Trim-2: v's only active out-neighbour w has v as its only active
+// out-neighbour (or the same with in-neighbours): {v, w} has no arc to (from)
+// any other active vertex and is strongly connected, so it is an SCC.  Both
+// ends are labelled by whichever of their threads sees the pair first (the
+// minimum id, the canonical label), so the partner never runs trim-1 alone.
+// Labels set during the pass only remove finished SCCs from the active set,
+// on which the rule stays sound.
+__global__ void kTrim2(const int32_t* __restrict__ list, int64_t n, const int32_t* __restrict__ fb,
+                       const int32_t* __restrict__ fa, const int32_t* __restrict__ bb, const int32_t* __restrict__ ba,
+                       int32_t* scc, int64_t* cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (i < n) {
+        const int32_t v = list[i];
+        if (scc[v] == -1) {
+            int32_t w = onlyActive(fb, fa, scc, v);
+            if (!(w >= 0 && onlyActive(fb, fa, scc, w) == v)) {
+                w = onlyActive(bb, ba, scc, v);
+                if (!(w >= 0 && onlyActive(bb, ba, scc, w) == v)) w = -1;
+            }
+            if (w >= 0) {
+                const int32_t lab = v < w ? v : w;
+                stRel(&scc[v], lab);
+                stRel(&scc[w], lab);
+                hit = true;
+            }
+        }
+    }
+    if (__syncthreads_or(hit) && threadIdx.x == 0 && ldAcq(cnt) == 0) stRel(cnt, (int64_t)1);
 }
 
 // argmax over the active list of min(outdeg * indeg, 2^33 - 1), ties to the
@@ -447,6 +497,10 @@ int64_t trimRounds(girhip_graph* h, int32_t*& list, int32_t*& alt, int64_t n, in
         launch(h, "scc_trim1", kTrim1, gridFor(n, blk), blk, 0, (const int32_t*)list, n, (const int32_t*)h->fw_beg.p,
                (const int32_t*)h->fw_adj.p, (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p, h->scc.p,
                h->dcnt.p + 3);
+        if (h->prm.scc_trim2 != 0)
+            launch(h, "scc_trim2", kTrim2, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
+                   (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p, (const int32_t*)h->bw_beg.p,
+                   (const int32_t*)h->bw_adj.p, h->scc.p, h->dcnt.p + 3);
         int64_t trimmed;
         readCounters(h, h->dcnt.p + 3, 1, &trimmed);
         if (trimmed == 0) break;
@@ -474,16 +528,24 @@ struct SccBlock {
 }  // namespace
 
 void runScc(girhip_graph* h) {
-    const SccBlock sb(h);
     const int32_t V = h->V;
-    const int blk = (int)h->prm.block;
-    Scratch& S = scratch(h);
     h->scc.ensure((size_t)(V > 0 ? V : 1));
     if (V == 0) {
         h->n_scc = 0;
         return;
     }
     fillI32(h, h->scc.p, -1, V);
+    runSccActive(h);
+}
+
+// SCCs of the vertices labelled -1 ("active"); every other label stays.  The
+// active set must be a union of whole SCCs of the current graph (all of it
+// for a build; the affected SCCs for a repair after an update batch).
+void runSccActive(girhip_graph* h) {
+    const SccBlock sb(h);
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
     S.q[2].ensure(V);
     S.iv[0].ensure(V);
     S.iv[1].ensure(V);
@@ -570,11 +632,7 @@ void runDag(girhip_graph* h) {
     const int blk = (int)h->prm.block;
     Scratch& S = scratch(h);
     const int B = bitsFor((uint64_t)(V > 1 ? V - 1 : 1));
-    h->rank.ensure((size_t)(V > 0 ? V : 1));
-    h->dag_beg.ensure((size_t)V + 1);
-    S.iv[2].ensure((size_t)V + 1);
-    int32_t* ideg = S.iv[2].p;
-    fillI32(h, ideg, 0, (int64_t)V + 1);
+    h->mem_ok = false;  // member lists follow the labels
     int64_t ncross = 0;
     if (h->E > 0) {
         S.keys[0].ensure((size_t)h->E);
@@ -601,6 +659,24 @@ void runDag(girhip_graph* h) {
         h->dag_mult.ensure((size_t)nd);
         launch(h, "dag_write", kDagWrite, gridFor(nd, blk), blk, 0, (const uint64_t*)S.keys[0].p,
                (const int32_t*)S.q[2].p, nd, ncross, B, h->dag_src.p, h->dag_dst.p, h->dag_mult.p);
+    }
+    dagFinish(h, nd, B, true);
+}
+
+// With the arcs (dag_src / dst / mult, nd of them, sorted by (src, dst))
+// written: the in-arc CSR, the out-arc offsets and, with ranks, the Kahn
+// levels.
+void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    h->rank.ensure((size_t)(V > 0 ? V : 1));
+    h->dag_beg.ensure((size_t)V + 1);
+    S.iv[2].ensure((size_t)V + 1);
+    int32_t* ideg = S.iv[2].p;
+    fillI32(h, ideg, 0, (int64_t)V + 1);
+    if (nd > 0) {
+        S.keys[0].ensure((size_t)nd);
         // in-degrees from the arcs sorted by destination (no contended atomics)
         launch(h, "dag_tkeys", kDagTKeys, gridFor(nd, blk), blk, 0, (const int32_t*)h->dag_src.p,
                (const int32_t*)h->dag_dst.p, nd, B, S.keys[0].p);
@@ -618,10 +694,10 @@ void runDag(girhip_graph* h) {
         fillI32(h, h->dag_in_beg.p, 0, (int64_t)V + 1);
     }
     h->n_dag = nd;
-    h->mem_ok = false;
     // DAG CSR offsets by source SCC (arcs are sorted by src)
     launch(h, "dag_beg", kDagBeg, gridFor((int64_t)V + 1, blk), blk, 0, (const int32_t*)h->dag_src.p, nd, V,
            h->dag_beg.p);
+    if (!ranks) return;
     // Kahn levels
     ReachBufs R = reachBufs(h);
     fillI64(h, h->dcnt.p + 12, 0, 1);
@@ -653,6 +729,285 @@ void runDag(girhip_graph* h) {
         readCounters(h, kc + L % KAHN_RING, 1, &nq);
         if (nq == 0) break;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Repair of the SCC layer after an update batch, instead of a rebuild.
+// Every change (u, v) is classified by the labels and ranks the layer held
+// before the batch (the reference's cases, gir.cpp:3007-3036 for inserts and
+// 3062-3131 for deletes, taken over a whole batch as applyBatch does,
+// gir.cpp:3634-4111):
+//   insert inside an SCC                -> nothing;
+//   insert between SCCs in rank order   -> condensation arc +1;
+//   insert against the rank order       -> condensation arc +1, and it may
+//                                          close a cycle (SCCs merge);
+//   delete between SCCs                 -> condensation arc -1;
+//   delete inside an SCC                -> the SCC may split.
+// The multiplicity changes are merged into the condensation first (a sort of
+// its arcs and the changes by key, summed per key).  With no merge or split
+// possible that is all: the partition stands and every arc still goes up
+// in rank.  Otherwise the SCCs that may change - each SCC that lost an inner
+// arc, and each SCC both reachable from the head SCC of a rank-violating
+// insert and reaching the tail SCC of one, over the updated condensation with
+// the ranks outside [min head rank, max tail rank] pruned (a cycle through
+// such an insert stays inside that window) - are relabelled -1 and the SCC
+// pipeline runs on them alone: a union of whole old SCCs that holds every
+// new cycle holds every new SCC that differs.  A changed partition rebuilds
+// the condensation and its ranks; rank-violating inserts that closed no
+// cycle re-rank it.
+namespace {
+
+enum : int { IR_DELTA = 0, IR_BACK, IR_DIRTY, IR_MINR, IR_MAXR, IR_AFF, IR_N };
+
+__global__ void kIrClassify(const uint64_t* __restrict__ ik, int64_t ni, const uint64_t* __restrict__ dk, int64_t nd,
+                            int B, const int32_t* __restrict__ scc, const int64_t* __restrict__ rank,
+                            uint64_t* __restrict__ dkeys, uint32_t* __restrict__ dvals, int32_t* __restrict__ bsu,
+                            int32_t* __restrict__ bsv, uint8_t* __restrict__ dirty, int64_t* cnt) {
+    const uint64_t mask = (1ULL << B) - 1;
+    const int64_t n = ni + nd, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+        const int64_t i = base + laneId();
+        const bool act = i < n, ins = i < ni;
+        int32_t su = 0, sv = 0;
+        if (act) {
+            const uint64_t k = ins ? ik[i] : dk[i - ni];
+            su = scc[(int32_t)(k >> B)];
+            sv = scc[(int32_t)(k & mask)];
+        }
+        const bool cross = act && su != sv;
+        const bool back = cross && ins && rank[su] >= rank[sv];
+        const bool inner_del = act && !ins && su == sv;
+        if (inner_del) dirty[su] = 1;
+        const int64_t dpos = wavePush(cnt + IR_DELTA, cross);
+        if (cross) {
+            dkeys[dpos] = ((uint64_t)(uint32_t)su << B) | (uint32_t)sv;
+            dvals[dpos] = ins ? 1u : 0xffffffffu;  // +1 / -1
+        }
+        const int64_t bpos = wavePush(cnt + IR_BACK, back);
+        if (back) {
+            bsu[bpos] = su;
+            bsv[bpos] = sv;
+        }
+        // rank window of the rank-violating inserts (one atomic per wave)
+        long long mn = back ? rank[sv] : LLONG_MAX, mx = back ? rank[su] : -1;
+        for (int o = WAVE / 2; o > 0; o >>= 1) {
+            const long long a = __shfl_xor(mn, o), b = __shfl_xor(mx, o);
+            mn = a < mn ? a : mn;
+            mx = b > mx ? b : mx;
+        }
+        const int ndel = __popcll(__ballot(inner_del));
+        if (laneId() == 0) {
+            if (mx >= 0) {
+                atomicMin((long long*)(cnt + IR_MINR), mn);
+                atomicMax((long long*)(cnt + IR_MAXR), mx);
+            }
+            if (ndel) atomicAdd((unsigned long long*)(cnt + IR_DIRTY), (unsigned long long)ndel);
+        }
+    }
+}
+
+// The condensation's arcs as keys with their multiplicities.
+__global__ void kIrDagKeys(const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
+                           const int32_t* __restrict__ mult, int64_t nd, int B, uint64_t* __restrict__ keys,
+                           uint32_t* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nd) {
+        keys[i] = ((uint64_t)(uint32_t)src[i] << B) | (uint32_t)dst[i];
+        vals[i] = (uint32_t)mult[i];
+    }
+}
+
+// Sorted (key, signed count) pairs: at the first element of every key run,
+// the run's sum (int32) and a flag that it is positive.
+__global__ void kIrRunSums(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
+                           int32_t* __restrict__ flag, int32_t* __restrict__ sum) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t f = 0;
+    if (i == 0 || keys[i] != keys[i - 1]) {
+        int32_t acc = 0;
+        for (int64_t j = i; j < n && keys[j] == keys[i]; ++j) acc += (int32_t)vals[j];
+        sum[i] = acc;
+        f = acc > 0;
+    }
+    flag[i] = f;
+}
+
+__global__ void kIrDagWrite(const uint64_t* __restrict__ keys, const int32_t* __restrict__ flag,
+                            const int32_t* __restrict__ sum, const int64_t* __restrict__ pos, int64_t n, int B,
+                            int32_t* __restrict__ src, int32_t* __restrict__ dst, int32_t* __restrict__ mult) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    const int64_t r = pos[i];
+    src[r] = (int32_t)(keys[i] >> B);
+    dst[r] = (int32_t)(keys[i] & ((1ULL << B) - 1));
+    mult[r] = sum[i];
+}
+
+// Condensation sweeps inside a rank window: forward visits rank <= lim,
+// backward rank >= lim (ids that name no SCC carry rank -1).
+struct RankWindow {
+    const int64_t* rank;
+    int64_t lim;
+    int fwd;
+    __device__ bool self(int32_t c) const {
+        const int64_t r = rank[c];
+        return r >= 0 && (fwd ? r <= lim : r >= lim);
+    }
+    __device__ bool operator()(int32_t c, int32_t) const { return self(c); }
+};
+
+// The distinct ids among n seed ids: flag them, then list the flagged ids
+// (a batch can hold more rank-violating inserts than there are SCCs).
+__global__ void kIrSeedFlags(const int32_t* __restrict__ ids, int64_t n, uint8_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[ids[i]] = 1;
+}
+__global__ void kIrSeedList(const uint8_t* __restrict__ flag, int32_t V, int32_t* __restrict__ out, int64_t* cnt) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool f = v < V && flag[v];
+    const int64_t pos = blockReserve(f ? 1 : 0, cnt);
+    if (f) out[pos] = (int32_t)v;
+}
+
+// Vertices of the SCCs that may change: cnt += their number; with apply,
+// they are also labelled -1.
+__global__ void kIrMark(int32_t* scc, int32_t V, const uint8_t* __restrict__ dirty, const uint64_t* __restrict__ fw,
+                        const uint64_t* __restrict__ bw, int use_region, int apply, int64_t* cnt) {
+    int64_t a = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = scc[v];
+        const bool in = dirty[c] || (use_region && bitOf(fw, c) && bitOf(bw, c));
+        if (in) {
+            if (apply) scc[v] = -1;
+            ++a;
+        }
+    }
+    blockAtomicAdd(a, cnt);
+}
+
+}  // namespace
+
+// The change lists of a batch just applied to the store (ik / dk: sorted
+// packed (u << B) | v keys).  Returns false when the layer must be rebuilt
+// instead: knob scc_incremental = 0, a batch of more than
+// max(4096, V / 1024) changes, or more than max(65536, V / 8) vertices in
+// SCCs that may change (a rebuild then costs about the same and skips the
+// repair's own passes; on a power-law graph a random batch almost always
+// deletes an arc inside the giant SCC).
+bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* dk, int64_t nd, int B) {
+    if (h->prm.scc_incremental == 0) return false;
+    if (ni + nd > std::max<int64_t>(4096, (int64_t)h->V / 1024)) return false;
+    const SccBlock sb(h);
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    const int64_t n = ni + nd;
+    h->last_ms["batch_scc_path"] = 0;
+    h->last_ms["batch_scc_affected"] = 0;
+    if (n == 0 || V == 0) return true;
+    S.ir_cnt.ensure(IR_N);
+    S.ir_dirty.ensure((size_t)V);
+    S.ir_bsu.ensure((size_t)std::max<int64_t>(ni, 1));
+    S.ir_bsv.ensure((size_t)std::max<int64_t>(ni, 1));
+    const int64_t cap = h->n_dag + n;
+    S.ir_keys.ensure((size_t)cap);
+    S.ir_vals.ensure((size_t)cap);
+    int64_t* cnt = S.ir_cnt.p;
+    fillI64(h, cnt, 0, IR_N);
+    fillI64(h, cnt + IR_MINR, LLONG_MAX, 1);
+    fillI64(h, cnt + IR_MAXR, -1, 1);
+    fillU8(h, S.ir_dirty.p, 0, V);
+    // deltas go after the condensation's own arcs in the key array
+    const int64_t nd0 = h->n_dag;
+    launch(h, "ir_classify", kIrClassify, std::min<unsigned>(gridFor(n, blk), 4096), blk, 0, ik, ni, dk, nd, B,
+           (const int32_t*)h->scc.p, (const int64_t*)h->rank.p, S.ir_keys.p + nd0, S.ir_vals.p + nd0, S.ir_bsu.p,
+           S.ir_bsv.p, S.ir_dirty.p, cnt);
+    int64_t c[IR_N];
+    readCounters(h, cnt, IR_N, c);
+    const int64_t ndelta = c[IR_DELTA], nback = c[IR_BACK], ndirty = c[IR_DIRTY];
+    if (ndelta == 0 && ndirty == 0) {  // inserts inside SCCs only
+        h->last_ms["batch_scc_path"] = 1;
+        return true;
+    }
+    const int64_t big = std::max<int64_t>(65536, (int64_t)V / 8);
+    auto affected = [&](bool region, bool apply) {
+        fillI64(h, cnt + IR_AFF, 0, 1);
+        launch(h, "ir_mark", kIrMark, boundedGrid(V, blk), blk, 0, h->scc.p, V, (const uint8_t*)S.ir_dirty.p,
+               (const uint64_t*)S.bits[3].p, (const uint64_t*)reachBufs(h).vis, region ? 1 : 0, apply ? 1 : 0,
+               cnt + IR_AFF);
+        int64_t a;
+        readCounters(h, cnt + IR_AFF, 1, &a);
+        return a;
+    };
+    if (ndirty > 0 && affected(false, false) > big) return false;
+    // condensation multiplicities
+    if (ndelta > 0) {
+        const int64_t m = nd0 + ndelta;
+        if (nd0 > 0)
+            launch(h, "ir_dag_keys", kIrDagKeys, gridFor(nd0, blk), blk, 0, (const int32_t*)h->dag_src.p,
+                   (const int32_t*)h->dag_dst.p, (const int32_t*)h->dag_mult.p, nd0, B, S.ir_keys.p, S.ir_vals.p);
+        radixSort(h, S.ir_keys.p, S.ir_vals.p, m, 2 * B);
+        S.iv[0].ensure((size_t)m);
+        S.iv[1].ensure((size_t)m);
+        S.off2.ensure((size_t)m + 1);
+        launch(h, "ir_run_sums", kIrRunSums, gridFor(m, blk), blk, 0, (const uint64_t*)S.ir_keys.p,
+               (const uint32_t*)S.ir_vals.p, m, S.iv[0].p, S.iv[1].p);
+        scanI32toI64(h, S.iv[0].p, S.off2.p, m);
+        int64_t nd1;
+        readCounters(h, S.off2.p + m, 1, &nd1);
+        h->dag_src.ensureSlack((size_t)std::max<int64_t>(nd1, 1));
+        h->dag_dst.ensureSlack((size_t)std::max<int64_t>(nd1, 1));
+        h->dag_mult.ensureSlack((size_t)std::max<int64_t>(nd1, 1));
+        launch(h, "ir_dag_write", kIrDagWrite, gridFor(m, blk), blk, 0, (const uint64_t*)S.ir_keys.p,
+               (const int32_t*)S.iv[0].p, (const int32_t*)S.iv[1].p, (const int64_t*)S.off2.p, m, B, h->dag_src.p,
+               h->dag_dst.p, h->dag_mult.p);
+        dagFinish(h, nd1, B, false);
+    }
+    if (nback == 0 && ndirty == 0) {  // the partition and the ranks stand
+        h->last_ms["batch_scc_path"] = 1;
+        return true;
+    }
+    // SCCs that may merge: reachable from a rank-violating insert's head and
+    // reaching its tail (or another's) inside the rank window
+    ReachBufs R = reachBufs(h);
+    S.bits[3].ensure((size_t)R.W);
+    if (nback > 0) {
+        S.ir_seed.ensure((size_t)V);
+        // distinct seed SCCs into the sweep's first queue -> their number
+        auto seeds = [&](const int32_t* ids) {
+            fillU8(h, S.ir_seed.p, 0, V);
+            launch(h, "ir_seed_flags", kIrSeedFlags, gridFor(nback, blk), blk, 0, ids, nback, S.ir_seed.p);
+            fillI64(h, cnt + IR_AFF, 0, 1);
+            launch(h, "ir_seed_list", kIrSeedList, gridFor(V, blk), blk, 0, (const uint8_t*)S.ir_seed.p, V, R.q[0],
+                   cnt + IR_AFF);
+            int64_t ns;
+            readCounters(h, cnt + IR_AFF, 1, &ns);
+            return ns;
+        };
+        fillU64(h, R.vis, 0, R.W);
+        int64_t ns = seeds(S.ir_bsv.p);
+        runReach(h, h->dag_beg.p, h->dag_dst.p, h->dag_in_beg.p, h->dag_in_src.p, R, ns, 0, h->n_scc, h->n_dag,
+                 RankWindow{h->rank.p, c[IR_MAXR], 1}, NoVisit{}, true, nullptr, nullptr, true);
+        HIPCHK(hipMemcpyAsync(S.bits[3].p, R.vis, sizeof(uint64_t) * R.W, hipMemcpyDeviceToDevice, h->st));
+        fillU64(h, R.vis, 0, R.W);
+        ns = seeds(S.ir_bsu.p);
+        runReach(h, h->dag_in_beg.p, h->dag_in_src.p, h->dag_beg.p, h->dag_dst.p, R, ns, 0, h->n_scc, h->n_dag,
+                 RankWindow{h->rank.p, c[IR_MINR], 0}, NoVisit{}, true, nullptr, nullptr, true);
+    }
+    int64_t naff = affected(nback > 0, false);
+    h->last_ms["batch_scc_affected"] = (double)naff;
+    if (naff > big) return false;
+    if (naff == 0) {  // no cycle closed: re-rank the condensation
+        h->last_ms["batch_scc_path"] = 2;
+        dagFinish(h, h->n_dag, B, true);
+        return true;
+    }
+    h->last_ms["batch_scc_path"] = 3;
+    affected(nback > 0, true);
+    runSccActive(h);
+    runDag(h);
+    return true;
 }
 
 }  // namespace gir

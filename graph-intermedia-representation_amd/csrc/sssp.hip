@@ -1101,7 +1101,9 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         copyIn(h, S.src.p, s.data(), sizeof(int32_t) * (size_t)k);
         // 32-bit distances unless weights alone could overflow them
         const bool try32 = h->prm.sssp_dist32 != 0 && (uint64_t)maxWeight(h) < 0x7fffffffull;
-        const SplitView* sv = k > 0 ? splitView(h, delta) : nullptr;
+        // the split view separates light from heavy arcs; with no arc
+        // heavier than delta the rows are all light as they stand
+        const SplitView* sv = (k > 0 && (uint64_t)maxWeight(h) > delta) ? splitView(h, delta) : nullptr;
         // Engine: the interleaved one shares arcs between sources whose
         // distances fall in the same bucket, which pays once buckets are
         // wide; with narrow buckets and a heavy phase per bucket the

@@ -272,7 +272,7 @@ def test_tuning_knobs_do_not_change_results():
                      ("wcc_rounds", 0), ("scc_trim_rounds", 0), ("scc_pivot", 0), ("sssp_delta", 1),
                      ("persist", 1), ("chunk", 1), ("sssp_dist32", 0), ("xcd", 1), ("split", 0), ("scc_block", 256), ("scc_block", 1024),
                      ("sssp_delta", 100000), ("sssp_interleave", 0), ("sssp_interleave", 1),
-                     ("sssp_ms_batch", 3), ("sssp_ms_batch", 32)]:
+                     ("sssp_ms_batch", 3), ("sssp_ms_batch", 32), ("scc_trim2", 1)]:
             say("knob", k, v)
             old = G.get_param(k)
             G.set_param(k, v)
@@ -315,6 +315,33 @@ def test_sssp_interleaved_batch_sizes():
         d = G.sssp(base[:8], 1 << 30)
         for i, x in enumerate(base[:8]):
             assert (d[i] == oracle.dijkstra(fb, fa, big, int(x))).all()
+
+
+def test_trim2_pairs():
+    # trim-2 (trim_2_3.h:161-308): a chain of 2-cycles (pair i -> pair i+1)
+    # is peeled one pair per trim round from its end (out-direction rule)
+    # and, reversed, from its start (in-direction rule); mixed with a random
+    # graph.  Labels equal the oracle's with and without trim-2.
+    n = 300
+    a = np.arange(n)
+    src = np.concatenate([2 * a, 2 * a + 1, 2 * a[:-1] + 1])
+    dst = np.concatenate([2 * a + 1, 2 * a, 2 * a[1:]])
+    V0 = 2 * n
+    rs, rd = girhip.uniform_edges(4000, 12000, seed=44)
+    src = np.concatenate([src, rs + V0, np.array([V0 - 1])])
+    dst = np.concatenate([dst, rd + V0, np.array([V0 + 5])])
+    V = V0 + 4000
+    for flip in (False, True):
+        s_, d_ = (dst, src) if flip else (src, dst)
+        fb, fa, bb, ba = girhip.csr_from_edges(V, s_, d_)
+        want, nw = oracle.scc(fb, fa)
+        say("trim2 chain flip", flip)
+        with girhip.Graph(fb, fa, bb, ba) as G:
+            for t2, rounds in ((0, 3), (1, 3), (1, 1000), (1, 0)):
+                G.set_param("scc_trim2", t2)
+                G.set_param("scc_trim_rounds", rounds)
+                s, ns, _ = G.scc_dag()
+                assert ns == nw and np.array_equal(s, want), (flip, t2, rounds)
 
 
 def test_unsorted_duplicate_input_is_normalised():
@@ -416,6 +443,54 @@ def test_batch_sequence_matches_recompute():
             lv = G.bfs(src)
             for i, x in enumerate(src):
                 assert (lv[i] == oracle.bfs_textbook(fb, fa, x)).all()
+
+
+def test_incremental_scc_repair_paths():
+    # the repair of the SCC layer after a batch (classified changes instead of
+    # a rebuild): each path against the oracle, and the path the library took
+    # ("batch_scc_path": 1 multiplicities only, 2 re-rank, 3 relabel the SCCs
+    # that may change)
+    g = load_golden("example_graph")
+    fb, fa = g["fw_beg"], g["fw_adj"]
+    V = len(fb) - 1
+    with girhip.Graph(fb, fa) as G:
+        G.wcc()
+        s0, _, _ = G.scc_dag()
+        r = G.ranks()
+        rk = lambda x: r[s0[x]]
+        present = set(zip(np.repeat(np.arange(V), np.diff(fb)).tolist(), fa.tolist()))
+        # a forward pair between SCCs in different WCCs (no path either way)
+        pairs = [(a, b) for a in range(V) for b in range(V) if s0[a] != s0[b] and (a, b) not in present
+                 and ((a < 12) != (b < 12))]
+        fwd = next((a, b) for a, b in pairs if rk(a) < rk(b))
+        back = next((a, b) for a, b in pairs if rk(a) > rk(b))
+        steps = [("insert inside an SCC", (0, 2, 1), 1), ("insert in rank order", fwd + (1,), 1),
+                 ("delete between SCCs", (3, 5, 0), 1), ("insert against rank order, no cycle", back + (1,), 2),
+                 ("insert closing a cycle", (6, 0, 1), 3), ("delete inside an SCC", (9, 2, 0), 3)]
+        for what, (u, v, ins), path in steps:
+            say("repair path:", what)
+            G.apply_batch([u], [v], [ins])
+            fb, fa, bb, ba = oracle.apply_batch(fb, fa, [u], [v], [ins])
+            check_layers(G, fb, fa, bb, ba, what, rebuild=False)
+            assert G.last_ms("batch_scc_path") == path, (what, G.last_ms("batch_scc_path"))
+    # random batches on a graph with a large SCC: same layers with the repair
+    # switched off; inner deletes that keep the SCC together take path 3 too
+    V, fb, fa, bb, ba = girhip.rmat_graph(12, seed=37)
+    rng = np.random.default_rng(38)
+    with girhip.Graph(fb, fa, bb, ba) as G, girhip.Graph(fb, fa, bb, ba) as H:
+        H.set_param("scc_incremental", 0)
+        for X in (G, H):
+            X.wcc()
+            X.scc_dag()
+        for n in (1, 1, 5, 50, 2000):
+            u, v, ins = _random_batch(rng, fb, fa, n, V)
+            say("repair vs rebuild n", n)
+            G.apply_batch(u, v, ins)
+            H.apply_batch(u, v, ins)
+            fb, fa, bb, ba = oracle.apply_batch(fb, fa, u, v, ins)
+            check_layers(G, fb, fa, bb, ba, f"repair n={n}", rebuild=False)
+            assert np.array_equal(G.get_scc()[0], H.get_scc()[0])
+            assert H.last_ms("batch_scc_path") == 4
 
 
 def test_ingest_then_sync():
