@@ -249,15 +249,25 @@ extern "C" int girhip_query_batch(girhip_graph* h, const int32_t* src, const uin
     });
 }
 
-// Learned tuning: features -> MLP (on the device) -> delta and trim rounds.
+// SCC schedules the learned model chooses among (its second output is an
+// index into this table): (pivot rule, trim rounds), in order of the work done
+// before colouring.  Pivot rule 0: the remainder goes straight to colouring;
+// 1: one forward-backward sweep from the vertex of largest out * in degree
+// (the iSpan rule, scc_common.h:98-158).  The reference's default is (1, 3).
+constexpr int N_SCHED = 9;
+constexpr int SCHED[N_SCHED][2] = {{0, 1}, {0, 3}, {1, 0}, {1, 1}, {1, 2}, {1, 3}, {1, 4}, {1, 6}, {1, 8}};
+
+// Learned tuning: features -> MLP (on the device) -> delta and SCC schedule.
 // norm: 36 floats = feature mean[16], feature scale[16], target mean[2],
-// target scale[2]; outputs are log2(delta) and log2(trim rounds + 1).
-// The reference's defaults are the fallback (non-finite or out-of-range
-// predictions, features far outside the training range) and, when judge > 0,
-// the judge: SSSP from min(k, judge) of the sources with the predicted delta
-// and with the default; the faster one is kept.  out (may be NULL): 4 doubles
-// = chosen delta, chosen trim rounds, raw predicted delta, 1 if the
-// prediction was used for delta.
+// target scale[2]; outputs are log2(delta) and an index into SCHED.
+// The reference's defaults (delta 16, pivot rule 1 with 3 trim rounds) are
+// the fallback (non-finite or out-of-range predictions, features far outside
+// the training range) and, when judge > 0, the judge of both predictions:
+// SSSP from min(k, judge) of the sources with the predicted delta and with
+// the default, and one buildSCC_DAG with each schedule; the faster setting is
+// kept.  out (may be NULL): 6 doubles = chosen delta, chosen trim rounds, raw
+// predicted delta, 1 if the predicted delta was kept, chosen pivot rule, 1
+// if the predicted SCC schedule was kept.
 extern "C" int girhip_learned_tune(girhip_graph* h, girhip_mlp* model, const float* norm, const int32_t* src,
                                    int32_t k, int32_t judge, double* out) {
     return guarded([&] {
@@ -272,15 +282,21 @@ extern "C" int girhip_learned_tune(girhip_graph* h, girhip_mlp* model, const flo
             if (!std::isfinite(x[i]) || std::fabs(x[i]) > 8) sane = false;  // far outside the training data
         }
         if (girhip_mlp_forward(model, x, 1, y) != GIRHIP_OK) raise(GIRHIP_EHIP, girhip_last_error());
-        const double ld = y[0] * norm[34] + norm[32], lt = y[1] * norm[35] + norm[33];
+        const double ld = y[0] * norm[34] + norm[32], ls = y[1] * norm[35] + norm[33];
         const Params defaults;
-        double delta = defaults.sssp_delta, trim = defaults.scc_trim_rounds, raw = std::exp2(ld);
-        bool used = false;
+        double delta = defaults.sssp_delta, raw = std::exp2(ld);
+        double trim = defaults.scc_trim_rounds, pivot = defaults.scc_pivot;
+        bool used = false, sched_used = false;
         if (sane && std::isfinite(ld) && ld >= 0 && ld <= 24) {
             delta = std::max(1.0, std::round(std::exp2(ld)));
             used = true;
         }
-        if (sane && std::isfinite(lt) && lt >= 0 && lt <= 5) trim = std::round(std::exp2(lt) - 1);
+        if (sane && std::isfinite(ls) && ls > -0.5 && ls < N_SCHED - 0.5) {
+            const int si = (int)std::lround(ls);
+            pivot = SCHED[si][0];
+            trim = SCHED[si][1];
+            sched_used = !(pivot == defaults.scc_pivot && trim == defaults.scc_trim_rounds);
+        }
         if (used && judge > 0 && k > 0 && src) {
             const int32_t kj = std::min(k, judge);
             h->prm.sssp_delta = defaults.sssp_delta;
@@ -294,13 +310,30 @@ extern "C" int girhip_learned_tune(girhip_graph* h, girhip_mlp* model, const flo
                 used = false;
             }
         }
+        if (sched_used && judge > 0) {
+            auto build = [&](double p, double t) {
+                h->prm.scc_pivot = p;
+                h->prm.scc_trim_rounds = t;
+                if (girhip_scc_dag(h, nullptr, nullptr, nullptr) != GIRHIP_OK) raise(GIRHIP_EHIP, girhip_last_error());
+                return h->last_ms["scc_dag"];
+            };
+            const double t_def = build(defaults.scc_pivot, defaults.scc_trim_rounds);
+            if (build(pivot, trim) > t_def) {
+                pivot = defaults.scc_pivot;
+                trim = defaults.scc_trim_rounds;
+                sched_used = false;
+            }
+        }
         h->prm.sssp_delta = delta;
         h->prm.scc_trim_rounds = trim;
+        h->prm.scc_pivot = pivot;
         if (out) {
             out[0] = delta;
             out[1] = trim;
             out[2] = raw;
             out[3] = used ? 1 : 0;
+            out[4] = pivot;
+            out[5] = sched_used ? 1 : 0;
         }
     });
 }

@@ -572,14 +572,16 @@ class Graph:
         return best.value, times
 
     def learned_tune(self, model, norm, sources, judge=0):
-        """Set delta and SCC trim rounds from the learned model (defaults as
-        fallback / judge) -> dict(delta, trim, raw_delta, used)."""
+        """Set delta and the SCC schedule (pivot rule, trim rounds) from the
+        learned model (defaults as fallback / judge) -> dict(delta, trim,
+        raw_delta, used, pivot, schedule_used)."""
         src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
         norm = np.ascontiguousarray(np.asarray(norm, dtype=np.float32))
-        out = np.zeros(4, np.float64)
+        out = np.zeros(6, np.float64)
         _check(lib().girhip_learned_tune(self._h, model._m, norm.ctypes.data, src.ctypes.data if len(src) else None,
                                          len(src), int(judge), out.ctypes.data))
-        return {"delta": out[0], "trim": out[1], "raw_delta": out[2], "used": bool(out[3])}
+        return {"delta": out[0], "trim": out[1], "raw_delta": out[2], "used": bool(out[3]), "pivot": out[4],
+                "schedule_used": bool(out[5])}
 
     def all_kernel_stats(self):
         """{kernel name: (launches, ms)} recorded in profile mode."""

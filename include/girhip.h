@@ -368,12 +368,16 @@ int girhip_mlp_grad(girhip_mlp* m, const float* x, const float* t, int32_t n, fl
 int girhip_mlp_train(girhip_mlp* m, const float* x, const float* t, int32_t n, int32_t epochs, float lr,
                      float* loss);
 
-/* Learned tuning of the handle's knobs: features -> model -> delta and SCC
- * trim rounds, with the reference's defaults as fallback and (judge > 0) as
- * judge on an SSSP probe from min(k, judge) sources.  norm: 36 floats
+/* Learned tuning of the handle's knobs: features -> model -> delta and the
+ * SCC schedule (pivot rule and trim rounds: the model's second output indexes
+ * a table of 9 schedules, csrc/tune.hip), with the reference's defaults
+ * (delta 16, pivot rule 1, 3 trim rounds) as fallback and, when judge > 0, as
+ * judge: an SSSP probe from min(k, judge) sources with each delta and one
+ * buildSCC_DAG with each schedule, the faster kept.  norm: 36 floats
  * (feature mean[16], feature scale[16], target mean[2], target scale[2]).
- * out (may be NULL): chosen delta, chosen trim rounds, raw predicted delta,
- * 1 if the predicted delta was kept. */
+ * out (may be NULL): 6 doubles = chosen delta, chosen trim rounds, raw
+ * predicted delta, 1 if the predicted delta was kept, chosen pivot rule, 1 if
+ * the predicted schedule was kept. */
 int girhip_learned_tune(girhip_graph* h, girhip_mlp* model, const float* norm, const int32_t* src, int32_t k,
                         int32_t judge, double* out);
 

@@ -24,5 +24,13 @@ def test_dataset_labels_per_graph_and_k():
     X, Y = tuning.dataset({"feat": feat, "kind": kind, "value": value, "ms": ms, "gid": gid})
     assert X.shape == (2, 16)
     assert Y[:, 0].tolist() == [6.0, 4.0]  # log2 64 for k=1, log2 16 for k=8
-    # trims 1 and 3 within 5% -> label log2(1 + 1) = 1 (lower median of {1, 2})
+    # SCC schedules 1 and 3 within 5% -> label 1 (lower median of {1, 3}, no log)
     assert Y[:, 1].tolist() == [1.0, 1.0]
+    ms[7] = 4.5  # schedule 3 alone fastest
+    X, Y = tuning.dataset({"feat": feat, "kind": kind, "value": value, "ms": ms, "gid": gid})
+    assert Y[:, 1].tolist() == [3.0, 3.0]
+
+
+def test_schedule_table_holds_the_default():
+    assert tuning.SCHEDULES[tuning.DEFAULT_SCHEDULE] == (1, 3)  # pivot rule 1, 3 trim rounds
+    assert len(set(tuning.SCHEDULES)) == len(tuning.SCHEDULES) == 9
