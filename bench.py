@@ -199,6 +199,100 @@ def stream_runs(fb, fa, B=10000, NB=5, cpu=True):
     return out
 
 
+def held_out(dev, cpu=True):
+    """Learned vs default tuning on graphs the model never saw (RMAT scale 23,
+    the 4800 x 4800 lattice), 4 sources each: SSSP and SCC/DAG times with the
+    reference's defaults (delta 16, pivot rule 1, 3 trim rounds) and with the
+    raw prediction (and, on RMAT-23, the prediction judged by the default),
+    results checked identical; on the lattice also the WCC / SCC build and
+    BFS beside the compiled reference (one run, 1 BFS + 1 SSSP source)."""
+    import torch
+    import girhip
+    import tuning
+    model_path = os.path.join(PKG, "tuned_mlp.npz")
+    if not os.path.exists(model_path):
+        return None
+    model, norm = tuning.load_model(model_path)
+    res = {}
+    for name in ("rmat23", "grid4800"):
+        t0 = time.time()
+        if name == "rmat23":
+            V, fb, fa, bb, ba = girhip.rmat_graph(23, 16, 0.57, 0.19, 0.19, seed=5)
+            wmax = 64
+        else:
+            V, fb, fa, bb, ba = girhip.grid_graph(4800, 4800, 0.9, 0.8, seed=3)
+            wmax = 1000
+        w = girhip.weights(len(fa), 1, wmax, 12)
+        src = girhip.sources(fb, 4, 9)
+        log(f"held-out {name}: V={V} E={len(fa)} generated in {time.time() - t0:.1f}s")
+        G = girhip.Graph(*[torch.from_numpy(x).to(dev) for x in (fb, fa, bb, ba)])
+        G.set_weights(torch.from_numpy(w.view(np.int32)).to(dev))
+        dist = torch.empty((len(src), V), dtype=torch.int64, device=dev)
+
+        def defaults():
+            G.set_param("sssp_delta", 16)
+            G.set_param("scc_trim_rounds", 3)
+            G.set_param("scc_pivot", 1)
+
+        defaults()
+        G.wcc()
+        G.scc_dag()  # warm
+        G.wcc()
+        wcc_ms = G.last_ms("wcc")
+        s_def, _, _ = G.scc_dag()
+        scc_def = G.last_ms("scc_dag")
+        G.sssp(src, 0, out=dist)
+        sssp_def = G.last_ms("sssp")
+        ref = dist.clone()
+        r = G.learned_tune(model, norm, src, judge=0)  # the raw prediction
+        G.sssp(src, 0, out=dist)
+        G.sssp(src, 0, out=dist)
+        sssp_l = G.last_ms("sssp")
+        same = bool(torch.equal(dist, ref))
+        s_l, _, _ = G.scc_dag()
+        scc_l = G.last_ms("scc_dag")
+        same = same and bool(np.array_equal(s_l, s_def))
+        rec = {"V": V, "E": len(fa), "sources": len(src), "weights": f"uniform [1, {wmax}]",
+               "sssp_ms_default": round(sssp_def, 3), "learned_delta": r["delta"],
+               "sssp_ms_learned": round(sssp_l, 3), "sssp_gain": round(sssp_def / sssp_l, 3),
+               "learned_schedule": [r["pivot"], r["trim"]], "scc_dag_ms_default": round(scc_def, 3),
+               "scc_dag_ms_learned": round(scc_l, 3), "scc_dag_gain": round(scc_def / scc_l, 3),
+               "results_identical": same}
+        if name == "rmat23":
+            defaults()
+            rj = G.learned_tune(model, norm, src, judge=len(src))
+            G.sssp(src, 0, out=dist)
+            rec["judged_delta"] = rj["delta"]
+            rec["judged_kept_prediction"] = rj["used"]
+            rec["sssp_ms_judged"] = round(G.last_ms("sssp"), 3)
+            rec["results_identical"] = rec["results_identical"] and bool(torch.equal(dist, ref))
+        else:
+            # the lattice beside the compiled reference
+            defaults()
+            lv = torch.empty((len(src), V), dtype=torch.int32, device=dev)
+            G.bfs(src, out=lv)
+            rec["bfs_ms"] = round(G.last_ms("bfs"), 3)
+            rec["wcc_ms"] = round(wcc_ms, 3)
+            rec["build_ms"] = round(wcc_ms + scc_def, 3)
+            del lv
+            if cpu:
+                rr = cpu_baseline(fb, fa, bb, ba, src, src, n_bfs=1, n_sssp=1)
+                if rr is not None:
+                    b = rr[0].get("build", {})
+                    rec["cpu_reference"] = {"build_ms": round(b.get("wcc_ms", 0) + b.get("scc_dag_ms", 0), 1),
+                                            "bfs_ms_per_source": rr[0]["bfs"]["ms"],
+                                            "sssp_ms_per_source_hash_delta16": rr[0]["sssp"]["ms"],
+                                            "threads": rr[0].get("threads")}
+                    rec["speedup_build_vs_cpu"] = round(rec["cpu_reference"]["build_ms"] / rec["build_ms"], 2)
+                    rec["speedup_bfs_vs_cpu"] = round(rr[0]["bfs"]["ms"] / (rec["bfs_ms"] / len(src)), 2)
+        G.close()
+        del dist, ref
+        torch.cuda.empty_cache()
+        log(f"held-out {name}: {rec}")
+        res[name] = rec
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,6 +408,42 @@ def main():
         hash_ms.append(G.last_ms("sssp"))
     sssp_hash_ms_per_source = min(hash_ms) / N_SSSP
     gpu_out["sssp_hash"] = dist[:2].cpu().numpy().astype(np.uint64)
+    # the reference's own API: one source per call (GIR::bfs(int) /
+    # GIR::sssp(int), hash weights, delta 16), results into device memory;
+    # the copy of one result row to pageable host memory (what the C++
+    # class's return by value adds) is timed apart
+    single = None
+    if not args.no_extras:
+        lv1 = torch.empty(V, dtype=torch.int32, device=dev)
+        d1 = torch.empty(V, dtype=torch.int64, device=dev)
+        tb, ts, mb = [], [], []
+        for x in bfs_src[:8]:
+            _, m1 = G.bfs([int(x)], out=lv1, edges=True)
+            G.bfs([int(x)], out=lv1)
+            tb.append(G.last_ms("bfs"))
+            mb.append((int(m1[0]), int((lv1 >= 0).sum())))
+        for x in sssp_src[:4]:
+            G.sssp([int(x)], 16, out=d1)
+            G.sssp([int(x)], 16, out=d1)
+            ts.append(G.last_ms("sssp"))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            lv1.cpu()
+        t_lv = (time.perf_counter() - t0) / 3 * 1e3
+        t0 = time.perf_counter()
+        for _ in range(3):
+            d1.cpu()
+        t_d = (time.perf_counter() - t0) / 3 * 1e3
+        bfs1 = float(np.median(tb))
+        alg1 = float(np.mean([4 * V + 8 * n_ + 8 * m_ for m_, n_ in mb]))
+        single = {"bfs_sources": len(tb), "bfs_ms_per_call": round(bfs1, 3),
+                  "bfs_gbs_algorithmic": round(alg1 / (bfs1 * 1e-3) / 1e9, 1),
+                  "sssp_sources": len(ts), "sssp_ms_per_call": round(float(np.median(ts)), 3),
+                  "sssp_weights": "GIR::edgeWeight hash", "delta": 16,
+                  "d2h_levels_ms": round(t_lv, 3), "d2h_dist_ms": round(t_d, 3)}
+        del lv1, d1
+        log(f"single-source calls: {single}")
     G.set_weights(d_w)
 
     # algorithmic bytes (per unit: one build, one source) -> achieved GB/s
@@ -377,7 +507,8 @@ def main():
         s_ms = float(np.median([t[0] for t in t_l]))
         c_ms = float(np.median([t[1] for t in t_l]))
         same = bool(torch.equal(dist, ref_dist) and torch.equal(scc, ref_scc))
-        learned = {"delta": res["delta"], "trim_rounds": res["trim"], "raw_delta": round(res["raw_delta"], 2),
+        learned = {"delta": res["delta"], "trim_rounds": res["trim"], "pivot_rule": res["pivot"],
+                   "schedule_kept": res["schedule_used"], "raw_delta": round(res["raw_delta"], 2),
                    "prediction_kept": res["used"], "judge_sources": N_SSSP,
                    # with the prediction rejected the SSSP runs the default knobs, so
                    # its gain is 1 up to timing noise
@@ -388,6 +519,7 @@ def main():
                    "scc_dag_gain": round(med["scc_dag"] / c_ms, 3), "results_identical": same}
         G.set_param("sssp_delta", args.delta)
         G.set_param("scc_trim_rounds", 3)
+        G.set_param("scc_pivot", 1)
         del ref_dist, ref_scc
         log(f"learned: {learned}")
 
@@ -690,6 +822,14 @@ def main():
     if cpu and two and "bfs2_ms_per_source" in cpu:
         two["speedup_bfs_vs_cpu"] = round(cpu["bfs2_ms_per_source"] / two["bfs_ms_per_source"], 2)
         two["speedup_sssp_vs_cpu"] = round(cpu["sssp2_ms_per_source"] / two["sssp_ms_per_source"], 2)
+    if single and cpu:
+        single["speedup_bfs_vs_cpu"] = round(cpu["bfs_ms_per_source"] / single["bfs_ms_per_call"], 2)
+        single["speedup_sssp_vs_cpu"] = round(cpu["sssp_ms_per_source"] / single["sssp_ms_per_call"], 2)
+        single["speedup_bfs_vs_cpu_with_d2h"] = round(
+            cpu["bfs_ms_per_source"] / (single["bfs_ms_per_call"] + single["d2h_levels_ms"]), 2)
+        single["speedup_sssp_vs_cpu_with_d2h"] = round(
+            cpu["sssp_ms_per_source"] / (single["sssp_ms_per_call"] + single["d2h_dist_ms"]), 2)
+    out["single_source"] = single
     out["two_level"] = two
     if stream and "reference_fmu" in stream:
         for m in ("fmu", "lazy", "inc"):
@@ -710,6 +850,7 @@ def main():
     if kernels:
         out["kernels"] = kernels
     G.close()
+    out["held_out"] = None if args.no_extras else held_out(dev, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 
