@@ -98,6 +98,7 @@ struct Params {
     double sssp_ms_batch = 8;    // sources per interleaved batch (1..32)
     double scc_trim2 = 0;        // 1: each SCC trim round also runs trim-2 (pairs)
     double scc_incremental = 1;  // 1: update batches repair the SCC layer; 0: rebuild it
+    double sssp_unroll = 2;      // arcs per lane at a time in the SSSP relaxation passes (1, 2 or 4)
 };
 
 struct KStat {
@@ -656,39 +657,44 @@ __device__ __forceinline__ int64_t waveFlat(int32_t v, uint64_t pay, int32_t aux
     return acc;
 }
 
+// Rows of the lanes with heavy set go to the heavy list (row, aux, range,
+// payload and the arc offset of each row): list slots and arc offsets are
+// reserved together, (rows << 40) + arcs in one atomic, so offsets grow
+// with slots and the heavy pass needs no prefix launch.  Whole wave.
+__device__ __forceinline__ void listHeavyRows(bool heavy, int32_t v, uint64_t pay, int32_t aux, int64_t b,
+                                              int64_t end, const HeavyList& H) {
+    const uint64_t hm = __ballot(heavy);
+    if (!hm) return;
+    const int lane = laneId();
+    const int64_t len = heavy ? end - b : 0;
+    int64_t incl = len;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int64_t o = __shfl_up(incl, d);
+        if (lane >= d) incl += o;
+    }
+    const int64_t tot = __shfl(incl, WAVE - 1);
+    const int leader = __ffsll((long long)hm) - 1;
+    unsigned long long old = 0;
+    if (lane == leader)
+        old = atomicAdd((unsigned long long*)&H.ctl[0],
+                        ((unsigned long long)__popcll(hm) << HEAVY_SHIFT) + (unsigned long long)tot);
+    old = __shfl(old, leader);
+    if (heavy) {
+        const int64_t pos = (int64_t)(old >> HEAVY_SHIFT) + __popcll(hm & lanesBelow());
+        H.row[pos] = v;
+        H.aux[pos] = aux;
+        H.b[pos] = b;
+        H.e[pos] = end;
+        H.pay[pos] = pay;
+        H.off[pos] = (int64_t)(old & HEAVY_MASK) + incl - len;
+    }
+}
+
 template <int SMALL = 16, class F>
 __device__ __forceinline__ int64_t waveRowsH(bool act, int32_t v, uint64_t pay, int32_t aux, int64_t b, int64_t end,
                                              const HeavyList& H, const F& f) {
     const bool heavy = act && (end - b) > HEAVY_T;
-    const uint64_t hm = __ballot(heavy);
-    if (hm) {
-        // list slots and arc offsets are reserved together, (rows << 40) +
-        // arcs in one atomic, so offsets grow with slots and the heavy pass
-        // needs no prefix launch
-        const int lane = laneId();
-        const int64_t len = heavy ? end - b : 0;
-        int64_t incl = len;
-        for (int d = 1; d < WAVE; d <<= 1) {
-            const int64_t o = __shfl_up(incl, d);
-            if (lane >= d) incl += o;
-        }
-        const int64_t tot = __shfl(incl, WAVE - 1);
-        const int leader = __ffsll((long long)hm) - 1;
-        unsigned long long old = 0;
-        if (lane == leader)
-            old = atomicAdd((unsigned long long*)&H.ctl[0], ((unsigned long long)__popcll(hm) << HEAVY_SHIFT) +
-                                                                 (unsigned long long)tot);
-        old = __shfl(old, leader);
-        if (heavy) {
-            const int64_t pos = (int64_t)(old >> HEAVY_SHIFT) + __popcll(hm & lanesBelow());
-            H.row[pos] = v;
-            H.aux[pos] = aux;
-            H.b[pos] = b;
-            H.e[pos] = end;
-            H.pay[pos] = pay;
-            H.off[pos] = (int64_t)(old & HEAVY_MASK) + incl - len;
-        }
-    }
+    listHeavyRows(heavy, v, pay, aux, b, end, H);
     const bool mine = act && !heavy;
     const int first = mine ? (int)(end - b < SMALL ? end - b : SMALL) : 0;
     int64_t acc = waveFlat(v, pay, aux, b, first, f);
@@ -703,6 +709,77 @@ __device__ __forceinline__ int64_t waveRowsH(bool act, int32_t v, uint64_t pay, 
         for (int64_t j = hb + laneId(); j < he; j += WAVE) acc += f(hv, hp, ha, j);
     }
     return acc;
+}
+
+// waveRowsH for functors with a batched form that hides memory latency: U
+// arcs per lane at a time, f.many<U>(p, pay, aux, j, ok) with one entry per
+// arc (ok = entry holds an arc).  The functor issues the arcs' loads for all
+// U entries before it uses any of them, so a lane keeps U dependent chains
+// (arc -> head's state -> update) in flight instead of one.  No return value.
+template <int U, class F>
+__device__ __forceinline__ void waveFlatU(int32_t v, uint64_t pay, int32_t aux, int64_t b, int cnt, const F& f) {
+    const int lane = laneId();
+    int incl = cnt;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int o = __shfl_up(incl, d);
+        if (lane >= d) incl += o;
+    }
+    const int total = __shfl(incl, WAVE - 1);
+    for (int t0 = 0; t0 < total; t0 += U * WAVE) {
+        int32_t ov[U], oa[U];
+        uint64_t op[U];
+        int64_t oj[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * WAVE + lane;
+            int below = 0;
+            for (int step = WAVE / 2; step >= 1; step >>= 1)
+                if (__shfl(incl, below + step - 1) <= t) below += step;
+            const int own = below < WAVE ? below : WAVE - 1;
+            const int excl = __shfl(incl, own) - __shfl(cnt, own);
+            ov[u] = __shfl(v, own);
+            oa[u] = __shfl(aux, own);
+            op[u] = __shfl(pay, own);
+            oj[u] = __shfl(b, own) + (t - excl);
+            ok[u] = t < total;
+        }
+        f.template many<U>(ov, op, oa, oj, ok);
+    }
+}
+
+template <int U, int SMALL = 16, class F>
+__device__ __forceinline__ void waveRowsHU(bool act, int32_t v, uint64_t pay, int32_t aux, int64_t b, int64_t end,
+                                           const HeavyList& H, const F& f) {
+    const bool heavy = act && (end - b) > HEAVY_T;
+    listHeavyRows(heavy, v, pay, aux, b, end, H);
+    const bool mine = act && !heavy;
+    const int first = mine ? (int)(end - b < SMALL ? end - b : SMALL) : 0;
+    waveFlatU<U>(v, pay, aux, b, first, f);
+    const int64_t i = b + first;
+    uint64_t rest = __ballot(mine && i < end);
+    while (rest) {
+        const int l = __ffsll((long long)rest) - 1;
+        rest &= rest - 1;
+        int32_t hv[U], ha[U];
+        uint64_t hp[U];
+        int64_t hj[U];
+        bool ok[U];
+        const int32_t v0 = __shfl(v, l), a0 = __shfl(aux, l);
+        const uint64_t p0 = __shfl(pay, l);
+        const int64_t hb = __shfl(i, l), he = __shfl(end, l);
+        for (int64_t j0 = hb; j0 < he; j0 += U * WAVE) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                hv[u] = v0;
+                ha[u] = a0;
+                hp[u] = p0;
+                hj[u] = j0 + u * WAVE + laneId();
+                ok[u] = hj[u] < he;
+            }
+            f.template many<U>(hv, hp, ha, hj, ok);
+        }
+    }
 }
 
 __device__ __forceinline__ int64_t waveSum(int64_t x) {
@@ -838,6 +915,45 @@ __device__ __forceinline__ bool setBit(uint64_t* bits, uint64_t* sum, int32_t x)
     uint64_t* sp = &sum[x >> 12];
     if (!(*sp & sb)) atomicOr((unsigned long long*)sp, (unsigned long long)sb);
     return true;
+}
+
+// Sets bit x of a bitmap with a summary, as setBit, without awaiting any
+// atomic: the word's atomicOr is unconditional, the summary's is behind a
+// plain pre-check of the summary word sw that the caller loaded earlier
+// (sum[x >> 12], read before its other updates so the load is in flight
+// with them).  Returns true when sw showed the summary bit clear; within a
+// kernel bits are only set, so the first setter of a summary bit always
+// sees it clear and "some call returned true" <=> "the bitmap went from
+// empty to non-empty" for a bitmap that was empty at launch.
+__device__ __forceinline__ bool markBit(uint64_t* bits, uint64_t* sum, int32_t x, uint64_t sw) {
+    atomicOr((unsigned long long*)&bits[x >> 6], 1ULL << (x & 63));
+    const uint64_t sb = 1ULL << ((x >> 6) & 63);
+    if (sw & sb) return false;
+    atomicOr((unsigned long long*)&sum[x >> 12], (unsigned long long)sb);
+    return true;
+}
+
+// markBit for one vertex per lane (lanes with act), for vertex lists in
+// which equal bitmap words sit in consecutive lanes (a queue compacted in
+// vertex order): the lanes of a run with the same word OR their bits
+// together (segmented scan over the wave) and the run's first lane issues
+// the run's one atomicOr and the summary update.  Whole wave.
+__device__ __forceinline__ void waveMarkBits(uint64_t* bits, uint64_t* sum, int32_t x, bool act) {
+    const int lane = laneId();
+    const long long word = act ? (long long)(x >> 6) : -1LL;
+    uint64_t acc = act ? 1ULL << (x & 63) : 0ULL;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint64_t o = __shfl_down(acc, d);
+        const long long ow = __shfl_down(word, d);
+        if (lane + d < WAVE && ow == word) acc |= o;
+    }
+    const long long pw = __shfl_up(word, 1);
+    if (act && (lane == 0 || pw != word)) {
+        atomicOr((unsigned long long*)&bits[word], (unsigned long long)acc);
+        const uint64_t sb = 1ULL << (word & 63);
+        uint64_t* sp = &sum[word >> 6];
+        if (!(*sp & sb)) atomicOr((unsigned long long*)sp, (unsigned long long)sb);
+    }
 }
 
 // Wave-level walk over a summary bitmap: the waves of the grid take tasks of

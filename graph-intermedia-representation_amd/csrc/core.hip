@@ -597,7 +597,7 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"xcd", &Params::xcd}, {"split", &Params::split},
                {"scc_block", &Params::scc_block}, {"sssp_interleave", &Params::sssp_interleave},
                {"sssp_ms_batch", &Params::sssp_ms_batch}, {"scc_trim2", &Params::scc_trim2},
-               {"scc_incremental", &Params::scc_incremental}};
+               {"scc_incremental", &Params::scc_incremental}, {"sssp_unroll", &Params::sssp_unroll}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;
@@ -617,6 +617,8 @@ extern "C" int girhip_set_param(girhip_graph* h, const char* name, double value)
             raise(GIRHIP_EINVAL, "set_param: sssp_ms_batch must be in [1, 32]");
         if (std::strcmp(name, "sssp_interleave") == 0 && !(value == 0 || value == 1 || value == 2))
             raise(GIRHIP_EINVAL, "set_param: sssp_interleave must be 0, 1 or 2");
+        if (std::strcmp(name, "sssp_unroll") == 0 && !(value == 1 || value == 2 || value == 4))
+            raise(GIRHIP_EINVAL, "set_param: sssp_unroll must be 1, 2 or 4");
         if (!(value == value)) raise(GIRHIP_EINVAL, "set_param: NaN");
         *s = value;
     });

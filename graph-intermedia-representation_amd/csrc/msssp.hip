@@ -171,6 +171,75 @@ struct MsRelax {
         mark(x, near, far);
         return 0;
     }
+    // U arcs at a time (segments of at most 32 bytes): the arcs' slots, then
+    // both ends' distance segments and the heads' summary words, are loaded
+    // for all U before any is used; the updates (atomicMin per improved
+    // source, the masks' and bitmaps' atomicOr) are then issued without
+    // awaiting a result or pre-reading the masks.
+    template <int U>
+    __device__ void many(const int32_t (&p)[U], const uint64_t (&pay)[U], const int32_t (&)[U],
+                         const int64_t (&j)[U], const bool (&ok)[U]) const {
+        static_assert(kVecSeg<T, KS>, "batched relaxation needs segments of at most 32 bytes");
+        int32_t x[U];
+        uint64_t wt[U];
+        bool go[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            go[u] = ok[u];
+            x[u] = go[u] ? fa[j[u]] : 0;
+            wt[u] = (go[u] && w) ? (uint64_t)w[j[u]] : 0;
+        }
+        const uint64_t hi = (uint64_t)*S.c(M_HI);
+        T* dist = (T*)S.dist;
+        T du[U][KS], dx[U][KS];
+        uint64_t nsw[U], fsw[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (go[u] && !w) wt[u] = hashWeight(p[u], x[u]);
+            if (go[u] && !split && ((wt[u] > delta) != (heavy != 0))) go[u] = false;
+            nsw[u] = fsw[u] = 0;
+            if (go[u]) {
+                loadSeg<T, KS>(dist + (size_t)p[u] * KS, du[u]);
+                loadSeg<T, KS>(dist + (size_t)x[u] * KS, dx[u]);
+                nsw[u] = S.nsum[x[u] >> 12];
+                fsw[u] = S.fsum[x[u] >> 12];
+            }
+        }
+        bool ovf = false, nset = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!go[u]) continue;
+            const uint32_t m = (uint32_t)pay[u];
+            T* px = dist + (size_t)x[u] * KS;
+            uint32_t near = 0, far = 0;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                if (!((m >> s) & 1)) continue;
+                const uint64_t nd = widen(du[u][s]) + wt[u];
+                if constexpr (sizeof(T) == 4) {
+                    if (nd >= 0xffffffffull) {
+                        ovf = true;
+                        continue;
+                    }
+                }
+                if (nd < widen(dx[u][s])) {
+                    atomicMinT(px + s, (T)nd);
+                    if (nd < hi) near |= 1u << s;
+                    else far |= 1u << s;
+                }
+            }
+            if (near) {
+                atomicOr(&S.nmask[x[u]], near);
+                nset |= markBit(S.nbits, S.nsum, x[u], nsw[u]);
+            }
+            if (far) {
+                atomicOr(&S.fmask[x[u]], far);
+                markBit(S.fbits, S.fsum, x[u], fsw[u]);
+            }
+        }
+        if (ovf) *S.c(M_OVF) = 1;
+        if (nset) *S.c(par ? M_NSET1 : M_NSET0) = 1;
+    }
 };
 
 __device__ __forceinline__ void resetRound(const MsState& S, int par) {
@@ -208,7 +277,7 @@ __global__ void kMsInit(MsState S, const int32_t* __restrict__ src, int k, uint6
 // queue (heavy arcs); grid stride, one lane per queue entry.  track_done:
 // the entries' sources join the vertex's done mask (its heavy arcs are due
 // when the bucket drains).
-template <class T, int KS>
+template <class T, int KS, int U>
 __global__ void kMsRelax(MsRelax<T, KS> f, const int32_t* __restrict__ fb, const int32_t* __restrict__ mid,
                          HeavyList H, int track_done, int heavy_phase) {
     const MsState& S = f.S;
@@ -222,9 +291,9 @@ __global__ void kMsRelax(MsRelax<T, KS> f, const int32_t* __restrict__ fb, const
         const bool act = i < n;
         const int32_t u = act ? q[i] : 0;
         const uint32_t m = act ? qm[i] : 0;
-        if (act && track_done) {
-            if ((S.dmask[u] & m) != m) atomicOr(&S.dmask[u], m);
-            setBit(S.dbits, S.dsum, u);
+        if (track_done) {
+            if (act) atomicOr(&S.dmask[u], m);
+            waveMarkBits(S.dbits, S.dsum, u, act);
         }
         int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
         if (mid && act) {
@@ -232,7 +301,8 @@ __global__ void kMsRelax(MsRelax<T, KS> f, const int32_t* __restrict__ fb, const
             if (heavy_phase) b = c;
             else e = c;
         }
-        waveRowsH(act, u, (uint64_t)m, 0, b, e, H, f);
+        if constexpr (U > 1) waveRowsHU<U>(act, u, (uint64_t)m, 0, b, e, H, f);
+        else waveRowsH(act, u, (uint64_t)m, 0, b, e, H, f);
     }
 }
 
@@ -573,6 +643,18 @@ bool msRun(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, const gi
     const int sp = sv ? 1 : 0;
     const MsRelax<T, KS> fl[2] = {{S, fa, w, delta, 0, 0, sp}, {S, fa, w, delta, 0, 1, sp}};
     const MsRelax<T, KS> fh[2] = {{S, fa, w, delta, 1, 0, sp}, {S, fa, w, delta, 1, 1, sp}};
+    // arcs per lane at a time: the knob, capped so that the U segment pairs
+    // held in registers stay within 64 bytes each (1 where segments are
+    // longer than 32 bytes: they are read a distance at a time)
+    constexpr int SEG = KS * (int)sizeof(T);
+    const int ur = !kVecSeg<T, KS> ? 1 : std::min<int>((int)h->prm.sssp_unroll, std::max(1, 64 / SEG));
+    auto relaxK = kMsRelax<T, KS, 1>;
+    if constexpr (kVecSeg<T, KS> && SEG <= 32) {
+        if (ur == 2) relaxK = kMsRelax<T, KS, 2>;
+    }
+    if constexpr (kVecSeg<T, KS> && SEG <= 16) {
+        if (ur == 4) relaxK = kMsRelax<T, KS, 4>;
+    }
     const unsigned hgrid = std::min<unsigned>(gridFor(V, blk), 4096);
     // bitmap passes: one wave per task of TW words, at most 2048 blocks
     const unsigned sgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (W2 * TPS * WAVE + blk - 1) / blk));
@@ -586,11 +668,11 @@ bool msRun(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, const gi
                                      : gridFor(maxn, blk);
         int p = par;
         for (int t = 0; t < R; ++t) {
-            launch(h, "sssp_ms_relax", kMsRelax<T, KS>, rgrid, blk, 0, fl[p], fb, mid, H, heavy, 0);
+            launch(h, "sssp_ms_relax", relaxK, rgrid, blk, 0, fl[p], fb, mid, H, heavy, 0);
             launchHeavyStore(h, "sssp_ms_relax_rows", fl[p]);
             launch(h, "sssp_ms_compact", kMsCompact<T, KS>, sgrid, blk, 0, S, p, heavy);
             if (heavy) {
-                launch(h, "sssp_ms_relax_heavy", kMsRelax<T, KS>, hgrid, blk, 0, fh[p], fb, mid, H, 0, 1);
+                launch(h, "sssp_ms_relax_heavy", relaxK, hgrid, blk, 0, fh[p], fb, mid, H, 0, 1);
                 launchHeavyStore(h, "sssp_ms_relax_heavy_rows", fh[p]);
                 launch(h, "sssp_ms_farmin", kMsFarMin<T, KS>, sgrid, blk, 0, S, p);
             }

@@ -118,6 +118,13 @@ __device__ __forceinline__ uint64_t dMin(const SsspBatch& B, int s, int32_t x, u
     return atomicMin((unsigned long long*)B.dist + i, (unsigned long long)nd);
 }
 
+// atomicMin whose previous value is not awaited
+__device__ __forceinline__ void dMinNoRet(const SsspBatch& B, int s, int32_t x, uint64_t nd) {
+    const size_t i = (size_t)s * B.V + x;
+    if (B.d32) atomicMin((unsigned int*)B.dist + i, (unsigned int)nd);
+    else atomicMin((unsigned long long*)B.dist + i, (unsigned long long)nd);
+}
+
 __device__ __forceinline__ int64_t* nearCount(const SsspBatch& B, int par, int s) {
     return cp(B, par ? C_NEAR1 : C_NEAR0, s);
 }
@@ -202,6 +209,57 @@ struct RelaxF {
             }
         }
         return 0;
+    }
+    // U arcs of rows p (owner distances dp, all of one source s) at a time:
+    // the arcs' slots, then every head's distance and summary words, are
+    // loaded for all U before any is used; the updates (atomicMin, bitmap
+    // and summary atomicOr) are then issued without awaiting a result.
+    template <int U>
+    __device__ void many(const int32_t (&p)[U], const uint64_t (&dp)[U], const int32_t (&sv)[U],
+                         const int64_t (&j)[U], const bool (&ok)[U]) const {
+        const int s = sv[0];  // one source per wave
+        int32_t x[U];
+        uint64_t wt[U];
+        bool go[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            go[u] = ok[u];
+            x[u] = go[u] ? fa[j[u]] : 0;
+            wt[u] = (go[u] && !unit && w) ? (uint64_t)w[j[u]] : 1;
+        }
+        uint64_t dx[U], nsw[U], fsw[U];
+        const uint64_t hi = (uint64_t)*cp(B, C_HI, s);
+        uint64_t* nb = B.nearbits + (size_t)s * B.W;
+        uint64_t* ns = B.nsum + (size_t)s * B.W2;
+        uint64_t* fbits = B.farbits + (size_t)s * B.W;
+        uint64_t* fs = B.fsum + (size_t)s * B.W2;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (go[u] && !unit && !w) wt[u] = hashWeight(p[u], x[u]);
+            if (go[u] && unit && heavy) go[u] = false;
+            if (go[u] && !unit && !mid && ((wt[u] > delta) != (heavy != 0))) go[u] = false;
+            if (go[u] && scope && scope[x[u]] != scope[p[u]]) go[u] = false;
+            dx[u] = go[u] ? dGet(B, s, x[u]) : 0;
+            nsw[u] = go[u] ? ns[x[u] >> 12] : 0;
+            fsw[u] = go[u] ? fs[x[u] >> 12] : 0;
+        }
+        bool ovf = false, nset = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!go[u]) continue;
+            const uint64_t nd = dp[u] + wt[u];
+            if (B.d32 && nd >= 0xffffffffull) {  // beyond 32 bits: the batch is redone with 64
+                ovf = true;
+                continue;
+            }
+            if (nd < dx[u]) {  // (dx: a plain read, safe as a pre-check as distances only fall)
+                dMinNoRet(B, s, x[u], nd);
+                if (nd < hi) nset |= markBit(nb, ns, x[u], nsw[u]);
+                else markBit(fbits, fs, x[u], fsw[u]);
+            }
+        }
+        if (ovf) B.ctl[C_OVF] = 1;
+        if (nset) *nearSet(B, par, s) = 1;
     }
 };
 
@@ -315,6 +373,8 @@ __device__ __forceinline__ void resetRound(const SsspBatch& B, int par, int s) {
 // Relax the n queued vertices q of source s (grid stride, uniform per wave).
 // track_done: mark them processed (their heavy arcs are due when the bucket
 // drains).
+// U: arcs per lane at a time (waveRowsHU) where above 1.
+template <int U>
 __device__ __forceinline__ void relaxQueue(const RelaxF& f, const int32_t* __restrict__ fb, HeavyList H,
                                            const int32_t* q, int64_t n, int s, bool track_done, int64_t b0,
                                            int64_t nb) {
@@ -325,14 +385,15 @@ __device__ __forceinline__ void relaxQueue(const RelaxF& f, const int32_t* __res
         const bool act = i < n;
         const int32_t u = act ? q[i] : 0;
         const uint64_t du = act ? dGet(B, s, u) : 0;
-        if (act && track_done) setBit(B.donebits + (size_t)s * B.W, B.dsum + (size_t)s * B.W2, u);
+        if (track_done) waveMarkBits(B.donebits + (size_t)s * B.W, B.dsum + (size_t)s * B.W2, u, act);
         int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
         if (f.mid && act) {
             const int32_t m = f.mid[u];
             if (f.heavy) b = m;
             else e = m;
         }
-        waveRowsH(act, u, du, s, b, e, H, f);
+        if constexpr (U > 1) waveRowsHU<U>(act, u, du, s, b, e, H, f);
+        else waveRowsH(act, u, du, s, b, e, H, f);
     }
 }
 
@@ -478,11 +539,12 @@ __device__ __forceinline__ void splitPhase(const SsspBatch& B, int par, uint64_t
 
 // ---- one kernel per phase (grid row y = source) ----------------------------
 
+template <int U>
 __global__ void kBRelax(RelaxF f, const int32_t* __restrict__ fb, HeavyList H, int track_done) {
     const SsspBatch& B = f.B;
     const Row r = gridRow(B);
     if (r.b == 0 && threadIdx.x == 0) resetRound(B, f.par, r.s);
-    relaxQueue(f, fb, H, B.qn[f.par] + (size_t)r.s * B.V, *nearCount(B, f.par, r.s), r.s, track_done != 0, r.b,
+    relaxQueue<U>(f, fb, H, B.qn[f.par] + (size_t)r.s * B.V, *nearCount(B, f.par, r.s), r.s, track_done != 0, r.b,
                r.nb);
 }
 
@@ -492,10 +554,11 @@ __global__ void kBCompact(SsspBatch B, int par, int heavy, int G) {
     compactPhase(B, par, heavy, G, r.s, sh, r.b, r.nb);
 }
 
+template <int U>
 __global__ void kBRelaxHeavy(RelaxF f, const int32_t* __restrict__ fb, HeavyList H) {
     const SsspBatch& B = f.B;
     const Row r = gridRow(B);
-    relaxQueue(f, fb, H, B.qh + (size_t)r.s * B.V, *cp(B, C_HQ, r.s), r.s, false, r.b, r.nb);
+    relaxQueue<U>(f, fb, H, B.qh + (size_t)r.s * B.V, *cp(B, C_HQ, r.s), r.s, false, r.b, r.nb);
 }
 
 __global__ void kBFarMin(SsspBatch B, int par, int G) {
@@ -531,13 +594,13 @@ __global__ void __launch_bounds__(256) kSsspPersist(RelaxF fl, RelaxF fh, const 
         if (blockIdx.x == 0 && threadIdx.x < k) resetRound(B, par, threadIdx.x);
         fl.par = par;
         fh.par = par;
-        for (int s = 0; s < k; ++s) relaxQueue(fl, fb, H, B.qn[par] + (size_t)s * B.V, ldAcq(nearCount(B, par, s)),
+        for (int s = 0; s < k; ++s) relaxQueue<1>(fl, fb, H, B.qn[par] + (size_t)s * B.V, ldAcq(nearCount(B, par, s)),
                                                s, heavy != 0, blockIdx.x, gridDim.x);
         gridSync(bar);
         for (int s = 0; s < k; ++s) compactPhase(B, par, heavy, G, s, sh, blockIdx.x, gridDim.x);
         gridSync(bar);
         if (heavy) {
-            for (int s = 0; s < k; ++s) relaxQueue(fh, fb, H, B.qh + (size_t)s * B.V, ldAcq(cp(B, C_HQ, s)), s,
+            for (int s = 0; s < k; ++s) relaxQueue<1>(fh, fb, H, B.qh + (size_t)s * B.V, ldAcq(cp(B, C_HQ, s)), s,
                                                    false, blockIdx.x, gridDim.x);
             gridSync(bar);
             for (int s = 0; s < k; ++s) farMinPhase(B, par, G, s, sh, blockIdx.x, gridDim.x);
@@ -664,8 +727,12 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
     // heavy arcs exist only if some weight exceeds delta
     const int heavy = !unit && (uint64_t)maxWeight(h) > delta ? 1 : 0;
     const int u = unit ? 1 : 0;
+    const int ur = (int)h->prm.sssp_unroll;
     const RelaxF fl[2] = {{B, fa, w, delta, 0, 0, scope, u, mid}, {B, fa, w, delta, 0, 1, scope, u, mid}};
     const RelaxF fh[2] = {{B, fa, w, delta, 1, 0, scope, u, mid}, {B, fa, w, delta, 1, 1, scope, u, mid}};
+    // relaxation kernels by arcs per lane at a time
+    auto relaxK = ur == 4 ? kBRelax<4> : ur == 2 ? kBRelax<2> : kBRelax<1>;
+    auto relaxHK = ur == 4 ? kBRelaxHeavy<4> : ur == 2 ? kBRelaxHeavy<2> : kBRelaxHeavy<1>;
     const unsigned hgrid = std::min<unsigned>(gridFor(V, blk), 4096);
     const int32_t* fb = h->fw_beg.p;
     int par = 0;
@@ -715,11 +782,11 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
         }
         auto issue = [&](int p) {
             for (int t = 0; t < R; ++t) {
-                launch(h, "sssp_relax", kBRelax, rgrid * k, blk, 0, fl[p], fb, heavyList(h), heavy);
+                launch(h, "sssp_relax", relaxK, rgrid * k, blk, 0, fl[p], fb, heavyList(h), heavy);
                 launchHeavyStore(h, "sssp_relax_rows", fl[p]);
                 launch(h, "sssp_compact", kBCompact, sgrid * k, blk, 0, B, p, heavy, G);
                 if (heavy) {
-                    launch(h, "sssp_relax_heavy", kBRelaxHeavy, hgrid * k, blk, 0, fh[p], fb, heavyList(h));
+                    launch(h, "sssp_relax_heavy", relaxHK, hgrid * k, blk, 0, fh[p], fb, heavyList(h));
                     launchHeavyStore(h, "sssp_relax_heavy_rows", fh[p]);
                     launch(h, "sssp_farmin", kBFarMin, sgrid * k, blk, 0, B, p, G);
                 }
@@ -1106,14 +1173,19 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         const SplitView* sv = (k > 0 && (uint64_t)maxWeight(h) > delta) ? splitView(h, delta) : nullptr;
         // Engine: the interleaved one shares arcs between sources whose
         // distances fall in the same bucket, which pays once buckets are
-        // wide; with narrow buckets and a heavy phase per bucket the
-        // per-source layout measured faster (RMAT-22, 8 sources, weights
-        // [1, 64]: delta 16 26 vs 28 ms, delta 32 27 vs 19 ms, delta 128 30 vs
-        // 14 ms; one source: interleaved at every delta).  Auto (2): the
-        // interleaved engine for one source or when 2 delta >= the largest
-        // weight.
+        // wide; with narrow buckets, many sources and a heavy phase per
+        // bucket the per-source layout measured faster.  RMAT-22, ms,
+        // per-source vs interleaved: weights [1, 64], 8 sources: delta 16
+        // 24.9 vs 19.3, delta 32 27.4 vs 15.1; 16 sources: delta 16 41.4 vs
+        // 49.7, delta 32 47.9 vs 34.8; hash weights [1, 100], 8 sources:
+        // delta 16 23.2 vs 30.3, delta 32 27.4 vs 26.9; one source: the
+        // interleaved engine at every delta.  Auto (2): interleaved for one
+        // source, for up to 8 sources when 4 delta >= the largest weight, and
+        // for more when 2 delta >= it.
         const double mode = h->prm.sssp_interleave;
-        const bool inter = mode == 1 || (mode == 2 && (k == 1 || 2 * delta >= (uint64_t)maxWeight(h)));
+        const uint64_t mw = (uint64_t)maxWeight(h);
+        const bool inter =
+            mode == 1 || (mode == 2 && (k == 1 || (k <= 8 ? 4 * delta >= mw : 2 * delta >= mw)));
         if (inter) {
             // batches of up to sssp_ms_batch sources sharing buckets, distances
             // interleaved; the batches are made even (10 -> 5 + 5, not 8 + 2)

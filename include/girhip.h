@@ -281,6 +281,18 @@ int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
  *   given delta, kept for the two deltas used last, E x 8 bytes each,
  *   rebuilt after the store or the weights change; 0: rows classify their
  *   arcs on the fly),
+ *   "sssp_interleave" (2: automatic - the shared-bucket engine with
+ *   interleaved distances for one source, for up to 8 sources when
+ *   4 delta >= the largest weight, for more when 2 delta >= it; 1: always;
+ *   0: never, one distance layout per source),
+ *   "sssp_ms_batch" (8: sources per interleaved batch, 1..32),
+ *   "sssp_unroll" (2: arcs per lane that a relaxation pass loads before it
+ *   updates any, 1, 2 or 4),
+ *   "scc_trim2" (0; 1: every SCC trim round also peels pairs whose only
+ *   active neighbours are each other),
+ *   "scc_incremental" (1: update batches repair the SCC/DAG layer by
+ *   classifying the changes, falling back to a rebuild for large batches or
+ *   large affected SCCs; 0: always rebuild),
  *   "profile" (0: 1 times every kernel)
  * Values only change work order; results never depend on them. */
 int girhip_set_param(girhip_graph* h, const char* name, double value);

@@ -272,7 +272,8 @@ def test_tuning_knobs_do_not_change_results():
                      ("wcc_rounds", 0), ("scc_trim_rounds", 0), ("scc_pivot", 0), ("sssp_delta", 1),
                      ("persist", 1), ("chunk", 1), ("sssp_dist32", 0), ("xcd", 1), ("split", 0), ("scc_block", 256), ("scc_block", 1024),
                      ("sssp_delta", 100000), ("sssp_interleave", 0), ("sssp_interleave", 1),
-                     ("sssp_ms_batch", 3), ("sssp_ms_batch", 32), ("scc_trim2", 1)]:
+                     ("sssp_ms_batch", 3), ("sssp_ms_batch", 32), ("scc_trim2", 1), ("sssp_unroll", 1),
+                     ("sssp_unroll", 2)]:
             say("knob", k, v)
             old = G.get_param(k)
             G.set_param(k, v)
@@ -300,9 +301,10 @@ def test_sssp_interleaved_batch_sizes():
             if k >= 3:
                 src[1] = src[0]  # the same vertex twice in one batch
                 src[2] = -1 if k % 2 else V  # out of range: all unreachable
-            for split in (1, 0):
-                say("interleaved sssp k", k, "split", split)
+            for split, unroll in ((1, 4), (0, 2), (1, 1)):
+                say("interleaved sssp k", k, "split", split, "unroll", unroll)
                 G.set_param("split", split)
+                G.set_param("sssp_unroll", unroll)
                 d = G.sssp(src, 7)
                 for i, x in enumerate(src):
                     ref = want[int(x)] if 0 <= x < V else np.full(V, INF, np.uint64)
