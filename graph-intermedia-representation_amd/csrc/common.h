@@ -143,7 +143,8 @@ struct Scratch {
     DevBuf<int32_t> ms_q[3];
     DevBuf<uint32_t> ms_qm[3];
     DevBuf<int64_t> ms_ctl;
-    DevBuf<int32_t> src;  // query sources
+    DevBuf<uint64_t> gbits;  // condensation keys: one bit per vertex, set iff its label is the big one
+    DevBuf<int32_t> src;     // query sources
     // mixed query / update lists: source degrees, output rows, trivial
     // sources, live-source rows, host-output staging
     DevBuf<int32_t> rb_deg, rb_rows, rb_tsrc;
@@ -700,6 +701,27 @@ __device__ __forceinline__ int64_t waveRowsH(bool act, int32_t v, uint64_t pay, 
     int64_t acc = waveFlat(v, pay, aux, b, first, f);
     const int64_t i = b + first;
     uint64_t rest = __ballot(mine && i < end);
+    while (rest) {
+        const int l = __ffsll((long long)rest) - 1;
+        rest &= rest - 1;
+        const int32_t hv = __shfl(v, l), ha = __shfl(aux, l);
+        const uint64_t hp = __shfl(pay, l);
+        const int64_t hb = __shfl(i, l), he = __shfl(end, l);
+        for (int64_t j = hb + laneId(); j < he; j += WAVE) acc += f(hv, hp, ha, j);
+    }
+    return acc;
+}
+
+// waveRowsH with no heavy list: a long row is walked by its whole wave, 64
+// arcs at a time.  For passes whose arcs must be evaluated twice by the same
+// lanes in the same order (count, then write at the counted offsets).
+template <int SMALL = 16, class F>
+__device__ __forceinline__ int64_t waveRowsAll(bool act, int32_t v, uint64_t pay, int32_t aux, int64_t b,
+                                               int64_t end, const F& f) {
+    const int first = act ? (int)(end - b < SMALL ? end - b : SMALL) : 0;
+    int64_t acc = waveFlat(v, pay, aux, b, first, f);
+    const int64_t i = b + first;
+    uint64_t rest = __ballot(act && i < end);
     while (rest) {
         const int l = __ffsll((long long)rest) - 1;
         rest &= rest - 1;

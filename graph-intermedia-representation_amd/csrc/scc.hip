@@ -256,68 +256,127 @@ __global__ void kColorAssign(const int32_t* __restrict__ list, int64_t n, const 
 }
 
 // ---- DAG ----
-// Edge-parallel: DK_ITEMS coalesced slots per thread and block step, one
-// reservation per block step for the crossing arcs' packed keys.  A slot's
-// row (the arc's tail) comes from the block's window of row starts and
-// labels staged in LDS (a span of 4096 slots covers a few hundred rows), so
-// no per-slot row array has to be built after every change of the store.
-constexpr int DK_ITEMS = 16;
-constexpr int DK_ROWS = 2048;  // rows a span may cover through LDS (else global searches)
-
-// last r in [lo, hi] with beg[r] <= x (beg[lo] <= x)
-__device__ __forceinline__ int32_t rowOf(const int32_t* beg, int32_t lo, int32_t hi, int64_t x) {
-    while (lo < hi) {
-        const int32_t m = lo + ((hi - lo + 1) >> 1);
-        if (beg[m] <= x) lo = m;
-        else hi = m - 1;
+// The condensation's arcs are the crossing arcs (ends with different
+// labels), keyed (tail label << B) | head label, sorted and run-length
+// counted.  The big SCC (the label of the vertex of largest degree) gets a
+// bitmap, gbits (bit v set iff scc[v] == G; V / 8 bytes, L2-resident where
+// scc[] is not).
+__global__ void kMaxDegVertex(const int32_t* __restrict__ fb, const int32_t* __restrict__ bb, int32_t V,
+                              unsigned long long* best) {
+    unsigned long long k = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t d = (uint64_t)(fb[v + 1] - fb[v]) + (uint64_t)(bb[v + 1] - bb[v]);
+        const unsigned long long c = (d << 32) | (uint64_t)(0xffffffffu - (uint32_t)v);
+        k = c > k ? c : k;
     }
-    return lo;
-}
-
-__global__ void kDagKeysEdge(const int32_t* __restrict__ beg, int32_t V, const int32_t* __restrict__ adj, int64_t E,
-                             const int32_t* __restrict__ scc, int B, uint64_t* __restrict__ keys, int64_t* cnt) {
-    __shared__ int32_t rb[DK_ROWS + 1];
-    __shared__ int32_t rl[DK_ROWS];
-    __shared__ int32_t win[2];
-    const int64_t span = (int64_t)blockDim.x * DK_ITEMS;
-    for (int64_t base = (int64_t)blockIdx.x * span; base < E; base += (int64_t)gridDim.x * span) {
-        const int64_t last = (base + span < E ? base + span : E) - 1;
-        if (threadIdx.x == 0) {
-            const int32_t r0 = rowOf(beg, 0, V - 1, base);
-            win[0] = r0;
-            win[1] = rowOf(beg, r0, V - 1, last) - r0 + 1;
-        }
-        __syncthreads();
-        const int32_t r0 = win[0], nr = win[1];
-        const bool staged = nr <= DK_ROWS;
-        if (staged)
-            for (int i = threadIdx.x; i <= nr; i += blockDim.x) {
-                rb[i] = beg[r0 + i];
-                if (i < nr) rl[i] = scc[r0 + i];
-            }
-        __syncthreads();
-        uint64_t kv[DK_ITEMS];
-        bool cr[DK_ITEMS];
-        int n = 0;
-#pragma unroll
-        for (int t = 0; t < DK_ITEMS; ++t) {
-            const int64_t e = base + (int64_t)t * blockDim.x + threadIdx.x;
-            cr[t] = false;
-            kv[t] = 0;
-            if (e < E) {
-                const int32_t a = staged ? rl[rowOf(rb, 0, nr - 1, e)] : scc[rowOf(beg, r0, r0 + nr - 1, e)];
-                const int32_t c = scc[adj[e]];
-                cr[t] = a != c;
-                kv[t] = ((uint64_t)(uint32_t)a << B) | (uint32_t)c;
-            }
-            n += cr[t];
-        }
-        int64_t pos = blockReserve(n, cnt);
-#pragma unroll
-        for (int t = 0; t < DK_ITEMS; ++t)
-            if (cr[t]) keys[pos++] = kv[t];
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        const unsigned long long o = __shfl_xor(k, d);
+        k = o > k ? o : k;
+    }
+    // one atomic per block
+    __shared__ unsigned long long part[1024 / WAVE];
+    const int w = threadIdx.x / WAVE, nw = (blockDim.x + WAVE - 1) / WAVE;
+    if (laneId() == 0) part[w] = k;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < nw; ++i) k = part[i] > k ? part[i] : k;
+        if (k) atomicMax(best, k);
     }
 }
+
+// G = the label of the vertex of largest degree; bit v of bits = (scc[v] == G)
+__global__ void kLabelBits(const int32_t* __restrict__ scc, int32_t V, const unsigned long long* __restrict__ best,
+                           uint64_t* __restrict__ bits, int32_t* gout) {
+    const int32_t top = (int32_t)(0xffffffffu - (uint32_t)(*best & 0xffffffffULL));
+    const int32_t G = scc[top];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *gout = G;
+    const int64_t W = ((int64_t)V + 63) / 64;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < W * 64; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = base + threadIdx.x;
+        const uint64_t m = __ballot(v < V && scc[v] == G);
+        if (laneId() == 0) bits[v >> 6] = m;
+    }
+}
+
+// Every crossing arc has an end outside the big SCC: the arcs of the rows of
+// the vertices outside it (list) are all the crossing arcs - forward rows
+// give the arcs leaving such a vertex, backward rows the arcs entering it
+// from the big SCC - so the condensation keys come from those rows alone
+// (on a power-law graph a few percent of the arcs) instead of every arc.
+__global__ void kNonBig(const uint64_t* __restrict__ bits, int32_t V, int32_t* __restrict__ list, int64_t* cnt) {
+    const int64_t W = ((int64_t)V + 63) / 64;
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < W; b0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t w = b0 + threadIdx.x;
+        uint64_t m = 0;
+        if (w < W) {
+            m = ~bits[w];
+            const int64_t rest = (int64_t)V - w * 64;
+            if (rest < 64) m &= (1ULL << rest) - 1;
+        }
+        int64_t pos = blockReserve(__popcll(m), cnt);
+        while (m) {
+            const int b = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            list[pos++] = (int32_t)(w * 64 + b);
+        }
+    }
+}
+
+// Arc j of the forward (dir 0) or backward (dir 1) row of a vertex outside
+// the big SCC with label pay: does it cross SCCs, and its key.
+struct DagArc {
+    const int32_t* fa;
+    const int32_t* ba;
+    const int32_t* scc;
+    const uint64_t* gbits;
+    int32_t G;
+    int B;
+    __device__ __forceinline__ bool operator()(uint64_t pay, int32_t dir, int64_t j, uint64_t& key) const {
+        const int32_t a = (int32_t)pay;
+        if (dir == 0) {
+            const int32_t v = fa[j];
+            const int32_t c = ((gbits[v >> 6] >> (v & 63)) & 1) ? G : scc[v];
+            key = ((uint64_t)(uint32_t)a << B) | (uint32_t)c;
+            return c != a;
+        }
+        const int32_t x = ba[j];
+        key = ((uint64_t)(uint32_t)G << B) | (uint32_t)a;
+        return (gbits[x >> 6] >> (x & 63)) & 1;  // tails outside the big SCC own their arcs
+    }
+};
+
+// Block-synchronous over the list: the lanes count the crossing arcs they
+// evaluate, one reservation per block iteration gives each lane its offset,
+// and the same lanes evaluate the same arcs again, writing the keys there
+// (no counter hit per wave; long rows are walked by their wave).
+__global__ void kDagKeysRows(const int32_t* __restrict__ list, const int64_t* __restrict__ nlist,
+                             const int32_t* __restrict__ fb, const int32_t* __restrict__ bb, DagArc d,
+                             const int32_t* __restrict__ gl, uint64_t* __restrict__ keys, int64_t* cnt) {
+    const int64_t n = *nlist;
+    d.G = *gl;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        const bool act = i < n;
+        const int32_t u = act ? list[i] : 0;
+        const uint64_t a = act ? (uint64_t)(uint32_t)d.scc[u] : 0;
+        const int32_t f0 = act ? fb[u] : 0, f1 = act ? fb[u + 1] : 0;
+        const int32_t b0 = act ? bb[u] : 0, b1 = act ? bb[u + 1] : 0;
+        auto count = [&](int32_t, uint64_t pay, int32_t dir, int64_t j) -> int64_t {
+            uint64_t key;
+            return d(pay, dir, j, key) ? 1 : 0;
+        };
+        const int64_t m = waveRowsAll(act, u, a, 0, f0, f1, count) + waveRowsAll(act, u, a, 1, b0, b1, count);
+        int64_t off = blockReserve((int)m, cnt);
+        auto emit = [&](int32_t, uint64_t pay, int32_t dir, int64_t j) -> int64_t {
+            uint64_t key;
+            if (d(pay, dir, j, key)) keys[off++] = key;
+            return 0;
+        };
+        waveRowsAll(act, u, a, 0, f0, f1, emit);
+        waveRowsAll(act, u, a, 1, b0, b1, emit);
+    }
+}
+
 
 __global__ void kDagTKeys(const int32_t* __restrict__ src, const int32_t* __restrict__ dst, int64_t nd, int B,
                           uint64_t* __restrict__ tk) {
@@ -430,6 +489,7 @@ struct KahnF {
     int32_t* qout;
     int64_t* cnt;
     int32_t hub;
+    int* hub_lds;  // set by kKahnStep: the block sums its hub decrements here
     __device__ int64_t operator()(int32_t, uint64_t, int32_t, int64_t j) const {
         const int32_t t = ddst[j];
         const bool hot = t == hub;
@@ -438,7 +498,8 @@ struct KahnF {
         if (hot) {
             if (laneId() == __ffsll((long long)hm) - 1) {
                 const int c = __popcll(hm);
-                ready = atomicSub(&indeg[t], c) == c;
+                if (hub_lds) atomicAdd(hub_lds, c);
+                else ready = atomicSub(&indeg[t], c) == c;
             }
         } else {
             ready = atomicSub(&indeg[t], 1) == 1;
@@ -464,6 +525,13 @@ __global__ void kHub(const int32_t* __restrict__ indeg, int32_t V, int64_t* out)
 // over the queue, uniformly per wave.
 __global__ void kKahnStep(const int32_t* __restrict__ qin, const int64_t* nin, const int32_t* __restrict__ dbeg,
                           int64_t* rank, int32_t L, KahnF f, HeavyList H, int64_t* cz) {
+    // the hub's decrements: one LDS sum per block and one atomic per block
+    // at the end (a level of a power-law condensation holds many arcs into
+    // the giant SCC, and one counter hit by every wave serialises them)
+    __shared__ int hub_sum;
+    if (threadIdx.x == 0) hub_sum = 0;
+    __syncthreads();
+    f.hub_lds = &hub_sum;
     if (blockIdx.x == 0 && threadIdx.x == 0) *cz = 0;
     const int64_t n = *nin;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -474,6 +542,11 @@ __global__ void kKahnStep(const int32_t* __restrict__ qin, const int64_t* nin, c
         if (act) rank[s] = ((int64_t)L << 32) | (int64_t)(uint32_t)s;
         const int32_t b = act ? dbeg[s] : 0, e = act ? dbeg[s + 1] : 0;
         waveRowsH(act, s, 0, 0, b, e, H, f);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && hub_sum > 0 && atomicSub(&f.indeg[f.hub], hub_sum) == hub_sum) {
+        const int64_t pos = atomicAdd((unsigned long long*)f.cnt, 1ULL);
+        f.qout[pos] = f.hub;
     }
 }
 
@@ -636,10 +709,23 @@ void runDag(girhip_graph* h) {
     int64_t ncross = 0;
     if (h->E > 0) {
         S.keys[0].ensure((size_t)h->E);
+        S.gbits.ensure(((size_t)V + 63) / 64);
         fillI64(h, h->dcnt.p + 9, 0, 1);
-        const unsigned g = std::min<unsigned>(gridFor(h->E, blk * DK_ITEMS), 8192);
-        launch(h, "dag_keys", kDagKeysEdge, g, blk, 0, (const int32_t*)h->fw_beg.p, V, (const int32_t*)h->fw_adj.p,
-               h->E, (const int32_t*)h->scc.p, B, S.keys[0].p, h->dcnt.p + 9);
+        fillI64(h, h->dcnt.p + 14, 0, 1);
+        launch(h, "dag_top", kMaxDegVertex, boundedGrid(V, 256), 256, 0, (const int32_t*)h->fw_beg.p,
+               (const int32_t*)h->bw_beg.p, V, (unsigned long long*)(h->dcnt.p + 14));
+        launch(h, "dag_label_bits", kLabelBits, boundedGrid(((int64_t)V + 63) / 64 * 64, 256), 256, 0,
+               (const int32_t*)h->scc.p, V, (const unsigned long long*)(h->dcnt.p + 14), S.gbits.p,
+               (int32_t*)(h->dcnt.p + 15));
+        // the vertices outside the big SCC, then their rows' crossing arcs
+        S.iv[0].ensure((size_t)V);
+        fillI64(h, h->dcnt.p + 16, 0, 1);
+        launch(h, "dag_nonbig", kNonBig, boundedGrid(((int64_t)V + 63) / 64, 512), 512, 0, (const uint64_t*)S.gbits.p,
+               V, S.iv[0].p, h->dcnt.p + 16);
+        const DagArc d{h->fw_adj.p, h->bw_adj.p, h->scc.p, S.gbits.p, 0, B};
+        launch(h, "dag_keys", kDagKeysRows, std::min<unsigned>(gridFor(V, blk), 4096), blk, 0,
+               (const int32_t*)S.iv[0].p, (const int64_t*)(h->dcnt.p + 16), (const int32_t*)h->fw_beg.p,
+               (const int32_t*)h->bw_beg.p, d, (const int32_t*)(h->dcnt.p + 15), S.keys[0].p, h->dcnt.p + 9);
         readCounters(h, h->dcnt.p + 9, 1, &ncross);
     }
     int64_t nd = 0;
@@ -719,7 +805,7 @@ void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks) {
     while (true) {
         for (int t = 0; t < KAHN_CHUNK; ++t, ++L) {
             int64_t* cin = kc + L % KAHN_RING;
-            const KahnF f{h->dag_dst.p, ideg, R.q[cur ^ 1], kc + (L + 1) % KAHN_RING, hub};
+            const KahnF f{h->dag_dst.p, ideg, R.q[cur ^ 1], kc + (L + 1) % KAHN_RING, hub, nullptr};
             launch(h, "rank_kahn", kKahnStep, 1024, blk, 0, (const int32_t*)R.q[cur], (const int64_t*)cin,
                    (const int32_t*)h->dag_beg.p, h->rank.p, L, f, heavyList(h), kc + (L + 2) % KAHN_RING);
             launchHeavy(h, "rank_kahn_heavy", f);
