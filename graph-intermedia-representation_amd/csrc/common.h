@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <map>
 #include <string>
 #include <vector>
@@ -1075,6 +1076,16 @@ inline unsigned summaryGrid(int64_t W2, int G, int64_t reps, int block) {
 
 // The listed rows' arcs as one range [0, off[n]): every thread strides over
 // it and finds the owning row by binary search in off.
+// Functors whose arcs append to a queue can instead expose heavyClaim (the
+// arc's work, returning whether it appends an item), pushCounter and
+// heavyPlace: kHeavyRows then walks its range in block-wide steps and
+// reserves each step's items with one atomic per block (a hub's row of
+// 10^5 arcs would otherwise hit the queue counter once per wave).
+template <class F, class = void>
+struct HasHeavyClaim : std::false_type {};
+template <class F>
+struct HasHeavyClaim<F, std::void_t<decltype(&F::heavyClaim)>> : std::true_type {};
+
 template <class F>
 __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
     const uint64_t c = (uint64_t)ldAcq(&H.ctl[0]);
@@ -1087,7 +1098,7 @@ __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
     const int64_t p0 = (int64_t)blockIdx.x * per, p1 = p0 + per < T ? p0 + per : T;
     int64_t acc = 0;
     int64_t k = 0;
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    auto rowOfArc = [&](int64_t p) {
         if (!(p >= H.off[k] && p < H.off[k] + (H.e[k] - H.b[k]))) {
             int64_t lo = 0, hi = n - 1;  // last k with off[k] <= p
             while (lo < hi) {
@@ -1097,7 +1108,24 @@ __global__ void kHeavyRows(HeavyList H, F f, int64_t* sum) {
             }
             k = lo;
         }
-        acc += f(H.row[k], H.pay[k], H.aux[k], H.b[k] + (p - H.off[k]));
+    };
+    if constexpr (HasHeavyClaim<F>::value) {
+        for (int64_t pb = p0; pb < p1; pb += blockDim.x) {  // (uniform per block)
+            const int64_t p = pb + threadIdx.x;
+            bool push = false;
+            int32_t item = 0;
+            if (p < p1) {
+                rowOfArc(p);
+                acc += f.heavyClaim(H.row[k], H.pay[k], H.aux[k], H.b[k] + (p - H.off[k]), push, item);
+            }
+            const int64_t pos = blockReserve(push ? 1 : 0, f.pushCounter());
+            if (push) f.heavyPlace(pos, item);
+        }
+    } else {
+        for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+            rowOfArc(p);
+            acc += f(H.row[k], H.pay[k], H.aux[k], H.b[k] + (p - H.off[k]));
+        }
     }
     if (sum) {
         acc = waveSum(acc);

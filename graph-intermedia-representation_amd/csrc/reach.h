@@ -69,6 +69,23 @@ struct TopDownF {
         }
         return 0;
     }
+    // the heavy pass's form: claim, and the queue slot comes from the block
+    __device__ int64_t heavyClaim(int32_t p, uint64_t, int32_t, int64_t j, bool& push, int32_t& item) const {
+        const int32_t w = adj[j];
+        bool claim = false;
+        if (flt(w, p)) {
+            const uint64_t bit = 1ULL << (w & 63);
+            if (!(vis[w >> 6] & bit))
+                claim = !(atomicOr((unsigned long long*)&vis[w >> 6], (unsigned long long)bit) & bit);
+        }
+        push = claim;
+        item = w;
+        if (!claim) return 0;
+        on(w, L + 1);
+        return beg[w + 1] - beg[w];
+    }
+    __device__ int64_t* pushCounter() const { return &cnt[0]; }
+    __device__ void heavyPlace(int64_t pos, int32_t w) const { qout[pos] = w; }
 };
 
 // The input queue length is read on the device (the previous level's

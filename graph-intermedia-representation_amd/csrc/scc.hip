@@ -384,38 +384,27 @@ __global__ void kDagTKeys(const int32_t* __restrict__ src, const int32_t* __rest
     if (r < nd) tk[r] = ((uint64_t)(uint32_t)dst[r] << B) | (uint32_t)src[r];
 }
 
-// In-degree of every SCC id from the arcs sorted by destination.
-__global__ void kIndegFromKeys(const uint64_t* __restrict__ tk, int64_t nd, int B, int32_t V,
-                               int32_t* __restrict__ indeg) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= V) return;
-    auto lb = [&](uint64_t key) {
-        int64_t lo = 0, hi = nd;
-        while (lo < hi) {
-            const int64_t m = lo + ((hi - lo) >> 1);
-            if (tk[m] < key) lo = m + 1;
-            else hi = m;
-        }
-        return lo;
-    };
-    indeg[t] = (int32_t)(lb((uint64_t)(t + 1) << B) - lb((uint64_t)t << B));
+// CSR offsets from the sorted row ids of nd arcs (id(i) = ids[i], or
+// keys[i] >> B): thread i in [0, nd] writes beg[v] = i for every v in
+// (id(i - 1), id(i)] (id(nd) = V), i.e. the first arc of every row - one
+// pass over the arcs instead of a binary search per row.
+__global__ void kBegScatter(const int32_t* __restrict__ ids, const uint64_t* __restrict__ keys, int B, int64_t nd,
+                            int32_t V, int32_t* __restrict__ beg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nd) return;
+    auto id = [&](int64_t k) -> int64_t { return ids ? (int64_t)ids[k] : (int64_t)(keys[k] >> B); };
+    const int64_t cur = i < nd ? id(i) : (int64_t)V;
+    const int64_t prev = i > 0 ? id(i - 1) : -1;
+    for (int64_t v = prev + 1; v <= cur; ++v) beg[v] = (int32_t)i;
 }
 
-// In-arc CSR of the DAG from its arcs' keys sorted by (dst, src).
-__global__ void kDagInCsr(const uint64_t* __restrict__ tk, int64_t nd, int B, int32_t V, int32_t* __restrict__ beg,
-                          int32_t* __restrict__ src) {
+// indeg[v] = beg[v + 1] - beg[v]; with keys, also src[i] = the low B bits
+// of keys[i] (the in-arc CSR's sources).
+__global__ void kDegAndSrc(const int32_t* __restrict__ beg, int32_t V, int32_t* __restrict__ indeg,
+                           const uint64_t* __restrict__ keys, int64_t nd, int B, int32_t* __restrict__ src) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nd) src[i] = (int32_t)(tk[i] & ((1ULL << B) - 1));
-    if (i <= V) {
-        const uint64_t key = (uint64_t)i << B;
-        int64_t lo = 0, hi = nd;
-        while (lo < hi) {
-            const int64_t m = lo + ((hi - lo) >> 1);
-            if (tk[m] < key) lo = m + 1;
-            else hi = m;
-        }
-        beg[i] = (int32_t)lo;
-    }
+    if (i < V) indeg[i] = beg[i + 1] - beg[i];
+    if (i < nd) src[i] = (int32_t)(keys[i] & ((1ULL << B) - 1));
 }
 
 __global__ void kRunFlags(const uint64_t* __restrict__ keys, int64_t n, int32_t* __restrict__ flag) {
@@ -440,19 +429,6 @@ __global__ void kDagWrite(const uint64_t* __restrict__ keys, const int32_t* __re
     src[r] = a;
     dst[r] = b;
     mult[r] = (int32_t)(j - i);
-}
-
-// DAG CSR offsets: arcs are sorted by source, so beg[s] = lower_bound(src, s).
-__global__ void kDagBeg(const int32_t* __restrict__ src, int64_t nd, int32_t V, int32_t* __restrict__ beg) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > V) return;
-    int64_t lo = 0, hi = nd;
-    while (lo < hi) {
-        const int64_t m = lo + ((hi - lo) >> 1);
-        if (src[m] < s) lo = m + 1;
-        else hi = m;
-    }
-    beg[s] = (int32_t)lo;
 }
 
 // ---- ranks (Kahn levels) ----
@@ -547,6 +523,55 @@ __global__ void kKahnStep(const int32_t* __restrict__ qin, const int64_t* nin, c
     if (threadIdx.x == 0 && hub_sum > 0 && atomicSub(&f.indeg[f.hub], hub_sum) == hub_sum) {
         const int64_t pos = atomicAdd((unsigned long long*)f.cnt, 1ULL);
         f.qout[pos] = f.hub;
+    }
+}
+
+// The heavy rows of a Kahn level (kHeavyRows' walk: the listed rows' arcs
+// concatenated, a contiguous range per block) with the level kernel's
+// aggregation: the block walks its range in block-wide steps, reserves the
+// step's ready heads with one atomic, and sums its hub decrements in LDS (a
+// power-law condensation's big SCC can hold 10^5 out-arcs in one row, whose
+// heads would otherwise hit the queue counter once per wave).
+__global__ void kKahnHeavy(HeavyList H, KahnF f) {
+    const uint64_t c = (uint64_t)ldAcq(&H.ctl[0]);
+    if (c == 0) return;
+    __shared__ int hub_sum;
+    if (threadIdx.x == 0) hub_sum = 0;
+    __syncthreads();
+    const int64_t n = (int64_t)(c >> HEAVY_SHIFT), T = (int64_t)(c & HEAVY_MASK);
+    const int64_t per = (T + gridDim.x - 1) / gridDim.x;
+    const int64_t p0 = (int64_t)blockIdx.x * per, p1 = p0 + per < T ? p0 + per : T;
+    int64_t k = 0;
+    for (int64_t pb = p0; pb < p1; pb += blockDim.x) {
+        const int64_t p = pb + threadIdx.x;
+        bool ready = false;
+        int32_t t = 0;
+        if (p < p1) {
+            if (!(p >= H.off[k] && p < H.off[k] + (H.e[k] - H.b[k]))) {
+                int64_t lo = 0, hi = n - 1;  // last k with off[k] <= p
+                while (lo < hi) {
+                    const int64_t m = (lo + hi + 1) >> 1;
+                    if (H.off[m] <= p) lo = m;
+                    else hi = m - 1;
+                }
+                k = lo;
+            }
+            t = f.ddst[H.b[k] + (p - H.off[k])];
+            if (t == f.hub) atomicAdd(&hub_sum, 1);
+            else ready = atomicSub(&f.indeg[t], 1) == 1;
+        }
+        const int64_t pos = blockReserve(ready ? 1 : 0, f.cnt);
+        if (ready) f.qout[pos] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && hub_sum > 0 && atomicSub(&f.indeg[f.hub], hub_sum) == hub_sum) {
+        const int64_t pos = atomicAdd((unsigned long long*)f.cnt, 1ULL);
+        f.qout[pos] = f.hub;
+    }
+    // the last block to finish clears the list (as kHeavyRows)
+    if (threadIdx.x == 0 && atomicAdd((unsigned long long*)&H.ctl[1], 1ULL) == gridDim.x - 1) {
+        H.ctl[0] = 0;
+        H.ctl[1] = 0;
     }
 }
 
@@ -760,29 +785,29 @@ void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks) {
     h->dag_beg.ensure((size_t)V + 1);
     S.iv[2].ensure((size_t)V + 1);
     int32_t* ideg = S.iv[2].p;
-    fillI32(h, ideg, 0, (int64_t)V + 1);
     if (nd > 0) {
         S.keys[0].ensure((size_t)nd);
-        // in-degrees from the arcs sorted by destination (no contended atomics)
+        // the arcs sorted by destination: the in-arc CSR of the DAG, and the
+        // in-degrees as its row lengths (no contended atomics)
         launch(h, "dag_tkeys", kDagTKeys, gridFor(nd, blk), blk, 0, (const int32_t*)h->dag_src.p,
                (const int32_t*)h->dag_dst.p, nd, B, S.keys[0].p);
         radixSort(h, S.keys[0].p, nullptr, nd, 2 * B);
-        launch(h, "dag_indeg", kIndegFromKeys, gridFor(V, blk), blk, 0, (const uint64_t*)S.keys[0].p, nd, B, V, ideg);
-        // the same sorted keys are the in-arc CSR of the DAG
         h->dag_in_src.ensure((size_t)nd);
         h->dag_in_beg.ensure((size_t)V + 1);
-        launch(h, "dag_in", kDagInCsr, gridFor(std::max<int64_t>(nd, (int64_t)V + 1), blk), blk, 0,
-               (const uint64_t*)S.keys[0].p, nd, B, V, h->dag_in_beg.p, h->dag_in_src.p);
-    }
-    if (nd == 0) {
+        launch(h, "dag_in", kBegScatter, gridFor(nd + 1, blk), blk, 0, (const int32_t*)nullptr,
+               (const uint64_t*)S.keys[0].p, B, nd, V, h->dag_in_beg.p);
+        launch(h, "dag_indeg", kDegAndSrc, gridFor(std::max<int64_t>(nd, V), blk), blk, 0,
+               (const int32_t*)h->dag_in_beg.p, V, ideg, (const uint64_t*)S.keys[0].p, nd, B, h->dag_in_src.p);
+    } else {
+        fillI32(h, ideg, 0, (int64_t)V + 1);
         h->dag_in_beg.ensure((size_t)V + 1);
         h->dag_in_src.ensure(1);
         fillI32(h, h->dag_in_beg.p, 0, (int64_t)V + 1);
     }
     h->n_dag = nd;
     // DAG CSR offsets by source SCC (arcs are sorted by src)
-    launch(h, "dag_beg", kDagBeg, gridFor((int64_t)V + 1, blk), blk, 0, (const int32_t*)h->dag_src.p, nd, V,
-           h->dag_beg.p);
+    launch(h, "dag_beg", kBegScatter, gridFor(nd + 1, blk), blk, 0, (const int32_t*)h->dag_src.p,
+           (const uint64_t*)nullptr, 0, nd, V, h->dag_beg.p);
     if (!ranks) return;
     // Kahn levels
     ReachBufs R = reachBufs(h);
@@ -808,7 +833,7 @@ void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks) {
             const KahnF f{h->dag_dst.p, ideg, R.q[cur ^ 1], kc + (L + 1) % KAHN_RING, hub, nullptr};
             launch(h, "rank_kahn", kKahnStep, 1024, blk, 0, (const int32_t*)R.q[cur], (const int64_t*)cin,
                    (const int32_t*)h->dag_beg.p, h->rank.p, L, f, heavyList(h), kc + (L + 2) % KAHN_RING);
-            launchHeavy(h, "rank_kahn_heavy", f);
+            launch(h, "rank_kahn_heavy", kKahnHeavy, 1024, 256, 0, heavyList(h), f);
             cur ^= 1;
         }
         int64_t nq;
