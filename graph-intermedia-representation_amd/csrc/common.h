@@ -100,6 +100,7 @@ struct Params {
     double scc_trim2 = 0;        // 1: each SCC trim round also runs trim-2 (pairs)
     double scc_incremental = 1;  // 1: update batches repair the SCC layer; 0: rebuild it
     double sssp_unroll = 2;      // arcs per lane at a time in the SSSP relaxation passes (1, 2 or 4)
+    double bfs_alpha_auto = 1;   // 1: the per-source reach raises bfs_alpha to 64 on graphs with hubs
 };
 
 struct KStat {

@@ -281,6 +281,9 @@ int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
  *   given delta, kept for the two deltas used last, E x 8 bytes each,
  *   rebuilt after the store or the weights change; 0: rows classify their
  *   arcs on the fly),
+ *   "bfs_alpha_auto" (1: the per-source reachability engine - single-source
+ *   BFS, the SCC forward-backward sweeps - raises bfs_alpha to 64 on graphs
+ *   whose largest degree is at least 64 times the mean; 0: bfs_alpha as set),
  *   "sssp_interleave" (2: automatic - the shared-bucket engine with
  *   interleaved distances for one source, for up to 8 sources when
  *   4 delta >= the largest weight, for more when 2 delta >= it; 1: always;

@@ -273,7 +273,7 @@ def test_tuning_knobs_do_not_change_results():
                      ("persist", 1), ("chunk", 1), ("sssp_dist32", 0), ("xcd", 1), ("split", 0), ("scc_block", 256), ("scc_block", 1024),
                      ("sssp_delta", 100000), ("sssp_interleave", 0), ("sssp_interleave", 1),
                      ("sssp_ms_batch", 3), ("sssp_ms_batch", 32), ("scc_trim2", 1), ("sssp_unroll", 1),
-                     ("sssp_unroll", 2)]:
+                     ("sssp_unroll", 2), ("bfs_alpha_auto", 0), ("bfs_alpha", 1000)]:
             say("knob", k, v)
             old = G.get_param(k)
             G.set_param(k, v)
