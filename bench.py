@@ -40,6 +40,7 @@ WORKLOADS = {
     "grid4800": dict(kind="grid", rows=4800, cols=4800, p_keep=0.9, p_bidir=0.8, seed=3, wlo=1, whi=1000),
 }
 N_BFS, N_SSSP = 64, 8
+N_SINGLE = 100  # single-edge inserts and deletes (each)
 SEED_W, SEED_BFS, SEED_SSSP = 12, 7, 8
 
 
@@ -74,7 +75,7 @@ def block_partition(bid, v):
 
 
 def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=None, reach_pairs=None,
-                 two_level=False, bc_src=None, bicc=False, reps=1, outputs=False):
+                 two_level=False, bc_src=None, bicc=False, reps=1, outputs=False, single=None):
     """Time the compiled reference (oracle/_ref/gir_ref: the reference's own
     gir.cpp / gir_query.cpp) on this host: one buildWCC + buildSCC_DAG, BFS
     from n_bfs and SSSP (its hash weights, delta 16) from n_sssp sources, and
@@ -83,7 +84,10 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
     GIR::bc over those sources (scores returned for comparison).  reps: the
     build and the BFS / SSSP passes are timed that many times (medians
     reported); outputs: the reference's canonical WCC / SCC labels, DAG arcs,
-    BFS levels and SSSP distances are returned under "outputs"."""
+    BFS levels and SSSP distances are returned under "outputs".  single
+    (u, v, insert arrays, not with batch): the ops applied one
+    GIR::insertEdge / deleteEdge call each, timed per kind; with outputs the
+    WCC / SCC labels after them are returned as post_wcc / post_scc."""
     import girhip
     ref = os.path.join(ROOT, "oracle", "_ref", "gir_ref")
     if not os.path.exists(ref):
@@ -97,6 +101,9 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
         if batch is not None:
             np.stack([np.asarray(x, np.int32) for x in batch], axis=1).tofile(os.path.join(td, "ops.bin"))
             cmd += ["--batch", os.path.join(td, "ops.bin")]
+        if single is not None:
+            np.stack([np.asarray(x, np.int32) for x in single], axis=1).tofile(os.path.join(td, "single.bin"))
+            cmd += ["--single", os.path.join(td, "single.bin")]
         if reach_pairs is not None:
             np.ascontiguousarray(reach_pairs, np.int32).tofile(os.path.join(td, "reach_pairs.bin"))
             cmd += ["--reach", os.path.join(td, "reach_pairs.bin")]
@@ -126,8 +133,14 @@ def cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=4, n_sssp=2, batch=Non
             rd = lambda f, t: np.fromfile(os.path.join(td, f), t)
             V = len(fb) - 1
             res = {"wcc": rd("wcc.bin", np.int32), "scc": rd("scc.bin", np.int32),
-                   "dag": rd("dag.bin", np.int32).reshape(-1, 2),
-                   "bfs": rd("bfs.bin", np.int32).reshape(-1, V), "sssp": rd("sssp.bin", np.uint64).reshape(-1, V)}
+                   "dag": rd("dag.bin", np.int32).reshape(-1, 2)}
+            if n_bfs > 0:
+                res["bfs"] = rd("bfs.bin", np.int32).reshape(-1, V)
+            if n_sssp > 0:
+                res["sssp"] = rd("sssp.bin", np.uint64).reshape(-1, V)
+            if single is not None:
+                res["post_wcc"] = rd("post_wcc.bin", np.int32)
+                res["post_scc"] = rd("post_scc.bin", np.int32)
     rec = {}
     multi = {}
     for line in out.splitlines():
@@ -649,6 +662,54 @@ def main():
                             "rebuild_wcc_scc_ms": round(rebuild, 3)})
         log(f"updates: {updates}")
 
+    # ---- single-edge updates (the reference's SPU phase, bench_main.cpp:
+    #      116-125: one GIR::insertEdge / deleteEdge call per op): on a fresh
+    #      graph with its layers built, N_SINGLE inserts of absent uniform
+    #      pairs, then N_SINGLE deletes of present arcs, one
+    #      girhip_apply_batch call each ----
+    spu = None
+    spu_ops = None
+    if not args.no_extras:
+        srng = np.random.default_rng(31)
+        keys0 = (np.repeat(np.arange(V, dtype=np.int64), np.diff(fb)) << 32) | fa.astype(np.int64)
+        ck = np.unique((srng.integers(0, V, 4 * N_SINGLE).astype(np.int64) << 32) | srng.integers(0, V, 4 * N_SINGLE))
+        ik = srng.permutation(ck[~np.isin(ck, keys0) & ((ck >> 32) != (ck & 0xffffffff))])[:N_SINGLE]
+        dk = srng.permutation(np.unique(keys0[srng.integers(0, len(keys0), 2 * N_SINGLE)]))[:N_SINGLE]
+        del keys0
+        both = np.concatenate([ik, dk])
+        su, sv = (both >> 32).astype(np.int32), (both & 0xffffffff).astype(np.int32)
+        sins = np.concatenate([np.ones(len(ik), np.uint8), np.zeros(len(dk), np.uint8)])
+        spu_ops = (su, sv, sins)
+        GS = girhip.Graph(*d_csr)
+        GS.wcc(out=lab)
+        GS.scc_dag(out=scc)
+        # one untimed op of each kind first (work buffers of the batch path)
+        GS.apply_batch(su[:1], sv[:1], sins[:1])
+        GS.apply_batch(su[:1], sv[:1], (1 - sins[:1]).astype(np.uint8))
+        wall = {1: [], 0: []}
+        dev_ms = {1: [], 0: []}
+        paths = {}
+        for i in range(len(su)):
+            t0 = time.perf_counter()
+            GS.apply_batch(su[i:i + 1], sv[i:i + 1], sins[i:i + 1])
+            wall[int(sins[i])].append((time.perf_counter() - t0) * 1e3)
+            dev_ms[int(sins[i])].append(GS.last_ms("batch"))
+            p_ = int(GS.last_ms("batch_scc_path"))
+            paths[p_] = paths.get(p_, 0) + 1
+        spu = {"inserts": len(ik), "deletes": len(dk),
+                  "insert_ms_per_op": round(float(np.mean(wall[1])), 4),
+                  "delete_ms_per_op": round(float(np.mean(wall[0])), 4),
+                  "insert_device_ms_per_op": round(float(np.mean(dev_ms[1])), 4),
+                  "delete_device_ms_per_op": round(float(np.mean(dev_ms[0])), 4),
+                  "ops_per_s": round(len(su) / (sum(wall[1]) + sum(wall[0])) * 1e3, 1),
+                  "scc_repair_paths": {str(k): v for k, v in sorted(paths.items())},
+                  "timing": "host wall clock per girhip_apply_batch call of one op (copies and sync included)"}
+        gs_scc = GS.get_scc()[0]
+        gs_wcc = GS.get_wcc()[0]
+        GS.close()
+        del GS
+        log(f"single-edge updates: {spu}")
+
     # ---- batched form (girhip_run_batch): 1,024 mixed BFS/SSSP queries from
     #      uniform random sources interleaved with update batches of 10^3,
     #      10^5 and 10^6 ops, one call; the reference's hash weights, as its
@@ -797,6 +858,19 @@ def main():
                        + cpu["batch_ms"] + cpu["batch1e5_ms"] + cpu["batch1e6_ms"])
                 queries["reference_one_call_at_a_time_ms_est"] = round(est, 1)
                 queries["speedup_vs_cpu_est"] = round(est / (queries["queries_ms"] + queries["updates_ms"]), 2)
+            if spu is not None:
+                rs = cpu_baseline(fb, fa, bb, ba, bfs_src, sssp_src, n_bfs=0, n_sssp=0, single=spu_ops,
+                                  outputs=True)
+                if rs is not None and "single" in rs[0]:
+                    c1 = rs[0]["single"]
+                    cpu["single_insert_ms_per_op"] = round(c1["insert_ms"] / max(1, c1["inserts"]), 4)
+                    cpu["single_delete_ms_per_op"] = round(c1["delete_ms"] / max(1, c1["deletes"]), 4)
+                    cpu["single_ops"] = c1["inserts"] + c1["deletes"]
+                    so = rs[0].get("outputs") or {}
+                    spu["matches_reference"] = bool(np.array_equal(so.get("post_wcc"), gs_wcc)
+                                                       and np.array_equal(so.get("post_scc"), gs_scc))
+                    spu["speedup_insert_vs_cpu"] = round(cpu["single_insert_ms_per_op"] / spu["insert_ms_per_op"], 3)
+                    spu["speedup_delete_vs_cpu"] = round(cpu["single_delete_ms_per_op"] / spu["delete_ms_per_op"], 3)
             log(f"cpu build {cpu_build:.1f} ms")
 
     out = {
@@ -856,6 +930,7 @@ def main():
     out["reachable"] = reach
     out["query_batch"] = queries
     out["update_batches"] = updates
+    out["single_edge"] = spu
     if kernels:
         out["kernels"] = kernels
     G.close()

@@ -8,7 +8,7 @@
 // results with canonical (minimum-vertex) component ids:
 //
 //   gir_ref CSR_DIR OUT_DIR [--bfs SRC.bin] [--sssp SRC.bin] [--delta D]
-//           [--batch OPS.bin] [--pairs PAIRS.bin] [--reach PAIRS.bin] [--two-level] [--bc SRC.bin]
+//           [--batch OPS.bin | --single OPS.bin] [--pairs PAIRS.bin] [--reach PAIRS.bin] [--two-level] [--bc SRC.bin]
 //           [--bicc] [--no-output] [--reps R]
 //
 //   --reps R: the build and the BFS / SSSP passes are timed R times (one
@@ -29,8 +29,9 @@
 //   bc.bin   double[V] GIR::bc over the sources of --bc (gir_query.cpp:728-778)
 //   bicc.bin int32[2*k] (block, vertex) pairs of GIR::getBCTComponents over
 //            every SCC after buildBCT (gir.cpp:2063, 4140-4149); with --bicc
-//   post_{wcc,scc,dag}.bin  the same after GIR::applyBatch of OPS.bin
-//                        (int32 triples u, v, insert) with blocks on demand
+//   post_{wcc,scc,dag}.bin  the same after GIR::applyBatch of OPS.bin (int32
+//            triples u, v, insert) with blocks on demand, or after the ops of
+//            --single applied one GIR::insertEdge / deleteEdge call each
 // One JSON line per timed phase goes to stdout, prefixed "@@".
 #include <sys/stat.h>
 
@@ -122,13 +123,13 @@ void writeLayers(const std::string& out, const std::string& pre, const Layers& L
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--pairs F] [--reach F] [--two-level] [--bc F] [--bicc] [--no-output] [--reps R]\n");
+        std::fprintf(stderr, "usage: gir_ref CSR_DIR OUT_DIR [--bfs F] [--sssp F] [--delta D] [--batch F] [--single F] [--pairs F] [--reach F] [--two-level] [--bc F] [--bicc] [--no-output] [--reps R]\n");
         return 2;
     }
     std::string dir = argv[1], out = argv[2];
     if (dir.back() != '/') dir += '/';
     if (out.back() != '/') out += '/';
-    std::string bfs_f, sssp_f, batch_f, pairs_f, reach_f, bc_f;
+    std::string bfs_f, sssp_f, batch_f, single_f, pairs_f, reach_f, bc_f;
     uint64_t delta = 16;
     bool write = true, two_level = false, bicc = false;
     int reps = 1;
@@ -139,6 +140,7 @@ int main(int argc, char** argv) {
         else if (a == "--sssp") sssp_f = next();
         else if (a == "--delta") delta = std::strtoull(next().c_str(), nullptr, 10);
         else if (a == "--batch") batch_f = next();
+        else if (a == "--single") single_f = next();
         else if (a == "--pairs") pairs_f = next();
         else if (a == "--reach") reach_f = next();
         else if (a == "--no-output") write = false;
@@ -271,6 +273,30 @@ int main(int argc, char** argv) {
         for (size_t i = 0; i < r.size(); ++i) r[i] = g.reachable(pr[2 * i], pr[2 * i + 1]) ? 1 : 0;
         std::printf("@@{\"phase\": \"reach\", \"pairs\": %zu, \"ms\": %.3f}\n", r.size(), msSince(t0));
         writeVec(out + "reach.bin", r.data(), r.size());  // small: written even with --no-output
+    }
+    if (!single_f.empty()) {
+        // bench_main.cpp:116-125 (runSingle): one GIR::insertEdge /
+        // deleteEdge call per op, timed per kind; the block layer on demand
+        // as in the batch phase below.  The ops go to this graph, so
+        // --single is not combined with --batch (the bench runs them apart).
+        g.buildBCT();
+        g.setBlocksOnDemand(true);
+        auto ops = readVec<int>(single_f);
+        double ms[2] = {0, 0};
+        long long cnt[2] = {0, 0};
+        for (size_t i = 0; i + 2 < ops.size(); i += 3) {
+            const bool ins = ops[i + 2] != 0;
+            auto t0 = Clock::now();
+            if (ins) g.insertEdge(ops[i], ops[i + 1]);
+            else g.deleteEdge(ops[i], ops[i + 1]);
+            ms[ins ? 1 : 0] += msSince(t0);
+            ++cnt[ins ? 1 : 0];
+        }
+        std::printf("@@{\"phase\": \"single\", \"inserts\": %lld, \"insert_ms\": %.3f, \"deletes\": %lld, "
+                    "\"delete_ms\": %.3f}\n",
+                    cnt[1], ms[1], cnt[0], ms[0]);
+        Layers P = snapshot(g);
+        if (write) writeLayers(out, "post_", P);
     }
     if (!batch_f.empty()) {
         // The reference's drivers build the block layer before any update
