@@ -388,8 +388,20 @@ void applyStore(girhip_graph* h, const int32_t* u, const int32_t* v, const uint8
 // store the layers were current with (nullptr: layers are stale, rebuild).
 void repairLayers(girhip_graph* h, const Changes* C, int B) {
     const bool changed = C && (C->ni + C->nd) > 0;
+    // a few deletes: does each one's tail still reach its head?  Then no
+    // delete splits a WCC or an SCC.
+    const uint8_t* bypass = nullptr;
+    bool all_bypassed = false;
+    if (C && C->nd > 0 && C->nd <= BYPASS_MAX && h->prm.update_bypass != 0 &&
+        ((h->wcc_req && h->wcc_ok) || (h->scc_req && h->scc_ok))) {
+        Scratch& S = scratch(h);
+        S.bp_found.ensure((size_t)BYPASS_MAX);
+        all_bypassed = deletesBypassed(h, C->dk, C->nd, B, S.bp_found.p);
+        bypass = S.bp_found.p;
+    }
+    h->last_ms["batch_all_bypassed"] = all_bypassed ? 1 : 0;
     if (h->wcc_req) {
-        if (h->wcc_ok && C && C->nd == 0) {
+        if (h->wcc_ok && C && (C->nd == 0 || all_bypassed)) {
             if (C->ni) runWccHookArcs(h, C->ik, C->ni, B);  // insert-only: hook the new arcs
         } else if (!h->wcc_ok || changed) {
             runWcc(h);
@@ -397,7 +409,7 @@ void repairLayers(girhip_graph* h, const Changes* C, int B) {
         h->wcc_ok = true;
     }
     if (h->scc_req) {
-        if (h->scc_ok && C && changed && repairScc(h, C->ik, C->ni, C->dk, C->nd, B)) {
+        if (h->scc_ok && C && changed && repairScc(h, C->ik, C->ni, C->dk, C->nd, B, bypass)) {
             // repaired from the classified changes
         } else if (!h->scc_ok || changed) {
             runScc(h);

@@ -101,6 +101,7 @@ struct Params {
     double scc_incremental = 1;  // 1: update batches repair the SCC layer; 0: rebuild it
     double sssp_unroll = 2;      // arcs per lane at a time in the SSSP relaxation passes (1, 2 or 4)
     double bfs_alpha_auto = 1;   // 1: the per-source reach raises bfs_alpha to 64 on graphs with hubs
+    double update_bypass = 1;    // 1: batches of few deletes test them for a bypass path first
 };
 
 struct KStat {
@@ -146,6 +147,8 @@ struct Scratch {
     DevBuf<uint32_t> ms_qm[3];
     DevBuf<int64_t> ms_ctl;
     DevBuf<uint64_t> gbits;  // condensation keys: one bit per vertex, set iff its label is the big one
+    DevBuf<uint64_t> bp_bits;  // delete bypass check: one bitmap per delete
+    DevBuf<uint8_t> bp_found;
     DevBuf<int32_t> src;     // query sources
     // mixed query / update lists: source degrees, output rows, trivial
     // sources, live-source rows, host-output staging
@@ -456,7 +459,13 @@ void runDag(girhip_graph* h);
 void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks);
 // Repair the SCC layer after a batch whose sorted change keys are ik / dk
 // (scc.hip); false: rebuild it instead.
-bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* dk, int64_t nd, int B);
+// dbypass (device, nd bytes, may be null): deletes known not to change
+// reachability (their tail still reaches their head).
+bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* dk, int64_t nd, int B,
+               const uint8_t* dbypass);
+// Batches with at most BYPASS_MAX deletes test them for a bypass path.
+constexpr int64_t BYPASS_MAX = 8;
+bool deletesBypassed(girhip_graph* h, const uint64_t* dk, int64_t nd, int B, uint8_t* found);
 
 inline Scratch& scratch(girhip_graph* h) { return h->ws; }
 // Address of the allocation-free entry point's scratch array on device

@@ -873,7 +873,8 @@ enum : int { IR_DELTA = 0, IR_BACK, IR_DIRTY, IR_MINR, IR_MAXR, IR_AFF, IR_N };
 __global__ void kIrClassify(const uint64_t* __restrict__ ik, int64_t ni, const uint64_t* __restrict__ dk, int64_t nd,
                             int B, const int32_t* __restrict__ scc, const int64_t* __restrict__ rank,
                             uint64_t* __restrict__ dkeys, uint32_t* __restrict__ dvals, int32_t* __restrict__ bsu,
-                            int32_t* __restrict__ bsv, uint8_t* __restrict__ dirty, int64_t* cnt) {
+                            int32_t* __restrict__ bsv, uint8_t* __restrict__ dirty, int64_t* cnt,
+                            const uint8_t* __restrict__ dbypass) {
     const uint64_t mask = (1ULL << B) - 1;
     const int64_t n = ni + nd, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
@@ -887,7 +888,8 @@ __global__ void kIrClassify(const uint64_t* __restrict__ ik, int64_t ni, const u
         }
         const bool cross = act && su != sv;
         const bool back = cross && ins && rank[su] >= rank[sv];
-        const bool inner_del = act && !ins && su == sv;
+        // an inner delete whose tail still reaches its head splits nothing
+        const bool inner_del = act && !ins && su == sv && !(dbypass && dbypass[i - ni]);
         if (inner_del) dirty[su] = 1;
         const int64_t dpos = wavePush(cnt + IR_DELTA, cross);
         if (cross) {
@@ -1006,7 +1008,8 @@ __global__ void kIrMark(int32_t* scc, int32_t V, const uint8_t* __restrict__ dir
 // SCCs that may change (a rebuild then costs about the same and skips the
 // repair's own passes; on a power-law graph a random batch almost always
 // deletes an arc inside the giant SCC).
-bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* dk, int64_t nd, int B) {
+bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* dk, int64_t nd, int B,
+               const uint8_t* dbypass) {
     if (h->prm.scc_incremental == 0) return false;
     if (ni + nd > std::max<int64_t>(4096, (int64_t)h->V / 1024)) return false;
     const SccBlock sb(h);
@@ -1033,7 +1036,7 @@ bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* 
     const int64_t nd0 = h->n_dag;
     launch(h, "ir_classify", kIrClassify, std::min<unsigned>(gridFor(n, blk), 4096), blk, 0, ik, ni, dk, nd, B,
            (const int32_t*)h->scc.p, (const int64_t*)h->rank.p, S.ir_keys.p + nd0, S.ir_vals.p + nd0, S.ir_bsu.p,
-           S.ir_bsv.p, S.ir_dirty.p, cnt);
+           S.ir_bsv.p, S.ir_dirty.p, cnt, dbypass);
     int64_t c[IR_N];
     readCounters(h, cnt, IR_N, c);
     const int64_t ndelta = c[IR_DELTA], nback = c[IR_BACK], ndirty = c[IR_DIRTY];

@@ -273,7 +273,7 @@ def test_tuning_knobs_do_not_change_results():
                      ("persist", 1), ("chunk", 1), ("sssp_dist32", 0), ("xcd", 1), ("split", 0), ("scc_block", 256), ("scc_block", 1024),
                      ("sssp_delta", 100000), ("sssp_interleave", 0), ("sssp_interleave", 1),
                      ("sssp_ms_batch", 3), ("sssp_ms_batch", 32), ("scc_trim2", 1), ("sssp_unroll", 1),
-                     ("sssp_unroll", 2), ("bfs_alpha_auto", 0), ("bfs_alpha", 1000)]:
+                     ("sssp_unroll", 2), ("bfs_alpha_auto", 0), ("bfs_alpha", 1000), ("update_bypass", 0)]:
             say("knob", k, v)
             old = G.get_param(k)
             G.set_param(k, v)
@@ -456,6 +456,7 @@ def test_incremental_scc_repair_paths():
     fb, fa = g["fw_beg"], g["fw_adj"]
     V = len(fb) - 1
     with girhip.Graph(fb, fa) as G:
+        G.set_param("update_bypass", 0)  # the paths below, not the bypass test first
         G.wcc()
         s0, _, _ = G.scc_dag()
         r = G.ranks()
@@ -493,6 +494,49 @@ def test_incremental_scc_repair_paths():
             check_layers(G, fb, fa, bb, ba, f"repair n={n}", rebuild=False)
             assert np.array_equal(G.get_scc()[0], H.get_scc()[0])
             assert H.last_ms("batch_scc_path") == 4
+
+
+def test_delete_bypass():
+    # deletes whose tail still reaches its head (a path of at most four arcs)
+    # change no partition: the layers are repaired without relabelling; the
+    # others fall back to the general repair.  Every step against the oracle.
+    V, fb, fa, bb, ba = girhip.rmat_graph(12, edge_factor=8, seed=61)
+    rng = np.random.default_rng(62)
+    nby = 0
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.wcc()
+        G.scc_dag()
+        for step in range(24):
+            rows = np.repeat(np.arange(V, dtype=np.int32), np.diff(fb))
+            k = 1 if step < 12 else int(rng.integers(2, 9))
+            e = rng.choice(len(fa), k, replace=False)
+            u, v = rows[e], fa[e]
+            ins = np.zeros(k, np.uint8)
+            if step >= 18:  # mixed with inserts
+                u = np.concatenate([u, rng.integers(0, V, 5).astype(np.int32)])
+                v = np.concatenate([v, rng.integers(0, V, 5).astype(np.int32)])
+                ins = np.concatenate([ins, np.ones(5, np.uint8)])
+            say("bypass step", step, "deletes", k)
+            G.apply_batch(u, v, ins)
+            nby += int(G.last_ms("batch_all_bypassed"))
+            fb, fa, bb, ba = oracle.apply_batch(fb, fa, u, v, ins)
+            check_layers(G, fb, fa, bb, ba, f"bypass step {step}", rebuild=False)
+        assert nby > 0
+    # a 2-cycle and an SCC {2, 3, 4} with a chord: deleting the chord 2 -> 4
+    # keeps the path 2 -> 3 -> 4 (bypass); deleting 0 -> 1 splits the 2-cycle
+    # (no bypass); WCCs checked as well
+    fb2 = np.array([0, 1, 2, 4, 5, 6, 6], np.int32)  # 0->1, 1->0, 2->3, 2->4, 3->4, 4->2
+    fa2 = np.array([1, 0, 3, 4, 4, 2], np.int32)
+    with girhip.Graph(fb2, fa2) as G:
+        G.wcc()
+        G.scc_dag()
+        f1, a1 = fb2, fa2
+        for (u, v), want in (((2, 4), 1), ((0, 1), 0)):
+            say("bypass cycle delete", u, v)
+            G.apply_batch([u], [v], [0])
+            assert G.last_ms("batch_all_bypassed") == want
+            f1, a1, b1, c1 = oracle.apply_batch(f1, a1, [u], [v], [0])
+            check_layers(G, f1, a1, b1, c1, f"cycle delete {u}->{v}", rebuild=False)
 
 
 def test_ingest_then_sync():
