@@ -6,13 +6,17 @@ query-batch sizes k, the GPU times SSSP for a ladder of bucket widths delta
 and buildSCC_DAG for a ladder of trim-round counts; each timing is one
 record (features, knob, value, ms).  The MLP (16 -> 64 -> 64 -> 2, trained
 on the GPU by the library's own kernels) learns
-    y0 = log2(fastest delta)          for (graph, k)
-    y1 = log2(fastest trim rounds + 1) for the graph.
+    y0 = log2(fastest delta)                           for (graph, k)
+    y1 = index of the fastest SCC schedule in SCHEDULES for the graph
+where a schedule is (pivot rule, trim rounds): the remainder after trimming
+goes straight to colouring (pivot 0) or first through one forward-backward
+sweep from the vertex of largest out * in degree (pivot 1).
 At run time girhip_learned_tune applies it with the reference's defaults as
 fallback and judge.  Knobs only change work order; results never depend on
 them (tests/test_gpu_tuning.py checks that).
 
     python tuning.py collect OUT.npz [--graphs N] [--seed S]   (GPU)
+    python tuning.py merge OUT.npz RECORDS.npz [...]
     python tuning.py train RECORDS.npz [...] --out MODEL.npz  (GPU)
     python tuning.py evaluate [--model M] [--graphs rmat23,grid4800]  (GPU)
 """
@@ -30,7 +34,10 @@ sys.path.insert(0, HERE)
 import girhip  # noqa: E402
 
 DELTAS = [2.0 ** i for i in range(0, 17)]  # 1 .. 65536
-TRIMS = [0, 1, 2, 3, 4, 6, 8]
+# (pivot rule, trim rounds), the table girhip_learned_tune indexes
+# (csrc/tune.hip); the reference's default is (1, 3)
+SCHEDULES = [(0, 1), (0, 3), (1, 0), (1, 1), (1, 2), (1, 3), (1, 4), (1, 6), (1, 8)]
+DEFAULT_SCHEDULE = SCHEDULES.index((1, 3))
 KS = [1, 8, 32]
 MODEL_PATH = os.path.join(HERE, "tuned_mlp.npz")
 
@@ -64,7 +71,7 @@ def family_graph(rng):
         g = (V,) + girhip.csr_from_edges(V, s, dd)
         name = f"er{V}_d{deg:.1f}"
     wmax = int(rng.choice([0, 16, 64, 256, 1000, 10000]))  # 0: the reference's hash weights
-    return name, g, wmax
+    return name, g, wmax, kind
 
 
 def time_sssp(G, src, delta, reps=1):
@@ -83,7 +90,7 @@ def collect(n_graphs, seed, out_path, budget_s=None):
         if budget_s and time.time() - t0 > budget_s:
             log(f"time budget reached after {gi} graphs")
             break
-        name, (V, fb, fa, bb, ba), wmax = family_graph(rng)
+        name, (V, fb, fa, bb, ba), wmax, kind = family_graph(rng)
         with girhip.Graph(fb, fa, bb, ba) as G:
             if wmax:
                 G.set_weights(girhip.weights(len(fa), 1, wmax, int(rng.integers(1 << 30))))
@@ -104,13 +111,25 @@ def collect(n_graphs, seed, out_path, budget_s=None):
                     if (t > 8 * best and delta < 64) or (t > 3 * best and t > 1500):
                         break
             f = G.features(KS[0])
-            _, times = G.autotune("scc_trim_rounds", [], TRIMS, reps=1)
-            for T, t in zip(TRIMS, times):
+            # colouring alone takes one propagation per hop of the longest
+            # path inside an SCC: not tried on lattices or large graphs
+            colour_ok = kind != "grid" and V <= (1 << 20)
+            for si, (piv, trim) in enumerate(SCHEDULES):
+                if piv == 0 and not colour_ok:
+                    continue
+                G.set_param("scc_pivot", piv)
+                G.set_param("scc_trim_rounds", trim)
+                t = 1e300
+                for _ in range(2):
+                    G.scc_dag()
+                    t = min(t, G.last_ms("scc_dag"))
                 feats.append(f)
                 kinds.append(1)
-                vals.append(T)
+                vals.append(si)
                 ms.append(t)
                 gids.append(gi * 10)
+            G.set_param("scc_pivot", 1)
+            G.set_param("scc_trim_rounds", 3)
         names.append(name)
         log(f"graph {gi} {name} V={V} E={len(fa)} w={wmax or 'hash'} records={len(ms)} {time.time() - t0:.0f}s")
         # saved after every graph, so an interrupted run keeps what it measured
@@ -120,23 +139,39 @@ def collect(n_graphs, seed, out_path, budget_s=None):
     return len(ms)
 
 
-def robust_best(values, times, tol=0.05):
-    """Knob label of one timing ladder: the median (in log2) of the values
-    within tol of the fastest, so a flat optimum gives its centre rather than
-    whichever point timing noise favoured."""
+def merge_records(paths, out_path):
+    """Concatenate record files (graph ids made distinct per file)."""
+    cols = {k: [] for k in ("feat", "kind", "value", "ms", "gid", "names")}
+    off = 0
+    for p in paths:
+        with np.load(p, allow_pickle=False) as z:
+            for k in ("feat", "kind", "value", "ms"):
+                cols[k].append(z[k])
+            cols["gid"].append(z["gid"] + off)
+            cols["names"].append(z["names"])
+            off += 10 * (int(z["gid"].max()) // 10 + 1)
+    np.savez_compressed(out_path, **{k: np.concatenate(v) for k, v in cols.items()})
+    return sum(len(x) for x in cols["ms"])
+
+
+def robust_best(values, times, tol=0.05, log=True):
+    """Knob label of one timing ladder: the median (in log2 when log) of the
+    values within tol of the fastest, so a flat optimum gives its centre
+    rather than whichever point timing noise favoured."""
     t = np.asarray(times, np.float64)
     v = np.asarray(values, np.float64)
-    good = np.sort(np.log2(v[t <= t.min() * (1 + tol)]))
+    good = v[t <= t.min() * (1 + tol)]
+    good = np.sort(np.log2(good) if log else good)
     return float(good[(len(good) - 1) // 2])
 
 
 def dataset(records):
-    """Per (graph, k): features -> (log2 best delta, log2(best trim + 1))."""
+    """Per (graph, k): features -> (log2 best delta, best SCC schedule index)."""
     feat, kind, value, ms, gid = (records[k] for k in ("feat", "kind", "value", "ms", "gid"))
     best_trim = {}
     for g in np.unique(gid[kind == 1]):
         sel = (gid == g) & (kind == 1)
-        best_trim[g // 10] = robust_best(value[sel] + 1, ms[sel])
+        best_trim[g // 10] = robust_best(value[sel], ms[sel], log=False)
     X, Y = [], []
     for g in np.unique(gid[kind == 0]):
         sel = (gid == g) & (kind == 0)
@@ -212,6 +247,7 @@ def evaluate(model_path, which=("rmat23", "grid4800"), n_src=8, reps=2):
 
         G.set_param("sssp_delta", 16)
         G.set_param("scc_trim_rounds", 3)
+        G.set_param("scc_pivot", 1)
         sd, cd = run()
         ref_d, ref_s = dist.clone(), G.get_scc()[0].copy()
         res = G.learned_tune(model, norm, src, judge=0)  # the raw prediction
@@ -219,11 +255,12 @@ def evaluate(model_path, which=("rmat23", "grid4800"), n_src=8, reps=2):
         same = bool(torch.equal(ref_d, dist)) and bool((G.get_scc()[0] == ref_s).all())
         G.set_param("sssp_delta", 16)
         G.set_param("scc_trim_rounds", 3)
+        G.set_param("scc_pivot", 1)
         rj = G.learned_tune(model, norm, src, judge=n_src)  # judged by the default on the same sources
         sj, _ = run()
         same = same and bool(torch.equal(ref_d, dist))
         rec = {"graph": name, "V": V, "E": len(fa), "sources": n_src, "learned_delta": res["delta"],
-               "learned_trim": res["trim"], "model_used": res["used"], "sssp_ms_default": round(sd, 3),
+               "learned_trim": res["trim"], "learned_pivot": res["pivot"], "model_used": res["used"], "sssp_ms_default": round(sd, 3),
                "sssp_ms_learned": round(sl, 3), "sssp_gain": round(sd / sl, 3), "scc_dag_ms_default": round(cd, 3),
                "scc_dag_ms_learned": round(cl, 3), "scc_dag_gain": round(cd / cl, 3),
                "judged_delta": rj["delta"], "judged_kept_prediction": rj["used"], "sssp_ms_judged": round(sj, 3),
@@ -249,6 +286,9 @@ def main():
     t.add_argument("--out", default=MODEL_PATH)
     t.add_argument("--epochs", type=int, default=3000)
     t.add_argument("--lr", type=float, default=3e-3)
+    mg = sub.add_parser("merge")
+    mg.add_argument("out")
+    mg.add_argument("records", nargs="+")
     e = sub.add_parser("evaluate")
     e.add_argument("--model", default=MODEL_PATH)
     e.add_argument("--graphs", default="rmat23,grid4800")
@@ -260,6 +300,9 @@ def main():
         if args.out:
             with open(args.out, "w") as f:
                 json.dump(res, f, indent=1)
+        return
+    if args.cmd == "merge":
+        log(f"{merge_records(args.records, args.out)} records -> {args.out}")
         return
     if args.cmd == "collect":
         n = collect(args.graphs, args.seed, args.out, args.budget_s)
