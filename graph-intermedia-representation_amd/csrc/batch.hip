@@ -19,10 +19,12 @@
 //
 // Repair: the reference repairs WCC/SCC/DAG/ranks incrementally (SPU/FMU);
 // its post-condition is that every layer equals a from-scratch build on the
-// new edge set (gir_test.cpp:470-563 checks exactly that).  Here an
-// insert-only batch repairs WCC by hooking the new arcs into the existing
-// min-rooted forest (labels stay canonical); any other change, and SCC + DAG
-// + ranks, are rebuilt on the device from the new CSR.
+// new edge set (gir_test.cpp:470-563 checks exactly that).  Here SCC + DAG +
+// ranks are repaired from the classified changes (repairScc, scc.hip) or
+// rebuilt; then WCC: when no delete can split a WCC (its ends share an SCC of
+// the new graph, or their SCCs are still joined by a condensation arc, or the
+// delete was shown to be bypassed) the inserted arcs are hooked into the
+// existing min-rooted forest (labels stay canonical), else WCC is rebuilt.
 #include <vector>
 
 #include "common.h"
@@ -277,6 +279,45 @@ void mergeCsr(girhip_graph* h, DevBuf<int32_t>& beg, DevBuf<int32_t>& adj, DevBu
     if (w) w->swap(*w2);
 }
 
+// Deletes that cannot split a WCC, judged on the repaired SCC layer: the
+// ends are in one SCC of the new graph, or the condensation still has an arc
+// between their SCCs (either way round).  A delete that leaves one end with
+// no arc at all while the other end is in the big SCC (label g) only cuts
+// off that end (every arc it lost led into one strongly connected set): the
+// end becomes its own WCC here, unless it was its WCC's label.  cnt += the
+// deletes that fit neither case.
+__global__ void kWccSafe(const uint64_t* __restrict__ dk, int64_t nd, int B, const int32_t* __restrict__ scc,
+                         const int32_t* __restrict__ dbeg, const int32_t* __restrict__ ddst,
+                         const int32_t* __restrict__ fb, const int32_t* __restrict__ bb, int32_t g, int32_t* wcc,
+                         int64_t* cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (i < nd) {
+        const uint64_t k = dk[i];
+        const int32_t u = (int32_t)(k >> B), v = (int32_t)(k & ((1ULL << B) - 1));
+        const int32_t su = scc[u], sv = scc[v];
+        if (su != sv) {
+            auto has = [&](int32_t a, int32_t b) {
+                int32_t lo = dbeg[a], hi = dbeg[a + 1];
+                while (lo < hi) {
+                    const int32_t m = lo + ((hi - lo) >> 1);
+                    if (ddst[m] < b) lo = m + 1;
+                    else hi = m;
+                }
+                return lo < dbeg[a + 1] && ddst[lo] == b;
+            };
+            if (!has(su, sv) && !has(sv, su)) {
+                auto alone = [&](int32_t x) { return fb[x + 1] == fb[x] && bb[x + 1] == bb[x]; };
+                if (g >= 0 && sv == g && alone(u) && wcc[u] != u) wcc[u] = u;
+                else if (g >= 0 && su == g && alone(v) && wcc[v] != v) wcc[v] = v;
+                else bad = true;
+            }
+        }
+    }
+    const uint64_t m = __ballot(bad);
+    if (laneId() == 0 && m) atomicAdd((unsigned long long*)cnt, (unsigned long long)__popcll(m));
+}
+
 // Normalise + classify the batch; fills C (fw order) and the stats.
 void normalizeBatchDev(girhip_graph* h, const int32_t* u, const int32_t* v, const uint8_t* ins, const uint32_t* w,
                        int64_t n, Changes& C, girhip_batch_stats& st, int B) {
@@ -400,14 +441,6 @@ void repairLayers(girhip_graph* h, const Changes* C, int B) {
         bypass = S.bp_found.p;
     }
     h->last_ms["batch_all_bypassed"] = all_bypassed ? 1 : 0;
-    if (h->wcc_req) {
-        if (h->wcc_ok && C && (C->nd == 0 || all_bypassed)) {
-            if (C->ni) runWccHookArcs(h, C->ik, C->ni, B);  // insert-only: hook the new arcs
-        } else if (!h->wcc_ok || changed) {
-            runWcc(h);
-        }
-        h->wcc_ok = true;
-    }
     if (h->scc_req) {
         if (h->scc_ok && C && changed && repairScc(h, C->ik, C->ni, C->dk, C->nd, B, bypass)) {
             // repaired from the classified changes
@@ -417,6 +450,28 @@ void repairLayers(girhip_graph* h, const Changes* C, int B) {
             h->last_ms["batch_scc_path"] = 4;
         }
         h->scc_ok = true;
+    }
+    if (h->wcc_req) {
+        // deletes that keep every WCC together: judged on the repaired SCCs
+        bool keep = C && (C->nd == 0 || all_bypassed);
+        int64_t unsafe = -1;
+        if (!keep && h->wcc_ok && C && C->nd > 0 && h->scc_req && h->scc_ok) {
+            int64_t* c = h->dcnt.p + 52;
+            fillI64(h, c, 0, 1);
+            launch(h, "wcc_safe", kWccSafe, gridFor(C->nd, 256), 256, 0, (const uint64_t*)C->dk, C->nd, B,
+                   (const int32_t*)h->scc.p, (const int32_t*)h->dag_beg.p, (const int32_t*)h->dag_dst.p,
+                   (const int32_t*)h->fw_beg.p, (const int32_t*)h->bw_beg.p, h->gc_ok ? h->gc_label : -1, h->wcc.p,
+                   c);
+            readCounters(h, c, 1, &unsafe);
+            keep = unsafe == 0;
+        }
+        h->last_ms["batch_wcc_unsafe"] = (double)unsafe;
+        if (h->wcc_ok && keep) {
+            if (C->ni) runWccHookArcs(h, C->ik, C->ni, B);  // hook the new arcs
+        } else if (!h->wcc_ok || changed) {
+            runWcc(h);
+        }
+        h->wcc_ok = true;
     }
 }
 

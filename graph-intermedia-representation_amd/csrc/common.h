@@ -102,6 +102,7 @@ struct Params {
     double sssp_unroll = 2;      // arcs per lane at a time in the SSSP relaxation passes (1, 2 or 4)
     double bfs_alpha_auto = 1;   // 1: the per-source reach raises bfs_alpha to 64 on graphs with hubs
     double update_bypass = 1;    // 1: batches of few deletes test them for a bypass path first
+    double scc_cert = 1;         // 1: update batches check the big SCC against its spanning trees
 };
 
 struct KStat {
@@ -118,6 +119,10 @@ struct Scratch {
     DevBuf<uint64_t> keys[2];
     DevBuf<uint32_t> vals[2];
     DevBuf<int64_t> off, off2;
+    // radix sort: published tile words, tickets + digit counts, error word
+    DevBuf<uint64_t> rs_look, rs_aux;
+    DevBuf<int64_t> rs_err;
+    uint32_t rs_epoch = 0;
     DevBuf<uint8_t> stage;
     DevBuf<uint64_t> ms[3];  // bit-parallel BFS: seen / frontier / next words
     DevBuf<int64_t> mscnt;   // bit-parallel BFS: per-level counter ring
@@ -162,6 +167,20 @@ struct Scratch {
     DevBuf<uint32_t> ir_vals;
     DevBuf<int32_t> ir_bsu, ir_bsv;
     DevBuf<uint8_t> ir_seed;  // distinct SCC ids of a sweep's seeds
+    // labels before a relabelling, the vertices whose label changed, counters
+    DevBuf<int32_t> ir_old, ir_list;
+    DevBuf<int64_t> ir_cnt2;
+    // condensation merge: run flags / sums, unique keys and sums, new
+    // multiplicities, matched arc, new / removed flags and positions, the new
+    // and removed keys (forward and transposed, padded), counters
+    DevBuf<int32_t> dm_flag, dm_sum, dm_us, dm_nm, dm_nf, dm_rf, dm_newm;
+    DevBuf<int64_t> dm_pos, dm_fi, dm_pnew, dm_prem, dm_cnt;
+    DevBuf<uint64_t> dm_uk, dm_newk, dm_remk, dm_newt, dm_remt;
+    // big-SCC certificate check: orphaned / still-attached members (out-tree
+    // side, in-tree side), the heads / tails of deleted tree arcs
+    DevBuf<uint64_t> gc_bits[4];
+    DevBuf<int32_t> gc_cut[2];
+    DevBuf<int64_t> gc_cnt;
     // reachability query sweeps (a group of sweeps, V words each)
     DevBuf<uint64_t> rq_seen, rq_next, rq_tgt, rq_fb[2];
     DevBuf<int32_t> rq_fq[2], rq_tq, rq_pairs;
@@ -256,6 +275,10 @@ struct girhip_graph {
     gir::DevBuf<int32_t> dag_src, dag_dst, dag_mult;
     gir::DevBuf<int32_t> dag_beg;  // DAG CSR offsets by source SCC id (V+1)
     gir::DevBuf<int32_t> dag_in_beg, dag_in_src;  // DAG in-arcs by destination SCC id (V+1, n_dag)
+    gir::DevBuf<uint64_t> dag_tk;  // in-arc keys (dst << B | src), sorted (n_dag)
+    // spare arrays a merge of multiplicity changes writes into, then swaps
+    gir::DevBuf<int32_t> dag_src2, dag_dst2, dag_mult2;
+    gir::DevBuf<uint64_t> dag_tk2;
     gir::DevBuf<int32_t> mem_beg, mem_v;  // members by SCC id (V+1, V), built on demand
     bool mem_ok = false;
     // incremental single-source queries, [0] BFS levels, [1] SSSP distances:
@@ -270,9 +293,18 @@ struct girhip_graph {
     } inc[2];
     int64_t n_dag = 0;
     gir::DevBuf<int64_t> rank;
+    // certificate of the big SCC (label gc_label): an out-tree (gc_fpar[x] ->
+    // x is an arc) and an in-tree (x -> gc_bpar[x] is an arc) of its members
+    // rooted at gc_pivot, kept across update batches; built by the first
+    // batch that needs it after a full SCC build
+    gir::DevBuf<int32_t> gc_fpar, gc_bpar;
+    bool gc_ok = false;
+    int32_t gc_pivot = -1, gc_label = -1;
 
-    // host-pinned counters for small readbacks (HCNT_SLOTS entries)
+    // host-pinned counters for small readbacks (HCNT_SLOTS entries; the last
+    // one receives the radix sort's error word)
     int64_t* hcnt = nullptr;
+    bool rs_check = false;  // a radix sort ran since the last stream synchronisation
     gir::DevBuf<int64_t> dcnt;  // device counters (64 slots)
     gir::DevBuf<uint32_t> gbar;  // grid barrier of the persistent kernels (gir::BAR_WORDS)
 
@@ -463,6 +495,16 @@ void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks);
 // reachability (their tail still reaches their head).
 bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* dk, int64_t nd, int B,
                const uint8_t* dbypass);
+// Certificate of the big SCC (cert.hip).  certTarget picks the pivot (largest
+// outdeg * indeg) and the label of its SCC and marks the trees absent;
+// certRepair checks the big SCC after a batch (ncutf / ncutb: heads / tails
+// of deleted tree arcs in gc_cut[0] / [1]; ncand: vertices labelled -1 that
+// may join it) and rebuilds the trees when they are absent.
+struct CertStats {
+    int64_t dropped = 0, joined = 0, orphans = 0, swept = 0;
+};
+void certTarget(girhip_graph* h);
+CertStats certRepair(girhip_graph* h, int64_t ncutf, int64_t ncutb, int64_t ncand);
 // Batches with at most BYPASS_MAX deletes test them for a bypass path.
 constexpr int64_t BYPASS_MAX = 8;
 bool deletesBypassed(girhip_graph* h, const uint64_t* dk, int64_t nd, int B, uint8_t* found);

@@ -47,8 +47,20 @@ void flushKernelStats(girhip_graph* h) {
 }
 
 void syncStream(girhip_graph* h) {
+    const bool chk = h->rs_check && h->ws.rs_err.p;
+    if (chk)
+        HIPCHK(hipMemcpyAsync(h->hcnt + HCNT_SLOTS - 1, h->ws.rs_err.p, sizeof(int64_t), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     if (!h->pend_ev.empty()) flushKernelStats(h);
+    if (chk) {
+        h->rs_check = false;
+        if (h->hcnt[HCNT_SLOTS - 1] != 0) {
+            h->hcnt[HCNT_SLOTS - 1] = 0;
+            fillI64(h, h->ws.rs_err.p, 0, 1);
+            HIPCHK(hipStreamSynchronize(h->st));
+            raise(GIRHIP_EHIP, "radix sort: a tile waited too long for the tiles before it");
+        }
+    }
 }
 
 void tBegin(girhip_graph* h) { HIPCHK(hipEventRecord(h->t0, h->st)); }
@@ -100,7 +112,7 @@ HeavyList heavyList(girhip_graph* h) {
 }
 
 void readCounters(girhip_graph* h, const int64_t* dev, int n, int64_t* out) {
-    if (n > HCNT_SLOTS) raise(GIRHIP_EINVAL, "readCounters: too many slots");
+    if (n > HCNT_SLOTS - 1) raise(GIRHIP_EINVAL, "readCounters: too many slots");
     HIPCHK(hipMemcpyAsync(h->hcnt, dev, sizeof(int64_t) * n, hipMemcpyDeviceToHost, h->st));
     syncStream(h);
     std::memcpy(out, h->hcnt, sizeof(int64_t) * n);
@@ -598,7 +610,8 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"scc_block", &Params::scc_block}, {"sssp_interleave", &Params::sssp_interleave},
                {"sssp_ms_batch", &Params::sssp_ms_batch}, {"scc_trim2", &Params::scc_trim2},
                {"scc_incremental", &Params::scc_incremental}, {"sssp_unroll", &Params::sssp_unroll},
-               {"bfs_alpha_auto", &Params::bfs_alpha_auto}, {"update_bypass", &Params::update_bypass}};
+               {"bfs_alpha_auto", &Params::bfs_alpha_auto}, {"update_bypass", &Params::update_bypass},
+               {"scc_cert", &Params::scc_cert}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;

@@ -169,41 +169,102 @@ void scanI32toI64(girhip_graph* h, const int32_t* in, int64_t* out, int64_t n) {
 void scanI32toI32(girhip_graph* h, const int32_t* in, int32_t* out, int64_t n) { scanT(h, in, out, n); }
 
 // ---------------------------------------------------------------------------
-// LSD radix sort, 8-bit digits.  Tile = 256 threads x 16 rounds.
-constexpr int RS_T = 256, RS_R = 16, RS_TILE = RS_T * RS_R, RS_D = 256;
+// LSD radix sort, 8-bit digits, one launch per pass.  Before the passes one
+// kernel counts every pass's digits over the whole array (the global digit
+// starts).  A pass kernel takes a tile of RS_TILE keys in the order tiles
+// are handed out (a ticket counter, so every lower tile is already running),
+// counts its digits, publishes the counts, and reads the counts of the tiles
+// before it back to the first one that has published its running total
+// (each tile publishes that total as soon as it knows it), which gives the
+// tile's offset per digit; it then scatters its keys stably.  A published
+// word is (epoch << 48) | (state << 46) | count, state 1 = this tile's
+// count, 2 = running total through this tile; words of other passes carry
+// other epochs and read as not yet published.  A wait longer than RS_SPIN
+// polls gives up, sets the sort's error word and lets the kernel finish;
+// the host raises the error at its next stream synchronisation.
+constexpr int RS_T = 256, RS_R = 16, RS_TILE = RS_T * RS_R, RS_D = 256, RS_MAXP = 8;
+constexpr unsigned RS_SPIN = 1u << 22;
+constexpr uint64_t RS_CNT = (1ULL << 46) - 1;
 
-__global__ void kRadixHist(const uint64_t* __restrict__ keys, int64_t n, int shift, int32_t* __restrict__ hist,
-                           int nb) {
-    __shared__ int32_t c[RS_D];
-    c[threadIdx.x] = 0;
+__global__ void __launch_bounds__(RS_T) kRadixCount(const uint64_t* __restrict__ keys, int64_t n, int passes,
+                                                    unsigned long long* __restrict__ ghist) {
+    __shared__ uint32_t c[RS_MAXP][RS_D];
+    for (int p = 0; p < passes; ++p) c[p][threadIdx.x] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_R; ++r) {
-        int64_t idx = base + (int64_t)r * RS_T + threadIdx.x;
-        if (idx < n) atomicAdd(&c[(keys[idx] >> shift) & 255], 1);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        for (int p = 0; p < passes; ++p) atomicAdd(&c[p][(k >> (8 * p)) & 255], 1u);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * nb + blockIdx.x] = c[threadIdx.x];
+    for (int p = 0; p < passes; ++p)
+        if (c[p][threadIdx.x]) atomicAdd(&ghist[p * RS_D + threadIdx.x], (unsigned long long)c[p][threadIdx.x]);
 }
 
-// Stable scatter.  For each round of 256 consecutive keys, lanes of a wave
-// holding the same digit are matched with 8 ballots; ranks across the 4
-// waves come from per-wave digit counts in LDS; base[d] carries the block's
-// running offset per digit (from the scanned digit-major histogram).
-__global__ void kRadixScatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                              uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
-                              const int64_t* __restrict__ hscan, int nb) {
+__global__ void __launch_bounds__(RS_T) kRadixPass(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                   uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                   int64_t n, int shift, const unsigned long long* __restrict__ ghist,
+                                                   uint64_t* look, uint32_t epoch, unsigned long long* tick,
+                                                   int64_t* err) {
     __shared__ int64_t base[RS_D];
     __shared__ int32_t wc[RS_T / WAVE][RS_D];
+    __shared__ int32_t cnt[RS_D];
+    __shared__ int64_t tile_s;
     const int t = threadIdx.x, lane = laneId(), w = t / WAVE;
-    base[t] = hscan[(int64_t)t * nb + blockIdx.x];
-    const int64_t tile = (int64_t)blockIdx.x * RS_TILE;
+    if (t == 0) tile_s = (int64_t)atomicAdd(tick, 1ULL);
+    cnt[t] = 0;
+    __syncthreads();
+    const int64_t tile = tile_s;
+    const int64_t t0 = tile * RS_TILE;
+    uint64_t key[RS_R];
+    uint32_t val[RS_R];
+#pragma unroll
     for (int r = 0; r < RS_R; ++r) {
-        const int64_t idx = tile + (int64_t)r * RS_T + t;
+        const int64_t idx = t0 + (int64_t)r * RS_T + t;
+        key[r] = idx < n ? kin[idx] : 0;
+        val[r] = (idx < n && vin) ? vin[idx] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_R; ++r)
+        if (t0 + (int64_t)r * RS_T + t < n) atomicAdd(&cnt[(key[r] >> shift) & 255], 1);
+    __syncthreads();
+    // this tile's offset for digit t: the counts of the tiles before it
+    const uint64_t ep = (uint64_t)epoch << 48;
+    const int64_t mine = cnt[t];
+    int64_t ex = 0;
+    uint64_t* slot = look + tile * RS_D + t;
+    if (tile == 0) {
+        stRel<uint64_t>(slot, ep | (2ULL << 46) | (uint64_t)mine);
+    } else {
+        stRel<uint64_t>(slot, ep | (1ULL << 46) | (uint64_t)mine);
+        int64_t j = tile - 1;
+        unsigned spins = 0;
+        while (j >= 0) {
+            const uint64_t x = ldAcq(look + j * RS_D + t);
+            if ((uint32_t)(x >> 48) != epoch) {
+                if (++spins >= RS_SPIN) {
+                    stRel<int64_t>(err, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            ex += (int64_t)(x & RS_CNT);
+            if (((x >> 46) & 3) == 2) break;
+            --j;
+        }
+        stRel<uint64_t>(slot, ep | (2ULL << 46) | (uint64_t)(ex + mine));
+    }
+    // global start of digit t (exclusive scan of the pass's digit counts)
+    int64_t total;
+    const int64_t g = (int64_t)ghist[t];
+    const int64_t start = blockInclusiveScan(g, &total) - g;
+    base[t] = start + ex;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_R; ++r) {
+        const int64_t idx = t0 + (int64_t)r * RS_T + t;
         const bool valid = idx < n;
-        uint64_t key = valid ? kin[idx] : 0;
-        uint32_t val = (valid && vin) ? vin[idx] : 0;
-        const int d = (int)((key >> shift) & 255);
+        const int d = (int)((key[r] >> shift) & 255);
         for (int k = 0; k < RS_T / WAVE; ++k) wc[k][t] = 0;
         __syncthreads();
         uint64_t peers = __ballot(valid);
@@ -218,8 +279,8 @@ __global__ void kRadixScatter(const uint64_t* __restrict__ kin, const uint32_t* 
         if (valid) {
             int64_t off = base[d] + rank;
             for (int k = 0; k < w; ++k) off += wc[k][d];
-            kout[off] = key;
-            if (vout) vout[off] = val;
+            kout[off] = key[r];
+            if (vout) vout[off] = val[r];
         }
         __syncthreads();
         int64_t add = 0;
@@ -285,22 +346,35 @@ void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int k
         return;
     }
     Scratch& S = scratch(h);
-    const int nb = (int)((n + RS_TILE - 1) / RS_TILE);
+    const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    const int passes = std::min(RS_MAXP, std::max(1, (key_bits + 7) / 8));
     S.keys[1].ensure(n);
     if (vals) S.vals[1].ensure(n);
-    DevBuf<int32_t>& hist = S.iv[3];
-    hist.ensure((size_t)RS_D * nb);
-    DevBuf<int64_t>& hscan = S.off2;  // (the scan helper itself uses S.off)
-    hscan.ensure((size_t)RS_D * nb + 1);
+    // published words: one per (tile, digit); a new buffer starts cleared
+    if (S.rs_look.n < (size_t)(nb * RS_D)) {
+        S.rs_look.ensure((size_t)(nb * RS_D));
+        fillU64(h, S.rs_look.p, 0, nb * RS_D);
+    }
+    if (!S.rs_err.p) {
+        S.rs_err.ensure(1);
+        fillI64(h, S.rs_err.p, 0, 1);
+    }
+    // [0, RS_MAXP): tickets per pass; then the digit counts per pass
+    S.rs_aux.ensure((size_t)(RS_MAXP + RS_MAXP * RS_D));
+    uint64_t* aux = S.rs_aux.p;
+    fillU64(h, aux, 0, RS_MAXP + (int64_t)passes * RS_D);
+    launch(h, "radix_count", kRadixCount, (unsigned)std::min<int64_t>(nb, 1024), RS_T, 0, (const uint64_t*)keys, n,
+           passes, (unsigned long long*)(aux + RS_MAXP));
     uint64_t *ka = keys, *kb = S.keys[1].p;
     uint32_t *va = vals, *vb = vals ? S.vals[1].p : nullptr;
-    const int passes = (key_bits + 7) / 8;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        launch(h, "radix_hist", kRadixHist, (unsigned)nb, RS_T, 0, (const uint64_t*)ka, n, shift, hist.p, nb);
-        scanI32toI64(h, hist.p, hscan.p, (int64_t)RS_D * nb);
-        launch(h, "radix_scatter", kRadixScatter, (unsigned)nb, RS_T, 0, (const uint64_t*)ka, (const uint32_t*)va, kb,
-               vb, n, shift, (const int64_t*)hscan.p, nb);
+        if (++S.rs_epoch > 0xffffu) {  // epochs wrap: clear the words once
+            fillU64(h, S.rs_look.p, 0, (int64_t)S.rs_look.n);
+            S.rs_epoch = 1;
+        }
+        launch(h, "radix_pass", kRadixPass, (unsigned)nb, RS_T, 0, (const uint64_t*)ka, (const uint32_t*)va, kb, vb, n,
+               8 * p, (const unsigned long long*)(aux + RS_MAXP + p * RS_D), S.rs_look.p, S.rs_epoch,
+               (unsigned long long*)(aux + p), S.rs_err.p);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
@@ -308,6 +382,7 @@ void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int k
         HIPCHK(hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, h->st));
         if (vals) HIPCHK(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, h->st));
     }
+    h->rs_check = true;
 }
 
 }  // namespace gir

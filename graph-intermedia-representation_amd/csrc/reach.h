@@ -28,14 +28,15 @@ struct ReachBufs {
     int64_t W;  // bitmap words
 };
 
-// Seed: mark the initial frontier visited and call on(v, 0).
+// Seed: mark the initial frontier visited and call on(v, 0, -1) (a seed has
+// no parent).
 template <class Vis>
 __global__ void kReachSeed(const int32_t* __restrict__ q, int64_t n, uint64_t* vis, Vis on) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t v = q[i];
     atomicOr((unsigned long long*)&vis[v >> 6], 1ULL << (v & 63));
-    on(v, 0);
+    on(v, 0, -1);
 }
 
 // cnt[0] = next queue length, cnt[1] = sum of push-direction degrees of the
@@ -64,7 +65,7 @@ struct TopDownF {
         const int64_t pos = wavePush(&cnt[0], claim);
         if (claim) {
             qout[pos] = w;
-            on(w, L + 1);
+            on(w, L + 1, p);
             return beg[w + 1] - beg[w];
         }
         return 0;
@@ -81,7 +82,7 @@ struct TopDownF {
         push = claim;
         item = w;
         if (!claim) return 0;
-        on(w, L + 1);
+        on(w, L + 1, p);
         return beg[w + 1] - beg[w];
     }
     __device__ int64_t* pushCounter() const { return &cnt[0]; }
@@ -122,6 +123,7 @@ __global__ void kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __res
     const uint64_t w0 = word_ok ? vis[word] : ~0ULL;
     const bool act = v < V && !((w0 >> laneId()) & 1) && flt.self((int32_t)v);
     bool found = false;
+    int32_t par = -1;  // the frontier vertex found (kept only for visitors that record parents)
     // each lane scans the first BU_SMALL slots of its own row; rows with
     // slots left and no parent yet are then scanned by the whole wave, one
     // row at a time, 64 coalesced slots per step with a ballot exit (a hub's
@@ -132,6 +134,7 @@ __global__ void kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __res
         const int32_t p = radj[j];
         if (((fbits[p >> 6] >> (p & 63)) & 1) && flt((int32_t)v, p)) {
             found = true;
+            if constexpr (Vis::kParent) par = p;
             break;
         }
     }
@@ -141,19 +144,26 @@ __global__ void kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __res
         rest &= rest - 1;
         const int32_t rv = __shfl((int32_t)v, l), rb = __shfl(lim, l), re = __shfl(e, l);
         bool hit = false;
+        int32_t hp = -1;
         for (int32_t j0 = rb; j0 < re; j0 += WAVE) {
             const int32_t j = j0 + laneId();
             bool hj = false;
+            int32_t pj = -1;
             if (j < re) {
-                const int32_t p = radj[j];
-                hj = ((fbits[p >> 6] >> (p & 63)) & 1) && flt(rv, p);
+                pj = radj[j];
+                hj = ((fbits[pj >> 6] >> (pj & 63)) & 1) && flt(rv, pj);
             }
-            if (__ballot(hj)) {
+            const uint64_t hb = __ballot(hj);
+            if (hb) {
                 hit = true;
+                if constexpr (Vis::kParent) hp = __shfl(pj, __ffsll((long long)hb) - 1);
                 break;
             }
         }
-        if (laneId() == l && hit) found = true;
+        if (laneId() == l && hit) {
+            found = true;
+            par = hp;
+        }
     }
     const uint64_t nb = __ballot(found);
     if (laneId() == 0 && word_ok) {
@@ -166,7 +176,7 @@ __global__ void kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __res
     const int64_t pos = blockReservePacked(found, m, &cnt[0]);
     if (found) {
         qout[pos] = (int32_t)v;
-        on((int32_t)v, L + 1);
+        on((int32_t)v, L + 1, par);
     }
 }
 
@@ -175,17 +185,30 @@ static __global__ void kQueueToBits(const int32_t* __restrict__ q, int64_t n, ui
     if (i < n) atomicOr((unsigned long long*)&bits[q[i] >> 6], 1ULL << (q[i] & 63));
 }
 
-// Arc filters and visit hooks.
+// Arc filters and visit hooks.  A hook is called as on(v, level, parent)
+// when v is claimed (parent = the frontier vertex whose arc reached v; -1 for
+// a seed, and for bottom-up claims when the hook's kParent is false).
 struct AllArcs {
     __device__ bool operator()(int32_t, int32_t) const { return true; }
     __device__ bool self(int32_t) const { return true; }
 };
 struct NoVisit {
-    __device__ void operator()(int32_t, int32_t) const {}
+    static constexpr bool kParent = false;
+    __device__ void operator()(int32_t, int32_t, int32_t) const {}
 };
 struct WriteLevel {
+    static constexpr bool kParent = false;
     int32_t* level;
-    __device__ void operator()(int32_t v, int32_t L) const { level[v] = L; }
+    __device__ void operator()(int32_t v, int32_t L, int32_t) const { level[v] = L; }
+};
+// Records the claiming parent (seeds keep theirs): a BFS tree over the arcs
+// the sweep followed.
+struct WriteParent {
+    static constexpr bool kParent = true;
+    int32_t* par;
+    __device__ void operator()(int32_t v, int32_t, int32_t p) const {
+        if (p >= 0) par[v] = p;
+    }
 };
 
 struct ReachStats {
@@ -374,8 +397,8 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
             const TopDownF<Flt, Vis> f{pbeg, padj, B.vis, L + t, B.q[cur ^ 1], pair(L + t), flt, on};
             launch(h, "reach_topdown", kTopDown<Flt, Vis>, g, blk, 0, (const int32_t*)B.q[cur],
                    (const int64_t*)pair(L + t - 1), f, pair(L + t + 1), HL);
-            if (any_row_heavy)
-                launch(h, "reach_topdown_heavy", kHeavyRows<TopDownF<Flt, Vis>>, 1024, 256, 0, HL, f, pair(L + t) + 1);
+            if (any_row_heavy)  // (a quarter of the store's grid: the pass is launched at every level)
+                launch(h, "reach_topdown_heavy", kHeavyRows<TopDownF<Flt, Vis>>, 256, 256, 0, HL, f, pair(L + t) + 1);
             else
                 launchHeavyStoreH(h, HL, "reach_topdown_heavy", f, pair(L + t) + 1);
             cur ^= 1;

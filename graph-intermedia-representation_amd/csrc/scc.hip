@@ -387,15 +387,39 @@ __global__ void kDagTKeys(const int32_t* __restrict__ src, const int32_t* __rest
 // CSR offsets from the sorted row ids of nd arcs (id(i) = ids[i], or
 // keys[i] >> B): thread i in [0, nd] writes beg[v] = i for every v in
 // (id(i - 1), id(i)] (id(nd) = V), i.e. the first arc of every row - one
-// pass over the arcs instead of a binary search per row.
+// pass over the arcs instead of a binary search per row.  A run of more than
+// BEG_RUN empty rows is not written by its one thread: it goes on a list
+// (gaps[0] = count, then (first, last, i) triples) that kBegGaps fills with
+// a block per run, so a sparse id range costs no serial tail.  (The list
+// slots are reserved once per block: a condensation's sparse source ids put
+// a long run in most waves.)
+constexpr int BEG_RUN = 32;
 __global__ void kBegScatter(const int32_t* __restrict__ ids, const uint64_t* __restrict__ keys, int B, int64_t nd,
-                            int32_t V, int32_t* __restrict__ beg) {
+                            int32_t V, int32_t* __restrict__ beg, int64_t* gaps) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > nd) return;
     auto id = [&](int64_t k) -> int64_t { return ids ? (int64_t)ids[k] : (int64_t)(keys[k] >> B); };
-    const int64_t cur = i < nd ? id(i) : (int64_t)V;
-    const int64_t prev = i > 0 ? id(i - 1) : -1;
-    for (int64_t v = prev + 1; v <= cur; ++v) beg[v] = (int32_t)i;
+    int64_t cur = 0, prev = 0;
+    if (i <= nd) {
+        cur = i < nd ? id(i) : (int64_t)V;
+        prev = i > 0 ? id(i - 1) : -1;
+    }
+    const bool longrun = i <= nd && cur - prev > BEG_RUN;
+    const int64_t pos = blockReserve(longrun ? 1 : 0, gaps);  // (one atomic per block)
+    if (longrun) {
+        gaps[1 + 3 * pos] = prev + 1;
+        gaps[2 + 3 * pos] = cur;
+        gaps[3 + 3 * pos] = i;
+    } else if (i <= nd) {
+        for (int64_t v = prev + 1; v <= cur; ++v) beg[v] = (int32_t)i;
+    }
+}
+__global__ void kBegGaps(const int64_t* __restrict__ gaps, int32_t* __restrict__ beg) {
+    const int64_t n = gaps[0];
+    for (int64_t g = blockIdx.x; g < n; g += gridDim.x) {
+        const int64_t a = gaps[1 + 3 * g], b = gaps[2 + 3 * g];
+        const int32_t val = (int32_t)gaps[3 + 3 * g];
+        for (int64_t v = a + threadIdx.x; v <= b; v += blockDim.x) beg[v] = val;
+    }
 }
 
 // indeg[v] = beg[v + 1] - beg[v]; with keys, also src[i] = the low B bits
@@ -627,6 +651,7 @@ struct SccBlock {
 
 void runScc(girhip_graph* h) {
     const int32_t V = h->V;
+    h->gc_ok = false;  // the big SCC's trees are rebuilt by the next batch that needs them
     h->scc.ensure((size_t)(V > 0 ? V : 1));
     if (V == 0) {
         h->n_scc = 0;
@@ -774,10 +799,41 @@ void runDag(girhip_graph* h) {
     dagFinish(h, nd, B, true);
 }
 
+// CSR offsets (V + 1) of nd arcs sorted by row id (ids, or keys >> B).
+static void begFromIds(girhip_graph* h, const char* name, const int32_t* ids, const uint64_t* keys, int B, int64_t nd,
+                       int32_t* beg) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    // at most one long run per arc boundary, and at most V / BEG_RUN of them
+    const int64_t cap = std::min<int64_t>(nd + 1, (int64_t)V / BEG_RUN + 1);
+    S.off.ensure((size_t)(1 + 3 * cap));
+    fillI64(h, S.off.p, 0, 1);
+    launch(h, name, kBegScatter, gridFor(nd + 1, blk), blk, 0, ids, keys, B, nd, V, beg, S.off.p);
+    launch(h, "beg_gaps", kBegGaps, 512, 256, 0, (const int64_t*)S.off.p, beg);
+}
+
 // With the arcs (dag_src / dst / mult, nd of them, sorted by (src, dst))
 // written: the in-arc CSR, the out-arc offsets and, with ranks, the Kahn
 // levels.
+static void dagIndex(girhip_graph* h, int64_t nd, int B, bool ranks);
+
 void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks) {
+    const int blk = (int)h->prm.block;
+    if (nd > 0) {
+        // the arcs sorted by destination: the in-arc keys
+        h->dag_tk.ensureSlack((size_t)nd);
+        launch(h, "dag_tkeys", kDagTKeys, gridFor(nd, blk), blk, 0, (const int32_t*)h->dag_src.p,
+               (const int32_t*)h->dag_dst.p, nd, B, h->dag_tk.p);
+        radixSort(h, h->dag_tk.p, nullptr, nd, 2 * B);
+    }
+    dagIndex(h, nd, B, ranks);
+}
+
+// With the arcs (sorted by (src, dst)) and the in-arc keys (sorted by (dst,
+// src)) in place: the in-arc CSR, the in-degrees as its row lengths (no
+// contended atomics), the out-arc offsets and, with ranks, the Kahn levels.
+static void dagIndex(girhip_graph* h, int64_t nd, int B, bool ranks) {
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
     Scratch& S = scratch(h);
@@ -786,18 +842,11 @@ void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks) {
     S.iv[2].ensure((size_t)V + 1);
     int32_t* ideg = S.iv[2].p;
     if (nd > 0) {
-        S.keys[0].ensure((size_t)nd);
-        // the arcs sorted by destination: the in-arc CSR of the DAG, and the
-        // in-degrees as its row lengths (no contended atomics)
-        launch(h, "dag_tkeys", kDagTKeys, gridFor(nd, blk), blk, 0, (const int32_t*)h->dag_src.p,
-               (const int32_t*)h->dag_dst.p, nd, B, S.keys[0].p);
-        radixSort(h, S.keys[0].p, nullptr, nd, 2 * B);
-        h->dag_in_src.ensure((size_t)nd);
+        h->dag_in_src.ensureSlack((size_t)nd);
         h->dag_in_beg.ensure((size_t)V + 1);
-        launch(h, "dag_in", kBegScatter, gridFor(nd + 1, blk), blk, 0, (const int32_t*)nullptr,
-               (const uint64_t*)S.keys[0].p, B, nd, V, h->dag_in_beg.p);
+        begFromIds(h, "dag_in", nullptr, h->dag_tk.p, B, nd, h->dag_in_beg.p);
         launch(h, "dag_indeg", kDegAndSrc, gridFor(std::max<int64_t>(nd, V), blk), blk, 0,
-               (const int32_t*)h->dag_in_beg.p, V, ideg, (const uint64_t*)S.keys[0].p, nd, B, h->dag_in_src.p);
+               (const int32_t*)h->dag_in_beg.p, V, ideg, (const uint64_t*)h->dag_tk.p, nd, B, h->dag_in_src.p);
     } else {
         fillI32(h, ideg, 0, (int64_t)V + 1);
         h->dag_in_beg.ensure((size_t)V + 1);
@@ -806,8 +855,7 @@ void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks) {
     }
     h->n_dag = nd;
     // DAG CSR offsets by source SCC (arcs are sorted by src)
-    launch(h, "dag_beg", kBegScatter, gridFor(nd + 1, blk), blk, 0, (const int32_t*)h->dag_src.p,
-           (const uint64_t*)nullptr, 0, nd, V, h->dag_beg.p);
+    begFromIds(h, "dag_beg", h->dag_src.p, nullptr, 0, nd, h->dag_beg.p);
     if (!ranks) return;
     // Kahn levels
     ReachBufs R = reachBufs(h);
@@ -868,29 +916,51 @@ void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks) {
 // cycle re-rank it.
 namespace {
 
-enum : int { IR_DELTA = 0, IR_BACK, IR_DIRTY, IR_MINR, IR_MAXR, IR_AFF, IR_N };
+enum : int { IR_DELTA = 0, IR_BACK, IR_DIRTY, IR_MINR, IR_MAXR, IR_AFF, IR_CUTF, IR_CUTB, IR_GDEL, IR_GREG, IR_N };
+
+// The big SCC's certificate, as the classification sees it: its label g
+// (-1: none), and, when its trees are valid (fpar != nullptr), where the
+// heads / tails of deleted tree arcs go.
+struct IrCert {
+    int32_t g;
+    const int32_t* fpar;
+    const int32_t* bpar;
+    int32_t* cutf;
+    int32_t* cutb;
+};
 
 __global__ void kIrClassify(const uint64_t* __restrict__ ik, int64_t ni, const uint64_t* __restrict__ dk, int64_t nd,
                             int B, const int32_t* __restrict__ scc, const int64_t* __restrict__ rank,
                             uint64_t* __restrict__ dkeys, uint32_t* __restrict__ dvals, int32_t* __restrict__ bsu,
                             int32_t* __restrict__ bsv, uint8_t* __restrict__ dirty, int64_t* cnt,
-                            const uint8_t* __restrict__ dbypass) {
+                            const uint8_t* __restrict__ dbypass, IrCert gc) {
     const uint64_t mask = (1ULL << B) - 1;
     const int64_t n = ni + nd, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
         const int64_t i = base + laneId();
         const bool act = i < n, ins = i < ni;
-        int32_t su = 0, sv = 0;
+        int32_t su = 0, sv = 0, u = 0, v = 0;
         if (act) {
             const uint64_t k = ins ? ik[i] : dk[i - ni];
-            su = scc[(int32_t)(k >> B)];
-            sv = scc[(int32_t)(k & mask)];
+            u = (int32_t)(k >> B);
+            v = (int32_t)(k & mask);
+            su = scc[u];
+            sv = scc[v];
         }
         const bool cross = act && su != sv;
         const bool back = cross && ins && rank[su] >= rank[sv];
-        // an inner delete whose tail still reaches its head splits nothing
+        // an inner delete whose tail still reaches its head splits nothing;
+        // one inside the big SCC is checked against its trees instead
         const bool inner_del = act && !ins && su == sv && !(dbypass && dbypass[i - ni]);
-        if (inner_del) dirty[su] = 1;
+        const bool gdel = inner_del && su == gc.g;
+        if (inner_del && !gdel) dirty[su] = 1;
+        const bool cutf = gdel && gc.fpar && gc.fpar[v] == u;
+        const bool cutb = gdel && gc.bpar && gc.bpar[u] == v;
+        const int64_t pf = wavePush(cnt + IR_CUTF, cutf);
+        if (cutf) gc.cutf[pf] = v;
+        const int64_t pb = wavePush(cnt + IR_CUTB, cutb);
+        if (cutb) gc.cutb[pb] = u;
+        if (__ballot(gdel) && laneId() == 0) cnt[IR_GDEL] = 1;
         const int64_t dpos = wavePush(cnt + IR_DELTA, cross);
         if (cross) {
             dkeys[dpos] = ((uint64_t)(uint32_t)su << B) | (uint32_t)sv;
@@ -919,42 +989,66 @@ __global__ void kIrClassify(const uint64_t* __restrict__ ik, int64_t ni, const u
     }
 }
 
-// The condensation's arcs as keys with their multiplicities.
-__global__ void kIrDagKeys(const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
-                           const int32_t* __restrict__ mult, int64_t nd, int B, uint64_t* __restrict__ keys,
-                           uint32_t* __restrict__ vals) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nd) {
-        keys[i] = ((uint64_t)(uint32_t)src[i] << B) | (uint32_t)dst[i];
-        vals[i] = (uint32_t)mult[i];
+// Vertices whose label changed in a repair (old -> scc): listed in list,
+// cnt[0] = their number, cnt[1] = their forward + backward degree sum.
+__global__ void kIrChanged(const int32_t* __restrict__ old, const int32_t* __restrict__ scc, int32_t V,
+                           const int32_t* __restrict__ fb, const int32_t* __restrict__ bb, int32_t* __restrict__ list,
+                           int64_t* cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t deg = 0;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < V; base += stride) {
+        const int64_t v = base + threadIdx.x;
+        const bool ch = v < V && old[v] != scc[v];
+        if (ch) deg += (int64_t)(fb[v + 1] - fb[v]) + (bb[v + 1] - bb[v]);
+        const int64_t pos = blockReserve(ch ? 1 : 0, cnt);
+        if (ch) list[pos] = (int32_t)v;
     }
+    blockAtomicAdd(deg, cnt + 1);
 }
 
-// Sorted (key, signed count) pairs: at the first element of every key run,
-// the run's sum (int32) and a flag that it is positive.
-__global__ void kIrRunSums(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
-                           int32_t* __restrict__ flag, int32_t* __restrict__ sum) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t f = 0;
-    if (i == 0 || keys[i] != keys[i - 1]) {
-        int32_t acc = 0;
-        for (int64_t j = i; j < n && keys[j] == keys[i]; ++j) acc += (int32_t)vals[j];
-        sum[i] = acc;
-        f = acc > 0;
+// Condensation deltas of a relabelling, one block per changed vertex x: every
+// arc at x (out-arcs; in-arcs from unchanged tails, so an arc between two
+// changed vertices counts once) takes back its old key (-1) and adds its new
+// key (+1), each only when its ends lie in different SCCs.
+__global__ void kIrRelabelDeltas(const int32_t* __restrict__ list, const int32_t* __restrict__ old,
+                                 const int32_t* __restrict__ scc, const int32_t* __restrict__ fb,
+                                 const int32_t* __restrict__ fa, const int32_t* __restrict__ bb,
+                                 const int32_t* __restrict__ ba, int B, uint64_t* __restrict__ keys,
+                                 uint32_t* __restrict__ vals, int64_t* cnt) {
+    const int32_t x = list[blockIdx.x];
+    const int32_t ox = old[x], nx = scc[x];
+    for (int dir = 0; dir < 2; ++dir) {
+        const int32_t b = dir ? bb[x] : fb[x], e = dir ? bb[x + 1] : fb[x + 1];
+        const int32_t* adj = dir ? ba : fa;
+        for (int32_t j0 = b; j0 < e; j0 += blockDim.x) {
+            const int32_t j = j0 + (int32_t)threadIdx.x;
+            bool take_old = false, take_new = false;
+            uint64_t ko = 0, kn = 0;
+            if (j < e) {
+                const int32_t y = adj[j];
+                const int32_t oy = old[y], ny = scc[y];
+                if (!(dir && oy != ny)) {
+                    // (tail, head) = (x, y) forward, (y, x) backward
+                    const int32_t ot = dir ? oy : ox, oh = dir ? ox : oy;
+                    const int32_t nt = dir ? ny : nx, nh = dir ? nx : ny;
+                    take_old = ot != oh;
+                    take_new = nt != nh;
+                    ko = ((uint64_t)(uint32_t)ot << B) | (uint32_t)oh;
+                    kn = ((uint64_t)(uint32_t)nt << B) | (uint32_t)nh;
+                }
+            }
+            const int64_t po = wavePush(cnt, take_old);
+            if (take_old) {
+                keys[po] = ko;
+                vals[po] = 0xffffffffu;
+            }
+            const int64_t pn = wavePush(cnt, take_new);
+            if (take_new) {
+                keys[pn] = kn;
+                vals[pn] = 1u;
+            }
+        }
     }
-    flag[i] = f;
-}
-
-__global__ void kIrDagWrite(const uint64_t* __restrict__ keys, const int32_t* __restrict__ flag,
-                            const int32_t* __restrict__ sum, const int64_t* __restrict__ pos, int64_t n, int B,
-                            int32_t* __restrict__ src, int32_t* __restrict__ dst, int32_t* __restrict__ mult) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flag[i]) return;
-    const int64_t r = pos[i];
-    src[r] = (int32_t)(keys[i] >> B);
-    dst[r] = (int32_t)(keys[i] & ((1ULL << B) - 1));
-    mult[r] = sum[i];
 }
 
 // Condensation sweeps inside a rank window: forward visits rank <= lim,
@@ -985,12 +1079,17 @@ __global__ void kIrSeedList(const uint8_t* __restrict__ flag, int32_t V, int32_t
 
 // Vertices of the SCCs that may change: cnt += their number; with apply,
 // they are also labelled -1.
+// The big SCC (label excl, or -1) is left to its certificate: its vertices
+// are neither counted nor relabelled, and cnt[1] says whether the region holds it.
 __global__ void kIrMark(int32_t* scc, int32_t V, const uint8_t* __restrict__ dirty, const uint64_t* __restrict__ fw,
-                        const uint64_t* __restrict__ bw, int use_region, int apply, int64_t* cnt) {
+                        const uint64_t* __restrict__ bw, int use_region, int apply, int64_t* cnt, int32_t excl,
+                        int64_t* greg) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && excl >= 0)
+        *greg = use_region && bitOf(fw, excl) && bitOf(bw, excl) ? 1 : 0;
     int64_t a = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c = scc[v];
-        const bool in = dirty[c] || (use_region && bitOf(fw, c) && bitOf(bw, c));
+        const bool in = c != excl && (dirty[c] || (use_region && bitOf(fw, c) && bitOf(bw, c)));
         if (in) {
             if (apply) scc[v] = -1;
             ++a;
@@ -1001,13 +1100,300 @@ __global__ void kIrMark(int32_t* scc, int32_t V, const uint8_t* __restrict__ dir
 
 }  // namespace
 
+// ---- merging multiplicity changes into the condensation ----
+// The arcs are kept sorted by key (src << B | dst) and the in-arc keys by
+// (dst << B | src), so a batch's changes (signed per-arc counts) are merged
+// instead of re-sorting every arc: the changes are sorted and summed per
+// key; each key is matched against the arcs by binary search (new
+// multiplicity, or a new key); the new keys and the keys whose multiplicity
+// drops to zero form two short sorted lists (padded with DM_PAD to a fixed
+// length, so no host count is needed), and every kept arc moves to
+//   i + #(new keys below it) - #(removed keys below it),
+// every new key to its list index + #(arcs below it) - #(removed keys below
+// it).  The same placement runs on the in-arc keys with the transposed lists.
+namespace {
+
+__device__ __forceinline__ uint64_t arcKey(const int32_t* src, const int32_t* dst, int64_t i, int B) {
+    return ((uint64_t)(uint32_t)src[i] << B) | (uint32_t)dst[i];
+}
+// entries of a sorted sequence (accessor k, n entries) below x
+template <class K>
+__device__ __forceinline__ int64_t lowerBound(const K& k, int64_t n, uint64_t x) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t m = lo + ((hi - lo) >> 1);
+        if (k(m) < x) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+// sorted (key, signed count) pairs: at the first entry of every key, its sum
+// and a flag that the sum is not zero
+__global__ void kDmRuns(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
+                        int32_t* __restrict__ flag, int32_t* __restrict__ sum) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t f = 0;
+    if (i == 0 || keys[i] != keys[i - 1]) {
+        int32_t acc = 0;
+        for (int64_t j = i; j < n && keys[j] == keys[i]; ++j) acc += (int32_t)vals[j];
+        sum[i] = acc;
+        f = acc != 0;
+    }
+    flag[i] = f;
+}
+
+// unique changes (uk, us) matched against the arcs; entries past the unique
+// count (*nu) get no flags.  err: a change that would make a multiplicity
+// negative (the change lists would not describe the store).
+__global__ void kDmMatch(const uint64_t* __restrict__ keys, const int32_t* __restrict__ flag,
+                         const int32_t* __restrict__ sum, const int64_t* __restrict__ pos, int64_t n,
+                         const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
+                         const int32_t* __restrict__ mult, int64_t nd, int B, uint64_t* __restrict__ uk,
+                         int32_t* __restrict__ nm, int64_t* __restrict__ fi, int32_t* __restrict__ nf,
+                         int32_t* __restrict__ rf, int64_t* err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t nu = pos[n];
+    if (i >= nu) {  // (slot i of the unique lists)
+        nf[i] = 0;
+        rf[i] = 0;
+    }
+    if (!flag[i]) return;
+    const int64_t j = pos[i];
+    const uint64_t k = keys[i];
+    auto ak = [&](int64_t a) { return arcKey(src, dst, a, B); };
+    const int64_t a = lowerBound(ak, nd, k);
+    const bool found = a < nd && ak(a) == k;
+    const int32_t m = (found ? mult[a] : 0) + sum[i];
+    if (m < 0) atomicAdd((unsigned long long*)err, 1ULL);
+    uk[j] = k;
+    nm[j] = m;
+    fi[j] = found ? a : -1;
+    nf[j] = !found && m > 0;
+    rf[j] = found && m <= 0;
+}
+
+__global__ void kDmLists(const uint64_t* __restrict__ uk, const int32_t* __restrict__ nm,
+                         const int32_t* __restrict__ nf, const int32_t* __restrict__ rf,
+                         const int64_t* __restrict__ pnew, const int64_t* __restrict__ prem, int64_t n, int B,
+                         uint64_t* __restrict__ newk, int32_t* __restrict__ newm, uint64_t* __restrict__ remk,
+                         uint64_t* __restrict__ newt, uint64_t* __restrict__ remt) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t mask = (1ULL << B) - 1;
+    if (nf[j]) {
+        const uint64_t k = uk[j];
+        newk[pnew[j]] = k;
+        newm[pnew[j]] = nm[j];
+        newt[pnew[j]] = ((k & mask) << B) | (k >> B);
+    }
+    if (rf[j]) {
+        const uint64_t k = uk[j];
+        remk[prem[j]] = k;
+        remt[prem[j]] = ((k & mask) << B) | (k >> B);
+    }
+}
+
+// the forward arcs: kept arcs, then the new ones, then the changed
+// multiplicities (writing into the spare arrays)
+__global__ void kDmPlaceOld(const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
+                            const int32_t* __restrict__ mult, int64_t nd, int B, const uint64_t* __restrict__ newk,
+                            const uint64_t* __restrict__ remk, int64_t m, int32_t* __restrict__ osrc,
+                            int32_t* __restrict__ odst, int32_t* __restrict__ omult) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nd) return;
+    const uint64_t k = arcKey(src, dst, i, B);
+    auto nk = [&](int64_t a) { return newk[a]; };
+    auto rk = [&](int64_t a) { return remk[a]; };
+    const int64_t r = lowerBound(rk, m, k);
+    if (r < m && remk[r] == k) return;
+    const int64_t p = i + lowerBound(nk, m, k) - r;
+    osrc[p] = src[i];
+    odst[p] = dst[i];
+    omult[p] = mult[i];
+}
+__global__ void kDmPlaceNew(const uint64_t* __restrict__ newk, const int32_t* __restrict__ newm, int64_t m,
+                            const int32_t* __restrict__ src, const int32_t* __restrict__ dst, int64_t nd, int B,
+                            const uint64_t* __restrict__ remk, uint64_t pad, int32_t* __restrict__ osrc,
+                            int32_t* __restrict__ odst, int32_t* __restrict__ omult) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m || newk[j] == pad) return;
+    const uint64_t k = newk[j];
+    auto ak = [&](int64_t a) { return arcKey(src, dst, a, B); };
+    auto rk = [&](int64_t a) { return remk[a]; };
+    const int64_t p = j + lowerBound(ak, nd, k) - lowerBound(rk, m, k);
+    osrc[p] = (int32_t)(k >> B);
+    odst[p] = (int32_t)(k & ((1ULL << B) - 1));
+    omult[p] = newm[j];
+}
+__global__ void kDmUpdate(const uint64_t* __restrict__ uk, const int32_t* __restrict__ nm,
+                          const int64_t* __restrict__ fi, const int64_t* __restrict__ pos, int64_t n,
+                          const uint64_t* __restrict__ newk, const uint64_t* __restrict__ remk,
+                          int32_t* __restrict__ omult) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= pos[n] || fi[j] < 0 || nm[j] <= 0) return;
+    const uint64_t k = uk[j];
+    auto nk = [&](int64_t a) { return newk[a]; };
+    auto rk = [&](int64_t a) { return remk[a]; };
+    omult[fi[j] + lowerBound(nk, n, k) - lowerBound(rk, n, k)] = nm[j];
+}
+// the in-arc keys
+__global__ void kDmPlaceTk(const uint64_t* __restrict__ tk, int64_t nd, const uint64_t* __restrict__ newt,
+                           const uint64_t* __restrict__ remt, int64_t m, uint64_t pad, uint64_t* __restrict__ otk) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    auto tkk = [&](int64_t a) { return tk[a]; };
+    auto nt = [&](int64_t a) { return newt[a]; };
+    auto rt = [&](int64_t a) { return remt[a]; };
+    if (i < nd) {
+        const uint64_t k = tk[i];
+        const int64_t r = lowerBound(rt, m, k);
+        if (!(r < m && remt[r] == k)) otk[i + lowerBound(nt, m, k) - r] = k;
+    }
+    if (i < m && newt[i] != pad) {
+        const uint64_t k = newt[i];
+        otk[i + lowerBound(tkk, nd, k) - lowerBound(rt, m, k)] = k;
+    }
+}
+__global__ void kDmCounts(const int64_t* __restrict__ pnew, const int64_t* __restrict__ prem, int64_t n,
+                          int64_t* cnt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt[0] = pnew[n];
+        cnt[1] = prem[n];
+    }
+}
+
+}  // namespace
+
+// Merge n signed per-arc changes (dk: packed (src << B) | dst keys, dv: +1 /
+// -1 as uint32; sorted here in place) into the condensation; with ranks the
+// Kahn levels are recomputed.
+static void dagMerge(girhip_graph* h, int B, uint64_t* dk, uint32_t* dv, int64_t n, bool ranks) {
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    const int64_t nd0 = h->n_dag;
+    if (n <= 0) {
+        if (ranks) dagIndex(h, nd0, B, true);
+        return;
+    }
+    radixSort(h, dk, dv, n, 2 * B);
+    S.dm_flag.ensure((size_t)n);
+    S.dm_sum.ensure((size_t)n);
+    S.dm_pos.ensure((size_t)n + 1);
+    S.dm_uk.ensure((size_t)n);
+    S.dm_nm.ensure((size_t)n);
+    S.dm_fi.ensure((size_t)n);
+    S.dm_nf.ensure((size_t)n);
+    S.dm_rf.ensure((size_t)n);
+    S.dm_pnew.ensure((size_t)n + 1);
+    S.dm_prem.ensure((size_t)n + 1);
+    S.dm_newk.ensure((size_t)n);
+    S.dm_newm.ensure((size_t)n);
+    S.dm_remk.ensure((size_t)n);
+    S.dm_newt.ensure((size_t)n);
+    S.dm_remt.ensure((size_t)n);
+    S.dm_cnt.ensure(4);
+    const uint64_t pad = 1ULL << (2 * B);  // above every key
+    const unsigned g = gridFor(n, blk);
+    launch(h, "dm_runs", kDmRuns, g, blk, 0, (const uint64_t*)dk, (const uint32_t*)dv, n, S.dm_flag.p, S.dm_sum.p);
+    scanI32toI64(h, S.dm_flag.p, S.dm_pos.p, n);
+    fillI64(h, S.dm_cnt.p, 0, 4);
+    launch(h, "dm_match", kDmMatch, g, blk, 0, (const uint64_t*)dk, (const int32_t*)S.dm_flag.p,
+           (const int32_t*)S.dm_sum.p, (const int64_t*)S.dm_pos.p, n, (const int32_t*)h->dag_src.p,
+           (const int32_t*)h->dag_dst.p, (const int32_t*)h->dag_mult.p, nd0, B, S.dm_uk.p, S.dm_nm.p, S.dm_fi.p,
+           S.dm_nf.p, S.dm_rf.p, S.dm_cnt.p + 2);
+    scanI32toI64(h, S.dm_nf.p, S.dm_pnew.p, n);
+    scanI32toI64(h, S.dm_rf.p, S.dm_prem.p, n);
+    fillU64(h, S.dm_newk.p, pad, n);
+    fillU64(h, S.dm_remk.p, pad, n);
+    fillU64(h, S.dm_newt.p, pad, n);
+    fillU64(h, S.dm_remt.p, pad, n);
+    launch(h, "dm_lists", kDmLists, g, blk, 0, (const uint64_t*)S.dm_uk.p, (const int32_t*)S.dm_nm.p,
+           (const int32_t*)S.dm_nf.p, (const int32_t*)S.dm_rf.p, (const int64_t*)S.dm_pnew.p,
+           (const int64_t*)S.dm_prem.p, n, B, S.dm_newk.p, S.dm_newm.p, S.dm_remk.p, S.dm_newt.p, S.dm_remt.p);
+    launch(h, "dm_counts", kDmCounts, 1, 64, 0, (const int64_t*)S.dm_pnew.p, (const int64_t*)S.dm_prem.p, n,
+           S.dm_cnt.p);
+    int64_t c[3];
+    readCounters(h, S.dm_cnt.p, 3, c);
+    if (c[2] != 0) raise(GIRHIP_EHIP, "condensation merge: a change makes a multiplicity negative");
+    // the transposed lists in order (their padding stays behind them)
+    if (c[0] > 1) radixSort(h, S.dm_newt.p, nullptr, c[0], 2 * B);
+    if (c[1] > 1) radixSort(h, S.dm_remt.p, nullptr, c[1], 2 * B);
+    const int64_t nd1 = nd0 + c[0] - c[1];
+    h->dag_src2.ensureSlack((size_t)std::max<int64_t>(nd1, 1));
+    h->dag_dst2.ensureSlack((size_t)std::max<int64_t>(nd1, 1));
+    h->dag_mult2.ensureSlack((size_t)std::max<int64_t>(nd1, 1));
+    h->dag_tk2.ensureSlack((size_t)std::max<int64_t>(nd1, 1));
+    if (nd0 > 0)
+        launch(h, "dm_place_old", kDmPlaceOld, gridFor(nd0, blk), blk, 0, (const int32_t*)h->dag_src.p,
+               (const int32_t*)h->dag_dst.p, (const int32_t*)h->dag_mult.p, nd0, B, (const uint64_t*)S.dm_newk.p,
+               (const uint64_t*)S.dm_remk.p, n, h->dag_src2.p, h->dag_dst2.p, h->dag_mult2.p);
+    launch(h, "dm_place_new", kDmPlaceNew, g, blk, 0, (const uint64_t*)S.dm_newk.p, (const int32_t*)S.dm_newm.p, n,
+           (const int32_t*)h->dag_src.p, (const int32_t*)h->dag_dst.p, nd0, B, (const uint64_t*)S.dm_remk.p, pad,
+           h->dag_src2.p, h->dag_dst2.p, h->dag_mult2.p);
+    launch(h, "dm_update", kDmUpdate, g, blk, 0, (const uint64_t*)S.dm_uk.p, (const int32_t*)S.dm_nm.p,
+           (const int64_t*)S.dm_fi.p, (const int64_t*)S.dm_pos.p, n, (const uint64_t*)S.dm_newk.p,
+           (const uint64_t*)S.dm_remk.p, h->dag_mult2.p);
+    launch(h, "dm_place_tk", kDmPlaceTk, gridFor(std::max(nd0, n), blk), blk, 0, (const uint64_t*)h->dag_tk.p, nd0,
+           (const uint64_t*)S.dm_newt.p, (const uint64_t*)S.dm_remt.p, n, pad, h->dag_tk2.p);
+    h->dag_src.swap(h->dag_src2);
+    h->dag_dst.swap(h->dag_dst2);
+    h->dag_mult.swap(h->dag_mult2);
+    h->dag_tk.swap(h->dag_tk2);
+    dagIndex(h, nd1, B, ranks);
+}
+
+// The condensation after a relabelling (path 3 of repairScc): the current
+// arcs, the batch's multiplicity deltas not yet merged (nir of them, stored
+// after the arcs' own slots in ir_keys / ir_vals, under the old labels), and
+// the deltas of the rows of every vertex whose label changed (old labels in
+// ir_old), summed per key; then the in-arc CSR and the ranks.  A rebuild of
+// the condensation would read the rows of every vertex outside the big SCC.
+static void relabelDag(girhip_graph* h, int B, int64_t nir) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    h->mem_ok = false;  // member lists follow the labels
+    const int64_t nd0 = h->n_dag;
+    S.ir_list.ensure((size_t)V);
+    S.ir_cnt2.ensure(4);
+    int64_t* c2 = S.ir_cnt2.p;
+    fillI64(h, c2, 0, 2);
+    launch(h, "ir_changed", kIrChanged, boundedGrid(V, blk), blk, 0, (const int32_t*)S.ir_old.p,
+           (const int32_t*)h->scc.p, V, (const int32_t*)h->fw_beg.p, (const int32_t*)h->bw_beg.p, S.ir_list.p, c2);
+    int64_t r[2];
+    readCounters(h, c2, 2, r);
+    const int64_t nch = r[0], deg = r[1];
+    h->last_ms["batch_scc_changed"] = (double)nch;
+    const int64_t cap = nir + 2 * deg;
+    S.keys[0].ensure((size_t)std::max<int64_t>(cap, 1));
+    S.vals[0].ensure((size_t)std::max<int64_t>(cap, 1));
+    if (nir > 0) {
+        HIPCHK(hipMemcpyAsync(S.keys[0].p, S.ir_keys.p + nd0, sizeof(uint64_t) * nir, hipMemcpyDeviceToDevice, h->st));
+        HIPCHK(hipMemcpyAsync(S.vals[0].p, S.ir_vals.p + nd0, sizeof(uint32_t) * nir, hipMemcpyDeviceToDevice, h->st));
+    }
+    int64_t m = nir;
+    if (nch > 0) {
+        fillI64(h, c2 + 2, m, 1);
+        launch(h, "ir_relabel_deltas", kIrRelabelDeltas, (unsigned)nch, 256, 0, (const int32_t*)S.ir_list.p,
+               (const int32_t*)S.ir_old.p, (const int32_t*)h->scc.p, (const int32_t*)h->fw_beg.p,
+               (const int32_t*)h->fw_adj.p, (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p, B, S.keys[0].p,
+               S.vals[0].p, c2 + 2);
+        readCounters(h, c2 + 2, 1, &m);
+    }
+    dagMerge(h, B, S.keys[0].p, S.vals[0].p, m, true);
+}
+
 // The change lists of a batch just applied to the store (ik / dk: sorted
 // packed (u << B) | v keys).  Returns false when the layer must be rebuilt
 // instead: knob scc_incremental = 0, a batch of more than
 // max(4096, V / 1024) changes, or more than max(65536, V / 8) vertices in
-// SCCs that may change (a rebuild then costs about the same and skips the
-// repair's own passes; on a power-law graph a random batch almost always
-// deletes an arc inside the giant SCC).
+// SCCs that may change outside the big SCC.  The big SCC itself (the SCC of
+// the certificate's pivot) is never relabelled wholesale: its inner deletes
+// are checked against its spanning trees and the SCCs that may merge into
+// it are attached to it (certRepair), so a power-law graph's giant SCC costs
+// only its orphaned subtrees.
 bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* dk, int64_t nd, int B,
                const uint8_t* dbypass) {
     if (h->prm.scc_incremental == 0) return false;
@@ -1019,11 +1405,19 @@ bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* 
     const int64_t n = ni + nd;
     h->last_ms["batch_scc_path"] = 0;
     h->last_ms["batch_scc_affected"] = 0;
+    h->last_ms["batch_scc_cert"] = 0;
     if (n == 0 || V == 0) return true;
+    // the big SCC and its trees (pivot and label picked now when absent)
+    const bool use_cert = h->prm.scc_cert != 0;
+    if (!use_cert) h->gc_ok = false;
+    else if (!h->gc_ok) certTarget(h);
+    const int32_t g = use_cert ? h->gc_label : -1;
     S.ir_cnt.ensure(IR_N);
     S.ir_dirty.ensure((size_t)V);
     S.ir_bsu.ensure((size_t)std::max<int64_t>(ni, 1));
     S.ir_bsv.ensure((size_t)std::max<int64_t>(ni, 1));
+    S.gc_cut[0].ensure((size_t)std::max<int64_t>(nd, 1));
+    S.gc_cut[1].ensure((size_t)std::max<int64_t>(nd, 1));
     const int64_t cap = h->n_dag + n;
     S.ir_keys.ensure((size_t)cap);
     S.ir_vals.ensure((size_t)cap);
@@ -1032,53 +1426,43 @@ bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* 
     fillI64(h, cnt + IR_MINR, LLONG_MAX, 1);
     fillI64(h, cnt + IR_MAXR, -1, 1);
     fillU8(h, S.ir_dirty.p, 0, V);
+    const IrCert gc{g, h->gc_ok ? (const int32_t*)h->gc_fpar.p : nullptr,
+                    h->gc_ok ? (const int32_t*)h->gc_bpar.p : nullptr, S.gc_cut[0].p, S.gc_cut[1].p};
     // deltas go after the condensation's own arcs in the key array
     const int64_t nd0 = h->n_dag;
     launch(h, "ir_classify", kIrClassify, std::min<unsigned>(gridFor(n, blk), 4096), blk, 0, ik, ni, dk, nd, B,
            (const int32_t*)h->scc.p, (const int64_t*)h->rank.p, S.ir_keys.p + nd0, S.ir_vals.p + nd0, S.ir_bsu.p,
-           S.ir_bsv.p, S.ir_dirty.p, cnt, dbypass);
+           S.ir_bsv.p, S.ir_dirty.p, cnt, dbypass, gc);
     int64_t c[IR_N];
     readCounters(h, cnt, IR_N, c);
     const int64_t ndelta = c[IR_DELTA], nback = c[IR_BACK], ndirty = c[IR_DIRTY];
-    if (ndelta == 0 && ndirty == 0) {  // inserts inside SCCs only
+    const int64_t ncutf = c[IR_CUTF], ncutb = c[IR_CUTB];
+    // the big SCC needs a check: a deleted tree arc, or an inner delete while
+    // its trees are absent
+    const bool gdel = h->gc_ok ? (ncutf + ncutb) > 0 : c[IR_GDEL] != 0;
+    if (ndelta == 0 && ndirty == 0 && !gdel) {  // inserts inside SCCs only
         h->last_ms["batch_scc_path"] = 1;
         return true;
     }
     const int64_t big = std::max<int64_t>(65536, (int64_t)V / 8);
+    int64_t greg = 0;
     auto affected = [&](bool region, bool apply) {
         fillI64(h, cnt + IR_AFF, 0, 1);
         launch(h, "ir_mark", kIrMark, boundedGrid(V, blk), blk, 0, h->scc.p, V, (const uint8_t*)S.ir_dirty.p,
                (const uint64_t*)S.bits[3].p, (const uint64_t*)reachBufs(h).vis, region ? 1 : 0, apply ? 1 : 0,
-               cnt + IR_AFF);
-        int64_t a;
-        readCounters(h, cnt + IR_AFF, 1, &a);
-        return a;
+               cnt + IR_AFF, g, cnt + IR_GREG);
+        int64_t a[IR_GREG - IR_AFF + 1];
+        readCounters(h, cnt + IR_AFF, IR_GREG - IR_AFF + 1, a);
+        greg = region && g >= 0 ? a[IR_GREG - IR_AFF] : 0;
+        return a[0];
     };
     if (ndirty > 0 && affected(false, false) > big) return false;
-    // condensation multiplicities
-    if (ndelta > 0) {
-        const int64_t m = nd0 + ndelta;
-        if (nd0 > 0)
-            launch(h, "ir_dag_keys", kIrDagKeys, gridFor(nd0, blk), blk, 0, (const int32_t*)h->dag_src.p,
-                   (const int32_t*)h->dag_dst.p, (const int32_t*)h->dag_mult.p, nd0, B, S.ir_keys.p, S.ir_vals.p);
-        radixSort(h, S.ir_keys.p, S.ir_vals.p, m, 2 * B);
-        S.iv[0].ensure((size_t)m);
-        S.iv[1].ensure((size_t)m);
-        S.off2.ensure((size_t)m + 1);
-        launch(h, "ir_run_sums", kIrRunSums, gridFor(m, blk), blk, 0, (const uint64_t*)S.ir_keys.p,
-               (const uint32_t*)S.ir_vals.p, m, S.iv[0].p, S.iv[1].p);
-        scanI32toI64(h, S.iv[0].p, S.off2.p, m);
-        int64_t nd1;
-        readCounters(h, S.off2.p + m, 1, &nd1);
-        h->dag_src.ensureSlack((size_t)std::max<int64_t>(nd1, 1));
-        h->dag_dst.ensureSlack((size_t)std::max<int64_t>(nd1, 1));
-        h->dag_mult.ensureSlack((size_t)std::max<int64_t>(nd1, 1));
-        launch(h, "ir_dag_write", kIrDagWrite, gridFor(m, blk), blk, 0, (const uint64_t*)S.ir_keys.p,
-               (const int32_t*)S.iv[0].p, (const int32_t*)S.iv[1].p, (const int64_t*)S.off2.p, m, B, h->dag_src.p,
-               h->dag_dst.p, h->dag_mult.p);
-        dagFinish(h, nd1, B, false);
-    }
-    if (nback == 0 && ndirty == 0) {  // the partition and the ranks stand
+    // condensation multiplicities: merged now when the partition may stand or
+    // the merge sweeps need the updated condensation; a relabelling merges
+    // them with its own deltas
+    const bool dag_has_delta = ndelta > 0 && (nback > 0 || (ndirty == 0 && !gdel));
+    if (dag_has_delta) dagMerge(h, B, S.ir_keys.p + nd0, S.ir_vals.p + nd0, ndelta, false);
+    if (nback == 0 && ndirty == 0 && !gdel) {  // the partition and the ranks stand
         h->last_ms["batch_scc_path"] = 1;
         return true;
     }
@@ -1112,15 +1496,26 @@ bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* 
     int64_t naff = affected(nback > 0, false);
     h->last_ms["batch_scc_affected"] = (double)naff;
     if (naff > big) return false;
-    if (naff == 0) {  // no cycle closed: re-rank the condensation
+    const bool gwork = gdel || greg != 0;
+    if (naff == 0 && !gwork) {  // no cycle closed: re-rank the condensation
         h->last_ms["batch_scc_path"] = 2;
-        dagFinish(h, h->n_dag, B, true);
+        dagIndex(h, h->n_dag, B, true);
         return true;
     }
     h->last_ms["batch_scc_path"] = 3;
+    // the labels before the relabelling: the condensation is then updated
+    // from the rows of the vertices whose label changes
+    S.ir_old.ensure((size_t)V);
+    HIPCHK(hipMemcpyAsync(S.ir_old.p, h->scc.p, sizeof(int32_t) * V, hipMemcpyDeviceToDevice, h->st));
     affected(nback > 0, true);
+    if (gwork) {
+        h->last_ms["batch_scc_cert"] = h->gc_ok ? 1 : 2;
+        const CertStats cs = certRepair(h, h->gc_ok ? ncutf : 0, h->gc_ok ? ncutb : 0, naff);
+        h->last_ms["batch_scc_dropped"] = (double)cs.dropped;
+        h->last_ms["batch_scc_joined"] = (double)cs.joined;
+    }
     runSccActive(h);
-    runDag(h);
+    relabelDag(h, B, dag_has_delta ? 0 : ndelta);
     return true;
 }
 

@@ -273,7 +273,8 @@ def test_tuning_knobs_do_not_change_results():
                      ("persist", 1), ("chunk", 1), ("sssp_dist32", 0), ("xcd", 1), ("split", 0), ("scc_block", 256), ("scc_block", 1024),
                      ("sssp_delta", 100000), ("sssp_interleave", 0), ("sssp_interleave", 1),
                      ("sssp_ms_batch", 3), ("sssp_ms_batch", 32), ("scc_trim2", 1), ("sssp_unroll", 1),
-                     ("sssp_unroll", 2), ("bfs_alpha_auto", 0), ("bfs_alpha", 1000), ("update_bypass", 0)]:
+                     ("sssp_unroll", 2), ("bfs_alpha_auto", 0), ("bfs_alpha", 1000), ("update_bypass", 0),
+                     ("scc_cert", 0)]:
             say("knob", k, v)
             old = G.get_param(k)
             G.set_param(k, v)
@@ -457,6 +458,7 @@ def test_incremental_scc_repair_paths():
     V = len(fb) - 1
     with girhip.Graph(fb, fa) as G:
         G.set_param("update_bypass", 0)  # the paths below, not the bypass test first
+        G.set_param("scc_cert", 0)       # nor the big SCC's certificate (own test below)
         G.wcc()
         s0, _, _ = G.scc_dag()
         r = G.ranks()
@@ -494,6 +496,100 @@ def test_incremental_scc_repair_paths():
             check_layers(G, fb, fa, bb, ba, f"repair n={n}", rebuild=False)
             assert np.array_equal(G.get_scc()[0], H.get_scc()[0])
             assert H.last_ms("batch_scc_path") == 4
+
+
+def _giant(s):
+    lab, cnt = np.unique(s, return_counts=True)
+    return int(lab[np.argmax(cnt)])
+
+
+def test_big_scc_certificate():
+    # the big SCC across batches through its spanning trees (no relabelling of
+    # the giant): random mixed batches, deletes that strand members (every
+    # out-arc or every in-arc of a member), inserts that pull outside SCCs
+    # into it, and its smallest vertex leaving it (the canonical label moves);
+    # every step against the oracle and against a rebuild
+    V, fb, fa, bb, ba = girhip.rmat_graph(14, seed=71)
+    rng = np.random.default_rng(72)
+    paths = {}
+    drops = joins = 0
+    with girhip.Graph(fb, fa, bb, ba) as G, girhip.Graph(fb, fa, bb, ba) as H:
+        H.set_param("scc_incremental", 0)
+        for X in (G, H):
+            X.wcc()
+            X.scc_dag()
+
+        def step(u, v, ins, tag):
+            nonlocal fb, fa, bb, ba, drops, joins
+            u, v, ins = (np.asarray(x) for x in (u, v, ins))
+            say("certificate step", tag, "ops", len(u))
+            G.apply_batch(u.astype(np.int32), v.astype(np.int32), ins.astype(np.uint8))
+            H.apply_batch(u.astype(np.int32), v.astype(np.int32), ins.astype(np.uint8))
+            fb, fa, bb, ba = oracle.apply_batch(fb, fa, u, v, ins)
+            s = check_layers(G, fb, fa, bb, ba, tag, rebuild=False)
+            assert np.array_equal(s, H.get_scc()[0]), tag
+            c = int(G.last_ms("batch_scc_cert"))
+            paths[c] = paths.get(c, 0) + 1
+            if c:
+                drops += int(G.last_ms("batch_scc_dropped"))
+                joins += int(G.last_ms("batch_scc_joined"))
+            return s
+
+        s = G.get_scc()[0]
+        for i in range(6):
+            step(*_random_batch(rng, fb, fa, 200, V), f"random {i}")
+        for i in range(4):  # strand a member of the giant: all its out-arcs (or in-arcs) go
+            s = G.get_scc()[0]
+            gl = _giant(s)
+            mem = np.nonzero(s == gl)[0]
+            x = int(mem[rng.integers(0, len(mem))])
+            if i % 2 == 0:
+                nb = fa[fb[x]:fb[x + 1]]
+                step(np.full(len(nb), x), nb, np.zeros(len(nb)), f"strand out {x}")
+            else:
+                nb = ba[bb[x]:bb[x + 1]]
+                step(nb, np.full(len(nb), x), np.zeros(len(nb)), f"strand in {x}")
+        for i in range(3):  # outside vertices pulled in: x reached from the giant, y reaching it
+            s = G.get_scc()[0]
+            gl = _giant(s)
+            g0 = int(np.nonzero(s == gl)[0][0])
+            fwd = oracle.bfs_textbook(fb, fa, g0) >= 0
+            bwd = oracle.bfs_textbook(bb, ba, g0) >= 0
+            xs = np.nonzero(fwd & ~bwd)[0]
+            ys = np.nonzero(bwd & ~fwd)[0]
+            if len(xs) == 0 or len(ys) == 0:
+                continue
+            k = min(5, len(xs), len(ys))
+            step(xs[:k], ys[:k], np.ones(k), f"merge {k}")
+        # a member cut off entirely (its WCC splits too: the WCC layer is rebuilt)
+        s = G.get_scc()[0]
+        mem = np.nonzero(s == _giant(s))[0]
+        x = int(mem[len(mem) // 2])
+        outs, ins_ = fa[fb[x]:fb[x + 1]], ba[bb[x]:bb[x + 1]]
+        step(np.concatenate([np.full(len(outs), x), ins_]), np.concatenate([outs, np.full(len(ins_), x)]),
+             np.zeros(len(outs) + len(ins_)), f"isolate {x}")
+        assert G.last_ms("batch_wcc_unsafe") > 0
+        # the giant's smallest vertex leaves it
+        s = G.get_scc()[0]
+        gl = _giant(s)
+        nb = fa[fb[gl]:fb[gl + 1]]
+        s = step(np.full(len(nb), gl), nb, np.zeros(len(nb)), f"label vertex {gl} leaves")
+        assert s[gl] != _giant(s)
+        for i in range(4):
+            step(*_random_batch(rng, fb, fa, 300, V), f"random after {i}")
+    assert paths.get(1, 0) > 0 and paths.get(2, 0) > 0, paths
+    assert drops > 0 and joins > 0, (drops, joins)
+    # a lattice: deep trees (hundreds of levels), strands and random batches
+    V, fb, fa, bb, ba = girhip.grid_graph(120, 120, 0.9, 0.8, 5)
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.wcc()
+        G.scc_dag()
+        for i in range(5):
+            u, v, ins = _random_batch(rng, fb, fa, 60, V)
+            say("certificate lattice step", i)
+            G.apply_batch(u, v, ins)
+            fb, fa, bb, ba = oracle.apply_batch(fb, fa, u, v, ins)
+            check_layers(G, fb, fa, bb, ba, f"lattice {i}", rebuild=False)
 
 
 def test_delete_bypass():
