@@ -665,11 +665,13 @@ def main():
     # ---- single-edge updates (the reference's SPU phase, bench_main.cpp:
     #      116-125: one GIR::insertEdge / deleteEdge call per op): on a fresh
     #      graph with its layers built, N_SINGLE inserts of absent uniform
-    #      pairs, then N_SINGLE deletes of present arcs, one
-    #      girhip_apply_batch call each ----
+    #      pairs, then N_SINGLE deletes of present arcs, through the drop-in
+    #      GIR class (tests/cpp/gir_updates: each kind's calls until the
+    #      queued ops are applied; and one op applied at a time) ----
     spu = None
     spu_ops = None
-    if not args.no_extras:
+    upd_bin = os.path.join(ROOT, "tests", "cpp", "gir_updates")
+    if not args.no_extras and os.path.exists(upd_bin):
         srng = np.random.default_rng(31)
         keys0 = (np.repeat(np.arange(V, dtype=np.int64), np.diff(fb)) << 32) | fa.astype(np.int64)
         ck = np.unique((srng.integers(0, V, 4 * N_SINGLE).astype(np.int64) << 32) | srng.integers(0, V, 4 * N_SINGLE))
@@ -680,34 +682,22 @@ def main():
         su, sv = (both >> 32).astype(np.int32), (both & 0xffffffff).astype(np.int32)
         sins = np.concatenate([np.ones(len(ik), np.uint8), np.zeros(len(dk), np.uint8)])
         spu_ops = (su, sv, sins)
-        GS = girhip.Graph(*d_csr)
-        GS.wcc(out=lab)
-        GS.scc_dag(out=scc)
-        # one untimed op of each kind first (work buffers of the batch path)
-        GS.apply_batch(su[:1], sv[:1], sins[:1])
-        GS.apply_batch(su[:1], sv[:1], (1 - sins[:1]).astype(np.uint8))
-        wall = {1: [], 0: []}
-        dev_ms = {1: [], 0: []}
-        paths = {}
-        for i in range(len(su)):
-            t0 = time.perf_counter()
-            GS.apply_batch(su[i:i + 1], sv[i:i + 1], sins[i:i + 1])
-            wall[int(sins[i])].append((time.perf_counter() - t0) * 1e3)
-            dev_ms[int(sins[i])].append(GS.last_ms("batch"))
-            p_ = int(GS.last_ms("batch_scc_path"))
-            paths[p_] = paths.get(p_, 0) + 1
-        spu = {"inserts": len(ik), "deletes": len(dk),
-                  "insert_ms_per_op": round(float(np.mean(wall[1])), 4),
-                  "delete_ms_per_op": round(float(np.mean(wall[0])), 4),
-                  "insert_device_ms_per_op": round(float(np.mean(dev_ms[1])), 4),
-                  "delete_device_ms_per_op": round(float(np.mean(dev_ms[0])), 4),
-                  "ops_per_s": round(len(su) / (sum(wall[1]) + sum(wall[0])) * 1e3, 1),
-                  "scc_repair_paths": {str(k): v for k, v in sorted(paths.items())},
-                  "timing": "host wall clock per girhip_apply_batch call of one op (copies and sync included)"}
-        gs_scc = GS.get_scc()[0]
-        gs_wcc = GS.get_wcc()[0]
-        GS.close()
-        del GS
+        with tempfile.TemporaryDirectory() as td:
+            girhip.write_csr(os.path.join(td, "g"), fb, fa, bb, ba)
+            np.stack([su, sv, sins.astype(np.int32)], axis=1).astype(np.int32).tofile(os.path.join(td, "ops.bin"))
+            o = subprocess.run([upd_bin, os.path.join(td, "g") + "/", os.path.join(td, "ops.bin"), td + "/"],
+                               capture_output=True, text=True, check=True).stdout
+            r = next(json.loads(x[2:]) for x in o.splitlines() if x.startswith("@@"))
+            gs_scc = np.fromfile(os.path.join(td, "post_scc.bin"), np.int32)
+            gs_wcc = np.fromfile(os.path.join(td, "post_wcc.bin"), np.int32)
+        spu = {"inserts": r["inserts"], "deletes": r["deletes"],
+               "insert_ms_per_op": round(r["insert_ms"] / max(1, r["inserts"]), 4),
+               "delete_ms_per_op": round(r["delete_ms"] / max(1, r["deletes"]), 4),
+               "insert_ms_per_op_applied_each": round(r["insert_read_each_ms"] / max(1, r["inserts"]), 4),
+               "delete_ms_per_op_applied_each": round(r["delete_read_each_ms"] / max(1, r["deletes"]), 4),
+               "timing": "drop-in GIR class, host wall clock: each kind's insertEdge / deleteEdge calls until the "
+                         "queued ops are applied and the layers current on the device (GIR::synced); "
+                         "_applied_each: one op, then GIR::synced, per op"}
         log(f"single-edge updates: {spu}")
 
     # ---- batched form (girhip_run_batch): 1,024 mixed BFS/SSSP queries from
@@ -871,6 +861,10 @@ def main():
                                                        and np.array_equal(so.get("post_scc"), gs_scc))
                     spu["speedup_insert_vs_cpu"] = round(cpu["single_insert_ms_per_op"] / spu["insert_ms_per_op"], 3)
                     spu["speedup_delete_vs_cpu"] = round(cpu["single_delete_ms_per_op"] / spu["delete_ms_per_op"], 3)
+                    spu["speedup_insert_applied_each_vs_cpu"] = round(
+                        cpu["single_insert_ms_per_op"] / spu["insert_ms_per_op_applied_each"], 4)
+                    spu["speedup_delete_applied_each_vs_cpu"] = round(
+                        cpu["single_delete_ms_per_op"] / spu["delete_ms_per_op_applied_each"], 4)
             log(f"cpu build {cpu_build:.1f} ms")
 
     out = {

@@ -147,6 +147,20 @@ void certTarget(girhip_graph* h) {
     h->gc_label = (int32_t)r[2];
 }
 
+// c[0] = scc[p]
+__global__ void kCertLabelAt(const int32_t* __restrict__ scc, int32_t p, int64_t* c) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) c[0] = scc[p];
+}
+
+void certLabel(girhip_graph* h) {
+    Scratch& S = scratch(h);
+    S.gc_cnt.ensure(GC_N);
+    launch(h, "cert_label", kCertLabelAt, 1, 64, 0, (const int32_t*)h->scc.p, h->gc_pivot, S.gc_cnt.p);
+    int64_t g;
+    readCounters(h, S.gc_cnt.p, 1, &g);
+    h->gc_label = (int32_t)g;
+}
+
 CertStats certRepair(girhip_graph* h, int64_t ncutf, int64_t ncutb, int64_t ncand) {
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;

@@ -486,7 +486,7 @@ const int32_t* fwSrc(girhip_graph* h);
 // ---- layers ----
 void runWcc(girhip_graph* h);
 void runScc(girhip_graph* h);
-void runSccActive(girhip_graph* h);
+void runSccActive(girhip_graph* h, bool cert = false);
 void runDag(girhip_graph* h);
 void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks);
 // Repair the SCC layer after a batch whose sorted change keys are ik / dk
@@ -495,7 +495,8 @@ void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks);
 // reachability (their tail still reaches their head).
 bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* dk, int64_t nd, int B,
                const uint8_t* dbypass);
-// Certificate of the big SCC (cert.hip).  certTarget picks the pivot (largest
+// Certificate of the big SCC (cert.hip).  A build records the trees with its
+// pivot sweeps (runSccActive with cert).  certTarget picks the pivot (largest
 // outdeg * indeg) and the label of its SCC and marks the trees absent;
 // certRepair checks the big SCC after a batch (ncutf / ncutb: heads / tails
 // of deleted tree arcs in gc_cut[0] / [1]; ncand: vertices labelled -1 that
@@ -504,6 +505,7 @@ struct CertStats {
     int64_t dropped = 0, joined = 0, orphans = 0, swept = 0;
 };
 void certTarget(girhip_graph* h);
+void certLabel(girhip_graph* h);  // gc_label = the label of gc_pivot
 CertStats certRepair(girhip_graph* h, int64_t ncutf, int64_t ncutb, int64_t ncand);
 // Batches with at most BYPASS_MAX deletes test them for a bypass path.
 constexpr int64_t BYPASS_MAX = 8;

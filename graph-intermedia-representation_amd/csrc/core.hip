@@ -505,6 +505,10 @@ extern "C" int girhip_create(const int32_t* fw_beg, const int32_t* fw_adj, const
             } else {
                 buildTranspose(h);
             }
+            // the spare adjacency an update batch merges into, reserved now
+            // (two E-sized allocations would otherwise land in the first batch)
+            h->fw_adj2.ensureSlack((size_t)(h->E > 0 ? h->E : 1));
+            h->bw_adj2.ensureSlack((size_t)(h->E > 0 ? h->E : 1));
             syncStream(h);
         });
         if (rc != GIRHIP_OK) {

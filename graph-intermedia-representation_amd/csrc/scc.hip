@@ -651,20 +651,24 @@ struct SccBlock {
 
 void runScc(girhip_graph* h) {
     const int32_t V = h->V;
-    h->gc_ok = false;  // the big SCC's trees are rebuilt by the next batch that needs them
+    h->gc_ok = false;
     h->scc.ensure((size_t)(V > 0 ? V : 1));
     if (V == 0) {
         h->n_scc = 0;
         return;
     }
     fillI32(h, h->scc.p, -1, V);
-    runSccActive(h);
+    runSccActive(h, true);
 }
 
 // SCCs of the vertices labelled -1 ("active"); every other label stays.  The
 // active set must be a union of whole SCCs of the current graph (all of it
-// for a build; the affected SCCs for a repair after an update batch).
-void runSccActive(girhip_graph* h) {
+// for a build; the affected SCCs for a repair after an update batch).  With
+// cert (a build), the forward and backward sweeps from the pivot record
+// their parents: restricted to the pivot's SCC they are its out-tree and
+// in-tree (every vertex on a tree path from the pivot to a member reaches
+// the member, which reaches the pivot), the certificate update batches keep.
+void runSccActive(girhip_graph* h, bool cert) {
     const SccBlock sb(h);
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
@@ -691,13 +695,32 @@ void runSccActive(girhip_graph* h) {
         readCounters(h, h->dcnt.p + 4, 1, &key);
         const int32_t pivot = (int32_t)(0x7fffffff - (key & 0x7fffffffLL));
         const ActiveOnly act{h->scc.p};
+        if (cert) {
+            h->gc_fpar.ensure((size_t)V);
+            h->gc_bpar.ensure((size_t)V);
+            fillI32(h, h->gc_fpar.p + pivot, pivot, 1);
+            fillI32(h, h->gc_bpar.p + pivot, pivot, 1);
+        }
         fillU64(h, B.vis, 0, B.W);
         fillI32(h, B.q[0], pivot, 1);
-        runReach(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
+        if (cert)
+            runReach(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, B, 1, 0, n, h->E, act,
+                     WriteParent{h->gc_fpar.p});
+        else
+            runReach(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
         HIPCHK(hipMemcpyAsync(S.bits[3].p, B.vis, sizeof(uint64_t) * B.W, hipMemcpyDeviceToDevice, h->st));
         fillU64(h, B.vis, 0, B.W);
         fillI32(h, B.q[0], pivot, 1);
-        runReach(h, h->bw_beg.p, h->bw_adj.p, h->fw_beg.p, h->fw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
+        if (cert)
+            runReach(h, h->bw_beg.p, h->bw_adj.p, h->fw_beg.p, h->fw_adj.p, B, 1, 0, n, h->E, act,
+                     WriteParent{h->gc_bpar.p});
+        else
+            runReach(h, h->bw_beg.p, h->bw_adj.p, h->fw_beg.p, h->fw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
+        if (cert) {  // the trees hold; the pivot's label is read by the first batch that needs it
+            h->gc_ok = true;
+            h->gc_pivot = pivot;
+            h->gc_label = -1;
+        }
         fillI64(h, h->dcnt.p + 5, INT64_MAX, 1);
         launch(h, "scc_fwbw_min", kFwBwMin, boundedGrid(n, blk), blk, 0, (const int32_t*)list, n,
                (const uint64_t*)S.bits[3].p, (const uint64_t*)B.vis, h->dcnt.p + 5);
@@ -1411,6 +1434,7 @@ bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* 
     const bool use_cert = h->prm.scc_cert != 0;
     if (!use_cert) h->gc_ok = false;
     else if (!h->gc_ok) certTarget(h);
+    else if (h->gc_label < 0) certLabel(h);  // trees from the build: its pivot's label
     const int32_t g = use_cert ? h->gc_label : -1;
     S.ir_cnt.ensure(IR_N);
     S.ir_dirty.ensure((size_t)V);

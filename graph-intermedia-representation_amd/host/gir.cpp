@@ -135,21 +135,30 @@ GIR::GIR(graph* g) {
     init(g->fw_beg_pos, g->fw_csr, g->bw_beg_pos, g->bw_csr, g->vert_count, g->edge_count);
 }
 
-GIR::~GIR() { girhip_destroy(h); }
+GIR::~GIR() {
+    try {
+        flush();
+    } catch (...) {
+    }
+    girhip_destroy(h);
+}
 
 void GIR::buildWCC() {
+    flush();
     check(girhip_wcc(h, nullptr, nullptr), "buildWCC");
     built_wcc = true;
     stale = true, ++epoch;
 }
 
 void GIR::buildSCC_DAG() {
+    flush();
     check(girhip_scc_dag(h, nullptr, nullptr, nullptr), "buildSCC_DAG");
     built_scc = true;
     stale = true, ++epoch;
 }
 
 void GIR::refresh() const {
+    flush();
     if (!stale) return;
     if (built_wcc) {
         wcc_label.resize(V);
@@ -179,6 +188,7 @@ void GIR::refresh() const {
 }
 
 void GIR::applyOps(const std::vector<Update>& batch, bool repair) {
+    flush();
     const size_t n = batch.size();
     std::vector<int32_t> u(n), v(n);
     std::vector<uint8_t> ins(n);
@@ -194,18 +204,43 @@ void GIR::applyOps(const std::vector<Update>& batch, bool repair) {
     stale = true, ++epoch;
 }
 
-void GIR::insertEdge(int u, int v) { applyOps({{u, v, true}}, true); }
-void GIR::deleteEdge(int u, int v) { applyOps({{u, v, false}}, true); }
-void GIR::updateEdgeDirected(int u, int v, bool insert) { applyOps({{u, v, insert}}, true); }
+// Single-edge updates are queued and applied as one batch before anything
+// reads the graph or its layers (flush): the last op on an arc decides its
+// presence either way, so the state every reader sees is the one the ops
+// applied one by one would leave (gir.cpp:3043-3133 applies each at once).
+void GIR::queueOp(int u, int v, bool insert) {
+    queued.push_back({u, v, insert});
+    if (queued.size() >= kQueueMax) flush();
+}
+void GIR::flush() const {
+    if (queued.empty()) return;
+    const size_t n = queued.size();
+    std::vector<int32_t> u(n), v(n);
+    std::vector<uint8_t> ins(n);
+    for (size_t i = 0; i < n; ++i) {
+        u[i] = queued[i].u;
+        v[i] = queued[i].v;
+        ins[i] = queued[i].insert ? 1 : 0;
+    }
+    queued.clear();
+    girhip_batch_stats st;
+    check(girhip_apply_batch(h, u.data(), v.data(), ins.data(), nullptr, (int64_t)n, &st), "insertEdge / deleteEdge");
+    stale = true, ++epoch;
+}
+void GIR::insertEdge(int u, int v) { queueOp(u, v, true); }
+void GIR::deleteEdge(int u, int v) { queueOp(u, v, false); }
+void GIR::updateEdgeDirected(int u, int v, bool insert) { queueOp(u, v, insert); }
 void GIR::applyBatch(const std::vector<Update>& batch) { applyOps(batch, true); }
 void GIR::ingestBatch(const std::vector<Update>& batch) { applyOps(batch, false); }
 
 void GIR::sync() {
+    flush();
     check(girhip_sync(h), "sync");
     stale = true, ++epoch;
 }
 
 bool GIR::synced() const {
+    flush();
     int32_t s = 1;
     check(girhip_synced(h, &s), "synced");
     return s != 0;
@@ -286,6 +321,7 @@ std::vector<std::vector<int>> GIR::getWCCs() const {
 }
 
 graph* GIR::getDirectedGraph() const {
+    flush();
     int32_t v;
     int64_t E;
     check(girhip_size(h, &v, &E), "getDirectedGraph");
@@ -310,6 +346,7 @@ std::vector<int> GIR::getDirectedInNeighbors(int v) const {
     return std::vector<int>(g->bw_csr + g->bw_beg_pos[v], g->bw_csr + g->bw_beg_pos[v + 1]);
 }
 bool GIR::hasDirectedEdge(int u, int v) const {
+    flush();
     int32_t p = 0;
     check(girhip_has_edge(h, u, v, &p), "hasDirectedEdge");
     return p != 0;
@@ -327,11 +364,13 @@ uint32_t GIR::edgeWeight(int u, int v) {
 }
 
 QueryVec<int> GIR::bfsMany(const std::vector<int>& sources) {
+    flush();
     QueryVec<int> lv((size_t)V * sources.size());
     check(girhip_bfs(h, sources.data(), (int32_t)sources.size(), lv.data(), nullptr), "bfs");
     return lv;
 }
 QueryVec<uint64_t> GIR::ssspMany(const std::vector<int>& sources, uint64_t delta) {
+    flush();
     QueryVec<uint64_t> d((size_t)V * sources.size());
     check(girhip_sssp(h, sources.data(), (int32_t)sources.size(), delta, d.data(), nullptr), "sssp");
     return d;
@@ -339,6 +378,7 @@ QueryVec<uint64_t> GIR::ssspMany(const std::vector<int>& sources, uint64_t delta
 QueryVec<int> GIR::bfs(int source) { return bfsMany({source}); }
 
 QueryVec<int> GIR::bfsTwoLevel(int source) {
+    flush();
     if (!built_scc) buildSCC_DAG();
     QueryVec<int> lv((size_t)V);
     const int32_t s = source;
@@ -348,6 +388,7 @@ QueryVec<int> GIR::bfsTwoLevel(int source) {
 }
 
 QueryVec<uint64_t> GIR::ssspTwoLevel(int source, uint64_t delta) {
+    flush();
     if (!built_scc) buildSCC_DAG();
     QueryVec<uint64_t> d((size_t)V);
     const int32_t s = source;
@@ -357,6 +398,7 @@ QueryVec<uint64_t> GIR::ssspTwoLevel(int source, uint64_t delta) {
 }
 
 std::vector<char> GIR::reachableSccsPublic(int c) {
+    flush();
     if (!built_scc) buildSCC_DAG();
     if (!synced()) sync();
     refresh();
@@ -369,16 +411,19 @@ std::vector<char> GIR::reachableSccsPublic(int c) {
 }
 
 void GIR::trackIncremental(int source, bool weighted) {
+    flush();
     check(girhip_inc_track(h, source, weighted ? 1 : 0), "trackIncremental");
     inc_fresh[weighted] = false;
 }
 
 void GIR::updateIncremental(bool weighted) {
+    flush();
     check(girhip_inc_update(h, weighted ? 1 : 0), "updateIncremental");
     inc_fresh[weighted] = false;
 }
 
 uint64_t GIR::incDist(bool weighted, int v) const {
+    flush();
     if (!inc_fresh[weighted]) {
         inc_host[weighted].resize((size_t)V);
         check(girhip_inc_dist(h, weighted ? 1 : 0, inc_host[weighted].data()), "incDist");
@@ -388,6 +433,7 @@ uint64_t GIR::incDist(bool weighted, int v) const {
 }
 
 void GIR::ensureBlocks() {
+    flush();
     if (!built_scc) buildSCC_DAG();
     if (!synced()) sync();
     if (blk_ok && blk_epoch == epoch) return;
@@ -459,12 +505,14 @@ std::vector<std::vector<int>> GIR::getBCTComponents(int scc_id) {
 }
 
 QueryVec<double> GIR::bc(const std::vector<int>& sources) {
+    flush();
     QueryVec<double> sc((size_t)V);
     check(girhip_bc(h, sources.data(), (int32_t)sources.size(), sc.data()), "bc");
     return sc;
 }
 
 std::vector<char> GIR::reachableMany(const std::vector<int>& u, const std::vector<int>& v) {
+    flush();
     if (u.size() != v.size()) throw std::invalid_argument("reachableMany: u and v differ in length");
     if (!built_scc) buildSCC_DAG();
     if (!synced()) sync();
@@ -476,6 +524,7 @@ std::vector<char> GIR::reachableMany(const std::vector<int>& u, const std::vecto
 bool GIR::reachable(int u, int v) { return reachableMany({u}, {v})[0] != 0; }
 
 QueryVec<int> GIR::bfsInComponent(int source, bool directed, long long* visited) {
+    flush();
     if (directed ? !built_scc : !built_wcc) {
         if (directed) buildSCC_DAG();
         else buildWCC();
@@ -499,7 +548,10 @@ QueryVec<int> ParallelBFS::testParallelBFS(GIR& gir, int source, bool is_directe
 }
 QueryVec<uint64_t> GIR::sssp(int source, uint64_t delta) { return ssspMany({source}, delta); }
 
-void GIR::setWeights(const uint32_t* w_fw) { check(girhip_set_weights(h, w_fw), "setWeights"); }
+void GIR::setWeights(const uint32_t* w_fw) {
+    flush();
+    check(girhip_set_weights(h, w_fw), "setWeights");
+}
 
 void GIR::setQueryGranularity(size_t par_frontier, size_t large_unit) {
     (void)par_frontier;  // (a CPU threading threshold; every step runs on the device here)

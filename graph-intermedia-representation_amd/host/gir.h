@@ -82,7 +82,12 @@ public:
     void buildBCT() { ensureBlocks(); }
     void ensureBlocks();
 
-    // Updates (gir.h:104-119).
+    // Updates (gir.h:104-119).  insertEdge / deleteEdge / updateEdgeDirected
+    // queue the op; queued ops are applied as one batch (last op on an arc
+    // wins, which is what applying them one by one leaves) before any call
+    // that reads the graph or a layer, before applyBatch / ingestBatch /
+    // sync, and once kQueueMax ops are waiting.
+    static const size_t kQueueMax = 4096;
     void insertEdge(int u, int v);
     void deleteEdge(int u, int v);
     void applyBatch(const std::vector<Update>& batch);
@@ -173,6 +178,9 @@ private:
     void init(const int32_t* fb, const int32_t* fa, const int32_t* bb, const int32_t* ba, int32_t V_, int64_t E_);
     void refresh() const;  // pull labels after a change
     void applyOps(const std::vector<Update>& batch, bool repair);
+    void queueOp(int u, int v, bool insert);
+    void flush() const;  // apply the queued single-edge ops
+    mutable std::vector<Update> queued;
 
     girhip_graph* h = nullptr;
     int V = 0;

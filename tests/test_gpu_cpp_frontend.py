@@ -81,3 +81,42 @@ def test_reference_stream_caller_on_dropin(tmp_path, mode):
             assert np.array_equal(G.get_wcc()[0], oracle.wcc(f1, a1)[0]), s
             assert np.array_equal(G.bfs([src])[0], oracle.bfs_textbook(f1, a1, src)), s
             assert np.array_equal(G.sssp([src], 16)[0], oracle.dijkstra(f1, a1, None, src)), s
+
+
+UPD = os.path.join(ROOT, "tests", "cpp", "gir_updates")
+
+
+def test_single_edge_updates_through_dropin(tmp_path):
+    # tests/cpp/gir_updates drives GIR::insertEdge / deleteEdge one call per
+    # op (queued by the class, applied when read) and once more applying each
+    # op at once; the layers after the ops equal the oracle's on the updated
+    # store (canonical labels), including ops on present / absent arcs, u == v
+    # and out-of-range ids, which change nothing.
+    import numpy as np
+
+    import girhip
+    import oracle
+
+    assert os.path.exists(UPD), "build the C++ front end first (__graft_entry__.build())"
+    V, fb, fa, bb, ba = girhip.rmat_graph(11, edge_factor=8, seed=81)
+    rng = np.random.default_rng(82)
+    rows = np.repeat(np.arange(V, dtype=np.int32), np.diff(fb))
+    e = rng.integers(0, len(fa), 60)
+    ops = [(int(rows[i]), int(fa[i]), 0) for i in e]                              # deletes of present arcs
+    ops += [(int(a), int(b), 1) for a, b in rng.integers(0, V, (60, 2))]         # inserts of random pairs
+    ops += [(int(rows[e[0]]), int(fa[e[0]]), 1), (5, 5, 1), (V + 3, 1, 1), (1, -1, 0)]  # re-insert, invalid
+    d = str(tmp_path / "g")
+    girhip.write_csr(d, fb, fa, bb, ba)
+    np.array(ops, np.int32).tofile(str(tmp_path / "ops.bin"))
+    print("[case] cpp gir_updates rmat11", len(ops), "single ops", flush=True)
+    r = subprocess.run([UPD, d + "/", str(tmp_path / "ops.bin"), str(tmp_path) + "/"], capture_output=True,
+                       text=True, timeout=110)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rec = next(l for l in r.stdout.splitlines() if l.startswith("@@"))
+    assert '"inserts": 63' in rec and '"deletes": 61' in rec, rec
+    u, v, ins = (np.array([o[k] for o in ops], np.int64) for k in range(3))
+    ok = (u >= 0) & (u < V) & (v >= 0) & (v < V) & (u != v)
+    nfb, nfa, _, _ = oracle.apply_batch(fb, fa, u[ok].astype(np.int32), v[ok].astype(np.int32),
+                                        ins[ok].astype(np.uint8))
+    assert np.array_equal(np.fromfile(str(tmp_path / "post_scc.bin"), np.int32), oracle.scc(nfb, nfa)[0])
+    assert np.array_equal(np.fromfile(str(tmp_path / "post_wcc.bin"), np.int32), oracle.wcc(nfb, nfa)[0])

@@ -505,7 +505,8 @@ def _giant(s):
 
 def test_big_scc_certificate():
     # the big SCC across batches through its spanning trees (no relabelling of
-    # the giant): random mixed batches, deletes that strand members (every
+    # the giant; the build records them, and a batch rebuilds them when they
+    # are absent): random mixed batches, deletes that strand members (every
     # out-arc or every in-arc of a member), inserts that pull outside SCCs
     # into it, and its smallest vertex leaving it (the canonical label moves);
     # every step against the oracle and against a rebuild
@@ -537,7 +538,10 @@ def test_big_scc_certificate():
 
         s = G.get_scc()[0]
         for i in range(6):
+            if i == 3:  # one batch without the trees: the next one builds them from the pivot
+                G.set_param("scc_cert", 0)
             step(*_random_batch(rng, fb, fa, 200, V), f"random {i}")
+            G.set_param("scc_cert", 1)
         for i in range(4):  # strand a member of the giant: all its out-arcs (or in-arcs) go
             s = G.get_scc()[0]
             gl = _giant(s)
