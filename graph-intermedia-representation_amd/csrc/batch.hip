@@ -476,6 +476,31 @@ void repairLayers(girhip_graph* h, const Changes* C, int B) {
 }
 
 }  // namespace
+
+// Work buffers of a batch of up to n ops, reserved when a layer is first
+// built so that the first small batch allocates nothing large.
+void reserveBatch(girhip_graph* h, int64_t n) {
+    Scratch& S = scratch(h);
+    const size_t m = (size_t)std::max<int64_t>(n, 1);
+    S.b_u.ensureSlack(m);
+    S.b_v.ensureSlack(m);
+    S.b_ins.ensureSlack(m);
+    S.b_fi.ensureSlack(m);
+    S.b_fd.ensureSlack(m);
+    S.b_pi.ensureSlack(m + 1);
+    S.b_pd.ensureSlack(m + 1);
+    for (int i = 0; i < 2; ++i) {
+        S.b_ik[i].ensureSlack(m);
+        S.b_dk[i].ensureSlack(m);
+    }
+    if (h->has_w) {
+        S.b_w.ensureSlack(m);
+        S.b_iw.ensureSlack(m);
+    }
+    S.b_nb.ensure((size_t)h->V + 1);
+    S.bp_found.ensure((size_t)BYPASS_MAX);
+}
+
 }  // namespace gir
 
 using namespace gir;

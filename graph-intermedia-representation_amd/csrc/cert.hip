@@ -152,6 +152,15 @@ __global__ void kCertLabelAt(const int32_t* __restrict__ scc, int32_t p, int64_t
     if (blockIdx.x == 0 && threadIdx.x == 0) c[0] = scc[p];
 }
 
+void reserveCert(girhip_graph* h) {
+    Scratch& S = scratch(h);
+    const int64_t W = ((int64_t)h->V + 63) / 64;
+    for (auto& b : S.gc_bits) b.ensure((size_t)std::max<int64_t>(W, 1));
+    S.gc_cnt.ensure(GC_N);
+    h->gc_fpar.ensure((size_t)std::max<int32_t>(h->V, 1));
+    h->gc_bpar.ensure((size_t)std::max<int32_t>(h->V, 1));
+}
+
 void certLabel(girhip_graph* h) {
     Scratch& S = scratch(h);
     S.gc_cnt.ensure(GC_N);

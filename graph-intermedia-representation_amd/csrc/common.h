@@ -506,6 +506,11 @@ struct CertStats {
 };
 void certTarget(girhip_graph* h);
 void certLabel(girhip_graph* h);  // gc_label = the label of gc_pivot
+void reserveCert(girhip_graph* h);
+// Work buffers of the batch path / the SCC repair for batches of up to n ops
+// (batch.hip / scc.hip), reserved when a layer is first built.
+void reserveBatch(girhip_graph* h, int64_t n);
+void reserveRepair(girhip_graph* h, int64_t n);
 CertStats certRepair(girhip_graph* h, int64_t ncutf, int64_t ncutb, int64_t ncand);
 // Batches with at most BYPASS_MAX deletes test them for a bypass path.
 constexpr int64_t BYPASS_MAX = 8;

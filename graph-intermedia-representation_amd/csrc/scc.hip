@@ -1408,6 +1408,49 @@ static void relabelDag(girhip_graph* h, int B, int64_t nir) {
     dagMerge(h, B, S.keys[0].p, S.vals[0].p, m, true);
 }
 
+// Work buffers of a repair after a batch of up to n changes (and of the
+// certificate check), reserved when the layer is first built.
+void reserveRepair(girhip_graph* h, int64_t n) {
+    Scratch& S = scratch(h);
+    const int32_t V = h->V;
+    const size_t m = (size_t)std::max<int64_t>(n, 1), Vs = (size_t)std::max<int32_t>(V, 1);
+    S.ir_cnt.ensure(IR_N);
+    S.ir_cnt2.ensure(4);
+    S.ir_dirty.ensure(Vs);
+    S.ir_seed.ensure(Vs);
+    S.ir_old.ensure(Vs);
+    S.ir_list.ensure(Vs);
+    S.ir_bsu.ensure(m);
+    S.ir_bsv.ensure(m);
+    S.gc_cut[0].ensure(m);
+    S.gc_cut[1].ensure(m);
+    S.ir_keys.ensureSlack((size_t)h->n_dag + m);
+    S.ir_vals.ensureSlack((size_t)h->n_dag + m);
+    const size_t d = 4 * m;  // changes of a repair: the batch's and a relabelling's
+    S.dm_flag.ensure(d);
+    S.dm_sum.ensure(d);
+    S.dm_uk.ensure(d);
+    S.dm_nm.ensure(d);
+    S.dm_fi.ensure(d);
+    S.dm_nf.ensure(d);
+    S.dm_rf.ensure(d);
+    S.dm_newk.ensure(d);
+    S.dm_newm.ensure(d);
+    S.dm_remk.ensure(d);
+    S.dm_newt.ensure(d);
+    S.dm_remt.ensure(d);
+    S.dm_pos.ensure(d + 1);
+    S.dm_pnew.ensure(d + 1);
+    S.dm_prem.ensure(d + 1);
+    S.dm_cnt.ensure(4);
+    const size_t nd = (size_t)std::max<int64_t>(h->n_dag, 1) + m;
+    h->dag_src2.ensureSlack(nd);
+    h->dag_dst2.ensureSlack(nd);
+    h->dag_mult2.ensureSlack(nd);
+    h->dag_tk2.ensureSlack(nd);
+    reserveCert(h);
+}
+
 // The change lists of a batch just applied to the store (ik / dk: sorted
 // packed (u << B) | v keys).  Returns false when the layer must be rebuilt
 // instead: knob scc_incremental = 0, a batch of more than
@@ -1554,6 +1597,9 @@ extern "C" int girhip_scc_dag(girhip_graph* h, int32_t* scc, int32_t* n_scc, int
         runScc(h);
         runDag(h);
         tEnd(h, "scc_dag");
+        // what maintaining the layer under small batches needs (allocated once)
+        reserveBatch(h, 4096);
+        reserveRepair(h, 4096);
         h->scc_req = true;
         h->scc_ok = true;
         if (scc) copyOut(h, scc, h->scc.p, sizeof(int32_t) * (size_t)h->V);

@@ -195,6 +195,7 @@ extern "C" int girhip_wcc(girhip_graph* h, int32_t* label, int32_t* n_wcc) {
         tBegin(h);
         runWcc(h);
         tEnd(h, "wcc");
+        reserveBatch(h, 4096);  // (allocated once)
         h->wcc_req = true;
         h->wcc_ok = true;
         if (label) {
