@@ -13,11 +13,15 @@
 //   - buckets are common: every source processes bucket [lo, lo + delta)
 //     at the same time, so a vertex whose distances from several sources
 //     fall in the same bucket is visited once for all of them;
-//   - the work lists carry a source mask per vertex: near / far / done
-//     masks (uint32 per vertex) say for which sources a vertex entered the
-//     bucket this round, waits in a later bucket, or was processed in this
-//     bucket; one bitmap per set (with a summary bit per 64-bit word) lists
-//     the vertices whose mask is non-zero.
+//   - the work lists carry a source mask per vertex: near / done masks
+//     (uint32 per vertex) say for which sources a vertex entered the bucket
+//     this round or was processed in this bucket; one bitmap per set (with a
+//     summary bit per 64-bit word) lists the vertices whose mask is
+//     non-zero.  The far set (vertices waiting in a later bucket) is a bitmap
+//     only: its sources are read off the distances when it is walked (a
+//     finite distance at or past the bucket's end is a far entry, since a
+//     relaxation that set it marked the vertex), so a far relaxation costs
+//     the bitmap's atomicOr and no mask update.
 // Round (same phases as the per-source engine in sssp.hip):
 //   relax the queue's light arcs -> compact (near bitmap -> next queue; or,
 //   when the bucket drained, done bitmap -> heavy queue, or the far minimum)
@@ -46,7 +50,7 @@ constexpr int MSTR = 16;
 
 struct MsState {
     void* dist;  // [V][KS] of T
-    uint32_t *nmask, *fmask, *dmask;
+    uint32_t *nmask, *dmask;
     uint64_t *nbits, *fbits, *dbits;
     uint64_t *nsum, *fsum, *dsum;
     int32_t* q[2];
@@ -68,6 +72,8 @@ __device__ __forceinline__ uint64_t widen(T v) {
 // (KS <= 8 for 32-bit distances); longer ones one distance at a time.
 template <class T, int KS>
 constexpr bool kVecSeg = KS * sizeof(T) <= 32;
+template <int KS>
+constexpr uint32_t kAllSources = KS >= 32 ? 0xffffffffu : (1u << KS) - 1;
 
 template <class T, int KS>
 __device__ __forceinline__ void loadSeg(const T* p, T (&v)[KS]) {
@@ -118,10 +124,7 @@ struct MsRelax {
             // bit of a word raises the round's flag
             if (setBit(S.nbits, S.nsum, x)) *S.c(par ? M_NSET1 : M_NSET0) = 1;
         }
-        if (far) {
-            if ((S.fmask[x] & far) != far) atomicOr(&S.fmask[x], far);
-            setBit(S.fbits, S.fsum, x);
-        }
+        if (far) setBit(S.fbits, S.fsum, x);
     }
     __device__ int64_t operator()(int32_t u, uint64_t pay, int32_t, int64_t j) const {
         const int32_t x = fa[j];
@@ -232,10 +235,7 @@ struct MsRelax {
                 atomicOr(&S.nmask[x[u]], near);
                 nset |= markBit(S.nbits, S.nsum, x[u], nsw[u]);
             }
-            if (far) {
-                atomicOr(&S.fmask[x[u]], far);
-                markBit(S.fbits, S.fsum, x[u], fsw[u]);
-            }
+            if (far) markBit(S.fbits, S.fsum, x[u], fsw[u]);
         }
         if (ovf) *S.c(M_OVF) = 1;
         if (nset) *S.c(par ? M_NSET1 : M_NSET0) = 1;
@@ -446,7 +446,7 @@ __device__ __forceinline__ void farMinAll(const MsState& S, MsLds& sh) {
             const int total = expandGroup(m, k.word(grp), lds);
             for (int i = laneId(); i < total; i += WAVE) {
                 const int32_t x = lds[i];
-                forMaskDist<T, KS>(dist, x, S.fmask[x], [&](int, uint64_t v) {
+                forMaskDist<T, KS>(dist, x, kAllSources<KS>, [&](int, uint64_t v) {
                     if (v >= hi && v < mn) mn = v;
                 });
             }
@@ -506,11 +506,15 @@ __global__ void kMsSplit(MsState S, int par, uint64_t delta) {
     int32_t* lds = sh.ids[wib];
     uint64_t* keepw = sh.keep[wib];
     const T* dist = (const T*)S.dist;
-    auto classify = [&](int32_t x, uint32_t fm, uint32_t& keep, uint32_t& take) {
+    // a far entry of source s: a finite distance at or past hi (see above)
+    auto classify = [&](int32_t x, uint32_t& keep, uint32_t& take) {
         keep = take = 0;
-        forMaskDist<T, KS>(dist, x, fm, [&](int s, uint64_t v) {
-            if (v >= hi) keep |= 1u << s;
-            else if (v >= lo) take |= 1u << s;
+        forMaskDist<T, KS>(dist, x, kAllSources<KS>, [&](int s, uint64_t v) {
+            if (v >= hi) {
+                if (v != INF) keep |= 1u << s;
+            } else if (v >= lo) {
+                take |= 1u << s;
+            }
         });
     };
     const int64_t T_ = S.W2 * TPS;
@@ -524,7 +528,7 @@ __global__ void kMsSplit(MsState S, int par, uint64_t delta) {
             for (int i0 = 0; i0 < total; i0 += WAVE) {
                 const int i = i0 + lane;
                 uint32_t keep = 0, take = 0;
-                if (i < total) classify(lds[i], S.fmask[lds[i]], keep, take);
+                if (i < total) classify(lds[i], keep, take);
                 ntake += __popcll(__ballot(take != 0));
             }
             waveSync();
@@ -543,9 +547,7 @@ __global__ void kMsSplit(MsState S, int par, uint64_t delta) {
                 uint32_t keep = 0, take = 0;
                 if (i < total) {
                     const int32_t x = lds[i];
-                    const uint32_t fm = S.fmask[x];
-                    classify(x, fm, keep, take);
-                    if (keep != fm) S.fmask[x] = keep;
+                    classify(x, keep, take);
                     if (keep) atomicOr((unsigned long long*)&keepw[(x >> 6) & (GW - 1)], 1ULL << (x & 63));
                 }
                 const uint64_t bal = __ballot(take != 0);
@@ -598,7 +600,7 @@ bool msRun(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, const gi
     const size_t Vs = (size_t)std::max<int32_t>(V, 1);
     S_.ms_dist.ensure(sizeof(T) * Vs * KS);
     for (int i = 0; i < 3; ++i) {
-        S_.ms_mask[i].ensure(Vs);
+        if (i != 1) S_.ms_mask[i].ensure(Vs);  // (no far masks)
         S_.ms_bits[i].ensure((size_t)std::max<int64_t>(W, 1));
         S_.ms_sum[i].ensure((size_t)std::max<int64_t>(W2, 1));
         S_.ms_q[i].ensure(Vs);
@@ -608,7 +610,6 @@ bool msRun(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, const gi
     MsState S;
     S.dist = S_.ms_dist.p;
     S.nmask = S_.ms_mask[0].p;
-    S.fmask = S_.ms_mask[1].p;
     S.dmask = S_.ms_mask[2].p;
     S.nbits = S_.ms_bits[0].p;
     S.fbits = S_.ms_bits[1].p;
@@ -629,7 +630,7 @@ bool msRun(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, const gi
     if constexpr (sizeof(T) == 4) fillU32(h, (uint32_t*)S.dist, 0xffffffffu, (int64_t)V * KS);
     else fillU64(h, (uint64_t*)S.dist, INF, (int64_t)V * KS);
     for (int i = 0; i < 3; ++i) {
-        fillU32(h, S_.ms_mask[i].p, 0, V);
+        if (i != 1) fillU32(h, S_.ms_mask[i].p, 0, V);
         fillU64(h, S_.ms_bits[i].p, 0, W);
         fillU64(h, S_.ms_sum[i].p, 0, W2);
     }
