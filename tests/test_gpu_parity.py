@@ -347,8 +347,11 @@ def test_trim2_pairs():
                 assert ns == nw and np.array_equal(s, want), (flip, t2, rounds)
 
 
-def test_unsorted_duplicate_input_is_normalised():
-    V, fb, fa, bb, ba = girhip.rmat_graph(9, seed=35)
+@pytest.mark.parametrize("scale", [9, 18])
+def test_unsorted_duplicate_input_is_normalised(scale):
+    # (scale 18: 4.6 M arcs with duplicates, so the device sort runs over
+    # thousands of tiles; the undirected closure then sorts twice as many)
+    V, fb, fa, bb, ba = girhip.rmat_graph(scale, seed=35)
     rng = np.random.default_rng(0)
     # shuffle every row and duplicate some arcs (GIR normalises: gir.cpp:95-130)
     rows = np.repeat(np.arange(V), np.diff(fb))
@@ -362,11 +365,18 @@ def test_unsorted_duplicate_input_is_normalised():
     np.add.at(nfb, r2 + 1, 1)
     nfb = np.cumsum(nfb).astype(np.int32)
     nfa = a2[order].astype(np.int32)
-    say("normalise rmat9")
+    say("normalise rmat", scale)
     with girhip.Graph(nfb, nfa) as G:
         assert G.E == len(fa)
         gfb, gfa, gbb, gba = G.csr()
         assert (gfb == fb).all() and (gfa == fa).all() and (gbb == bb).all() and (gba == ba).all()
+        if scale > 9:
+            wl, _ = G.wcc()
+            src = girhip.sources(fb, 2, 36)
+            lv, vis = G.bfs_component(src, undirected=True)
+            for i, x in enumerate(src):
+                want, nv = oracle.bfs_component(fb, fa, bb, ba, wl, int(x), True)
+                assert np.array_equal(lv[i], want) and vis[i] == nv, "undirected"
 
 
 def test_bad_inputs_are_refused():
