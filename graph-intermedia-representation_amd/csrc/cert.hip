@@ -138,7 +138,7 @@ void certTarget(girhip_graph* h) {
     S.gc_cnt.ensure(GC_N);
     int64_t* c = S.gc_cnt.p;
     fillI64(h, c, 0, 1);
-    launch(h, "cert_pivot", kCertPivot, boundedGrid(V, 256), 256, 0, (const int32_t*)h->fw_beg.p,
+    launch(h, "cert_pivot", kCertPivot, reduceGrid(V, 256), 256, 0, (const int32_t*)h->fw_beg.p,
            (const int32_t*)h->bw_beg.p, V, (unsigned long long*)c);
     launch(h, "cert_label", kCertLabel, 1, 64, 0, (const int32_t*)h->scc.p, c);
     int64_t r[3];
@@ -236,7 +236,7 @@ CertStats certRepair(girhip_graph* h, int64_t ncutf, int64_t ncutb, int64_t ncan
         HIPCHK(hipMemcpyAsync(att, R.vis, sizeof(uint64_t) * W, hipMemcpyDeviceToDevice, h->st));
     }
     fillI64(h, c + GC_DROP, 0, 3);
-    launch(h, "cert_apply", kCertApply, boundedGrid(V, blk), blk, 0, h->scc.p, V, g, full ? 1 : 0,
+    launch(h, "cert_apply", kCertApply, reduceGrid(V, blk), blk, 0, h->scc.p, V, g, full ? 1 : 0,
            (const uint64_t*)S.gc_bits[0].p, (const uint64_t*)S.gc_bits[1].p, (const uint64_t*)S.gc_bits[2].p,
            (const uint64_t*)S.gc_bits[3].p, c);
     int64_t r[3];

@@ -689,7 +689,7 @@ void runSccActive(girhip_graph* h, bool cert) {
     // FW-BW from the pivot
     if (n > 0 && h->prm.scc_pivot != 0 && n > (int64_t)h->prm.scc_color_min) {
         fillI64(h, h->dcnt.p + 4, 0, 1);
-        launch(h, "scc_pivot", kPivot, boundedGrid(n, blk), blk, 0, (const int32_t*)list, n,
+        launch(h, "scc_pivot", kPivot, reduceGrid(n, blk), blk, 0, (const int32_t*)list, n,
                (const int32_t*)h->fw_beg.p, (const int32_t*)h->bw_beg.p, (unsigned long long*)(h->dcnt.p + 4));
         int64_t key;
         readCounters(h, h->dcnt.p + 4, 1, &key);
@@ -722,7 +722,7 @@ void runSccActive(girhip_graph* h, bool cert) {
             h->gc_label = -1;
         }
         fillI64(h, h->dcnt.p + 5, INT64_MAX, 1);
-        launch(h, "scc_fwbw_min", kFwBwMin, boundedGrid(n, blk), blk, 0, (const int32_t*)list, n,
+        launch(h, "scc_fwbw_min", kFwBwMin, reduceGrid(n, blk), blk, 0, (const int32_t*)list, n,
                (const uint64_t*)S.bits[3].p, (const uint64_t*)B.vis, h->dcnt.p + 5);
         launch(h, "scc_fwbw_assign", kFwBwAssign, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
                (const uint64_t*)S.bits[3].p, (const uint64_t*)B.vis, (const int64_t*)h->dcnt.p + 5, h->scc.p);
@@ -766,7 +766,7 @@ void runSccActive(girhip_graph* h, bool cert) {
         n = trimRounds(h, list, alt, n, 1);
     }
     fillI64(h, h->dcnt.p, 0, 1);
-    launch(h, "count_roots", kCountSelf, boundedGrid(V, blk), blk, 0, (const int32_t*)h->scc.p, V, h->dcnt.p);
+    launch(h, "count_roots", kCountSelf, reduceGrid(V, blk), blk, 0, (const int32_t*)h->scc.p, V, h->dcnt.p);
     int64_t ns;
     readCounters(h, h->dcnt.p, 1, &ns);
     h->n_scc = (int32_t)ns;
@@ -785,7 +785,7 @@ void runDag(girhip_graph* h) {
         S.gbits.ensure(((size_t)V + 63) / 64);
         fillI64(h, h->dcnt.p + 9, 0, 1);
         fillI64(h, h->dcnt.p + 14, 0, 1);
-        launch(h, "dag_top", kMaxDegVertex, boundedGrid(V, 256), 256, 0, (const int32_t*)h->fw_beg.p,
+        launch(h, "dag_top", kMaxDegVertex, reduceGrid(V, 256), 256, 0, (const int32_t*)h->fw_beg.p,
                (const int32_t*)h->bw_beg.p, V, (unsigned long long*)(h->dcnt.p + 14));
         launch(h, "dag_label_bits", kLabelBits, boundedGrid(((int64_t)V + 63) / 64 * 64, 256), 256, 0,
                (const int32_t*)h->scc.p, V, (const unsigned long long*)(h->dcnt.p + 14), S.gbits.p,
@@ -883,7 +883,7 @@ static void dagIndex(girhip_graph* h, int64_t nd, int B, bool ranks) {
     // Kahn levels
     ReachBufs R = reachBufs(h);
     fillI64(h, h->dcnt.p + 12, 0, 1);
-    launch(h, "dag_hub", kHub, boundedGrid(V, blk), blk, 0, (const int32_t*)ideg, V, h->dcnt.p + 12);
+    launch(h, "dag_hub", kHub, reduceGrid(V, blk), blk, 0, (const int32_t*)ideg, V, h->dcnt.p + 12);
     int64_t hubk;
     readCounters(h, h->dcnt.p + 12, 1, &hubk);
     const int32_t hub = hubk ? (int32_t)(hubk & 0xffffffffLL) : -1;
@@ -904,7 +904,7 @@ static void dagIndex(girhip_graph* h, int64_t nd, int B, bool ranks) {
             const KahnF f{h->dag_dst.p, ideg, R.q[cur ^ 1], kc + (L + 1) % KAHN_RING, hub, nullptr};
             launch(h, "rank_kahn", kKahnStep, 1024, blk, 0, (const int32_t*)R.q[cur], (const int64_t*)cin,
                    (const int32_t*)h->dag_beg.p, h->rank.p, L, f, heavyList(h), kc + (L + 2) % KAHN_RING);
-            launch(h, "rank_kahn_heavy", kKahnHeavy, 1024, 256, 0, heavyList(h), f);
+            launch(h, "rank_kahn_heavy", kKahnHeavy, 256, 256, 0, heavyList(h), f);
             cur ^= 1;
         }
         int64_t nq;
@@ -1383,7 +1383,7 @@ static void relabelDag(girhip_graph* h, int B, int64_t nir) {
     S.ir_cnt2.ensure(4);
     int64_t* c2 = S.ir_cnt2.p;
     fillI64(h, c2, 0, 2);
-    launch(h, "ir_changed", kIrChanged, boundedGrid(V, blk), blk, 0, (const int32_t*)S.ir_old.p,
+    launch(h, "ir_changed", kIrChanged, reduceGrid(V, blk), blk, 0, (const int32_t*)S.ir_old.p,
            (const int32_t*)h->scc.p, V, (const int32_t*)h->fw_beg.p, (const int32_t*)h->bw_beg.p, S.ir_list.p, c2);
     int64_t r[2];
     readCounters(h, c2, 2, r);
@@ -1515,7 +1515,7 @@ bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* 
     int64_t greg = 0;
     auto affected = [&](bool region, bool apply) {
         fillI64(h, cnt + IR_AFF, 0, 1);
-        launch(h, "ir_mark", kIrMark, boundedGrid(V, blk), blk, 0, h->scc.p, V, (const uint8_t*)S.ir_dirty.p,
+        launch(h, "ir_mark", kIrMark, reduceGrid(V, blk), blk, 0, h->scc.p, V, (const uint8_t*)S.ir_dirty.p,
                (const uint64_t*)S.bits[3].p, (const uint64_t*)reachBufs(h).vis, region ? 1 : 0, apply ? 1 : 0,
                cnt + IR_AFF, g, cnt + IR_GREG);
         int64_t a[IR_GREG - IR_AFF + 1];

@@ -60,36 +60,40 @@ __global__ void kWccCompress(int32_t* comp, int32_t V) {
     stRel(&comp[v], c);
 }
 
-// One block of 1024 threads: sample labels, bitonic sort in LDS, longest run.
-__global__ void kWccSample(const int32_t* __restrict__ comp, int32_t V, uint64_t seed, int64_t* out) {
-    __shared__ int32_t s[1024];
+// One block of 1024 threads: sample 1,024 labels and count them in an LDS
+// hash table (2,048 slots, linear probing); the most frequent label (ties:
+// the larger) is the giant.  Which label is taken only steers which
+// vertices skip step 4's arcs, never a label.
+__global__ void __launch_bounds__(1024) kWccSample(const int32_t* __restrict__ comp, int32_t V, uint64_t seed,
+                                                   int64_t* out) {
+    constexpr int SLOTS = 2048;
+    __shared__ int32_t key[SLOTS];
+    __shared__ int32_t cnt[SLOTS];
     __shared__ unsigned long long best;
     const int t = threadIdx.x;
+    for (int i = t; i < SLOTS; i += blockDim.x) {
+        key[i] = -1;
+        cnt[i] = 0;
+    }
+    if (t == 0) best = 0;
     uint64_t x = seed + (uint64_t)t * 0x9E3779B97F4A7C15ULL;
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
     x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
     x ^= x >> 31;
-    s[t] = comp[(int32_t)(((x >> 32) * (uint64_t)V) >> 32)];
-    if (t == 0) best = 0;
+    const int32_t lab = comp[(int32_t)(((x >> 32) * (uint64_t)V) >> 32)];
     __syncthreads();
-    for (int k = 2; k <= 1024; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const int p = t ^ j;
-            if (p > t) {
-                const bool up = (t & k) == 0;
-                const int32_t a = s[t], b = s[p];
-                if ((a > b) == up) {
-                    s[t] = b;
-                    s[p] = a;
-                }
-            }
-            __syncthreads();
+    uint32_t hs = (uint32_t)lab * 2654435761u;
+    for (int probe = 0; probe < SLOTS; ++probe) {  // (at most 1,024 keys in 2,048 slots)
+        const int i = (int)((hs + (uint32_t)probe) & (SLOTS - 1));
+        const int32_t old = atomicCAS(&key[i], -1, lab);
+        if (old == -1 || old == lab) {
+            atomicAdd(&cnt[i], 1);
+            break;
         }
-    if (t == 0 || s[t] != s[t - 1]) {
-        int len = 1;
-        while (t + len < 1024 && s[t + len] == s[t]) ++len;
-        atomicMax(&best, ((unsigned long long)len << 32) | (uint32_t)s[t]);
     }
+    __syncthreads();
+    for (int i = t; i < SLOTS; i += blockDim.x)
+        if (cnt[i]) atomicMax(&best, ((unsigned long long)cnt[i] << 32) | (uint32_t)key[i]);
     __syncthreads();
     if (t == 0) out[0] = (int64_t)(uint32_t)(best & 0xffffffffULL);
 }
@@ -146,7 +150,7 @@ void runWcc(girhip_graph* h) {
     launchHeavyStore(h, "wcc_hook_rest_heavy", f);
     launch(h, "wcc_compress", kWccCompress, g, blk, 0, comp, V);
     fillI64(h, h->dcnt.p, 0, 1);
-    launch(h, "count_roots", kCountSelf, boundedGrid(V, blk), blk, 0, (const int32_t*)comp, V, h->dcnt.p);
+    launch(h, "count_roots", kCountSelf, reduceGrid(V, blk), blk, 0, (const int32_t*)comp, V, h->dcnt.p);
     int64_t n;
     readCounters(h, h->dcnt.p, 1, &n);
     h->n_wcc = (int32_t)n;
@@ -169,7 +173,7 @@ void runWccHookArcs(girhip_graph* h, const uint64_t* keys, int64_t n, int B) {
     launch(h, "wcc_hook_keys", kWccHookKeys, gridFor(n, blk), blk, 0, keys, n, B, comp);
     launch(h, "wcc_compress", kWccCompress, gridFor(V, blk), blk, 0, comp, V);
     fillI64(h, h->dcnt.p, 0, 1);
-    launch(h, "count_roots", kCountSelf, boundedGrid(V, blk), blk, 0, (const int32_t*)comp, V, h->dcnt.p);
+    launch(h, "count_roots", kCountSelf, reduceGrid(V, blk), blk, 0, (const int32_t*)comp, V, h->dcnt.p);
     int64_t c;
     readCounters(h, h->dcnt.p, 1, &c);
     h->n_wcc = (int32_t)c;
