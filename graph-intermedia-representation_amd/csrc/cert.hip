@@ -220,7 +220,7 @@ CertStats certRepair(girhip_graph* h, int64_t ncutf, int64_t ncutb, int64_t ncan
                        (const uint64_t*)orph, R.fbits);
                 fillI64(h, c + GC_BU, 0, 2);
                 const CertOrphans f{h->scc.p, g, orph};
-                launch(h, "cert_boundary", kBottomUp<CertOrphans, WriteParent>, gridFor(V, blk), blk, 0, rb, ra, pb,
+                launch(h, "cert_boundary", kBottomUp<CertOrphans, WriteParent>, buGrid(V, blk), blk, 0, rb, ra, pb,
                        R.vis, (const uint64_t*)R.fbits, R.nbits, V, 0, R.q[0], c + GC_BU, c + GC_BUZ, f,
                        WriteParent{par});
                 int64_t r;

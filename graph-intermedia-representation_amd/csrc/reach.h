@@ -110,74 +110,108 @@ __global__ void kTopDown(const int32_t* __restrict__ qin, const int64_t* nin, To
 }
 
 constexpr int32_t BU_SMALL = 8;
+// A bottom-up block walks BU_TILES consecutive tiles of blockDim vertices,
+// gathering what it finds in LDS, and reserves its queue slots once: the
+// first bottom-up step of a sweep finds vertices in nearly every block, and
+// one counter hit by every block of a V-wide grid serialises thousands of
+// atomics.
+constexpr int BU_TILES = 4;
+inline unsigned buGrid(int64_t V, int block) { return gridFor(V, block * BU_TILES); }
 
 template <class Flt, class Vis>
-__global__ void kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __restrict__ radj,
-                          const int32_t* __restrict__ pbeg, uint64_t* vis, const uint64_t* __restrict__ fbits,
-                          uint64_t* __restrict__ nbits, int32_t V, int32_t L, int32_t* __restrict__ qout, int64_t* cnt,
-                          int64_t* cz, Flt flt, Vis on) {
+__global__ void __launch_bounds__(1024) kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __restrict__ radj,
+                                                  const int32_t* __restrict__ pbeg, uint64_t* vis,
+                                                  const uint64_t* __restrict__ fbits, uint64_t* __restrict__ nbits,
+                                                  int32_t V, int32_t L, int32_t* __restrict__ qout, int64_t* cnt,
+                                                  int64_t* cz, Flt flt, Vis on) {
+    __shared__ int32_t lst[BU_TILES * 1024];
+    __shared__ int lcount;
+    __shared__ int64_t wm[1024 / WAVE];
+    __shared__ int64_t base_s;
     if (blockIdx.x == 0 && threadIdx.x < 2) cz[threadIdx.x] = 0;
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t word = v >> 6;
-    const bool word_ok = (word << 6) < V;
-    const uint64_t w0 = word_ok ? vis[word] : ~0ULL;
-    const bool act = v < V && !((w0 >> laneId()) & 1) && flt.self((int32_t)v);
-    bool found = false;
-    int32_t par = -1;  // the frontier vertex found (kept only for visitors that record parents)
-    // each lane scans the first BU_SMALL slots of its own row; rows with
-    // slots left and no parent yet are then scanned by the whole wave, one
-    // row at a time, 64 coalesced slots per step with a ballot exit (a hub's
-    // long row no longer holds the other 63 lanes)
-    const int32_t b = act ? rbeg[v] : 0, e = act ? rbeg[v + 1] : 0;
-    const int32_t lim = e - b > BU_SMALL ? b + BU_SMALL : e;
-    for (int32_t j = b; j < lim; ++j) {
-        const int32_t p = radj[j];
-        if (((fbits[p >> 6] >> (p & 63)) & 1) && flt((int32_t)v, p)) {
-            found = true;
-            if constexpr (Vis::kParent) par = p;
-            break;
-        }
-    }
-    uint64_t rest = __ballot(act && !found && lim < e);
-    while (rest) {
-        const int l = __ffsll((long long)rest) - 1;
-        rest &= rest - 1;
-        const int32_t rv = __shfl((int32_t)v, l), rb = __shfl(lim, l), re = __shfl(e, l);
-        bool hit = false;
-        int32_t hp = -1;
-        for (int32_t j0 = rb; j0 < re; j0 += WAVE) {
-            const int32_t j = j0 + laneId();
-            bool hj = false;
-            int32_t pj = -1;
-            if (j < re) {
-                pj = radj[j];
-                hj = ((fbits[pj >> 6] >> (pj & 63)) & 1) && flt(rv, pj);
-            }
-            const uint64_t hb = __ballot(hj);
-            if (hb) {
-                hit = true;
-                if constexpr (Vis::kParent) hp = __shfl(pj, __ffsll((long long)hb) - 1);
+    if (threadIdx.x == 0) lcount = 0;
+    __syncthreads();
+    int64_t msum = 0;
+    for (int tile = 0; tile < BU_TILES; ++tile) {
+        const int64_t v = ((int64_t)blockIdx.x * BU_TILES + tile) * blockDim.x + threadIdx.x;
+        const int64_t word = v >> 6;
+        const bool word_ok = (word << 6) < V;
+        const uint64_t w0 = word_ok ? vis[word] : ~0ULL;
+        const bool act = v < V && !((w0 >> laneId()) & 1) && flt.self((int32_t)v);
+        bool found = false;
+        int32_t par = -1;  // the frontier vertex found (kept only for visitors that record parents)
+        // each lane scans the first BU_SMALL slots of its own row; rows with
+        // slots left and no parent yet are then scanned by the whole wave,
+        // one row at a time, 64 coalesced slots per step with a ballot exit
+        // (a hub's long row no longer holds the other 63 lanes)
+        const int32_t b = act ? rbeg[v] : 0, e = act ? rbeg[v + 1] : 0;
+        const int32_t lim = e - b > BU_SMALL ? b + BU_SMALL : e;
+        for (int32_t j = b; j < lim; ++j) {
+            const int32_t p = radj[j];
+            if (((fbits[p >> 6] >> (p & 63)) & 1) && flt((int32_t)v, p)) {
+                found = true;
+                if constexpr (Vis::kParent) par = p;
                 break;
             }
         }
-        if (laneId() == l && hit) {
-            found = true;
-            par = hp;
+        uint64_t rest = __ballot(act && !found && lim < e);
+        while (rest) {
+            const int l = __ffsll((long long)rest) - 1;
+            rest &= rest - 1;
+            const int32_t rv = __shfl((int32_t)v, l), rb = __shfl(lim, l), re = __shfl(e, l);
+            bool hit = false;
+            int32_t hp = -1;
+            for (int32_t j0 = rb; j0 < re; j0 += WAVE) {
+                const int32_t j = j0 + laneId();
+                bool hj = false;
+                int32_t pj = -1;
+                if (j < re) {
+                    pj = radj[j];
+                    hj = ((fbits[pj >> 6] >> (pj & 63)) & 1) && flt(rv, pj);
+                }
+                const uint64_t hb = __ballot(hj);
+                if (hb) {
+                    hit = true;
+                    if constexpr (Vis::kParent) hp = __shfl(pj, __ffsll((long long)hb) - 1);
+                    break;
+                }
+            }
+            if (laneId() == l && hit) {
+                found = true;
+                par = hp;
+            }
         }
-    }
-    const uint64_t nb = __ballot(found);
-    if (laneId() == 0 && word_ok) {
-        nbits[word] = nb;
-        if (nb) vis[word] = w0 | nb;
+        const uint64_t nb = __ballot(found);
+        if (laneId() == 0 && word_ok) {
+            nbits[word] = nb;
+            if (nb) vis[word] = w0 | nb;
+        }
+        if (found) msum += pbeg[v + 1] - pbeg[v];
+        int slot = 0;
+        if (laneId() == 0 && nb) slot = atomicAdd(&lcount, __popcll(nb));
+        slot = __shfl(slot, 0);
+        if (found) {
+            lst[slot + __popcll(nb & lanesBelow())] = (int32_t)v;
+            on((int32_t)v, L + 1, par);
+        }
     }
     // cnt[0] gets (found << 32) + their push-arc count, one atomic per block;
     // the host splits it into the level's (nf, mf) pair
-    const int64_t m = found ? pbeg[v + 1] - pbeg[v] : 0;
-    const int64_t pos = blockReservePacked(found, m, &cnt[0]);
-    if (found) {
-        qout[pos] = (int32_t)v;
-        on((int32_t)v, L + 1, par);
+    msum = waveSum(msum);
+    if (laneId() == 0) wm[threadIdx.x / WAVE] = msum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t m = 0;
+        for (int i = 0; i < (int)(blockDim.x / WAVE); ++i) m += wm[i];
+        const int n = lcount;
+        base_s = (n || m) ? (int64_t)(atomicAdd((unsigned long long*)&cnt[0],
+                                                ((unsigned long long)n << 32) + (unsigned long long)m) >>
+                                      32)
+                          : 0;
     }
+    __syncthreads();
+    const int n = lcount;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) qout[base_s + i] = lst[i];
 }
 
 static __global__ void kQueueToBits(const int32_t* __restrict__ q, int64_t n, uint64_t* bits) {
@@ -340,7 +374,7 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
                 launch(h, "reach_q2bits", kQueueToBits, gridFor(nf, blk), blk, 0, (const int32_t*)B.q[cur], nf,
                        B.fbits);
             }
-            launch(h, "reach_bottomup", kBottomUp<Flt, Vis>, gridFor(V, blk), blk, 0, rbeg, radj, pbeg, B.vis,
+            launch(h, "reach_bottomup", kBottomUp<Flt, Vis>, buGrid(V, blk), blk, 0, rbeg, radj, pbeg, B.vis,
                    (const uint64_t*)B.fbits, B.nbits, V, L, B.q[cur ^ 1], cn, pair(L + 1), flt, on);
             std::swap(B.fbits, B.nbits);
             bits_valid = true;
