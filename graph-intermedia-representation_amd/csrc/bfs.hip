@@ -328,31 +328,36 @@ __global__ void kMsPull(const int32_t* __restrict__ bb, const int32_t* __restric
 
 // Dense finish over all V: new bits -> seen, frontier words, level writes;
 // every F word is rewritten (one wave = one word), F's summary was cleared
-// before the launch.
+// before the launch.  A block walks MF_TILES tiles and adds its counts once
+// (one counter hit by every block of a V-wide grid serialises them).
+constexpr int MF_TILES = 16;
 __global__ void kMsFinish(uint64_t* seen, uint64_t* front, uint64_t* next, int32_t* __restrict__ levels, int32_t V,
                           int32_t L, const int32_t* __restrict__ fb, MsBits M, MsCnt* nxt) {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t nb = 0;
-    if (w < V) {
-        const uint64_t sw = seen[w];
-        nb = next[w] & ~sw;
-        seen[w] = sw | nb;
-        front[w] = nb;
-        next[w] = 0;
+    int64_t packed = 0;  // (new vertices << 32) + their push-arc count
+    for (int t = 0; t < MF_TILES; ++t) {
+        const int64_t w = ((int64_t)blockIdx.x * MF_TILES + t) * blockDim.x + threadIdx.x;
+        uint64_t nb = 0;
+        if (w < V) {
+            const uint64_t sw = seen[w];
+            nb = next[w] & ~sw;
+            seen[w] = sw | nb;
+            front[w] = nb;
+            next[w] = 0;
+        }
+        const uint64_t fw = __ballot(nb != 0);
+        if (fw) {
+            for (int s = 0; s < 64; ++s)
+                if ((nb >> s) & 1) levels[(size_t)s * V + w] = L + 1;
+        }
+        const int64_t word = w >> 6;
+        if (laneId() == 0 && word < M.W) {
+            M.F[word] = fw;
+            if (fw) atomicOr((unsigned long long*)&M.FS[word >> 6], 1ULL << (word & 63));
+        }
+        if (nb) packed += (1LL << 32) + (int64_t)(fb[w + 1] - fb[w]);
     }
-    const uint64_t fw = __ballot(nb != 0);
-    if (fw) {
-        for (int s = 0; s < 64; ++s)
-            if ((nb >> s) & 1) levels[(size_t)s * V + w] = L + 1;
-    }
-    const int64_t word = w >> 6;
-    if (laneId() == 0 && word < M.W) {
-        M.F[word] = fw;
-        if (fw) atomicOr((unsigned long long*)&M.FS[word >> 6], 1ULL << (word & 63));
-    }
-    // nxt->nf gets (new vertices << 32) + their push-arc count, one atomic
-    // per block; the host splits it into (nf, mf)
-    blockReservePacked(nb != 0, nb != 0 ? (int64_t)(fb[w + 1] - fb[w]) : 0, &nxt->nf);
+    // the host splits nxt->nf into (nf, mf)
+    blockAtomicAdd(packed, &nxt->nf);
 }
 
 __global__ void kMsUnreached(const uint64_t* __restrict__ seen, int32_t* __restrict__ levels, int32_t V,
@@ -439,7 +444,8 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
                 launchHeavyStore(h, "msbfs_push_heavy", f);
             }
             fillU64(h, M.FS, 0, M.W2);
-            launch(h, "msbfs_finish", kMsFinish, g, blk, 0, seen, front, next, levels, V, L, fb, M, nxt);
+            launch(h, "msbfs_finish", kMsFinish, gridFor(V, blk * MF_TILES), blk, 0, seen, front, next, levels, V, L,
+                   fb, M, nxt);
             int64_t r[2];
             readCounters(h, (const int64_t*)nxt, 2, r);
             nf = (int64_t)((uint64_t)r[0] >> 32);
