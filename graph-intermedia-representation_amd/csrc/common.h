@@ -103,6 +103,7 @@ struct Params {
     double bfs_alpha_auto = 1;   // 1: the per-source reach raises bfs_alpha to 64 on graphs with hubs
     double update_bypass = 1;    // 1: batches of few deletes test them for a bypass path first
     double scc_cert = 1;         // 1: update batches check the big SCC against its spanning trees
+    double repair_ops = 0;       // largest batch (changes) the SCC layer repairs; 0: max(4096, V / 256)
 };
 
 struct KStat {

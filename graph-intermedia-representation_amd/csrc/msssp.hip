@@ -392,17 +392,26 @@ __device__ __forceinline__ void drainAll(uint64_t* bits, uint64_t* sum, uint32_t
                                          uint32_t* qm, int64_t* cnt, int32_t* lds) {
     const int lane = laneId(), wpb = blockDim.x / WAVE, wib = threadIdx.x / WAVE;
     const int64_t T = W2 * TPS;
-    for (int64_t t0 = (int64_t)blockIdx.x * wpb; t0 < T; t0 += (int64_t)gridDim.x * wpb) {
+    const int64_t t_first = (int64_t)blockIdx.x * wpb, t_step = (int64_t)gridDim.x * wpb;
+    // the block's queue slots are reserved once: a first walk counts the set
+    // bits of the wave's tasks, the second drains them
+    int64_t mine_n = 0;
+    for (int64_t t0 = t_first; t0 < T; t0 += t_step) {
         const Task k = taskAt(sum, t0 + wib, T);
+        uint64_t mw = 0;
+        if (lane < TW && ((k.sub >> lane) & 1)) mw = bits[k.g * 64 + k.c * TW + lane];
+        mine_n += waveSum(__popcll(mw));
+    }
+    int64_t base = __shfl(blockReserve(lane == 0 ? (int)mine_n : 0, cnt), 0);
+    for (int64_t t0 = t_first; t0 < T; t0 += t_step) {
+        const Task k = taskAt(sum, t0 + wib, T);
+        if (!k.sub) continue;
         uint64_t mw = 0;  // lane l < TW: word l of the task
         if (lane < TW && ((k.sub >> lane) & 1)) {
             uint64_t* wp = bits + k.g * 64 + k.c * TW + lane;
             mw = *wp;
             if (mw) *wp = 0;
         }
-        const int n = (int)waveSum(__popcll(mw));
-        int64_t base = __shfl(blockReserve(lane == 0 ? n : 0, cnt), 0);
-        if (!k.sub) continue;
         for (int grp = 0; grp < TW / GW; ++grp) {
             const uint64_t m = __shfl(mw, grp * GW + (lane & (GW - 1)));
             const uint64_t mine = lane < GW ? m : 0;
@@ -518,9 +527,12 @@ __global__ void kMsSplit(MsState S, int par, uint64_t delta) {
         });
     };
     const int64_t T_ = S.W2 * TPS;
-    for (int64_t t0 = (int64_t)blockIdx.x * wpb; t0 < T_; t0 += (int64_t)gridDim.x * wpb) {
+    const int64_t t_first = (int64_t)blockIdx.x * wpb, t_step = (int64_t)gridDim.x * wpb;
+    // a first walk over the wave's tasks counts what goes to the queue, so
+    // the block reserves its slots once; the second walk writes them
+    int ntake = 0;
+    for (int64_t t0 = t_first; t0 < T_; t0 += t_step) {
         const Task k = taskAt(S.fsum, t0 + wib, T_);
-        int ntake = 0;
         for (int grp = 0; k.sub && grp < TW / GW; ++grp) {
             if (__ballot(k.lane(grp)) == 0) continue;
             const uint64_t m = k.lane(grp) ? S.fbits[k.word(grp)] : 0;
@@ -533,7 +545,10 @@ __global__ void kMsSplit(MsState S, int par, uint64_t delta) {
             }
             waveSync();
         }
-        int64_t base = __shfl(blockReserve(lane == 0 ? ntake : 0, cnt), 0);
+    }
+    int64_t base = __shfl(blockReserve(lane == 0 ? ntake : 0, cnt), 0);
+    for (int64_t t0 = t_first; t0 < T_; t0 += t_step) {
+        const Task k = taskAt(S.fsum, t0 + wib, T_);
         if (!k.sub) continue;
         uint64_t emptied = 0;
         for (int grp = 0; grp < TW / GW; ++grp) {
