@@ -1016,6 +1016,16 @@ __device__ __forceinline__ bool markBit(uint64_t* bits, uint64_t* sum, int32_t x
     return true;
 }
 
+// markBit with the half of the summary word that holds x's bit
+// (((const uint32_t*)sum)[x >> 11], little-endian), for callers short of
+// registers.
+__device__ __forceinline__ bool markBitHalf(uint64_t* bits, uint64_t* sum, int32_t x, uint32_t hw) {
+    atomicOr((unsigned long long*)&bits[x >> 6], 1ULL << (x & 63));
+    if ((hw >> ((x >> 6) & 31)) & 1) return false;
+    atomicOr((unsigned long long*)&sum[x >> 12], 1ULL << ((x >> 6) & 63));
+    return true;
+}
+
 // markBit for one vertex per lane (lanes with act), for vertex lists in
 // which equal bitmap words sit in consecutive lanes (a queue compacted in
 // vertex order): the lanes of a run with the same word OR their bits

@@ -184,18 +184,18 @@ struct MsRelax {
                          const int64_t (&j)[U], const bool (&ok)[U]) const {
         static_assert(kVecSeg<T, KS>, "batched relaxation needs segments of at most 32 bytes");
         int32_t x[U];
-        uint64_t wt[U];
+        uint32_t wt[U];
         bool go[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             go[u] = ok[u];
             x[u] = go[u] ? fa[j[u]] : 0;
-            wt[u] = (go[u] && w) ? (uint64_t)w[j[u]] : 0;
+            wt[u] = (go[u] && w) ? w[j[u]] : 0;
         }
         const uint64_t hi = (uint64_t)*S.c(M_HI);
         T* dist = (T*)S.dist;
         T du[U][KS], dx[U][KS];
-        uint64_t nsw[U], fsw[U];
+        uint32_t nsw[U], fsw[U];  // the halves of the summary words that hold x's bits
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (go[u] && !w) wt[u] = hashWeight(p[u], x[u]);
@@ -204,8 +204,8 @@ struct MsRelax {
             if (go[u]) {
                 loadSeg<T, KS>(dist + (size_t)p[u] * KS, du[u]);
                 loadSeg<T, KS>(dist + (size_t)x[u] * KS, dx[u]);
-                nsw[u] = S.nsum[x[u] >> 12];
-                fsw[u] = S.fsum[x[u] >> 12];
+                nsw[u] = ((const uint32_t*)S.nsum)[x[u] >> 11];
+                fsw[u] = ((const uint32_t*)S.fsum)[x[u] >> 11];
             }
         }
         bool ovf = false, nset = false;
@@ -218,24 +218,35 @@ struct MsRelax {
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 if (!((m >> s) & 1)) continue;
-                const uint64_t nd = widen(du[u][s]) + wt[u];
                 if constexpr (sizeof(T) == 4) {
-                    if (nd >= 0xffffffffull) {
+                    // du is finite (s is active at the tail): the 32-bit sum
+                    // overflows iff it wraps or reaches the unreached mark,
+                    // and otherwise compares with dx directly (unreached =
+                    // 0xffffffff is above every finite sum)
+                    const uint32_t nd = du[u][s] + wt[u];
+                    if (nd < du[u][s] || nd == 0xffffffffu) {
                         ovf = true;
                         continue;
                     }
-                }
-                if (nd < widen(dx[u][s])) {
-                    atomicMinT(px + s, (T)nd);
-                    if (nd < hi) near |= 1u << s;
-                    else far |= 1u << s;
+                    if (nd < dx[u][s]) {
+                        atomicMinT(px + s, (T)nd);
+                        if ((uint64_t)nd < hi) near |= 1u << s;
+                        else far |= 1u << s;
+                    }
+                } else {
+                    const uint64_t nd = widen(du[u][s]) + wt[u];
+                    if (nd < widen(dx[u][s])) {
+                        atomicMinT(px + s, (T)nd);
+                        if (nd < hi) near |= 1u << s;
+                        else far |= 1u << s;
+                    }
                 }
             }
             if (near) {
                 atomicOr(&S.nmask[x[u]], near);
-                nset |= markBit(S.nbits, S.nsum, x[u], nsw[u]);
+                nset |= markBitHalf(S.nbits, S.nsum, x[u], nsw[u]);
             }
-            if (far) markBit(S.fbits, S.fsum, x[u], fsw[u]);
+            if (far) markBitHalf(S.fbits, S.fsum, x[u], fsw[u]);
         }
         if (ovf) *S.c(M_OVF) = 1;
         if (nset) *S.c(par ? M_NSET1 : M_NSET0) = 1;

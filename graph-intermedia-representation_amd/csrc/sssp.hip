@@ -219,15 +219,16 @@ struct RelaxF {
                          const int64_t (&j)[U], const bool (&ok)[U]) const {
         const int s = sv[0];  // one source per wave
         int32_t x[U];
-        uint64_t wt[U];
+        uint32_t wt[U];
         bool go[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             go[u] = ok[u];
             x[u] = go[u] ? fa[j[u]] : 0;
-            wt[u] = (go[u] && !unit && w) ? (uint64_t)w[j[u]] : 1;
+            wt[u] = (go[u] && !unit && w) ? w[j[u]] : 1;
         }
-        uint64_t dx[U], nsw[U], fsw[U];
+        uint64_t dx[U];
+        uint32_t nsw[U], fsw[U];  // the halves of the summary words that hold x's bits
         const uint64_t hi = (uint64_t)*cp(B, C_HI, s);
         uint64_t* nb = B.nearbits + (size_t)s * B.W;
         uint64_t* ns = B.nsum + (size_t)s * B.W2;
@@ -240,8 +241,8 @@ struct RelaxF {
             if (go[u] && !unit && !mid && ((wt[u] > delta) != (heavy != 0))) go[u] = false;
             if (go[u] && scope && scope[x[u]] != scope[p[u]]) go[u] = false;
             dx[u] = go[u] ? dGet(B, s, x[u]) : 0;
-            nsw[u] = go[u] ? ns[x[u] >> 12] : 0;
-            fsw[u] = go[u] ? fs[x[u] >> 12] : 0;
+            nsw[u] = go[u] ? ((const uint32_t*)ns)[x[u] >> 11] : 0;
+            fsw[u] = go[u] ? ((const uint32_t*)fs)[x[u] >> 11] : 0;
         }
         bool ovf = false, nset = false;
 #pragma unroll
@@ -254,8 +255,8 @@ struct RelaxF {
             }
             if (nd < dx[u]) {  // (dx: a plain read, safe as a pre-check as distances only fall)
                 dMinNoRet(B, s, x[u], nd);
-                if (nd < hi) nset |= markBit(nb, ns, x[u], nsw[u]);
-                else markBit(fbits, fs, x[u], fsw[u]);
+                if (nd < hi) nset |= markBitHalf(nb, ns, x[u], nsw[u]);
+                else markBitHalf(fbits, fs, x[u], fsw[u]);
             }
         }
         if (ovf) B.ctl[C_OVF] = 1;
