@@ -690,12 +690,16 @@ bool msRun(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, const gi
     const int cmax = std::max(1, std::min(MS_CHUNK, (int)h->prm.chunk));
     int par = 0, R = 1;
     int64_t maxn = 1;
+    // a chunk's first round relaxes the queue read back (one lane per entry);
+    // the queues of its later rounds are unknown and may be large (a bucket
+    // that drains hands the next one its far entries), so those launches
+    // get enough waves to fill the chip
+    const unsigned gfill = std::min<unsigned>(gridFor(V, blk), (unsigned)std::max(1, (1 << 18) / blk));
     while (true) {
-        const unsigned rgrid = R > 1 ? std::min<unsigned>(1024, std::max<unsigned>(64, gridFor(4 * maxn, blk)))
-                                     : gridFor(maxn, blk);
+        const unsigned g0 = gridFor(maxn, blk), glater = std::max(g0, gfill);
         int p = par;
         for (int t = 0; t < R; ++t) {
-            launch(h, "sssp_ms_relax", relaxK, rgrid, blk, 0, fl[p], fb, mid, H, heavy, 0);
+            launch(h, "sssp_ms_relax", relaxK, t == 0 ? g0 : glater, blk, 0, fl[p], fb, mid, H, heavy, 0);
             launchHeavyStore(h, "sssp_ms_relax_rows", fl[p]);
             launch(h, "sssp_ms_compact", kMsCompact<T, KS>, sgrid, blk, 0, S, p, heavy);
             if (heavy) {
