@@ -147,10 +147,8 @@ struct Scratch {
     // done source masks, their vertex bitmaps and summaries, round queues
     // (vertex, mask) by parity and the heavy queue, control slots
     DevBuf<uint8_t> ms_dist;
-    DevBuf<uint32_t> ms_mask[3];
     DevBuf<uint64_t> ms_bits[3], ms_sum[3];
     DevBuf<int32_t> ms_q[3];
-    DevBuf<uint32_t> ms_qm[3];
     DevBuf<int64_t> ms_ctl;
     DevBuf<uint64_t> gbits;  // condensation keys: one bit per vertex, set iff its label is the big one
     DevBuf<uint64_t> bp_bits;  // delete bypass check: one bitmap per delete

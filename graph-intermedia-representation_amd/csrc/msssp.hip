@@ -13,23 +13,23 @@
 //   - buckets are common: every source processes bucket [lo, lo + delta)
 //     at the same time, so a vertex whose distances from several sources
 //     fall in the same bucket is visited once for all of them;
-//   - the work lists carry a source mask per vertex: near / done masks
-//     (uint32 per vertex) say for which sources a vertex entered the bucket
-//     this round or was processed in this bucket; one bitmap per set (with a
-//     summary bit per 64-bit word) lists the vertices whose mask is
-//     non-zero.  The far set (vertices waiting in a later bucket) is a bitmap
-//     only: its sources are read off the distances when it is walked (a
-//     finite distance at or past the bucket's end is a far entry, since a
-//     relaxation that set it marked the vertex), so a far relaxation costs
-//     the bitmap's atomicOr and no mask update.
+//   - the work sets are bitmaps (with a summary bit per 64-bit word): near
+//     (entered the bucket this round), done (processed in this bucket) and
+//     far (waiting in a later bucket).  No per-vertex source masks are
+//     kept: a queued vertex is relaxed for every source whose distance lies
+//     in the current bucket (read off its distance segment, which the
+//     relaxation loads anyway), which covers every source that lowered it;
+//     the far set's sources are likewise read off the distances (a finite
+//     distance at or past the bucket's end is a far entry, since the
+//     relaxation that set it marked the vertex).  A relaxation that lowers
+//     a distance thus costs the atomicMin and one bitmap atomicOr.
 // Round (same phases as the per-source engine in sssp.hip):
 //   relax the queue's light arcs -> compact (near bitmap -> next queue; or,
 //   when the bucket drained, done bitmap -> heavy queue, or the far minimum)
 //   [-> relax heavy arcs once -> far minimum] -> split (the bucket of the
 //   smallest far distance; far entries inside it -> next queue).
 // A relaxation pre-checks the distance with a plain load and then issues an
-// atomicMin; marks are one atomicOr on the head's mask per arc and set of
-// improved sources.  32-bit distances first (a batch is redone in 64 bits
+// atomicMin.  32-bit distances first (a batch is redone in 64 bits
 // if one would reach 2^32 - 1).
 //
 // HBM traffic per source (algorithmic, the same contract as the reference's
@@ -50,13 +50,10 @@ constexpr int MSTR = 16;
 
 struct MsState {
     void* dist;  // [V][KS] of T
-    uint32_t *nmask, *dmask;
     uint64_t *nbits, *fbits, *dbits;
     uint64_t *nsum, *fsum, *dsum;
     int32_t* q[2];
-    uint32_t* qm[2];
     int32_t* qh;
-    uint32_t* qhm;
     int64_t* ctl;
     int64_t V, W, W2;
     __device__ int64_t* c(int slot) const { return ctl + slot * MSTR; }
@@ -113,17 +110,14 @@ struct MsRelax {
     int heavy;
     int par;
     int split;
-    // The mask and the vertex's bitmap bit are both set (each behind a plain
-    // pre-check read: within a kernel masks and bits only grow, so a stale
-    // read costs at most a redundant atomic); neither atomic's old value is
-    // awaited, as the compaction that reads them is a later launch.
+    // The vertex's bitmap bit is set behind a plain pre-check read (within a
+    // kernel bits only grow, so a stale read costs at most a redundant
+    // atomic).  Which sources improved is not recorded: a queued vertex is
+    // relaxed for every source whose distance lies in the bucket.
     __device__ __forceinline__ void mark(int32_t x, uint32_t near, uint32_t far) const {
-        if (near) {
-            if ((S.nmask[x] & near) != near) atomicOr(&S.nmask[x], near);
-            // the near bitmap is empty at the start of every round: the first
-            // bit of a word raises the round's flag
-            if (setBit(S.nbits, S.nsum, x)) *S.c(par ? M_NSET1 : M_NSET0) = 1;
-        }
+        // the near bitmap is empty at the start of every round: the first
+        // bit of a word raises the round's flag
+        if (near && setBit(S.nbits, S.nsum, x)) *S.c(par ? M_NSET1 : M_NSET0) = 1;
         if (far) setBit(S.fbits, S.fsum, x);
     }
     __device__ int64_t operator()(int32_t u, uint64_t pay, int32_t, int64_t j) const {
@@ -242,10 +236,7 @@ struct MsRelax {
                     }
                 }
             }
-            if (near) {
-                atomicOr(&S.nmask[x[u]], near);
-                nset |= markBitHalf(S.nbits, S.nsum, x[u], nsw[u]);
-            }
+            if (near) nset |= markBitHalf(S.nbits, S.nsum, x[u], nsw[u]);
             if (far) markBitHalf(S.fbits, S.fsum, x[u], fsw[u]);
         }
         if (ovf) *S.c(M_OVF) = 1;
@@ -260,8 +251,8 @@ __device__ __forceinline__ void resetRound(const MsState& S, int par) {
     *S.c(M_HQ) = 0;
 }
 
-// The k sources: distance 0 and a first queue of distinct vertices with
-// their source masks (a vertex given twice gets both bits).
+// The k sources: distance 0 and a first queue of distinct vertices (a
+// vertex given twice is queued once, with both distances 0).
 template <class T, int KS>
 __global__ void kMsInit(MsState S, const int32_t* __restrict__ src, int k, uint64_t delta) {
     if (threadIdx.x != 0) return;
@@ -274,20 +265,40 @@ __global__ void kMsInit(MsState S, const int32_t* __restrict__ src, int k, uint6
         ((T*)S.dist)[(size_t)x * KS + s] = 0;
         int64_t i = 0;
         while (i < n && S.q[0][i] != x) ++i;
-        if (i == n) {
-            S.q[0][n] = x;
-            S.qm[0][n] = 0;
-            ++n;
-        }
-        S.qm[0][i] |= 1u << s;
+        if (i == n) S.q[0][n++] = x;
     }
     *S.c(M_NEAR0) = n;
 }
 
+// The sources whose distance at vertex u lies in [lo, hi): a queued vertex
+// is relaxed for these (in a light round: the bucket's sources, among them
+// every source that lowered u's distance last round; in the heavy pass: the
+// sources u settled for in this bucket).
+template <class T, int KS>
+__device__ __forceinline__ uint32_t bucketMask(const T* dist, int32_t u, uint64_t lo, uint64_t hi) {
+    const T* p = dist + (size_t)u * KS;
+    uint32_t m = 0;
+    if constexpr (kVecSeg<T, KS>) {
+        T d[KS];
+        loadSeg<T, KS>(p, d);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const uint64_t v = widen(d[s]);
+            if (v >= lo && v < hi) m |= 1u << s;
+        }
+    } else {
+        for (int s = 0; s < KS; ++s) {
+            const uint64_t v = widen(p[s]);
+            if (v >= lo && v < hi) m |= 1u << s;
+        }
+    }
+    return m;
+}
+
 // Relax the queue of parity par (light arcs), or with heavy_phase the heavy
-// queue (heavy arcs); grid stride, one lane per queue entry.  track_done:
-// the entries' sources join the vertex's done mask (its heavy arcs are due
-// when the bucket drains).
+// queue (heavy arcs); grid stride, one lane per queue entry, for the
+// sources of the entry's bucketMask.  track_done: the entries join the done
+// set (their heavy arcs are due when the bucket drains).
 template <class T, int KS, int U>
 __global__ void kMsRelax(MsRelax<T, KS> f, const int32_t* __restrict__ fb, const int32_t* __restrict__ mid,
                          HeavyList H, int track_done, int heavy_phase) {
@@ -295,17 +306,14 @@ __global__ void kMsRelax(MsRelax<T, KS> f, const int32_t* __restrict__ fb, const
     if (!heavy_phase && blockIdx.x == 0 && threadIdx.x == 0) resetRound(S, f.par);
     const int64_t n = heavy_phase ? *S.c(M_HQ) : *S.c(f.par ? M_NEAR1 : M_NEAR0);
     const int32_t* q = heavy_phase ? S.qh : S.q[f.par];
-    const uint32_t* qm = heavy_phase ? S.qhm : S.qm[f.par];
+    const uint64_t lo = (uint64_t)*S.c(M_LO), hi = (uint64_t)*S.c(M_HI);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
         const int64_t i = base + laneId();
         const bool act = i < n;
         const int32_t u = act ? q[i] : 0;
-        const uint32_t m = act ? qm[i] : 0;
-        if (track_done) {
-            if (act) atomicOr(&S.dmask[u], m);
-            waveMarkBits(S.dbits, S.dsum, u, act);
-        }
+        const uint32_t m = act ? bucketMask<T, KS>((const T*)S.dist, u, lo, hi) : 0;
+        if (track_done) waveMarkBits(S.dbits, S.dsum, u, act);
         int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
         if (mid && act) {
             const int32_t c = mid[u];
@@ -398,9 +406,9 @@ __device__ __forceinline__ void forMaskDist(const T* dist, int32_t x, uint32_t f
 }
 
 // Drains a bitmap: clears its words and summary bits, appends the vertices
-// with their masks (then cleared) to q / qm at counter *cnt.
-__device__ __forceinline__ void drainAll(uint64_t* bits, uint64_t* sum, uint32_t* mask, int64_t W2, int32_t* q,
-                                         uint32_t* qm, int64_t* cnt, int32_t* lds) {
+// to q at counter *cnt.
+__device__ __forceinline__ void drainAll(uint64_t* bits, uint64_t* sum, int64_t W2, int32_t* q, int64_t* cnt,
+                                         int32_t* lds) {
     const int lane = laneId(), wpb = blockDim.x / WAVE, wib = threadIdx.x / WAVE;
     const int64_t T = W2 * TPS;
     const int64_t t_first = (int64_t)blockIdx.x * wpb, t_step = (int64_t)gridDim.x * wpb;
@@ -428,12 +436,7 @@ __device__ __forceinline__ void drainAll(uint64_t* bits, uint64_t* sum, uint32_t
             const uint64_t mine = lane < GW ? m : 0;
             if (__ballot(mine != 0) == 0) continue;
             const int total = expandGroup(mine, k.word(grp), lds);
-            for (int i = lane; i < total; i += WAVE) {
-                const int32_t x = lds[i];
-                q[base + i] = x;
-                qm[base + i] = mask[x];
-                mask[x] = 0;
-            }
+            for (int i = lane; i < total; i += WAVE) q[base + i] = lds[i];
             base += total;
             waveSync();
         }
@@ -485,9 +488,9 @@ __global__ void kMsCompact(MsState S, int par, int heavy) {
     __shared__ MsLds sh;
     int32_t* lds = sh.ids[threadIdx.x / WAVE];
     if (ldAcq(S.c(par ? M_NSET1 : M_NSET0)))
-        drainAll(S.nbits, S.nsum, S.nmask, S.W2, S.q[par ^ 1], S.qm[par ^ 1], S.c(par ? M_NEAR0 : M_NEAR1), lds);
+        drainAll(S.nbits, S.nsum, S.W2, S.q[par ^ 1], S.c(par ? M_NEAR0 : M_NEAR1), lds);
     else if (heavy)
-        drainAll(S.dbits, S.dsum, S.dmask, S.W2, S.qh, S.qhm, S.c(M_HQ), lds);
+        drainAll(S.dbits, S.dsum, S.W2, S.qh, S.c(M_HQ), lds);
     else
         farMinAll<T, KS>(S, sh);
 }
@@ -520,7 +523,6 @@ __global__ void kMsSplit(MsState S, int par, uint64_t delta) {
         *S.c(M_HI) = (int64_t)hi;
     }
     int32_t* q = S.q[par ^ 1];
-    uint32_t* qm = S.qm[par ^ 1];
     int64_t* cnt = S.c(par ? M_NEAR0 : M_NEAR1);
     const int lane = laneId(), wpb = blockDim.x / WAVE, wib = threadIdx.x / WAVE;
     int32_t* lds = sh.ids[wib];
@@ -577,11 +579,7 @@ __global__ void kMsSplit(MsState S, int par, uint64_t delta) {
                     if (keep) atomicOr((unsigned long long*)&keepw[(x >> 6) & (GW - 1)], 1ULL << (x & 63));
                 }
                 const uint64_t bal = __ballot(take != 0);
-                if (take) {
-                    const int64_t pos = base + __popcll(bal & lanesBelow());
-                    q[pos] = lds[i];
-                    qm[pos] = take;
-                }
+                if (take) q[base + __popcll(bal & lanesBelow())] = lds[i];
                 base += __popcll(bal);
             }
             waveSync();
@@ -626,17 +624,13 @@ bool msRun(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, const gi
     const size_t Vs = (size_t)std::max<int32_t>(V, 1);
     S_.ms_dist.ensure(sizeof(T) * Vs * KS);
     for (int i = 0; i < 3; ++i) {
-        if (i != 1) S_.ms_mask[i].ensure(Vs);  // (no far masks)
         S_.ms_bits[i].ensure((size_t)std::max<int64_t>(W, 1));
         S_.ms_sum[i].ensure((size_t)std::max<int64_t>(W2, 1));
         S_.ms_q[i].ensure(Vs);
-        S_.ms_qm[i].ensure(Vs);
     }
     S_.ms_ctl.ensure(M_NSLOT * MSTR);
     MsState S;
     S.dist = S_.ms_dist.p;
-    S.nmask = S_.ms_mask[0].p;
-    S.dmask = S_.ms_mask[2].p;
     S.nbits = S_.ms_bits[0].p;
     S.fbits = S_.ms_bits[1].p;
     S.dbits = S_.ms_bits[2].p;
@@ -646,9 +640,6 @@ bool msRun(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, const gi
     S.q[0] = S_.ms_q[0].p;
     S.q[1] = S_.ms_q[1].p;
     S.qh = S_.ms_q[2].p;
-    S.qm[0] = S_.ms_qm[0].p;
-    S.qm[1] = S_.ms_qm[1].p;
-    S.qhm = S_.ms_qm[2].p;
     S.ctl = S_.ms_ctl.p;
     S.V = V;
     S.W = W;
@@ -656,7 +647,6 @@ bool msRun(girhip_graph* h, const int32_t* dsrc, int k, uint64_t delta, const gi
     if constexpr (sizeof(T) == 4) fillU32(h, (uint32_t*)S.dist, 0xffffffffu, (int64_t)V * KS);
     else fillU64(h, (uint64_t*)S.dist, INF, (int64_t)V * KS);
     for (int i = 0; i < 3; ++i) {
-        if (i != 1) fillU32(h, S_.ms_mask[i].p, 0, V);
         fillU64(h, S_.ms_bits[i].p, 0, W);
         fillU64(h, S_.ms_sum[i].p, 0, W2);
     }
