@@ -251,15 +251,14 @@ struct girhip_graph {
     int64_t max_deg = -1;  // largest fw/bw row length; -1 = not yet computed
     int64_t max_w = -1;    // largest stored weight; -1 = not yet computed
     // SSSP views of the forward rows for one bucket width delta (the two
-    // widths used last): each row's arcs reordered by weight class (see
-    // arcClass): those of weight <= delta first, mid[u] = where the heavier
-    // ones start, mid2[u] / mid3[u] = where classes 2 / 3 start; weights
+    // widths used last): each row's arcs reordered so that those of weight
+    // <= delta come first, mid[u] = where the heavier ones start, weights
     // (stored or hash) alongside.  delta 0 = empty; cleared when the store
     // or the weights change.
     struct SplitView {
         uint64_t delta = 0;
         uint64_t used = 0;
-        gir::DevBuf<int32_t> adj, mid, mid2, mid3;
+        gir::DevBuf<int32_t> adj, mid;
         gir::DevBuf<uint32_t> w;
     } split[2];
     uint64_t split_clock = 0;
@@ -1238,14 +1237,6 @@ static __global__ void kI64toI32(const int64_t* __restrict__ in, int32_t* __rest
 }
 
 // The reference's edge weight (gir_query.cpp:50-54): 1 + fmix64(key) mod 100.
-// Weight class of an arc for bucket width delta: 0 = light (w <= delta),
-// c = 1, 2 for delta*c < w <= delta*(c+1), 3 for every heavier arc.
-__device__ __forceinline__ int arcClass(uint32_t w, uint64_t delta) {
-    if (w <= delta) return 0;
-    const uint64_t c = ((uint64_t)w - 1) / delta;
-    return c < 3 ? (int)c : 3;
-}
-
 __device__ __forceinline__ uint32_t hashWeight(int32_t u, int32_t v) {
     uint64_t x = ((uint64_t)(uint32_t)u << 32) | (uint32_t)v;
     x ^= x >> 33;

@@ -265,16 +265,11 @@ struct RelaxF {
 };
 
 // ---- light/heavy split view -------------------------------------------------
-// For bucket width delta, every forward row reordered by weight class: the
-// light arcs (weight <= delta) first, then the heavy ones in classes c = 1,
-// 2, 3 (delta*c < weight <= delta*(c+1); class 3 takes every weight above
-// 3*delta).  mid[u] = where the heavy arcs start, mid2[u] / mid3[u] = where
-// classes 2 / 3 start.  A heavy arc of class c from a vertex settled in
-// bucket b lands in bucket b + c or later, which lets the interleaved engine
-// relax classes 2 and 3 up to c - 1 buckets late, once for the sources
-// settled in those buckets.  Rows of at most SPLIT_SHORT arcs take one
-// thread each (a count pass, then a placing pass); longer rows are listed
-// for kSplitLong, one block per row (block-wide ranks per class).  The
+// For bucket width delta, every forward row reordered so that its arcs of
+// weight <= delta come first; mid[u] = where the heavier ones start.  Rows of
+// at most SPLIT_SHORT arcs take one thread each (light arcs forward from the
+// row start, heavy ones backward from its end); longer rows are listed for
+// kSplitLong, one block per row (block-wide ranks of the light arcs).  The
 // order inside a class is free: distances do not depend on the order in
 // which arcs are relaxed.
 constexpr int SPLIT_SHORT = 64;
@@ -288,12 +283,9 @@ struct SplitArgs {
     int32_t* oadj;
     uint32_t* ow;
     int32_t* mid;
-    int32_t* mid2;
-    int32_t* mid3;
     int32_t* longq;
     int64_t* nlong;
     __device__ uint32_t weight(int32_t u, int64_t j, int32_t x) const { return w ? w[j] : hashWeight(u, x); }
-    __device__ int cls(uint32_t wt) const { return arcClass(wt, delta); }
 };
 
 __global__ void kSplitShort(SplitArgs A) {
@@ -305,87 +297,65 @@ __global__ void kSplitShort(SplitArgs A) {
         const int64_t pos = wavePush(A.nlong, lng);
         if (lng) A.longq[pos] = (int32_t)u;
         if (act && !lng) {
-            int32_t n[4] = {0, 0, 0, 0};
-            for (int64_t j = b; j < e; ++j) ++n[A.cls(A.weight((int32_t)u, j, A.adj[j]))];
-            int64_t o[4];
-            o[0] = b;
-            o[1] = b + n[0];
-            o[2] = o[1] + n[1];
-            o[3] = o[2] + n[2];
-            A.mid[u] = (int32_t)o[1];
-            A.mid2[u] = (int32_t)o[2];
-            A.mid3[u] = (int32_t)o[3];
+            int64_t lo = b, hi = e;
             for (int64_t j = b; j < e; ++j) {
                 const int32_t x = A.adj[j];
                 const uint32_t wt = A.weight((int32_t)u, j, x);
-                const int64_t at = o[A.cls(wt)]++;
-                A.oadj[at] = x;
-                A.ow[at] = wt;
+                const int64_t o = wt <= A.delta ? lo++ : --hi;
+                A.oadj[o] = x;
+                A.ow[o] = wt;
             }
+            A.mid[u] = (int32_t)lo;
         }
     }
 }
 
 __global__ void kSplitLong(SplitArgs A) {
-    __shared__ int32_t wcnt[4][1024 / WAVE];
-    __shared__ int64_t tot[4];
+    __shared__ int32_t wcnt[1024 / WAVE];
+    __shared__ int64_t tot;
     const int64_t n = *A.nlong;
     const int lane = laneId(), wv = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
     for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
         const int32_t u = A.longq[i];
         const int64_t b = A.beg[u], e = A.beg[u + 1];
-        int64_t c[4] = {0, 0, 0, 0};  // arcs of the row per class
-        for (int64_t j = b + threadIdx.x; j < e; j += blockDim.x) ++c[A.cls(A.weight(u, j, A.adj[j]))];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            c[k] = waveSum(c[k]);
-            if (lane == 0) wcnt[k][wv] = (int32_t)c[k];
-        }
+        int64_t c = 0;  // light arcs of the row
+        for (int64_t j = b + threadIdx.x; j < e; j += blockDim.x) c += A.weight(u, j, A.adj[j]) <= A.delta;
+        c = waveSum(c);
+        if (lane == 0) wcnt[wv] = (int32_t)c;
         __syncthreads();
-        if (threadIdx.x < 4) {
-            int64_t t = 0;
-            for (int k = 0; k < nw; ++k) t += wcnt[threadIdx.x][k];
-            tot[threadIdx.x] = t;
-        }
-        __syncthreads();
-        int64_t o[4];  // where the class's next arcs go (the earlier chunks placed)
-        o[0] = b;
-        o[1] = b + tot[0];
-        o[2] = o[1] + tot[1];
-        o[3] = o[2] + tot[2];
         if (threadIdx.x == 0) {
-            A.mid[u] = (int32_t)o[1];
-            A.mid2[u] = (int32_t)o[2];
-            A.mid3[u] = (int32_t)o[3];
+            int64_t t = 0;
+            for (int k = 0; k < nw; ++k) t += wcnt[k];
+            tot = t;
         }
+        __syncthreads();
+        const int64_t m = b + tot;
+        int64_t lc = 0;  // light arcs of the earlier chunks
         for (int64_t c0 = b; c0 < e; c0 += blockDim.x) {
             const int64_t j = c0 + threadIdx.x;
             const bool val = j < e;
             const int32_t x = val ? A.adj[j] : 0;
             const uint32_t wt = val ? A.weight(u, j, x) : 0;
-            const int cl = val ? A.cls(wt) : -1;
-            uint64_t bal[4];
+            const bool light = val && wt <= A.delta;
+            const uint64_t bal = __ballot(light);
             __syncthreads();  // wcnt: the previous chunk's (or the count's) reads are done
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                bal[k] = __ballot(cl == k);
-                if (lane == 0) wcnt[k][wv] = __popcll(bal[k]);
-            }
+            if (lane == 0) wcnt[wv] = __popcll(bal);
             __syncthreads();
-            int64_t before = 0;
-            for (int k = 0; k < nw && k < wv; ++k) before += wcnt[cl < 0 ? 0 : cl][k];
+            int64_t before = 0, all = 0;
+            for (int k = 0; k < nw; ++k) {
+                const int t = wcnt[k];
+                if (k < wv) before += t;
+                all += t;
+            }
             if (val) {
-                const int64_t at = o[cl] + before + __popcll(bal[cl] & lanesBelow());
-                A.oadj[at] = x;
-                A.ow[at] = wt;
+                const int64_t lr = lc + before + __popcll(bal & lanesBelow());  // light arcs before j in the row
+                const int64_t o = light ? b + lr : m + (j - b) - lr;
+                A.oadj[o] = x;
+                A.ow[o] = wt;
             }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                int64_t all = 0;
-                for (int q = 0; q < nw; ++q) all += wcnt[k][q];
-                o[k] += all;
-            }
+            lc += all;
         }
+        if (threadIdx.x == 0) A.mid[u] = (int32_t)m;
         __syncthreads();  // tot and wcnt are rewritten for the next row
     }
 }
@@ -728,13 +698,11 @@ static const SplitView* splitView(girhip_graph* h, uint64_t delta) {
     sv.adj.ensure((size_t)h->E);
     sv.w.ensure((size_t)h->E);
     sv.mid.ensure((size_t)(V > 0 ? V : 1));
-    sv.mid2.ensure((size_t)(V > 0 ? V : 1));
-    sv.mid3.ensure((size_t)(V > 0 ? V : 1));
     DevBuf<int32_t> longq;
     longq.ensure((size_t)(V > 0 ? V : 1));
     fillI64(h, h->dcnt.p, 0, 1);
     const SplitArgs A{h->fw_beg.p, h->fw_adj.p, h->has_w ? h->w_fw.p : nullptr, V, delta, sv.adj.p, sv.w.p,
-                      sv.mid.p, sv.mid2.p, sv.mid3.p, longq.p, h->dcnt.p};
+                      sv.mid.p, longq.p, h->dcnt.p};
     tBegin(h);
     launch(h, "sssp_split_short", kSplitShort, boundedGrid(V, 256), 256, 0, A);
     launch(h, "sssp_split_long", kSplitLong, 2048, 256, 0, A);
