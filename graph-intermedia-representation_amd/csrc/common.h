@@ -104,6 +104,7 @@ struct Params {
     double update_bypass = 1;    // 1: batches of few deletes test them for a bypass path first
     double scc_cert = 1;         // 1: update batches check the big SCC against its spanning trees
     double repair_ops = 0;       // largest batch (changes) the SCC layer repairs; 0: max(4096, V / 256)
+    double sssp_tiny = 1;        // 1: per-source SSSP rounds with small queues run one workgroup per source
 };
 
 struct KStat {
@@ -143,6 +144,7 @@ struct Scratch {
     DevBuf<uint32_t> s_d32;  // 32-bit distances of a batch
     DevBuf<uint64_t> s_near, s_far, s_done, s_nsum, s_fsum, s_dsum;
     DevBuf<int64_t> s_ctl;
+    DevBuf<int32_t> s_fl[2];  // far lists of the one-workgroup SSSP rounds
     // interleaved multi-source delta-stepping: distances [V][KS], near / far /
     // done source masks, their vertex bitmaps and summaries, round queues
     // (vertex, mask) by parity and the heavy queue, control slots

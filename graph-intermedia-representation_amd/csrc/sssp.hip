@@ -614,6 +614,313 @@ __global__ void __launch_bounds__(256) kSsspPersist(RelaxF fl, RelaxF fh, const 
     if (blockIdx.x == 0 && threadIdx.x == 0) *io = par;
 }
 
+// ---- small frontiers: one workgroup per source -------------------------------
+// A high-diameter graph (a lattice) walks tens of thousands of buckets of a
+// few hundred vertices each; there the multi-kernel rounds cost their five
+// launches per round, not their work.  The tiny engine runs the same rounds
+// for source s = blockIdx.x inside one workgroup of 1024 threads, separated
+// by __syncthreads only, until a queue outgrows TINY_CAP or the source is
+// settled.  A queued vertex has all its arcs relaxed at once (a lattice's
+// vertices are queued about once per bucket, so holding the heavy arcs back
+// to the end of the bucket saves nothing and costs a pass): there is no
+// heavy pass and no done set.  The near and far sets keep their bitmaps (so
+// that the multi-kernel rounds can take over at any round boundary: far
+// bits with their summary bits, an empty near bitmap, an empty done set)
+// and are also kept as lists: an id joins a list when its bit turns on (the
+// atomicOr's old value), so no pass walks a bitmap.  Rows are walked whole by the waves
+// (no heavy-row list): the engine runs only while no row is longer than
+// TINY_DEG arcs.  Distances written by one wave's atomics are read back by
+// other waves with agent-scope loads (dGetAcq): a vector-cache copy may be
+// stale.
+constexpr int TINY_CAP = 8192;    // a larger queue hands the rounds back to the grid kernels
+constexpr int TINY_ENTER = 1024;  // queues at most this long (every source) enter the tiny engine
+constexpr int64_t TINY_DEG = 1024;
+constexpr int TINY_ILP = 4;  // far-list entries per thread whose loads are in flight together
+
+// The wave's smallest value, folded into *m (LDS) by its first lane.
+__device__ __forceinline__ void waveMinLds(unsigned long long mn, unsigned long long* m) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_xor(mn, o);
+        mn = t < mn ? t : mn;
+    }
+    if (laneId() == 0 && mn != INF) atomicMin(m, mn);
+}
+
+__device__ __forceinline__ uint64_t dGetAcq(const SsspBatch& B, int s, int32_t x) {
+    const size_t i = (size_t)s * B.V + x;
+    if (B.d32) {
+        const uint32_t v = ldAcq((const uint32_t*)B.dist + i);
+        return v == 0xffffffffu ? INF : (uint64_t)v;
+    }
+    return ldAcq((const uint64_t*)B.dist + i);
+}
+
+// Sets bit x of a bitmap; true when this call turned it on.
+__device__ __forceinline__ bool bitOn(uint64_t* bits, int32_t x) {
+    const unsigned long long bit = 1ULL << (x & 63);
+    return (atomicOr((unsigned long long*)&bits[x >> 6], bit) & bit) == 0;
+}
+
+// RelaxF's arc rule for the tiny engine: a lowered head joins the next
+// queue (inside the bucket) or the far list (beyond it) when its bit turns
+// on; list slots come from LDS counters.  arcs: 0 light, 1 heavy, 2 every
+// arc of the row.
+struct TinyF {
+    SsspBatch B;
+    const int32_t* fa;
+    const uint32_t* w;
+    uint64_t delta;
+    int arcs;
+    const int32_t* scope;
+    int unit;
+    const int32_t* mid;
+    int s;
+    uint64_t hi;
+    int32_t* nq;      // next queue
+    int32_t* fl;      // far list
+    int* nq_n;        // LDS counters
+    int* fl_n;
+    unsigned long long* fmin;  // LDS: at most the smallest far distance
+    template <int U>
+    __device__ __forceinline__ void many(const int32_t (&p)[U], const uint64_t (&dp)[U], const int32_t (&)[U],
+                                         const int64_t (&j)[U], const bool (&ok)[U]) const {
+        int32_t x[U];
+        uint32_t wt[U];
+        uint64_t dx[U];
+        bool go[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            go[u] = ok[u];
+            x[u] = go[u] ? fa[j[u]] : 0;
+            wt[u] = (go[u] && !unit && w) ? w[j[u]] : 1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (go[u] && !unit && !w) wt[u] = hashWeight(p[u], x[u]);
+            if (go[u] && unit && arcs == 1) go[u] = false;
+            if (go[u] && !unit && !mid && arcs != 2 && ((wt[u] > delta) != (arcs == 1))) go[u] = false;
+            if (go[u] && scope && scope[x[u]] != scope[p[u]]) go[u] = false;
+            dx[u] = go[u] ? dGet(B, s, x[u]) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!go[u]) continue;
+            const uint64_t nd = dp[u] + wt[u];
+            if (B.d32 && nd >= 0xffffffffull) {  // beyond 32 bits: the batch is redone with 64
+                B.ctl[C_OVF] = 1;
+                continue;
+            }
+            if (nd >= dx[u]) continue;  // (a plain read, safe as a pre-check as distances only fall)
+            dMinNoRet(B, s, x[u], nd);
+            if (nd < hi) {
+                if (bitOn(B.nearbits + (size_t)s * B.W, x[u])) nq[atomicAdd(nq_n, 1)] = x[u];
+            } else {
+                atomicMin(fmin, (unsigned long long)nd);
+                if (bitOn(B.farbits + (size_t)s * B.W, x[u])) {
+                    atomicOr((unsigned long long*)&B.fsum[(size_t)s * B.W2 + (x[u] >> 12)],
+                             1ULL << ((x[u] >> 6) & 63));
+                    fl[atomicAdd(fl_n, 1)] = x[u];
+                }
+            }
+        }
+    }
+};
+
+// The ids of a bitmap's set bits appended to list at LDS counter *n (the
+// whole block; summaries say which words to read).
+__device__ __forceinline__ void listBits(const uint64_t* bits, const uint64_t* sum, int64_t W2, int32_t* list,
+                                         int* n) {
+    forSummaryB(sum, W2, 64, 0, 1, [&](int64_t g, uint64_t sm) {
+        const int l = laneId();
+        uint64_t m = ((sm >> l) & 1) ? bits[g * 64 + l] : 0;
+        const int c = __popcll(m);
+        int incl = c;
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int o = __shfl_up(incl, d);
+            if (l >= d) incl += o;
+        }
+        int base = 0;
+        if (l == WAVE - 1 && incl) base = atomicAdd(n, incl);
+        base = __shfl(base, WAVE - 1) + incl - c;
+        while (m) {
+            const int b = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            list[base++] = (int32_t)((g * 64 + l) * 64 + b);
+        }
+    });
+}
+
+// The phases of one source's rounds hand data from wave to wave of one
+// workgroup only: a workgroup barrier (with its workgroup-scope fences) is
+// the whole synchronisation.
+__device__ __forceinline__ void blockSync() { __syncthreads(); }
+
+// Relax the n vertices of list q (the arcs f.arcs names) with the whole
+// block.
+template <int U>
+__device__ __forceinline__ void tinyRelax(const TinyF& f, const int32_t* __restrict__ fb, const int32_t* q, int n) {
+    const SsspBatch& B = f.B;
+    const int s = f.s;
+    for (int base = (int)(threadIdx.x & ~(WAVE - 1)); base < n; base += (int)blockDim.x) {
+        const int i = base + laneId();
+        const bool act = i < n;
+        const int32_t u = act ? q[i] : 0;
+        const uint64_t du = act ? dGetAcq(B, s, u) : 0;
+        int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+        if (f.mid && act && f.arcs != 2) {
+            const int32_t m = f.mid[u];
+            if (f.arcs == 1) b = m;
+            else e = m;
+        }
+        waveFlatU<U>(u, du, s, b, act ? e - b : 0, f);
+    }
+}
+
+// Rounds of source blockIdx.x from the state left at a round boundary
+// (queue of parity par; an empty queue with far entries, as a seeded run
+// starts, goes straight to the advance) until its queue is empty after an
+// advance (settled), outgrows TINY_CAP, or max_rounds pass.  The queue is
+// left in parity par.
+// fl0 / fl1: far-list buffers of V ids per source.
+template <int U>
+__global__ void __launch_bounds__(1024) kSsspTiny(TinyF f, const int32_t* __restrict__ fb, int par0,
+                                                  int64_t max_rounds, int32_t* fl0, int32_t* fl1) {
+    __shared__ int nq_n, fl_n, fl2_n, dl_n;
+    // fmin: at most the smallest far distance at or past hi (the far list's
+    // minimum after each advance, lowered by every far relaxation since; a
+    // far vertex lowered into the bucket can leave it too small, which costs
+    // at most a visit of an empty bucket)
+    __shared__ unsigned long long fmin, fmin2;
+    const SsspBatch& B = f.B;
+    const int s = blockIdx.x;
+    const size_t vo = (size_t)s * B.V, wo = (size_t)s * B.W, so = (size_t)s * B.W2;
+    int par = par0;
+    int n = (int)ldAcq(nearCount(B, par, s));
+    if (n > TINY_CAP) return;
+    uint64_t lo = (uint64_t)ldAcq(cp(B, C_LO, s)), hi = (uint64_t)ldAcq(cp(B, C_HI, s));
+    int32_t* fl = fl0 + vo;
+    int32_t* fl2 = fl1 + vo;
+    if (threadIdx.x == 0) {
+        fl_n = 0;
+        dl_n = 0;
+        fmin = INF;
+    }
+    __syncthreads();
+    int32_t* dl = B.qh + vo;  // the done set handed over (heavy arcs pending)
+    if (f.arcs) listBits(B.donebits + wo, B.dsum + so, B.W2, dl, &dl_n);
+    listBits(B.farbits + wo, B.fsum + so, B.W2, fl, &fl_n);
+    blockSync();
+    {
+        unsigned long long mn = INF;
+        for (int i = threadIdx.x; i < fl_n; i += blockDim.x) {
+            const uint64_t v = dGetAcq(B, s, fl[i]);
+            if (v >= hi && v < mn) mn = v;
+        }
+        waveMinLds(mn, &fmin);
+    }
+    blockSync();
+    // the arc rule of this source's passes (arcs: see TinyF)
+    auto rule = [&](int arcs, int32_t* nq) {
+        return TinyF{B, f.fa, f.w, f.delta, arcs, f.scope, f.unit, f.mid, s, hi, nq, fl, &nq_n, &fl_n, &fmin};
+    };
+    if (dl_n > 0) {  // heavy arcs of the vertices already processed in this bucket
+        if (threadIdx.x == 0) nq_n = 0;
+        __syncthreads();
+        tinyRelax<U>(rule(1, B.qn[par ^ 1] + vo), fb, dl, dl_n);
+        blockSync();
+        for (int i = threadIdx.x; i < dl_n; i += blockDim.x) {
+            const int32_t u = dl[i];
+            B.donebits[wo + (u >> 6)] = 0;
+            B.dsum[so + (u >> 12)] = 0;
+        }
+        blockSync();
+    }
+    const int arcs = f.arcs ? 2 : 0;
+    // (an empty queue never ends a launch unless the source is settled: the
+    // grid kernels read an empty queue as settled)
+    for (int64_t r = 0; (r < max_rounds || n == 0) && n <= TINY_CAP; ++r) {
+        int32_t* q = B.qn[par] + vo;
+        int32_t* nq = B.qn[par ^ 1] + vo;
+        if (threadIdx.x == 0) nq_n = 0;
+        __syncthreads();
+        tinyRelax<U>(rule(arcs, nq), fb, q, n);
+        blockSync();
+        const int m = nq_n;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) B.nearbits[wo + (nq[i] >> 6)] = 0;
+        blockSync();
+        if (m > 0) {  // the bucket took new vertices: another round inside it
+            par ^= 1;
+            n = m;
+            continue;
+        }
+        // the bucket drained: the far minimum picks the next bucket
+        const int nf = fl_n;
+        const uint64_t mfar = fmin;
+        if (mfar == INF) {  // no far work left: settled
+            n = 0;
+            break;
+        }
+        lo = (mfar / f.delta) * f.delta;
+        hi = lo > INF - f.delta ? INF : lo + f.delta;
+        // far entries inside [lo, hi) -> queue, below it -> settled; both
+        // leave the far set (bit cleared); the rest stay in the list, and
+        // their minimum is the next far minimum.  TINY_ILP entries per
+        // thread at a time, their loads issued together.
+        if (threadIdx.x == 0) {
+            nq_n = 0;
+            fl2_n = 0;
+            fmin2 = INF;
+        }
+        __syncthreads();
+        unsigned long long mn = INF;
+        for (int i0 = threadIdx.x; i0 < nf; i0 += TINY_ILP * blockDim.x) {
+            int32_t x[TINY_ILP];
+            uint64_t v[TINY_ILP];
+#pragma unroll
+            for (int c = 0; c < TINY_ILP; ++c) {
+                const int i = i0 + c * blockDim.x;
+                x[c] = i < nf ? fl[i] : -1;
+            }
+#pragma unroll
+            for (int c = 0; c < TINY_ILP; ++c) v[c] = x[c] >= 0 ? dGetAcq(B, s, x[c]) : 0;
+#pragma unroll
+            for (int c = 0; c < TINY_ILP; ++c) {
+                if (x[c] < 0) continue;
+                if (v[c] >= hi && v[c] != INF) {
+                    fl2[atomicAdd(&fl2_n, 1)] = x[c];
+                    if (v[c] < mn) mn = v[c];
+                } else {
+                    atomicAnd((unsigned long long*)&B.farbits[wo + (x[c] >> 6)], ~(1ULL << (x[c] & 63)));
+                    if (v[c] >= lo && v[c] < hi) nq[atomicAdd(&nq_n, 1)] = x[c];
+                }
+            }
+        }
+        waveMinLds(mn, &fmin2);
+        blockSync();
+        if (threadIdx.x == 0) {
+            fmin = fmin2;
+            fl_n = fl2_n;
+        }
+        int32_t* t = fl;
+        fl = fl2;
+        fl2 = t;
+        par ^= 1;
+        n = nq_n;
+        __syncthreads();
+    }
+    // hand the state back at a round boundary, the queue in parity par0
+    if (par != par0) {
+        const int32_t* q = B.qn[par] + vo;
+        int32_t* d = B.qn[par0] + vo;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = q[i];
+    }
+    if (threadIdx.x == 0) {
+        *nearCount(B, par0, s) = n;
+        *cp(B, C_LO, s) = (int64_t)lo;
+        *cp(B, C_HI, s) = (int64_t)hi;
+    }
+}
+
 // One slot of every source -> the gather area (contiguous, for one readback).
 __global__ void kGatherSlot(int64_t* ctl, int slot) {
     const int s = threadIdx.x;
@@ -753,7 +1060,25 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
     const bool graphs = h->prm.graph != 0 && h->prm.profile == 0 && cmax > 1;
     GraphCache gc(h);
     int full = 0;
+    // small frontiers without long rows: the one-workgroup-per-source engine
+    const bool tiny_ok = h->prm.sssp_tiny != 0 && maxDegree(h) <= TINY_DEG;
+    auto tinyK = ur == 4 ? kSsspTiny<4> : ur == 2 ? kSsspTiny<2> : kSsspTiny<1>;
     while (true) {
+        if (tiny_ok && maxn <= TINY_ENTER) {
+            const size_t kv = (size_t)k * (size_t)std::max<int32_t>(V, 1);
+            S.s_fl[0].ensure(kv);
+            S.s_fl[1].ensure(kv);
+            const TinyF tf{B, fa, w, delta, heavy, scope, u, mid, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr};
+            launch(h, "sssp_tiny", tinyK, (unsigned)k, 1024, 0, tf, fb, par, (int64_t)1 << 20, S.s_fl[0].p,
+                   S.s_fl[1].p);
+            readSlot(h, S.s_ctl.p, par ? C_NEAR1 : C_NEAR0, r.data());
+            int64_t mx = 0;
+            for (int s = 0; s < k; ++s) mx = std::max(mx, r[s]);
+            if (mx == 0) break;
+            maxn = mx;
+            R = 1;
+            if (mx <= TINY_ENTER) continue;  // (the launch's round limit passed)
+        }
         if (persist_ok && maxn <= cap) {
             int64_t* io = S.s_ctl.p + C_SLOTS;
             fillI64(h, io, par, 1);
@@ -1186,12 +1511,15 @@ extern "C" int girhip_sssp(girhip_graph* h, const int32_t* src, int32_t k, uint6
         // 49.7, delta 32 47.9 vs 34.8; hash weights [1, 100], 8 sources:
         // delta 16 23.2 vs 30.3, delta 32 27.4 vs 26.9; one source: the
         // interleaved engine at every delta.  Auto (2): interleaved for one
-        // source, for up to 8 sources when 4 delta >= the largest weight, and
-        // for more when 2 delta >= it.
+        // source (unless its rows all fit the one-workgroup rounds, no row
+        // above TINY_DEG arcs, and delta is small against the weights: a
+        // lattice's many small buckets), for up to 8 sources when 4 delta >=
+        // the largest weight, and for more when 2 delta >= it.
         const double mode = h->prm.sssp_interleave;
         const uint64_t mw = (uint64_t)maxWeight(h);
-        const bool inter =
-            mode == 1 || (mode == 2 && (k == 1 || (k <= 8 ? 4 * delta >= mw : 2 * delta >= mw)));
+        const bool tiny_fit = h->prm.sssp_tiny != 0 && maxDegree(h) <= TINY_DEG && 4 * delta < mw;
+        const bool inter = mode == 1 || (mode == 2 && (k == 1 ? !tiny_fit : k <= 8 ? 4 * delta >= mw
+                                                                                   : 2 * delta >= mw));
         if (inter) {
             // batches of up to sssp_ms_batch sources sharing buckets, distances
             // interleaved; the batches are made even (10 -> 5 + 5, not 8 + 2)

@@ -152,6 +152,54 @@ def test_grid_parity():
         assert (G.bfs(src) == lv).all()
 
 
+def test_sssp_one_workgroup_rounds():
+    # the per-source rounds with small queues run one workgroup per source
+    # and hand back to the grid kernels when a queue outgrows the workgroup
+    # engine (and take over again when the queues shrink): a lattice (small
+    # buckets throughout, stored and hash weights, 1..5 sources, deltas from
+    # 1 to past the largest weight), a random graph whose middle rounds
+    # outgrow it, and the SCC-restricted two-level rounds; against Dijkstra,
+    # with the engine on and off
+    V, fb, fa, bb, ba = girhip.grid_graph(90, 110, seed=64)
+    src = girhip.sources(fb, 5, 65)
+    w = girhip.weights(len(fa), 1, 1000, 66)
+    want = [oracle.dijkstra(fb, fa, w, int(x)) for x in src]
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.set_weights(w)
+        G.set_param("sssp_interleave", 0)
+        for tiny in (1, 0):
+            G.set_param("sssp_tiny", tiny)
+            for k, delta in ((1, 16), (4, 1), (5, 16), (3, 700), (2, 5000)):
+                say("lattice sssp tiny", tiny, "k", k, "delta", delta)
+                d = G.sssp(src[:k], delta)
+                for i in range(k):
+                    assert (d[i] == want[i]).all(), (tiny, k, delta, i)
+        G.set_weights(None)
+        hw = np.array([oracle.edge_weight(u, x) for u, x in zip(np.repeat(np.arange(V), np.diff(fb)), fa)],
+                      np.uint32)
+        G.set_param("sssp_tiny", 1)
+        d = G.sssp(src[:4], 16)
+        for i in range(4):
+            assert (d[i] == oracle.dijkstra(fb, fa, hw, int(src[i]))).all(), ("hash", i)
+        say("lattice two-level")
+        G.scc_dag()
+        assert (G.sssp_two_level(src[:2], 16) == d[:2]).all()
+    # random graph: wide middle rounds
+    u, v = girhip.uniform_edges(60000, 480000, seed=67)
+    fb, fa, bb, ba = girhip.csr_from_edges(60000, u, v)
+    w = girhip.weights(len(fa), 1, 64, 68)
+    src = girhip.sources(fb, 3, 69)
+    with girhip.Graph(fb, fa, bb, ba) as G:
+        G.set_weights(w)
+        G.set_param("sssp_interleave", 0)
+        for tiny in (1, 0):
+            G.set_param("sssp_tiny", tiny)
+            say("uniform sssp tiny", tiny)
+            d = G.sssp(src, 8)
+            for i, x in enumerate(src):
+                assert (d[i] == oracle.dijkstra(fb, fa, w, int(x))).all(), (tiny, i)
+
+
 def test_bfs_many_levels_bit_parallel():
     # long thin grid: hundreds of tiny levels -> sparse queue mode, chunked
     # levels and the counter ring wrapping; duplicate and invalid sources
@@ -274,7 +322,7 @@ def test_tuning_knobs_do_not_change_results():
                      ("sssp_delta", 100000), ("sssp_interleave", 0), ("sssp_interleave", 1),
                      ("sssp_ms_batch", 3), ("sssp_ms_batch", 32), ("scc_trim2", 1), ("sssp_unroll", 1),
                      ("sssp_unroll", 2), ("bfs_alpha_auto", 0), ("bfs_alpha", 1000), ("update_bypass", 0),
-                     ("scc_cert", 0)]:
+                     ("scc_cert", 0), ("sssp_tiny", 0)]:
             say("knob", k, v)
             old = G.get_param(k)
             G.set_param(k, v)
