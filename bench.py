@@ -288,6 +288,10 @@ def held_out(dev, cpu=True):
             rec["wcc_ms"] = round(wcc_ms, 3)
             rec["build_ms"] = round(wcc_ms + scc_def, 3)
             del lv
+            # one source, the reference's hash weights, delta 16: the CPU leg's own query
+            G.set_weights(None)
+            G.sssp(src[:1], 16, out=dist[:1])
+            rec["sssp_hash_delta16_ms_per_source"] = round(G.last_ms("sssp"), 3)
             if cpu:
                 rr = cpu_baseline(fb, fa, bb, ba, src, src, n_bfs=1, n_sssp=1)
                 if rr is not None:
@@ -298,6 +302,7 @@ def held_out(dev, cpu=True):
                                             "threads": rr[0].get("threads")}
                     rec["speedup_build_vs_cpu"] = round(rec["cpu_reference"]["build_ms"] / rec["build_ms"], 2)
                     rec["speedup_bfs_vs_cpu"] = round(rr[0]["bfs"]["ms"] / (rec["bfs_ms"] / len(src)), 2)
+                    rec["speedup_sssp_hash_vs_cpu"] = round(rr[0]["sssp"]["ms"] / rec["sssp_hash_delta16_ms_per_source"], 2)
         G.close()
         del dist, ref
         torch.cuda.empty_cache()

@@ -635,7 +635,7 @@ __global__ void __launch_bounds__(256) kSsspPersist(RelaxF fl, RelaxF fh, const 
 constexpr int TINY_CAP = 8192;    // a larger queue hands the rounds back to the grid kernels
 constexpr int TINY_ENTER = 1024;  // queues at most this long (every source) enter the tiny engine
 constexpr int64_t TINY_DEG = 1024;
-constexpr int TINY_ILP = 4;  // far-list entries per thread whose loads are in flight together
+constexpr int TINY_ILP = 8;  // far-list entries per thread whose loads are in flight together
 
 // The wave's smallest value, folded into *m (LDS) by its first lane.
 __device__ __forceinline__ void waveMinLds(unsigned long long mn, unsigned long long* m) {
@@ -756,14 +756,18 @@ __device__ __forceinline__ void listBits(const uint64_t* bits, const uint64_t* s
 __device__ __forceinline__ void blockSync() { __syncthreads(); }
 
 // Relax the n vertices of list q (the arcs f.arcs names) with the whole
-// block.
+// block: each wave takes an equal share of the list (a short list spreads
+// over every wave instead of filling the first few), U arcs per lane at a
+// time.
 template <int U>
 __device__ __forceinline__ void tinyRelax(const TinyF& f, const int32_t* __restrict__ fb, const int32_t* q, int n) {
     const SsspBatch& B = f.B;
     const int s = f.s;
-    for (int base = (int)(threadIdx.x & ~(WAVE - 1)); base < n; base += (int)blockDim.x) {
+    const int nw = (int)blockDim.x / WAVE, per = (n + nw - 1) / nw;
+    const int beg = (int)(threadIdx.x / WAVE) * per, end = min(n, beg + per);
+    for (int base = beg; base < end; base += WAVE) {
         const int i = base + laneId();
-        const bool act = i < n;
+        const bool act = i < end;
         const int32_t u = act ? q[i] : 0;
         const uint64_t du = act ? dGetAcq(B, s, u) : 0;
         int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
@@ -1062,7 +1066,8 @@ static void ssspRounds(girhip_graph* h, const SsspBatch& B, uint64_t delta, cons
     int full = 0;
     // small frontiers without long rows: the one-workgroup-per-source engine
     const bool tiny_ok = h->prm.sssp_tiny != 0 && maxDegree(h) <= TINY_DEG;
-    auto tinyK = ur == 4 ? kSsspTiny<4> : ur == 2 ? kSsspTiny<2> : kSsspTiny<1>;
+    // (one workgroup per CU leaves registers to spare: 4 arcs per lane)
+    auto tinyK = kSsspTiny<4>;
     while (true) {
         if (tiny_ok && maxn <= TINY_ENTER) {
             const size_t kv = (size_t)k * (size_t)std::max<int32_t>(V, 1);
