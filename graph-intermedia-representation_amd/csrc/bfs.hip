@@ -57,6 +57,115 @@ int64_t outDegree(girhip_graph* h, int32_t v) {
     return (int64_t)p[1] - p[0];
 }
 
+// ---- small frontiers: one workgroup per source -----------------------------
+// A high-diameter graph (a lattice: thousands of levels of a few thousand
+// vertices) spends the grid engines' time in launches and level barriers.
+// Here source s = blockIdx.x walks all its levels inside one workgroup of
+// 1024 threads: the level's frontier list is shared out evenly over the
+// waves, each wave flattens its vertices' rows over its lanes (BFS_TINY_U
+// arcs per lane at a time), a head still at -1 is claimed by atomicCAS and
+// appended to the next list (LDS counter), and a workgroup barrier ends the
+// level.  A frontier above BFS_TINY_CAP stops the source (status 1): the
+// caller then runs the grid engine for it.  Used while no row is longer
+// than BFS_TINY_DEG arcs.
+constexpr int BFS_TINY_CAP = 1 << 15;
+constexpr int64_t BFS_TINY_DEG = 1024;
+constexpr int BFS_TINY_U = 4;
+
+struct BfsTinyF {
+    const int32_t* fa;
+    int32_t* lv;
+    int32_t* next;
+    int* next_n;
+    int32_t level;  // of the heads
+    template <int U>
+    __device__ __forceinline__ void many(const int32_t (&)[U], const uint64_t (&)[U], const int32_t (&)[U],
+                                         const int64_t (&j)[U], const bool (&ok)[U]) const {
+        int32_t x[U], old[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = ok[u] ? fa[j[u]] : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) old[u] = x[u] >= 0 ? lv[x[u]] : 0;  // (plain: a stale -1 only costs the CAS)
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (x[u] >= 0 && old[u] == -1 && atomicCAS(&lv[x[u]], -1, level) == -1) next[atomicAdd(next_n, 1)] = x[u];
+    }
+};
+
+// levels: k rows of V, -1 already; l0 / l1: frontier lists of V ids per
+// source; status[s]: 0 done (or no valid source), 1 stopped.
+__global__ void __launch_bounds__(1024) kBfsTiny(const int32_t* __restrict__ fb, const int32_t* __restrict__ fa,
+                                                 int32_t V, const int32_t* __restrict__ src, int32_t* levels,
+                                                 int32_t* l0, int32_t* l1, int32_t* status) {
+    __shared__ int next_n;
+    const int s = blockIdx.x;
+    const int32_t x0 = src[s];
+    if (x0 < 0 || x0 >= V) {
+        if (threadIdx.x == 0) status[s] = 0;
+        return;
+    }
+    int32_t* lv = levels + (size_t)s * V;
+    int32_t* cur = l0 + (size_t)s * V;
+    int32_t* nxt = l1 + (size_t)s * V;
+    if (threadIdx.x == 0) {
+        lv[x0] = 0;
+        cur[0] = x0;
+    }
+    __syncthreads();
+    const int nw = (int)blockDim.x / WAVE, wv = (int)threadIdx.x / WAVE;
+    int n = 1, st = 0;
+    for (int32_t L = 0; n > 0; ++L) {
+        if (threadIdx.x == 0) next_n = 0;
+        __syncthreads();
+        const BfsTinyF f{fa, lv, nxt, &next_n, L + 1};
+        const int per = (n + nw - 1) / nw, beg = wv * per, end = min(n, beg + per);
+        for (int base = beg; base < end; base += WAVE) {
+            const int i = base + laneId();
+            const bool act = i < end;
+            const int32_t u = act ? cur[i] : 0;
+            const int32_t b = act ? fb[u] : 0, e = act ? fb[u + 1] : 0;
+            waveFlatU<BFS_TINY_U>(u, 0, 0, b, e - b, f);
+        }
+        __syncthreads();
+        n = next_n;
+        if (n > BFS_TINY_CAP) {
+            st = 1;
+            break;
+        }
+        int32_t* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    if (threadIdx.x == 0) status[s] = st;
+}
+
+// Levels of k sources (device array dsrc, host copy hsrc) with the
+// one-workgroup engine; the sources it stopped are returned (their rows are
+// reset to -1 for the caller's grid engine).
+static std::vector<int32_t> bfsTiny(girhip_graph* h, const int32_t* dsrc, const std::vector<int32_t>& hsrc,
+                                    int32_t* out) {
+    Scratch& S = scratch(h);
+    const int32_t V = h->V;
+    const int k = (int)hsrc.size();
+    const size_t kv = (size_t)k * (size_t)std::max<int32_t>(V, 1);
+    S.s_fl[0].ensure(kv);
+    S.s_fl[1].ensure(kv);
+    S.iv[3].ensure((size_t)std::max(k, 1));
+    fillI32(h, out, -1, (int64_t)k * V);
+    launch(h, "bfs_tiny", kBfsTiny, (unsigned)k, 1024, 0, (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p,
+           V, dsrc, out, S.s_fl[0].p, S.s_fl[1].p, S.iv[3].p);
+    std::vector<int32_t> st(k);
+    HIPCHK(hipMemcpyAsync(st.data(), S.iv[3].p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, h->st));
+    syncStream(h);
+    std::vector<int32_t> again;
+    for (int i = 0; i < k; ++i)
+        if (st[i]) {
+            again.push_back(i);
+            fillI32(h, out + (size_t)i * V, -1, V);
+        }
+    return again;
+}
+
 // ---------------------------------------------------------------------------
 // Bit-parallel multi-source BFS (MS-BFS, Then et al., VLDB 2015): up to 64
 // sources share one traversal.  Bit s of a vertex's seen / frontier / next
@@ -521,23 +630,33 @@ extern "C" int girhip_bfs(girhip_graph* h, const int32_t* src, int32_t k, int32_
             out = (int32_t*)S.stage.p;
         }
         ReachBufs B = reachBufs(h);
-        const bool bitpar = k > 1 && h->prm.msbfs != 0;
+        // hub-free graphs, several sources: one workgroup per source first
+        // (a single source keeps the grid engine: one workgroup walks a
+        // lattice's levels of thousands of vertices slower than the grid)
+        const bool tiny = k > 1 && h->prm.bfs_tiny != 0 && maxDegree(h) <= BFS_TINY_DEG;
+        const bool bitpar = k > 1 && h->prm.msbfs != 0 && !tiny;
         int32_t* dsrc = nullptr;
-        if (bitpar) {
+        if (bitpar || tiny) {
             S.src.ensure((size_t)k);
             dsrc = S.src.p;
             copyIn(h, dsrc, s.data(), sizeof(int32_t) * (size_t)k);
             syncStream(h);
         }
         tBegin(h);
+        std::vector<int32_t> rest;  // sources for the per-source grid engine
+        if (tiny) {
+            rest = bfsTiny(h, dsrc, s, out);
+        } else if (!bitpar) {
+            fillI32(h, out, -1, (int64_t)n);
+            for (int32_t i = 0; i < k; ++i) rest.push_back(i);
+        }
         if (bitpar) {
             for (int32_t i0 = 0; i0 < k; i0 += 64) {
                 const int c = std::min<int32_t>(64, k - i0);
                 msbfs(h, dsrc + i0, c, out + (size_t)i0 * V);
             }
         } else {
-            fillI32(h, out, -1, (int64_t)n);
-            for (int32_t i = 0; i < k; ++i) {
+            for (const int32_t i : rest) {
                 const int32_t x = s[i];
                 if (x < 0 || x >= V) continue;
                 fillU64(h, B.vis, 0, B.W);

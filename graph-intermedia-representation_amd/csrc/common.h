@@ -105,6 +105,7 @@ struct Params {
     double scc_cert = 1;         // 1: update batches check the big SCC against its spanning trees
     double repair_ops = 0;       // largest batch (changes) the SCC layer repairs; 0: max(4096, V / 256)
     double sssp_tiny = 1;        // 1: per-source SSSP rounds with small queues run one workgroup per source
+    double bfs_tiny = 1;         // 1: BFS on graphs without long rows runs one workgroup per source first
 };
 
 struct KStat {

@@ -200,6 +200,36 @@ def test_sssp_one_workgroup_rounds():
                 assert (d[i] == oracle.dijkstra(fb, fa, w, int(x))).all(), (tiny, i)
 
 
+def test_bfs_one_workgroup_levels():
+    # BFS on graphs without long rows walks each source's levels in one
+    # workgroup; a frontier that outgrows it hands the source to the grid
+    # engine: a lattice (thousands of levels), a random graph whose middle
+    # levels are wide, invalid and repeated sources; against textbook BFS,
+    # with the engine on and off
+    V, fb, fa, bb, ba = girhip.grid_graph(70, 300, seed=71)
+    src = np.concatenate([girhip.sources(fb, 5, 72), [0, 0, -2, V]]).astype(np.int32)
+    u, v = girhip.uniform_edges(50000, 400000, seed=73)
+    gb, ga, gbb, gba = girhip.csr_from_edges(50000, u, v)
+    gsrc = girhip.sources(gb, 3, 74)
+    for tiny in (1, 0):
+        with girhip.Graph(fb, fa, bb, ba) as G:
+            G.set_param("bfs_tiny", tiny)
+            say("lattice bfs tiny", tiny)
+            lv, m = G.bfs(src, edges=True)
+            for i, x in enumerate(src):
+                want = oracle.bfs_textbook(fb, fa, x) if 0 <= x < V else np.full(V, -1, np.int32)
+                assert (lv[i] == want).all(), (tiny, i)
+                assert m[i] == int(np.diff(fb)[want >= 0].sum())
+            assert (G.bfs(src[:1]) == lv[:1]).all()
+            assert (G.bfs(src[:2]) == lv[:2]).all()
+        with girhip.Graph(gb, ga, gbb, gba) as G:
+            G.set_param("bfs_tiny", tiny)
+            say("uniform bfs tiny", tiny)
+            lv = G.bfs(gsrc)
+            for i, x in enumerate(gsrc):
+                assert (lv[i] == oracle.bfs_textbook(gb, ga, x)).all(), (tiny, i)
+
+
 def test_bfs_many_levels_bit_parallel():
     # long thin grid: hundreds of tiny levels -> sparse queue mode, chunked
     # levels and the counter ring wrapping; duplicate and invalid sources
@@ -322,7 +352,7 @@ def test_tuning_knobs_do_not_change_results():
                      ("sssp_delta", 100000), ("sssp_interleave", 0), ("sssp_interleave", 1),
                      ("sssp_ms_batch", 3), ("sssp_ms_batch", 32), ("scc_trim2", 1), ("sssp_unroll", 1),
                      ("sssp_unroll", 2), ("bfs_alpha_auto", 0), ("bfs_alpha", 1000), ("update_bypass", 0),
-                     ("scc_cert", 0), ("sssp_tiny", 0)]:
+                     ("scc_cert", 0), ("sssp_tiny", 0), ("bfs_tiny", 0)]:
             say("knob", k, v)
             old = G.get_param(k)
             G.set_param(k, v)
