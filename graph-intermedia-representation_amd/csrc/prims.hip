@@ -183,6 +183,7 @@ void scanI32toI32(girhip_graph* h, const int32_t* in, int32_t* out, int64_t n) {
 // polls gives up, sets the sort's error word and lets the kernel finish;
 // the host raises the error at its next stream synchronisation.
 constexpr int RS_T = 256, RS_R = 16, RS_TILE = RS_T * RS_R, RS_D = 256, RS_MAXP = 8;
+static_assert(RS_D == RS_T, "a pass's threads own one digit each");
 constexpr unsigned RS_SPIN = 1u << 22;
 constexpr uint64_t RS_CNT = (1ULL << 46) - 1;
 
@@ -207,29 +208,54 @@ __global__ void __launch_bounds__(RS_T) kRadixPass(const uint64_t* __restrict__ 
                                                    int64_t* err) {
     __shared__ int64_t base[RS_D];
     __shared__ int32_t wc[RS_T / WAVE][RS_D];
-    __shared__ int32_t cnt[RS_D];
     __shared__ int64_t tile_s;
     const int t = threadIdx.x, lane = laneId(), w = t / WAVE;
     if (t == 0) tile_s = (int64_t)atomicAdd(tick, 1ULL);
-    cnt[t] = 0;
+    for (int d = lane; d < RS_D; d += WAVE) wc[w][d] = 0;
     __syncthreads();
     const int64_t tile = tile_s;
-    const int64_t t0 = tile * RS_TILE;
+    // wave w holds a contiguous RS_R * 64 keys of the tile, item r of lane l
+    // at w0 + r * 64 + l: ranking wave by wave, item by item, lane by lane
+    // is then the keys' own order (stable)
+    const int64_t w0 = tile * RS_TILE + (int64_t)w * (RS_R * WAVE);
     uint64_t key[RS_R];
     uint32_t val[RS_R];
+    int rk[RS_R];
 #pragma unroll
     for (int r = 0; r < RS_R; ++r) {
-        const int64_t idx = t0 + (int64_t)r * RS_T + t;
+        const int64_t idx = w0 + (int64_t)r * WAVE + lane;
         key[r] = idx < n ? kin[idx] : 0;
         val[r] = (idx < n && vin) ? vin[idx] : 0;
     }
+    // rank of each key among the wave's keys of its digit (wave-private
+    // running counts, no workgroup barrier per item)
 #pragma unroll
-    for (int r = 0; r < RS_R; ++r)
-        if (t0 + (int64_t)r * RS_T + t < n) atomicAdd(&cnt[(key[r] >> shift) & 255], 1);
+    for (int r = 0; r < RS_R; ++r) {
+        const bool valid = w0 + (int64_t)r * WAVE + lane < n;
+        const int d = (int)((key[r] >> shift) & 255);
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int below = __popcll(peers & lanesBelow());
+        rk[r] = valid ? wc[w][d] + below : 0;
+        waveSync();
+        if (valid && below == 0) wc[w][d] += __popcll(peers);
+        waveSync();
+    }
     __syncthreads();
+    // digit t: the waves' counts -> exclusive prefix over the waves; the
+    // tile's count
+    int64_t mine = 0;
+    for (int k = 0; k < RS_T / WAVE; ++k) {
+        const int c = wc[k][t];
+        wc[k][t] = (int32_t)mine;
+        mine += c;
+    }
     // this tile's offset for digit t: the counts of the tiles before it
     const uint64_t ep = (uint64_t)epoch << 48;
-    const int64_t mine = cnt[t];
     int64_t ex = 0;
     uint64_t* slot = look + tile * RS_D + t;
     if (tile == 0) {
@@ -262,31 +288,11 @@ __global__ void __launch_bounds__(RS_T) kRadixPass(const uint64_t* __restrict__ 
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < RS_R; ++r) {
-        const int64_t idx = t0 + (int64_t)r * RS_T + t;
-        const bool valid = idx < n;
+        if (w0 + (int64_t)r * WAVE + lane >= n) continue;
         const int d = (int)((key[r] >> shift) & 255);
-        for (int k = 0; k < RS_T / WAVE; ++k) wc[k][t] = 0;
-        __syncthreads();
-        uint64_t peers = __ballot(valid);
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        const int rank = __popcll(peers & lanesBelow());
-        if (valid && rank == 0) wc[w][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int64_t off = base[d] + rank;
-            for (int k = 0; k < w; ++k) off += wc[k][d];
-            kout[off] = key[r];
-            if (vout) vout[off] = val[r];
-        }
-        __syncthreads();
-        int64_t add = 0;
-        for (int k = 0; k < RS_T / WAVE; ++k) add += wc[k][t];
-        base[t] += add;
-        __syncthreads();
+        const int64_t off = base[d] + wc[w][d] + rk[r];
+        kout[off] = key[r];
+        if (vout) vout[off] = val[r];
     }
 }
 
