@@ -781,6 +781,10 @@ def test_sssp_distances_beyond_32_bits():
             want = oracle.dijkstra(fb, fa, w, x)
             assert (d[i] == want).all()
         assert d[d != INF].max() > (1 << 32)
+        # a small delta against these weights: the per-source rounds in one
+        # workgroup per source, redone with 64-bit distances as well
+        say("sssp 64-bit redo, one-workgroup rounds")
+        assert (G.sssp(src, 1 << 26) == d).all()
 
 
 def test_reachable_queries():
