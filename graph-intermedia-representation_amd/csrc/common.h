@@ -106,6 +106,7 @@ struct Params {
     double repair_ops = 0;       // largest batch (changes) the SCC layer repairs; 0: max(4096, V / 256)
     double sssp_tiny = 1;        // 1: per-source SSSP rounds with small queues run one workgroup per source
     double bfs_tiny = 1;         // 1: BFS on graphs without long rows runs one workgroup per source first
+    double scc_pair = 1;         // 1: the SCC build's forward and backward sweeps share their launches
 };
 
 struct KStat {
@@ -131,6 +132,8 @@ struct Scratch {
     DevBuf<int64_t> mscnt;   // bit-parallel BFS: per-level counter ring
     DevBuf<uint64_t> msbm[4];  // bit-parallel BFS: frontier / touched bitmaps and summaries
     DevBuf<int64_t> rcnt;    // reachability engine: per-level counter ring
+    DevBuf<int64_t> rcnt2;   // the paired sweeps' two rings
+    DevBuf<int32_t> pq[2];   // the paired sweeps' second queue pair
     // update batches: per-op arrays (grown to the largest batch; b_pi / b_pd
     // also hold the change positions of the CSR merge) and the new offsets
     DevBuf<int32_t> b_u, b_v, b_fi, b_fd;
@@ -220,7 +223,7 @@ struct Scratch {
 
 }  // namespace gir
 
-constexpr int HCNT_SLOTS = 256;
+constexpr int HCNT_SLOTS = 512;
 
 namespace gir {
 // Grid barrier words: eight group arrival counters, the top counter, the

@@ -616,7 +616,7 @@ static double* paramSlot(girhip_graph* h, const char* name) {
                {"scc_incremental", &Params::scc_incremental}, {"sssp_unroll", &Params::sssp_unroll},
                {"bfs_alpha_auto", &Params::bfs_alpha_auto}, {"update_bypass", &Params::update_bypass},
                {"scc_cert", &Params::scc_cert}, {"repair_ops", &Params::repair_ops},
-               {"sssp_tiny", &Params::sssp_tiny}, {"bfs_tiny", &Params::bfs_tiny}};
+               {"sssp_tiny", &Params::sssp_tiny}, {"bfs_tiny", &Params::bfs_tiny}, {"scc_pair", &Params::scc_pair}};
     for (auto& t : tab)
         if (std::strcmp(t.n, name) == 0) return &(p.*(t.m));
     return nullptr;

@@ -454,6 +454,111 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
     return st;
 }
 
+// ---- two sweeps side by side ------------------------------------------------
+// The SCC build's forward and backward sweeps from the pivot are
+// independent; on a high-diameter graph each is thousands of levels of a
+// few thousand vertices, so its cost is one launch per level.  Here one
+// launch runs a top-down level of both: the first g1 blocks serve the
+// forward sweep, the rest the backward one (each part strides over its own
+// queue and writes its own counter ring).  Used while both frontiers are
+// small (below V / 64 vertices) and no row can reach the heavy list; a sweep
+// still running after that finishes with runReach from its frontier.
+template <class Flt, class Vis>
+__global__ void kTopDownPair(const int32_t* __restrict__ qin1, const int64_t* nin1, TopDownF<Flt, Vis> f1,
+                             int64_t* cz1, const int32_t* __restrict__ qin2, const int64_t* nin2,
+                             TopDownF<Flt, Vis> f2, int64_t* cz2, unsigned g1, HeavyList H) {
+    const bool first = blockIdx.x < g1;
+    const TopDownF<Flt, Vis>& f = first ? f1 : f2;
+    const int32_t* qin = first ? qin1 : qin2;
+    const int64_t b0 = first ? blockIdx.x : blockIdx.x - g1, nb = first ? g1 : gridDim.x - g1;
+    if (b0 == 0 && threadIdx.x < 2) (first ? cz1 : cz2)[threadIdx.x] = 0;
+    const int64_t n = *(first ? nin1 : nin2);
+    const int64_t stride = nb * blockDim.x;
+    int64_t m = 0;
+    for (int64_t base = b0 * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+        const int64_t i = base + laneId();
+        const bool act = i < n;
+        const int32_t u = act ? qin[i] : 0;
+        const int32_t b = act ? f.beg[u] : 0, e = act ? f.beg[u + 1] : 0;
+        m += waveRowsH(act, u, 0, 0, b, e, H, f);
+    }
+    blockAtomicAdd(m, &f.cnt[1]);
+}
+
+// Reachability from seed (one vertex) over (fb, fa) into B1 and over
+// (bb, ba) into B2 at the same time: both visited bitmaps clear on entry,
+// on1 / on2 the sweeps' visit hooks.  B1 / B2 need their own queues;
+// fbits / nbits are used only by a sweep's runReach finish (one sweep at a
+// time).  n_reach / m_total: the switch rule's size estimates.
+template <class Flt, class Vis>
+void runReachPair(girhip_graph* h, const int32_t* fb, const int32_t* fa, const int32_t* bb, const int32_t* ba,
+                  int32_t seed, ReachBufs& B1, Vis on1, ReachBufs& B2, Vis on2, int64_t n_reach, int64_t m_total,
+                  Flt flt) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    S.rcnt2.ensure(4 * REACH_RING);
+    int64_t* c1 = S.rcnt2.p;
+    int64_t* c2 = S.rcnt2.p + 2 * REACH_RING;
+    auto pr = [&](int64_t* c, int64_t L) { return c + 2 * ((L % REACH_RING + REACH_RING) % REACH_RING); };
+    fillI32(h, B1.q[0], seed, 1);
+    fillI32(h, B2.q[0], seed, 1);
+    launch(h, "reach_seed", kReachSeed<Vis>, 1, 64, 0, (const int32_t*)B1.q[0], (int64_t)1, B1.vis, on1);
+    launch(h, "reach_seed", kReachSeed<Vis>, 1, 64, 0, (const int32_t*)B2.q[0], (int64_t)1, B2.vis, on2);
+    fillI64(h, c1, 0, 4 * REACH_RING);
+    fillI64(h, pr(c1, -1), 1, 1);
+    fillI64(h, pr(c2, -1), 1, 1);
+    const HeavyList HL = heavyList(h);
+    int64_t nf[2] = {1, 1}, mf[2] = {0, 0}, mf_prev = 0, m_seen[2] = {0, 0};
+    int c_prev = 1, cur = 0;
+    int32_t L = 0;
+    while ((nf[0] > 0 || nf[1] > 0) && nf[0] * 64 < (int64_t)V && nf[1] * 64 < (int64_t)V) {
+        const int64_t mx = std::max(mf[0], mf[1]);
+        // small frontiers not growing fast: a chunk of levels per readback
+        const int C = (mf_prev > 0 && mx * 64 < (int64_t)V && mx <= 2 * mf_prev)
+                          ? std::min(std::max(1, std::min(REACH_CHUNK, (int)h->prm.chunk)), 2 * c_prev)
+                          : 1;
+        c_prev = C;
+        mf_prev = mx;
+        auto grid = [&](int k) {
+            const int64_t work = std::max<int64_t>(nf[k], mf[k]) * (C > 1 ? 4 : 1);
+            return C > 1 ? (unsigned)std::min<int64_t>(1024, std::max<int64_t>(64, (work + blk - 1) / blk))
+                         : std::max(1u, gridFor(nf[k], blk));
+        };
+        const unsigned g1 = grid(0), g2 = grid(1);
+        for (int t = 0; t < C; ++t) {
+            const TopDownF<Flt, Vis> f1{fb, fa, B1.vis, L + t, B1.q[cur ^ 1], pr(c1, L + t), flt, on1};
+            const TopDownF<Flt, Vis> f2{bb, ba, B2.vis, L + t, B2.q[cur ^ 1], pr(c2, L + t), flt, on2};
+            launch(h, "reach_topdown_pair", kTopDownPair<Flt, Vis>, g1 + g2, blk, 0, (const int32_t*)B1.q[cur],
+                   (const int64_t*)pr(c1, L + t - 1), f1, pr(c1, L + t + 1), (const int32_t*)B2.q[cur],
+                   (const int64_t*)pr(c2, L + t - 1), f2, pr(c2, L + t + 1), g1, HL);
+            cur ^= 1;
+        }
+        int64_t r[4 * REACH_RING];
+        readCounters(h, c1, 4 * REACH_RING, r);
+        const int32_t L0 = L;
+        for (int t = 0; t < C; ++t) {
+            for (int k = 0; k < 2; ++k) {
+                const int64_t* p = r + (pr(k ? c2 : c1, L0 + t) - c1);
+                nf[k] = p[0];
+                mf[k] = p[1];
+                m_seen[k] += p[1];
+            }
+            ++L;
+        }
+    }
+    // a sweep whose frontier outgrew the pair finishes alone from it
+    for (int k = 0; k < 2; ++k) {
+        if (nf[k] == 0) continue;
+        ReachBufs& B = k ? B2 : B1;
+        if (cur) HIPCHK(hipMemcpyAsync(B.q[0], B.q[1], sizeof(int32_t) * (size_t)nf[k], hipMemcpyDeviceToDevice, h->st));
+        if (k == 0)
+            runReach(h, fb, fa, bb, ba, B, nf[k], mf[k], n_reach, m_total - m_seen[k] + mf[k], flt, on1);
+        else
+            runReach(h, bb, ba, fb, fa, B, nf[k], mf[k], n_reach, m_total - m_seen[k] + mf[k], flt, on2);
+    }
+}
+
 // Reachability buffers from the handle's scratch (q[0], q[1], bits[0..2]).
 ReachBufs reachBufs(girhip_graph* h);
 

@@ -701,21 +701,39 @@ void runSccActive(girhip_graph* h, bool cert) {
             fillI32(h, h->gc_fpar.p + pivot, pivot, 1);
             fillI32(h, h->gc_bpar.p + pivot, pivot, 1);
         }
-        fillU64(h, B.vis, 0, B.W);
-        fillI32(h, B.q[0], pivot, 1);
-        if (cert)
-            runReach(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, B, 1, 0, n, h->E, act,
-                     WriteParent{h->gc_fpar.p});
-        else
-            runReach(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
-        HIPCHK(hipMemcpyAsync(S.bits[3].p, B.vis, sizeof(uint64_t) * B.W, hipMemcpyDeviceToDevice, h->st));
-        fillU64(h, B.vis, 0, B.W);
-        fillI32(h, B.q[0], pivot, 1);
-        if (cert)
-            runReach(h, h->bw_beg.p, h->bw_adj.p, h->fw_beg.p, h->fw_adj.p, B, 1, 0, n, h->E, act,
-                     WriteParent{h->gc_bpar.p});
-        else
-            runReach(h, h->bw_beg.p, h->bw_adj.p, h->fw_beg.p, h->fw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
+        if (h->prm.scc_pair != 0 && 2 * maxDegree(h) <= HEAVY_T) {
+            // both sweeps in shared launches: forward into bits[3], backward into B.vis
+            S.pq[0].ensure(V);
+            S.pq[1].ensure(V);
+            ReachBufs B1 = B, B2 = B;
+            B1.vis = S.bits[3].p;
+            B2.q[0] = S.pq[0].p;
+            B2.q[1] = S.pq[1].p;
+            fillU64(h, B1.vis, 0, B.W);
+            fillU64(h, B2.vis, 0, B.W);
+            if (cert)
+                runReachPair(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, pivot, B1,
+                             WriteParent{h->gc_fpar.p}, B2, WriteParent{h->gc_bpar.p}, n, h->E, act);
+            else
+                runReachPair(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, pivot, B1, NoVisit{}, B2,
+                             NoVisit{}, n, h->E, act);
+        } else {
+            fillU64(h, B.vis, 0, B.W);
+            fillI32(h, B.q[0], pivot, 1);
+            if (cert)
+                runReach(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, B, 1, 0, n, h->E, act,
+                         WriteParent{h->gc_fpar.p});
+            else
+                runReach(h, h->fw_beg.p, h->fw_adj.p, h->bw_beg.p, h->bw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
+            HIPCHK(hipMemcpyAsync(S.bits[3].p, B.vis, sizeof(uint64_t) * B.W, hipMemcpyDeviceToDevice, h->st));
+            fillU64(h, B.vis, 0, B.W);
+            fillI32(h, B.q[0], pivot, 1);
+            if (cert)
+                runReach(h, h->bw_beg.p, h->bw_adj.p, h->fw_beg.p, h->fw_adj.p, B, 1, 0, n, h->E, act,
+                         WriteParent{h->gc_bpar.p});
+            else
+                runReach(h, h->bw_beg.p, h->bw_adj.p, h->fw_beg.p, h->fw_adj.p, B, 1, 0, n, h->E, act, NoVisit{});
+        }
         if (cert) {  // the trees hold; the pivot's label is read by the first batch that needs it
             h->gc_ok = true;
             h->gc_pivot = pivot;

@@ -352,7 +352,7 @@ def test_tuning_knobs_do_not_change_results():
                      ("sssp_delta", 100000), ("sssp_interleave", 0), ("sssp_interleave", 1),
                      ("sssp_ms_batch", 3), ("sssp_ms_batch", 32), ("scc_trim2", 1), ("sssp_unroll", 1),
                      ("sssp_unroll", 2), ("bfs_alpha_auto", 0), ("bfs_alpha", 1000), ("update_bypass", 0),
-                     ("scc_cert", 0), ("sssp_tiny", 0), ("bfs_tiny", 0)]:
+                     ("scc_cert", 0), ("sssp_tiny", 0), ("bfs_tiny", 0), ("scc_pair", 0)]:
             say("knob", k, v)
             old = G.get_param(k)
             G.set_param(k, v)
