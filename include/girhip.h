@@ -296,6 +296,19 @@ int girhip_has_edge(girhip_graph* h, int32_t u, int32_t v, int32_t* present);
  *   "scc_incremental" (1: update batches repair the SCC/DAG layer by
  *   classifying the changes, falling back to a rebuild for large batches or
  *   large affected SCCs; 0: always rebuild),
+ *   "update_bypass" (1: a batch with at most 8 deletes first looks for a
+ *   path of at most four arcs from each deleted arc's tail to its head,
+ *   which proves that no WCC or SCC splits), "scc_cert" (1: update batches
+ *   check the big SCC against the forward / backward spanning trees its
+ *   build recorded; 0: the same sweeps over all its members),
+ *   "repair_ops" (0: the SCC layer repairs batches of up to
+ *   max(4096, V / 256) changes and rebuilds above; n: up to n),
+ *   "sssp_tiny" (1: on graphs with no row above 1,024 arcs, per-source SSSP
+ *   rounds with small queues run one workgroup per source), "bfs_tiny" (1:
+ *   on such graphs, a BFS of several sources runs each source's levels in
+ *   one workgroup while its frontier stays small), "scc_pair" (1: on such
+ *   graphs, the SCC build's forward and backward sweeps from the pivot share
+ *   one launch per level while both frontiers are small),
  *   "profile" (0: 1 times every kernel)
  * Values only change work order; results never depend on them. */
 int girhip_set_param(girhip_graph* h, const char* name, double value);
