@@ -435,13 +435,15 @@ __global__ void kMsPull(const int32_t* __restrict__ bb, const int32_t* __restric
     next[w] = acc & ~sw;
 }
 
-// Dense finish over all V: new bits -> seen, frontier words, level writes;
-// every F word is rewritten (one wave = one word), F's summary was cleared
-// before the launch.  A block walks MF_TILES tiles and adds its counts once
-// (one counter hit by every block of a V-wide grid serialises them).
+// Dense finish over all V: new bits -> seen, frontier words, and either the
+// level's new-bit words (lw: kept for kMsLevels, one 8-byte store per
+// vertex) or, with lw null, the level writes themselves; every F word is
+// rewritten (one wave = one word), F's summary was cleared before the
+// launch.  A block walks MF_TILES tiles and adds its counts once (one counter
+// hit by every block of a V-wide grid serialises them).
 constexpr int MF_TILES = 16;
 __global__ void kMsFinish(uint64_t* seen, uint64_t* front, uint64_t* next, int32_t* __restrict__ levels, int32_t V,
-                          int32_t L, const int32_t* __restrict__ fb, MsBits M, MsCnt* nxt) {
+                          int32_t L, const int32_t* __restrict__ fb, MsBits M, MsCnt* nxt, uint64_t* __restrict__ lw) {
     int64_t packed = 0;  // (new vertices << 32) + their push-arc count
     for (int t = 0; t < MF_TILES; ++t) {
         const int64_t w = ((int64_t)blockIdx.x * MF_TILES + t) * blockDim.x + threadIdx.x;
@@ -452,9 +454,10 @@ __global__ void kMsFinish(uint64_t* seen, uint64_t* front, uint64_t* next, int32
             seen[w] = sw | nb;
             front[w] = nb;
             next[w] = 0;
+            if (lw) lw[w] = nb;
         }
         const uint64_t fw = __ballot(nb != 0);
-        if (fw) {
+        if (fw && !lw) {
             for (int s = 0; s < 64; ++s)
                 if ((nb >> s) & 1) levels[(size_t)s * V + w] = L + 1;
         }
@@ -469,13 +472,44 @@ __global__ void kMsFinish(uint64_t* seen, uint64_t* front, uint64_t* next, int32
     blockAtomicAdd(packed, &nxt->nf);
 }
 
-__global__ void kMsUnreached(const uint64_t* __restrict__ seen, int32_t* __restrict__ levels, int32_t V,
-                             uint64_t all) {
+// The dense levels whose new-bit words were kept: n consecutive levels
+// first + 1 .. first + n, level j's words at lw + j * V.
+constexpr int MS_KEPT = 7;
+struct MsKept {
+    const uint64_t* lw;
+    int32_t first, n;
+};
+
+// Level rows of every (source, vertex) pair first reached at a kept dense
+// level (first + 1 + j for kept level j) or never reached (-1).  Pairs
+// reached at other levels were written by their level's finish and are left
+// alone.  The kept words are folded into three bit planes of the code j + 1
+// (pairs are reached once, so a bit is set in at most one kept level); each
+// source's row is then one coalesced 256-byte segment per wave, written
+// once, where writes at every level covered a row segment a few lanes at a
+// time.
+__global__ void kMsLevels(const uint64_t* __restrict__ seen, MsKept K, int32_t* __restrict__ levels, int32_t V,
+                          int32_t k) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t un = w < V ? (~seen[w] & all) : 0;
-    if (__ballot(un != 0)) {
-        for (int s = 0; s < 64; ++s)
-            if ((un >> s) & 1) levels[(size_t)s * V + w] = -1;
+    if (w >= V) return;
+    uint64_t p0 = 0, p1 = 0, p2 = 0;
+#pragma unroll
+    for (int j = 0; j < MS_KEPT; ++j) {
+        if (j < K.n) {
+            const uint64_t b = K.lw[(size_t)j * V + w];
+            if ((j + 1) & 1) p0 |= b;
+            if ((j + 1) & 2) p1 |= b;
+            if ((j + 1) & 4) p2 |= b;
+        }
+    }
+    const uint64_t un = ~seen[w];
+    const uint64_t mine = un | p0 | p1 | p2;
+    for (int s = 0; s < k; ++s) {
+        if ((mine >> s) & 1) {
+            const int32_t c = (int32_t)((p0 >> s) & 1) | (int32_t)(((p1 >> s) & 1) << 1) |
+                              (int32_t)(((p2 >> s) & 1) << 2);
+            levels[(size_t)s * V + w] = c ? K.first + c : -1;
+        }
     }
 }
 
@@ -530,6 +564,7 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
     // persistent sparse levels need every row below the heavy-list threshold
     bool persist_ok = h->prm.persist != 0 && maxDegree(h) <= HEAVY_T;
     const unsigned g = gridFor(V, blk);
+    MsKept K{nullptr, 0, 0};
     int32_t L = 0;
     while (nf > 0) {
         if ((int64_t)V >= min_reach) {
@@ -553,8 +588,19 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
                 launchHeavyStore(h, "msbfs_push_heavy", f);
             }
             fillU64(h, M.FS, 0, M.W2);
+            // a dense level extending the kept run keeps its words; others write levels now
+            uint64_t* lw = nullptr;
+            if (K.n < MS_KEPT && (K.n == 0 || L == K.first + K.n)) {
+                if (K.n == 0) {
+                    S.msl.ensure((size_t)MS_KEPT * nV);
+                    K.lw = S.msl.p;
+                    K.first = L;
+                }
+                lw = S.msl.p + (size_t)K.n * nV;
+                ++K.n;
+            }
             launch(h, "msbfs_finish", kMsFinish, gridFor(V, blk * MF_TILES), blk, 0, seen, front, next, levels, V, L,
-                   fb, M, nxt);
+                   fb, M, nxt, lw);
             int64_t r[2];
             readCounters(h, (const int64_t*)nxt, 2, r);
             nf = (int64_t)((uint64_t)r[0] >> 32);
@@ -608,7 +654,7 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
         }
         L += C;
     }
-    launch(h, "msbfs_unreached", kMsUnreached, g, blk, 0, (const uint64_t*)seen, levels, V, all);
+    launch(h, "msbfs_levels", kMsLevels, g, blk, 0, (const uint64_t*)seen, K, levels, V, (int32_t)k);
 }
 
 }  // namespace gir

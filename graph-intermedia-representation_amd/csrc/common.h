@@ -131,6 +131,7 @@ struct Scratch {
     DevBuf<uint64_t> ms[3];  // bit-parallel BFS: seen / frontier / next words
     DevBuf<int64_t> mscnt;   // bit-parallel BFS: per-level counter ring
     DevBuf<uint64_t> msbm[4];  // bit-parallel BFS: frontier / touched bitmaps and summaries
+    DevBuf<uint64_t> msl;      // bit-parallel BFS: new-bit words of up to 7 dense levels
     DevBuf<int64_t> rcnt;    // reachability engine: per-level counter ring
     DevBuf<int64_t> rcnt2;   // the paired sweeps' two rings
     DevBuf<int32_t> pq[2];   // the paired sweeps' second queue pair

@@ -230,6 +230,41 @@ def test_bfs_one_workgroup_levels():
                 assert (lv[i] == oracle.bfs_textbook(gb, ga, x)).all(), (tiny, i)
 
 
+def test_bfs_bit_parallel_dense_level_runs():
+    # a ring of 12 layers, each layer's vertices all pointing into the next:
+    # with sources spread over the layers every level's union frontier spans
+    # most of V, so there are more dense levels than the finish keeps words
+    # for (the rest write their levels at once); a second graph has dense
+    # levels, then sparse ones, then dense ones again (only the first run is
+    # kept).  Against textbook BFS per source.
+    rng = np.random.default_rng(81)
+    Lr, W = 12, 48
+    layer = lambda i: np.arange(i * W, (i + 1) * W)
+    u = np.concatenate([np.repeat(layer(i), W) for i in range(Lr)])
+    v = np.concatenate([np.tile(layer((i + 1) % Lr), W) for i in range(Lr)])
+    keep = rng.random(len(u)) < 0.5
+    V = Lr * W
+    fb, fa, bb, ba = girhip.csr_from_edges(V, u[keep], v[keep])
+    src = np.concatenate([rng.integers(0, V, 40), [V - 1]]).astype(np.int32)
+    # dense-sparse-dense: a wide fan, a long thin path, a second wide fan
+    F, P = 600, 40
+    hub2 = F + P
+    u2 = np.concatenate([np.zeros(F, np.int64), np.arange(1, F + 1), np.arange(F + 1, hub2),
+                         np.full(F, hub2), np.arange(hub2 + 1, hub2 + F + 1)])
+    v2 = np.concatenate([np.arange(1, F + 1), np.full(F, F + 1), np.arange(F + 2, hub2 + 1),
+                         np.arange(hub2 + 1, hub2 + F + 1), np.full(F, 0)])
+    V2 = hub2 + F + 1
+    gb, ga, gbb, gba = girhip.csr_from_edges(V2, u2, v2)
+    src2 = np.array([0, 1, 2, F + 3, hub2, hub2 + 1], np.int32)
+    for fbv, fav, bbv, bav, ss, n in ((fb, fa, bb, ba, src, V), (gb, ga, gbb, gba, src2, V2)):
+        with girhip.Graph(fbv, fav, bbv, bav) as G:
+            G.set_param("bfs_tiny", 0)
+            say("dense-level runs V", n, "sources", len(ss))
+            lv = G.bfs(ss)
+            for i, x in enumerate(ss):
+                assert (lv[i] == oracle.bfs_textbook(fbv, fav, x)).all(), (n, i)
+
+
 def test_bfs_many_levels_bit_parallel():
     # long thin grid: hundreds of tiny levels -> sparse queue mode, chunked
     # levels and the counter ring wrapping; duplicate and invalid sources
