@@ -471,9 +471,11 @@ void fillU8(girhip_graph* h, uint8_t* p, uint8_t v, int64_t n);
 // out[0..n] = exclusive prefix sums of in[0..n) (out[n] = total).
 void scanI32toI64(girhip_graph* h, const int32_t* in, int64_t* out, int64_t n);
 void scanI32toI32(girhip_graph* h, const int32_t* in, int32_t* out, int64_t n);
-// Stable LSD radix sort of (key, value) pairs by the low key_bits of key.
-// vals may be null.  Result is in keys/vals.
-void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits);
+// Stable LSD radix sort of (key, value) pairs by bits [lo_bit, key_bits) of
+// key (the pairs keep their input order on ties there: input already ordered
+// by the bits below lo_bit comes out ordered by all key_bits).  vals may be
+// null.  Result is in keys/vals.
+void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits, int lo_bit = 0);
 // Number of bits needed to hold values up to x.
 inline int bitsFor(uint64_t x) {
     int b = 0;

@@ -187,13 +187,13 @@ static_assert(RS_D == RS_T, "a pass's threads own one digit each");
 constexpr unsigned RS_SPIN = 1u << 22;
 constexpr uint64_t RS_CNT = (1ULL << 46) - 1;
 
-__global__ void __launch_bounds__(RS_T) kRadixCount(const uint64_t* __restrict__ keys, int64_t n, int passes,
+__global__ void __launch_bounds__(RS_T) kRadixCount(const uint64_t* __restrict__ keys, int64_t n, int lo, int passes,
                                                     unsigned long long* __restrict__ ghist) {
     __shared__ uint32_t c[RS_MAXP][RS_D];
     for (int p = 0; p < passes; ++p) c[p][threadIdx.x] = 0;
     __syncthreads();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = keys[i];
+        const uint64_t k = keys[i] >> lo;
         for (int p = 0; p < passes; ++p) atomicAdd(&c[p][(k >> (8 * p)) & 255], 1u);
     }
     __syncthreads();
@@ -344,16 +344,16 @@ __global__ void __launch_bounds__(SS_T) kSmallSort(uint64_t* keys, uint32_t* val
     }
 }
 
-void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits) {
-    if (n <= 1) return;
+void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits, int lo_bit) {
+    if (n <= 1 || lo_bit >= key_bits) return;
     if (n <= SS_N) {
-        const uint64_t mask = key_bits >= 64 ? ~0ULL : (1ULL << key_bits) - 1;
+        const uint64_t mask = (key_bits >= 64 ? ~0ULL : (1ULL << key_bits) - 1) & ~((1ULL << lo_bit) - 1);
         launch(h, "small_sort", kSmallSort, 1, SS_T, 0, keys, vals, (int)n, mask);
         return;
     }
     Scratch& S = scratch(h);
     const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
-    const int passes = std::min(RS_MAXP, std::max(1, (key_bits + 7) / 8));
+    const int passes = std::min(RS_MAXP, std::max(1, (key_bits - lo_bit + 7) / 8));
     S.keys[1].ensure(n);
     if (vals) S.vals[1].ensure(n);
     // published words: one per (tile, digit); a new buffer starts cleared
@@ -370,7 +370,7 @@ void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int k
     uint64_t* aux = S.rs_aux.p;
     fillU64(h, aux, 0, RS_MAXP + (int64_t)passes * RS_D);
     launch(h, "radix_count", kRadixCount, (unsigned)std::min<int64_t>(nb, 1024), RS_T, 0, (const uint64_t*)keys, n,
-           passes, (unsigned long long*)(aux + RS_MAXP));
+           lo_bit, passes, (unsigned long long*)(aux + RS_MAXP));
     uint64_t *ka = keys, *kb = S.keys[1].p;
     uint32_t *va = vals, *vb = vals ? S.vals[1].p : nullptr;
     for (int p = 0; p < passes; ++p) {
@@ -379,7 +379,7 @@ void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int k
             S.rs_epoch = 1;
         }
         launch(h, "radix_pass", kRadixPass, (unsigned)nb, RS_T, 0, (const uint64_t*)ka, (const uint32_t*)va, kb, vb, n,
-               8 * p, (const unsigned long long*)(aux + RS_MAXP + p * RS_D), S.rs_look.p, S.rs_epoch,
+               lo_bit + 8 * p, (const unsigned long long*)(aux + RS_MAXP + p * RS_D), S.rs_look.p, S.rs_epoch,
                (unsigned long long*)(aux + p), S.rs_err.p);
         std::swap(ka, kb);
         std::swap(va, vb);

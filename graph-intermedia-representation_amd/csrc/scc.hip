@@ -862,11 +862,13 @@ static void dagIndex(girhip_graph* h, int64_t nd, int B, bool ranks);
 void dagFinish(girhip_graph* h, int64_t nd, int B, bool ranks) {
     const int blk = (int)h->prm.block;
     if (nd > 0) {
-        // the arcs sorted by destination: the in-arc keys
+        // the arcs sorted by destination: the in-arc keys (dst << B | src),
+        // written in arc order, are already ordered by src, so a stable sort
+        // on the dst bits alone orders them by (dst, src)
         h->dag_tk.ensureSlack((size_t)nd);
         launch(h, "dag_tkeys", kDagTKeys, gridFor(nd, blk), blk, 0, (const int32_t*)h->dag_src.p,
                (const int32_t*)h->dag_dst.p, nd, B, h->dag_tk.p);
-        radixSort(h, h->dag_tk.p, nullptr, nd, 2 * B);
+        radixSort(h, h->dag_tk.p, nullptr, nd, 2 * B, B);
     }
     dagIndex(h, nd, B, ranks);
 }
