@@ -56,8 +56,13 @@ struct SameColour {
 constexpr int CA_ITEMS = 8;
 __host__ __device__ constexpr int64_t caSpan(int block) { return (int64_t)block * CA_ITEMS; }
 
-__global__ void kCompactActive(const int32_t* __restrict__ in, int64_t n, bool all, const int32_t* __restrict__ scc,
-                               int32_t* __restrict__ out, int64_t* cnt) {
+// With all, the entries are the vertices 0..n-1; then (fb, bb given) an
+// active vertex without out-arcs or without in-arcs is labelled as its own
+// SCC here (no cycle can pass through it) instead of entering the list: the
+// trim rounds then start without the vertices a degree test settles.
+__global__ void kCompactActive(const int32_t* __restrict__ in, int64_t n, bool all, int32_t* __restrict__ scc,
+                               int32_t* __restrict__ out, int64_t* cnt, const int32_t* __restrict__ fb,
+                               const int32_t* __restrict__ bb) {
     const int64_t i0 = (int64_t)blockIdx.x * caSpan(blockDim.x) + threadIdx.x;
     int32_t v[CA_ITEMS];
     int k = 0;
@@ -69,8 +74,13 @@ __global__ void kCompactActive(const int32_t* __restrict__ in, int64_t n, bool a
         if (i < n) {
             v[t] = all ? (int32_t)i : in[i];
             if (scc[v[t]] == -1) {
-                keep |= 1u << t;
-                ++k;
+                if (fb && (fb[v[t] + 1] == fb[v[t]] || bb[v[t] + 1] == bb[v[t]])) {
+                    scc[v[t]] = v[t];
+                }
+                else {
+                    keep |= 1u << t;
+                    ++k;
+                }
             }
         }
     }
@@ -605,7 +615,8 @@ int64_t compactActive(girhip_graph* h, const int32_t* in, int64_t n, bool all, i
     const int blk = (int)h->prm.block;
     fillI64(h, h->dcnt.p + 2, 0, 1);
     launch(h, "scc_compact", kCompactActive, (unsigned)((n + caSpan(blk) - 1) / caSpan(blk)), blk, 0, in, n, all,
-           (const int32_t*)h->scc.p, out, h->dcnt.p + 2);
+           h->scc.p, out, h->dcnt.p + 2, all ? (const int32_t*)h->fw_beg.p : nullptr,
+           all ? (const int32_t*)h->bw_beg.p : nullptr);
     int64_t c;
     readCounters(h, h->dcnt.p + 2, 1, &c);
     return c;
