@@ -132,6 +132,7 @@ struct Scratch {
     DevBuf<int64_t> mscnt;   // bit-parallel BFS: per-level counter ring
     DevBuf<uint64_t> msbm[4];  // bit-parallel BFS: frontier / touched bitmaps and summaries
     DevBuf<uint64_t> msl;      // bit-parallel BFS: new-bit words of up to 7 dense levels
+    DevBuf<int64_t> tcnt;      // SCC trim phase: device-held list lengths, pivot key, round flags
     DevBuf<int64_t> rcnt;    // reachability engine: per-level counter ring
     DevBuf<int64_t> rcnt2;   // the paired sweeps' two rings
     DevBuf<int32_t> pq[2];   // the paired sweeps' second queue pair

@@ -60,34 +60,47 @@ __host__ __device__ constexpr int64_t caSpan(int block) { return (int64_t)block 
 // active vertex without out-arcs or without in-arcs is labelled as its own
 // SCC here (no cycle can pass through it) instead of entering the list: the
 // trim rounds then start without the vertices a degree test settles.
-__global__ void kCompactActive(const int32_t* __restrict__ in, int64_t n, bool all, int32_t* __restrict__ scc,
-                               int32_t* __restrict__ out, int64_t* cnt, const int32_t* __restrict__ fb,
-                               const int32_t* __restrict__ bb) {
-    const int64_t i0 = (int64_t)blockIdx.x * caSpan(blockDim.x) + threadIdx.x;
-    int32_t v[CA_ITEMS];
-    int k = 0;
-    unsigned keep = 0;
+//
+// List lengths: n, or with n_dev the smaller of n and *n_dev (a length a
+// kernel earlier on the stream wrote: the host launches for an upper bound
+// and reads no count back).  The list kernels walk their range grid-stride.
+__device__ __forceinline__ int64_t listLen(int64_t n, const int64_t* n_dev) {
+    if (!n_dev) return n;
+    const int64_t d = *n_dev;
+    return d < n ? d : n;
+}
+__global__ void kCompactActive(const int32_t* __restrict__ in, int64_t n, const int64_t* __restrict__ n_dev, bool all,
+                               int32_t* __restrict__ scc, int32_t* __restrict__ out, int64_t* cnt,
+                               const int32_t* __restrict__ fb, const int32_t* __restrict__ bb) {
+    n = listLen(n, n_dev);
+    const int64_t span = caSpan(blockDim.x);
+    for (int64_t s0 = (int64_t)blockIdx.x * span; s0 < n; s0 += (int64_t)gridDim.x * span) {
+        const int64_t i0 = s0 + threadIdx.x;
+        int32_t v[CA_ITEMS];
+        int k = 0;
+        unsigned keep = 0;
 #pragma unroll
-    for (int t = 0; t < CA_ITEMS; ++t) {
-        const int64_t i = i0 + (int64_t)t * blockDim.x;
-        v[t] = 0;
-        if (i < n) {
-            v[t] = all ? (int32_t)i : in[i];
-            if (scc[v[t]] == -1) {
-                if (fb && (fb[v[t] + 1] == fb[v[t]] || bb[v[t] + 1] == bb[v[t]])) {
-                    scc[v[t]] = v[t];
-                }
-                else {
-                    keep |= 1u << t;
-                    ++k;
+        for (int t = 0; t < CA_ITEMS; ++t) {
+            const int64_t i = i0 + (int64_t)t * blockDim.x;
+            v[t] = 0;
+            if (i < n) {
+                v[t] = all ? (int32_t)i : in[i];
+                if (scc[v[t]] == -1) {
+                    if (fb && (fb[v[t] + 1] == fb[v[t]] || bb[v[t] + 1] == bb[v[t]])) {
+                        scc[v[t]] = v[t];
+                    }
+                    else {
+                        keep |= 1u << t;
+                        ++k;
+                    }
                 }
             }
         }
-    }
-    int64_t pos = blockReserve(k, cnt);
+        int64_t pos = blockReserve(k, cnt);
 #pragma unroll
-    for (int t = 0; t < CA_ITEMS; ++t)
-        if ((keep >> t) & 1) out[pos++] = v[t];
+        for (int t = 0; t < CA_ITEMS; ++t)
+            if ((keep >> t) & 1) out[pos++] = v[t];
+    }
 }
 
 __device__ __forceinline__ bool anyActive(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj,
@@ -100,16 +113,19 @@ __device__ __forceinline__ bool anyActive(const int32_t* __restrict__ beg, const
     return false;
 }
 
-__global__ void kTrim1(const int32_t* __restrict__ list, int64_t n, const int32_t* __restrict__ fb,
-                       const int32_t* __restrict__ fa, const int32_t* __restrict__ bb, const int32_t* __restrict__ ba,
-                       int32_t* scc, int64_t* cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// prev (may be null): the previous round's flag; a round after one that
+// trimmed nothing has nothing to trim and returns at once.
+__global__ void kTrim1(const int32_t* __restrict__ list, int64_t n, const int64_t* __restrict__ n_dev,
+                       const int32_t* __restrict__ fb, const int32_t* __restrict__ fa, const int32_t* __restrict__ bb,
+                       const int32_t* __restrict__ ba, int32_t* scc, int64_t* cnt, const int64_t* __restrict__ prev) {
+    if (prev && *prev == 0) return;
+    n = listLen(n, n_dev);
     bool trim = false;
-    if (i < n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t v = list[i];
-        if (scc[v] == -1) {
-            trim = !anyActive(fb, fa, scc, v) || !anyActive(bb, ba, scc, v);
-            if (trim) stRel(&scc[v], v);
+        if (scc[v] == -1 && (!anyActive(fb, fa, scc, v) || !anyActive(bb, ba, scc, v))) {
+            stRel(&scc[v], v);
+            trim = true;
         }
     }
     // only "any trimmed" matters to the host: a block that trimmed raises the
@@ -140,12 +156,12 @@ __device__ __forceinline__ int32_t onlyActive(const int32_t* __restrict__ beg, c
 // minimum id, the canonical label), so the partner never runs trim-1 alone.
 // Labels set during the pass only remove finished SCCs from the active set,
 // on which the rule stays sound.
-__global__ void kTrim2(const int32_t* __restrict__ list, int64_t n, const int32_t* __restrict__ fb,
-                       const int32_t* __restrict__ fa, const int32_t* __restrict__ bb, const int32_t* __restrict__ ba,
-                       int32_t* scc, int64_t* cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void kTrim2(const int32_t* __restrict__ list, int64_t n, const int64_t* __restrict__ n_dev,
+                       const int32_t* __restrict__ fb, const int32_t* __restrict__ fa, const int32_t* __restrict__ bb,
+                       const int32_t* __restrict__ ba, int32_t* scc, int64_t* cnt) {
+    n = listLen(n, n_dev);
     bool hit = false;
-    if (i < n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t v = list[i];
         if (scc[v] == -1) {
             int32_t w = onlyActive(fb, fa, scc, v);
@@ -167,8 +183,9 @@ __global__ void kTrim2(const int32_t* __restrict__ list, int64_t n, const int32_
 // argmax over the active list of min(outdeg * indeg, 2^33 - 1), ties to the
 // smaller vertex id: key = (score << 31) | (2^31 - 1 - v).
 // (grid-stride over a bounded grid; one atomic per block)
-__global__ void kPivot(const int32_t* __restrict__ list, int64_t n, const int32_t* __restrict__ fb,
-                       const int32_t* __restrict__ bb, unsigned long long* best) {
+__global__ void kPivot(const int32_t* __restrict__ list, int64_t n, const int64_t* __restrict__ n_dev,
+                       const int32_t* __restrict__ fb, const int32_t* __restrict__ bb, unsigned long long* best) {
+    n = listLen(n, n_dev);
     uint64_t key = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t v = list[i];
@@ -611,12 +628,12 @@ __global__ void kKahnHeavy(HeavyList H, KahnF f) {
 
 namespace {
 
-int64_t compactActive(girhip_graph* h, const int32_t* in, int64_t n, bool all, int32_t* out) {
+int64_t compactActive(girhip_graph* h, const int32_t* in, int64_t n, int32_t* out) {
     const int blk = (int)h->prm.block;
     fillI64(h, h->dcnt.p + 2, 0, 1);
-    launch(h, "scc_compact", kCompactActive, (unsigned)((n + caSpan(blk) - 1) / caSpan(blk)), blk, 0, in, n, all,
-           h->scc.p, out, h->dcnt.p + 2, all ? (const int32_t*)h->fw_beg.p : nullptr,
-           all ? (const int32_t*)h->bw_beg.p : nullptr);
+    launch(h, "scc_compact", kCompactActive, (unsigned)((n + caSpan(blk) - 1) / caSpan(blk)), blk, 0, in, n,
+           (const int64_t*)nullptr, false, h->scc.p, out, h->dcnt.p + 2, (const int32_t*)nullptr,
+           (const int32_t*)nullptr);
     int64_t c;
     readCounters(h, h->dcnt.p + 2, 1, &c);
     return c;
@@ -627,20 +644,66 @@ int64_t trimRounds(girhip_graph* h, int32_t*& list, int32_t*& alt, int64_t n, in
     const int blk = (int)h->prm.block;
     for (int r = 0; r < rounds && n > 0; ++r) {
         fillI64(h, h->dcnt.p + 3, 0, 1);
-        launch(h, "scc_trim1", kTrim1, gridFor(n, blk), blk, 0, (const int32_t*)list, n, (const int32_t*)h->fw_beg.p,
-               (const int32_t*)h->fw_adj.p, (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p, h->scc.p,
-               h->dcnt.p + 3);
+        launch(h, "scc_trim1", kTrim1, gridFor(n, blk), blk, 0, (const int32_t*)list, n, (const int64_t*)nullptr,
+               (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p, (const int32_t*)h->bw_beg.p,
+               (const int32_t*)h->bw_adj.p, h->scc.p, h->dcnt.p + 3, (const int64_t*)nullptr);
         if (h->prm.scc_trim2 != 0)
-            launch(h, "scc_trim2", kTrim2, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
+            launch(h, "scc_trim2", kTrim2, gridFor(n, blk), blk, 0, (const int32_t*)list, n, (const int64_t*)nullptr,
                    (const int32_t*)h->fw_beg.p, (const int32_t*)h->fw_adj.p, (const int32_t*)h->bw_beg.p,
                    (const int32_t*)h->bw_adj.p, h->scc.p, h->dcnt.p + 3);
         int64_t trimmed;
         readCounters(h, h->dcnt.p + 3, 1, &trimmed);
         if (trimmed == 0) break;
-        n = compactActive(h, list, n, false, alt);
+        n = compactActive(h, list, n, alt);
         std::swap(list, alt);
     }
     return n;
+}
+
+// The initial active list (vertices labelled -1; those without out- or
+// in-arcs labelled as their own SCCs on the way), the trim rounds over it
+// and, with pivot, the pivot key over what remains.  The list lengths stay
+// on the device: each kernel is launched for an upper bound (V) over a
+// bounded grid and reads the current length, so the phase costs one host
+// readback (length + key) instead of two per round.  list ends as the
+// remaining active list; returns its length.
+int64_t trimPhase(girhip_graph* h, int32_t*& list, int32_t*& alt, int rounds, bool pivot, int64_t* key) {
+    const int32_t V = h->V;
+    const int blk = (int)h->prm.block;
+    Scratch& S = scratch(h);
+    rounds = std::max(0, std::min(rounds, 64));
+    // c[0..rounds]: list lengths after each compaction, then the pivot key,
+    // then the rounds' flags
+    S.tcnt.ensure((size_t)(2 * rounds + 2));
+    int64_t* c = S.tcnt.p;
+    int64_t* pk = c + rounds + 1;
+    int64_t* fl = pk + 1;
+    fillI64(h, c, 0, 2 * rounds + 2);
+    const int32_t* fb = h->fw_beg.p;
+    const int32_t* bb = h->bw_beg.p;
+    const unsigned gc = std::min<unsigned>((unsigned)((V + caSpan(blk) - 1) / caSpan(blk)), 2048);
+    const unsigned gt = std::min<unsigned>(gridFor(V, blk), 4096);
+    launch(h, "scc_compact", kCompactActive, gc, blk, 0, (const int32_t*)nullptr, (int64_t)V, (const int64_t*)nullptr,
+           true, h->scc.p, list, c, fb, bb);
+    for (int r = 0; r < rounds; ++r) {
+        launch(h, "scc_trim1", kTrim1, gt, blk, 0, (const int32_t*)list, (int64_t)V, (const int64_t*)(c + r), fb,
+               (const int32_t*)h->fw_adj.p, bb, (const int32_t*)h->bw_adj.p, h->scc.p, fl + r,
+               (const int64_t*)(r > 0 ? fl + r - 1 : nullptr));
+        if (h->prm.scc_trim2 != 0)
+            launch(h, "scc_trim2", kTrim2, gt, blk, 0, (const int32_t*)list, (int64_t)V, (const int64_t*)(c + r), fb,
+                   (const int32_t*)h->fw_adj.p, bb, (const int32_t*)h->bw_adj.p, h->scc.p, fl + r);
+        launch(h, "scc_compact", kCompactActive, gc, blk, 0, (const int32_t*)list, (int64_t)V,
+               (const int64_t*)(c + r), false, h->scc.p, alt, c + r + 1, (const int32_t*)nullptr,
+               (const int32_t*)nullptr);
+        std::swap(list, alt);
+    }
+    if (pivot)
+        launch(h, "scc_pivot", kPivot, reduceGrid(V, blk), blk, 0, (const int32_t*)list, (int64_t)V,
+               (const int64_t*)(c + rounds), fb, bb, (unsigned long long*)pk);
+    int64_t r[2];
+    readCounters(h, c + rounds, 2, r);
+    *key = r[1];
+    return r[0];
 }
 
 }  // namespace
@@ -694,16 +757,11 @@ void runSccActive(girhip_graph* h, bool cert) {
     int32_t* color = S.iv[1].p;
     int32_t* stamp = S.iv[2].p;
     ReachBufs B = reachBufs(h);
-    int64_t n = compactActive(h, nullptr, V, true, list);
-    n = trimRounds(h, list, alt, n, (int)h->prm.scc_trim_rounds);
+    int64_t key = 0;
+    int64_t n = trimPhase(h, list, alt, (int)h->prm.scc_trim_rounds, h->prm.scc_pivot != 0, &key);
 
     // FW-BW from the pivot
     if (n > 0 && h->prm.scc_pivot != 0 && n > (int64_t)h->prm.scc_color_min) {
-        fillI64(h, h->dcnt.p + 4, 0, 1);
-        launch(h, "scc_pivot", kPivot, reduceGrid(n, blk), blk, 0, (const int32_t*)list, n,
-               (const int32_t*)h->fw_beg.p, (const int32_t*)h->bw_beg.p, (unsigned long long*)(h->dcnt.p + 4));
-        int64_t key;
-        readCounters(h, h->dcnt.p + 4, 1, &key);
         const int32_t pivot = (int32_t)(0x7fffffff - (key & 0x7fffffffLL));
         const ActiveOnly act{h->scc.p};
         if (cert) {
@@ -755,7 +813,7 @@ void runSccActive(girhip_graph* h, bool cert) {
                (const uint64_t*)S.bits[3].p, (const uint64_t*)B.vis, h->dcnt.p + 5);
         launch(h, "scc_fwbw_assign", kFwBwAssign, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
                (const uint64_t*)S.bits[3].p, (const uint64_t*)B.vis, (const int64_t*)h->dcnt.p + 5, h->scc.p);
-        n = compactActive(h, list, n, false, alt);
+        n = compactActive(h, list, n, alt);
         std::swap(list, alt);
         n = trimRounds(h, list, alt, n, (int)h->prm.scc_trim_rounds);
     }
@@ -790,7 +848,7 @@ void runSccActive(girhip_graph* h, bool cert) {
                  SameColour{h->scc.p, color}, NoVisit{});
         launch(h, "scc_color_assign", kColorAssign, gridFor(n, blk), blk, 0, (const int32_t*)list, n,
                (const uint64_t*)B.vis, (const int32_t*)color, h->scc.p);
-        n = compactActive(h, list, n, false, alt);
+        n = compactActive(h, list, n, alt);
         std::swap(list, alt);
         n = trimRounds(h, list, alt, n, 1);
     }
