@@ -222,7 +222,7 @@ CertStats certRepair(girhip_graph* h, int64_t ncutf, int64_t ncutb, int64_t ncan
                 const CertOrphans f{h->scc.p, g, orph};
                 launch(h, "cert_boundary", kBottomUp<CertOrphans, WriteParent>, buGrid(V, blk), blk, 0, rb, ra, pb,
                        R.vis, (const uint64_t*)R.fbits, R.nbits, V, 0, R.q[0], c + GC_BU, c + GC_BUZ, f,
-                       WriteParent{par});
+                       WriteParent{par}, (const int64_t*)nullptr, (int64_t)0, (int64_t)0);
                 int64_t r;
                 readCounters(h, c + GC_BU, 1, &r);
                 const int64_t n0 = (int64_t)((uint64_t)r >> 32), m0 = (int64_t)((uint64_t)r & 0xffffffffULL);

@@ -116,14 +116,25 @@ constexpr int32_t BU_SMALL = 8;
 // one counter hit by every block of a V-wide grid serialises thousands of
 // atomics.
 constexpr int BU_TILES = 4;
+// bottom-up levels queued per host readback (the later ones guarded)
+constexpr int BU_CHAIN = 3;
 inline unsigned buGrid(int64_t V, int block) { return gridFor(V, block * BU_TILES); }
 
+// guard (may be null): the previous bottom-up level's packed count, for a
+// level queued behind it without a host readback; the level runs only if
+// the host's own rule would have run it (the previous level found vertices
+// and nf * beta >= n_reach), else every block returns at once.
 template <class Flt, class Vis>
 __global__ void __launch_bounds__(1024) kBottomUp(const int32_t* __restrict__ rbeg, const int32_t* __restrict__ radj,
                                                   const int32_t* __restrict__ pbeg, uint64_t* vis,
                                                   const uint64_t* __restrict__ fbits, uint64_t* __restrict__ nbits,
                                                   int32_t V, int32_t L, int32_t* __restrict__ qout, int64_t* cnt,
-                                                  int64_t* cz, Flt flt, Vis on) {
+                                                  int64_t* cz, Flt flt, Vis on, const int64_t* __restrict__ guard,
+                                                  int64_t n_reach, int64_t beta) {
+    if (guard) {
+        const int64_t nf_prev = (int64_t)((uint64_t)*guard >> 32);
+        if (nf_prev == 0 || nf_prev * beta < n_reach) return;
+    }
     __shared__ int32_t lst[BU_TILES * 1024];
     __shared__ int lcount;
     __shared__ int64_t wm[1024 / WAVE];
@@ -374,20 +385,34 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
                 launch(h, "reach_q2bits", kQueueToBits, gridFor(nf, blk), blk, 0, (const int32_t*)B.q[cur], nf,
                        B.fbits);
             }
-            launch(h, "reach_bottomup", kBottomUp<Flt, Vis>, buGrid(V, blk), blk, 0, rbeg, radj, pbeg, B.vis,
-                   (const uint64_t*)B.fbits, B.nbits, V, L, B.q[cur ^ 1], cn, pair(L + 1), flt, on);
-            std::swap(B.fbits, B.nbits);
-            bits_valid = true;
-            int64_t r[2];
-            readCounters(h, cn, 2, r);
-            nf = (int64_t)((uint64_t)r[0] >> 32);
-            mf = (int64_t)((uint64_t)r[0] & 0xffffffffULL);
-            fillI64(h, cn, nf, 1);  // the level's pair as the other kernels read it
-            fillI64(h, cn + 1, mf, 1);
-            m_unexplored -= mf;
-            st.reached += nf;
-            cur ^= 1;
-            ++L;
+            // BU_CHAIN levels per readback: each level after the first runs
+            // only if the host's rule would have run it (kBottomUp's guard)
+            for (int t = 0; t < BU_CHAIN; ++t) {
+                launch(h, "reach_bottomup", kBottomUp<Flt, Vis>, buGrid(V, blk), blk, 0, rbeg, radj, pbeg, B.vis,
+                       (const uint64_t*)B.fbits, B.nbits, V, L + t, B.q[((cur + t) & 1) ^ 1], pair(L + t),
+                       pair(L + t + 1), flt, on, (const int64_t*)(t > 0 ? pair(L + t - 1) : nullptr), n_reach, beta);
+                std::swap(B.fbits, B.nbits);
+            }
+            int64_t r[2 * REACH_RING];
+            readCounters(h, c, 2 * REACH_RING, r);
+            int ran = 0;
+            for (int t = 0; t < BU_CHAIN; ++t) {
+                if (t > 0 && (nf == 0 || nf * beta < n_reach)) break;
+                const int64_t pk = r[pair(L + t) - c];
+                nf = (int64_t)((uint64_t)pk >> 32);
+                mf = (int64_t)((uint64_t)pk & 0xffffffffULL);
+                m_unexplored -= mf;
+                st.reached += nf;
+                ++ran;
+            }
+            // every chained level ran: the bitmaps are current; else the
+            // chain stopped where the rule leaves bottom-up (or the sweep ends)
+            bits_valid = ran == BU_CHAIN;
+            int64_t* cl = pair(L + ran - 1);
+            fillI64(h, cl, nf, 1);  // the last level's pair as the other kernels read it
+            fillI64(h, cl + 1, mf, 1);
+            cur ^= ran & 1;
+            L += ran;
             mf_prev = 0;
             continue;
         }
