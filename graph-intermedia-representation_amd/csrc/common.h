@@ -279,6 +279,7 @@ struct girhip_graph {
     int32_t n_wcc = 0;
     gir::DevBuf<int32_t> scc;
     int32_t n_scc = 0;
+    bool n_scc_pending = false;  // n_scc still on the device (dcnt slot DC_SCC): read by sccCount
     gir::DevBuf<int32_t> dag_src, dag_dst, dag_mult;
     gir::DevBuf<int32_t> dag_beg;  // DAG CSR offsets by source SCC id (V+1)
     gir::DevBuf<int32_t> dag_in_beg, dag_in_src;  // DAG in-arcs by destination SCC id (V+1, n_dag)
@@ -477,6 +478,10 @@ void scanI32toI32(girhip_graph* h, const int32_t* in, int32_t* out, int64_t n);
 // by the bits below lo_bit comes out ordered by all key_bits).  vals may be
 // null.  Result is in keys/vals.
 void radixSort(girhip_graph* h, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits, int lo_bit = 0);
+// dcnt slot of the SCC count an SCC pass leaves on the device.
+constexpr int DC_SCC = 61;
+// The SCC count (read from the device if the last SCC pass left it there).
+int32_t sccCount(girhip_graph* h);
 // Number of bits needed to hold values up to x.
 inline int bitsFor(uint64_t x) {
     int b = 0;

@@ -729,6 +729,7 @@ void runScc(girhip_graph* h) {
     h->scc.ensure((size_t)(V > 0 ? V : 1));
     if (V == 0) {
         h->n_scc = 0;
+        h->n_scc_pending = false;
         return;
     }
     fillI32(h, h->scc.p, -1, V);
@@ -852,11 +853,22 @@ void runSccActive(girhip_graph* h, bool cert) {
         std::swap(list, alt);
         n = trimRounds(h, list, alt, n, 1);
     }
-    fillI64(h, h->dcnt.p, 0, 1);
-    launch(h, "count_roots", kCountSelf, reduceGrid(V, blk), blk, 0, (const int32_t*)h->scc.p, V, h->dcnt.p);
-    int64_t ns;
-    readCounters(h, h->dcnt.p, 1, &ns);
-    h->n_scc = (int32_t)ns;
+    // the count stays on the device until someone needs it (the DAG pass
+    // reads it with its first count)
+    fillI64(h, h->dcnt.p + DC_SCC, 0, 1);
+    launch(h, "count_roots", kCountSelf, reduceGrid(V, blk), blk, 0, (const int32_t*)h->scc.p, V,
+           h->dcnt.p + DC_SCC);
+    h->n_scc_pending = true;
+}
+
+int32_t sccCount(girhip_graph* h) {
+    if (h->n_scc_pending) {
+        int64_t ns;
+        readCounters(h, h->dcnt.p + DC_SCC, 1, &ns);
+        h->n_scc = (int32_t)ns;
+        h->n_scc_pending = false;
+    }
+    return h->n_scc;
 }
 
 void runDag(girhip_graph* h) {
@@ -886,7 +898,13 @@ void runDag(girhip_graph* h) {
         launch(h, "dag_keys", kDagKeysRows, std::min<unsigned>(gridFor(V, blk), 4096), blk, 0,
                (const int32_t*)S.iv[0].p, (const int64_t*)(h->dcnt.p + 16), (const int32_t*)h->fw_beg.p,
                (const int32_t*)h->bw_beg.p, d, (const int32_t*)(h->dcnt.p + 15), S.keys[0].p, h->dcnt.p + 9);
-        readCounters(h, h->dcnt.p + 9, 1, &ncross);
+        int64_t rr[DC_SCC - 9 + 1];
+        readCounters(h, h->dcnt.p + 9, DC_SCC - 9 + 1, rr);
+        ncross = rr[0];
+        if (h->n_scc_pending) {
+            h->n_scc = (int32_t)rr[DC_SCC - 9];
+            h->n_scc_pending = false;
+        }
     }
     int64_t nd = 0;
     if (ncross > 0) {
@@ -1644,12 +1662,12 @@ bool repairScc(girhip_graph* h, const uint64_t* ik, int64_t ni, const uint64_t* 
         };
         fillU64(h, R.vis, 0, R.W);
         int64_t ns = seeds(S.ir_bsv.p);
-        runReach(h, h->dag_beg.p, h->dag_dst.p, h->dag_in_beg.p, h->dag_in_src.p, R, ns, 0, h->n_scc, h->n_dag,
+        runReach(h, h->dag_beg.p, h->dag_dst.p, h->dag_in_beg.p, h->dag_in_src.p, R, ns, 0, sccCount(h), h->n_dag,
                  RankWindow{h->rank.p, c[IR_MAXR], 1}, NoVisit{}, true, nullptr, nullptr, true);
         HIPCHK(hipMemcpyAsync(S.bits[3].p, R.vis, sizeof(uint64_t) * R.W, hipMemcpyDeviceToDevice, h->st));
         fillU64(h, R.vis, 0, R.W);
         ns = seeds(S.ir_bsu.p);
-        runReach(h, h->dag_in_beg.p, h->dag_in_src.p, h->dag_beg.p, h->dag_dst.p, R, ns, 0, h->n_scc, h->n_dag,
+        runReach(h, h->dag_in_beg.p, h->dag_in_src.p, h->dag_beg.p, h->dag_dst.p, R, ns, 0, sccCount(h), h->n_dag,
                  RankWindow{h->rank.p, c[IR_MINR], 0}, NoVisit{}, true, nullptr, nullptr, true);
     }
     int64_t naff = affected(nback > 0, false);
@@ -1696,7 +1714,7 @@ extern "C" int girhip_scc_dag(girhip_graph* h, int32_t* scc, int32_t* n_scc, int
         h->scc_ok = true;
         if (scc) copyOut(h, scc, h->scc.p, sizeof(int32_t) * (size_t)h->V);
         syncStream(h);
-        if (n_scc) *n_scc = h->n_scc;
+        if (n_scc) *n_scc = sccCount(h);
         if (n_dag) *n_dag = h->n_dag;
     });
 }
@@ -1707,7 +1725,7 @@ extern "C" int girhip_get_scc(girhip_graph* h, int32_t* scc, int32_t* n_scc, int
         if (!h->scc_req) raise(GIRHIP_ESTATE, "get_scc: call girhip_scc_dag first");
         if (scc) copyOut(h, scc, h->scc.p, sizeof(int32_t) * (size_t)h->V);
         syncStream(h);
-        if (n_scc) *n_scc = h->n_scc;
+        if (n_scc) *n_scc = sccCount(h);
         if (n_dag) *n_dag = h->n_dag;
     });
 }
