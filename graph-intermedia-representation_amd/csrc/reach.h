@@ -458,8 +458,10 @@ ReachStats runReach(girhip_graph* h, const int32_t* pbeg, const int32_t* padj, c
                    (const int64_t*)pair(L + t - 1), f, pair(L + t + 1), HL);
             if (any_row_heavy)  // (a quarter of the store's grid: the pass is launched at every level)
                 launch(h, "reach_topdown_heavy", kHeavyRows<TopDownF<Flt, Vis>>, 256, 256, 0, HL, f, pair(L + t) + 1);
-            else
-                launchHeavyStoreH(h, HL, "reach_topdown_heavy", f, pair(L + t) + 1);
+            else if (2 * maxDegree(h) > HEAVY_T)  // (blocks of 1,024: a quarter of the queue reservations,
+                                                   // which serialise on one counter, for the same threads)
+                launch(h, "reach_topdown_heavy", kHeavyRows<TopDownF<Flt, Vis>>, 256, 1024, 0, HL, f,
+                       pair(L + t) + 1);
             cur ^= 1;
         }
         int64_t r[2 * REACH_RING];
