@@ -115,7 +115,7 @@ constexpr int32_t BU_SMALL = 8;
 // first bottom-up step of a sweep finds vertices in nearly every block, and
 // one counter hit by every block of a V-wide grid serialises thousands of
 // atomics.
-constexpr int BU_TILES = 4;
+constexpr int BU_TILES = 8;
 // bottom-up levels queued per host readback (the later ones guarded)
 constexpr int BU_CHAIN = 3;
 inline unsigned buGrid(int64_t V, int block) { return gridFor(V, block * BU_TILES); }
