@@ -521,8 +521,11 @@ __global__ void kMsLevels(const uint64_t* __restrict__ seen, MsKept K, int32_t* 
 // it once per distinct level at which sources reach it), so pull starts when
 // the frontier's arcs exceed E / alpha and ends when its size falls below
 // V / beta.  Push levels are sparse while the frontier holds under a quarter
-// of V; sparse levels whose frontier is tiny and not growing fast run
-// MS_CHUNK at a time with one readback per chunk.
+// of V and its rows under E / 64 arcs (a sparse pass has one wave per 4,096
+// vertices: a few thousand frontier vertices with long rows, such as the
+// sources' hub neighbours, kept RMAT-22's second level at 0.9 ms there and
+// take 0.12 ms as a dense push); sparse levels whose frontier is tiny and
+// not growing fast run MS_CHUNK at a time with one readback per chunk.
 void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
     const int32_t V = h->V;
     const int blk = (int)h->prm.block;
@@ -571,7 +574,7 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
             if (!pull && mf * alpha > E) pull = true;
             else if (pull && nf * beta < (int64_t)V) pull = false;
         }
-        const bool sparse = !pull && nf * 4 < (int64_t)V;
+        const bool sparse = !pull && nf * 4 < (int64_t)V && mf * 64 < E;
         if (!sparse) {
             MsCnt* nxt = &ring[(L + 1) % MS_RING];
             if (pull) {
