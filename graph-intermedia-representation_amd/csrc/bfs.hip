@@ -386,7 +386,7 @@ __global__ void kMsPush(const int32_t* __restrict__ fb, const uint64_t* __restri
     waveRowsH(act, (int32_t)v, fr, 0, b, e, H, f);
 }
 
-// *act = OR of every frontier word: the sources still expanding (bounded
+// *act = OR of every frontier word: the sources still expanding (reduction
 // grid, one atomic per block; *act cleared before the launch).
 __global__ void kMsActive(const uint64_t* __restrict__ front, int32_t V, unsigned long long* act) {
     __shared__ uint64_t part[1024 / WAVE];
@@ -580,7 +580,7 @@ void msbfs(girhip_graph* h, const int32_t* dsrc, int k, int32_t* levels) {
             if (pull) {
                 uint64_t* act = (uint64_t*)(S.mscnt.p + 2 * MS_RING + 1);
                 fillU64(h, act, 0, 1);
-                launch(h, "msbfs_active", kMsActive, boundedGrid(V, blk), blk, 0, (const uint64_t*)front, V,
+                launch(h, "msbfs_active", kMsActive, reduceGrid(V, blk), blk, 0, (const uint64_t*)front, V,
                        (unsigned long long*)act);
                 launch(h, "msbfs_pull", kMsPull, g, blk, 0, (const int32_t*)h->bw_beg.p, (const int32_t*)h->bw_adj.p,
                        (const uint64_t*)front, (const uint64_t*)seen, next, V, (const uint64_t*)act, nxt);
